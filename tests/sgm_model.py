@@ -1,0 +1,158 @@
+"""Independent Python models used by the tests to pin the C oracle.
+
+``schedule`` enumerates the visits of DenseLayer.Compute's sheared-tile wavefront
+(DenseStereoSgm.cs:206-222) in order. ``label_eval`` evaluates a uniform-range layer from that order
+with a *label* model of the line buffers: each (half, slot) holds the path states of the last visit that
+wrote it, the two margin slots never change and the guard words between path segments stay 2000. It
+shares no code with the oracle's literal buffer arithmetic (LIdx offsets, stale cleanup, Rd), so their
+agreement pins that arithmetic. The brute-force census and box filter are written from the definitions.
+"""
+import numpy as np
+
+INIT = 2000
+
+
+def schedule(W, H, T, xsw=False):
+    """Yield (dir, x, y, cur) for every Pixel call, in the reference's order."""
+    for d in (1, -1):
+        gwN, ghN = (W + 2 * T - 1) // T, (H + T - 1) // T
+        for j in range(ghN):
+            for i in range(gwN):
+                x0 = i * T if d > 0 else W - 1 - i * T
+                y0 = j * T if d > 0 else H - 1 - j * T
+                yEnd = max(-1, min(H, y0 + d * T))
+                r = 0
+                while True:
+                    y = y0 + d * r
+                    if y == yEnd:
+                        break
+                    xs = max(0, min(x0 - d * r, W if (xsw and d > 0) else W - 1))
+                    xe = max(-1, min(W, x0 + d * (T - r)))
+                    if xs != xe:
+                        cur = 1 if (r & 1) == 0 else 0
+                        x = xs
+                        while x != xe:
+                            yield d, x, y, cur
+                            x += d
+                    r += 1
+
+
+def step(raw, prev, pmin, p1, p2):
+    """One path step (DenseStereoSgm.cs:288-298) on int64 vectors; guards at d=-1 and d=cap are 2000."""
+    cap = raw.size
+    lm = np.concatenate(([INIT], prev[:-1]))
+    lp = np.concatenate((prev[1:], [INIT]))
+    p2p = min(p2 + pmin, 65535)
+    m = np.minimum(np.minimum(np.minimum(lm + p1, 65535), prev), np.minimum(np.minimum(lp + p1, 65535), p2p))
+    lc = np.maximum(np.minimum(raw.astype(np.int64) + m, 65535) - pmin, 0)
+    assert lc.size == cap
+    return lc, int(lc.min())
+
+
+def label_eval(raw, T, p1, p2, xsw=False, return_pass1=False, return_visits=False):
+    """raw: uint8 [H, W, cap] (uniform full range). Returns uint16 agg [H, W, cap] (and pass-1 agg)."""
+    H, W, cap = raw.shape
+    init = (np.full(cap, INIT, np.int64), INIT)
+    slots = {}  # (half, slot) -> list of 4 (vec, min)
+    agg = np.zeros((H, W, cap), np.int64)
+    agg1 = None
+    visits = {}
+    last_dir = 1
+    for d, x, y, cur in schedule(W, H, T, xsw):
+        if d != last_dir:
+            agg1 = agg.copy()
+            last_dir = d
+        visits[(d, x, y)] = visits.get((d, x, y), 0) + 1
+        other = 1 - cur
+        slot = x + 1
+        states = []
+        for p in range(4):
+            ns = slot + (0 if p == 2 else (d if p == 3 else -d))
+            half = cur if p == 0 else other
+            st = slots.get((half, ns))
+            prev, pmin = init if (st is None or ns in (0, W + 1)) else st[p]
+            lc, mn = step(raw[y, x], prev, pmin, p1, p2)
+            states.append((lc, mn))
+            agg[y, x] += lc
+        slots[(cur, slot)] = states
+    if agg1 is None:
+        agg1 = agg.copy()
+    out = [np.minimum(agg, 65535).astype(np.uint16)]
+    if return_pass1:
+        out.append(np.minimum(agg1, 65535).astype(np.uint16))
+    if return_visits:
+        out.append(visits)
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def census_bruteforce(img, radius):
+    H, W = img.shape
+    out = np.zeros((H, W), np.uint64)
+    for y in range(H):
+        for x in range(W):
+            c = int(img[y, x])
+            code, bit = 0, 0
+            for dy in range(-radius, radius + 1):
+                for dx in range(-radius, radius + 1):
+                    if dx == 0 and dy == 0:
+                        continue
+                    v = int(img[min(max(y + dy, 0), H - 1), min(max(x + dx, 0), W - 1)])
+                    if v < c:
+                        code |= 1 << bit
+                    bit += 1
+            out[y, x] = code
+    return out
+
+
+def cost_bruteforce(cl, cr, D, cap):
+    H, W = cl.shape
+    raw = np.full((H, W, cap), 255, np.uint8)
+    for y in range(H):
+        for x in range(W):
+            for d in range(D):
+                raw[y, x, d] = bin(int(cl[y, x]) ^ int(cr[y, max(x - d, 0)])).count("1")
+    return raw
+
+
+def box3_bruteforce(img, maxval):
+    H, W = img.shape
+    out = np.zeros_like(img)
+    for y in range(H):
+        for x in range(W):
+            ys = range(max(y - 1, 0), min(y + 1, H - 1) + 1)
+            xs = range(max(x - 1, 0), min(x + 1, W - 1) + 1)
+            s = sum(int(img[yy, xx]) for yy in ys for xx in xs)
+            inv = np.float32(1.0) / (np.float32(len(xs)) * np.float32(len(ys)))
+            v = np.float32(s) * inv
+            out[y, x] = min(max(int(np.rint(v)), 0), maxval)
+    return out
+
+
+def wta_sub_lr_model(agg, D, tol=1):
+    """Straight restatement of the WTA / parabola / L-R definitions, one pixel at a time."""
+    H, W, cap = agg.shape
+    disp = np.zeros((H, W), np.uint16)
+    sub = np.zeros((H, W), np.float32)
+    valid = np.zeros((H, W), np.uint8)
+    for y in range(H):
+        dl = []
+        for x in range(W):
+            a = [int(v) for v in agg[y, x, :D]]
+            arg = a.index(min(a))
+            dl.append(arg)
+            disp[y, x] = arg
+            v = np.float32(arg)
+            if 0 < arg < D - 1:
+                l, c, r = a[arg - 1], a[arg], a[arg + 1]
+                den = l - 2 * c + r
+                if den > 0:
+                    v = np.float32(arg) + (np.float32(0.5) * np.float32(l - r)) / np.float32(den)
+            sub[y, x] = v
+        dr = []
+        for xr in range(W):
+            cand = [int(agg[y, xr + d, d]) for d in range(D) if xr + d < W]
+            dr.append(cand.index(min(cand)))
+        for x in range(W):
+            xr = x - dl[x]
+            valid[y, x] = 1 if (xr >= 0 and abs(dl[x] - dr[xr]) <= tol) else 0
+    return disp, sub, valid
