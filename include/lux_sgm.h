@@ -1,0 +1,136 @@
+/*
+ * lux_sgm.h -- C ABI of the MI355X (gfx950) dense-stereo SGM library (liblux_sgm.so).
+ *
+ * Drop-in for the depth hot path of Lux.Engine (mprovenz/lux): the body of DenseLayer.Compute
+ * (Lux.Engine/Pipeline/Registration/DenseStereoSgm.cs:181-232) and a rectified census/AD front end that
+ * feeds it. Plain C types only; no pointer is kept after a call returns. Status codes: 0 = OK, < 0 = error
+ * class (the message is available from lux_sgm_last_error()). The binding a C# caller adds is shown in
+ * INTEGRATION.md (P/Invoke, as Lux.Engine/Mtp/GPhoto2/GPhoto2Interop.cs:9-51 and
+ * Lux.DepthModel/OnnxDepthNetwork.cs:46-95 bind native code).
+ *
+ * Threading: contexts are independent; one context is used by one thread at a time (calls on one context
+ * are serialised internally). Results do not depend on concurrency (CaptureSession.cs:108-109).
+ */
+#ifndef LUX_SGM_H
+#define LUX_SGM_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error classes. LUX_SGM_EUNSUPPORTED mirrors the reference's NotSupportedException
+ * (DenseStereoSgm.cs:69, :343). */
+#define LUX_SGM_OK 0
+#define LUX_SGM_EINVAL (-1)
+#define LUX_SGM_EUNSUPPORTED (-2)
+#define LUX_SGM_ENOMEM (-3)
+#define LUX_SGM_EDEVICE (-4)
+#define LUX_SGM_ECANCELED (-5)
+
+#define LUX_SGM_COST_CENSUS 0
+#define LUX_SGM_COST_AD 1
+
+typedef struct lux_sgm_ctx lux_sgm_ctx;
+
+/* Parameters of one rectified pair. P1/P2 are the integer penalties of every path step; the reference's
+ * image-adaptive P2 reduces to trunc(P1*P2Scale*0x3f7ffb19) with zero guidance weights
+ * (DenseStereoSgm.cs:267-274), so P2 here is that effective integer. `tile` is the reference's
+ * wavefront tile T (DenseStereo.cs:26; even values only), whose border side effects
+ * (DenseStereoSgm.cs:206-222) are reproduced bit for bit. */
+typedef struct {
+    int32_t width, height;     /* pixels */
+    int32_t stride;            /* elements per image row, >= width */
+    int32_t bits;              /* 8: uint8 pixels; 9..16: uint16 pixels with that many significant bits */
+    int32_t cost;              /* LUX_SGM_COST_CENSUS or LUX_SGM_COST_AD */
+    int32_t census_radius;     /* 1, 2 (5x5) or 3 (7x7) */
+    int32_t ad_cap;            /* AD: per-pixel cap, <= 28 */
+    int32_t disparities;       /* D: planes 0..D-1 (DenseLayer Count) */
+    int32_t align;             /* lanes per block (DenseLayer Align): cap = roundup(D, align) */
+    int32_t p1, p2;            /* penalties, 0..65535 */
+    int32_t tile;              /* T of the reference schedule, even, >= 2 */
+    int32_t xs_clamp_w;        /* 1: LUX_SGM_XSW=1 behaviour (DenseStereoSgm.cs:31,217) */
+    int32_t lr_tolerance;      /* |dL - dR| <= tol is valid (RealStereo.cs:418-421) */
+} lux_sgm_rect_params;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int lux_sgm_version(void);
+
+/* Create a context on HIP device `device`. hbm_budget bounds the bytes the context may hold on the device
+ * (0 = no bound); a request above it fails with LUX_SGM_ENOMEM, as the reference's 1 GiB rule refuses a
+ * layer (DenseStereoSgm.cs:330,339-344). Replaces constructing DenseLayer/DenseStereoPyramid state. */
+int lux_sgm_create(int device, uint64_t hbm_budget, lux_sgm_ctx** out);
+int lux_sgm_destroy(lux_sgm_ctx* ctx);
+
+/* UTF-8 message of the last failed call on ctx (owned by ctx, valid until the next call on it). With
+ * ctx == NULL: the last failure of lux_sgm_create on this thread. */
+const char* lux_sgm_last_error(const lux_sgm_ctx* ctx);
+
+/* Device bytes a rectified pair needs (the DenseLayer.MemoryEstimate analogue,
+ * DenseStereoSgm.cs:142-156). */
+int lux_sgm_rect_bytes(const lux_sgm_rect_params* p, uint64_t* bytes);
+
+/* Rectified pair from host memory: cost volume (census or AD) -> 8-path SGM with the reference's
+ * schedule -> first-minimum WTA + parabolic subpixel + left/right check.
+ *   left/right: uint8 or uint16 images (p->bits), p->stride elements per row.
+ *   disp   [H*W] uint16  integer disparity (DenseStereoSgm.cs:225-231)   (may be NULL)
+ *   sub    [H*W] float   disparity + parabola offset                     (may be NULL)
+ *   valid  [H*W] uint8   1 where the L/R check passes                     (may be NULL)
+ *   agg    [H*W*cap] uint16 final aggregate, cap = roundup(D, align)      (may be NULL)
+ * Host buffers are read/written only during the call. */
+int lux_sgm_rectified(lux_sgm_ctx* ctx, const lux_sgm_rect_params* p, const void* left, const void* right,
+                      uint16_t* disp, float* sub, uint8_t* valid, uint16_t* agg);
+
+/* Same, with every pointer in device memory and work enqueued on `stream` (a hipStream_t, NULL = the
+ * context's stream). Returns once the work is enqueued; the caller synchronises the stream. agg (if not
+ * NULL) receives the aggregate with cell stride cap. */
+int lux_sgm_rectified_device(lux_sgm_ctx* ctx, const lux_sgm_rect_params* p, const void* d_left, const void* d_right,
+                             uint16_t* d_disp, float* d_sub, uint8_t* d_valid, uint16_t* d_agg, void* stream);
+
+/* Parameters of one DenseLayer with a uniform plane range (start 0, count = planes, cap =
+ * roundup(planes, align) for every pixel: the layer-0 form of DenseLayer.Init, DenseStereoSgm.cs:43-47). */
+typedef struct {
+    int32_t width, height;
+    int32_t planes;            /* Count */
+    int32_t align;             /* Align (8 or 2) */
+    int32_t p1, p2;            /* P1 and the effective constant P2 */
+    int32_t tile;              /* P.Tile, even */
+    int32_t xs_clamp_w;        /* LUX_SGM_XSW */
+} lux_sgm_layer_params;
+
+/* DenseLayer.Compute (DenseStereoSgm.cs:181-232) over a precomputed raw cost volume (the bytes
+ * DenseLayer.Raw holds, as after the Threads > 1 row-parallel stage, :186-205), host memory:
+ *   raw       [H*W*cap] uint8  (padding lanes [planes, cap) = 255, DenseStereo.cs:380)
+ *   agg       [H*W*cap] uint16 out: DenseLayer.Agg                       (may be NULL)
+ *   agg_pass1 [H*W*cap] uint16 out: DenseLayer.AggPass1 (KeepPass1)      (may be NULL)
+ *   wta       [H*W]     uint16 out: plane index start+arg of the WTA      (may be NULL)
+ * Per-pixel ranges (RangeFromPrevious, :91-140) and skip masks (:65-87) are LUX_SGM_EUNSUPPORTED. */
+int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
+                          uint16_t* agg_pass1, uint16_t* wta);
+
+/* Stage timing, the per-layer timing log of the reference's pyramid driver (DenseStereoSgm.cs:347-349)
+ * at kernel granularity. With profiling on, each rectified request records HIP events on its stream
+ * between launches; lux_sgm_stage_times waits for the last profiled request and writes the durations in
+ * ms of its stages, in order: census+cost, the four pass +1 sweeps (->, down-right, down, down-left),
+ * the pass +1 border replay, the four pass -1 sweeps (<-, up-left, up, up-right), the pass -1 border
+ * replay, WTA+subpixel+L/R. Returns the number of stages written (12), or < 0. */
+int lux_sgm_set_profiling(lux_sgm_ctx* ctx, int on);
+int lux_sgm_stage_times(lux_sgm_ctx* ctx, float* ms, int32_t cap);
+
+/* Schedule plan inspection (host only, no device work): for the reference's wavefront over W x H with
+ * tile T, the number of visits of the pass +1 border column (x = W-1) and of the pass -1 border column
+ * (x = 0), and the seed row (the last pass +1 row written to line-buffer half 0, -1 if none). Returns
+ * LUX_SGM_EUNSUPPORTED where the schedule falls outside what the device path reproduces (odd T). */
+int lux_sgm_plan_info(int32_t width, int32_t height, int32_t tile, int32_t xs_clamp_w, int32_t* visits_plus,
+                      int32_t* visits_minus, int32_t* seed_row);
+
+/* The visit lists themselves: per visit 4 int32 (y, half, label of path 0, label of path 1); a label is
+ * -1 for the initial 2000 fill, y' >= 0 for the neighbour column's state at row y' of the same pass, or
+ * -2 - c for pass +1's leftover half-0 state of cell c. `cap` entries of room in each list. */
+int lux_sgm_plan_visits(int32_t width, int32_t height, int32_t tile, int32_t xs_clamp_w, int32_t* plus, int32_t cap_plus,
+                        int32_t* minus, int32_t cap_minus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,50 @@
+// Internal interface between the C ABI (api.hip) and the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lux {
+
+// Device layout of one uniform-range layer: cell-major volumes raw u8 / agg u16 [H][W][cs], where
+// cs = roundup(cap, max(dpl, 4)) and dpl (planes per lane, a power of two >= 2) is chosen so that one
+// 64-lane wavefront covers a cell: 64 * dpl >= cap.
+struct Geometry {
+    int W = 0, H = 0, D = 0, cap = 0, dpl = 0, cs = 0;
+};
+Geometry geometry(int W, int H, int D, int align);
+
+struct SgmBuffers {
+    const uint8_t* raw = nullptr;   // [H][W][cs]
+    uint16_t* agg = nullptr;        // [H][W][cs]
+    uint16_t* seeds = nullptr;      // [3][W][cs]  pass +1 half-0 leftovers of paths 1..3
+    uint16_t* seedmins = nullptr;   // [3][W]
+    uint16_t* cols = nullptr;       // [2 passes][2 paths][H][cs]  neighbour-column states for the replays
+    uint16_t* colmins = nullptr;    // [2][2][H]
+    const int* visits_plus = nullptr;
+    int n_plus = 0;
+    const int* visits_minus = nullptr;
+    int n_minus = 0;
+    int seed_row = -1;
+};
+
+// Census codes of one image (T = uint8_t or uint16_t pixels).
+hipError_t launch_census(const void* img, bool img16, int W, int H, int stride, int radius, uint64_t* codes,
+                         hipStream_t s);
+// raw[y][x][d] for d < cs (census Hamming / truncated AD; 255 for d >= D).
+hipError_t launch_cost_census(const uint64_t* cl, const uint64_t* cr, const Geometry& g, uint8_t* raw, hipStream_t s);
+hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, int stride, int adcap, const Geometry& g,
+                          uint8_t* raw, hipStream_t s);
+// The eight path sweeps and the two border replays; agg_pass1 (if not null) receives agg after pass +1.
+// marks (if not null, kSgmMarks events) are recorded after each of the 10 launches, in launch order.
+constexpr int kSgmMarks = 10;
+hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
+                      hipEvent_t* marks = nullptr);
+// First-minimum WTA + parabola subpixel + left/right check, one row per workgroup.
+hipError_t launch_wta(const Geometry& g, const uint16_t* agg, int tol, uint16_t* disp, float* sub, uint8_t* valid,
+                      hipStream_t s);
+// Repack [H][W][cs] <-> [H][W][cap] on the device.
+hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s);
+hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,
+                              hipStream_t s);
+
+}  // namespace lux

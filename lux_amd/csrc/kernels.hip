@@ -1,0 +1,683 @@
+// gfx950 kernels of the dense-stereo SGM hot path (DenseStereoSgm.cs:181-232 and its cost front end).
+//
+// Layout: one cell = the cs planes of one pixel, contiguous (cell-major [y][x][cs]); a 64-lane wavefront
+// owns one cell per step, lane l holding planes [l*DPL, (l+1)*DPL) as DPL/2 packed u16 pairs. Every path
+// step is then one coalesced cs-byte raw load and a cs*2-byte aggregate read-modify-write, neighbour planes
+// d-1 / d+1 come from v_alignbit inside a lane and DPP wave_shr/wave_shl across lanes, and the per-step
+// minimum over planes is a DPP row_shr / row_bcast reduction. All arithmetic is the reference's exact u16
+// saturating form (DenseStereoSgm.cs:288-298) on v_pk_add_u16/v_pk_sub_u16 with clamp and v_pk_min_u16.
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace lux {
+
+namespace {
+
+constexpr uint32_t kInit2 = 0x07d007d0u;  // 2000 in both halves: the line buffers' fill value (:161-162)
+constexpr int kBlock = 256;                // four wavefronts per workgroup
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_add_sat(as2(a), as2(b))); }
+__device__ __forceinline__ uint32_t psub(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_sub_sat(as2(a), as2(b))); }
+__device__ __forceinline__ uint32_t pmin(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_min(as2(a), as2(b))); }
+
+// DPP cross-lane moves. `old` is what a lane without a source lane receives.
+__device__ __forceinline__ uint32_t shr1(uint32_t v, uint32_t old)  // lane i <- lane i-1 (wave_shr:1)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t shl1(uint32_t v, uint32_t old)  // lane i <- lane i+1 (wave_shl:1)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v)
+{
+    uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROWMASK, 0xf, false);
+    return v < o ? v : o;
+}
+// Minimum of a u32 over the 64 lanes, returned wave-uniform.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+    v = dpp_min<0x111, 0xf>(v);  // row_shr:1
+    v = dpp_min<0x112, 0xf>(v);  // row_shr:2
+    v = dpp_min<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_min<0x118, 0xf>(v);  // row_shr:8  -> lane 15 of each row holds the row minimum
+    v = dpp_min<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_min<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3 -> lane 63 holds the minimum
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// ------------------------------------------------------------------ lane-local cell data
+template <int NP>
+struct Cell {
+    uint32_t v[NP];
+};
+
+// Planes [lane*2*NP, +2*NP) of a cell: raw bytes widened to packed u16 pairs.
+template <int NP>
+__device__ __forceinline__ void load_raw(const uint8_t* p, bool active, Cell<NP>& r)
+{
+    if constexpr (NP == 1) {
+        uint32_t b = active ? (uint32_t)(*(const uint16_t*)p) : 0u;
+        r.v[0] = __builtin_amdgcn_perm(0u, b, 0x0c010c00u);
+    } else {
+        uint32_t w[NP / 2];
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) w[k] = active ? ((const uint32_t*)p)[k] : 0u;
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) {
+            r.v[2 * k] = __builtin_amdgcn_perm(0u, w[k], 0x0c010c00u);
+            r.v[2 * k + 1] = __builtin_amdgcn_perm(0u, w[k], 0x0c030c02u);
+        }
+    }
+}
+template <int NP>
+__device__ __forceinline__ void load_u16(const uint16_t* p, bool active, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) r.v[k] = active ? ((const uint32_t*)p)[k] : 0u;
+}
+template <int NP>
+__device__ __forceinline__ void store_u16(uint16_t* p, bool active, const Cell<NP>& r)
+{
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) ((uint32_t*)p)[k] = r.v[k];
+    }
+}
+
+// One path step of DenseLayer.Pixel (:276-314) for the planes this lane owns.
+//   m  = min(min(sat(Lm+P1), L0), min(sat(Lp+P1), P2p)),  P2p = min(P2 + minPrev, 65535)
+//   lc = satsub(sat(raw + m), minPrev)
+// Lm/Lp of the range ends read the guard words (2000). min(sat(a+P1), sat(b+P1)) = sat(min(a,b)+P1) as
+// saturating addition of a constant is monotone. Planes beyond cap (mask 0) hold 2000 like the guards and
+// are left out of the minimum. Returns the new minPrev (wave-uniform).
+template <int NP, bool MASKED>
+__device__ __forceinline__ uint32_t path_step(const Cell<NP>& prev, uint32_t minprev, const Cell<NP>& raw,
+                                              uint32_t p1x2, uint32_t p2, const Cell<NP>& mask, Cell<NP>& lc)
+{
+    uint32_t p2p = p2 + minprev;
+    p2p = p2p > 65535u ? 65535u : p2p;
+    const uint32_t p2x2 = p2p | (p2p << 16), minx2 = minprev | (minprev << 16);
+    const uint32_t left = shr1(prev.v[NP - 1], kInit2);   // plane l*DPL-1 in the high half
+    const uint32_t right = shl1(prev.v[0], kInit2);       // plane (l+1)*DPL in the low half
+    uint32_t run = 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        const uint32_t lm = __builtin_amdgcn_alignbit(prev.v[k], k == 0 ? left : prev.v[k - 1], 16);
+        const uint32_t lp = __builtin_amdgcn_alignbit(k == NP - 1 ? right : prev.v[k + 1], prev.v[k], 16);
+        uint32_t m = pmin(padd(pmin(lm, lp), p1x2), prev.v[k]);
+        m = pmin(m, p2x2);
+        uint32_t t = psub(padd(raw.v[k], m), minx2);
+        if constexpr (MASKED) {
+            t = (t & mask.v[k]) | (kInit2 & ~mask.v[k]);
+            run = pmin(run, t | ~mask.v[k]);
+        } else {
+            run = pmin(run, t);
+        }
+        lc.v[k] = t;
+    }
+    const uint32_t lo = run & 0xffffu, hi = run >> 16;
+    return wave_min(lo < hi ? lo : hi);
+}
+
+template <int NP>
+__device__ __forceinline__ void make_mask(int lane, int cap, Cell<NP>& mask, bool& active_any)
+{
+    const int e0 = lane * 2 * NP;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        uint32_t m = 0;
+        if (e0 + 2 * k < cap) m |= 0xffffu;
+        if (e0 + 2 * k + 1 < cap) m |= 0xffff0000u;
+        mask.v[k] = m;
+    }
+    active_any = e0 < cap;
+}
+
+template <int NP>
+__device__ __forceinline__ void set_init(Cell<NP>& c)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) c.v[k] = kInit2;
+}
+
+// Chains of one path direction: the cells whose predecessor (x-DX, y-DY) lies outside the image start one.
+template <int DX, int DY>
+__device__ __forceinline__ int chain_count(int W, int H) { return DY == 0 ? H : (DX == 0 ? W : W + H - 1); }
+
+template <int DX, int DY>
+__device__ __forceinline__ void chain_start(int c, int W, int H, int& x0, int& y0, int& n)
+{
+    if constexpr (DY == 0) {
+        y0 = c; x0 = DX > 0 ? 0 : W - 1; n = W;
+    } else if constexpr (DX == 0) {
+        x0 = c; y0 = DY > 0 ? 0 : H - 1; n = H;
+    } else {
+        if (c < W) {
+            x0 = c; y0 = DY > 0 ? 0 : H - 1;
+        } else {
+            const int k = c - W + 1;
+            y0 = DY > 0 ? k : H - 1 - k; x0 = DX > 0 ? 0 : W - 1;
+        }
+        const int nx = DX > 0 ? W - x0 : x0 + 1, ny = DY > 0 ? H - y0 : y0 + 1;
+        n = nx < ny ? nx : ny;
+    }
+}
+
+struct SweepArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    uint16_t* seeds;     // [3][W][cs]
+    uint16_t* seedmins;  // [3][W]
+    int seed_row;
+    uint16_t* cols;      // [2][H][cs] (this pass)
+    uint16_t* colmins;   // [2][H]
+};
+
+// One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
+// (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
+// 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
+template <int NP, int DIR, int P, bool FIRST, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
+{
+    constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
+    constexpr int DY = DIR * (P == 0 ? 0 : 1);
+    const int lane = threadIdx.x & 63;
+    const int chain = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    const int W = a.W, H = a.H, cs = a.cs;
+    if (chain >= chain_count<DX, DY>(W, H)) return;
+    int x, y, n;
+    chain_start<DX, DY>(chain, W, H, x, y, n);
+    const int bx = DIR > 0 ? W - 1 : 0;
+    if (P == 2 && x == bx) return;  // the border column's vertical chain is the replay's
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const int lo = lane * 2 * NP;
+
+    Cell<NP> state;
+    uint32_t minprev = 2000;
+    set_init(state);
+    if (DIR < 0 && P != 0 && y == H - 1) {
+        // pass -1's first row reads half 0 as pass +1 left it (:28-29, :219)
+        const int sc = x - DX;
+        if (sc >= 0 && sc < W) {
+            load_u16<NP>(a.seeds + ((size_t)(P - 1) * W + sc) * cs + lo, active, state);
+            if (MASKED) {
+#pragma unroll
+                for (int k = 0; k < NP; k++) state.v[k] = (state.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+            }
+            minprev = a.seedmins[(P - 1) * W + sc];
+        }
+    }
+    // software pipeline: the loads of step i+1 are in flight while step i computes
+    size_t cell = (size_t)y * W + x;
+    Cell<NP> raw, agg, nraw, nagg;
+    load_raw<NP>(a.raw + cell * cs + lo, active, raw);
+    if (!FIRST) load_u16<NP>(a.agg + cell * cs + lo, active, agg);
+    for (int i = 0; i < n; i++) {
+        const size_t ncell = cell + (ptrdiff_t)DY * W + DX;
+        if (i + 1 < n) {
+            load_raw<NP>(a.raw + ncell * cs + lo, active, nraw);
+            if (!FIRST) load_u16<NP>(a.agg + ncell * cs + lo, active, nagg);
+        }
+        Cell<NP> lc;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        if (x != bx) {
+            Cell<NP> out;
+#pragma unroll
+            for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
+            store_u16<NP>(a.agg + cell * cs + lo, active, out);
+            if (DIR > 0 && P != 0 && y == a.seed_row) {
+                store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, active, lc);
+                if (lane == 0) a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
+            }
+        } else if (FIRST) {
+            Cell<NP> z;
+#pragma unroll
+            for (int k = 0; k < NP; k++) z.v[k] = 0;
+            store_u16<NP>(a.agg + cell * cs + lo, active, z);
+        }
+        if (P <= 1 && x == bx - DIR) {  // the border replay reads this column's path 0 / 1 states
+            store_u16<NP>(a.cols + ((size_t)P * H + y) * cs + lo, active, lc);
+            if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
+        }
+        state = lc;
+        raw = nraw;
+        agg = nagg;
+        cell = ncell;
+        x += DX;
+        y += DY;
+    }
+}
+
+struct ReplayArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    int dir, col;
+    const int* visits;
+    int nvis;
+    const uint16_t* cols;     // [2][H][cs] of this pass
+    const uint16_t* colmins;  // [2][H]
+    uint16_t* seeds;          // [3][W][cs]
+    uint16_t* seedmins;       // [3][W]
+};
+
+// Replays the visits of one border column in schedule order (one wavefront). Paths 0 and 1 read the
+// neighbour-column states named by the plan's labels, path 2 reads the border slot of the other line-buffer
+// half (kept here, in LDS, per half), path 3 reads the margin slot (2000). Every visit adds its four path
+// costs to the cell's aggregate, so double-visited cells get both sets (:217, :296).
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(64) void sgm_replay(ReplayArgs a)
+{
+    __shared__ uint32_t halfs[2][3][64 * NP];
+    __shared__ uint32_t halfmin[2][3];
+    const int lane = threadIdx.x;
+    const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    for (int h = 0; h < 2; h++)
+        for (int p = 0; p < 3; p++) {
+            Cell<NP> s;
+            set_init(s);
+            uint32_t m = 2000;
+            if (a.dir < 0 && h == 0) {  // pass -1 starts from pass +1's half 0 of this slot
+                load_u16<NP>(a.seeds + ((size_t)p * W + a.col) * cs + lo, active, s);
+                if (MASKED) {
+#pragma unroll
+                    for (int k = 0; k < NP; k++) s.v[k] = (s.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+                }
+                m = a.seedmins[p * W + a.col];
+            }
+#pragma unroll
+            for (int k = 0; k < NP; k++) halfs[h][p][lane * NP + k] = s.v[k];
+            if (lane == 0) halfmin[h][p] = m;
+        }
+    __syncthreads();
+    for (int i = 0; i < a.nvis; i++) {
+        const int y = a.visits[4 * i], cur = a.visits[4 * i + 1];
+        const int lab[2] = {a.visits[4 * i + 2], a.visits[4 * i + 3]};
+        const size_t cell = (size_t)y * W + a.col;
+        Cell<NP> raw, agg;
+        load_raw<NP>(a.raw + cell * cs + lo, active, raw);
+        load_u16<NP>(a.agg + cell * cs + lo, active, agg);
+        Cell<NP> lc[4];
+        uint32_t mins[4];
+        for (int p = 0; p < 4; p++) {
+            Cell<NP> prev;
+            uint32_t pm = 2000;
+            set_init(prev);
+            if (p < 2) {
+                const int l = lab[p];
+                if (l >= 0) {
+                    load_u16<NP>(a.cols + ((size_t)p * a.H + l) * cs + lo, active, prev);
+                    pm = a.colmins[p * a.H + l];
+                } else if (l <= -2) {
+                    const int c = -2 - l;
+                    load_u16<NP>(a.seeds + ((size_t)0 * W + c) * cs + lo, active, prev);  // path 1 seed
+                    pm = a.seedmins[0 * W + c];
+                }
+                if (MASKED) {
+#pragma unroll
+                    for (int k = 0; k < NP; k++) prev.v[k] = (prev.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+                }
+            } else if (p == 2) {
+#pragma unroll
+                for (int k = 0; k < NP; k++) prev.v[k] = halfs[1 - cur][1][lane * NP + k];
+                pm = halfmin[1 - cur][1];
+            }
+            mins[p] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc[p]);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; k++) agg.v[k] = padd(padd(agg.v[k], lc[0].v[k]), padd(lc[1].v[k], padd(lc[2].v[k], lc[3].v[k])));
+        store_u16<NP>(a.agg + cell * cs + lo, active, agg);
+        __syncthreads();
+        for (int p = 1; p < 4; p++) {
+#pragma unroll
+            for (int k = 0; k < NP; k++) halfs[cur][p - 1][lane * NP + k] = lc[p].v[k];
+            if (lane == 0) halfmin[cur][p - 1] = mins[p];
+        }
+        __syncthreads();
+    }
+    if (a.dir > 0) {  // the seed table's entry for the border cell: half 0 as pass +1 leaves it
+        for (int p = 0; p < 3; p++) {
+            Cell<NP> s;
+#pragma unroll
+            for (int k = 0; k < NP; k++) s.v[k] = halfs[0][p][lane * NP + k];
+            store_u16<NP>(a.seeds + ((size_t)p * W + a.col) * cs + lo, active, s);
+            if (lane == 0) a.seedmins[p * W + a.col] = (uint16_t)halfmin[0][p];
+        }
+    }
+}
+
+// Seeds of interior cells when pass +1 never writes half 0 (H = 1): the initial fill.
+__global__ void fill_u16(uint16_t* p, size_t n, uint16_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------ census and cost volume
+template <typename T, int R>
+__global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* out)
+{
+    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int c = img[(size_t)y * stride + x];
+    uint64_t code = 0;
+    int bit = 0;
+#pragma unroll
+    for (int dy = -R; dy <= R; dy++) {
+        const int yy = min(max(y + dy, 0), H - 1);
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            const int xx = min(max(x + dx, 0), W - 1);
+            const int v = img[(size_t)yy * stride + xx];
+            code |= (uint64_t)(v < c) << bit;
+            bit++;
+        }
+    }
+    out[(size_t)y * W + x] = code;
+}
+
+// Four planes per thread: consecutive threads write consecutive 4-byte groups of a cell.
+__global__ void cost_census_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int D, int cs, uint8_t* raw)
+{
+    const int groups = cs >> 2;
+    const size_t total = (size_t)W * H * groups;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t cell = t / groups;
+        const int d0 = (int)(t - cell * groups) * 4;
+        const int y = (int)(cell / W), x = (int)(cell - (size_t)y * W);
+        const uint64_t a = cl[cell];
+        const uint64_t* rrow = cr + (size_t)y * W;
+        uint32_t w = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int d = d0 + j;
+            const uint32_t v = d < D ? (uint32_t)__popcll(a ^ rrow[max(x - d, 0)]) : 255u;
+            w |= v << (8 * j);
+        }
+        ((uint32_t*)raw)[t] = w;
+    }
+}
+
+template <typename T>
+__global__ void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride, int D, int cs, int shift, int adcap,
+                               uint8_t* raw)
+{
+    const int groups = cs >> 2;
+    const size_t total = (size_t)W * H * groups;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t cell = t / groups;
+        const int d0 = (int)(t - cell * groups) * 4;
+        const int y = (int)(cell / W), x = (int)(cell - (size_t)y * W);
+        int lv[9];
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+                lv[(dy + 1) * 3 + dx + 1] = L[(size_t)min(max(y + dy, 0), H - 1) * stride + min(max(x + dx, 0), W - 1)];
+        uint32_t w = 0;
+        for (int j = 0; j < 4; j++) {
+            const int d = d0 + j;
+            uint32_t v = 255u;
+            if (d < D) {
+                const int xr = max(x - d, 0);
+                int s = 0;
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int b = R[(size_t)min(max(y + dy, 0), H - 1) * stride + min(max(xr + dx, 0), W - 1)];
+                        const int df = abs(lv[(dy + 1) * 3 + dx + 1] - b) >> shift;
+                        s += df < adcap ? df : adcap;
+                    }
+                v = (uint32_t)s;
+            }
+            w |= v << (8 * j);
+        }
+        ((uint32_t*)raw)[t] = w;
+    }
+}
+
+// ------------------------------------------------------------------ WTA + subpixel + L/R
+// One row per workgroup. dL: first minimum over planes [0, D) (:225-231), found as the minimum of the
+// key (agg << 16 | d). dR(xr) = first minimum of agg(xr+d, d): the same keys, min-combined into an LDS
+// row with atomicMin. Subpixel: parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly
+// rounded division, no contraction), no offset at the range ends.
+template <int NP>
+__global__ __launch_bounds__(kBlock) void wta_kernel(const uint16_t* agg, int W, int H, int cs, int D, int tol,
+                                                      uint16_t* disp, float* sub, uint8_t* valid)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t* dr = lds;                       // W keys
+    uint16_t* dl = (uint16_t*)(lds + W);      // W disparities
+    const int y = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < W; i += kBlock) dr[i] = 0xffffffffu;
+    __syncthreads();
+    const int lo = lane * 2 * NP;
+    const bool active = lo < cs;
+    for (int x = wave; x < W; x += kBlock / 64) {
+        const size_t cell = (size_t)y * W + x;
+        Cell<NP> a;
+        load_u16<NP>(agg + cell * cs + lo, active, a);
+        uint32_t best = 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int d = lo + 2 * k + h;
+                if (d < D) {
+                    const uint32_t key = ((h ? a.v[k] >> 16 : a.v[k] & 0xffffu) << 16) | (uint32_t)d;
+                    best = key < best ? key : best;
+                    if (x - d >= 0) atomicMin(&dr[x - d], key);
+                }
+            }
+        best = wave_min(best);
+        if (lane == 0) {
+            const int d = (int)(best & 0xffffu);
+            dl[x] = (uint16_t)d;
+            if (disp) disp[cell] = (uint16_t)d;
+            if (sub) {
+                float v = (float)d;
+                if (d > 0 && d < D - 1) {
+                    const int l = agg[cell * cs + d - 1], c = (int)(best >> 16), r = agg[cell * cs + d + 1];
+                    const int den = l - 2 * c + r;
+                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+                }
+                sub[cell] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (valid)
+        for (int x = threadIdx.x; x < W; x += kBlock) {
+            const int d = dl[x], xr = x - d;
+            uint8_t ok = 0;
+            if (xr >= 0) {
+                const int r = (int)(dr[xr] & 0xffffu);
+                ok = (uint8_t)(abs(d - r) <= tol);
+            }
+            valid[(size_t)y * W + x] = ok;
+        }
+}
+
+template <typename T>
+__global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t cells, int n, T fill)
+{
+    const size_t total = cells * (size_t)dcs;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t / dcs;
+        const int d = (int)(t - c * dcs);
+        dst[t] = d < n ? src[c * scs + d] : fill;
+    }
+}
+
+inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((total + block - 1) / block, 1u << 20); }
+
+// ------------------------------------------------------------------ launchers
+template <int NP, bool MASKED>
+hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
+                   hipEvent_t* marks)
+{
+    int mk = 0;
+    auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
+    SweepArgs a;
+    a.raw = b.raw;
+    a.agg = b.agg;
+    a.W = g.W; a.H = g.H; a.cs = g.cs; a.cap = g.cap;
+    a.p1x2 = (uint32_t)p1 | ((uint32_t)p1 << 16);
+    a.p2 = (uint32_t)p2;
+    a.seeds = b.seeds;
+    a.seedmins = b.seedmins;
+    a.seed_row = b.seed_row;
+    const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
+    auto blocks = [&](int chains) { return (chains + 3) / 4; };
+    const int nh = g.H, nv = g.W, nd = g.W + g.H - 1;
+    if (b.seed_row < 0) {  // pass +1 never writes half 0: every seed is the initial fill
+        fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
+        fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedmins, (size_t)3 * g.W, 2000);
+    }
+    // pass +1: ->, down-right, down, down-left
+    a.cols = b.cols; a.colmins = b.colmins;
+    sgm_sweep<NP, 1, 0, true, MASKED><<<blocks(nh), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, 1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, 1, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, 1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    mark();
+    ReplayArgs r;
+    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
+    r.p1x2 = a.p1x2; r.p2 = a.p2;
+    r.seeds = b.seeds; r.seedmins = b.seedmins;
+    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
+    r.cols = b.cols; r.colmins = b.colmins;
+    sgm_replay<NP, MASKED><<<1, 64, 0, s>>>(r);
+    mark();
+    if (agg_pass1) {
+        hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    // pass -1: <-, up-left, up, up-right
+    a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
+    sgm_sweep<NP, -1, 0, false, MASKED><<<blocks(nh), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, -1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, -1, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+    mark();
+    sgm_sweep<NP, -1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    mark();
+    r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
+    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz;
+    sgm_replay<NP, MASKED><<<1, 64, 0, s>>>(r);
+    mark();
+    return hipGetLastError();
+}
+
+}  // namespace
+
+Geometry geometry(int W, int H, int D, int align)
+{
+    Geometry g;
+    g.W = W; g.H = H; g.D = D;
+    g.cap = ((D + align - 1) / align) * align;
+    int dpl = 2;
+    while (64 * dpl < g.cap) dpl *= 2;
+    g.dpl = dpl;
+    const int q = dpl > 4 ? dpl : 4;
+    g.cs = ((g.cap + q - 1) / q) * q;
+    return g;
+}
+
+hipError_t launch_census(const void* img, bool img16, int W, int H, int stride, int radius, uint64_t* codes, hipStream_t s)
+{
+    dim3 blk(16, 16), grd((W + 15) / 16, (H + 15) / 16);
+#define LUX_CENSUS(T, R) census_kernel<T, R><<<grd, blk, 0, s>>>((const T*)img, W, H, stride, codes)
+    if (img16) {
+        if (radius == 1) LUX_CENSUS(uint16_t, 1); else if (radius == 2) LUX_CENSUS(uint16_t, 2); else LUX_CENSUS(uint16_t, 3);
+    } else {
+        if (radius == 1) LUX_CENSUS(uint8_t, 1); else if (radius == 2) LUX_CENSUS(uint8_t, 2); else LUX_CENSUS(uint8_t, 3);
+    }
+#undef LUX_CENSUS
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_census(const uint64_t* cl, const uint64_t* cr, const Geometry& g, uint8_t* raw, hipStream_t s)
+{
+    const size_t total = (size_t)g.W * g.H * (g.cs / 4);
+    cost_census_kernel<<<grid_for(total, 256), 256, 0, s>>>(cl, cr, g.W, g.H, g.D, g.cs, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, int stride, int adcap, const Geometry& g,
+                          uint8_t* raw, hipStream_t s)
+{
+    const size_t total = (size_t)g.W * g.H * (g.cs / 4);
+    const int shift = img16 ? bits - 8 : 0;
+    if (img16)
+        cost_ad_kernel<uint16_t><<<grid_for(total, 256), 256, 0, s>>>((const uint16_t*)L, (const uint16_t*)R, g.W, g.H,
+                                                                      stride, g.D, g.cs, shift, adcap, raw);
+    else
+        cost_ad_kernel<uint8_t><<<grid_for(total, 256), 256, 0, s>>>((const uint8_t*)L, (const uint8_t*)R, g.W, g.H,
+                                                                     stride, g.D, g.cs, shift, adcap, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
+                      hipEvent_t* m)
+{
+    const bool full = g.cap == 64 * g.dpl;
+    switch (g.dpl) {
+        case 2: return full ? run_sgm<1, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<1, true>(g, b, p1, p2, agg_pass1, s, m);
+        case 4: return full ? run_sgm<2, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<2, true>(g, b, p1, p2, agg_pass1, s, m);
+        case 8: return full ? run_sgm<4, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<4, true>(g, b, p1, p2, agg_pass1, s, m);
+        case 16: return full ? run_sgm<8, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<8, true>(g, b, p1, p2, agg_pass1, s, m);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_wta(const Geometry& g, const uint16_t* agg, int tol, uint16_t* disp, float* sub, uint8_t* valid,
+                      hipStream_t s)
+{
+    const size_t lds = (size_t)g.W * 4 + (size_t)g.W * 2 + 16;
+    switch (g.dpl) {
+        case 2: wta_kernel<1><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 4: wta_kernel<2><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 8: wta_kernel<4><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 16: wta_kernel<8><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s)
+{
+    const size_t total = (size_t)cells * dst_cs;
+    restride_kernel<uint16_t><<<grid_for(total, 256), 256, 0, s>>>(src, src_cs, dst, dst_cs, (size_t)cells, n, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,
+                              hipStream_t s)
+{
+    const size_t total = (size_t)cells * dst_cs;
+    restride_kernel<uint8_t><<<grid_for(total, 256), 256, 0, s>>>(src, src_cs, dst, dst_cs, (size_t)cells, n, fill);
+    return hipGetLastError();
+}
+
+}  // namespace lux

@@ -1,0 +1,215 @@
+"""Host-side mirror of Lux.Engine's dense-stereo layer interface over liblux_sgm.so (ctypes).
+
+``DenseLayer`` keeps the reference's public surface (DenseStereoSgm.cs:13-26, 172-174): ``Agg``, ``Raw``,
+``Start``, ``Count``, ``Cap``, ``Depth``-as-plane-index, ``AggPass1``, ``Threads``, ``KeepPass1`` and
+``compute()``; the work runs on the MI355X through the C ABI (include/lux_sgm.h). ``rectified`` is the
+rectified-pair front end (census / AD cost -> SGM -> WTA + subpixel + L/R). There is no CPU path here:
+without the built library every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblux_sgm.so")
+
+OK, EINVAL, EUNSUPPORTED, ENOMEM, EDEVICE, ECANCELED = 0, -1, -2, -3, -4, -5
+COST_CENSUS, COST_AD = 0, 1
+
+EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
+           "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
+           "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times")
+STAGES = ("cost", "sweep+0", "sweep+1", "sweep+2", "sweep+3", "replay+", "sweep-0", "sweep-1", "sweep-2",
+          "sweep-3", "replay-", "wta")
+
+
+class SgmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lux_sgm error {code}: {msg}")
+        self.code = code
+
+
+class NotSupported(SgmError):
+    """The reference's NotSupportedException (DenseStereoSgm.cs:69, :343)."""
+
+
+class RectParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("width", "height", "stride", "bits", "cost", "census_radius", "ad_cap",
+                                              "disparities", "align", "p1", "p2", "tile", "xs_clamp_w", "lr_tolerance")]
+
+
+class LayerParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("width", "height", "planes", "align", "p1", "p2", "tile", "xs_clamp_w")]
+
+
+_LIB = None
+
+
+def lib():
+    """Load liblux_sgm.so; raises if it was not built (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or `make -C lux_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64
+        L.lux_sgm_version.restype = ctypes.c_int
+        L.lux_sgm_create.argtypes = [ctypes.c_int, u64, ctypes.POINTER(vp)]
+        L.lux_sgm_destroy.argtypes = [vp]
+        L.lux_sgm_last_error.argtypes = [vp]
+        L.lux_sgm_last_error.restype = ctypes.c_char_p
+        L.lux_sgm_rect_bytes.argtypes = [vp, ctypes.POINTER(u64)]
+        L.lux_sgm_rectified.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_rectified_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_compute_layer.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
+        L.lux_sgm_plan_visits.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32]
+        L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.lux_sgm_stage_times.argtypes = [vp, vp, i32]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _check(rc, ctx=None):
+    if rc != OK:
+        msg = lib().lux_sgm_last_error(ctx).decode("utf-8", "replace")
+        raise (NotSupported if rc == EUNSUPPORTED else SgmError)(rc, msg)
+
+
+def plan_info(W, H, T, xsw=False):
+    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().lux_sgm_plan_info(W, H, T, int(bool(xsw)), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    return a.value, b.value, c.value
+
+
+def plan_visits(W, H, T, xsw=False):
+    n_plus, n_minus, seed_row = plan_info(W, H, T, xsw)
+    plus = np.zeros((max(n_plus, 1), 4), np.int32)
+    minus = np.zeros((max(n_minus, 1), 4), np.int32)
+    _check(lib().lux_sgm_plan_visits(W, H, T, int(bool(xsw)), _p(plus), n_plus, _p(minus), n_minus))
+    return plus[:n_plus], minus[:n_minus], seed_row
+
+
+class Context:
+    """One device context (lux_sgm_create). hbm_budget in bytes, 0 = unbounded."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _check(lib().lux_sgm_create(device, hbm_budget, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().lux_sgm_destroy(self.handle)
+            self.handle = None
+
+    def set_profiling(self, on=True):
+        _check(lib().lux_sgm_set_profiling(self.handle, int(bool(on))), self.handle)
+
+    def stage_times(self):
+        """Durations (ms) of the stages of the last profiled request, keyed by STAGES."""
+        ms = np.zeros(16, np.float32)
+        n = lib().lux_sgm_stage_times(self.handle, _p(ms), 16)
+        if n < 0:
+            _check(n, self.handle)
+        return dict(zip(STAGES, ms[:n].tolist()))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rect_params(W, H, D, *, bits=8, cost="census", radius=3, ad_cap=28, align=8, p1=8, p2=96, tile=64, xsw=False,
+                tol=1, stride=None):
+    p = RectParams()
+    p.width, p.height, p.stride, p.bits = W, H, stride or W, bits
+    p.cost = COST_CENSUS if cost == "census" else COST_AD
+    p.census_radius, p.ad_cap, p.disparities, p.align = radius, ad_cap, D, align
+    p.p1, p.p2, p.tile, p.xs_clamp_w, p.lr_tolerance = p1, p2, tile, int(bool(xsw)), tol
+    return p
+
+
+def rect_bytes(W, H, D, **kw):
+    out = ctypes.c_uint64()
+    _check(lib().lux_sgm_rect_bytes(ctypes.byref(rect_params(W, H, D, **kw)), ctypes.byref(out)))
+    return out.value
+
+
+def rectified(ctx, left, right, D, *, want_agg=False, **kw):
+    """Rectified pair from host arrays. Returns dict(disp, sub, valid[, agg])."""
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    H, W = left.shape
+    bits = kw.pop("bits", 8 if left.dtype == np.uint8 else 10)
+    if (left.dtype == np.uint8) != (bits == 8):
+        raise ValueError("bits does not match the image dtype")
+    p = rect_params(W, H, D, bits=bits, **kw)
+    cap = ((D + p.align - 1) // p.align) * p.align
+    disp = np.empty((H, W), np.uint16)
+    sub = np.empty((H, W), np.float32)
+    valid = np.empty((H, W), np.uint8)
+    agg = np.empty((H, W, cap), np.uint16) if want_agg else None
+    _check(lib().lux_sgm_rectified(ctx.handle, ctypes.byref(p), _p(left), _p(right), _p(disp), _p(sub), _p(valid),
+                                   _p(agg)), ctx.handle)
+    out = dict(disp=disp, sub=sub, valid=valid)
+    if want_agg:
+        out["agg"] = agg
+    return out
+
+
+def rectified_device(ctx, p, d_left, d_right, d_disp, d_sub, d_valid, d_agg=0, stream=0):
+    """Device-pointer entry point: pointers are ints (e.g. torch tensor data_ptr()); enqueue only."""
+    _check(lib().lux_sgm_rectified_device(ctx.handle, ctypes.byref(p), ctypes.c_void_p(d_left),
+                                          ctypes.c_void_p(d_right), ctypes.c_void_p(d_disp), ctypes.c_void_p(d_sub),
+                                          ctypes.c_void_p(d_valid), ctypes.c_void_p(d_agg or None),
+                                          ctypes.c_void_p(stream or None)), ctx.handle)
+
+
+class DenseLayer:
+    """Mirror of DenseLayer (DenseStereoSgm.cs:13-26) for a uniform plane range (layer-0 form of Init).
+
+    Set ``Raw`` (uint8 [H, W, Cap]) and call ``compute()``; ``Agg`` (uint16 [H, W, Cap]), ``AggPass1``
+    (with ``KeepPass1``) and ``Wta`` (plane index start+arg, the argument of ``Depth = Planes[...]``,
+    :225-231) are filled from the device.
+    """
+
+    def __init__(self, ctx, W, H, planes, *, align=8, p1=1, p2=0, tile=4, xsw=False):
+        self.ctx, self.W, self.H, self.Align, self.Tile, self.Xsw = ctx, W, H, align, tile, xsw
+        self.P1, self.P2 = p1, p2
+        cap = ((planes + align - 1) // align) * align
+        self.Start = np.zeros(W * H, np.uint16)
+        self.Count = np.full(W * H, planes, np.uint16)
+        self.Cap = np.full(W * H, cap, np.uint16)
+        self.MinLo, self.MaxHi = 0, cap
+        self.Raw = None
+        self.Agg = None
+        self.AggPass1 = None
+        self.Wta = None
+        self.Threads = 1
+        self.KeepPass1 = False
+
+    def compute(self):
+        cap = int(self.Cap[0])
+        raw = np.ascontiguousarray(self.Raw, np.uint8).reshape(self.H, self.W, cap)
+        p = LayerParams()
+        p.width, p.height, p.planes, p.align = self.W, self.H, int(self.Count[0]), self.Align
+        p.p1, p.p2, p.tile, p.xs_clamp_w = self.P1, self.P2, self.Tile, int(bool(self.Xsw))
+        self.Agg = np.empty((self.H, self.W, cap), np.uint16)
+        self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
+        self.Wta = np.empty((self.H, self.W), np.uint16)
+        _check(lib().lux_sgm_compute_layer(self.ctx.handle, ctypes.byref(p), _p(raw), _p(self.Agg), _p(self.AggPass1),
+                                           _p(self.Wta)), self.ctx.handle)
+        return self

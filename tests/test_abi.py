@@ -1,0 +1,83 @@
+"""CPU tests of the C ABI library: it loads, exports what include/lux_sgm.h declares, and its schedule planner
+agrees with the independent label simulation of the reference schedule (no device calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sgm_model as M
+from lux_amd import sgm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sgm.lib()
+    text = open(os.path.join(ROOT, "include", "lux_sgm.h")).read()
+    declared = set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(lux_sgm_\w+)\s*\(", text, re.M))
+    assert declared == set(sgm.EXPORTS), declared ^ set(sgm.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.lux_sgm_version() >= 100
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.Context(0)
+    assert e.value.code == sgm.EDEVICE
+
+
+def expected_plan(W, H, T, xsw=False):
+    """Border visit lists from the full label simulation (independent of the C++ planner)."""
+    lab = {}
+    nvis = {}
+    plus, minus = [], []
+    for d, x, y, cur in M.schedule(W, H, T, xsw):
+        nvis[(d, x, y)] = nvis.get((d, x, y), 0) + 1
+        other = 1 - cur
+        if (d > 0 and x == W - 1) or (d < 0 and x == 0):
+            nbc = x - d
+            codes = []
+            for p, half in ((0, cur), (1, other)):
+                L = lab.get((half, nbc + 1)) if 0 <= nbc < W else None
+                if L is None:
+                    codes.append(-1)
+                elif L[0] == d:
+                    assert L[1] == nbc and L[3] == 1
+                    codes.append(L[2])
+                else:
+                    assert p == 1 and half == 0
+                    codes.append(-2 - nbc)
+            (plus if d > 0 else minus).append([y, cur] + codes)
+        lab[(cur, x + 1)] = (d, x, y, nvis[(d, x, y)])
+    return np.array(plus, np.int32).reshape(-1, 4), np.array(minus, np.int32).reshape(-1, 4)
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16])
+def test_planner_matches_label_simulation(T):
+    for W in sorted({1, 2, 3, T - 1, T, T + 1, 2 * T - 1, 2 * T, 2 * T + 1, 3 * T + 5} - {0}):
+        for H in sorted({1, 2, 3, T - 1, T, T + 1, 2 * T + 3} - {0}):
+            for xsw in (False, True):
+                plus, minus, seed_row = sgm.plan_visits(W, H, T, xsw)
+                eplus, eminus = expected_plan(W, H, T, xsw)
+                np.testing.assert_array_equal(plus, eplus, err_msg=f"W={W} H={H} T={T} xsw={xsw}")
+                np.testing.assert_array_equal(minus, eminus, err_msg=f"W={W} H={H} T={T} xsw={xsw}")
+                # seed row: the last pass +1 row written to half 0
+                rows = [y for y in range(H) if (y % T) % 2 == 1]
+                assert seed_row == (max(rows) if rows else -1)
+
+
+def test_planner_large_shapes_and_odd_tiles():
+    n_plus, n_minus, seed_row = sgm.plan_info(4160, 3120, 64)
+    assert n_plus >= 3120 and n_minus >= 3120 and seed_row == 3119   # 3119 % 64 = 47 is odd: half 0
+    n_plus, n_minus, _ = sgm.plan_info(640, 480, 4)
+    assert n_plus == 480 + 120 and n_minus == 480 + 120   # 120 double visits per pass (T=4)
+    n_plus, _, _ = sgm.plan_info(640, 480, 4, xsw=True)
+    assert n_plus == 480                                  # LUX_SGM_XSW=1 removes pass +1's double visits
+    with pytest.raises(sgm.NotSupported):
+        sgm.plan_info(64, 48, 5)

@@ -1,0 +1,101 @@
+"""Parity of the gfx950 path with the CPU oracle (bit-exact: integer disparities, aggregates, validity;
+subpixel compared bit for bit too, within the stated 1e-4 px tolerance a fortiori)."""
+import numpy as np
+import pytest
+
+from lux_amd import scene, sgm
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+SUB_TOL = 1e-4  # px
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sgm.Context(0)
+    yield c
+    c.close()
+
+
+def _layer_case(ctx, W, H, planes, T, p1, p2, align=8, xsw=False, seed=0, hi=49):
+    rng = np.random.default_rng(seed)
+    cap = ((planes + align - 1) // align) * align
+    raw = rng.integers(0, hi, (H, W, cap)).astype(np.uint8)
+    raw[:, :, planes:] = 255
+    print(f"layer W={W} H={H} planes={planes} cap={cap} T={T} p1={p1} p2={p2} xsw={xsw}", flush=True)
+    L = sgm.DenseLayer(ctx, W, H, planes, align=align, p1=p1, p2=p2, tile=T, xsw=xsw)
+    L.Raw = raw
+    L.KeepPass1 = True
+    L.compute()
+    start = np.zeros(W * H, np.uint16)
+    count = np.full(W * H, planes, np.uint16)
+    agg, agg1, wta = o.layer(raw.reshape(-1), W, H, start, count, T=T, align=align, p1=p1, p2=p2, xsw=xsw,
+                             keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1.reshape(H, W, cap))
+    np.testing.assert_array_equal(L.Agg, agg.reshape(H, W, cap))
+    np.testing.assert_array_equal(L.Wta, wta)
+
+
+@pytest.mark.parametrize("W,H,T", [(1, 1, 2), (1, 7, 2), (2, 5, 4), (3, 3, 2), (7, 9, 4), (8, 8, 4), (9, 13, 4),
+                                   (16, 9, 8), (17, 17, 8), (33, 12, 16), (70, 40, 64), (130, 70, 64)])
+def test_layer_shapes(ctx, W, H, T):
+    _layer_case(ctx, W, H, 16, T, 10, 120, seed=W * 100 + H)
+    _layer_case(ctx, W, H, 16, T, 10, 120, seed=W * 100 + H + 1, xsw=True)
+
+
+@pytest.mark.parametrize("planes,align", [(8, 8), (13, 8), (64, 8), (96, 8), (128, 8), (192, 8), (256, 8),
+                                          (320, 8), (512, 8), (6, 2), (66, 2), (1000, 8)])
+def test_layer_plane_counts(ctx, planes, align):
+    _layer_case(ctx, 21, 11, planes, 4, 8, 96, align=align, seed=planes)
+
+
+def test_layer_penalties(ctx):
+    _layer_case(ctx, 19, 14, 24, 4, 0, 0, seed=1)
+    _layer_case(ctx, 19, 14, 24, 4, 2500, 3000, seed=2)   # penalties past the 2000 guards
+    _layer_case(ctx, 40, 3, 8, 4, 65535, 65535, seed=3, hi=256)  # saturating adds
+
+
+def _rect_case(ctx, W, H, D, *, bits, cost, radius=3, T=64, p1=8, p2=96, seed=0, planes=6, sigma=4.0, xsw=False):
+    left, right, _ = scene.make_pair(W, H, max(D - 1, 1), planes=planes, sigma=sigma, seed=seed, bits=bits)
+    print(f"rectified W={W} H={H} D={D} bits={bits} cost={cost} r={radius} T={T}", flush=True)
+    got = sgm.rectified(ctx, left, right, D, want_agg=True, bits=bits, cost=cost, radius=radius, p1=p1, p2=p2,
+                        tile=T, xsw=xsw)
+    want = o.rectified(left, right, D, cost=cost, radius=radius, bits=bits, p1=p1, p2=p2, T=T, xsw=xsw, want_agg=True)
+    np.testing.assert_array_equal(got["agg"], want["agg"])
+    np.testing.assert_array_equal(got["disp"], want["disp"])
+    np.testing.assert_array_equal(got["valid"], want["valid"])
+    assert np.abs(got["sub"] - want["sub"]).max() <= SUB_TOL
+    np.testing.assert_array_equal(got["sub"], want["sub"])
+    return got, want
+
+
+@pytest.mark.parametrize("bits,cost,radius", [(8, "census", 2), (10, "census", 3), (8, "ad", 0), (10, "ad", 0),
+                                              (12, "census", 1)])
+def test_rectified_front_ends(ctx, bits, cost, radius):
+    _rect_case(ctx, 97, 53, 32, bits=bits, cost=cost, radius=radius or 3, T=8, seed=bits)
+
+
+def test_rectified_config0(ctx):
+    # 640x480 uint8, census 5x5, D=64, P1=10, P2=120, L/R tolerance 1 (seeded planar scene, 4 planes, sigma 2)
+    got, want = _rect_case(ctx, 640, 480, 64, bits=8, cost="census", radius=2, T=64, p1=10, p2=120, planes=4, sigma=2.0)
+    assert got["valid"].mean() > 0.8
+
+
+def test_rectified_mid_size_u16(ctx):
+    _rect_case(ctx, 1040, 780, 96, bits=10, cost="census", radius=3, T=64, p1=8, p2=96, seed=5)
+
+
+def test_full_module_properties(ctx):
+    # 4160x3120 uint16 10-bit, census 7x7, D=256: deterministic, accurate against the generator's field
+    W, H, D = 4160, 3120, 256
+    left, right, truth = scene.make_pair(W, H, D - 1, planes=6, sigma=4.0, seed=1, bits=10)
+    print("full module 4160x3120 D=256", flush=True)
+    a = sgm.rectified(ctx, left, right, D, bits=10, cost="census", radius=3, p1=8, p2=96, tile=64)
+    b = sgm.rectified(ctx, left, right, D, bits=10, cost="census", radius=3, p1=8, p2=96, tile=64)
+    for k in ("disp", "sub", "valid"):
+        np.testing.assert_array_equal(a[k], b[k])
+    v = a["valid"] > 0
+    assert v.mean() > 0.8
+    err = np.abs(a["disp"].astype(np.int64) - truth)
+    assert (err[v] <= 1).mean() > 0.95
+    assert np.abs(a["sub"] - a["disp"]).max() <= 0.5
