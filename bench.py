@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=1040, help="rows of the CPU-baseline band (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -89,7 +89,9 @@ def main():
     valid = torch.empty((H, W), dtype=torch.uint8, device=dev)
     ctx = sgm.Context(0)
     p = sgm.rect_params(W, H, D, bits=BITS, cost="census", radius=RADIUS, p1=P1, p2=P2, tile=TILE, tol=1)
-    stream = torch.cuda.current_stream()
+    # a stream of our own (default priority): the library enqueues on it, the events below are recorded on it
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
 
     def request():
         sgm.rectified_device(ctx, p, dl.data_ptr(), dr.data_ptr(), disp.data_ptr(), sub.data_ptr(), valid.data_ptr(),

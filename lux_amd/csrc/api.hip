@@ -2,6 +2,7 @@
 // cache and the entry points that replace DenseLayer.Compute (DenseStereoSgm.cs:181-232).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -109,7 +110,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     int rc = get_plan(c, g.W, g.H, T, xsw, &plan);
     if (rc) return rc;
     const size_t cells = (size_t)g.W * g.H;
-    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis;
+    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial;
     if ((rc = get_buf(c, "raw", cells * g.cs, &raw))) return rc;
     if ((rc = get_buf(c, "agg", cells * g.cs * 2, &agg))) return rc;
     if ((rc = get_buf(c, "seeds", (size_t)3 * g.W * g.cs * 2, &seeds))) return rc;
@@ -117,6 +118,8 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     if ((rc = get_buf(c, "cols", (size_t)4 * g.H * g.cs * 2, &cols))) return rc;
     if ((rc = get_buf(c, "colmins", (size_t)4 * g.H * 2, &colmins))) return rc;
     const size_t nv = plan->plus.size() + plan->minus.size();
+    const size_t maxvis = std::max(plan->plus.size(), plan->minus.size()) / 4 + 1;
+    if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
     hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), plan->plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -132,6 +135,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.seedmins = (uint16_t*)seedmins;
     w->b.cols = (uint16_t*)cols;
     w->b.colmins = (uint16_t*)colmins;
+    w->b.partial = (uint16_t*)partial;
     w->b.visits_plus = (const int*)vis;
     w->b.n_plus = (int)plan->plus.size() / 4;
     w->b.visits_minus = (const int*)vis + plan->plus.size();

@@ -20,6 +20,7 @@ struct SgmBuffers {
     uint16_t* seedmins = nullptr;   // [3][W]
     uint16_t* cols = nullptr;       // [2 passes][2 paths][H][cs]  neighbour-column states for the replays
     uint16_t* colmins = nullptr;    // [2][2][H]
+    uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch
     const int* visits_plus = nullptr;
     int n_plus = 0;
     const int* visits_minus = nullptr;

@@ -264,98 +264,151 @@ struct ReplayArgs {
     int W, H, cs, cap;
     uint32_t p1x2, p2;
     int dir, col;
-    const int* visits;
+    const int* visits;        // 4 ints per visit: y, half, path-0 label, path-1 label
     int nvis;
     const uint16_t* cols;     // [2][H][cs] of this pass
     const uint16_t* colmins;  // [2][H]
     uint16_t* seeds;          // [3][W][cs]
     uint16_t* seedmins;       // [3][W]
+    uint16_t* partial;        // [nvis][cs]: paths 0 + 1 + 3 of each visit
 };
 
-// Replays the visits of one border column in schedule order (one wavefront). Paths 0 and 1 read the
-// neighbour-column states named by the plan's labels, path 2 reads the border slot of the other line-buffer
-// half (kept here, in LDS, per half), path 3 reads the margin slot (2000). Every visit adds its four path
-// costs to the cell's aggregate, so double-visited cells get both sets (:217, :296).
+// The state a path-0 / path-1 neighbour read returns, by plan label: -1 the initial fill, y >= 0 the saved
+// neighbour-column state of row y, -2-c pass +1's half-0 leftover of cell c (path 1 only).
 template <int NP, bool MASKED>
-__global__ __launch_bounds__(64) void sgm_replay(ReplayArgs a)
+__device__ __forceinline__ void label_state(const ReplayArgs& a, int p, int l, int lo, bool active, const Cell<NP>& mask,
+                                            Cell<NP>& prev, uint32_t& pm)
 {
-    __shared__ uint32_t halfs[2][3][64 * NP];
-    __shared__ uint32_t halfmin[2][3];
-    const int lane = threadIdx.x;
-    const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
+    set_init(prev);
+    pm = 2000;
+    if (l >= 0) {
+        load_u16<NP>(a.cols + ((size_t)p * a.H + l) * a.cs + lo, active, prev);
+        pm = a.colmins[p * a.H + l];
+    } else if (l <= -2) {
+        const int c = -2 - l;
+        load_u16<NP>(a.seeds + (size_t)c * a.cs + lo, active, prev);  // seed table of path 1
+        pm = a.seedmins[c];
+    }
+    if (MASKED) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) prev.v[k] = (prev.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+    }
+}
+
+// Paths 0, 1 and 3 of every border visit, one wavefront per visit, all in parallel: their neighbours are
+// fixed by the plan (saved neighbour-column states, pass +1 seeds, or the margin slot for path 3), so
+// they do not chain. Each visit's sum of the three path costs goes to `partial`.
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (i >= a.nvis) return;
+    const int y = a.visits[4 * i], l0 = a.visits[4 * i + 2], l1 = a.visits[4 * i + 3];
     Cell<NP> mask;
     bool active;
     make_mask<NP>(lane, a.cap, mask, active);
-    for (int h = 0; h < 2; h++)
-        for (int p = 0; p < 3; p++) {
-            Cell<NP> s;
-            set_init(s);
-            uint32_t m = 2000;
-            if (a.dir < 0 && h == 0) {  // pass -1 starts from pass +1's half 0 of this slot
-                load_u16<NP>(a.seeds + ((size_t)p * W + a.col) * cs + lo, active, s);
-                if (MASKED) {
+    const int lo = lane * 2 * NP;
+    Cell<NP> raw, prev, lc0, lc1, lc3;
+    load_raw<NP>(a.raw + ((size_t)y * a.W + a.col) * a.cs + lo, active, raw);
+    uint32_t pm;
+    label_state<NP, MASKED>(a, 0, l0, lo, active, mask, prev, pm);
+    path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc0);
+    label_state<NP, MASKED>(a, 1, l1, lo, active, mask, prev, pm);
+    path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc1);
+    set_init(prev);
+    path_step<NP, MASKED>(prev, 2000, raw, a.p1x2, a.p2, mask, lc3);
+    Cell<NP> sum;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) s.v[k] = (s.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
-                }
-                m = a.seedmins[p * W + a.col];
-            }
+    for (int k = 0; k < NP; k++) sum.v[k] = padd(padd(lc0.v[k], lc1.v[k]), lc3.v[k]);
+    store_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, sum);
+}
+
+// Path 2 of the border column: one chain through the column's visits in schedule order. Each visit reads
+// the state the other line-buffer half holds for the border slot and leaves its own in the current half
+// (:219, :265); every visit adds its four path costs to the cell, so double-visited cells get both sets
+// (:217, :296). Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's
+// seed-table entry (half 0 as the pass leaves it).
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
+{
+    const int lane = threadIdx.x;
+    const int cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    Cell<NP> h0, h1;
+    uint32_t m0 = 2000, m1 = 2000;
+    set_init(h0);
+    set_init(h1);
+    if (a.dir < 0) {
+        load_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, active, h0);  // path 2's seed
+        if (MASKED) {
 #pragma unroll
-            for (int k = 0; k < NP; k++) halfs[h][p][lane * NP + k] = s.v[k];
-            if (lane == 0) halfmin[h][p] = m;
+            for (int k = 0; k < NP; k++) h0.v[k] = (h0.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
         }
-    __syncthreads();
-    for (int i = 0; i < a.nvis; i++) {
-        const int y = a.visits[4 * i], cur = a.visits[4 * i + 1];
-        const int lab[2] = {a.visits[4 * i + 2], a.visits[4 * i + 3]};
-        const size_t cell = (size_t)y * W + a.col;
-        Cell<NP> raw, agg;
-        load_raw<NP>(a.raw + cell * cs + lo, active, raw);
-        load_u16<NP>(a.agg + cell * cs + lo, active, agg);
-        Cell<NP> lc[4];
-        uint32_t mins[4];
-        for (int p = 0; p < 4; p++) {
-            Cell<NP> prev;
-            uint32_t pm = 2000;
-            set_init(prev);
-            if (p < 2) {
-                const int l = lab[p];
-                if (l >= 0) {
-                    load_u16<NP>(a.cols + ((size_t)p * a.H + l) * cs + lo, active, prev);
-                    pm = a.colmins[p * a.H + l];
-                } else if (l <= -2) {
-                    const int c = -2 - l;
-                    load_u16<NP>(a.seeds + ((size_t)0 * W + c) * cs + lo, active, prev);  // path 1 seed
-                    pm = a.seedmins[0 * W + c];
-                }
-                if (MASKED) {
-#pragma unroll
-                    for (int k = 0; k < NP; k++) prev.v[k] = (prev.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
-                }
-            } else if (p == 2) {
-#pragma unroll
-                for (int k = 0; k < NP; k++) prev.v[k] = halfs[1 - cur][1][lane * NP + k];
-                pm = halfmin[1 - cur][1];
-            }
-            mins[p] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc[p]);
-        }
-#pragma unroll
-        for (int k = 0; k < NP; k++) agg.v[k] = padd(padd(agg.v[k], lc[0].v[k]), padd(lc[1].v[k], padd(lc[2].v[k], lc[3].v[k])));
-        store_u16<NP>(a.agg + cell * cs + lo, active, agg);
-        __syncthreads();
-        for (int p = 1; p < 4; p++) {
-#pragma unroll
-            for (int k = 0; k < NP; k++) halfs[cur][p - 1][lane * NP + k] = lc[p].v[k];
-            if (lane == 0) halfmin[cur][p - 1] = mins[p];
-        }
-        __syncthreads();
+        m0 = a.seedmins[1 * a.W + a.col];
     }
-    if (a.dir > 0) {  // the seed table's entry for the border cell: half 0 as pass +1 leaves it
-        for (int p = 0; p < 3; p++) {
-            Cell<NP> s;
+    Cell<NP> raw, part, agg, nraw, npart, nagg;
+    int y = 0, cur = 0, ny = 0, ncur = 0;
+    if (a.nvis > 0) {
+        y = a.visits[0];
+        cur = a.visits[1];
+        load_raw<NP>(a.raw + ((size_t)y * a.W + a.col) * cs + lo, active, raw);
+        load_u16<NP>(a.partial + lo, active, part);
+        load_u16<NP>(a.agg + ((size_t)y * a.W + a.col) * cs + lo, active, agg);
+    }
+    for (int i = 0; i < a.nvis; i++) {
+        if (i + 1 < a.nvis) {
+            ny = a.visits[4 * (i + 1)];
+            ncur = a.visits[4 * (i + 1) + 1];
+            load_raw<NP>(a.raw + ((size_t)ny * a.W + a.col) * cs + lo, active, nraw);
+            load_u16<NP>(a.partial + (size_t)(i + 1) * cs + lo, active, npart);
+            // a later visit of the same cell must see this visit's aggregate: forward it instead of reloading
+            if (ny != y) load_u16<NP>(a.agg + ((size_t)ny * a.W + a.col) * cs + lo, active, nagg);
+        }
+        Cell<NP> lc;
+        uint32_t mn;
+        if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
+        else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
 #pragma unroll
-            for (int k = 0; k < NP; k++) s.v[k] = halfs[0][p][lane * NP + k];
-            store_u16<NP>(a.seeds + ((size_t)p * W + a.col) * cs + lo, active, s);
-            if (lane == 0) a.seedmins[p * W + a.col] = (uint16_t)halfmin[0][p];
+        for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], padd(part.v[k], lc.v[k]));
+        store_u16<NP>(a.agg + ((size_t)y * a.W + a.col) * cs + lo, active, agg);
+        if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
+        if (i + 1 < a.nvis) {
+            if (ny != y) agg = nagg;
+            raw = nraw;
+            part = npart;
+            y = ny;
+            cur = ncur;
+        }
+    }
+    if (a.dir > 0) {
+        // path 2: the chain's half-0 state; paths 1 and 3: recomputed for the last visit that wrote half 0
+        store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, active, h0);
+        if (lane == 0) a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
+        int last = -1;
+        for (int i = a.nvis - 1; i >= 0; i--)
+            if (a.visits[4 * i + 1] == 0) { last = i; break; }
+        Cell<NP> s1, s3;
+        uint32_t sm1 = 2000, sm3 = 2000;
+        set_init(s1);
+        set_init(s3);
+        if (last >= 0) {
+            const int yl = a.visits[4 * last];
+            Cell<NP> r, prev;
+            uint32_t pm;
+            load_raw<NP>(a.raw + ((size_t)yl * a.W + a.col) * cs + lo, active, r);
+            label_state<NP, MASKED>(a, 1, a.visits[4 * last + 3], lo, active, mask, prev, pm);
+            sm1 = path_step<NP, MASKED>(prev, pm, r, a.p1x2, a.p2, mask, s1);
+            set_init(prev);
+            sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, a.p2, mask, s3);
+        }
+        store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, active, s1);
+        store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, active, s3);
+        if (lane == 0) {
+            a.seedmins[0 * a.W + a.col] = (uint16_t)sm1;
+            a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
         }
     }
 }
@@ -390,38 +443,36 @@ __global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* 
     out[(size_t)y * W + x] = code;
 }
 
-// Four planes per thread: consecutive threads write consecutive 4-byte groups of a cell.
-__global__ void cost_census_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int D, int cs, uint8_t* raw)
+// Four planes per thread: consecutive threads write consecutive 4-byte groups of a cell; blockIdx.y = row.
+__global__ __launch_bounds__(kBlock) void cost_census_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int D,
+                                                              int cs, uint8_t* raw)
 {
     const int groups = cs >> 2;
-    const size_t total = (size_t)W * H * groups;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t cell = t / groups;
-        const int d0 = (int)(t - cell * groups) * 4;
-        const int y = (int)(cell / W), x = (int)(cell - (size_t)y * W);
-        const uint64_t a = cl[cell];
-        const uint64_t* rrow = cr + (size_t)y * W;
-        uint32_t w = 0;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= W * groups) return;
+    const int y = blockIdx.y;
+    const int x = t / groups, d0 = (t - x * groups) * 4;
+    const uint64_t* rrow = cr + (size_t)y * W;
+    const uint64_t a = cl[(size_t)y * W + x];
+    uint32_t w = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int d = d0 + j;
-            const uint32_t v = d < D ? (uint32_t)__popcll(a ^ rrow[max(x - d, 0)]) : 255u;
-            w |= v << (8 * j);
-        }
-        ((uint32_t*)raw)[t] = w;
+    for (int j = 0; j < 4; j++) {
+        const int d = d0 + j;
+        const uint32_t v = d < D ? (uint32_t)__popcll(a ^ rrow[max(x - d, 0)]) : 255u;
+        w |= v << (8 * j);
     }
+    ((uint32_t*)(raw + (size_t)y * W * cs))[t] = w;
 }
 
 template <typename T>
-__global__ void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride, int D, int cs, int shift, int adcap,
-                               uint8_t* raw)
+__global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride, int D, int cs,
+                                                          int shift, int adcap, uint8_t* raw)
 {
     const int groups = cs >> 2;
-    const size_t total = (size_t)W * H * groups;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t cell = t / groups;
-        const int d0 = (int)(t - cell * groups) * 4;
-        const int y = (int)(cell / W), x = (int)(cell - (size_t)y * W);
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < W * groups) {
+        const int y = blockIdx.y;
+        const int x = t / groups, d0 = (t - x * groups) * 4;
         int lv[9];
 #pragma unroll
         for (int dy = -1; dy <= 1; dy++)
@@ -452,65 +503,103 @@ __global__ void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride,
 }
 
 // ------------------------------------------------------------------ WTA + subpixel + L/R
-// One row per workgroup. dL: first minimum over planes [0, D) (:225-231), found as the minimum of the
-// key (agg << 16 | d). dR(xr) = first minimum of agg(xr+d, d): the same keys, min-combined into an LDS
-// row with atomicMin. Subpixel: parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly
-// rounded division, no contraction), no offset at the range ends.
+// Plane q of a cell held in lanes (2*NP planes per lane), read wave-uniformly.
 template <int NP>
-__global__ __launch_bounds__(kBlock) void wta_kernel(const uint16_t* agg, int W, int H, int cs, int D, int tol,
-                                                      uint16_t* disp, float* sub, uint8_t* valid)
+__device__ __forceinline__ uint32_t plane_of(const Cell<NP>& c, int q)
 {
-    extern __shared__ uint32_t lds[];
-    uint32_t* dr = lds;                       // W keys
-    uint16_t* dl = (uint16_t*)(lds + W);      // W disparities
-    const int y = blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < W; i += kBlock) dr[i] = 0xffffffffu;
-    __syncthreads();
-    const int lo = lane * 2 * NP;
+    const int ln = q / (2 * NP), e = q - ln * 2 * NP;
+    uint32_t w = c.v[0];
+#pragma unroll
+    for (int k = 1; k < NP; k++) w = (e >> 1) == k ? c.v[k] : w;
+    w = (uint32_t)__builtin_amdgcn_readlane((int)w, ln);
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// One wavefront per row, marching along x.
+//  dL(x): first minimum over planes [0, D) (DenseStereoSgm.cs:225-231) = the minimum of the keys
+//         agg << 16 | d over the cell.
+//  dR(xr) = first minimum over d of agg(xr+d, d): a shift register of keys along the planes. After cell x,
+//         register plane p holds min_{j<=p} key(x-p+j, j), so plane D-1 completes xr = x-D+1; each step
+//         shifts it by one plane (v_mov inside a lane, DPP wave_shr across lanes) and folds in the cell's keys.
+//  sub(x): parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly rounded, no contraction);
+//         no offset at the range ends.
+//  valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol.
+template <int NP>
+__global__ __launch_bounds__(64) void wta_rows(const uint16_t* agg, int W, int H, int cs, int D, int tol, uint16_t* disp,
+                                               float* sub, uint8_t* valid)
+{
+    constexpr int E = 2 * NP;
+    extern __shared__ uint16_t wlds[];
+    uint16_t* dl = wlds;
+    uint16_t* dr = wlds + W;
+    const int y = blockIdx.x, lane = threadIdx.x, lo = lane * E;
     const bool active = lo < cs;
-    for (int x = wave; x < W; x += kBlock / 64) {
-        const size_t cell = (size_t)y * W + x;
-        Cell<NP> a;
-        load_u16<NP>(agg + cell * cs + lo, active, a);
+    const int lastLane = (D - 1) / E, lastE = (D - 1) - lastLane * E;
+    uint32_t R[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
+    const uint16_t* row = agg + (size_t)y * W * cs;
+    Cell<NP> c, nc;
+    load_u16<NP>(row + lo, active, c);
+    uint32_t myd = 0;
+    float mys = 0.f;
+    const int steps = W + D - 1;
+    for (int x = 0; x < steps; x++) {
+        const bool have = x < W;
+        if (x + 1 < W) load_u16<NP>(row + (size_t)(x + 1) * cs + lo, active, nc);
+        uint32_t key[E];
         uint32_t best = 0xffffffffu;
 #pragma unroll
-        for (int k = 0; k < NP; k++)
+        for (int e = 0; e < E; e++) {
+            const int d = lo + e;
+            const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
+            key[e] = (have && d < D) ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+            best = key[e] < best ? key[e] : best;
+        }
+        // right-image disparity: shift by one plane, fold in this cell's keys, emit plane D-1
+        const uint32_t carry = shr1(R[E - 1], 0xffffffffu);
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int d = lo + 2 * k + h;
-                if (d < D) {
-                    const uint32_t key = ((h ? a.v[k] >> 16 : a.v[k] & 0xffffu) << 16) | (uint32_t)d;
-                    best = key < best ? key : best;
-                    if (x - d >= 0) atomicMin(&dr[x - d], key);
-                }
-            }
-        best = wave_min(best);
-        if (lane == 0) {
+        for (int e = E - 1; e > 0; e--) R[e] = R[e - 1];
+        R[0] = carry;
+#pragma unroll
+        for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+        const int xr = x - (D - 1);
+        if (xr >= 0) {
+            uint32_t w = R[0];
+#pragma unroll
+            for (int e = 1; e < E; e++) w = lastE == e ? R[e] : w;
+            w = (uint32_t)__builtin_amdgcn_readlane((int)w, lastLane);
+            if (lane == 0) dr[xr] = (uint16_t)(w & 0xffffu);
+        }
+        if (have) {
+            best = wave_min(best);
             const int d = (int)(best & 0xffffu);
-            dl[x] = (uint16_t)d;
-            if (disp) disp[cell] = (uint16_t)d;
-            if (sub) {
-                float v = (float)d;
-                if (d > 0 && d < D - 1) {
-                    const int l = agg[cell * cs + d - 1], c = (int)(best >> 16), r = agg[cell * cs + d + 1];
-                    const int den = l - 2 * c + r;
-                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
-                }
-                sub[cell] = v;
+            float v = (float)d;
+            if (d > 0 && d < D - 1) {
+                const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
+                const int den = l - 2 * cc + r;
+                if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
             }
+            if (lane == (x & 63)) {
+                myd = (uint32_t)d;
+                mys = v;
+            }
+            if ((x & 63) == 63 || x == W - 1) {  // flush 64 consecutive pixels, coalesced
+                const int x0 = x & ~63;
+                if (lane <= (x & 63)) {
+                    dl[x0 + lane] = (uint16_t)myd;
+                    if (disp) disp[(size_t)y * W + x0 + lane] = (uint16_t)myd;
+                    if (sub) sub[(size_t)y * W + x0 + lane] = mys;
+                }
+            }
+            c = nc;
         }
     }
     __syncthreads();
     if (valid)
-        for (int x = threadIdx.x; x < W; x += kBlock) {
+        for (int x = lane; x < W; x += 64) {
             const int d = dl[x], xr = x - d;
-            uint8_t ok = 0;
-            if (xr >= 0) {
-                const int r = (int)(dr[xr] & 0xffffu);
-                ok = (uint8_t)(abs(d - r) <= tol);
-            }
-            valid[(size_t)y * W + x] = ok;
+            valid[(size_t)y * W + x] = (uint8_t)(xr >= 0 && abs(d - (int)dr[xr]) <= tol);
         }
 }
 
@@ -565,8 +654,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
     r.p1x2 = a.p1x2; r.p2 = a.p2;
     r.seeds = b.seeds; r.seedmins = b.seedmins;
     r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
-    r.cols = b.cols; r.colmins = b.colmins;
-    sgm_replay<NP, MASKED><<<1, 64, 0, s>>>(r);
+    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
     mark();
     if (agg_pass1) {
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
@@ -583,8 +673,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
     sgm_sweep<NP, -1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
     mark();
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
-    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz;
-    sgm_replay<NP, MASKED><<<1, 64, 0, s>>>(r);
+    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
     mark();
     return hipGetLastError();
 }
@@ -619,22 +710,22 @@ hipError_t launch_census(const void* img, bool img16, int W, int H, int stride, 
 
 hipError_t launch_cost_census(const uint64_t* cl, const uint64_t* cr, const Geometry& g, uint8_t* raw, hipStream_t s)
 {
-    const size_t total = (size_t)g.W * g.H * (g.cs / 4);
-    cost_census_kernel<<<grid_for(total, 256), 256, 0, s>>>(cl, cr, g.W, g.H, g.D, g.cs, raw);
+    dim3 grd((g.W * (g.cs / 4) + kBlock - 1) / kBlock, g.H);
+    cost_census_kernel<<<grd, kBlock, 0, s>>>(cl, cr, g.W, g.H, g.D, g.cs, raw);
     return hipGetLastError();
 }
 
 hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, int stride, int adcap, const Geometry& g,
                           uint8_t* raw, hipStream_t s)
 {
-    const size_t total = (size_t)g.W * g.H * (g.cs / 4);
+    dim3 grd((g.W * (g.cs / 4) + kBlock - 1) / kBlock, g.H);
     const int shift = img16 ? bits - 8 : 0;
     if (img16)
-        cost_ad_kernel<uint16_t><<<grid_for(total, 256), 256, 0, s>>>((const uint16_t*)L, (const uint16_t*)R, g.W, g.H,
-                                                                      stride, g.D, g.cs, shift, adcap, raw);
+        cost_ad_kernel<uint16_t><<<grd, kBlock, 0, s>>>((const uint16_t*)L, (const uint16_t*)R, g.W, g.H, stride, g.D, g.cs,
+                                                        shift, adcap, raw);
     else
-        cost_ad_kernel<uint8_t><<<grid_for(total, 256), 256, 0, s>>>((const uint8_t*)L, (const uint8_t*)R, g.W, g.H,
-                                                                     stride, g.D, g.cs, shift, adcap, raw);
+        cost_ad_kernel<uint8_t><<<grd, kBlock, 0, s>>>((const uint8_t*)L, (const uint8_t*)R, g.W, g.H, stride, g.D, g.cs,
+                                                       shift, adcap, raw);
     return hipGetLastError();
 }
 
@@ -654,12 +745,12 @@ hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, ui
 hipError_t launch_wta(const Geometry& g, const uint16_t* agg, int tol, uint16_t* disp, float* sub, uint8_t* valid,
                       hipStream_t s)
 {
-    const size_t lds = (size_t)g.W * 4 + (size_t)g.W * 2 + 16;
+    const size_t lds = (size_t)g.W * 4;
     switch (g.dpl) {
-        case 2: wta_kernel<1><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 4: wta_kernel<2><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 8: wta_kernel<4><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 16: wta_kernel<8><<<g.H, kBlock, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 2: wta_rows<1><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 4: wta_rows<2><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 8: wta_rows<4><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
+        case 16: wta_rows<8><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
