@@ -117,7 +117,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     if ((rc = get_buf(c, "seedmins", (size_t)3 * g.W * 2, &seedmins))) return rc;
     if ((rc = get_buf(c, "cols", (size_t)4 * g.H * g.cs * 2, &cols))) return rc;
     if ((rc = get_buf(c, "colmins", (size_t)4 * g.H * 2, &colmins))) return rc;
-    const size_t nv = plan->plus.size() + plan->minus.size();
+    const size_t nv = plan->plus.size() + plan->minus.size() + 4 * (size_t)g.H;
     const size_t maxvis = std::max(plan->plus.size(), plan->minus.size()) / 4 + 1;
     if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
@@ -125,6 +125,12 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     if (e == hipSuccess)
         e = hipMemcpyAsync((int*)vis + plan->plus.size(), plan->minus.data(), plan->minus.size() * sizeof(int),
                            hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((int*)vis + plan->plus.size() + plan->minus.size(), plan->cellvis_plus.data(),
+                           plan->cellvis_plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((int*)vis + plan->plus.size() + plan->minus.size() + 2 * (size_t)g.H,
+                           plan->cellvis_minus.data(), plan->cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(c, e, "upload plan");
     w->g = g;
     w->raw = (uint8_t*)raw;
@@ -140,6 +146,8 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.n_plus = (int)plan->plus.size() / 4;
     w->b.visits_minus = (const int*)vis + plan->plus.size();
     w->b.n_minus = (int)plan->minus.size() / 4;
+    w->b.cellvis_plus = (const int*)vis + plan->plus.size() + plan->minus.size();
+    w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
     w->b.seed_row = plan->seed_row;
     return 0;
 }

@@ -25,6 +25,8 @@ struct SgmBuffers {
     int n_plus = 0;
     const int* visits_minus = nullptr;
     int n_minus = 0;
+    const int* cellvis_plus = nullptr;   // [H][2] visit indices of each border-column row (-1: none)
+    const int* cellvis_minus = nullptr;
     int seed_row = -1;
 };
 

@@ -326,12 +326,14 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 
 // Path 2 of the border column: one chain through the column's visits in schedule order. Each visit reads
 // the state the other line-buffer half holds for the border slot and leaves its own in the current half
-// (:219, :265); every visit adds its four path costs to the cell, so double-visited cells get both sets
-// (:217, :296). Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's
-// seed-table entry (half 0 as the pass leaves it).
+// (:219, :265). The chain adds its path cost to the visit's `partial` (the aggregate is updated afterwards,
+// per cell, by sgm_replay_add), so it only reads data fixed before it starts and can prefetch deeply.
+// Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's seed-table
+// entry (half 0 as the pass leaves it).
 template <int NP, bool MASKED>
 __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
 {
+    constexpr int PF = 8;
     const int lane = threadIdx.x;
     const int cs = a.cs, lo = lane * 2 * NP;
     Cell<NP> mask;
@@ -349,38 +351,36 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
         }
         m0 = a.seedmins[1 * a.W + a.col];
     }
-    Cell<NP> raw, part, agg, nraw, npart, nagg;
-    int y = 0, cur = 0, ny = 0, ncur = 0;
-    if (a.nvis > 0) {
-        y = a.visits[0];
-        cur = a.visits[1];
-        load_raw<NP>(a.raw + ((size_t)y * a.W + a.col) * cs + lo, active, raw);
-        load_u16<NP>(a.partial + lo, active, part);
-        load_u16<NP>(a.agg + ((size_t)y * a.W + a.col) * cs + lo, active, agg);
-    }
-    for (int i = 0; i < a.nvis; i++) {
-        if (i + 1 < a.nvis) {
-            ny = a.visits[4 * (i + 1)];
-            ncur = a.visits[4 * (i + 1) + 1];
-            load_raw<NP>(a.raw + ((size_t)ny * a.W + a.col) * cs + lo, active, nraw);
-            load_u16<NP>(a.partial + (size_t)(i + 1) * cs + lo, active, npart);
-            // a later visit of the same cell must see this visit's aggregate: forward it instead of reloading
-            if (ny != y) load_u16<NP>(a.agg + ((size_t)ny * a.W + a.col) * cs + lo, active, nagg);
-        }
-        Cell<NP> lc;
-        uint32_t mn;
-        if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
-        else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
+    Cell<NP> rb[PF], pb[PF];
 #pragma unroll
-        for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], padd(part.v[k], lc.v[k]));
-        store_u16<NP>(a.agg + ((size_t)y * a.W + a.col) * cs + lo, active, agg);
-        if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
-        if (i + 1 < a.nvis) {
-            if (ny != y) agg = nagg;
-            raw = nraw;
-            part = npart;
-            y = ny;
-            cur = ncur;
+    for (int j = 0; j < PF; j++)
+        if (j < a.nvis) {
+            const int yj = a.visits[4 * j];
+            load_raw<NP>(a.raw + ((size_t)yj * a.W + a.col) * cs + lo, active, rb[j]);
+            load_u16<NP>(a.partial + (size_t)j * cs + lo, active, pb[j]);
+        }
+    for (int i0 = 0; i0 < a.nvis; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int i = i0 + j;
+            if (i < a.nvis) {
+                const int cur = a.visits[4 * i + 1];
+                const Cell<NP> raw = rb[j], part = pb[j];
+                const int ip = i + PF;
+                if (ip < a.nvis) {
+                    const int yp = a.visits[4 * ip];
+                    load_raw<NP>(a.raw + ((size_t)yp * a.W + a.col) * cs + lo, active, rb[j]);
+                    load_u16<NP>(a.partial + (size_t)ip * cs + lo, active, pb[j]);
+                }
+                Cell<NP> lc, sum;
+                uint32_t mn;
+                if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
+                else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
+#pragma unroll
+                for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
+                store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
+                if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
+            }
         }
     }
     if (a.dir > 0) {
@@ -411,6 +411,31 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
             a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
         }
     }
+}
+
+// Adds every border visit's four path costs to its cell (one wavefront per row of the border column;
+// `cellvis` lists each row's visits, at most two per pass).
+template <int NP>
+__global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int* cellvis)
+{
+    const int lane = threadIdx.x & 63;
+    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (y >= a.H) return;
+    const int lo = lane * 2 * NP;
+    const bool active = lo < a.cap;
+    const size_t cell = ((size_t)y * a.W + a.col) * a.cs + lo;
+    Cell<NP> agg, p;
+    load_u16<NP>(a.agg + cell, active, agg);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int i = cellvis[2 * y + j];
+        if (i >= 0) {
+            load_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, p);
+#pragma unroll
+            for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], p.v[k]);
+        }
+    }
+    store_u16<NP>(a.agg + cell, active, agg);
 }
 
 // Seeds of interior cells when pass +1 never writes half 0 (H = 1): the initial fill.
@@ -498,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R,
             }
             w |= v << (8 * j);
         }
-        ((uint32_t*)raw)[t] = w;
+        ((uint32_t*)(raw + (size_t)y * W * cs))[t] = w;
     }
 }
 
@@ -539,60 +564,68 @@ __global__ __launch_bounds__(64) void wta_rows(const uint16_t* agg, int W, int H
 #pragma unroll
     for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
     const uint16_t* row = agg + (size_t)y * W * cs;
-    Cell<NP> c, nc;
-    load_u16<NP>(row + lo, active, c);
+    constexpr int PF = 4;  // cells in flight ahead of the one being reduced
+    Cell<NP> buf[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (j < W) load_u16<NP>(row + (size_t)j * cs + lo, active, buf[j]);
     uint32_t myd = 0;
     float mys = 0.f;
     const int steps = W + D - 1;
-    for (int x = 0; x < steps; x++) {
-        const bool have = x < W;
-        if (x + 1 < W) load_u16<NP>(row + (size_t)(x + 1) * cs + lo, active, nc);
-        uint32_t key[E];
-        uint32_t best = 0xffffffffu;
+    for (int x0 = 0; x0 < steps; x0 += PF) {
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int d = lo + e;
-            const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
-            key[e] = (have && d < D) ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
-            best = key[e] < best ? key[e] : best;
-        }
-        // right-image disparity: shift by one plane, fold in this cell's keys, emit plane D-1
-        const uint32_t carry = shr1(R[E - 1], 0xffffffffu);
+        for (int j = 0; j < PF; j++) {
+            const int x = x0 + j;
+            if (x >= steps) break;
+            const bool have = x < W;
+            const Cell<NP> c = buf[j];
+            if (x + PF < W) load_u16<NP>(row + (size_t)(x + PF) * cs + lo, active, buf[j]);
+            uint32_t key[E];
+            uint32_t best = 0xffffffffu;
 #pragma unroll
-        for (int e = E - 1; e > 0; e--) R[e] = R[e - 1];
-        R[0] = carry;
-#pragma unroll
-        for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
-        const int xr = x - (D - 1);
-        if (xr >= 0) {
-            uint32_t w = R[0];
-#pragma unroll
-            for (int e = 1; e < E; e++) w = lastE == e ? R[e] : w;
-            w = (uint32_t)__builtin_amdgcn_readlane((int)w, lastLane);
-            if (lane == 0) dr[xr] = (uint16_t)(w & 0xffffu);
-        }
-        if (have) {
-            best = wave_min(best);
-            const int d = (int)(best & 0xffffu);
-            float v = (float)d;
-            if (d > 0 && d < D - 1) {
-                const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
-                const int den = l - 2 * cc + r;
-                if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+            for (int e = 0; e < E; e++) {
+                const int d = lo + e;
+                const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
+                key[e] = (have && d < D) ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+                best = key[e] < best ? key[e] : best;
             }
-            if (lane == (x & 63)) {
-                myd = (uint32_t)d;
-                mys = v;
+            // right-image disparity: shift by one plane, fold in this cell's keys, emit plane D-1
+            const uint32_t carry = shr1(R[E - 1], 0xffffffffu);
+#pragma unroll
+            for (int e = E - 1; e > 0; e--) R[e] = R[e - 1];
+            R[0] = carry;
+#pragma unroll
+            for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+            const int xr = x - (D - 1);
+            if (xr >= 0) {
+                uint32_t w = R[0];
+#pragma unroll
+                for (int e = 1; e < E; e++) w = lastE == e ? R[e] : w;
+                w = (uint32_t)__builtin_amdgcn_readlane((int)w, lastLane);
+                if (lane == 0) dr[xr] = (uint16_t)(w & 0xffffu);
             }
-            if ((x & 63) == 63 || x == W - 1) {  // flush 64 consecutive pixels, coalesced
-                const int x0 = x & ~63;
-                if (lane <= (x & 63)) {
-                    dl[x0 + lane] = (uint16_t)myd;
-                    if (disp) disp[(size_t)y * W + x0 + lane] = (uint16_t)myd;
-                    if (sub) sub[(size_t)y * W + x0 + lane] = mys;
+            if (have) {
+                best = wave_min(best);
+                const int d = (int)(best & 0xffffu);
+                float v = (float)d;
+                if (d > 0 && d < D - 1) {
+                    const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
+                    const int den = l - 2 * cc + r;
+                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+                }
+                if (lane == (x & 63)) {
+                    myd = (uint32_t)d;
+                    mys = v;
+                }
+                if ((x & 63) == 63 || x == W - 1) {  // flush 64 consecutive pixels, coalesced
+                    const int xb = x & ~63;
+                    if (lane <= (x & 63)) {
+                        dl[xb + lane] = (uint16_t)myd;
+                        if (disp) disp[(size_t)y * W + xb + lane] = (uint16_t)myd;
+                        if (sub) sub[(size_t)y * W + xb + lane] = mys;
+                    }
                 }
             }
-            c = nc;
         }
     }
     __syncthreads();
@@ -657,6 +690,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
     r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
     sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
+    sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
     mark();
     if (agg_pass1) {
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
@@ -676,6 +710,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
     sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
+    sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
     mark();
     return hipGetLastError();
 }

@@ -159,6 +159,18 @@ Plan make_plan(int W, int H, int T, int xsw)
             }
         }
     }
+    // per-row visit index lists of the border columns (at most two visits per cell and pass)
+    for (int pi = 0; pi < 2 && P.status == 0; pi++) {
+        const std::vector<int>& list = pi == 0 ? P.plus : P.minus;
+        std::vector<int>& cv = pi == 0 ? P.cellvis_plus : P.cellvis_minus;
+        cv.assign((size_t)2 * H, -1);
+        for (size_t i = 0; i < list.size() / 4; i++) {
+            int y = list[4 * i];
+            if (cv[2 * y] < 0) cv[2 * y] = (int)i;
+            else if (cv[2 * y + 1] < 0) cv[2 * y + 1] = (int)i;
+            else fail("a border cell is visited more than twice in one pass");
+        }
+    }
     // every cell must have been visited in both passes; interior tracked cells exactly once
     for (int pi = 0; pi < 2; pi++)
         for (size_t k = 0; k < tr.cells.size(); k++) {

@@ -14,6 +14,8 @@ struct Plan {
     // (label -1: initial fill; y' >= 0: neighbour column at row y', same pass; -2-c: pass +1 seed of cell c)
     std::vector<int> plus;   // pass +1, column W-1
     std::vector<int> minus;  // pass -1, column 0
+    // per border-column row, the indices of its visits in plus / minus (two slots, -1 = none)
+    std::vector<int> cellvis_plus, cellvis_minus;
 };
 
 Plan make_plan(int W, int H, int T, int xsw);
