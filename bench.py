@@ -23,14 +23,16 @@ P1, P2, TILE, RADIUS, BITS = 8, 96, 64, 3, 10
 HBM_PEAK_GBS = 8000.0
 PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1, agg 2), WTA fused
 # algorithmic HBM bytes per cell of each stage (what the stage must move at minimum)
-STAGE_BYTES = {"cost": 1, "sweep+0": 3, "wta": 2}
-for s in ("sweep+1", "sweep+2", "sweep+3", "sweep-0", "sweep-1", "sweep-2", "sweep-3"):
+# sweep+0 computes the census cost and writes raw (1) + agg (2); sweep-0 reads raw (1) + agg (2) and writes
+# only the per-pixel WTA outputs; the others read raw (1) and read-modify-write agg (2 + 2)
+STAGE_BYTES = {"sweep+0": 3, "sweep-0": 3}
+for s in ("sweep+1", "sweep+2", "sweep+3", "sweep-1", "sweep-2", "sweep-3"):
     STAGE_BYTES[s] = 5
-KERNEL_NAMES = {"sweep+0": "sgm_sweep<pass +1, path 0 (->)>", "sweep+1": "sgm_sweep<pass +1, path 1 (down-right)>",
-                "sweep+2": "sgm_sweep<pass +1, path 2 (down)>", "sweep+3": "sgm_sweep<pass +1, path 3 (down-left)>",
-                "sweep-0": "sgm_sweep<pass -1, path 0 (<-)>", "sweep-1": "sgm_sweep<pass -1, path 1 (up-left)>",
+KERNEL_NAMES = {"sweep+0": "sgm_row_fwd (pass +1 path 0, census cost fused)",
+                "sweep+1": "sgm_sweep<pass +1, path 1 (down-right)>", "sweep+2": "sgm_sweep<pass +1, path 2 (down)>",
+                "sweep+3": "sgm_sweep<pass +1, path 3 (down-left)>", "sweep-1": "sgm_sweep<pass -1, path 1 (up-left)>",
                 "sweep-2": "sgm_sweep<pass -1, path 2 (up)>", "sweep-3": "sgm_sweep<pass -1, path 3 (up-right)>",
-                "cost": "census_kernel x2 + cost_census_kernel", "wta": "wta_kernel"}
+                "sweep-0": "sgm_row_bwd_wta (pass -1 path 0, WTA fused)"}
 
 
 def host_info():

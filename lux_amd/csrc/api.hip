@@ -110,7 +110,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     int rc = get_plan(c, g.W, g.H, T, xsw, &plan);
     if (rc) return rc;
     const size_t cells = (size_t)g.W * g.H;
-    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial;
+    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial, *drb, *dr0;
     if ((rc = get_buf(c, "raw", cells * g.cs, &raw))) return rc;
     if ((rc = get_buf(c, "agg", cells * g.cs * 2, &agg))) return rc;
     if ((rc = get_buf(c, "seeds", (size_t)3 * g.W * g.cs * 2, &seeds))) return rc;
@@ -120,6 +120,8 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     const size_t nv = plan->plus.size() + plan->minus.size() + 4 * (size_t)g.H;
     const size_t maxvis = std::max(plan->plus.size(), plan->minus.size()) / 4 + 1;
     if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
+    if ((rc = get_buf(c, "wta_dr", cells * 2, &drb))) return rc;
+    if ((rc = get_buf(c, "wta_dr0", (size_t)g.H * 4, &dr0))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
     hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), plan->plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -136,7 +138,10 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->raw = (uint8_t*)raw;
     w->agg = (uint16_t*)agg;
     w->b.raw = w->raw;
+    w->b.raw_out = w->raw;
     w->b.agg = w->agg;
+    w->b.dr = (uint16_t*)drb;
+    w->b.dr0 = (uint32_t*)dr0;
     w->b.seeds = (uint16_t*)seeds;
     w->b.seedmins = (uint16_t*)seedmins;
     w->b.cols = (uint16_t*)cols;
@@ -156,7 +161,7 @@ int check_rect(lux_sgm_ctx* c, const lux_sgm_rect_params* p)
 {
     if (!p) return fail(c, LUX_SGM_EINVAL, "null params");
     if (p->width <= 0 || p->height <= 0 || p->stride < p->width) return fail(c, LUX_SGM_EINVAL, "bad image size/stride");
-    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240 (one LDS row of the WTA)");
+    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240");
     if (!(p->bits == 8 || (p->bits >= 9 && p->bits <= 16))) return fail(c, LUX_SGM_EINVAL, "bits must be 8 or 9..16");
     if (p->cost != LUX_SGM_COST_CENSUS && p->cost != LUX_SGM_COST_AD) return fail(c, LUX_SGM_EINVAL, "unknown cost");
     if (p->cost == LUX_SGM_COST_CENSUS && (p->census_radius < 1 || p->census_radius > 3))
@@ -181,30 +186,30 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
     const bool img16 = p->bits > 8;
     hipError_t e;
     if (c->profiling) (void)hipEventRecord(c->ev[0], s);
-    if (p->cost == LUX_SGM_COST_CENSUS) {
+    if (p->cost == LUX_SGM_COST_CENSUS) {  // the codes here; the cost volume inside the first sweep
         void *cl, *cr;
         const size_t n = (size_t)g.W * g.H * 8;
         if ((rc = get_buf(c, "census_l", n, &cl)) || (rc = get_buf(c, "census_r", n, &cr))) return rc;
         e = lux::launch_census(dl, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)cl, s);
         if (e == hipSuccess) e = lux::launch_census(dr, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)cr, s);
-        if (e == hipSuccess) e = lux::launch_cost_census((const uint64_t*)cl, (const uint64_t*)cr, g, w.raw, s);
+        w.b.cl = (const uint64_t*)cl;
+        w.b.cr = (const uint64_t*)cr;
     } else {
         e = lux::launch_cost_ad(dl, dr, img16, p->bits, p->stride, p->ad_cap, g, w.raw, s);
     }
     if (e != hipSuccess) return hip_fail(c, e, "cost volume");
     if (c->profiling) (void)hipEventRecord(c->ev[1], s);
-    e = lux::launch_sgm(g, w.b, p->p1, p->p2, nullptr, s, c->profiling ? c->ev + 2 : nullptr);
+    lux::SgmOutputs o;
+    void* td = disp;
+    if (!disp && (rc = get_buf(c, "disp_tmp", (size_t)g.W * g.H * 2, &td))) return rc;
+    o.disp = (uint16_t*)td;
+    o.sub = sub;
+    o.valid = valid;
+    o.tol = p->lr_tolerance;
+    o.write_agg = agg_out != nullptr;
+    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, nullptr, s, c->profiling ? c->ev + 2 : nullptr);
     if (e != hipSuccess) return hip_fail(c, e, "aggregation");
-    if (disp || sub || valid) {
-        void* td = disp;
-        if (!disp && (rc = get_buf(c, "disp_tmp", (size_t)g.W * g.H * 2, &td))) return rc;
-        e = lux::launch_wta(g, w.agg, p->lr_tolerance, (uint16_t*)td, sub, valid, s);
-        if (e != hipSuccess) return hip_fail(c, e, "wta");
-    }
-    if (c->profiling) {
-        (void)hipEventRecord(c->ev[2 + lux::kSgmMarks], s);
-        c->nev = 3 + lux::kSgmMarks;
-    }
+    if (c->profiling) c->nev = 2 + lux::kSgmMarks;
     if (agg_out) {
         e = lux::launch_restride_u16(w.agg, g.cs, agg_out, g.cap, g.W * g.H, g.cap, s);
         if (e != hipSuccess) return hip_fail(c, e, "agg copy");
@@ -344,7 +349,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     if (!p || !raw) return fail(c, LUX_SGM_EINVAL, "null argument");
     if (p->width <= 0 || p->height <= 0 || p->planes <= 0 || p->align <= 0 || p->tile < 2)
         return fail(c, LUX_SGM_EINVAL, "bad layer parameters");
-    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240 (one LDS row of the WTA)");
+    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240");
     if (((p->planes + p->align - 1) / p->align) * p->align > 1024)
         return fail(c, LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
     if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535) return fail(c, LUX_SGM_EINVAL, "penalties must be 0..65535");
@@ -358,15 +363,18 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     void *stage, *a1 = nullptr, *dw = nullptr;
     if ((rc = get_buf(c, "layer_stage", cells * g.cap * 2, &stage))) return rc;
     if (agg_pass1 && (rc = get_buf(c, "layer_pass1", cells * g.cs * 2, &a1))) return rc;
-    if (wta && (rc = get_buf(c, "layer_wta", cells * 2, &dw))) return rc;
+    if ((rc = get_buf(c, "layer_wta", cells * 2, &dw))) return rc;
     hipError_t e = hipMemcpyAsync(stage, raw, cells * g.cap, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = lux::launch_restride_u8((const uint8_t*)stage, g.cap, w.raw, g.cs, (int)cells, g.cap, 255, s);
     if (e != hipSuccess) return hip_fail(c, e, "upload raw");
-    e = lux::launch_sgm(g, w.b, p->p1, p->p2, (uint16_t*)a1, s);
+    lux::SgmOutputs o;
+    o.disp = (uint16_t*)dw;
+    o.tol = 0;
+    o.write_agg = agg != nullptr;
+    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
     if (e != hipSuccess) return hip_fail(c, e, "aggregation");
     if (wta) {
-        e = lux::launch_wta(g, w.agg, 0, (uint16_t*)dw, nullptr, nullptr, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(c, e, "wta");
     }
     uint16_t* outs[2] = {agg, agg_pass1};

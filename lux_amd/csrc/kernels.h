@@ -15,6 +15,11 @@ Geometry geometry(int W, int H, int D, int align);
 
 struct SgmBuffers {
     const uint8_t* raw = nullptr;   // [H][W][cs]
+    uint8_t* raw_out = nullptr;     // same buffer, written by the first sweep when the census cost is fused
+    const uint64_t* cl = nullptr;   // census codes [H][W]; null: raw is given
+    const uint64_t* cr = nullptr;
+    uint16_t* dr = nullptr;         // [H][W] right-image disparity (scratch)
+    uint32_t* dr0 = nullptr;        // [H]
     uint16_t* agg = nullptr;        // [H][W][cs]
     uint16_t* seeds = nullptr;      // [3][W][cs]  pass +1 half-0 leftovers of paths 1..3
     uint16_t* seedmins = nullptr;   // [3][W]
@@ -33,18 +38,22 @@ struct SgmBuffers {
 // Census codes of one image (T = uint8_t or uint16_t pixels).
 hipError_t launch_census(const void* img, bool img16, int W, int H, int stride, int radius, uint64_t* codes,
                          hipStream_t s);
-// raw[y][x][d] for d < cs (census Hamming / truncated AD; 255 for d >= D).
-hipError_t launch_cost_census(const uint64_t* cl, const uint64_t* cr, const Geometry& g, uint8_t* raw, hipStream_t s);
+// raw[y][x][d] for d < cs (truncated AD; 255 for d >= D). The census cost is computed inside the first sweep.
 hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, int stride, int adcap, const Geometry& g,
                           uint8_t* raw, hipStream_t s);
-// The eight path sweeps and the two border replays; agg_pass1 (if not null) receives agg after pass +1.
-// marks (if not null, kSgmMarks events) are recorded after each of the 10 launches, in launch order.
-constexpr int kSgmMarks = 10;
-hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
-                      hipEvent_t* marks = nullptr);
-// First-minimum WTA + parabola subpixel + left/right check, one row per workgroup.
-hipError_t launch_wta(const Geometry& g, const uint16_t* agg, int tol, uint16_t* disp, float* sub, uint8_t* valid,
-                      hipStream_t s);
+struct SgmOutputs {
+    uint16_t* disp = nullptr;       // [H][W] required (WTA plane index)
+    float* sub = nullptr;           // may be null
+    uint8_t* valid = nullptr;       // may be null
+    int tol = 1;
+    int write_agg = 0;              // leave the final aggregate in the agg volume
+};
+// The eight path sweeps (the census cost fused into the first when b.cl is set, the winner-take-all into
+// the last), the two border replays and the column-0 / validity finish. agg_pass1 (if not null) receives
+// agg after pass +1. marks (if not null, kSgmMarks events) are recorded after each stage, in launch order.
+constexpr int kSgmMarks = 11;
+hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                      hipStream_t s, hipEvent_t* marks = nullptr);
 // Repack [H][W][cs] <-> [H][W][cap] on the device.
 hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s);
 hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,

@@ -146,6 +146,18 @@ __device__ __forceinline__ void set_init(Cell<NP>& c)
     for (int k = 0; k < NP; k++) c.v[k] = kInit2;
 }
 
+// Plane q of a cell held in lanes (2*NP planes per lane), read wave-uniformly.
+template <int NP>
+__device__ __forceinline__ uint32_t plane_of(const Cell<NP>& c, int q)
+{
+    const int ln = q / (2 * NP), e = q - ln * 2 * NP;
+    uint32_t w = c.v[0];
+#pragma unroll
+    for (int k = 1; k < NP; k++) w = (e >> 1) == k ? c.v[k] : w;
+    w = (uint32_t)__builtin_amdgcn_readlane((int)w, ln);
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
 // Chains of one path direction: the cells whose predecessor (x-DX, y-DY) lies outside the image start one.
 template <int DX, int DY>
 __device__ __forceinline__ int chain_count(int W, int H) { return DY == 0 ? H : (DX == 0 ? W : W + H - 1); }
@@ -258,6 +270,262 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     }
 }
 
+// ------------------------------------------------------------------ horizontal paths (rows)
+struct RowArgs {
+    const uint8_t* raw;     // raw volume (read; written when the cost is fused)
+    uint8_t* raw_out;       // fused census cost: where the raw bytes go
+    const uint64_t* cl;     // fused census cost: census codes of the left / right images
+    const uint64_t* cr;
+    uint16_t* agg;
+    int W, H, cs, cap, D;
+    uint32_t p1x2, p2;
+    uint16_t* cols;         // [2][H][cs] of this pass (slot 0: path 0)
+    uint16_t* colmins;      // [2][H]
+    uint16_t* disp;         // WTA outputs
+    float* sub;
+    uint16_t* dr;           // [H][W] right-image disparity
+    uint32_t* dr0;          // [H] dR(0) without cell (0, 0)
+    int write_agg;          // store the final aggregate too
+};
+
+// Census Hamming costs of the lane's planes of cell (x, y): popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255
+// above (the cost of a plane left of the image repeats the last in-image plane, DenseStereo.cs:273-274).
+template <int NP>
+__device__ __forceinline__ void census_cell(const uint64_t* cl, const uint64_t* crow, int x, int y, int W, int D, int lo,
+                                            bool active, Cell<NP>& raw, uint32_t (&bytes)[NP / 2 + (NP == 1)])
+{
+    const uint64_t a = cl[(size_t)y * W + x];
+    uint32_t v[2 * NP];
+#pragma unroll
+    for (int e = 0; e < 2 * NP; e++) {
+        const int d = lo + e;
+        const uint64_t b = active ? crow[max(x - d, 0)] : 0ull;
+        v[e] = d < D ? (uint32_t)__popcll(a ^ b) : 255u;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
+    if constexpr (NP == 1) {
+        bytes[0] = v[0] | (v[1] << 8);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) bytes[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+    }
+}
+
+// Pass +1, path 0 (->): one wavefront per row, x = 0 .. W-1. The first aggregate writer (writes lc instead
+// of adding; the border column W-1 gets 0 and is filled by the replay). COST: computes the census cost of
+// each cell on the way and writes the raw volume for the later sweeps. Saves the W-2 column's states for
+// the border replay.
+template <int NP, bool MASKED, bool COST>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
+{
+    constexpr int PF = 4;
+    const int lane = threadIdx.x & 63;
+    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (y >= a.H) return;
+    const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const bool store_lane = lo < cs;
+    Cell<NP> state;
+    set_init(state);
+    uint32_t minprev = 2000;
+    const size_t rowcell = (size_t)y * W;
+    const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
+    Cell<NP> rb[PF];
+    uint32_t bb[PF][NP / 2 + (NP == 1)];
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (j < W) {
+            if constexpr (COST) census_cell<NP>(a.cl, crow, j, y, W, a.D, lo, store_lane, rb[j], bb[j]);
+            else load_raw<NP>(a.raw + (rowcell + j) * cs + lo, active, rb[j]);
+        }
+    for (int x0 = 0; x0 < W; x0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int x = x0 + j;
+            if (x < W) {
+                const Cell<NP> raw = rb[j];
+                if constexpr (COST) {  // the raw bytes of this cell for the later sweeps
+                    uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
+                    if (store_lane) {
+                        if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bb[j][0];
+                        else {
+#pragma unroll
+                            for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bb[j][k];
+                        }
+                    }
+                }
+                if (x + PF < W) {
+                    if constexpr (COST) census_cell<NP>(a.cl, crow, x + PF, y, W, a.D, lo, store_lane, rb[j], bb[j]);
+                    else load_raw<NP>(a.raw + (rowcell + x + PF) * cs + lo, active, rb[j]);
+                }
+                Cell<NP> lc;
+                minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+                Cell<NP> out;
+#pragma unroll
+                for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
+                store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
+                if (x == W - 2) {
+                    store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+                    if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+                }
+                state = lc;
+            }
+        }
+    }
+}
+
+// Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
+// winner-take-all fused: one wavefront per row, x = W-1 .. 1.
+//  dL(x): first minimum over planes [0, D) of the final aggregate (DenseStereoSgm.cs:225-231), as the
+//         minimum of the keys agg << 16 | d.
+//  sub(x): parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly rounded division, no
+//         contraction), no offset at the range ends (CrossViewCost.cs:197-202).
+//  dR(xr) = first minimum over d of agg(xr+d, d): a shift register of keys along the planes. After cell x,
+//         register plane p holds min over j >= p of key(x-p+j, j), i.e. dR candidates of xr = x-p, so
+//         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
+//         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
+// Saves the column-1 states for the border replay.
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
+{
+    constexpr int PF = 4, E = 2 * NP;
+    const int lane = threadIdx.x & 63;
+    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (y >= a.H) return;
+    const int W = a.W, cs = a.cs, D = a.D, lo = lane * E;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    Cell<NP> state;
+    set_init(state);
+    uint32_t minprev = 2000;
+    uint32_t R[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
+    const size_t rowcell = (size_t)y * W;
+    Cell<NP> rb[PF], ab[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int x = W - 1 - j;
+        if (x >= 1) {
+            load_raw<NP>(a.raw + (rowcell + x) * cs + lo, active, rb[j]);
+            load_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, ab[j]);
+        }
+    }
+    uint32_t myd = 0, myr = 0;
+    float mys = 0.f;
+    for (int x0 = W - 1; x0 >= 1; x0 -= PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int x = x0 - j;
+            if (x >= 1) {
+                const Cell<NP> raw = rb[j], agg = ab[j];
+                if (x - PF >= 1) {
+                    load_raw<NP>(a.raw + (rowcell + x - PF) * cs + lo, active, rb[j]);
+                    load_u16<NP>(a.agg + (rowcell + x - PF) * cs + lo, active, ab[j]);
+                }
+                Cell<NP> lc, fin;
+                minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+#pragma unroll
+                for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
+                if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+                if (x == 1) {
+                    store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+                    if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+                }
+                state = lc;
+                // winner-take-all on the final aggregate of cell x
+                uint32_t key[E];
+                uint32_t best = 0xffffffffu;
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int d = lo + e;
+                    const uint32_t v = (e & 1) ? (fin.v[e >> 1] >> 16) : (fin.v[e >> 1] & 0xffffu);
+                    key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+                    best = key[e] < best ? key[e] : best;
+                }
+                const uint32_t carry = shl1(R[0], 0xffffffffu);
+#pragma unroll
+                for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
+                R[E - 1] = carry;
+#pragma unroll
+                for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+                const uint32_t rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
+                best = wave_min(best);
+                const int d = (int)(best & 0xffffu);
+                float v = (float)d;
+                if (d > 0 && d < D - 1) {
+                    const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
+                    const int den = l - 2 * cc + r;
+                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+                }
+                if (lane == (x & 63)) {
+                    myd = (uint32_t)d;
+                    mys = v;
+                    myr = rdone & 0xffffu;
+                }
+                if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
+                    const int xb = x & ~63, l0 = x & 63;
+                    const int l1 = min(63, W - 1 - xb);
+                    if (lane >= l0 && lane <= l1) {
+                        const size_t o = rowcell + xb + lane;
+                        a.disp[o] = (uint16_t)myd;
+                        if (a.sub) a.sub[o] = mys;
+                        a.dr[o] = (uint16_t)myr;
+                    }
+                }
+            }
+        }
+    }
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
+    if (lane == 0) a.dr0[y] = r1;
+}
+
+// Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
+// valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol (RealStereo.cs:418-421).
+template <int NP>
+__global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* valid)
+{
+    const int y = blockIdx.x, lane = threadIdx.x;
+    const int W = a.W, cs = a.cs, D = a.D, lo = lane * 2 * NP;
+    const bool active = lo < cs;
+    const size_t rowcell = (size_t)y * W;
+    Cell<NP> c;
+    load_u16<NP>(a.agg + rowcell * cs + lo, active, c);
+    uint32_t best = 0xffffffffu;
+#pragma unroll
+    for (int e = 0; e < 2 * NP; e++) {
+        const int d = lo + e;
+        const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
+        const uint32_t key = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+        best = key < best ? key : best;
+    }
+    best = wave_min(best);
+    const int d = (int)(best & 0xffffu);
+    float v = (float)d;
+    if (d > 0 && d < D - 1) {
+        const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
+        const int den = l - 2 * cc + r;
+        if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+    }
+    const uint32_t k00 = (plane_of<NP>(c, 0) << 16);
+    const uint32_t part = a.dr0[y];
+    if (lane == 0) {
+        a.disp[rowcell] = (uint16_t)d;
+        if (a.sub) a.sub[rowcell] = v;
+        a.dr[rowcell] = (uint16_t)((k00 < part ? k00 : part) & 0xffffu);
+    }
+    __syncthreads();
+    if (valid)
+        for (int x = lane; x < W; x += 64) {
+            const int dd = a.disp[rowcell + x], xr = x - dd;
+            valid[rowcell + x] = (uint8_t)(xr >= 0 && abs(dd - (int)a.dr[rowcell + xr]) <= tol);
+        }
+}
+
 struct ReplayArgs {
     const uint8_t* raw;
     uint16_t* agg;
@@ -352,10 +620,12 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
         m0 = a.seedmins[1 * a.W + a.col];
     }
     Cell<NP> rb[PF], pb[PF];
+    int cb[PF];  // half of each prefetched visit
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (j < a.nvis) {
             const int yj = a.visits[4 * j];
+            cb[j] = a.visits[4 * j + 1];
             load_raw<NP>(a.raw + ((size_t)yj * a.W + a.col) * cs + lo, active, rb[j]);
             load_u16<NP>(a.partial + (size_t)j * cs + lo, active, pb[j]);
         }
@@ -364,11 +634,12 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
         for (int j = 0; j < PF; j++) {
             const int i = i0 + j;
             if (i < a.nvis) {
-                const int cur = a.visits[4 * i + 1];
+                const int cur = cb[j];
                 const Cell<NP> raw = rb[j], part = pb[j];
                 const int ip = i + PF;
                 if (ip < a.nvis) {
                     const int yp = a.visits[4 * ip];
+                    cb[j] = a.visits[4 * ip + 1];
                     load_raw<NP>(a.raw + ((size_t)yp * a.W + a.col) * cs + lo, active, rb[j]);
                     load_u16<NP>(a.partial + (size_t)ip * cs + lo, active, pb[j]);
                 }
@@ -468,27 +739,8 @@ __global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* 
     out[(size_t)y * W + x] = code;
 }
 
-// Four planes per thread: consecutive threads write consecutive 4-byte groups of a cell; blockIdx.y = row.
-__global__ __launch_bounds__(kBlock) void cost_census_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int D,
-                                                              int cs, uint8_t* raw)
-{
-    const int groups = cs >> 2;
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= W * groups) return;
-    const int y = blockIdx.y;
-    const int x = t / groups, d0 = (t - x * groups) * 4;
-    const uint64_t* rrow = cr + (size_t)y * W;
-    const uint64_t a = cl[(size_t)y * W + x];
-    uint32_t w = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int d = d0 + j;
-        const uint32_t v = d < D ? (uint32_t)__popcll(a ^ rrow[max(x - d, 0)]) : 255u;
-        w |= v << (8 * j);
-    }
-    ((uint32_t*)(raw + (size_t)y * W * cs))[t] = w;
-}
-
+// Truncated 3x3 absolute-difference cost, four planes per thread (consecutive threads write consecutive
+// 4-byte groups of a cell); blockIdx.y = row.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride, int D, int cs,
                                                           int shift, int adcap, uint8_t* raw)
@@ -527,115 +779,6 @@ __global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R,
     }
 }
 
-// ------------------------------------------------------------------ WTA + subpixel + L/R
-// Plane q of a cell held in lanes (2*NP planes per lane), read wave-uniformly.
-template <int NP>
-__device__ __forceinline__ uint32_t plane_of(const Cell<NP>& c, int q)
-{
-    const int ln = q / (2 * NP), e = q - ln * 2 * NP;
-    uint32_t w = c.v[0];
-#pragma unroll
-    for (int k = 1; k < NP; k++) w = (e >> 1) == k ? c.v[k] : w;
-    w = (uint32_t)__builtin_amdgcn_readlane((int)w, ln);
-    return (e & 1) ? (w >> 16) : (w & 0xffffu);
-}
-
-// One wavefront per row, marching along x.
-//  dL(x): first minimum over planes [0, D) (DenseStereoSgm.cs:225-231) = the minimum of the keys
-//         agg << 16 | d over the cell.
-//  dR(xr) = first minimum over d of agg(xr+d, d): a shift register of keys along the planes. After cell x,
-//         register plane p holds min_{j<=p} key(x-p+j, j), so plane D-1 completes xr = x-D+1; each step
-//         shifts it by one plane (v_mov inside a lane, DPP wave_shr across lanes) and folds in the cell's keys.
-//  sub(x): parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly rounded, no contraction);
-//         no offset at the range ends.
-//  valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol.
-template <int NP>
-__global__ __launch_bounds__(64) void wta_rows(const uint16_t* agg, int W, int H, int cs, int D, int tol, uint16_t* disp,
-                                               float* sub, uint8_t* valid)
-{
-    constexpr int E = 2 * NP;
-    extern __shared__ uint16_t wlds[];
-    uint16_t* dl = wlds;
-    uint16_t* dr = wlds + W;
-    const int y = blockIdx.x, lane = threadIdx.x, lo = lane * E;
-    const bool active = lo < cs;
-    const int lastLane = (D - 1) / E, lastE = (D - 1) - lastLane * E;
-    uint32_t R[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
-    const uint16_t* row = agg + (size_t)y * W * cs;
-    constexpr int PF = 4;  // cells in flight ahead of the one being reduced
-    Cell<NP> buf[PF];
-#pragma unroll
-    for (int j = 0; j < PF; j++)
-        if (j < W) load_u16<NP>(row + (size_t)j * cs + lo, active, buf[j]);
-    uint32_t myd = 0;
-    float mys = 0.f;
-    const int steps = W + D - 1;
-    for (int x0 = 0; x0 < steps; x0 += PF) {
-#pragma unroll
-        for (int j = 0; j < PF; j++) {
-            const int x = x0 + j;
-            if (x >= steps) break;
-            const bool have = x < W;
-            const Cell<NP> c = buf[j];
-            if (x + PF < W) load_u16<NP>(row + (size_t)(x + PF) * cs + lo, active, buf[j]);
-            uint32_t key[E];
-            uint32_t best = 0xffffffffu;
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                const int d = lo + e;
-                const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
-                key[e] = (have && d < D) ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
-                best = key[e] < best ? key[e] : best;
-            }
-            // right-image disparity: shift by one plane, fold in this cell's keys, emit plane D-1
-            const uint32_t carry = shr1(R[E - 1], 0xffffffffu);
-#pragma unroll
-            for (int e = E - 1; e > 0; e--) R[e] = R[e - 1];
-            R[0] = carry;
-#pragma unroll
-            for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
-            const int xr = x - (D - 1);
-            if (xr >= 0) {
-                uint32_t w = R[0];
-#pragma unroll
-                for (int e = 1; e < E; e++) w = lastE == e ? R[e] : w;
-                w = (uint32_t)__builtin_amdgcn_readlane((int)w, lastLane);
-                if (lane == 0) dr[xr] = (uint16_t)(w & 0xffffu);
-            }
-            if (have) {
-                best = wave_min(best);
-                const int d = (int)(best & 0xffffu);
-                float v = (float)d;
-                if (d > 0 && d < D - 1) {
-                    const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
-                    const int den = l - 2 * cc + r;
-                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
-                }
-                if (lane == (x & 63)) {
-                    myd = (uint32_t)d;
-                    mys = v;
-                }
-                if ((x & 63) == 63 || x == W - 1) {  // flush 64 consecutive pixels, coalesced
-                    const int xb = x & ~63;
-                    if (lane <= (x & 63)) {
-                        dl[xb + lane] = (uint16_t)myd;
-                        if (disp) disp[(size_t)y * W + xb + lane] = (uint16_t)myd;
-                        if (sub) sub[(size_t)y * W + xb + lane] = mys;
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (valid)
-        for (int x = lane; x < W; x += 64) {
-            const int d = dl[x], xr = x - d;
-            valid[(size_t)y * W + x] = (uint8_t)(xr >= 0 && abs(d - (int)dr[xr]) <= tol);
-        }
-}
-
 template <typename T>
 __global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t cells, int n, T fill)
 {
@@ -651,8 +794,8 @@ inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((tot
 
 // ------------------------------------------------------------------ launchers
 template <int NP, bool MASKED>
-hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
-                   hipEvent_t* marks)
+hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                   hipStream_t s, hipEvent_t* marks)
 {
     int mk = 0;
     auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
@@ -665,6 +808,11 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
     a.seeds = b.seeds;
     a.seedmins = b.seedmins;
     a.seed_row = b.seed_row;
+    RowArgs ra;
+    ra.raw = b.raw; ra.raw_out = b.raw_out; ra.cl = b.cl; ra.cr = b.cr; ra.agg = b.agg;
+    ra.W = g.W; ra.H = g.H; ra.cs = g.cs; ra.cap = g.cap; ra.D = g.D;
+    ra.p1x2 = a.p1x2; ra.p2 = a.p2;
+    ra.disp = o.disp; ra.sub = o.sub; ra.dr = b.dr; ra.dr0 = b.dr0; ra.write_agg = o.write_agg;
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
     const int nh = g.H, nv = g.W, nd = g.W + g.H - 1;
@@ -672,9 +820,11 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
         fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
         fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedmins, (size_t)3 * g.W, 2000);
     }
-    // pass +1: ->, down-right, down, down-left
+    // pass +1: -> (with the census cost fused when codes are given), down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
-    sgm_sweep<NP, 1, 0, true, MASKED><<<blocks(nh), kBlock, 0, s>>>(a);
+    ra.cols = b.cols; ra.colmins = b.colmins;
+    if (b.cl) sgm_row_fwd<NP, MASKED, true><<<blocks(nh), kBlock, 0, s>>>(ra);
+    else sgm_row_fwd<NP, MASKED, false><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
     sgm_sweep<NP, 1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
     mark();
@@ -696,21 +846,24 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint1
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
     }
-    // pass -1: <-, up-left, up, up-right
+    // pass -1: up-left, up, up-right, then <- with the winner-take-all fused (the last contribution)
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
-    sgm_sweep<NP, -1, 0, false, MASKED><<<blocks(nh), kBlock, 0, s>>>(a);
-    mark();
+    ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
     sgm_sweep<NP, -1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
     mark();
     sgm_sweep<NP, -1, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
     mark();
     sgm_sweep<NP, -1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
     mark();
+    sgm_row_bwd_wta<NP, MASKED><<<blocks(nh), kBlock, 0, s>>>(ra);
+    mark();
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
     sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
     sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
+    mark();
+    wta_finish<NP><<<g.H, 64, 0, s>>>(ra, o.tol, o.valid);
     mark();
     return hipGetLastError();
 }
@@ -743,13 +896,6 @@ hipError_t launch_census(const void* img, bool img16, int W, int H, int stride, 
     return hipGetLastError();
 }
 
-hipError_t launch_cost_census(const uint64_t* cl, const uint64_t* cr, const Geometry& g, uint8_t* raw, hipStream_t s)
-{
-    dim3 grd((g.W * (g.cs / 4) + kBlock - 1) / kBlock, g.H);
-    cost_census_kernel<<<grd, kBlock, 0, s>>>(cl, cr, g.W, g.H, g.D, g.cs, raw);
-    return hipGetLastError();
-}
-
 hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, int stride, int adcap, const Geometry& g,
                           uint8_t* raw, hipStream_t s)
 {
@@ -764,31 +910,17 @@ hipError_t launch_cost_ad(const void* L, const void* R, bool img16, int bits, in
     return hipGetLastError();
 }
 
-hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, int p1, int p2, uint16_t* agg_pass1, hipStream_t s,
-                      hipEvent_t* m)
+hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                      hipStream_t s, hipEvent_t* m)
 {
     const bool full = g.cap == 64 * g.dpl;
     switch (g.dpl) {
-        case 2: return full ? run_sgm<1, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<1, true>(g, b, p1, p2, agg_pass1, s, m);
-        case 4: return full ? run_sgm<2, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<2, true>(g, b, p1, p2, agg_pass1, s, m);
-        case 8: return full ? run_sgm<4, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<4, true>(g, b, p1, p2, agg_pass1, s, m);
-        case 16: return full ? run_sgm<8, false>(g, b, p1, p2, agg_pass1, s, m) : run_sgm<8, true>(g, b, p1, p2, agg_pass1, s, m);
+        case 2: return full ? run_sgm<1, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<1, true>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 4: return full ? run_sgm<2, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<2, true>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 8: return full ? run_sgm<4, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<4, true>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 16: return full ? run_sgm<8, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<8, true>(g, b, o, p1, p2, agg_pass1, s, m);
         default: return hipErrorInvalidValue;
     }
-}
-
-hipError_t launch_wta(const Geometry& g, const uint16_t* agg, int tol, uint16_t* disp, float* sub, uint8_t* valid,
-                      hipStream_t s)
-{
-    const size_t lds = (size_t)g.W * 4;
-    switch (g.dpl) {
-        case 2: wta_rows<1><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 4: wta_rows<2><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 8: wta_rows<4><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        case 16: wta_rows<8><<<g.H, 64, lds, s>>>(agg, g.W, g.H, g.cs, g.D, tol, disp, sub, valid); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
 }
 
 hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s)
