@@ -75,6 +75,52 @@ __device__ __forceinline__ void load_raw(const uint8_t* p, bool active, Cell<NP>
         }
     }
 }
+// The same planes as packed bytes, loaded without widening: a prefetch that stays in flight until use.
+template <int NP>
+struct RawW {
+    uint32_t w[NP == 1 ? 1 : NP / 2];
+};
+template <int NP>
+__device__ __forceinline__ void load_raw_w(const uint8_t* p, bool active, RawW<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.w[0] = active ? (uint32_t)(*(const uint16_t*)p) : 0u;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) r.w[k] = active ? ((const uint32_t*)p)[k] : 0u;
+    }
+}
+// Unconditional forms for prefetch rings: the caller passes an address that is always in bounds (idle lanes
+// read lane 0's planes), so the load needs no select that would wait for it.
+template <int NP>
+__device__ __forceinline__ void ld_raw_w(const uint8_t* p, RawW<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.w[0] = (uint32_t)(*(const uint16_t*)p);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) r.w[k] = ((const uint32_t*)p)[k];
+    }
+}
+template <int NP>
+__device__ __forceinline__ void ld_u16(const uint16_t* p, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) r.v[k] = ((const uint32_t*)p)[k];
+}
+template <int NP>
+__device__ __forceinline__ void widen(const RawW<NP>& b, Cell<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.v[0] = __builtin_amdgcn_perm(0u, b.w[0], 0x0c010c00u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) {
+            r.v[2 * k] = __builtin_amdgcn_perm(0u, b.w[k], 0x0c010c00u);
+            r.v[2 * k + 1] = __builtin_amdgcn_perm(0u, b.w[k], 0x0c030c02u);
+        }
+    }
+}
 template <int NP>
 __device__ __forceinline__ void load_u16(const uint16_t* p, bool active, Cell<NP>& r)
 {
@@ -229,17 +275,28 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
             minprev = a.seedmins[(P - 1) * W + sc];
         }
     }
-    // software pipeline: the loads of step i+1 are in flight while step i computes
-    size_t cell = (size_t)y * W + x;
-    Cell<NP> raw, agg, nraw, nagg;
-    load_raw<NP>(a.raw + cell * cs + lo, active, raw);
-    if (!FIRST) load_u16<NP>(a.agg + cell * cs + lo, active, agg);
-    for (int i = 0; i < n; i++) {
-        const size_t ncell = cell + (ptrdiff_t)DY * W + DX;
-        if (i + 1 < n) {
-            load_raw<NP>(a.raw + ncell * cs + lo, active, nraw);
-            if (!FIRST) load_u16<NP>(a.agg + ncell * cs + lo, active, nagg);
-        }
+    // software pipeline: PF cells in flight; raw stays packed until its step. Prefetch addresses are clamped
+    // to the chain's last cell and idle lanes read lane 0's planes, so every load is unconditional.
+    constexpr int PF = 4;
+    const ptrdiff_t step = (ptrdiff_t)DY * W + DX;
+    const size_t cell0 = (size_t)y * W + x;
+    const int lol = active ? lo : 0;
+    size_t cell = cell0;
+    RawW<NP> rb[PF];
+    Cell<NP> ab[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
+        ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
+        if (!FIRST) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
+    }
+    auto body = [&](int i, RawW<NP>& rbj, Cell<NP>& abj) {
+        Cell<NP> raw;
+        widen<NP>(rbj, raw);
+        const Cell<NP> agg = abj;
+        const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
+        ld_raw_w<NP>(a.raw + cp * cs + lol, rbj);
+        if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, abj);
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
         if (x != bx) {
@@ -262,12 +319,18 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
             if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
         }
         state = lc;
-        raw = nraw;
-        agg = nagg;
-        cell = ncell;
+        cell += step;
         x += DX;
         y += DY;
+    };
+    int i0 = 0;
+    for (; i0 + PF <= n; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(i0 + j, rb[j], ab[j]);
     }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < n) body(i0 + j, rb[j], ab[j]);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
@@ -288,20 +351,30 @@ struct RowArgs {
     int write_agg;          // store the final aggregate too
 };
 
-// Census Hamming costs of the lane's planes of cell (x, y): popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255
-// above (the cost of a plane left of the image repeats the last in-image plane, DenseStereo.cs:273-274).
+// Census inputs of the lane's planes of cell (x, y), prefetched as loaded: the left code and the right codes
+// cr(max(x-d, 0)) of the lane's planes d.
 template <int NP>
-__device__ __forceinline__ void census_cell(const uint64_t* cl, const uint64_t* crow, int x, int y, int W, int D, int lo,
-                                            bool active, Cell<NP>& raw, uint32_t (&bytes)[NP / 2 + (NP == 1)])
+struct CensusIn {
+    uint64_t a;
+    uint64_t b[2 * NP];
+};
+template <int NP>
+__device__ __forceinline__ void load_census(const uint64_t* cl, const uint64_t* crow, int x, int y, int W, int lo, bool active,
+                                            CensusIn<NP>& c)
 {
-    const uint64_t a = cl[(size_t)y * W + x];
+    c.a = active ? cl[(size_t)y * W + x] : 0ull;
+#pragma unroll
+    for (int e = 0; e < 2 * NP; e++) c.b[e] = active ? crow[max(x - lo - e, 0)] : 0ull;
+}
+// Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
+// repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
+template <int NP>
+__device__ __forceinline__ void census_cost(const CensusIn<NP>& c, int D, int lo, Cell<NP>& raw,
+                                            uint32_t (&bytes)[NP / 2 + (NP == 1)])
+{
     uint32_t v[2 * NP];
 #pragma unroll
-    for (int e = 0; e < 2 * NP; e++) {
-        const int d = lo + e;
-        const uint64_t b = active ? crow[max(x - d, 0)] : 0ull;
-        v[e] = d < D ? (uint32_t)__popcll(a ^ b) : 255u;
-    }
+    for (int e = 0; e < 2 * NP; e++) v[e] = lo + e < D ? (uint32_t)__popcll(c.a ^ c.b[e]) : 255u;
 #pragma unroll
     for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
     if constexpr (NP == 1) {
@@ -333,48 +406,54 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     uint32_t minprev = 2000;
     const size_t rowcell = (size_t)y * W;
     const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
-    Cell<NP> rb[PF];
-    uint32_t bb[PF][NP / 2 + (NP == 1)];
+    const int lol = store_lane ? lo : 0;
+    CensusIn<NP> cb[COST ? PF : 1];
+    RawW<NP> rb[COST ? 1 : PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int xj = min(j, W - 1);
+        if constexpr (COST) load_census<NP>(a.cl, crow, xj, y, W, lol, true, cb[j]);
+        else ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+    }
+    auto body = [&](int x, int j) {
+        Cell<NP> raw;
+        const int xp = min(x + PF, W - 1);
+        if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
+            uint32_t bytes[NP / 2 + (NP == 1)];
+            census_cost<NP>(cb[j], a.D, lo, raw, bytes);
+            uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
+            if (store_lane) {
+                if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
+                else {
+#pragma unroll
+                    for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bytes[k];
+                }
+            }
+            load_census<NP>(a.cl, crow, xp, y, W, lol, true, cb[j]);
+        } else {
+            widen<NP>(rb[j], raw);
+            ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
+        }
+        Cell<NP> lc;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        Cell<NP> out;
+#pragma unroll
+        for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
+        store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
+        if (x == W - 2) {
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+        }
+        state = lc;
+    };
+    int x0 = 0;
+    for (; x0 + PF <= W; x0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(x0 + j, j);
+    }
 #pragma unroll
     for (int j = 0; j < PF; j++)
-        if (j < W) {
-            if constexpr (COST) census_cell<NP>(a.cl, crow, j, y, W, a.D, lo, store_lane, rb[j], bb[j]);
-            else load_raw<NP>(a.raw + (rowcell + j) * cs + lo, active, rb[j]);
-        }
-    for (int x0 = 0; x0 < W; x0 += PF) {
-#pragma unroll
-        for (int j = 0; j < PF; j++) {
-            const int x = x0 + j;
-            if (x < W) {
-                const Cell<NP> raw = rb[j];
-                if constexpr (COST) {  // the raw bytes of this cell for the later sweeps
-                    uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
-                    if (store_lane) {
-                        if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bb[j][0];
-                        else {
-#pragma unroll
-                            for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bb[j][k];
-                        }
-                    }
-                }
-                if (x + PF < W) {
-                    if constexpr (COST) census_cell<NP>(a.cl, crow, x + PF, y, W, a.D, lo, store_lane, rb[j], bb[j]);
-                    else load_raw<NP>(a.raw + (rowcell + x + PF) * cs + lo, active, rb[j]);
-                }
-                Cell<NP> lc;
-                minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
-                Cell<NP> out;
-#pragma unroll
-                for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
-                store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
-                if (x == W - 2) {
-                    store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
-                    if (lane == 0) a.colmins[y] = (uint16_t)minprev;
-                }
-                state = lc;
-            }
-        }
-    }
+        if (x0 + j < W) body(x0 + j, j);
 }
 
 // Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
@@ -406,80 +485,85 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 #pragma unroll
     for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
     const size_t rowcell = (size_t)y * W;
-    Cell<NP> rb[PF], ab[PF];
+    const int lol = active ? lo : 0;
+    RawW<NP> rb[PF];
+    Cell<NP> ab[PF];
 #pragma unroll
     for (int j = 0; j < PF; j++) {
-        const int x = W - 1 - j;
-        if (x >= 1) {
-            load_raw<NP>(a.raw + (rowcell + x) * cs + lo, active, rb[j]);
-            load_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, ab[j]);
-        }
+        const int xj = max(W - 1 - j, 0);
+        ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
     }
     uint32_t myd = 0, myr = 0;
     float mys = 0.f;
-    for (int x0 = W - 1; x0 >= 1; x0 -= PF) {
+    auto body = [&](int x, int j) {
+        Cell<NP> raw;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> agg = ab[j];
+        const int xp = max(x - PF, 0);
+        ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
+        ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        Cell<NP> lc, fin;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
 #pragma unroll
-        for (int j = 0; j < PF; j++) {
-            const int x = x0 - j;
-            if (x >= 1) {
-                const Cell<NP> raw = rb[j], agg = ab[j];
-                if (x - PF >= 1) {
-                    load_raw<NP>(a.raw + (rowcell + x - PF) * cs + lo, active, rb[j]);
-                    load_u16<NP>(a.agg + (rowcell + x - PF) * cs + lo, active, ab[j]);
-                }
-                Cell<NP> lc, fin;
-                minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
+        if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+        if (x == 1) {
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+        }
+        state = lc;
+        // winner-take-all on the final aggregate of cell x
+        uint32_t key[E];
+        uint32_t best = 0xffffffffu;
 #pragma unroll
-                for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
-                if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
-                if (x == 1) {
-                    store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
-                    if (lane == 0) a.colmins[y] = (uint16_t)minprev;
-                }
-                state = lc;
-                // winner-take-all on the final aggregate of cell x
-                uint32_t key[E];
-                uint32_t best = 0xffffffffu;
+        for (int e = 0; e < E; e++) {
+            const int d = lo + e;
+            const uint32_t v = (e & 1) ? (fin.v[e >> 1] >> 16) : (fin.v[e >> 1] & 0xffffu);
+            key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+            best = key[e] < best ? key[e] : best;
+        }
+        const uint32_t carry = shl1(R[0], 0xffffffffu);
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    const int d = lo + e;
-                    const uint32_t v = (e & 1) ? (fin.v[e >> 1] >> 16) : (fin.v[e >> 1] & 0xffffu);
-                    key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
-                    best = key[e] < best ? key[e] : best;
-                }
-                const uint32_t carry = shl1(R[0], 0xffffffffu);
+        for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
+        R[E - 1] = carry;
 #pragma unroll
-                for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
-                R[E - 1] = carry;
-#pragma unroll
-                for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
-                const uint32_t rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
-                best = wave_min(best);
-                const int d = (int)(best & 0xffffu);
-                float v = (float)d;
-                if (d > 0 && d < D - 1) {
-                    const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
-                    const int den = l - 2 * cc + r;
-                    if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
-                }
-                if (lane == (x & 63)) {
-                    myd = (uint32_t)d;
-                    mys = v;
-                    myr = rdone & 0xffffu;
-                }
-                if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
-                    const int xb = x & ~63, l0 = x & 63;
-                    const int l1 = min(63, W - 1 - xb);
-                    if (lane >= l0 && lane <= l1) {
-                        const size_t o = rowcell + xb + lane;
-                        a.disp[o] = (uint16_t)myd;
-                        if (a.sub) a.sub[o] = mys;
-                        a.dr[o] = (uint16_t)myr;
-                    }
-                }
+        for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+        const uint32_t rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
+        best = wave_min(best);
+        const int d = (int)(best & 0xffffu);
+        float v = (float)d;
+        if (d > 0 && d < D - 1) {
+            const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
+            const int den = l - 2 * cc + r;
+            if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+        }
+        if (lane == (x & 63)) {
+            myd = (uint32_t)d;
+            mys = v;
+            myr = rdone & 0xffffu;
+        }
+        if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
+            const int xb = x & ~63, l0 = x & 63;
+            const int l1 = min(63, W - 1 - xb);
+            if (lane >= l0 && lane <= l1) {
+                const size_t o = rowcell + xb + lane;
+                a.disp[o] = (uint16_t)myd;
+                if (a.sub) a.sub[o] = mys;
+                a.dr[o] = (uint16_t)myr;
             }
         }
+    };
+    // x = W-1 .. 1: (W-1) steps
+    const int n = W - 1;
+    int i0 = 0;
+    for (; i0 + PF <= n; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j);
     }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < n) body(W - 1 - (i0 + j), j);
     const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
     if (lane == 0) a.dr0[y] = r1;
 }
@@ -599,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 // Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's seed-table
 // entry (half 0 as the pass leaves it).
 template <int NP, bool MASKED>
-__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
+__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
 {
     constexpr int PF = 8;
     const int lane = threadIdx.x;
@@ -619,58 +703,63 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a)
         }
         m0 = a.seedmins[1 * a.W + a.col];
     }
-    Cell<NP> rb[PF], pb[PF];
+    const int lol = active ? lo : 0;
+    const int nv = a.nvis;
+    RawW<NP> rb[PF];
+    Cell<NP> pb[PF];
     int cb[PF];  // half of each prefetched visit
-#pragma unroll
-    for (int j = 0; j < PF; j++)
-        if (j < a.nvis) {
-            const int yj = a.visits[4 * j];
-            cb[j] = a.visits[4 * j + 1];
-            load_raw<NP>(a.raw + ((size_t)yj * a.W + a.col) * cs + lo, active, rb[j]);
-            load_u16<NP>(a.partial + (size_t)j * cs + lo, active, pb[j]);
-        }
-    for (int i0 = 0; i0 < a.nvis; i0 += PF) {
+    if (nv > 0) {
 #pragma unroll
         for (int j = 0; j < PF; j++) {
-            const int i = i0 + j;
-            if (i < a.nvis) {
-                const int cur = cb[j];
-                const Cell<NP> raw = rb[j], part = pb[j];
-                const int ip = i + PF;
-                if (ip < a.nvis) {
-                    const int yp = a.visits[4 * ip];
-                    cb[j] = a.visits[4 * ip + 1];
-                    load_raw<NP>(a.raw + ((size_t)yp * a.W + a.col) * cs + lo, active, rb[j]);
-                    load_u16<NP>(a.partial + (size_t)ip * cs + lo, active, pb[j]);
-                }
-                Cell<NP> lc, sum;
-                uint32_t mn;
-                if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
-                else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
-#pragma unroll
-                for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
-                store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
-                if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
-            }
+            const int ij = min(j, nv - 1);
+            cb[j] = vis[4 * ij + 1];
+            ld_raw_w<NP>(a.raw + ((size_t)vis[4 * ij] * a.W + a.col) * cs + lol, rb[j]);
+            ld_u16<NP>(a.partial + (size_t)ij * cs + lol, pb[j]);
         }
     }
+    auto body = [&](int i, int j) {
+        const int cur = cb[j];
+        Cell<NP> raw;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> part = pb[j];
+        const int ip = min(i + PF, nv - 1);
+        cb[j] = vis[4 * ip + 1];
+        ld_raw_w<NP>(a.raw + ((size_t)vis[4 * ip] * a.W + a.col) * cs + lol, rb[j]);
+        ld_u16<NP>(a.partial + (size_t)ip * cs + lol, pb[j]);
+        Cell<NP> lc, sum;
+        uint32_t mn;
+        if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
+        else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
+#pragma unroll
+        for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
+        store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
+        if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
+    };
+    int i0 = 0;
+    for (; i0 + PF <= nv; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(i0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < nv) body(i0 + j, j);
     if (a.dir > 0) {
         // path 2: the chain's half-0 state; paths 1 and 3: recomputed for the last visit that wrote half 0
         store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, active, h0);
         if (lane == 0) a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
         int last = -1;
         for (int i = a.nvis - 1; i >= 0; i--)
-            if (a.visits[4 * i + 1] == 0) { last = i; break; }
+            if (vis[4 * i + 1] == 0) { last = i; break; }
         Cell<NP> s1, s3;
         uint32_t sm1 = 2000, sm3 = 2000;
         set_init(s1);
         set_init(s3);
         if (last >= 0) {
-            const int yl = a.visits[4 * last];
+            const int yl = vis[4 * last];
             Cell<NP> r, prev;
             uint32_t pm;
             load_raw<NP>(a.raw + ((size_t)yl * a.W + a.col) * cs + lo, active, r);
-            label_state<NP, MASKED>(a, 1, a.visits[4 * last + 3], lo, active, mask, prev, pm);
+            label_state<NP, MASKED>(a, 1, vis[4 * last + 3], lo, active, mask, prev, pm);
             sm1 = path_step<NP, MASKED>(prev, pm, r, a.p1x2, a.p2, mask, s1);
             set_init(prev);
             sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, a.p2, mask, s3);
@@ -839,7 +928,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
     r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r, r.visits);
     sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
     mark();
     if (agg_pass1) {
@@ -860,7 +949,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r, r.visits);
     sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
     mark();
     wta_finish<NP><<<g.H, 64, 0, s>>>(ra, o.tol, o.valid);
