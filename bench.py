@@ -23,16 +23,16 @@ P1, P2, TILE, RADIUS, BITS = 8, 96, 64, 3, 10
 HBM_PEAK_GBS = 8000.0
 PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1, agg 2), WTA fused
 # algorithmic HBM bytes per cell of each stage (what the stage must move at minimum)
-# sweep+0 computes the census cost and writes raw (1) + agg (2); sweep-0 reads raw (1) + agg (2) and writes
-# only the per-pixel WTA outputs; the others read raw (1) and read-modify-write agg (2 + 2)
-STAGE_BYTES = {"sweep+0": 3, "sweep-0": 3}
-for s in ("sweep+1", "sweep+2", "sweep+3", "sweep-1", "sweep-2", "sweep-3"):
-    STAGE_BYTES[s] = 5
-KERNEL_NAMES = {"sweep+0": "sgm_row_fwd (pass +1 path 0, census cost fused)",
-                "sweep+1": "sgm_sweep<pass +1, path 1 (down-right)>", "sweep+2": "sgm_sweep<pass +1, path 2 (down)>",
-                "sweep+3": "sgm_sweep<pass +1, path 3 (down-left)>", "sweep-1": "sgm_sweep<pass -1, path 1 (up-left)>",
-                "sweep-2": "sgm_sweep<pass -1, path 2 (up)>", "sweep-3": "sgm_sweep<pass -1, path 3 (up-right)>",
-                "sweep-0": "sgm_row_bwd_wta (pass -1 path 0, WTA fused)"}
+# algorithmic HBM bytes per cell of each timed stage:
+#   rows+  (pass +1 path ->, census cost fused): writes raw (1) and agg (2)
+#   bands+ (pass +1 paths 1-3 fused): reads raw (1), read-modify-writes agg (2 + 2)
+#   bands- (pass -1 paths 1-3 fused): the same
+#   rows-  (pass -1 path <-, winner-take-all fused): reads raw (1) and agg (2)
+STAGE_BYTES = {"rows+": 3, "bands+": 5, "bands-": 5, "rows-": 3}
+KERNEL_NAMES = {"rows+": "sgm_row_fwd (pass +1 path ->, census cost fused)",
+                "bands+": "sgm_band<pass +1> (paths down-right, down, down-left fused; one launch per 16-row band)",
+                "bands-": "sgm_band<pass -1> (paths up-left, up, up-right fused; one launch per 16-row band)",
+                "rows-": "sgm_row_bwd_wta (pass -1 path <-, winner-take-all fused)"}
 
 
 def host_info():
@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for counter runs: a few requests, then a 1 GiB device copy as the byte-count calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is a design note.",
@@ -102,6 +104,18 @@ def main():
     for _ in range(args.warmup):
         request()
     torch.cuda.synchronize()
+    if args.profile_run:
+        for _ in range(args.steps):
+            request()
+        torch.cuda.synchronize()
+        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)  # calibration: reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_run": True, "requests": args.steps, "copy_bytes": 1 << 30}))
+        ctx.close()
+        return
     # timed region: K requests back to back, HIP events on the stream the kernels run on
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()

@@ -111,10 +111,10 @@ int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const
 /* Stage timing, the per-layer timing log of the reference's pyramid driver (DenseStereoSgm.cs:347-349)
  * at kernel granularity. With profiling on, each rectified request records HIP events on its stream
  * between launches; lux_sgm_stage_times waits for the last profiled request and writes the durations in
- * ms of its stages, in order: census codes (or the AD cost volume), the four pass +1 sweeps (-> with the
- * census cost fused, down-right, down, down-left), the pass +1 border replay, the pass -1 sweeps up-left,
- * up, up-right and <- with the winner-take-all fused, the pass -1 border replay, and the column-0 /
- * validity finish. Returns the number of stages written (12), or < 0. */
+ * ms of its stages, in order: census codes (or the AD cost volume), pass +1 path -> (census cost fused),
+ * pass +1 paths down-right/down/down-left, the pass +1 border replay, pass -1 paths up-left/up/up-right,
+ * pass -1 path <- (winner-take-all fused), the pass -1 border replay, and the column-0 / validity finish.
+ * Returns the number of stages written (8), or < 0. */
 int lux_sgm_set_profiling(lux_sgm_ctx* ctx, int on);
 int lux_sgm_stage_times(lux_sgm_ctx* ctx, float* ms, int32_t cap);
 

@@ -110,7 +110,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     int rc = get_plan(c, g.W, g.H, T, xsw, &plan);
     if (rc) return rc;
     const size_t cells = (size_t)g.W * g.H;
-    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial, *drb, *dr0;
+    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial, *drb, *dr0, *bnd, *bndmin;
     if ((rc = get_buf(c, "raw", cells * g.cs, &raw))) return rc;
     if ((rc = get_buf(c, "agg", cells * g.cs * 2, &agg))) return rc;
     if ((rc = get_buf(c, "seeds", (size_t)3 * g.W * g.cs * 2, &seeds))) return rc;
@@ -121,6 +121,8 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     const size_t maxvis = std::max(plan->plus.size(), plan->minus.size()) / 4 + 1;
     if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
     if ((rc = get_buf(c, "wta_dr", cells * 2, &drb))) return rc;
+    if ((rc = get_buf(c, "band_boundary", (size_t)6 * g.W * g.cs * 2, &bnd))) return rc;
+    if ((rc = get_buf(c, "band_boundary_min", (size_t)6 * g.W * 2, &bndmin))) return rc;
     if ((rc = get_buf(c, "wta_dr0", (size_t)g.H * 4, &dr0))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
     hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), plan->plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
@@ -142,6 +144,8 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.agg = w->agg;
     w->b.dr = (uint16_t*)drb;
     w->b.dr0 = (uint32_t*)dr0;
+    w->b.bnd = (uint16_t*)bnd;
+    w->b.bndmin = (uint16_t*)bndmin;
     w->b.seeds = (uint16_t*)seeds;
     w->b.seedmins = (uint16_t*)seedmins;
     w->b.cols = (uint16_t*)cols;

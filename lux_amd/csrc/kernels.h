@@ -26,6 +26,8 @@ struct SgmBuffers {
     uint16_t* cols = nullptr;       // [2 passes][2 paths][H][cs]  neighbour-column states for the replays
     uint16_t* colmins = nullptr;    // [2][2][H]
     uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch
+    uint16_t* bnd = nullptr;        // [2][3][W][cs]  band boundary states (ping-pong)
+    uint16_t* bndmin = nullptr;     // [2][3][W]
     const int* visits_plus = nullptr;
     int n_plus = 0;
     const int* visits_minus = nullptr;
@@ -51,7 +53,7 @@ struct SgmOutputs {
 // The eight path sweeps (the census cost fused into the first when b.cl is set, the winner-take-all into
 // the last), the two border replays and the column-0 / validity finish. agg_pass1 (if not null) receives
 // agg after pass +1. marks (if not null, kSgmMarks events) are recorded after each stage, in launch order.
-constexpr int kSgmMarks = 11;
+constexpr int kSgmMarks = 7;
 hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                       hipStream_t s, hipEvent_t* marks = nullptr);
 // Repack [H][W][cs] <-> [H][W][cap] on the device.

@@ -9,6 +9,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace lux {
 
@@ -610,6 +611,203 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
         }
 }
 
+// ------------------------------------------------------------------ fused vertical + diagonal paths (bands)
+// Paths 1, 2, 3 of one pass in one kernel, so each cell's raw cost is read once and its aggregate
+// read-modify-written once for the three. Rows advance in bands of kBandRows; a band is one launch, with one
+// workgroup per kStrip-column strip. A strip computes its three paths row by row from the states the previous
+// band left at its last row (a ping-pong boundary buffer). The diagonal paths also run over halo columns,
+// one fewer per row (kBandRows-1 at the band's first row), so a strip needs no other strip of its band.
+// Waves own kBandG consecutive columns each and swap their edge columns' states through LDS once per row.
+constexpr int kBandRows = 16, kStrip = 32, kBandWaves = 16, kBandG = 4;
+static_assert(kStrip + 2 * kBandRows <= kBandWaves * kBandG, "band halo does not fit the waves");
+
+struct BandArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    int y0, rows;                 // band rows y0, y0+DIR, ..., rows of them
+    const uint16_t* bin;          // [3][W][cs] states of the row before the band (null: the initial fill)
+    const uint16_t* binmin;       // [3][W]
+    uint16_t* bout;               // the band's last row, for the next band
+    uint16_t* boutmin;
+    uint16_t* cols;               // [2][H][cs] of this pass: slot 1 takes path 1 of the column next to the border
+    uint16_t* colmins;
+};
+
+template <int NP, int DIR, bool MASKED>
+__global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
+{
+    constexpr int G = kBandG, NW = kBandWaves;
+    // path j (0, 1, 2) is the reference's path j+1 of pass DIR; its predecessor column is x - DIR, x, x + DIR
+    constexpr int JL = DIR > 0 ? 0 : 2;  // the path reading column x-1
+    constexpr int JR = DIR > 0 ? 2 : 0;  // the path reading column x+1
+    __shared__ uint32_t edge[2][NW][2][64 * NP];
+    __shared__ uint32_t edgemin[2][NW][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int W = a.W, cs = a.cs, rows = a.rows, H1 = rows - 1;
+    const int xa = blockIdx.x * kStrip, xb = min(W, xa + kStrip);
+    // columns [xa - kBandRows, ...): the strip, the halo, and one guard column each side that only ever holds
+    // the boundary state its neighbour reads in the band's first row
+    const int cbase = xa - kBandRows + wv * G;
+    const int bx = DIR > 0 ? W - 1 : 0;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const int lo = lane * 2 * NP, lol = active ? lo : 0;
+
+    // the states of the row before the band ("row -1"): the boundary buffer, or the initial fill
+    Cell<NP> st[G][3];
+    uint32_t sm[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int x = cbase + g;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            set_init(st[g][j]);
+            sm[g][j] = 2000;
+            if (a.bin && x >= 0 && x < W) {
+                load_u16<NP>(a.bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (MASKED) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) st[g][j].v[q] = (st[g][j].v[q] & mask.v[q]) | (kInit2 & ~mask.v[q]);
+                }
+                sm[g][j] = a.binmin[j * W + x];
+            }
+        }
+    }
+    // the region of path j at band row k (L = halo width left at that row)
+    auto need = [&](int x, int j, int k) -> bool {
+        if (x < 0 || x >= W) return false;
+        const int L = H1 - k;
+        if (j == JL) return x >= xa - L && x < xb;
+        if (j == JR) return x >= xa && x < xb + L;
+        return x >= xa && x < xb && x != bx;
+    };
+    auto need_raw = [&](int x, int k) -> bool {
+        const int L = H1 - k;
+        return x >= 0 && x < W && x >= xa - L && x < xb + L;
+    };
+    auto load_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G]) {
+        const int y = a.y0 + DIR * k;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need_raw(x, k)) ld_raw_w<NP>(a.raw + ((size_t)y * W + x) * cs + lol, rb[g]);
+            if (x >= xa && x < xb) ld_u16<NP>(a.agg + ((size_t)y * W + x) * cs + lol, ab[g]);
+        }
+    };
+    auto do_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G], RawW<NP> (&nrb)[G], Cell<NP> (&nab)[G]) {
+        const int y = a.y0 + DIR * k;
+        const int par = k & 1;
+        // edge exchange: the previous row's states of this wave's boundary columns
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            edge[par][wv][0][lane * NP + q] = st[G - 1][JL].v[q];
+            edge[par][wv][1][lane * NP + q] = st[0][JR].v[q];
+        }
+        if (lane == 0) {
+            edgemin[par][wv][0] = sm[G - 1][JL];
+            edgemin[par][wv][1] = sm[0][JR];
+        }
+        __syncthreads();
+        if (k + 1 < rows) load_row(k + 1, nrb, nab);
+        // path costs accumulate straight into the strip cells' prefetched aggregates ab[g]
+        // the path reading x-1: columns right to left, so x-1 still holds the previous row
+#pragma unroll
+        for (int g = G - 1; g >= 0; g--) {
+            const int x = cbase + g;
+            if (need(x, JL, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                if (x - 1 < 0) { set_init(prev); pm = 2000; }
+                else if (g > 0) { prev = st[g - 1][JL]; pm = sm[g - 1][JL]; }
+                else {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) prev.v[q] = edge[par][wv - 1][0][lane * NP + q];
+                    pm = edgemin[par][wv - 1][0];
+                }
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][JL] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][JL] = lc;
+                if (x >= xa && x < xb) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+                }
+            }
+        }
+        // the vertical path
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need(x, 1, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                prev = st[g][1];
+                pm = sm[g][1];
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][1] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][1] = lc;
+#pragma unroll
+                for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+            }
+        }
+        // the path reading x+1: columns left to right
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need(x, JR, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                if (x + 1 >= W) { set_init(prev); pm = 2000; }
+                else if (g < G - 1) { prev = st[g + 1][JR]; pm = sm[g + 1][JR]; }
+                else {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) prev.v[q] = edge[par][wv + 1][1][lane * NP + q];
+                    pm = edgemin[par][wv + 1][1];
+                }
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][JR] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][JR] = lc;
+                if (x >= xa && x < xb) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+                }
+            }
+        }
+        // the strip's own cells: aggregate, seeds, the replay's column, the boundary for the next band
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (x >= xa && x < xb) {
+                const size_t cell = (size_t)y * W + x;
+                if (x != bx) store_u16<NP>(a.agg + cell * cs + lo, active, ab[g]);
+                if (x == bx - DIR) {
+                    store_u16<NP>(a.cols + ((size_t)a.H + y) * cs + lo, active, st[g][0]);
+                    if (lane == 0) a.colmins[a.H + y] = (uint16_t)sm[g][0];
+                }
+                if (k == rows - 1) {
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        store_u16<NP>(a.bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                        if (lane == 0) a.boutmin[j * W + x] = (uint16_t)sm[g][j];
+                    }
+                }
+            }
+        }
+    };
+    RawW<NP> rA[G], rB[G];
+    Cell<NP> aA[G], aB[G];
+    load_row(0, rA, aA);
+    for (int k = 0; k < rows; k += 2) {
+        do_row(k, rA, aA, rB, aB);
+        if (k + 1 < rows) do_row(k + 1, rB, aB, rA, aA);
+    }
+}
+
 struct ReplayArgs {
     const uint8_t* raw;
     uint16_t* agg;
@@ -909,17 +1107,55 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
         fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedmins, (size_t)3 * g.W, 2000);
     }
-    // pass +1: -> (with the census cost fused when codes are given), down-right, down, down-left
+    // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
+    auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
+        constexpr int DIR = decltype(dirc)::value;
+        if constexpr (NP <= 2) {
+            BandArgs ba;
+            ba.raw = b.raw; ba.agg = b.agg; ba.W = g.W; ba.H = g.H; ba.cs = g.cs; ba.cap = g.cap;
+            ba.p1x2 = a.p1x2; ba.p2 = a.p2;
+            ba.cols = cols; ba.colmins = colmins;
+            const int strips = (g.W + kStrip - 1) / kStrip;
+            const size_t bsz = (size_t)3 * g.W * g.cs, bmsz = (size_t)3 * g.W;
+            // pass +1 bands end at the seed row, whose states become the seed table (half 0 as pass +1
+            // leaves it for the interior cells): rows [0, seed_row] in bands of kBandRows, then the rest
+            const int split = DIR > 0 && b.seed_row >= 0 ? b.seed_row + 1 : g.H;
+            int done = 0, bi = 0;
+            while (done < g.H) {
+                const int lim = done < split ? split : g.H;
+                const int rows = min(kBandRows, lim - done);
+                ba.y0 = DIR > 0 ? done : g.H - 1 - done;
+                ba.rows = rows;
+                if (bi == 0) {
+                    ba.bin = DIR > 0 ? nullptr : b.seeds;
+                    ba.binmin = DIR > 0 ? nullptr : b.seedmins;
+                } else {
+                    ba.bin = b.bnd + ((bi - 1) & 1) * bsz;
+                    ba.binmin = b.bndmin + ((bi - 1) & 1) * bmsz;
+                }
+                ba.bout = b.bnd + (bi & 1) * bsz;
+                ba.boutmin = b.bndmin + (bi & 1) * bmsz;
+                sgm_band<NP, DIR, MASKED><<<strips, kBandWaves * 64, 0, s>>>(ba);
+                done += rows;
+                if (DIR > 0 && done == split && split < g.H + 1 && b.seed_row >= 0) {
+                    (void)hipMemcpyAsync(b.seeds, ba.bout, bsz * 2, hipMemcpyDeviceToDevice, s);
+                    (void)hipMemcpyAsync(b.seedmins, ba.boutmin, bmsz * 2, hipMemcpyDeviceToDevice, s);
+                }
+                bi++;
+            }
+        } else {
+            sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+        }
+    };
+    // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
     if (b.cl) sgm_row_fwd<NP, MASKED, true><<<blocks(nh), kBlock, 0, s>>>(ra);
     else sgm_row_fwd<NP, MASKED, false><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
-    sgm_sweep<NP, 1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
-    mark();
-    sgm_sweep<NP, 1, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
-    mark();
-    sgm_sweep<NP, 1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
     mark();
     ReplayArgs r;
     r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
@@ -935,14 +1171,11 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
     }
-    // pass -1: up-left, up, up-right, then <- with the winner-take-all fused (the last contribution)
+    // pass -1: up-left, up, up-right (starting from pass +1's half-0 leftovers), then <- with the
+    // winner-take-all fused (the last contribution)
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
-    sgm_sweep<NP, -1, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
-    mark();
-    sgm_sweep<NP, -1, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
-    mark();
-    sgm_sweep<NP, -1, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+    vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
     mark();
     sgm_row_bwd_wta<NP, MASKED><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();

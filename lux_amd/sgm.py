@@ -20,8 +20,7 @@ COST_CENSUS, COST_AD = 0, 1
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times")
-STAGES = ("census", "sweep+0", "sweep+1", "sweep+2", "sweep+3", "replay+", "sweep-1", "sweep-2", "sweep-3",
-          "sweep-0", "replay-", "finish")
+STAGES = ("census", "rows+", "bands+", "replay+", "bands-", "rows-", "replay-", "finish")
 
 
 class SgmError(RuntimeError):
