@@ -789,22 +789,33 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
                     store_u16<NP>(a.cols + ((size_t)a.H + y) * cs + lo, active, st[g][0]);
                     if (lane == 0) a.colmins[a.H + y] = (uint16_t)sm[g][0];
                 }
-                if (k == rows - 1) {
-#pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        store_u16<NP>(a.bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
-                        if (lane == 0) a.boutmin[j * W + x] = (uint16_t)sm[g][j];
-                    }
-                }
             }
         }
     };
-    RawW<NP> rA[G], rB[G];
-    Cell<NP> aA[G], aB[G];
-    load_row(0, rA, aA);
-    for (int k = 0; k < rows; k += 2) {
-        do_row(k, rA, aA, rB, aB);
-        if (k + 1 < rows) do_row(k + 1, rB, aB, rA, aA);
+    // one row body: the next row's loads go out at the start of a row and are taken over at its end, a row
+    // of work later
+    RawW<NP> rb[G], nrb[G];
+    Cell<NP> ab[G], nab[G];
+    load_row(0, rb, ab);
+    for (int k = 0; k < rows; k++) {
+        do_row(k, rb, ab, nrb, nab);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            rb[g] = nrb[g];
+            ab[g] = nab[g];
+        }
+    }
+    // the band's last row: the boundary the next band starts from
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int x = cbase + g;
+        if (x >= xa && x < xb) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                store_u16<NP>(a.bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (lane == 0) a.boutmin[j * W + x] = (uint16_t)sm[g][j];
+            }
+        }
     }
 }
 
