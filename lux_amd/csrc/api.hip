@@ -34,6 +34,9 @@ struct lux_sgm_ctx {
     hipStream_t stream = nullptr;
     std::map<std::string, DevBuf> bufs;
     std::map<std::tuple<int, int, int, int>, std::shared_ptr<lux::Plan>> plans;
+    // band-tile synchronisation: flags hold the epoch of the request that wrote them
+    int epoch = 0;
+    void* sync_zeroed = nullptr;
     // stage timing (lux_sgm_set_profiling): events recorded between the launches of one request
     bool profiling = false;
     hipEvent_t ev[16] = {};
@@ -122,6 +125,15 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
     if ((rc = get_buf(c, "wta_dr", cells * 2, &drb))) return rc;
     if ((rc = get_buf(c, "band_boundary", (size_t)6 * g.W * g.cs * 2, &bnd))) return rc;
+    const int tiles = lux::band_tiles(g.W, g.H);
+    void* sync;
+    if ((rc = get_buf(c, "band_sync", (size_t)(4 + 2 * tiles) * sizeof(int), &sync))) return rc;
+    if (sync != c->sync_zeroed) {  // a new buffer: no flag may hold a live epoch
+        hipError_t ez = lux::launch_fill_u32((uint32_t*)sync, 4 + 2 * (size_t)tiles, 0u, s);
+        if (ez != hipSuccess) return hip_fail(c, ez, "zero band flags");
+        c->sync_zeroed = sync;
+    }
+    if (++c->epoch <= 0) c->epoch = 1;
     if ((rc = get_buf(c, "band_boundary_min", (size_t)6 * g.W * 2, &bndmin))) return rc;
     if ((rc = get_buf(c, "wta_dr0", (size_t)g.H * 4, &dr0))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
@@ -146,6 +158,11 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.dr0 = (uint32_t*)dr0;
     w->b.bnd = (uint16_t*)bnd;
     w->b.bndmin = (uint16_t*)bndmin;
+    w->b.err = (int*)sync;
+    w->b.ticket = (int*)sync + 2;
+    w->b.flags = (int*)sync + 4;
+    w->b.max_tiles = tiles;
+    w->b.epoch = c->epoch;
     w->b.seeds = (uint16_t*)seeds;
     w->b.seedmins = (uint16_t*)seedmins;
     w->b.cols = (uint16_t*)cols;
@@ -158,6 +175,24 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.cellvis_plus = (const int*)vis + plan->plus.size() + plan->minus.size();
     w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
     w->b.seed_row = plan->seed_row;
+    return 0;
+}
+
+// After the stream has drained: a band tile that gave up waiting leaves the error word set.
+int check_band_error(lux_sgm_ctx* c, hipStream_t s)
+{
+    auto it = c->bufs.find("band_sync");
+    if (it == c->bufs.end() || !it->second.p) return 0;
+    int err = 0;
+    hipError_t e = hipMemcpyAsync(&err, it->second.p, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "read band error word");
+    if (err) {
+        int zero = 0;
+        (void)hipMemcpyAsync(it->second.p, &zero, sizeof(int), hipMemcpyHostToDevice, s);
+        (void)hipStreamSynchronize(s);
+        return fail(c, LUX_SGM_EDEVICE, "a band tile timed out waiting for its neighbours");
+    }
     return 0;
 }
 
@@ -342,7 +377,7 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
     if (e == hipSuccess && agg) e = hipMemcpyAsync(agg, da, px * g.cap * 2, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "download results");
-    return 0;
+    return check_band_error(c, s);
 }
 
 int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
@@ -392,7 +427,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
-    return 0;
+    return check_band_error(c, s);
 }
 
 int lux_sgm_set_profiling(lux_sgm_ctx* c, int on)

@@ -28,6 +28,11 @@ struct SgmBuffers {
     uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch
     uint16_t* bnd = nullptr;        // [2][3][W][cs]  band boundary states (ping-pong)
     uint16_t* bndmin = nullptr;     // [2][3][W]
+    int* ticket = nullptr;          // [2] tile tickets of the two band launches
+    int* flags = nullptr;           // [2][max_tiles] tile completion epochs
+    int max_tiles = 0;
+    int epoch = 0;                  // > 0, distinct per request
+    int* err = nullptr;             // band wait gave up
     const int* visits_plus = nullptr;
     int n_plus = 0;
     const int* visits_minus = nullptr;
@@ -56,6 +61,10 @@ struct SgmOutputs {
 constexpr int kSgmMarks = 7;
 hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                       hipStream_t s, hipEvent_t* marks = nullptr);
+// Fill n u32 with v (a kernel, not a memset: safe inside captured graphs).
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+// Band tiles of a W x H layer (the flag array size per pass).
+int band_tiles(int W, int H);
 // Repack [H][W][cs] <-> [H][W][cap] on the device.
 hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s);
 hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,

@@ -626,14 +626,35 @@ struct BandArgs {
     uint16_t* agg;
     int W, H, cs, cap;
     uint32_t p1x2, p2;
-    int y0, rows;                 // band rows y0, y0+DIR, ..., rows of them
-    const uint16_t* bin;          // [3][W][cs] states of the row before the band (null: the initial fill)
-    const uint16_t* binmin;       // [3][W]
-    uint16_t* bout;               // the band's last row, for the next band
-    uint16_t* boutmin;
+    int split;                    // bands of kBandRows cover rows [0, split) then [split, H) (in pass order)
+    int strips;                   // tiles per band
+    const uint16_t* bin0;         // [3][W][cs] states of the row before the first band (null: the initial fill)
+    const uint16_t* bin0min;      // [3][W]
+    uint16_t* bnd;                // [2][3][W][cs] boundary ping-pong: band b writes slot b & 1
+    uint16_t* bndmin;             // [2][3][W]
+    uint16_t* extra;              // the last row of band `extra_band` also goes here (the pass +1 seed table)
+    uint16_t* extramin;
+    int extra_band;
     uint16_t* cols;               // [2][H][cs] of this pass: slot 1 takes path 1 of the column next to the border
     uint16_t* colmins;
+    int* ticket;                  // tile order (zeroed before the launch)
+    int* flags;                   // [bands][strips]: epoch when the tile's boundary is written
+    int epoch;
+    int* err;                     // set when a wait gives up
 };
+
+// Band b, in pass order: its first row (distance from the pass's first row) and its row count.
+__device__ __forceinline__ void band_rows(int b, int split, int H, int& first, int& rows)
+{
+    const int nsplit = (split + kBandRows - 1) / kBandRows;
+    if (b < nsplit) {
+        first = b * kBandRows;
+        rows = min(kBandRows, split - first);
+    } else {
+        first = split + (b - nsplit) * kBandRows;
+        rows = min(kBandRows, H - first);
+    }
+}
 
 template <int NP, int DIR, bool MASKED>
 __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
@@ -644,9 +665,37 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
     constexpr int JR = DIR > 0 ? 2 : 0;  // the path reading column x+1
     __shared__ uint32_t edge[2][NW][2][64 * NP];
     __shared__ uint32_t edgemin[2][NW][2];
+    __shared__ int tile_s;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int W = a.W, cs = a.cs, rows = a.rows, H1 = rows - 1;
-    const int xa = blockIdx.x * kStrip, xb = min(W, xa + kStrip);
+    // tiles run in ticket order, band by band: every tile this one waits for took an earlier ticket, so it
+    // has started and (by induction) finishes: no deadlock whatever the number of resident workgroups
+    if (threadIdx.x == 0) tile_s = atomicAdd(a.ticket, 1);
+    __syncthreads();
+    const int tile = tile_s;
+    const int band = tile / a.strips, strip = tile - band * a.strips;
+    int first, rows;
+    band_rows(band, a.split, a.H, first, rows);
+    const int y0 = DIR > 0 ? first : a.H - 1 - first;
+    const int W = a.W, cs = a.cs, H1 = rows - 1;
+    const int xa = strip * kStrip, xb = min(W, xa + kStrip);
+    const size_t bsz = (size_t)3 * W * cs, bmsz = (size_t)3 * W;
+    const uint16_t* bin = band == 0 ? a.bin0 : a.bnd + ((band - 1) & 1) * bsz;
+    const uint16_t* binmin = band == 0 ? a.bin0min : a.bndmin + ((band - 1) & 1) * bmsz;
+    if (band > 0 && threadIdx.x == 0) {
+        // wait for the previous band's tiles whose last row this tile's halo reads (strips s-1, s, s+1)
+        for (int d = max(strip - 1, 0); d <= min(strip + 1, a.strips - 1); d++) {
+            const int* f = a.flags + (band - 1) * a.strips + d;
+            int spins = 0;
+            while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1 << 22)) {
+                    atomicOr(a.err, 1);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
     // columns [xa - kBandRows, ...): the strip, the halo, and one guard column each side that only ever holds
     // the boundary state its neighbour reads in the band's first row
     const int cbase = xa - kBandRows + wv * G;
@@ -666,13 +715,13 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
         for (int j = 0; j < 3; j++) {
             set_init(st[g][j]);
             sm[g][j] = 2000;
-            if (a.bin && x >= 0 && x < W) {
-                load_u16<NP>(a.bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+            if (bin && x >= 0 && x < W) {
+                load_u16<NP>(bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
                 if (MASKED) {
 #pragma unroll
                     for (int q = 0; q < NP; q++) st[g][j].v[q] = (st[g][j].v[q] & mask.v[q]) | (kInit2 & ~mask.v[q]);
                 }
-                sm[g][j] = a.binmin[j * W + x];
+                sm[g][j] = binmin[j * W + x];
             }
         }
     }
@@ -689,7 +738,7 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
         return x >= 0 && x < W && x >= xa - L && x < xb + L;
     };
     auto load_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G]) {
-        const int y = a.y0 + DIR * k;
+        const int y = y0 + DIR * k;
 #pragma unroll
         for (int g = 0; g < G; g++) {
             const int x = cbase + g;
@@ -698,7 +747,7 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
         }
     };
     auto do_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G], RawW<NP> (&nrb)[G], Cell<NP> (&nab)[G]) {
-        const int y = a.y0 + DIR * k;
+        const int y = y0 + DIR * k;
         const int par = k & 1;
         // edge exchange: the previous row's states of this wave's boundary columns
 #pragma unroll
@@ -806,16 +855,28 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
         }
     }
     // the band's last row: the boundary the next band starts from
+    uint16_t* bout = a.bnd + (band & 1) * bsz;
+    uint16_t* boutmin = a.bndmin + (band & 1) * bmsz;
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int x = cbase + g;
         if (x >= xa && x < xb) {
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                store_u16<NP>(a.bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
-                if (lane == 0) a.boutmin[j * W + x] = (uint16_t)sm[g][j];
+                store_u16<NP>(bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (lane == 0) boutmin[j * W + x] = (uint16_t)sm[g][j];
+                if (band == a.extra_band) {
+                    store_u16<NP>(a.extra + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                    if (lane == 0) a.extramin[j * W + x] = (uint16_t)sm[g][j];
+                }
             }
         }
+    }
+    // publish: the whole workgroup's stores, then the flag (agent-scope release)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(a.flags + band * a.strips + strip, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1012,6 +1073,10 @@ __global__ void fill_u16(uint16_t* p, size_t n, uint16_t v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
 
 // ------------------------------------------------------------------ census and cost volume
 template <typename T, int R>
@@ -1126,34 +1191,24 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             ba.raw = b.raw; ba.agg = b.agg; ba.W = g.W; ba.H = g.H; ba.cs = g.cs; ba.cap = g.cap;
             ba.p1x2 = a.p1x2; ba.p2 = a.p2;
             ba.cols = cols; ba.colmins = colmins;
-            const int strips = (g.W + kStrip - 1) / kStrip;
-            const size_t bsz = (size_t)3 * g.W * g.cs, bmsz = (size_t)3 * g.W;
-            // pass +1 bands end at the seed row, whose states become the seed table (half 0 as pass +1
-            // leaves it for the interior cells): rows [0, seed_row] in bands of kBandRows, then the rest
-            const int split = DIR > 0 && b.seed_row >= 0 ? b.seed_row + 1 : g.H;
-            int done = 0, bi = 0;
-            while (done < g.H) {
-                const int lim = done < split ? split : g.H;
-                const int rows = min(kBandRows, lim - done);
-                ba.y0 = DIR > 0 ? done : g.H - 1 - done;
-                ba.rows = rows;
-                if (bi == 0) {
-                    ba.bin = DIR > 0 ? nullptr : b.seeds;
-                    ba.binmin = DIR > 0 ? nullptr : b.seedmins;
-                } else {
-                    ba.bin = b.bnd + ((bi - 1) & 1) * bsz;
-                    ba.binmin = b.bndmin + ((bi - 1) & 1) * bmsz;
-                }
-                ba.bout = b.bnd + (bi & 1) * bsz;
-                ba.boutmin = b.bndmin + (bi & 1) * bmsz;
-                sgm_band<NP, DIR, MASKED><<<strips, kBandWaves * 64, 0, s>>>(ba);
-                done += rows;
-                if (DIR > 0 && done == split && split < g.H + 1 && b.seed_row >= 0) {
-                    (void)hipMemcpyAsync(b.seeds, ba.bout, bsz * 2, hipMemcpyDeviceToDevice, s);
-                    (void)hipMemcpyAsync(b.seedmins, ba.boutmin, bmsz * 2, hipMemcpyDeviceToDevice, s);
-                }
-                bi++;
-            }
+            ba.strips = (g.W + kStrip - 1) / kStrip;
+            // pass +1 bands end at the seed row, whose states become the seed table (half 0 as pass +1 leaves
+            // it for the interior cells): rows [0, seed_row] in bands of kBandRows, then the rest
+            ba.split = DIR > 0 && b.seed_row >= 0 ? b.seed_row + 1 : g.H;
+            const int nbands = (ba.split + kBandRows - 1) / kBandRows + (g.H - ba.split + kBandRows - 1) / kBandRows;
+            ba.bin0 = DIR > 0 ? nullptr : b.seeds;
+            ba.bin0min = DIR > 0 ? nullptr : b.seedmins;
+            ba.bnd = b.bnd;
+            ba.bndmin = b.bndmin;
+            ba.extra = b.seeds;
+            ba.extramin = b.seedmins;
+            ba.extra_band = DIR > 0 && b.seed_row >= 0 ? (ba.split + kBandRows - 1) / kBandRows - 1 : -1;
+            ba.ticket = b.ticket + (DIR > 0 ? 0 : 1);
+            ba.flags = b.flags + (DIR > 0 ? 0 : (size_t)b.max_tiles);
+            ba.epoch = b.epoch;
+            ba.err = b.err;
+            fill_u32<<<1, 1, 0, s>>>((uint32_t*)ba.ticket, 1, 0u);
+            sgm_band<NP, DIR, MASKED><<<nbands * ba.strips, kBandWaves * 64, 0, s>>>(ba);
         } else {
             sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
             sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
@@ -1202,6 +1257,18 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
 }
 
 }  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s)
+{
+    fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+int band_tiles(int W, int H)
+{
+    // at most one extra (short) band from the split at the seed row
+    return ((H + kBandRows - 1) / kBandRows + 1) * ((W + kStrip - 1) / kStrip);
+}
 
 Geometry geometry(int W, int H, int D, int align)
 {
