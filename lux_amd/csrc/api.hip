@@ -134,7 +134,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
         c->sync_zeroed = sync;
     }
     if (++c->epoch <= 0) c->epoch = 1;
-    if ((rc = get_buf(c, "band_boundary_min", (size_t)6 * g.W * 2, &bndmin))) return rc;
+    if ((rc = get_buf(c, "band_boundary_min", (size_t)6 * g.W * 4, &bndmin))) return rc;
     if ((rc = get_buf(c, "wta_dr0", (size_t)g.H * 4, &dr0))) return rc;
     if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
     hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), plan->plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
@@ -157,7 +157,7 @@ int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t 
     w->b.dr = (uint16_t*)drb;
     w->b.dr0 = (uint32_t*)dr0;
     w->b.bnd = (uint16_t*)bnd;
-    w->b.bndmin = (uint16_t*)bndmin;
+    w->b.bndmin = (uint32_t*)bndmin;
     w->b.err = (int*)sync;
     w->b.ticket = (int*)sync + 2;
     w->b.flags = (int*)sync + 4;

@@ -27,7 +27,7 @@ struct SgmBuffers {
     uint16_t* colmins = nullptr;    // [2][2][H]
     uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch
     uint16_t* bnd = nullptr;        // [2][3][W][cs]  band boundary states (ping-pong)
-    uint16_t* bndmin = nullptr;     // [2][3][W]
+    uint32_t* bndmin = nullptr;     // [2][3][W]
     int* ticket = nullptr;          // [2] tile tickets of the two band launches
     int* flags = nullptr;           // [2][max_tiles] tile completion epochs
     int max_tiles = 0;

@@ -109,6 +109,24 @@ __device__ __forceinline__ void ld_u16(const uint16_t* p, Cell<NP>& r)
 #pragma unroll
     for (int k = 0; k < NP; k++) r.v[k] = ((const uint32_t*)p)[k];
 }
+// Agent-scope relaxed atomic accesses (global_load / global_store with sc1): the band-boundary states go
+// through these so they are coherent across XCDs without a full L2 writeback / invalidate per tile.
+template <int NP>
+__device__ __forceinline__ void store_u16_agent(uint16_t* p, bool active, const Cell<NP>& r)
+{
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+            __hip_atomic_store((uint32_t*)p + k, r.v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+template <int NP>
+__device__ __forceinline__ void load_u16_agent(const uint16_t* p, bool active, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+        r.v[k] = active ? __hip_atomic_load((const uint32_t*)p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+}
 template <int NP>
 __device__ __forceinline__ void widen(const RawW<NP>& b, Cell<NP>& r)
 {
@@ -629,9 +647,9 @@ struct BandArgs {
     int split;                    // bands of kBandRows cover rows [0, split) then [split, H) (in pass order)
     int strips;                   // tiles per band
     const uint16_t* bin0;         // [3][W][cs] states of the row before the first band (null: the initial fill)
-    const uint16_t* bin0min;      // [3][W]
+    const uint16_t* bin0min;      // [3][W] (u16, the seed table's)
     uint16_t* bnd;                // [2][3][W][cs] boundary ping-pong: band b writes slot b & 1
-    uint16_t* bndmin;             // [2][3][W]
+    uint32_t* bndmin;             // [2][3][W]
     uint16_t* extra;              // the last row of band `extra_band` also goes here (the pass +1 seed table)
     uint16_t* extramin;
     int extra_band;
@@ -680,13 +698,14 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
     const int xa = strip * kStrip, xb = min(W, xa + kStrip);
     const size_t bsz = (size_t)3 * W * cs, bmsz = (size_t)3 * W;
     const uint16_t* bin = band == 0 ? a.bin0 : a.bnd + ((band - 1) & 1) * bsz;
-    const uint16_t* binmin = band == 0 ? a.bin0min : a.bndmin + ((band - 1) & 1) * bmsz;
+    const uint32_t* binmin = band == 0 ? nullptr : a.bndmin + ((band - 1) & 1) * bmsz;
     if (band > 0 && threadIdx.x == 0) {
-        // wait for the previous band's tiles whose last row this tile's halo reads (strips s-1, s, s+1)
+        // wait for the previous band's tiles whose last row this tile's halo reads (strips s-1, s, s+1); their
+        // boundary stores were drained before the flag, and are read below with agent-scope loads
         for (int d = max(strip - 1, 0); d <= min(strip + 1, a.strips - 1); d++) {
             const int* f = a.flags + (band - 1) * a.strips + d;
             int spins = 0;
-            while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+            while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
                 __builtin_amdgcn_s_sleep(2);
                 if (++spins > (1 << 22)) {
                     atomicOr(a.err, 1);
@@ -716,12 +735,13 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
             set_init(st[g][j]);
             sm[g][j] = 2000;
             if (bin && x >= 0 && x < W) {
-                load_u16<NP>(bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                load_u16_agent<NP>(bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
                 if (MASKED) {
 #pragma unroll
                     for (int q = 0; q < NP; q++) st[g][j].v[q] = (st[g][j].v[q] & mask.v[q]) | (kInit2 & ~mask.v[q]);
                 }
-                sm[g][j] = binmin[j * W + x];
+                sm[g][j] = binmin ? __hip_atomic_load(binmin + j * W + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : (uint32_t)a.bin0min[j * W + x];
             }
         }
     }
@@ -856,15 +876,15 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
     }
     // the band's last row: the boundary the next band starts from
     uint16_t* bout = a.bnd + (band & 1) * bsz;
-    uint16_t* boutmin = a.bndmin + (band & 1) * bmsz;
+    uint32_t* boutmin = a.bndmin + (band & 1) * bmsz;
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int x = cbase + g;
         if (x >= xa && x < xb) {
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                store_u16<NP>(bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
-                if (lane == 0) boutmin[j * W + x] = (uint16_t)sm[g][j];
+                store_u16_agent<NP>(bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (lane == 0) __hip_atomic_store(boutmin + j * W + x, sm[g][j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (band == a.extra_band) {
                     store_u16<NP>(a.extra + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
                     if (lane == 0) a.extramin[j * W + x] = (uint16_t)sm[g][j];
@@ -872,12 +892,11 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
             }
         }
     }
-    // publish: the whole workgroup's stores, then the flag (agent-scope release)
+    // publish: every wave drains its stores (the boundary's agent-scope stores are then visible device-wide),
+    // then one flag store
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_store(a.flags + band * a.strips + strip, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) __hip_atomic_store(a.flags + band * a.strips + strip, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct ReplayArgs {
