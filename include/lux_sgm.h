@@ -108,6 +108,12 @@ typedef struct {
 int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
                           uint16_t* agg_pass1, uint16_t* wta);
 
+/* Context options (no counterpart in the reference: execution strategy only, results are identical).
+ * LUX_SGM_OPT_FUSED_BANDS: 1 runs the three vertical/diagonal paths of each pass fused in band tiles that
+ * synchronise through device flags (up to 256 planes); 0 (default) runs one sweep kernel per path. */
+#define LUX_SGM_OPT_FUSED_BANDS 1
+int lux_sgm_set_option(lux_sgm_ctx* ctx, int32_t option, int32_t value);
+
 /* Stage timing, the per-layer timing log of the reference's pyramid driver (DenseStereoSgm.cs:347-349)
  * at kernel granularity. With profiling on, each rectified request records HIP events on its stream
  * between launches; lux_sgm_stage_times waits for the last profiled request and writes the durations in

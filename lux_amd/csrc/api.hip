@@ -34,6 +34,7 @@ struct lux_sgm_ctx {
     hipStream_t stream = nullptr;
     std::map<std::string, DevBuf> bufs;
     std::map<std::tuple<int, int, int, int>, std::shared_ptr<lux::Plan>> plans;
+    int opt_bands = 0;
     // band-tile synchronisation: flags hold the epoch of the request that wrote them
     int epoch = 0;
     void* sync_zeroed = nullptr;
@@ -246,6 +247,7 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
     o.valid = valid;
     o.tol = p->lr_tolerance;
     o.write_agg = agg_out != nullptr;
+    o.bands = c->opt_bands;
     e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, nullptr, s, c->profiling ? c->ev + 2 : nullptr);
     if (e != hipSuccess) return hip_fail(c, e, "aggregation");
     if (c->profiling) c->nev = 2 + lux::kSgmMarks;
@@ -410,6 +412,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     o.disp = (uint16_t*)dw;
     o.tol = 0;
     o.write_agg = agg != nullptr;
+    o.bands = c->opt_bands;
     e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
     if (e != hipSuccess) return hip_fail(c, e, "aggregation");
     if (wta) {
@@ -428,6 +431,16 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
     return check_band_error(c, s);
+}
+
+int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    switch (option) {
+        case LUX_SGM_OPT_FUSED_BANDS: c->opt_bands = value != 0; return LUX_SGM_OK;
+        default: return fail(c, LUX_SGM_EINVAL, "unknown option");
+    }
 }
 
 int lux_sgm_set_profiling(lux_sgm_ctx* c, int on)

@@ -54,6 +54,7 @@ struct SgmOutputs {
     uint8_t* valid = nullptr;       // may be null
     int tol = 1;
     int write_agg = 0;              // leave the final aggregate in the agg volume
+    int bands = 0;                  // paths 1-3 of a pass in fused band tiles (planes <= 256) instead of sweeps
 };
 // The eight path sweeps (the census cost fused into the first when b.cl is set, the winner-take-all into
 // the last), the two border replays and the column-0 / validity finish. agg_pass1 (if not null) receives

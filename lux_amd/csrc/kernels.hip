@@ -636,7 +636,7 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
 // band left at its last row (a ping-pong boundary buffer). The diagonal paths also run over halo columns,
 // one fewer per row (kBandRows-1 at the band's first row), so a strip needs no other strip of its band.
 // Waves own kBandG consecutive columns each and swap their edge columns' states through LDS once per row.
-constexpr int kBandRows = 16, kStrip = 32, kBandWaves = 16, kBandG = 4;
+constexpr int kBandRows = 8, kStrip = 16, kBandWaves = 8, kBandG = 4;
 static_assert(kStrip + 2 * kBandRows <= kBandWaves * kBandG, "band halo does not fit the waves");
 
 struct BandArgs {
@@ -1205,6 +1205,12 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
     auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
         constexpr int DIR = decltype(dirc)::value;
+        if (NP > 2 || !o.bands) {
+            sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            return;
+        }
         if constexpr (NP <= 2) {
             BandArgs ba;
             ba.raw = b.raw; ba.agg = b.agg; ba.W = g.W; ba.H = g.H; ba.cs = g.cs; ba.cap = g.cap;

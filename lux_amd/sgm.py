@@ -19,7 +19,8 @@ COST_CENSUS, COST_AD = 0, 1
 
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
-           "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times")
+           "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option")
+OPT_FUSED_BANDS = 1
 STAGES = ("census", "rows+", "bands+", "replay+", "bands-", "rows-", "replay-", "finish")
 
 
@@ -66,6 +67,7 @@ def lib():
         L.lux_sgm_plan_visits.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32]
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
         L.lux_sgm_stage_times.argtypes = [vp, vp, i32]
+        L.lux_sgm_set_option.argtypes = [vp, i32, i32]
         _LIB = L
     return _LIB
 
@@ -106,6 +108,9 @@ class Context:
         if self.handle:
             lib().lux_sgm_destroy(self.handle)
             self.handle = None
+
+    def set_option(self, option, value):
+        _check(lib().lux_sgm_set_option(self.handle, option, value), self.handle)
 
     def set_profiling(self, on=True):
         _check(lib().lux_sgm_set_profiling(self.handle, int(bool(on))), self.handle)

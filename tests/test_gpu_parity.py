@@ -10,9 +10,10 @@ pytestmark = pytest.mark.gpu
 SUB_TOL = 1e-4  # px
 
 
-@pytest.fixture(scope="module")
-def ctx():
+@pytest.fixture(scope="module", params=["sweeps", "bands"])
+def ctx(request):
     c = sgm.Context(0)
+    c.set_option(sgm.OPT_FUSED_BANDS, int(request.param == "bands"))
     yield c
     c.close()
 
