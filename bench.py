@@ -24,15 +24,20 @@ HBM_PEAK_GBS = 8000.0
 PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1, agg 2), WTA fused
 # algorithmic HBM bytes per cell of each stage (what the stage must move at minimum)
 # algorithmic HBM bytes per cell of each timed stage:
-#   rows+  (pass +1 path ->, census cost fused): writes raw (1) and agg (2)
-#   bands+ (pass +1 paths 1-3 fused): reads raw (1), read-modify-writes agg (2 + 2)
-#   bands- (pass -1 paths 1-3 fused): the same
-#   rows-  (pass -1 path <-, winner-take-all fused): reads raw (1) and agg (2)
-STAGE_BYTES = {"rows+": 3, "bands+": 5, "bands-": 5, "rows-": 3}
+#   rows+ (pass +1 path ->, census cost fused): writes raw (1) and agg (2)
+#   vd+k / vd-k (one vertical or diagonal path sweep): reads raw (1), read-modify-writes agg (2 + 2);
+#     with fused bands vd+1 / vd-1 is the one launch for the pass's three paths, still 5 B/cell
+#   rows- (pass -1 path <-, winner-take-all fused): reads raw (1) and agg (2)
+STAGE_BYTES = {"rows+": 3, "rows-": 3}
+for _k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3"):
+    STAGE_BYTES[_k] = 5
 KERNEL_NAMES = {"rows+": "sgm_row_fwd (pass +1 path ->, census cost fused)",
-                "bands+": "sgm_band<pass +1> (paths down-right, down, down-left fused; one launch per 16-row band)",
-                "bands-": "sgm_band<pass -1> (paths up-left, up, up-right fused; one launch per 16-row band)",
-                "rows-": "sgm_row_bwd_wta (pass -1 path <-, winner-take-all fused)"}
+                "rows-": "sgm_row_bwd_wta (pass -1 path <-, winner-take-all fused)",
+                "vd+1": "sgm_sweep<pass +1, path 1 (down-right)>", "vd+2": "sgm_sweep<pass +1, path 2 (down)>",
+                "vd+3": "sgm_sweep<pass +1, path 3 (down-left)>", "vd-1": "sgm_sweep<pass -1, path 1 (up-left)>",
+                "vd-2": "sgm_sweep<pass -1, path 2 (up)>", "vd-3": "sgm_sweep<pass -1, path 3 (up-right)>"}
+BAND_NAMES = {"vd+1": "sgm_band<pass +1> (paths 1-3 fused, ticketed band tiles)",
+              "vd-1": "sgm_band<pass -1> (paths 1-3 fused, ticketed band tiles)"}
 
 
 def host_info():
@@ -67,6 +72,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
     ap.add_argument("--profile-run", action="store_true",
                     help="for counter runs: a few requests, then a 1 GiB device copy as the byte-count calibration")
     args = ap.parse_args()
@@ -92,6 +98,7 @@ def main():
     sub = torch.empty((H, W), dtype=torch.float32, device=dev)
     valid = torch.empty((H, W), dtype=torch.uint8, device=dev)
     ctx = sgm.Context(0)
+    ctx.set_option(sgm.OPT_FUSED_BANDS, args.bands)
     p = sgm.rect_params(W, H, D, bits=BITS, cost="census", radius=RADIUS, p1=P1, p2=P2, tile=TILE, tol=1)
     # a stream of our own (default priority): the library enqueues on it, the events below are recorded on it
     stream = torch.cuda.Stream()
@@ -145,6 +152,7 @@ def main():
     ctx.set_profiling(False)
     avg = {k: float(np.mean(v)) for k, v in stages.items()}
     dom = max((k for k in avg if k in STAGE_BYTES), key=lambda k: avg[k])
+    names = dict(KERNEL_NAMES, **(BAND_NAMES if args.bands else {}))
     cells = W * H * D
     achieved = STAGE_BYTES[dom] * cells / (avg[dom] / 1e3) / 1e9
     traffic = None
@@ -185,12 +193,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain GPU drop-in, one 4160x3120 uint16 10-bit pair, census 7x7, D=256, 8 paths, "
                                "P1=8, P2=96, WTA + subpixel + L/R check (tol 1), reference schedule T=64",
+                   "vertical_diagonal": "fused bands" if args.bands else "per-path sweeps",
                    "width": W, "height": H, "disparities": D, "cells": cells, "inputs_sha256": digest},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "form": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": KERNEL_NAMES.get(dom, dom), "avg_ms": round(avg[dom], 4),
+                     "kernel": names.get(dom, dom), "avg_ms": round(avg[dom], 4),
                      "algorithmic_bytes": STAGE_BYTES[dom] * cells},
         "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL,
                           "achieved": round(PATH_BYTES_PER_CELL * cells / (ms / 1e3) / 1e9, 1),

@@ -118,9 +118,10 @@ int lux_sgm_set_option(lux_sgm_ctx* ctx, int32_t option, int32_t value);
  * at kernel granularity. With profiling on, each rectified request records HIP events on its stream
  * between launches; lux_sgm_stage_times waits for the last profiled request and writes the durations in
  * ms of its stages, in order: census codes (or the AD cost volume), pass +1 path -> (census cost fused),
- * pass +1 paths down-right/down/down-left, the pass +1 border replay, pass -1 paths up-left/up/up-right,
- * pass -1 path <- (winner-take-all fused), the pass -1 border replay, and the column-0 / validity finish.
- * Returns the number of stages written (8), or < 0. */
+ * pass +1 paths down-right, down, down-left (three stages; with LUX_SGM_OPT_FUSED_BANDS the first holds
+ * the one fused launch and the other two are empty), the pass +1 border replay, pass -1 paths up-left, up,
+ * up-right (likewise), pass -1 path <- (winner-take-all fused), the pass -1 border replay, and the
+ * column-0 / validity finish. Returns the number of stages written (12), or < 0. */
 int lux_sgm_set_profiling(lux_sgm_ctx* ctx, int on);
 int lux_sgm_stage_times(lux_sgm_ctx* ctx, float* ms, int32_t cap);
 

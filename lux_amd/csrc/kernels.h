@@ -59,7 +59,7 @@ struct SgmOutputs {
 // The eight path sweeps (the census cost fused into the first when b.cl is set, the winner-take-all into
 // the last), the two border replays and the column-0 / validity finish. agg_pass1 (if not null) receives
 // agg after pass +1. marks (if not null, kSgmMarks events) are recorded after each stage, in launch order.
-constexpr int kSgmMarks = 7;
+constexpr int kSgmMarks = 11;
 hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                       hipStream_t s, hipEvent_t* marks = nullptr);
 // Fill n u32 with v (a kernel, not a memset: safe inside captured graphs).

@@ -1207,8 +1207,11 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         constexpr int DIR = decltype(dirc)::value;
         if (NP > 2 || !o.bands) {
             sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            mark();
             sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+            mark();
             sgm_sweep<NP, DIR, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            mark();
             return;
         }
         if constexpr (NP <= 2) {
@@ -1234,6 +1237,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             ba.err = b.err;
             fill_u32<<<1, 1, 0, s>>>((uint32_t*)ba.ticket, 1, 0u);
             sgm_band<NP, DIR, MASKED><<<nbands * ba.strips, kBandWaves * 64, 0, s>>>(ba);
+            mark();  // one launch for the three paths: the next two stages are empty
+            mark();
+            mark();
         } else {
             sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
             sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
@@ -1247,7 +1253,6 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     else sgm_row_fwd<NP, MASKED, false><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
     vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
-    mark();
     ReplayArgs r;
     r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
     r.p1x2 = a.p1x2; r.p2 = a.p2;
@@ -1267,7 +1272,6 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
     vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
-    mark();
     sgm_row_bwd_wta<NP, MASKED><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;

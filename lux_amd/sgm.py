@@ -21,7 +21,7 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option")
 OPT_FUSED_BANDS = 1
-STAGES = ("census", "rows+", "bands+", "replay+", "bands-", "rows-", "replay-", "finish")
+STAGES = ("census", "rows+", "vd+1", "vd+2", "vd+3", "replay+", "vd-1", "vd-2", "vd-3", "rows-", "replay-", "finish")
 
 
 class SgmError(RuntimeError):
