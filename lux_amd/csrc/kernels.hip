@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     }
     // software pipeline: PF cells in flight; raw stays packed until its step. Prefetch addresses are clamped
     // to the chain's last cell and idle lanes read lane 0's planes, so every load is unconditional.
-    constexpr int PF = 4;
+    constexpr int PF = 8;
     const ptrdiff_t step = (ptrdiff_t)DY * W + DX;
     const size_t cell0 = (size_t)y * W + x;
     const int lol = active ? lo : 0;
@@ -411,7 +411,7 @@ __device__ __forceinline__ void census_cost(const CensusIn<NP>& c, int D, int lo
 template <int NP, bool MASKED, bool COST>
 __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 {
-    constexpr int PF = 4;
+    constexpr int PF = 8;
     const int lane = threadIdx.x & 63;
     const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 template <int NP, bool MASKED>
 __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 {
-    constexpr int PF = 4, E = 2 * NP;
+    constexpr int PF = 8, E = 2 * NP;
     const int lane = threadIdx.x & 63;
     const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
