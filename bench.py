@@ -159,7 +159,8 @@ def main():
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get(dom)
+            with open(tf) as fh:
+                traffic = json.load(fh).get("bands" if args.bands else "sweeps", {}).get(dom)
         except (OSError, ValueError):
             traffic = None
 

@@ -4,9 +4,8 @@
  * Drop-in for the depth hot path of Lux.Engine (mprovenz/lux): the body of DenseLayer.Compute
  * (Lux.Engine/Pipeline/Registration/DenseStereoSgm.cs:181-232) and a rectified census/AD front end that
  * feeds it. Plain C types only; no pointer is kept after a call returns. Status codes: 0 = OK, < 0 = error
- * class (the message is available from lux_sgm_last_error()). The binding a C# caller adds is shown in
- * INTEGRATION.md (P/Invoke, as Lux.Engine/Mtp/GPhoto2/GPhoto2Interop.cs:9-51 and
- * Lux.DepthModel/OnnxDepthNetwork.cs:46-95 bind native code).
+ * class (the message is available from lux_sgm_last_error()). A C# caller binds it over P/Invoke, as
+ * Lux.Engine/Mtp/GPhoto2/GPhoto2Interop.cs:9-51 and Lux.DepthModel/OnnxDepthNetwork.cs:46-95 bind native code.
  *
  * Threading: contexts are independent; one context is used by one thread at a time (calls on one context
  * are serialised internally). Results do not depend on concurrency (CaptureSession.cs:108-109).
