@@ -6,6 +6,14 @@ Prints ONE JSON line. `value` = Gcells/s = W*H*D*8 paths / time per request, inp
 `roofline` = the dominant kernel's algorithmic HBM bytes per launch / its average duration (HIP events on
 the stream the kernels run on). `cpu_baseline` = the CPU oracle (the reference's algorithm restated in C:
 row-parallel cost, serial aggregation) on a bounded band of the same workload, on this host.
+
+Two more legs go into the same line:
+  `batched`: one capture of 10 pairs (6 x 2080x1560 D=128, 4 x 3120x2340 D=192, independent seeds) as one graph
+             launch with the autotuned schedule, against the same pairs run one after another (plain drop-in);
+  `served`:  a mixed stream of depth requests (sizes uniform over 640x480 D64, 1040x780 D96, 2080x1560 D128,
+             4160x3120 D256; host images in, host results out) through the request loop under a 96 GB budget:
+             latency p50/p99 at Poisson arrivals of 40 req/s, and throughput at saturation against the plain
+             drop-in running the same requests in order.
 """
 import argparse
 import json
@@ -65,6 +73,159 @@ def cpu_baseline(left, right, rows, threads):
     return cells * 8 / t / 1e9, t, out["times_ms"]
 
 
+BATCH = [(2080, 1560, 128)] * 6 + [(3120, 2340, 192)] * 4
+SERVE_SIZES = [(640, 480, 64), (1040, 780, 96), (2080, 1560, 128), (4160, 3120, 256)]
+
+
+def scene_params(sgm, w, h, d):
+    return sgm.rect_params(w, h, d, bits=BITS, cost="census", radius=RADIUS, p1=P1, p2=P2, tile=TILE, tol=1)
+
+
+def bench_batched(np, torch, sgm, scene, reps):
+    """Ten pairs of one capture: plain one-by-one on one stream vs one tuned graph launch."""
+    dev = torch.device("cuda:0")
+    pairs, keep, cells = [], [], 0
+    for i, (w, h, d) in enumerate(BATCH):
+        left, right, _ = scene.make_pair(w, h, d - 1, planes=6, sigma=4.0, seed=100 + i, bits=BITS)
+        tl = torch.from_numpy(left.view(np.int16)).to(dev)
+        tr = torch.from_numpy(right.view(np.int16)).to(dev)
+        od = torch.empty((h, w), dtype=torch.int16, device=dev)
+        osub = torch.empty((h, w), dtype=torch.float32, device=dev)
+        ov = torch.empty((h, w), dtype=torch.uint8, device=dev)
+        pr = sgm.Pair()
+        pr.params = scene_params(sgm, w, h, d)
+        pr.left, pr.right, pr.disp, pr.sub, pr.valid = (tl.data_ptr(), tr.data_ptr(), od.data_ptr(), osub.data_ptr(),
+                                                        ov.data_ptr())
+        pairs.append(pr)
+        keep += [tl, tr, od, osub, ov]
+        cells += w * h * d
+    stream = torch.cuda.Stream()
+    ctx = sgm.Context(0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def plain_all():
+        for pr in pairs:
+            sgm.rectified_device(ctx, pr.params, pr.left, pr.right, pr.disp, pr.sub, pr.valid, 0, stream.cuda_stream)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_plain = timed(plain_all)
+    ref = [t.clone() for t in keep[2::5]]  # plain disparities
+    best, tried = sgm.batch_tune(ctx, pairs, stream=stream.cuda_stream, reps=2)
+    ms_batch = timed(lambda: sgm.batch(ctx, pairs, stream=stream.cuda_stream))
+    same = all(bool(torch.equal(a, b)) for a, b in zip(ref, keep[2::5]))
+    ctx.close()
+    del keep
+    return {"workload": "6 x 2080x1560 D=128 + 4 x 3120x2340 D=192, uint16 10-bit, census 7x7, seeds 100..109",
+            "cells": cells, "ms_per_batch": round(ms_batch, 3), "gcells_s": round(cells * 8 / ms_batch / 1e6, 3),
+            "plain_ms": round(ms_plain, 3), "plain_gcells_s": round(cells * 8 / ms_plain / 1e6, 3),
+            "speedup_vs_plain": round(ms_plain / ms_batch, 3), "schedule": best,
+            "candidates": tried, "identical_to_plain": same}
+
+
+def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
+    """Mixed request stream: plain drop-in in order, served at saturation, served at Poisson arrivals."""
+    import threading
+    import queue
+
+    rng = np.random.default_rng(7)
+    sizes = rng.integers(0, len(SERVE_SIZES), n_req)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    scenes = {}
+    for k, (w, h, d) in enumerate(SERVE_SIZES):
+        scenes[k] = [scene.make_pair(w, h, d - 1, planes=6, sigma=4.0, seed=200 + 10 * k + j, bits=BITS)[:2]
+                     for j in range(2)]
+    params = {k: scene_params(sgm, *SERVE_SIZES[k]) for k in scenes}
+    cells = int(sum(SERVE_SIZES[k][0] * SERVE_SIZES[k][1] * SERVE_SIZES[k][2] for k in sizes))
+
+    def outputs(k):
+        w, h, _ = SERVE_SIZES[k]
+        return (np.empty((h, w), np.uint16), np.empty((h, w), np.float32), np.empty((h, w), np.uint8))
+
+    # plain drop-in: lux_sgm_rectified (host images in, host results out), one request after another
+    ctx = sgm.Context(0)
+    for k in scenes:  # warm every size's workspace once
+        sgm.rectified(ctx, *scenes[k][0], SERVE_SIZES[k][2], bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+    plain_lat = []
+    t0 = time.perf_counter()
+    for i, k in enumerate(sizes):
+        a = time.perf_counter()
+        sgm.rectified(ctx, *scenes[k][i % 2], SERVE_SIZES[k][2], bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+        plain_lat.append((time.perf_counter() - a) * 1e3)
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+
+    srv = sgm.Server(0, int(budget_gb * 1e9), slots=slots)
+
+    def run(arrivals):
+        """Submit in order (at arrival times if given, else keeping `window` in flight); collect on a thread."""
+        q = queue.Queue()
+        stats, free = [], {k: [] for k in scenes}
+        sem = threading.Semaphore(window if arrivals is None else n_req)
+
+        def collector():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                t, k, outs = item
+                stats.append(srv.wait(t))
+                free[k].append(outs)
+                sem.release()
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t_start = time.perf_counter()
+        for i, k in enumerate(sizes):
+            if arrivals is not None:
+                delay = t_start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            sem.acquire()
+            outs = free[k].pop() if free[k] else outputs(k)
+            t = srv.submit(params[k], *scenes[k][i % 2], disp=outs[0], sub=outs[1], valid=outs[2])
+            q.put((t, k, outs))
+        q.put(None)
+        th.join()
+        return time.perf_counter() - t_start, stats
+
+    for k in scenes:  # warm every slot's workspace for every size
+        run_w = [srv.submit(params[k], *scenes[k][0]) for _ in range(slots)]
+        for t in run_w:
+            srv.wait(t)
+    sat_s, sat_stats = run(None)
+    poi_s, poi_stats = run(np.cumsum(gaps))
+    info = srv.info()
+    srv.close()
+    lat = np.array([st["total_ms"] for st in poi_stats])
+    sat_lat = np.array([st["total_ms"] for st in sat_stats])
+    return {"workload": f"{n_req} requests, sizes uniform over 640x480 D64 / 1040x780 D96 / 2080x1560 D128 / "
+                        f"4160x3120 D256 (seed 7), uint16 10-bit, census 7x7; host images in, host results out",
+            "cells": cells, "slots": slots, "budget_gb": budget_gb,
+            "poisson": {"rate_per_s": rate, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3), "wall_s": round(poi_s, 3),
+                        "latency": "submit -> device work and result download complete"},
+            "saturation": {"window": window, "wall_s": round(sat_s, 3),
+                           "gcells_s": round(cells * 8 / sat_s / 1e9, 3),
+                           "req_per_s": round(n_req / sat_s, 3),
+                           "p50_ms": round(float(np.percentile(sat_lat, 50)), 3),
+                           "p99_ms": round(float(np.percentile(sat_lat, 99)), 3)},
+            "plain": {"wall_s": round(plain_s, 3), "gcells_s": round(cells * 8 / plain_s / 1e9, 3),
+                      "req_per_s": round(n_req / plain_s, 3),
+                      "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(plain_lat, 99)), 3)},
+            "served_vs_plain_throughput": round(plain_s / sat_s, 3),
+            "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +234,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
+    ap.add_argument("--forms", default="batch,serve", help="extra legs: batch, serve (comma list; '' for none)")
+    ap.add_argument("--serve-requests", type=int, default=500)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--profile-run", action="store_true",
                     help="for counter runs: a few requests, then a 1 GiB device copy as the byte-count calibration")
     args = ap.parse_args()
@@ -179,6 +344,17 @@ def main():
                           f"threads, aggregation {tms[1] / 1e3:.1f} s serial as DenseStereoSgm.cs:171, "
                           f"WTA {tms[2] / 1e3:.1f} s); host {model}, {ncpu} CPUs")}
 
+    ctx.close()
+    del dl, dr
+    forms = [f for f in args.forms.split(",") if f]
+    batched = served = None
+    if "batch" in forms:
+        print("bench: batched leg", file=sys.stderr, flush=True)
+        batched = bench_batched(np, torch, sgm, scene, max(2, args.steps // 4))
+    if "serve" in forms:
+        print("bench: served leg", file=sys.stderr, flush=True)
+        served = bench_served(np, sgm, scene, args.serve_requests, args.serve_rate, 96, args.serve_slots, 16)
+
     out = {
         "metric": "SGM Gcells/s (W·H·D·paths) + p50/p99 depth-request latency, served vs plain",
         "value": round(gcells, 3),
@@ -209,8 +385,9 @@ def main():
         "accuracy": {"valid_frac": round(float(v.mean()), 4), "within1_of_truth_on_valid": round(acc, 4)},
         "wall_s": round(t_wall, 3),
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
-    ctx.close()
     print(json.dumps(out))
 
 

@@ -137,6 +137,92 @@ int lux_sgm_plan_info(int32_t width, int32_t height, int32_t tile, int32_t xs_cl
 int lux_sgm_plan_visits(int32_t width, int32_t height, int32_t tile, int32_t xs_clamp_w, int32_t* plus, int32_t cap_plus,
                         int32_t* minus, int32_t cap_minus);
 
+/* ---------------------------------------------------------------------------------------------------------
+ * Batched form: all pairs of one capture (ExportBuild.cs:87-112 runs the dense stage once per capture) in one
+ * graph launch. Every pointer of a pair is device memory; disp/sub/valid may each be NULL.
+ */
+typedef struct {
+    lux_sgm_rect_params params;
+    const void* left;
+    const void* right;
+    uint16_t* disp;
+    float* sub;
+    uint8_t* valid;
+} lux_sgm_pair;
+
+/* A batch schedule: pairs spread over `branches` concurrent graph branches, longest first onto the least
+ * loaded branch; `bands` selects the fused-band paths (LUX_SGM_OPT_FUSED_BANDS). ms: measured time of one
+ * batch under this schedule (0 when not measured). */
+typedef struct {
+    int32_t bands;
+    int32_t branches;
+    float ms;
+} lux_sgm_schedule;
+
+/* Enqueue the n pairs on `stream` (NULL = the context's stream) as one hipGraph launch. The graph is built on
+ * the first call with a given batch (descriptors and pointers) and replayed afterwards. The schedule is the
+ * one lux_sgm_batch_tune chose for pairs of these shapes, else one branch per pair up to four. Returns once
+ * enqueued; results are those of lux_sgm_rectified_device on each pair alone, bit for bit. No counterpart in
+ * the reference (its dense stage runs one layer at a time, DenseStereoSgm.cs:346-351). */
+int lux_sgm_batch(lux_sgm_ctx* ctx, int32_t n, const lux_sgm_pair* pairs, void* stream);
+
+/* Autotuner: time each candidate schedule on this batch (each run once to warm, then `reps` timed runs) and
+ * keep the fastest for batches of the same shapes. best receives it; tried (if not NULL) receives up to cap
+ * candidates with their times. Returns the number of candidates tried, or < 0. */
+int lux_sgm_batch_tune(lux_sgm_ctx* ctx, int32_t n, const lux_sgm_pair* pairs, void* stream, int32_t reps,
+                       lux_sgm_schedule* best, lux_sgm_schedule* tried, int32_t cap);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Served form: a request loop over `slots` concurrent device slots (each a stream and a pooled workspace)
+ * that admits depth requests in arrival order as slots and HBM budget free up, so that requests of
+ * mixed sizes overlap on the device. It replaces the per-capture memory admission of the reference's job
+ * queue (JobQueue.cs:23-31) and the 1 GiB layer rule (DenseStereoSgm.cs:330,339-344) for this stage.
+ */
+typedef struct lux_sgm_server lux_sgm_server;
+
+#define LUX_SGM_MEM_HOST 0   /* images and outputs are host memory: inputs are copied at submit */
+#define LUX_SGM_MEM_DEVICE 1 /* device memory, valid until the request completes */
+
+#define LUX_SGM_REQ_PENDING 0
+#define LUX_SGM_REQ_RUNNING 1
+#define LUX_SGM_REQ_DONE 2
+
+typedef struct {
+    double queued_ms;   /* submit -> admitted to a slot */
+    double run_ms;      /* admitted -> device work complete */
+    double total_ms;    /* submit -> complete: the request's latency */
+    int32_t slot;
+    int32_t status;     /* LUX_SGM_OK or the error class */
+} lux_sgm_request_stats;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    int32_t pending, running, slots, reserved;
+    uint64_t budget, bytes_held, bytes_peak;
+} lux_sgm_server_info;
+
+/* hbm_budget bounds the device bytes of all slot workspaces together (0 = no bound); slots in 1..16. */
+int lux_sgm_server_create(int device, uint64_t hbm_budget, int32_t slots, lux_sgm_server** out);
+/* Waits for running requests, cancels pending ones, frees everything. */
+int lux_sgm_server_destroy(lux_sgm_server* srv);
+const char* lux_sgm_server_last_error(const lux_sgm_server* srv);
+/* Queue one rectified request. mem = LUX_SGM_MEM_HOST: left/right are copied before the call returns and
+ * disp/sub/valid (each may be NULL) are written when lux_sgm_server_wait reports completion.
+ * mem = LUX_SGM_MEM_DEVICE: every pointer is device memory that stays valid until completion. A request
+ * larger than the whole budget fails here with LUX_SGM_ENOMEM. */
+int lux_sgm_server_submit(lux_sgm_server* srv, const lux_sgm_rect_params* p, int32_t mem, const void* left,
+                          const void* right, uint16_t* disp, float* sub, uint8_t* valid, uint64_t* ticket);
+/* State of a request (LUX_SGM_REQ_*), without waiting. */
+int lux_sgm_server_poll(lux_sgm_server* srv, uint64_t ticket, int32_t* state);
+/* Wait up to timeout_ms (< 0: no limit) for a request; on completion copy host outputs, fill stats (may be
+ * NULL), forget the ticket and return the request's status. LUX_SGM_ECANCELED on timeout (the request
+ * stays queued) or for a cancelled request. */
+int lux_sgm_server_wait(lux_sgm_server* srv, uint64_t ticket, int32_t timeout_ms, lux_sgm_request_stats* stats);
+/* Cancel a request that has not been admitted yet (the reference cancels between stages,
+ * StereoAsyncApi.cs:51-63); a running request completes. */
+int lux_sgm_server_cancel(lux_sgm_server* srv, uint64_t ticket);
+int lux_sgm_server_info_get(lux_sgm_server* srv, lux_sgm_server_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,5 @@
-// C ABI of liblux_sgm.so (declared in include/lux_sgm.h): contexts, pooled device buffers, the schedule-plan
-// cache and the entry points that replace DenseLayer.Compute (DenseStereoSgm.cs:181-232).
+// C ABI of liblux_sgm.so (declared in include/lux_sgm.h): contexts and the entry points that replace
+// DenseLayer.Compute (DenseStereoSgm.cs:181-232), the rectified front end, and the batched form.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,46 +9,43 @@
 #include <memory>
 #include <mutex>
 #include <string>
-#include <tuple>
+#include <vector>
 
 #include "../../include/lux_sgm.h"
-#include "kernels.h"
-#include "plan.h"
+#include "engine.h"
 
 namespace {
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-};
-
 thread_local std::string g_create_error;
+
+constexpr int kMaxBranches = 8;
+
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+};
 
 }  // namespace
 
 struct lux_sgm_ctx {
     int device = 0;
-    uint64_t budget = 0;
     std::string err;
     std::mutex mu;
     hipStream_t stream = nullptr;
-    std::map<std::string, DevBuf> bufs;
-    std::map<std::tuple<int, int, int, int>, std::shared_ptr<lux::Plan>> plans;
+    lux::Budget budget;
+    lux::Workspace ws;                                   // single requests and layers
+    std::vector<std::unique_ptr<lux::Workspace>> bws;     // one per pair of a batch
+    hipStream_t side[kMaxBranches] = {};
+    hipEvent_t fork = nullptr, join[kMaxBranches] = {};
+    std::map<std::string, GraphEntry> graphs;             // batch descriptor -> instantiated graph
+    std::map<std::string, lux_sgm_schedule> tuned;        // batch shapes -> schedule
+    uint64_t gen = 0;                                     // bumped when a batch workspace is reallocated
     int opt_bands = 0;
-    // band-tile synchronisation: flags hold the epoch of the request that wrote them
-    int epoch = 0;
-    void* sync_zeroed = nullptr;
     // stage timing (lux_sgm_set_profiling): events recorded between the launches of one request
     bool profiling = false;
     hipEvent_t ev[16] = {};
     int nev = 0;
-
-    uint64_t held() const
-    {
-        uint64_t s = 0;
-        for (auto& kv : bufs) s += kv.second.n;
-        return s;
-    }
 };
 
 namespace {
@@ -59,202 +56,184 @@ int fail(lux_sgm_ctx* c, int code, const std::string& msg)
     return code;
 }
 
-int hip_fail(lux_sgm_ctx* c, hipError_t e, const char* where)
-{
-    std::string m = std::string(where) + ": " + hipGetErrorString(e);
-    return fail(c, e == hipErrorOutOfMemory ? LUX_SGM_ENOMEM : LUX_SGM_EDEVICE, m);
-}
+int hip_fail(lux_sgm_ctx* c, hipError_t e, const char* where) { return lux::hip_status(e, where, c ? &c->err : nullptr); }
 
-// Grow-only pooled device buffer (the ScratchPool pattern, Lux.Engine/Pipeline/ScratchPool.cs:6-15).
-int get_buf(lux_sgm_ctx* c, const char* name, size_t bytes, void** out)
-{
-    DevBuf& b = c->bufs[name];
-    if (b.n < bytes) {
-        uint64_t other = c->held() - b.n;
-        if (c->budget && other + bytes > c->budget)
-            return fail(c, LUX_SGM_ENOMEM, std::string("HBM budget exceeded allocating ") + name);
-        if (b.p) (void)hipFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
-        hipError_t e = hipMalloc(&b.p, bytes);
-        if (e != hipSuccess) {
-            b.p = nullptr;
-            return hip_fail(c, e, "hipMalloc");
-        }
-        b.n = bytes;
-    }
-    *out = b.p;
-    return 0;
-}
-
-int get_plan(lux_sgm_ctx* c, int W, int H, int T, int xsw, std::shared_ptr<lux::Plan>* out)
-{
-    auto key = std::make_tuple(W, H, T, xsw);
-    auto it = c->plans.find(key);
-    if (it == c->plans.end()) {
-        auto p = std::make_shared<lux::Plan>(lux::make_plan(W, H, T, xsw));
-        if (p->status != 0) return fail(c, p->status == -1 ? LUX_SGM_EINVAL : LUX_SGM_EUNSUPPORTED, p->error);
-        it = c->plans.emplace(key, p).first;
-    }
-    *out = it->second;
-    return 0;
-}
-
-struct Work {
-    lux::Geometry g;
-    lux::SgmBuffers b;
-    uint8_t* raw = nullptr;
-    uint16_t* agg = nullptr;
-};
-
-// Device buffers for one uniform-range layer, plus the plan's visit lists uploaded.
-int prepare(lux_sgm_ctx* c, const lux::Geometry& g, int T, int xsw, hipStream_t s, Work* w)
-{
-    std::shared_ptr<lux::Plan> plan;
-    int rc = get_plan(c, g.W, g.H, T, xsw, &plan);
-    if (rc) return rc;
-    const size_t cells = (size_t)g.W * g.H;
-    void *raw, *agg, *seeds, *seedmins, *cols, *colmins, *vis, *partial, *drb, *dr0, *bnd, *bndmin;
-    if ((rc = get_buf(c, "raw", cells * g.cs, &raw))) return rc;
-    if ((rc = get_buf(c, "agg", cells * g.cs * 2, &agg))) return rc;
-    if ((rc = get_buf(c, "seeds", (size_t)3 * g.W * g.cs * 2, &seeds))) return rc;
-    if ((rc = get_buf(c, "seedmins", (size_t)3 * g.W * 2, &seedmins))) return rc;
-    if ((rc = get_buf(c, "cols", (size_t)4 * g.H * g.cs * 2, &cols))) return rc;
-    if ((rc = get_buf(c, "colmins", (size_t)4 * g.H * 2, &colmins))) return rc;
-    const size_t nv = plan->plus.size() + plan->minus.size() + 4 * (size_t)g.H;
-    const size_t maxvis = std::max(plan->plus.size(), plan->minus.size()) / 4 + 1;
-    if ((rc = get_buf(c, "replay_partial", maxvis * g.cs * 2, &partial))) return rc;
-    if ((rc = get_buf(c, "wta_dr", cells * 2, &drb))) return rc;
-    if ((rc = get_buf(c, "band_boundary", (size_t)6 * g.W * g.cs * 2, &bnd))) return rc;
-    const int tiles = lux::band_tiles(g.W, g.H);
-    void* sync;
-    if ((rc = get_buf(c, "band_sync", (size_t)(4 + 2 * tiles) * sizeof(int), &sync))) return rc;
-    if (sync != c->sync_zeroed) {  // a new buffer: no flag may hold a live epoch
-        hipError_t ez = lux::launch_fill_u32((uint32_t*)sync, 4 + 2 * (size_t)tiles, 0u, s);
-        if (ez != hipSuccess) return hip_fail(c, ez, "zero band flags");
-        c->sync_zeroed = sync;
-    }
-    if (++c->epoch <= 0) c->epoch = 1;
-    if ((rc = get_buf(c, "band_boundary_min", (size_t)6 * g.W * 4, &bndmin))) return rc;
-    if ((rc = get_buf(c, "wta_dr0", (size_t)g.H * 4, &dr0))) return rc;
-    if ((rc = get_buf(c, "visits", (nv + 4) * sizeof(int), &vis))) return rc;
-    hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), plan->plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync((int*)vis + plan->plus.size(), plan->minus.data(), plan->minus.size() * sizeof(int),
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync((int*)vis + plan->plus.size() + plan->minus.size(), plan->cellvis_plus.data(),
-                           plan->cellvis_plus.size() * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync((int*)vis + plan->plus.size() + plan->minus.size() + 2 * (size_t)g.H,
-                           plan->cellvis_minus.data(), plan->cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return hip_fail(c, e, "upload plan");
-    w->g = g;
-    w->raw = (uint8_t*)raw;
-    w->agg = (uint16_t*)agg;
-    w->b.raw = w->raw;
-    w->b.raw_out = w->raw;
-    w->b.agg = w->agg;
-    w->b.dr = (uint16_t*)drb;
-    w->b.dr0 = (uint32_t*)dr0;
-    w->b.bnd = (uint16_t*)bnd;
-    w->b.bndmin = (uint32_t*)bndmin;
-    w->b.err = (int*)sync;
-    w->b.ticket = (int*)sync + 2;
-    w->b.flags = (int*)sync + 4;
-    w->b.max_tiles = tiles;
-    w->b.epoch = c->epoch;
-    w->b.seeds = (uint16_t*)seeds;
-    w->b.seedmins = (uint16_t*)seedmins;
-    w->b.cols = (uint16_t*)cols;
-    w->b.colmins = (uint16_t*)colmins;
-    w->b.partial = (uint16_t*)partial;
-    w->b.visits_plus = (const int*)vis;
-    w->b.n_plus = (int)plan->plus.size() / 4;
-    w->b.visits_minus = (const int*)vis + plan->plus.size();
-    w->b.n_minus = (int)plan->minus.size() / 4;
-    w->b.cellvis_plus = (const int*)vis + plan->plus.size() + plan->minus.size();
-    w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
-    w->b.seed_row = plan->seed_row;
-    return 0;
-}
-
-// After the stream has drained: a band tile that gave up waiting leaves the error word set.
-int check_band_error(lux_sgm_ctx* c, hipStream_t s)
-{
-    auto it = c->bufs.find("band_sync");
-    if (it == c->bufs.end() || !it->second.p) return 0;
-    int err = 0;
-    hipError_t e = hipMemcpyAsync(&err, it->second.p, sizeof(int), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(c, e, "read band error word");
-    if (err) {
-        int zero = 0;
-        (void)hipMemcpyAsync(it->second.p, &zero, sizeof(int), hipMemcpyHostToDevice, s);
-        (void)hipStreamSynchronize(s);
-        return fail(c, LUX_SGM_EDEVICE, "a band tile timed out waiting for its neighbours");
-    }
-    return 0;
-}
-
-int check_rect(lux_sgm_ctx* c, const lux_sgm_rect_params* p)
-{
-    if (!p) return fail(c, LUX_SGM_EINVAL, "null params");
-    if (p->width <= 0 || p->height <= 0 || p->stride < p->width) return fail(c, LUX_SGM_EINVAL, "bad image size/stride");
-    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240");
-    if (!(p->bits == 8 || (p->bits >= 9 && p->bits <= 16))) return fail(c, LUX_SGM_EINVAL, "bits must be 8 or 9..16");
-    if (p->cost != LUX_SGM_COST_CENSUS && p->cost != LUX_SGM_COST_AD) return fail(c, LUX_SGM_EINVAL, "unknown cost");
-    if (p->cost == LUX_SGM_COST_CENSUS && (p->census_radius < 1 || p->census_radius > 3))
-        return fail(c, LUX_SGM_EINVAL, "census radius must be 1..3");
-    if (p->cost == LUX_SGM_COST_AD && (p->ad_cap < 0 || p->ad_cap > 28)) return fail(c, LUX_SGM_EINVAL, "ad_cap must be 0..28");
-    if (p->disparities <= 0 || p->align <= 0) return fail(c, LUX_SGM_EINVAL, "disparities/align must be positive");
-    if (((p->disparities + p->align - 1) / p->align) * p->align > 1024)
-        return fail(c, LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
-    if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535) return fail(c, LUX_SGM_EINVAL, "penalties must be 0..65535");
-    if (p->tile < 2) return fail(c, LUX_SGM_EINVAL, "tile must be >= 2");
-    if (p->lr_tolerance < 0) return fail(c, LUX_SGM_EINVAL, "negative L/R tolerance");
-    return 0;
-}
-
-int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, const void* dr, uint16_t* disp, float* sub,
-                uint8_t* valid, uint16_t* agg_out, hipStream_t s)
+int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, const void* dr, const lux::RectOut& o,
+                hipStream_t s)
 {
     lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
-    Work w;
-    int rc = prepare(c, g, p->tile, p->xs_clamp_w, s, &w);
+    lux::Prepared w;
+    int rc = lux::prepare(c->ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, s, &w, &c->err);
     if (rc) return rc;
-    const bool img16 = p->bits > 8;
-    hipError_t e;
-    if (c->profiling) (void)hipEventRecord(c->ev[0], s);
-    if (p->cost == LUX_SGM_COST_CENSUS) {  // the codes here; the cost volume inside the first sweep
-        void *cl, *cr;
-        const size_t n = (size_t)g.W * g.H * 8;
-        if ((rc = get_buf(c, "census_l", n, &cl)) || (rc = get_buf(c, "census_r", n, &cr))) return rc;
-        e = lux::launch_census(dl, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)cl, s);
-        if (e == hipSuccess) e = lux::launch_census(dr, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)cr, s);
-        w.b.cl = (const uint64_t*)cl;
-        w.b.cr = (const uint64_t*)cr;
-    } else {
-        e = lux::launch_cost_ad(dl, dr, img16, p->bits, p->stride, p->ad_cap, g, w.raw, s);
+    lux::EngineOpts opt;
+    opt.bands = c->opt_bands;
+    rc = lux::enqueue_rect(c->ws, w, p, dl, dr, o, opt, s, c->profiling ? c->ev : nullptr, &c->err);
+    if (!rc && c->profiling) c->nev = 2 + lux::kSgmMarks;
+    return rc;
+}
+
+void destroy_graphs(lux_sgm_ctx* c)
+{
+    for (auto& kv : c->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
-    if (e != hipSuccess) return hip_fail(c, e, "cost volume");
-    if (c->profiling) (void)hipEventRecord(c->ev[1], s);
-    lux::SgmOutputs o;
-    void* td = disp;
-    if (!disp && (rc = get_buf(c, "disp_tmp", (size_t)g.W * g.H * 2, &td))) return rc;
-    o.disp = (uint16_t*)td;
-    o.sub = sub;
-    o.valid = valid;
-    o.tol = p->lr_tolerance;
-    o.write_agg = agg_out != nullptr;
-    o.bands = c->opt_bands;
-    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, nullptr, s, c->profiling ? c->ev + 2 : nullptr);
-    if (e != hipSuccess) return hip_fail(c, e, "aggregation");
-    if (c->profiling) c->nev = 2 + lux::kSgmMarks;
-    if (agg_out) {
-        e = lux::launch_restride_u16(w.agg, g.cs, agg_out, g.cap, g.W * g.H, g.cap, s);
-        if (e != hipSuccess) return hip_fail(c, e, "agg copy");
+    c->graphs.clear();
+}
+
+std::string shape_key(int n, const lux_sgm_pair* pairs)
+{
+    std::string k((const char*)&n, sizeof n);
+    for (int i = 0; i < n; i++) k.append((const char*)&pairs[i].params, sizeof(lux_sgm_rect_params));
+    return k;
+}
+
+lux_sgm_schedule default_schedule(int n)
+{
+    lux_sgm_schedule s;
+    s.bands = 0;
+    s.branches = std::min(n, 4);
+    s.ms = 0.f;
+    return s;
+}
+
+// Longest pair first onto the least loaded branch.
+std::vector<int> assign_branches(int n, const lux_sgm_pair* pairs, int branches)
+{
+    std::vector<int> order(n), br(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto cells = [&](int i) {
+        const lux_sgm_rect_params& p = pairs[i].params;
+        return (double)p.width * p.height * p.disparities;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<double> load(branches, 0.0);
+    for (int i : order) {
+        int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        br[i] = k;
+        load[k] += cells(i);
     }
+    return br;
+}
+
+int ensure_batch_streams(lux_sgm_ctx* c)
+{
+    if (c->fork) return 0;
+    hipError_t e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    for (int k = 0; e == hipSuccess && k < kMaxBranches; k++) {
+        e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
+    }
+    return e == hipSuccess ? 0 : hip_fail(c, e, "batch streams");
+}
+
+// Prepare every pair's workspace (allocation and plan upload happen here, outside any capture).
+int prepare_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<lux::Prepared>* prep)
+{
+    while ((int)c->bws.size() < n) {
+        c->bws.emplace_back(new lux::Workspace());
+        c->bws.back()->device = c->device;
+        c->bws.back()->budget = &c->budget;
+    }
+    prep->assign(n, lux::Prepared());
+    for (int i = 0; i < n; i++) {
+        const lux_sgm_rect_params* p = &pairs[i].params;
+        int rc = lux::check_rect(p, &c->err);
+        if (rc) return rc;
+        if (!pairs[i].left || !pairs[i].right) return fail(c, LUX_SGM_EINVAL, "null image in batch");
+        lux::Workspace& ws = *c->bws[i];
+        std::map<std::string, void*> before;
+        for (auto& kv : ws.bufs) before[kv.first] = kv.second.p;
+        lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
+        rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, c->stream, &(*prep)[i], &c->err);
+        if (rc) return rc;
+        for (auto& kv : ws.bufs) {
+            auto it = before.find(kv.first);
+            if (it == before.end() || it->second != kv.second.p) {
+                c->gen++;
+                break;
+            }
+        }
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "prepare batch");
+}
+
+// Capture the batch into a graph: fork from the capture stream onto the branches, one request per pair.
+int capture_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<lux::Prepared>& prep,
+                  const lux_sgm_schedule& sch, GraphEntry* out)
+{
+    const int nb = std::max(1, std::min({sch.branches, n, kMaxBranches}));
+    std::vector<int> br = assign_branches(n, pairs, nb);
+    lux::EngineOpts opt;
+    opt.bands = sch.bands;
+    opt.replayable = true;
+    hipStream_t cs = c->stream;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamBeginCapture");
+    int rc = 0;
+    e = hipEventRecord(c->fork, cs);
+    for (int k = 0; e == hipSuccess && k < nb; k++) e = hipStreamWaitEvent(c->side[k], c->fork, 0);
+    for (int i = 0; e == hipSuccess && !rc && i < n; i++) {
+        lux::RectOut o;
+        o.disp = pairs[i].disp;
+        o.sub = pairs[i].sub;
+        o.valid = pairs[i].valid;
+        rc = lux::enqueue_rect(*c->bws[i], prep[i], &pairs[i].params, pairs[i].left, pairs[i].right, o, opt, c->side[br[i]],
+                               nullptr, &c->err);
+    }
+    for (int k = 0; e == hipSuccess && k < nb; k++) {
+        e = hipEventRecord(c->join[k], c->side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cs, c->join[k], 0);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ee = hipStreamEndCapture(cs, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e == hipSuccess) e = ee;
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(c, e, "capture batch");
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    out->graph = graph;
+    out->exec = exec;
+    out->gen = c->gen;
+    return 0;
+}
+
+// The instantiated graph of this batch under schedule sch (built on first use, cached by descriptor).
+int batch_graph(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, const lux_sgm_schedule& sch, hipGraphExec_t* exec)
+{
+    int rc = ensure_batch_streams(c);
+    if (rc) return rc;
+    std::vector<lux::Prepared> prep;
+    if ((rc = prepare_batch(c, n, pairs, &prep))) return rc;
+    std::string key((const char*)&sch.bands, sizeof sch.bands);
+    key.append((const char*)&sch.branches, sizeof sch.branches);
+    key.append((const char*)&n, sizeof n);
+    key.append((const char*)pairs, sizeof(lux_sgm_pair) * (size_t)n);
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end() && it->second.gen != c->gen) {  // a workspace moved since: rebuild
+        (void)hipGraphExecDestroy(it->second.exec);
+        (void)hipGraphDestroy(it->second.graph);
+        c->graphs.erase(it);
+        it = c->graphs.end();
+    }
+    if (it == c->graphs.end()) {
+        if (c->graphs.size() >= 16) destroy_graphs(c);
+        GraphEntry ge;
+        if ((rc = capture_batch(c, n, pairs, prep, sch, &ge))) return rc;
+        it = c->graphs.emplace(key, ge).first;
+    }
+    *exec = it->second.exec;
     return 0;
 }
 
@@ -262,7 +241,7 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
 
 extern "C" {
 
-int lux_sgm_version(void) { return 100; }
+int lux_sgm_version(void) { return 200; }
 
 int lux_sgm_create(int device, uint64_t hbm_budget, lux_sgm_ctx** out)
 {
@@ -293,7 +272,9 @@ int lux_sgm_create(int device, uint64_t hbm_budget, lux_sgm_ctx** out)
     }
     auto* c = new lux_sgm_ctx();
     c->device = device;
-    c->budget = hbm_budget;
+    c->budget.limit = hbm_budget;
+    c->ws.device = device;
+    c->ws.budget = &c->budget;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -309,10 +290,18 @@ int lux_sgm_destroy(lux_sgm_ctx* c)
     if (!c) return LUX_SGM_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& kv : c->bufs)
-        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto s : c->side)
+        if (s) (void)hipStreamSynchronize(s);
+    destroy_graphs(c);
+    c->ws.release();
+    c->bws.clear();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < kMaxBranches; k++) {
+        if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LUX_SGM_OK;
@@ -323,14 +312,9 @@ const char* lux_sgm_last_error(const lux_sgm_ctx* c) { return c ? c->err.c_str()
 int lux_sgm_rect_bytes(const lux_sgm_rect_params* p, uint64_t* bytes)
 {
     if (!p || !bytes || p->width <= 0 || p->height <= 0 || p->disparities <= 0 || p->align <= 0) return LUX_SGM_EINVAL;
-    lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
-    const uint64_t cells = (uint64_t)g.W * g.H;
-    uint64_t b = cells * g.cs * 3                      // raw u8 + agg u16
-                 + cells * 16                          // census codes
-                 + (uint64_t)3 * g.W * g.cs * 2 + (uint64_t)4 * g.H * g.cs * 2  // seeds, replay columns
-                 + cells * 2 * (p->bits > 8 ? 2 : 1)   // images
-                 + cells * 7;                          // outputs
-    *bytes = b;
+    const uint64_t px = (uint64_t)p->width * p->height;
+    *bytes = lux::rect_workspace_bytes(p) + px * 2 * (p->bits > 8 ? 2 : 1)  // images
+             + px * 7;                                                     // outputs
     return LUX_SGM_OK;
 }
 
@@ -339,12 +323,17 @@ int lux_sgm_rectified_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const
 {
     if (!c) return LUX_SGM_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    int rc = check_rect(c, p);
+    int rc = lux::check_rect(p, &c->err);
     if (rc) return rc;
     if (!d_left || !d_right) return fail(c, LUX_SGM_EINVAL, "null image");
     (void)hipSetDevice(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    return rect_device(c, p, d_left, d_right, d_disp, d_sub, d_valid, d_agg, s);
+    lux::RectOut o;
+    o.disp = d_disp;
+    o.sub = d_sub;
+    o.valid = d_valid;
+    o.agg = d_agg;
+    return rect_device(c, p, d_left, d_right, o, s);
 }
 
 int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* left, const void* right, uint16_t* disp,
@@ -352,7 +341,7 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
 {
     if (!c) return LUX_SGM_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    int rc = check_rect(c, p);
+    int rc = lux::check_rect(p, &c->err);
     if (rc) return rc;
     if (!left || !right) return fail(c, LUX_SGM_EINVAL, "null image");
     (void)hipSetDevice(c->device);
@@ -362,16 +351,21 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
     const size_t imgb = (size_t)p->height * p->stride * esz;
     lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
     void *dl, *dr, *dd, *ds, *dv, *da = nullptr;
-    if ((rc = get_buf(c, "img_l", imgb, &dl)) || (rc = get_buf(c, "img_r", imgb, &dr))) return rc;
-    if ((rc = get_buf(c, "out_disp", px * 2, &dd)) || (rc = get_buf(c, "out_sub", px * 4, &ds)) ||
-        (rc = get_buf(c, "out_valid", px, &dv)))
+    lux::Workspace& ws = c->ws;
+    if ((rc = ws.get("img_l", imgb, &dl, &c->err)) || (rc = ws.get("img_r", imgb, &dr, &c->err))) return rc;
+    if ((rc = ws.get("out_disp", px * 2, &dd, &c->err)) || (rc = ws.get("out_sub", px * 4, &ds, &c->err)) ||
+        (rc = ws.get("out_valid", px, &dv, &c->err)))
         return rc;
-    if (agg && (rc = get_buf(c, "out_agg", px * g.cap * 2, &da))) return rc;
+    if (agg && (rc = ws.get("out_agg", px * g.cap * 2, &da, &c->err))) return rc;
     hipError_t e = hipMemcpyAsync(dl, left, imgb, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, right, imgb, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(c, e, "upload images");
-    rc = rect_device(c, p, dl, dr, disp ? (uint16_t*)dd : nullptr, sub ? (float*)ds : nullptr,
-                     valid ? (uint8_t*)dv : nullptr, agg ? (uint16_t*)da : nullptr, s);
+    lux::RectOut o;
+    o.disp = disp ? (uint16_t*)dd : nullptr;
+    o.sub = sub ? (float*)ds : nullptr;
+    o.valid = valid ? (uint8_t*)dv : nullptr;
+    o.agg = agg ? (uint16_t*)da : nullptr;
+    rc = rect_device(c, p, dl, dr, o, s);
     if (rc) return rc;
     if (disp) e = hipMemcpyAsync(disp, dd, px * 2, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && sub) e = hipMemcpyAsync(sub, ds, px * 4, hipMemcpyDeviceToHost, s);
@@ -379,7 +373,7 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
     if (e == hipSuccess && agg) e = hipMemcpyAsync(agg, da, px * g.cap * 2, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "download results");
-    return check_band_error(c, s);
+    return lux::check_band_error(ws, s, &c->err);
 }
 
 int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
@@ -387,24 +381,20 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
 {
     if (!c) return LUX_SGM_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!p || !raw) return fail(c, LUX_SGM_EINVAL, "null argument");
-    if (p->width <= 0 || p->height <= 0 || p->planes <= 0 || p->align <= 0 || p->tile < 2)
-        return fail(c, LUX_SGM_EINVAL, "bad layer parameters");
-    if (p->width > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240");
-    if (((p->planes + p->align - 1) / p->align) * p->align > 1024)
-        return fail(c, LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
-    if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535) return fail(c, LUX_SGM_EINVAL, "penalties must be 0..65535");
+    if (!raw) return fail(c, LUX_SGM_EINVAL, "null argument");
+    int rc = lux::check_layer(p, &c->err);
+    if (rc) return rc;
     (void)hipSetDevice(c->device);
     hipStream_t s = c->stream;
     lux::Geometry g = lux::geometry(p->width, p->height, p->planes, p->align);
-    Work w;
-    int rc = prepare(c, g, p->tile, p->xs_clamp_w, s, &w);
-    if (rc) return rc;
+    lux::Workspace& ws = c->ws;
+    lux::Prepared w;
+    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err))) return rc;
     const size_t cells = (size_t)g.W * g.H;
     void *stage, *a1 = nullptr, *dw = nullptr;
-    if ((rc = get_buf(c, "layer_stage", cells * g.cap * 2, &stage))) return rc;
-    if (agg_pass1 && (rc = get_buf(c, "layer_pass1", cells * g.cs * 2, &a1))) return rc;
-    if ((rc = get_buf(c, "layer_wta", cells * 2, &dw))) return rc;
+    if ((rc = ws.get("layer_stage", cells * g.cap * 2, &stage, &c->err))) return rc;
+    if (agg_pass1 && (rc = ws.get("layer_pass1", cells * g.cs * 2, &a1, &c->err))) return rc;
+    if ((rc = ws.get("layer_wta", cells * 2, &dw, &c->err))) return rc;
     hipError_t e = hipMemcpyAsync(stage, raw, cells * g.cap, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = lux::launch_restride_u8((const uint8_t*)stage, g.cap, w.raw, g.cs, (int)cells, g.cap, 255, s);
     if (e != hipSuccess) return hip_fail(c, e, "upload raw");
@@ -430,7 +420,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
-    return check_band_error(c, s);
+    return lux::check_band_error(ws, s, &c->err);
 }
 
 int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)
@@ -471,6 +461,81 @@ int lux_sgm_stage_times(lux_sgm_ctx* c, float* ms, int32_t cap)
         if (e != hipSuccess) return hip_fail(c, e, "hipEventElapsedTime");
     }
     return c->nev - 1;
+}
+
+int lux_sgm_batch(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, void* stream)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n <= 0 || !pairs) return fail(c, LUX_SGM_EINVAL, "empty batch");
+    (void)hipSetDevice(c->device);
+    auto t = c->tuned.find(shape_key(n, pairs));
+    const lux_sgm_schedule sch = t != c->tuned.end() ? t->second : default_schedule(n);
+    hipGraphExec_t exec;
+    int rc = batch_graph(c, n, pairs, sch, &exec);
+    if (rc) return rc;
+    hipError_t e = hipGraphLaunch(exec, stream ? (hipStream_t)stream : c->stream);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "hipGraphLaunch");
+}
+
+int lux_sgm_batch_tune(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, void* stream, int32_t reps,
+                       lux_sgm_schedule* best, lux_sgm_schedule* tried, int32_t cap)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n <= 0 || !pairs) return fail(c, LUX_SGM_EINVAL, "empty batch");
+    if (reps < 1) reps = 1;
+    (void)hipSetDevice(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    bool bands_ok = true;  // the fused bands take up to 256 planes
+    for (int i = 0; i < n; i++)
+        if (lux::geometry(pairs[i].params.width, pairs[i].params.height, pairs[i].params.disparities,
+                          pairs[i].params.align).dpl > 4)
+            bands_ok = false;
+    std::vector<lux_sgm_schedule> cand;
+    for (int bands = 0; bands <= (bands_ok ? 1 : 0); bands++)
+        for (int b : {1, 2, 3, 4, 6, 8}) {
+            if (b > n) continue;
+            lux_sgm_schedule sc;
+            sc.bands = bands;
+            sc.branches = b;
+            sc.ms = 0.f;
+            cand.push_back(sc);
+        }
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return hip_fail(c, e, "hipEventCreate");
+    int rc = 0, ntried = 0;
+    lux_sgm_schedule win = cand[0];
+    win.ms = -1.f;
+    for (auto& sc : cand) {
+        hipGraphExec_t exec;
+        if ((rc = batch_graph(c, n, pairs, sc, &exec))) break;
+        e = hipGraphLaunch(exec, s);  // warm
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(exec, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "tune batch");
+            break;
+        }
+        for (int i = 0; i < n && !rc; i++) rc = lux::check_band_error(*c->bws[i], s, &c->err);
+        if (rc) break;
+        sc.ms = ms / reps;
+        if (tried && ntried < cap) tried[ntried] = sc;
+        ntried++;
+        if (win.ms < 0.f || sc.ms < win.ms) win = sc;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->tuned[shape_key(n, pairs)] = win;
+    if (best) *best = win;
+    return ntried;
 }
 
 int lux_sgm_plan_info(int32_t width, int32_t height, int32_t tile, int32_t xs_clamp_w, int32_t* visits_plus,

@@ -1,0 +1,109 @@
+// Request engine shared by the entry points of liblux_sgm.so: pooled device workspaces under an HBM budget,
+// the schedule-plan cache, and the enqueue of one rectified request or one layer onto a stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lux_sgm.h"
+#include "kernels.h"
+#include "plan.h"
+
+namespace lux {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+// Device bytes held by the workspaces of one context or server against its HBM budget (0 = unbounded):
+// the device-side form of the reference's memory admission (DenseStereoSgm.cs:142-156, :330,339-344).
+struct Budget {
+    uint64_t limit = 0;
+    uint64_t used = 0;
+    uint64_t peak = 0;
+    std::mutex mu;
+    // Account for growing an allocation from `old_n` to `new_n` bytes; false if it would exceed the limit.
+    bool grow(uint64_t old_n, uint64_t new_n);
+    void release(uint64_t n);
+};
+
+// The device buffers of one request in flight. Buffers are grow-only and reused by the next request with the
+// same or a smaller shape (the ScratchPool pattern, Lux.Engine/Pipeline/ScratchPool.cs:6-15).
+struct Workspace {
+    int device = 0;
+    Budget* budget = nullptr;
+    std::map<std::string, DevBuf> bufs;
+    const Plan* plan = nullptr;      // whose visit lists are in "visits"
+    void* sync_zeroed = nullptr;     // band flag array known to hold no live epoch
+    int epoch = 0;
+
+    Workspace() = default;
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() { release(); }
+    int get(const char* name, size_t bytes, void** out, std::string* err);
+    uint64_t held() const;
+    void release();
+};
+
+// One request prepared on a workspace: geometry and the kernel-side buffer table.
+struct Prepared {
+    Geometry g;
+    SgmBuffers b;
+    uint8_t* raw = nullptr;
+    uint16_t* agg = nullptr;
+};
+
+struct RectOut {
+    uint16_t* disp = nullptr;
+    float* sub = nullptr;
+    uint8_t* valid = nullptr;
+    uint16_t* agg = nullptr;  // [H*W*cap]
+};
+
+struct EngineOpts {
+    int bands = 0;
+    // The launches may be captured into a graph and replayed: band flags are cleared by a kernel before each
+    // band launch instead of relying on a fresh epoch per request.
+    bool replayable = false;
+};
+
+// The named device buffers of one request: prepare() allocates exactly these, and the server admits a request
+// by the growth they need in a slot's workspace. host_img_bytes > 0 adds the staging of host images and outputs.
+struct BufferNeed {
+    const char* name;
+    size_t bytes;
+};
+std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes = 0);
+// Bytes by which ws must grow to hold `need` (0: it already does).
+uint64_t growth(const Workspace& ws, const std::vector<BufferNeed>& need);
+
+int check_rect(const lux_sgm_rect_params* p, std::string* err);
+int check_layer(const lux_sgm_layer_params* p, std::string* err);
+// Device bytes one rectified request holds in a workspace (inputs and outputs excluded; 0 if its schedule is
+// unsupported).
+uint64_t rect_workspace_bytes(const lux_sgm_rect_params* p);
+
+// Plans are pure functions of (W, H, T, xsw): one process-wide cache.
+int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::string* err);
+
+// Allocate the workspace for geometry g and upload the plan's visit lists (copies from host memory on s).
+// Nothing here may be captured into a graph; enqueue_* below may.
+int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err);
+
+// Enqueue one rectified request on s: cost volume, the eight paths, WTA, subpixel, L/R. Kernel launches and
+// device-to-device copies only. ev (if not null) receives 2 + kSgmMarks events for the stage times.
+int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const void* dl, const void* dr, const RectOut& o,
+                 const EngineOpts& opt, hipStream_t s, hipEvent_t* ev, std::string* err);
+
+// After s has drained: a band tile that gave up waiting for its neighbours leaves an error word set.
+int check_band_error(Workspace& ws, hipStream_t s, std::string* err);
+
+int hip_status(hipError_t e, const char* where, std::string* err);
+
+}  // namespace lux

@@ -1,0 +1,309 @@
+// Request engine: workspaces, HBM budget, plan cache, and the enqueue of one request (declared in engine.h).
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <tuple>
+
+namespace lux {
+
+bool Budget::grow(uint64_t old_n, uint64_t new_n)
+{
+    std::lock_guard<std::mutex> lk(mu);
+    if (limit && used - old_n + new_n > limit) return false;
+    used = used - old_n + new_n;
+    peak = std::max(peak, used);
+    return true;
+}
+
+void Budget::release(uint64_t n)
+{
+    std::lock_guard<std::mutex> lk(mu);
+    used -= std::min(used, n);
+}
+
+int hip_status(hipError_t e, const char* where, std::string* err)
+{
+    if (err) *err = std::string(where) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? LUX_SGM_ENOMEM : LUX_SGM_EDEVICE;
+}
+
+static int fail(int code, const std::string& msg, std::string* err)
+{
+    if (err) *err = msg;
+    return code;
+}
+
+int Workspace::get(const char* name, size_t bytes, void** out, std::string* err)
+{
+    DevBuf& b = bufs[name];
+    if (b.n < bytes) {
+        if (budget && !budget->grow(b.n, bytes))
+            return fail(LUX_SGM_ENOMEM, std::string("HBM budget exceeded allocating ") + name, err);
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            if (budget) budget->release(bytes);
+            return hip_status(e, "hipMalloc", err);
+        }
+        b.n = bytes;
+        if (name == std::string("visits")) plan = nullptr;
+    }
+    *out = b.p;
+    return 0;
+}
+
+uint64_t Workspace::held() const
+{
+    uint64_t s = 0;
+    for (auto& kv : bufs) s += kv.second.n;
+    return s;
+}
+
+void Workspace::release()
+{
+    if (!bufs.empty()) (void)hipSetDevice(device);
+    for (auto& kv : bufs) {
+        if (kv.second.p) (void)hipFree(kv.second.p);
+        if (budget) budget->release(kv.second.n);
+    }
+    bufs.clear();
+    plan = nullptr;
+    sync_zeroed = nullptr;
+}
+
+int check_rect(const lux_sgm_rect_params* p, std::string* err)
+{
+    if (!p) return fail(LUX_SGM_EINVAL, "null params", err);
+    if (p->width <= 0 || p->height <= 0 || p->stride < p->width) return fail(LUX_SGM_EINVAL, "bad image size/stride", err);
+    if (p->width > 10240) return fail(LUX_SGM_EUNSUPPORTED, "width above 10240", err);
+    if (!(p->bits == 8 || (p->bits >= 9 && p->bits <= 16))) return fail(LUX_SGM_EINVAL, "bits must be 8 or 9..16", err);
+    if (p->cost != LUX_SGM_COST_CENSUS && p->cost != LUX_SGM_COST_AD) return fail(LUX_SGM_EINVAL, "unknown cost", err);
+    if (p->cost == LUX_SGM_COST_CENSUS && (p->census_radius < 1 || p->census_radius > 3))
+        return fail(LUX_SGM_EINVAL, "census radius must be 1..3", err);
+    if (p->cost == LUX_SGM_COST_AD && (p->ad_cap < 0 || p->ad_cap > 28))
+        return fail(LUX_SGM_EINVAL, "ad_cap must be 0..28", err);
+    if (p->disparities <= 0 || p->align <= 0) return fail(LUX_SGM_EINVAL, "disparities/align must be positive", err);
+    if (((p->disparities + p->align - 1) / p->align) * p->align > 1024)
+        return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
+    if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535)
+        return fail(LUX_SGM_EINVAL, "penalties must be 0..65535", err);
+    if (p->tile < 2) return fail(LUX_SGM_EINVAL, "tile must be >= 2", err);
+    if (p->lr_tolerance < 0) return fail(LUX_SGM_EINVAL, "negative L/R tolerance", err);
+    return 0;
+}
+
+int check_layer(const lux_sgm_layer_params* p, std::string* err)
+{
+    if (!p) return fail(LUX_SGM_EINVAL, "null params", err);
+    if (p->width <= 0 || p->height <= 0 || p->planes <= 0 || p->align <= 0 || p->tile < 2)
+        return fail(LUX_SGM_EINVAL, "bad layer parameters", err);
+    if (p->width > 10240) return fail(LUX_SGM_EUNSUPPORTED, "width above 10240", err);
+    if (((p->planes + p->align - 1) / p->align) * p->align > 1024)
+        return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
+    if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535)
+        return fail(LUX_SGM_EINVAL, "penalties must be 0..65535", err);
+    return 0;
+}
+
+std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes)
+{
+    const size_t cells = (size_t)g.W * g.H;
+    const size_t maxvis = std::max(plan.plus.size(), plan.minus.size()) / 4 + 1;
+    const size_t nv = plan.plus.size() + plan.minus.size() + 4 * (size_t)g.H;
+    std::vector<BufferNeed> v = {
+        {"raw", cells * g.cs},
+        {"agg", cells * g.cs * 2},
+        {"seeds", (size_t)3 * g.W * g.cs * 2},
+        {"seedmins", (size_t)3 * g.W * 2},
+        {"cols", (size_t)4 * g.H * g.cs * 2},
+        {"colmins", (size_t)4 * g.H * 2},
+        {"replay_partial", maxvis * g.cs * 2},
+        {"wta_dr", cells * 2},
+        {"wta_dr0", (size_t)g.H * 4},
+        {"band_boundary", (size_t)6 * g.W * g.cs * 2},
+        {"band_boundary_min", (size_t)6 * g.W * 4},
+        {"band_sync", (size_t)(4 + 2 * band_tiles(g.W, g.H)) * sizeof(int)},
+        {"visits", (nv + 4) * sizeof(int)},
+        {"disp_tmp", cells * 2},
+    };
+    if (census) {
+        v.push_back({"census_l", cells * 8});
+        v.push_back({"census_r", cells * 8});
+    }
+    if (host_img_bytes) {
+        v.push_back({"img_l", host_img_bytes});
+        v.push_back({"img_r", host_img_bytes});
+        v.push_back({"out_disp", cells * 2});
+        v.push_back({"out_sub", cells * 4});
+        v.push_back({"out_valid", cells});
+    }
+    return v;
+}
+
+uint64_t growth(const Workspace& ws, const std::vector<BufferNeed>& need)
+{
+    uint64_t g = 0;
+    for (const BufferNeed& b : need) {
+        auto it = ws.bufs.find(b.name);
+        const size_t have = it == ws.bufs.end() ? 0 : it->second.n;
+        if (b.bytes > have) g += b.bytes - have;
+    }
+    return g;
+}
+
+uint64_t rect_workspace_bytes(const lux_sgm_rect_params* p)
+{
+    Geometry g = geometry(p->width, p->height, p->disparities, p->align);
+    std::shared_ptr<Plan> plan;
+    if (plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, nullptr)) return 0;
+    uint64_t s = 0;
+    for (const BufferNeed& b : request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS)) s += b.bytes;
+    return s;
+}
+
+int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::string* err)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int>, std::shared_ptr<Plan>> cache;
+    auto key = std::make_tuple(W, H, T, xsw);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            *out = it->second;
+            return 0;
+        }
+    }
+    auto p = std::make_shared<Plan>(make_plan(W, H, T, xsw));
+    if (p->status != 0) return fail(p->status == -1 ? LUX_SGM_EINVAL : LUX_SGM_EUNSUPPORTED, p->error, err);
+    std::lock_guard<std::mutex> lk(mu);
+    *out = cache.emplace(key, p).first->second;
+    return 0;
+}
+
+int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err)
+{
+    std::shared_ptr<Plan> plan;
+    int rc = plan_for(g.W, g.H, T, xsw, &plan, err);
+    if (rc) return rc;
+    std::map<std::string, void*> ptr;
+    for (const BufferNeed& nb : request_buffers(g, *plan, census)) {
+        void* q;
+        if ((rc = ws.get(nb.name, nb.bytes, &q, err))) return rc;
+        ptr[nb.name] = q;
+    }
+    void *raw = ptr["raw"], *agg = ptr["agg"], *seeds = ptr["seeds"], *seedmins = ptr["seedmins"], *cols = ptr["cols"],
+         *colmins = ptr["colmins"], *vis = ptr["visits"], *partial = ptr["replay_partial"], *drb = ptr["wta_dr"],
+         *dr0 = ptr["wta_dr0"], *bnd = ptr["band_boundary"], *bndmin = ptr["band_boundary_min"], *sync = ptr["band_sync"];
+    const int tiles = band_tiles(g.W, g.H);
+    if (sync != ws.sync_zeroed) {  // a new buffer: no flag may hold a live epoch
+        hipError_t ez = launch_fill_u32((uint32_t*)sync, 4 + 2 * (size_t)tiles, 0u, s);
+        if (ez != hipSuccess) return hip_status(ez, "zero band flags", err);
+        ws.sync_zeroed = sync;
+    }
+    if (++ws.epoch <= 0) ws.epoch = 1;
+    if (ws.plan != plan.get()) {  // upload the visit lists once per plan
+        const size_t np = plan->plus.size(), nm = plan->minus.size();
+        hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), np * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync((int*)vis + np, plan->minus.data(), nm * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync((int*)vis + np + nm, plan->cellvis_plus.data(), plan->cellvis_plus.size() * sizeof(int),
+                               hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync((int*)vis + np + nm + 2 * (size_t)g.H, plan->cellvis_minus.data(),
+                               plan->cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_status(e, "upload plan", err);
+        ws.plan = plan.get();
+    }
+    if (census) {
+        w->b.cl = (const uint64_t*)ptr["census_l"];
+        w->b.cr = (const uint64_t*)ptr["census_r"];
+    }
+    void* dtmp = ptr["disp_tmp"];
+    w->g = g;
+    w->raw = (uint8_t*)raw;
+    w->agg = (uint16_t*)agg;
+    w->b.raw = w->raw;
+    w->b.raw_out = w->raw;
+    w->b.agg = w->agg;
+    w->b.dr = (uint16_t*)drb;
+    w->b.dr0 = (uint32_t*)dr0;
+    w->b.bnd = (uint16_t*)bnd;
+    w->b.bndmin = (uint32_t*)bndmin;
+    w->b.err = (int*)sync;
+    w->b.ticket = (int*)sync + 2;
+    w->b.flags = (int*)sync + 4;
+    w->b.max_tiles = tiles;
+    w->b.epoch = ws.epoch;
+    w->b.seeds = (uint16_t*)seeds;
+    w->b.seedmins = (uint16_t*)seedmins;
+    w->b.cols = (uint16_t*)cols;
+    w->b.colmins = (uint16_t*)colmins;
+    w->b.partial = (uint16_t*)partial;
+    w->b.visits_plus = (const int*)vis;
+    w->b.n_plus = (int)plan->plus.size() / 4;
+    w->b.visits_minus = (const int*)vis + plan->plus.size();
+    w->b.n_minus = (int)plan->minus.size() / 4;
+    w->b.cellvis_plus = (const int*)vis + plan->plus.size() + plan->minus.size();
+    w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
+    w->b.seed_row = plan->seed_row;
+    w->b.disp_tmp = (uint16_t*)dtmp;
+    return 0;
+}
+
+int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const void* dl, const void* dr, const RectOut& o,
+                 const EngineOpts& opt, hipStream_t s, hipEvent_t* ev, std::string* err)
+{
+    (void)ws;
+    const Geometry& g = w.g;
+    const bool img16 = p->bits > 8;
+    hipError_t e;
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (p->cost == LUX_SGM_COST_CENSUS) {  // the codes here; the cost volume inside the first sweep
+        e = launch_census(dl, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)w.b.cl, s);
+        if (e == hipSuccess) e = launch_census(dr, img16, g.W, g.H, p->stride, p->census_radius, (uint64_t*)w.b.cr, s);
+    } else {
+        w.b.cl = w.b.cr = nullptr;
+        e = launch_cost_ad(dl, dr, img16, p->bits, p->stride, p->ad_cap, g, w.raw, s);
+    }
+    if (e != hipSuccess) return hip_status(e, "cost volume", err);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    SgmOutputs so;
+    so.disp = o.disp ? o.disp : w.b.disp_tmp;
+    so.sub = o.sub;
+    so.valid = o.valid;
+    so.tol = p->lr_tolerance;
+    so.write_agg = o.agg != nullptr;
+    so.bands = opt.bands;
+    w.b.zero_flags = opt.replayable;
+    e = launch_sgm(g, w.b, so, p->p1, p->p2, nullptr, s, ev ? ev + 2 : nullptr);
+    if (e != hipSuccess) return hip_status(e, "aggregation", err);
+    if (o.agg) {
+        e = launch_restride_u16(w.agg, g.cs, o.agg, g.cap, g.W * g.H, g.cap, s);
+        if (e != hipSuccess) return hip_status(e, "agg copy", err);
+    }
+    return 0;
+}
+
+int check_band_error(Workspace& ws, hipStream_t s, std::string* err)
+{
+    auto it = ws.bufs.find("band_sync");
+    if (it == ws.bufs.end() || !it->second.p) return 0;
+    int word = 0;
+    hipError_t e = hipMemcpyAsync(&word, it->second.p, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e, "read band error word", err);
+    if (word) {
+        (void)launch_fill_u32((uint32_t*)it->second.p, 1, 0u, s);
+        (void)hipStreamSynchronize(s);
+        return fail(LUX_SGM_EDEVICE, "a band tile timed out waiting for its neighbours", err);
+    }
+    return 0;
+}
+
+}  // namespace lux

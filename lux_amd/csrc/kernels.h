@@ -40,6 +40,8 @@ struct SgmBuffers {
     const int* cellvis_plus = nullptr;   // [H][2] visit indices of each border-column row (-1: none)
     const int* cellvis_minus = nullptr;
     int seed_row = -1;
+    uint16_t* disp_tmp = nullptr;   // [H][W] disparity when the caller wants none
+    int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
 };
 
 // Census codes of one image (T = uint8_t or uint16_t pixels).

@@ -1236,6 +1236,8 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             ba.epoch = b.epoch;
             ba.err = b.err;
             fill_u32<<<1, 1, 0, s>>>((uint32_t*)ba.ticket, 1, 0u);
+            if (b.zero_flags)
+                fill_u32<<<grid_for((size_t)b.max_tiles, 256), 256, 0, s>>>((uint32_t*)ba.flags, (size_t)b.max_tiles, 0u);
             sgm_band<NP, DIR, MASKED><<<nbands * ba.strips, kBandWaves * 64, 0, s>>>(ba);
             mark();  // one launch for the three paths: the next two stages are empty
             mark();

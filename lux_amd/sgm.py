@@ -19,7 +19,12 @@ COST_CENSUS, COST_AD = 0, 1
 
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
-           "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option")
+           "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
+           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
+           "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_poll", "lux_sgm_server_wait",
+           "lux_sgm_server_cancel", "lux_sgm_server_info_get")
+MEM_HOST, MEM_DEVICE = 0, 1
+REQ_PENDING, REQ_RUNNING, REQ_DONE = 0, 1, 2
 OPT_FUSED_BANDS = 1
 STAGES = ("census", "rows+", "vd+1", "vd+2", "vd+3", "replay+", "vd-1", "vd-2", "vd-3", "rows-", "replay-", "finish")
 
@@ -37,6 +42,28 @@ class NotSupported(SgmError):
 class RectParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("width", "height", "stride", "bits", "cost", "census_radius", "ad_cap",
                                               "disparities", "align", "p1", "p2", "tile", "xs_clamp_w", "lr_tolerance")]
+
+
+class Pair(ctypes.Structure):
+    """lux_sgm_pair: one pair of a batch, every pointer device memory (0 = none)."""
+    _fields_ = [("params", RectParams), ("left", ctypes.c_void_p), ("right", ctypes.c_void_p),
+                ("disp", ctypes.c_void_p), ("sub", ctypes.c_void_p), ("valid", ctypes.c_void_p)]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("bands", ctypes.c_int32), ("branches", ctypes.c_int32), ("ms", ctypes.c_float)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("slot", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("pending", ctypes.c_int32), ("running", ctypes.c_int32), ("slots", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("budget", ctypes.c_uint64), ("bytes_held", ctypes.c_uint64),
+                ("bytes_peak", ctypes.c_uint64)]
 
 
 class LayerParams(ctypes.Structure):
@@ -68,6 +95,17 @@ def lib():
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
         L.lux_sgm_stage_times.argtypes = [vp, vp, i32]
         L.lux_sgm_set_option.argtypes = [vp, i32, i32]
+        L.lux_sgm_batch.argtypes = [vp, i32, vp, vp]
+        L.lux_sgm_batch_tune.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32]
+        L.lux_sgm_server_create.argtypes = [ctypes.c_int, u64, i32, ctypes.POINTER(vp)]
+        L.lux_sgm_server_destroy.argtypes = [vp]
+        L.lux_sgm_server_last_error.argtypes = [vp]
+        L.lux_sgm_server_last_error.restype = ctypes.c_char_p
+        L.lux_sgm_server_submit.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, ctypes.POINTER(u64)]
+        L.lux_sgm_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
+        L.lux_sgm_server_wait.argtypes = [vp, u64, i32, vp]
+        L.lux_sgm_server_cancel.argtypes = [vp, u64]
+        L.lux_sgm_server_info_get.argtypes = [vp, vp]
         _LIB = L
     return _LIB
 
@@ -217,3 +255,93 @@ class DenseLayer:
         _check(lib().lux_sgm_compute_layer(self.ctx.handle, ctypes.byref(p), _p(raw), _p(self.Agg), _p(self.AggPass1),
                                            _p(self.Wta)), self.ctx.handle)
         return self
+
+
+def batch(ctx, pairs, stream=0):
+    """Enqueue a list of Pair as one graph launch (lux_sgm_batch)."""
+    arr = (Pair * len(pairs))(*pairs)
+    _check(lib().lux_sgm_batch(ctx.handle, len(pairs), arr, ctypes.c_void_p(stream or None)), ctx.handle)
+
+
+def batch_tune(ctx, pairs, stream=0, reps=2):
+    """Time every candidate schedule on this batch; returns (best, [tried...]) as dicts."""
+    arr = (Pair * len(pairs))(*pairs)
+    best = Schedule()
+    tried = (Schedule * 32)()
+    n = lib().lux_sgm_batch_tune(ctx.handle, len(pairs), arr, ctypes.c_void_p(stream or None), reps,
+                                 ctypes.byref(best), tried, 32)
+    if n < 0:
+        _check(n, ctx.handle)
+    as_dict = lambda s: dict(bands=s.bands, branches=s.branches, ms=s.ms)  # noqa: E731
+    return as_dict(best), [as_dict(tried[i]) for i in range(min(n, 32))]
+
+
+def _check_srv(rc, srv):
+    if rc != OK:
+        msg = lib().lux_sgm_server_last_error(srv).decode("utf-8", "replace")
+        raise (NotSupported if rc == EUNSUPPORTED else SgmError)(rc, msg)
+
+
+class Server:
+    """The served form (lux_sgm_server_*): requests admitted over `slots` device slots under an HBM budget."""
+
+    def __init__(self, device=0, hbm_budget=0, slots=4):
+        h = ctypes.c_void_p()
+        _check_srv(lib().lux_sgm_server_create(device, hbm_budget, slots, ctypes.byref(h)), None)
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, p, left, right, *, disp=None, sub=None, valid=None):
+        """Host arrays (numpy): inputs are copied at once; outputs are written when wait() returns."""
+        t = ctypes.c_uint64()
+        _check_srv(lib().lux_sgm_server_submit(self.handle, ctypes.byref(p), MEM_HOST, _p(left), _p(right), _p(disp),
+                                               _p(sub), _p(valid), ctypes.byref(t)), self.handle)
+        self._keep[t.value] = (disp, sub, valid)
+        return t.value
+
+    def submit_device(self, p, d_left, d_right, d_disp=0, d_sub=0, d_valid=0):
+        """Device pointers (ints) that stay valid until the request completes."""
+        t = ctypes.c_uint64()
+        vp = lambda x: ctypes.c_void_p(x or None)  # noqa: E731
+        _check_srv(lib().lux_sgm_server_submit(self.handle, ctypes.byref(p), MEM_DEVICE, vp(d_left), vp(d_right),
+                                               vp(d_disp), vp(d_sub), vp(d_valid), ctypes.byref(t)), self.handle)
+        return t.value
+
+    def poll(self, ticket):
+        st = ctypes.c_int32()
+        _check_srv(lib().lux_sgm_server_poll(self.handle, ticket, ctypes.byref(st)), self.handle)
+        return st.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Wait for a request; returns its stats as a dict (raises on a failed request)."""
+        st = RequestStats()
+        rc = lib().lux_sgm_server_wait(self.handle, ticket, timeout_ms, ctypes.byref(st))
+        if rc != ECANCELED:  # on a time-out the request is still queued and will write the outputs
+            self._keep.pop(ticket, None)
+        _check_srv(rc, self.handle)
+        return dict(queued_ms=st.queued_ms, run_ms=st.run_ms, total_ms=st.total_ms, slot=st.slot)
+
+    def cancel(self, ticket):
+        _check_srv(lib().lux_sgm_server_cancel(self.handle, ticket), self.handle)
+
+    def info(self):
+        i = ServerInfo()
+        _check_srv(lib().lux_sgm_server_info_get(self.handle, ctypes.byref(i)), self.handle)
+        return {k: getattr(i, k) for k, _ in ServerInfo._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.handle:
+            lib().lux_sgm_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

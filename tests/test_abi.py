@@ -30,6 +30,12 @@ def test_create_without_device_fails_cleanly():
     with pytest.raises(sgm.SgmError) as e:
         sgm.Context(0)
     assert e.value.code == sgm.EDEVICE
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.Server(0, 0, slots=2)
+    assert e.value.code == sgm.EDEVICE
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.Server(0, 0, slots=0)
+    assert e.value.code == sgm.EINVAL
 
 
 def expected_plan(W, H, T, xsw=False):

@@ -1,0 +1,203 @@
+"""The batched (one graph launch) and served (request loop) forms: every result equals the plain drop-in's on
+the same pair alone, bit for bit, whatever the mix, schedule or admission order; the plain drop-in itself is
+checked against the oracle in test_gpu_parity.py and once more here."""
+import numpy as np
+import pytest
+
+from lux_amd import scene, sgm
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+# (W, H, D, bits, cost, radius, seed): mixed sizes, disparity ranges, pixel types and costs
+MIX = [
+    (96, 64, 32, 8, "census", 2, 1),
+    (160, 120, 64, 10, "census", 3, 2),
+    (130, 70, 48, 8, "ad", 0, 3),
+    (200, 100, 96, 10, "census", 3, 4),
+    (64, 48, 16, 8, "census", 1, 5),
+    (250, 90, 128, 10, "census", 3, 6),
+    (77, 33, 24, 12, "ad", 0, 7),
+]
+
+
+def _scene(W, H, D, bits, seed):
+    left, right, _ = scene.make_pair(W, H, max(D - 1, 1), planes=4, sigma=2.0, seed=seed, bits=bits)
+    return left, right
+
+
+def _params(W, H, D, bits, cost, radius, tile=16):
+    return sgm.rect_params(W, H, D, bits=bits, cost=cost, radius=max(radius, 1), ad_cap=20, p1=8, p2=96, tile=tile)
+
+
+@pytest.fixture(scope="module")
+def cases():
+    out = []
+    with sgm.Context(0) as c:
+        for (W, H, D, bits, cost, radius, seed) in MIX:
+            left, right = _scene(W, H, D, bits, seed)
+            p = _params(W, H, D, bits, cost, radius)
+            r = sgm.rectified(c, left, right, D, bits=bits, cost=cost, radius=max(radius, 1), ad_cap=20, p1=8, p2=96,
+                              tile=16)
+            out.append(dict(W=W, H=H, D=D, bits=bits, left=left, right=right, p=p, alone=r))
+    return out
+
+
+def test_alone_matches_oracle(cases):
+    c = cases[0]
+    left, right = c["left"], c["right"]
+    ref = o.rectified(left, right, c["D"], bits=8, cost="census", radius=2, p1=8, p2=96, T=16, align=8, tol=1)
+    np.testing.assert_array_equal(c["alone"]["disp"], ref["disp"])
+    np.testing.assert_array_equal(c["alone"]["valid"], ref["valid"])
+    np.testing.assert_allclose(c["alone"]["sub"], ref["sub"], rtol=0, atol=1e-4)
+
+
+def _device_pairs(torch, cases):
+    dev = torch.device("cuda:0")
+    pairs, outs, keep = [], [], []
+    for c in cases:
+        dt = torch.uint8 if c["bits"] == 8 else torch.int16
+        L = torch.from_numpy(c["left"].view(np.uint8 if c["bits"] == 8 else np.int16)).to(dev)
+        R = torch.from_numpy(c["right"].view(np.uint8 if c["bits"] == 8 else np.int16)).to(dev)
+        assert L.dtype == dt
+        d = torch.zeros((c["H"], c["W"]), dtype=torch.int16, device=dev)
+        s = torch.zeros((c["H"], c["W"]), dtype=torch.float32, device=dev)
+        v = torch.zeros((c["H"], c["W"]), dtype=torch.uint8, device=dev)
+        pr = sgm.Pair()
+        pr.params = c["p"]
+        pr.left, pr.right, pr.disp, pr.sub, pr.valid = L.data_ptr(), R.data_ptr(), d.data_ptr(), s.data_ptr(), v.data_ptr()
+        pairs.append(pr)
+        outs.append((d, s, v))
+        keep += [L, R]
+    return pairs, outs, keep
+
+
+def _check_outs(outs, cases, what):
+    for (d, s, v), c in zip(outs, cases):
+        tag = f"{what} {c['W']}x{c['H']} D{c['D']}"
+        np.testing.assert_array_equal(d.cpu().numpy().view(np.uint16), c["alone"]["disp"], err_msg=tag)
+        np.testing.assert_array_equal(s.cpu().numpy(), c["alone"]["sub"], err_msg=tag)
+        np.testing.assert_array_equal(v.cpu().numpy(), c["alone"]["valid"], err_msg=tag)
+
+
+def test_batch_graph_replays(cases):
+    torch = pytest.importorskip("torch")
+    with sgm.Context(0) as ctx:
+        pairs, outs, keep = _device_pairs(torch, cases)
+        stream = torch.cuda.Stream()
+        for rep in range(3):  # replays of one cached graph, with eager work and syncs between
+            print(f"batch replay {rep}", flush=True)
+            for d, s, v in outs:
+                d.fill_(-1)
+                s.fill_(-7.0)
+                v.fill_(9)
+            torch.cuda.synchronize()
+            sgm.batch(ctx, pairs, stream=stream.cuda_stream)
+            stream.synchronize()
+            _check_outs(outs, cases, f"batch replay {rep}")
+        del keep
+
+
+def test_batch_tuned_schedules(cases):
+    torch = pytest.importorskip("torch")
+    with sgm.Context(0) as ctx:
+        pairs, outs, keep = _device_pairs(torch, cases)
+        stream = torch.cuda.Stream()
+        best, tried = sgm.batch_tune(ctx, pairs, stream=stream.cuda_stream, reps=1)
+        print("tuned", best, tried, flush=True)
+        assert len(tried) >= 4 and best["ms"] > 0
+        assert {t["bands"] for t in tried} == {0, 1}  # every pair fits the band kernel (cap <= 256)
+        _check_outs(outs, cases, "after tuning")  # the last candidate's run
+        for d, s, v in outs:
+            d.fill_(-1)
+        torch.cuda.synchronize()
+        sgm.batch(ctx, pairs, stream=stream.cuda_stream)  # the tuned schedule
+        stream.synchronize()
+        _check_outs(outs, cases, "tuned batch")
+        del keep
+
+
+def _submit_all(srv, cases, order):
+    tickets = {}
+    for i in order:
+        c = cases[i]
+        outs = (np.full((c["H"], c["W"]), 0xffff, np.uint16), np.full((c["H"], c["W"]), -1, np.float32),
+                np.full((c["H"], c["W"]), 7, np.uint8))
+        t = srv.submit(c["p"], c["left"], c["right"], disp=outs[0], sub=outs[1], valid=outs[2])
+        tickets[t] = (i, outs)
+    return tickets
+
+
+def _collect(srv, tickets, cases):
+    stats = []
+    for t, (i, (d, s, v)) in tickets.items():
+        st = srv.wait(t, timeout_ms=60000)
+        stats.append(st)
+        c = cases[i]
+        tag = f"served {c['W']}x{c['H']} D{c['D']}"
+        np.testing.assert_array_equal(d, c["alone"]["disp"], err_msg=tag)
+        np.testing.assert_array_equal(s, c["alone"]["sub"], err_msg=tag)
+        np.testing.assert_array_equal(v, c["alone"]["valid"], err_msg=tag)
+    return stats
+
+
+def test_server_mix_equals_alone(cases):
+    order = list(range(len(cases))) * 3
+    np.random.default_rng(0).shuffle(order)
+    with sgm.Server(0, 0, slots=4) as srv:
+        print(f"served mix of {len(order)} requests over 4 slots", flush=True)
+        tickets = _submit_all(srv, cases, order)
+        stats = _collect(srv, tickets, cases)
+        info = srv.info()
+        assert info["completed"] == len(order) and info["failed"] == 0
+        assert len({st["slot"] for st in stats}) > 1  # the work was spread over slots
+        assert all(st["total_ms"] >= st["run_ms"] >= 0 for st in stats)
+
+
+def test_server_device_memory(cases):
+    torch = pytest.importorskip("torch")
+    pairs, outs, keep = _device_pairs(torch, cases)
+    torch.cuda.synchronize()
+    with sgm.Server(0, 0, slots=3) as srv:
+        ts = [srv.submit_device(pr.params, pr.left, pr.right, pr.disp, pr.sub, pr.valid) for pr in pairs]
+        for t in ts:
+            srv.wait(t, timeout_ms=60000)
+    _check_outs(outs, cases, "served device")
+    del keep
+
+
+def test_server_budget_admission(cases):
+    big = cases[5]  # the largest workspace of the mix
+    need = sgm.rect_bytes(big["W"], big["H"], big["D"], bits=big["bits"], tile=16)
+    budget = int(need * 1.5)  # room for one large request and some small ones at a time
+    with sgm.Server(0, budget, slots=4) as srv:
+        order = [5, 3, 5, 1, 5, 0, 3]
+        tickets = _submit_all(srv, cases, order)
+        _collect(srv, tickets, cases)
+        info = srv.info()
+        assert info["bytes_peak"] <= budget
+        assert info["completed"] == len(order)
+        huge = sgm.rect_params(4000, 3000, 256, bits=10)
+        with pytest.raises(sgm.SgmError) as e:
+            srv.submit(huge, np.zeros((3000, 4000), np.uint16), np.zeros((3000, 4000), np.uint16))
+        assert e.value.code == sgm.ENOMEM
+
+
+def test_server_cancel_pending(cases):
+    with sgm.Server(0, 0, slots=1) as srv:
+        c = cases[5]
+        ts = [srv.submit(c["p"], c["left"], c["right"]) for _ in range(6)]
+        last = ts[-1]
+        try:
+            srv.cancel(last)
+            cancelled = True
+        except sgm.SgmError:
+            cancelled = False  # it was admitted already
+        for t in ts[:-1]:
+            srv.wait(t, timeout_ms=60000)
+        if cancelled:
+            with pytest.raises(sgm.SgmError) as e:
+                srv.wait(last, timeout_ms=60000)
+            assert e.value.code == sgm.ECANCELED
+        else:
+            srv.wait(last, timeout_ms=60000)
