@@ -370,30 +370,33 @@ struct RowArgs {
     int write_agg;          // store the final aggregate too
 };
 
-// Census inputs of the lane's planes of cell (x, y), prefetched as loaded: the left code and the right codes
-// cr(max(x-d, 0)) of the lane's planes d.
-template <int NP>
-struct CensusIn {
-    uint64_t a;
-    uint64_t b[2 * NP];
-};
-template <int NP>
-__device__ __forceinline__ void load_census(const uint64_t* cl, const uint64_t* crow, int x, int y, int W, int lo, bool active,
-                                            CensusIn<NP>& c)
+// Census codes along a row. Lane l owns planes d = lo .. lo+2*NP-1 (lo = 2*NP*l) and keeps the window
+// win[e] = cr(max(x - lo - e, 0)) of the right-image codes its planes compare with at the current x. Moving
+// to x+1 the window shifts by one code: win[e] <- win[e-1], and win[0] <- the left neighbour lane's
+// win[2*NP-1] = cr(max(x + 1 - lo, 0)) (DPP wave_shr), lane 0 taking cr(x+1) itself. So a step reads two
+// wave-uniform codes, cl(x) and cr(x+1), instead of 2*NP per lane.
+__device__ __forceinline__ uint64_t shr1_64(uint64_t v)
 {
-    c.a = active ? cl[(size_t)y * W + x] : 0ull;
+    const uint32_t lo = shr1((uint32_t)v, 0u), hi = shr1((uint32_t)(v >> 32), 0u);
+    return ((uint64_t)hi << 32) | lo;
+}
+template <int NP>
+__device__ __forceinline__ void census_advance(uint64_t (&win)[2 * NP], uint64_t fresh, int lane)
+{
+    const uint64_t from = shr1_64(win[2 * NP - 1]);
 #pragma unroll
-    for (int e = 0; e < 2 * NP; e++) c.b[e] = active ? crow[max(x - lo - e, 0)] : 0ull;
+    for (int e = 2 * NP - 1; e > 0; e--) win[e] = win[e - 1];
+    win[0] = lane == 0 ? fresh : from;
 }
 // Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
 // repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
 template <int NP>
-__device__ __forceinline__ void census_cost(const CensusIn<NP>& c, int D, int lo, Cell<NP>& raw,
+__device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2 * NP], int D, int lo, Cell<NP>& raw,
                                             uint32_t (&bytes)[NP / 2 + (NP == 1)])
 {
     uint32_t v[2 * NP];
 #pragma unroll
-    for (int e = 0; e < 2 * NP; e++) v[e] = lo + e < D ? (uint32_t)__popcll(c.a ^ c.b[e]) : 255u;
+    for (int e = 0; e < 2 * NP; e++) v[e] = lo + e < D ? (uint32_t)__popcll(cl ^ win[e]) : 255u;
 #pragma unroll
     for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
     if constexpr (NP == 1) {
@@ -413,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 {
     constexpr int PF = 8;
     const int lane = threadIdx.x & 63;
-    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    const int y = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
     if (y >= a.H) return;
     const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
     Cell<NP> mask;
@@ -425,21 +428,33 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     uint32_t minprev = 2000;
     const size_t rowcell = (size_t)y * W;
     const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
+    const uint64_t* clrow = COST ? a.cl + rowcell : nullptr;
     const int lol = store_lane ? lo : 0;
-    CensusIn<NP> cb[COST ? PF : 1];
+    // COST: the lane's window of right codes, and the prefetched wave-uniform codes cl(x), cr(x+1)
+    uint64_t win[2 * NP];
+    uint64_t rl[COST ? PF : 1], rr[COST ? PF : 1];
     RawW<NP> rb[COST ? 1 : PF];
+    if constexpr (COST) {
+        const uint64_t c0 = crow[0];
+#pragma unroll
+        for (int e = 0; e < 2 * NP; e++) win[e] = c0;
+    }
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = min(j, W - 1);
-        if constexpr (COST) load_census<NP>(a.cl, crow, xj, y, W, lol, true, cb[j]);
-        else ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        if constexpr (COST) {
+            rl[j] = clrow[xj];
+            rr[j] = crow[min(j + 1, W - 1)];
+        } else {
+            ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        }
     }
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         const int xp = min(x + PF, W - 1);
         if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
             uint32_t bytes[NP / 2 + (NP == 1)];
-            census_cost<NP>(cb[j], a.D, lo, raw, bytes);
+            census_cost<NP>(rl[j], win, a.D, lo, raw, bytes);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
             if (store_lane) {
                 if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
@@ -448,7 +463,9 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
                     for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bytes[k];
                 }
             }
-            load_census<NP>(a.cl, crow, xp, y, W, lol, true, cb[j]);
+            census_advance<NP>(win, rr[j], lane);
+            rl[j] = clrow[xp];
+            rr[j] = crow[min(x + PF + 1, W - 1)];
         } else {
             widen<NP>(rb[j], raw);
             ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);

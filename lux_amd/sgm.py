@@ -79,6 +79,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or `make -C lux_amd/csrc`")
+        # PyTorch-ROCm carries its own HIP runtime whose file is named differently from the system one: load it
+        # first so that the library binds to it, and one runtime (not two competing for the device) serves the
+        # process when tensors and the library share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64
         L.lux_sgm_version.restype = ctypes.c_int
