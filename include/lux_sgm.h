@@ -103,9 +103,25 @@ typedef struct {
  *   agg       [H*W*cap] uint16 out: DenseLayer.Agg                       (may be NULL)
  *   agg_pass1 [H*W*cap] uint16 out: DenseLayer.AggPass1 (KeepPass1)      (may be NULL)
  *   wta       [H*W]     uint16 out: plane index start+arg of the WTA      (may be NULL)
- * Per-pixel ranges (RangeFromPrevious, :91-140) and skip masks (:65-87) are LUX_SGM_EUNSUPPORTED. */
+ * Skipped pixels (SkipMask, :65-87) need nothing here: their raw bytes (0 in range, 255 padding, :242) are
+ * part of `raw`. Per-pixel ranges: lux_sgm_compute_layer_ranges. */
 int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
                           uint16_t* agg_pass1, uint16_t* wta);
+
+/* DenseLayer.Compute with per-pixel plane ranges (DenseLayer Start/Count/Cap, DenseStereoSgm.cs:15-26, :43-58, as
+ * RangeFromPrevious leaves them, :91-140). start/count: [H*W] uint16; cap_i = roundup(count_i, align).
+ *   raw       jagged: cap_i bytes per pixel in raster order (DenseLayer.Raw[i] concatenated)
+ *   agg       out, jagged like raw, uint16 (DenseLayer.Agg[i])                       (may be NULL)
+ *   agg_pass1 out, the same after pass +1 (DenseLayer.AggPass1)                      (may be NULL)
+ *   wta       [H*W] out: start + first-minimum index over [0, count) (:225-231)       (may be NULL)
+ * p->planes is the layer's plane count (start + count <= planes). The device path keeps each path vector in
+ * its cell's own plane frame and realigns it to the next cell's start; that is the reference's line-buffer
+ * result as long as no range's padding reaches the last word of its path segment: start - MinLo + cap <=
+ * MaxHi for every pixel (MinLo = min start, MaxHi = roundup(max(start + count), align)). A layer that breaks
+ * it, or has a cap above 128, gets LUX_SGM_EUNSUPPORTED. */
+int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
+                                 const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
+                                 uint16_t* wta);
 
 /* Context options (no counterpart in the reference: execution strategy only, results are identical).
  * LUX_SGM_OPT_FUSED_BANDS: 1 runs the three vertical/diagonal paths of each pass fused in band tiles that

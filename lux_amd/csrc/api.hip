@@ -423,6 +423,94 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     return lux::check_band_error(ws, s, &c->err);
 }
 
+int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start,
+                                 const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
+                                 uint16_t* wta)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!raw || !start || !count) return fail(c, LUX_SGM_EINVAL, "null argument");
+    int rc = lux::check_layer(p, &c->err);
+    if (rc) return rc;
+    const int W = p->width, H = p->height, align = p->align;
+    const size_t cells = (size_t)W * H;
+    // ranges as DenseLayer.Init leaves them (:53-58): cap = roundup(count, align); MinLo = min start,
+    // MaxHi = roundup(max(start + count), align) (:46, :51, :136)
+    std::vector<uint16_t> cap(cells);
+    std::vector<uint64_t> off(cells);
+    uint64_t total = 0;
+    int minlo = 1 << 30, maxhi = 0, maxcap = 0, maxcount = 0;
+    for (size_t i = 0; i < cells; i++) {
+        if ((int)start[i] + (int)count[i] > p->planes) return fail(c, LUX_SGM_EINVAL, "a range passes the last plane");
+        const int cp = ((count[i] + align - 1) / align) * align;
+        cap[i] = (uint16_t)cp;
+        off[i] = total;
+        total += (uint64_t)cp;
+        minlo = std::min(minlo, (int)start[i]);
+        maxhi = std::max(maxhi, (int)start[i] + (int)count[i]);
+        maxcap = std::max(maxcap, cp);
+        maxcount = std::max(maxcount, (int)count[i]);
+    }
+    maxhi = ((maxhi + align - 1) / align) * align;
+    if (maxcap == 0) return fail(c, LUX_SGM_EINVAL, "every range is empty");
+    if (maxcap > 128) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 128 lanes");
+    for (size_t i = 0; i < cells; i++)
+        if ((int)start[i] - minlo + (int)cap[i] > maxhi)
+            return fail(c, LUX_SGM_EUNSUPPORTED,
+                        "a range's padding reaches the end of its line-buffer segment (start - MinLo + cap > MaxHi): "
+                        "the reference then writes into the next path's segment, which the device path does not model");
+    (void)hipSetDevice(c->device);
+    hipStream_t s = c->stream;
+    lux::Geometry g = lux::geometry(W, H, maxcount, align);
+    lux::Workspace& ws = c->ws;
+    lux::Prepared w;
+    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err))) return rc;
+    void *vs, *vc, *vn, *ss, *sc, *vo, *stage, *a1 = nullptr, *dw;
+    if ((rc = ws.get("vstart", cells * 2, &vs, &c->err)) || (rc = ws.get("vcap", cells * 2, &vc, &c->err)) ||
+        (rc = ws.get("vcount", cells * 2, &vn, &c->err)) || (rc = ws.get("seedstart", (size_t)3 * W * 2, &ss, &c->err)) ||
+        (rc = ws.get("seedcap", (size_t)3 * W * 2, &sc, &c->err)) || (rc = ws.get("voff", cells * 8, &vo, &c->err)) ||
+        (rc = ws.get("layer_stage", std::max<size_t>(total * 2, 2), &stage, &c->err)) ||
+        (rc = ws.get("layer_wta", cells * 2, &dw, &c->err)))
+        return rc;
+    if (agg_pass1 && (rc = ws.get("layer_pass1", cells * g.cs * 2, &a1, &c->err))) return rc;
+    hipError_t e = hipMemcpyAsync(vs, start, cells * 2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vc, cap.data(), cells * 2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vn, count, cells * 2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vo, off.data(), cells * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && total) e = hipMemcpyAsync(stage, raw, total, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = lux::launch_unpack_u8((const uint8_t*)stage, (const uint64_t*)vo, (const uint16_t*)vc, w.raw, g.cs, cells, s);
+    if (e != hipSuccess) return hip_fail(c, e, "upload ranges and raw");
+    w.b.vstart = (const uint16_t*)vs;
+    w.b.vcap = (const uint16_t*)vc;
+    w.b.vcount = (const uint16_t*)vn;
+    w.b.seedstart = (uint16_t*)ss;
+    w.b.seedcap = (uint16_t*)sc;
+    lux::SgmOutputs o;
+    o.disp = (uint16_t*)dw;
+    o.tol = 0;
+    o.write_agg = agg != nullptr;
+    o.bands = 0;
+    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
+    if (e != hipSuccess) return hip_fail(c, e, "aggregation");
+    if (wta) {
+        e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(c, e, "wta");
+    }
+    uint16_t* outs[2] = {agg, agg_pass1};
+    const uint16_t* srcs[2] = {w.agg, (const uint16_t*)a1};
+    for (int k = 0; k < 2; k++) {
+        if (!outs[k] || !total) continue;
+        e = lux::launch_pack_u16(srcs[k], (const uint64_t*)vo, (const uint16_t*)vc, (uint16_t*)stage, g.cs, cells, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(outs[k], stage, total * 2, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(c, e, "download aggregate");
+    }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "compute_layer_ranges");
+    return 0;
+}
+
 int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)
 {
     if (!c) return LUX_SGM_EINVAL;

@@ -41,6 +41,13 @@ struct SgmBuffers {
     const int* cellvis_minus = nullptr;
     int seed_row = -1;
     uint16_t* disp_tmp = nullptr;   // [H][W] disparity when the caller wants none
+    // per-pixel plane ranges (null: every pixel has planes [0, cap)): start, cap and count of every cell, and
+    // the start / cap of the cell behind each seed-table entry. Needs cap <= 128 (dpl 2); no fused bands.
+    const uint16_t* vstart = nullptr;
+    const uint16_t* vcap = nullptr;
+    const uint16_t* vcount = nullptr;
+    uint16_t* seedstart = nullptr;  // [3][W]
+    uint16_t* seedcap = nullptr;
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
 };
 
@@ -70,6 +77,11 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 int band_tiles(int W, int H);
 // Repack [H][W][cs] <-> [H][W][cap] on the device.
 hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s);
+// Jagged per-pixel arrays (cap[i] entries at off[i]) <-> the cell-major volume of stride cs (255 past a cap).
+hipError_t launch_unpack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
+                            hipStream_t s);
+hipError_t launch_pack_u16(const uint16_t* src, const uint64_t* off, const uint16_t* cap, uint16_t* dst, int cs, size_t cells,
+                           hipStream_t s);
 hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,
                               hipStream_t s);
 

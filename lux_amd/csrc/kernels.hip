@@ -161,15 +161,18 @@ __device__ __forceinline__ void store_u16(uint16_t* p, bool active, const Cell<N
 // Lm/Lp of the range ends read the guard words (2000). min(sat(a+P1), sat(b+P1)) = sat(min(a,b)+P1) as
 // saturating addition of a constant is monotone. Planes beyond cap (mask 0) hold 2000 like the guards and
 // are left out of the minimum. Returns the new minPrev (wave-uniform).
+// edge_l / edge_r (both halves equal) are the neighbour planes just below lane 0's first plane and just above
+// lane 63's last: the guard value 2000 unless a realigned range (below) puts a stored plane there.
 template <int NP, bool MASKED>
 __device__ __forceinline__ uint32_t path_step(const Cell<NP>& prev, uint32_t minprev, const Cell<NP>& raw,
-                                              uint32_t p1x2, uint32_t p2, const Cell<NP>& mask, Cell<NP>& lc)
+                                              uint32_t p1x2, uint32_t p2, const Cell<NP>& mask, Cell<NP>& lc,
+                                              uint32_t edge_l = kInit2, uint32_t edge_r = kInit2)
 {
     uint32_t p2p = p2 + minprev;
     p2p = p2p > 65535u ? 65535u : p2p;
     const uint32_t p2x2 = p2p | (p2p << 16), minx2 = minprev | (minprev << 16);
-    const uint32_t left = shr1(prev.v[NP - 1], kInit2);   // plane l*DPL-1 in the high half
-    const uint32_t right = shl1(prev.v[0], kInit2);       // plane (l+1)*DPL in the low half
+    const uint32_t left = shr1(prev.v[NP - 1], edge_l);   // plane l*DPL-1 in the high half
+    const uint32_t right = shl1(prev.v[0], edge_r);       // plane (l+1)*DPL in the low half
     uint32_t run = 0xffffffffu;
 #pragma unroll
     for (int k = 0; k < NP; k++) {
@@ -209,6 +212,74 @@ __device__ __forceinline__ void set_init(Cell<NP>& c)
 {
 #pragma unroll
     for (int k = 0; k < NP; k++) c.v[k] = kInit2;
+}
+
+// ------------------------------------------------------------------ per-pixel plane ranges
+// With per-pixel ranges (DenseLayer Start/Count/Cap, DenseStereoSgm.cs:15-26, :43-58) a cell stores planes
+// [start, start+cap) of its path vector, lane l holding relative planes 2l, 2l+1 (one packed pair per lane,
+// cap <= 128). The line-buffer slot a successor reads (:265, :288-298) then holds those planes and 2000
+// everywhere else: the stale cleanup (:254-264) refills the previous occupant's range before each write, and
+// the guard words sit between the path segments. That holds as long as no pixel's range reaches the last word
+// of its segment (start - MinLo + cap <= MaxHi), which the host checks. So a successor starting at s sees
+// the predecessor (start sp, cap cp) through this realignment: relative plane j of the successor is plane
+// j + delta (delta = s - sp) of the predecessor, 2000 outside [0, cp); edge_l / edge_r are planes -1 and
+// 128 in the successor's frame.
+struct Range {
+    int s = 0, c = 0;  // start, cap of the cell whose state a vector holds (c = 0: the initial fill)
+};
+__device__ __forceinline__ uint32_t rel_plane(uint32_t v, int r, int cp)
+{
+    if (r < 0 || r >= cp) return 2000u;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, r >> 1);
+    return (r & 1) ? (w >> 16) : (w & 0xffffu);
+}
+__device__ __forceinline__ void realign(Cell<1>& st, const Range& from, int s, int lane, uint32_t& edge_l,
+                                        uint32_t& edge_r)
+{
+    const int delta = s - from.s, cp = from.c;
+    const uint32_t v = st.v[0];
+    const int r0 = 2 * lane + delta;
+    uint32_t out;
+    if ((delta & 1) == 0) {
+        const int sl = lane + (delta >> 1);
+        out = (uint32_t)__builtin_amdgcn_ds_bpermute((sl & 63) << 2, (int)v);
+    } else {
+        const int sa = lane + ((delta - 1) >> 1);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute((sa & 63) << 2, (int)v);
+        const uint32_t b = (uint32_t)__builtin_amdgcn_ds_bpermute(((sa + 1) & 63) << 2, (int)v);
+        out = __builtin_amdgcn_alignbit(b, a, 16);
+    }
+    const bool ok0 = r0 >= 0 && r0 < cp, ok1 = r0 + 1 >= 0 && r0 + 1 < cp;
+    st.v[0] = (ok0 ? (out & 0xffffu) : 2000u) | (ok1 ? (out & 0xffff0000u) : (2000u << 16));
+    const uint32_t el = rel_plane(v, delta - 1, cp), er = rel_plane(v, 128 + delta, cp);
+    edge_l = el | (el << 16);
+    edge_r = er | (er << 16);
+}
+// Move a chain's state into the frame of the next cell (start s); a no-op when the start does not change
+// (the planes past the predecessor's cap already hold 2000).
+template <int NP, bool VAR>
+__device__ __forceinline__ void reframe(Cell<NP>& st, Range& from, int s, int c, int lane, uint32_t& edge_l,
+                                        uint32_t& edge_r)
+{
+    edge_l = kInit2;
+    edge_r = kInit2;
+    if constexpr (VAR) {
+        static_assert(NP == 1, "per-pixel ranges take one packed pair per lane");
+        if (s != from.s) realign(*(Cell<1>*)&st, from, s, lane, edge_l, edge_r);
+        from.s = s;
+        from.c = c;
+    }
+}
+// A state loaded from a saved table (seeds, border columns) holds only the lanes of the stored stride: give
+// every plane past the stored cap the 2000 the line buffer has there.
+template <int NP>
+__device__ __forceinline__ void clip_to_cap(Cell<NP>& st, int cap, int lane)
+{
+    Cell<NP> m;
+    bool any;
+    make_mask<NP>(lane, cap, m, any);
+#pragma unroll
+    for (int k = 0; k < NP; k++) st.v[k] = (st.v[k] & m.v[k]) | (kInit2 & ~m.v[k]);
 }
 
 // Plane q of a cell held in lanes (2*NP planes per lane), read wave-uniformly.
@@ -256,12 +327,16 @@ struct SweepArgs {
     int seed_row;
     uint16_t* cols;      // [2][H][cs] (this pass)
     uint16_t* colmins;   // [2][H]
+    const uint16_t* vstart;  // per-pixel ranges: start and cap of every cell [H][W] (null: uniform)
+    const uint16_t* vcap;
+    uint16_t* seedstart;     // [3][W] start / cap of the cell each seed-table entry belongs to
+    uint16_t* seedcap;
 };
 
 // One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
 // (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
 // 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
-template <int NP, int DIR, int P, bool FIRST, bool MASKED>
+template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR>
 __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 {
     constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
@@ -281,13 +356,18 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 
     Cell<NP> state;
     uint32_t minprev = 2000;
+    Range from;
     set_init(state);
     if (DIR < 0 && P != 0 && y == H - 1) {
         // pass -1's first row reads half 0 as pass +1 left it (:28-29, :219)
         const int sc = x - DX;
         if (sc >= 0 && sc < W) {
-            load_u16<NP>(a.seeds + ((size_t)(P - 1) * W + sc) * cs + lo, active, state);
-            if (MASKED) {
+            load_u16<NP>(a.seeds + ((size_t)(P - 1) * W + sc) * cs + lo, lo < cs, state);
+            if (VAR) {
+                from.s = a.seedstart[(P - 1) * W + sc];
+                from.c = a.seedcap[(P - 1) * W + sc];
+                clip_to_cap<NP>(state, from.c, lane);
+            } else if (MASKED) {
 #pragma unroll
                 for (int k = 0; k < NP; k++) state.v[k] = (state.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
             }
@@ -299,33 +379,51 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     constexpr int PF = 8;
     const ptrdiff_t step = (ptrdiff_t)DY * W + DX;
     const size_t cell0 = (size_t)y * W + x;
-    const int lol = active ? lo : 0;
+    const int lol = (VAR ? lo < cs : active) ? lo : 0;
     size_t cell = cell0;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
+    uint32_t rs[VAR ? PF : 1];  // ranges of the prefetched cells: start | cap << 16
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
         ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
         if (!FIRST) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16);
     }
-    auto body = [&](int i, RawW<NP>& rbj, Cell<NP>& abj) {
+    auto body = [&](int i, int j) {
         Cell<NP> raw;
-        widen<NP>(rbj, raw);
-        const Cell<NP> agg = abj;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> agg = ab[j];
+        uint32_t el = kInit2, er = kInit2;
+        int cs_ = 0, cc_ = 0;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            cs_ = (int)(r & 0xffffu);
+            cc_ = (int)(r >> 16);
+            reframe<NP, VAR>(state, from, cs_, cc_, lane, el, er);
+            make_mask<NP>(lane, cc_, mask, active);
+        }
         const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
-        ld_raw_w<NP>(a.raw + cp * cs + lol, rbj);
-        if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, abj);
+        ld_raw_w<NP>(a.raw + cp * cs + lol, rb[j]);
+        if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cp] | ((uint32_t)a.vcap[cp] << 16);
         Cell<NP> lc;
-        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er);
         if (x != bx) {
             Cell<NP> out;
 #pragma unroll
             for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
             store_u16<NP>(a.agg + cell * cs + lo, active, out);
             if (DIR > 0 && P != 0 && y == a.seed_row) {
-                store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, active, lc);
-                if (lane == 0) a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
+                store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, lo < cs, lc);
+                if (lane == 0) {
+                    a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
+                    if (VAR) {
+                        a.seedstart[(P - 1) * W + x] = (uint16_t)cs_;
+                        a.seedcap[(P - 1) * W + x] = (uint16_t)cc_;
+                    }
+                }
             }
         } else if (FIRST) {
             Cell<NP> z;
@@ -334,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
             store_u16<NP>(a.agg + cell * cs + lo, active, z);
         }
         if (P <= 1 && x == bx - DIR) {  // the border replay reads this column's path 0 / 1 states
-            store_u16<NP>(a.cols + ((size_t)P * H + y) * cs + lo, active, lc);
+            store_u16<NP>(a.cols + ((size_t)P * H + y) * cs + lo, lo < cs, lc);
             if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
         }
         state = lc;
@@ -345,11 +443,11 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     int i0 = 0;
     for (; i0 + PF <= n; i0 += PF) {
 #pragma unroll
-        for (int j = 0; j < PF; j++) body(i0 + j, rb[j], ab[j]);
+        for (int j = 0; j < PF; j++) body(i0 + j, j);
     }
 #pragma unroll
     for (int j = 0; j < PF; j++)
-        if (i0 + j < n) body(i0 + j, rb[j], ab[j]);
+        if (i0 + j < n) body(i0 + j, j);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
@@ -368,6 +466,9 @@ struct RowArgs {
     uint16_t* dr;           // [H][W] right-image disparity
     uint32_t* dr0;          // [H] dR(0) without cell (0, 0)
     int write_agg;          // store the final aggregate too
+    const uint16_t* vstart; // per-pixel ranges [H][W]: start, cap and count (null: uniform)
+    const uint16_t* vcap;
+    const uint16_t* vcount;
 };
 
 // Census codes along a row. Lane l owns planes d = lo .. lo+2*NP-1 (lo = 2*NP*l) and keeps the window
@@ -411,9 +512,10 @@ __device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2
 // of adding; the border column W-1 gets 0 and is filled by the replay). COST: computes the census cost of
 // each cell on the way and writes the raw volume for the later sweeps. Saves the W-2 column's states for
 // the border replay.
-template <int NP, bool MASKED, bool COST>
+template <int NP, bool MASKED, bool COST, bool VAR>
 __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 {
+    static_assert(!(COST && VAR), "per-pixel ranges take their raw volume from the cost kernel");
     constexpr int PF = 8;
     const int lane = threadIdx.x & 63;
     const int y = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
@@ -434,6 +536,8 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     uint64_t win[2 * NP];
     uint64_t rl[COST ? PF : 1], rr[COST ? PF : 1];
     RawW<NP> rb[COST ? 1 : PF];
+    uint32_t rs[VAR ? PF : 1];  // VAR: start | cap << 16 of the prefetched cells
+    Range from;
     if constexpr (COST) {
         const uint64_t c0 = crow[0];
 #pragma unroll
@@ -448,10 +552,18 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
         } else {
             ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         }
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
     }
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         const int xp = min(x + PF, W - 1);
+        uint32_t el = kInit2, er = kInit2;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            reframe<NP, VAR>(state, from, (int)(r & 0xffffu), (int)(r >> 16), lane, el, er);
+            make_mask<NP>(lane, (int)(r >> 16), mask, active);
+            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+        }
         if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
             uint32_t bytes[NP / 2 + (NP == 1)];
             census_cost<NP>(rl[j], win, a.D, lo, raw, bytes);
@@ -471,13 +583,13 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
             ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
         }
         Cell<NP> lc;
-        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er);
         Cell<NP> out;
 #pragma unroll
         for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
         store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
         if (x == W - 2) {
-            store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, lc);
             if (lane == 0) a.colmins[y] = (uint16_t)minprev;
         }
         state = lc;
@@ -503,14 +615,15 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 //         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
 //         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
 // Saves the column-1 states for the border replay.
-template <int NP, bool MASKED>
+template <int NP, bool MASKED, bool VAR>
 __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 {
     constexpr int PF = 8, E = 2 * NP;
     const int lane = threadIdx.x & 63;
     const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
-    const int W = a.W, cs = a.cs, D = a.D, lo = lane * E;
+    const int W = a.W, cs = a.cs, lo = lane * E;
+    int D = a.D;
     Cell<NP> mask;
     bool active;
     make_mask<NP>(lane, a.cap, mask, active);
@@ -521,14 +634,20 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 #pragma unroll
     for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
     const size_t rowcell = (size_t)y * W;
-    const int lol = active ? lo : 0;
+    const int lol = (VAR ? lo < cs : active) ? lo : 0;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
+    uint32_t rs[VAR ? PF : 1], rn[VAR ? PF : 1];  // VAR: start | cap << 16, and count, of the prefetched cells
+    Range from;
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        if constexpr (VAR) {
+            rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
+            rn[j] = a.vcount[rowcell + xj];
+        }
     }
     uint32_t myd = 0, myr = 0;
     float mys = 0.f;
@@ -537,15 +656,26 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
         widen<NP>(rb[j], raw);
         const Cell<NP> agg = ab[j];
         const int xp = max(x - PF, 0);
+        uint32_t el = kInit2, er = kInit2;
+        int s0 = 0;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            s0 = (int)(r & 0xffffu);
+            D = __builtin_amdgcn_readfirstlane((int)rn[j]);
+            reframe<NP, VAR>(state, from, s0, (int)(r >> 16), lane, el, er);
+            make_mask<NP>(lane, (int)(r >> 16), mask, active);
+            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+            rn[j] = a.vcount[rowcell + xp];
+        }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
         ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
         Cell<NP> lc, fin;
-        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc);
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er);
 #pragma unroll
         for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
         if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
         if (x == 1) {
-            store_u16<NP>(a.cols + (size_t)y * cs + lo, active, lc);
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, lc);
             if (lane == 0) a.colmins[y] = (uint16_t)minprev;
         }
         state = lc;
@@ -559,23 +689,26 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
             key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
             best = key[e] < best ? key[e] : best;
         }
-        const uint32_t carry = shl1(R[0], 0xffffffffu);
+        uint32_t rdone = 0;
+        if constexpr (!VAR) {  // the right-image disparity needs one plane range for the whole row
+            const uint32_t carry = shl1(R[0], 0xffffffffu);
 #pragma unroll
-        for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
-        R[E - 1] = carry;
+            for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
+            R[E - 1] = carry;
 #pragma unroll
-        for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
-        const uint32_t rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
+            for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+            rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
+        }
         best = wave_min(best);
-        const int d = (int)(best & 0xffffu);
-        float v = (float)d;
+        const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+        float v = (float)(d + s0);
         if (d > 0 && d < D - 1) {
             const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
             const int den = l - 2 * cc + r;
-            if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
         }
         if (lane == (x & 63)) {
-            myd = (uint32_t)d;
+            myd = (uint32_t)(d + s0);  // the plane index start + arg (:230)
             mys = v;
             myr = rdone & 0xffffu;
         }
@@ -586,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
                 const size_t o = rowcell + xb + lane;
                 a.disp[o] = (uint16_t)myd;
                 if (a.sub) a.sub[o] = mys;
-                a.dr[o] = (uint16_t)myr;
+                if (!VAR) a.dr[o] = (uint16_t)myr;
             }
         }
     };
@@ -600,19 +733,22 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(W - 1 - (i0 + j), j);
-    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
-    if (lane == 0) a.dr0[y] = r1;
+    if constexpr (!VAR) {
+        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
+        if (lane == 0) a.dr0[y] = r1;
+    }
 }
 
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
 // valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol (RealStereo.cs:418-421).
-template <int NP>
+template <int NP, bool VAR>
 __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* valid)
 {
     const int y = blockIdx.x, lane = threadIdx.x;
-    const int W = a.W, cs = a.cs, D = a.D, lo = lane * 2 * NP;
+    const size_t rowcell = (size_t)y * a.W;
+    const int W = a.W, cs = a.cs, D = VAR ? (int)a.vcount[rowcell] : a.D, lo = lane * 2 * NP;
+    const int s0 = VAR ? (int)a.vstart[rowcell] : 0;
     const bool active = lo < cs;
-    const size_t rowcell = (size_t)y * W;
     Cell<NP> c;
     load_u16<NP>(a.agg + rowcell * cs + lo, active, c);
     uint32_t best = 0xffffffffu;
@@ -624,20 +760,21 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
         best = key < best ? key : best;
     }
     best = wave_min(best);
-    const int d = (int)(best & 0xffffu);
-    float v = (float)d;
+    const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+    float v = (float)(d + s0);
     if (d > 0 && d < D - 1) {
         const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
         const int den = l - 2 * cc + r;
-        if (den > 0) v = (float)d + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+        if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
     }
+    if (lane == 0) {
+        a.disp[rowcell] = (uint16_t)(d + s0);
+        if (a.sub) a.sub[rowcell] = v;
+    }
+    if constexpr (VAR) return;  // no right-image disparity with per-pixel ranges
     const uint32_t k00 = (plane_of<NP>(c, 0) << 16);
     const uint32_t part = a.dr0[y];
-    if (lane == 0) {
-        a.disp[rowcell] = (uint16_t)d;
-        if (a.sub) a.sub[rowcell] = v;
-        a.dr[rowcell] = (uint16_t)((k00 < part ? k00 : part) & 0xffffu);
-    }
+    if (lane == 0) a.dr[rowcell] = (uint16_t)((k00 < part ? k00 : part) & 0xffffu);
     __syncthreads();
     if (valid)
         for (int x = lane; x < W; x += 64) {
@@ -929,51 +1066,81 @@ struct ReplayArgs {
     uint16_t* seeds;          // [3][W][cs]
     uint16_t* seedmins;       // [3][W]
     uint16_t* partial;        // [nvis][cs]: paths 0 + 1 + 3 of each visit
+    const uint16_t* vstart;   // per-pixel ranges [H][W] (null: uniform)
+    const uint16_t* vcap;
+    uint16_t* seedstart;      // [3][W]
+    uint16_t* seedcap;
 };
 
 // The state a path-0 / path-1 neighbour read returns, by plan label: -1 the initial fill, y >= 0 the saved
-// neighbour-column state of row y, -2-c pass +1's half-0 leftover of cell c (path 1 only).
-template <int NP, bool MASKED>
-__device__ __forceinline__ void label_state(const ReplayArgs& a, int p, int l, int lo, bool active, const Cell<NP>& mask,
-                                            Cell<NP>& prev, uint32_t& pm)
+// neighbour-column state of row y, -2-c pass +1's half-0 leftover of cell c (path 1 only). `from` receives
+// the range of the cell the state belongs to.
+template <int NP, bool MASKED, bool VAR>
+__device__ __forceinline__ void label_state(const ReplayArgs& a, int p, int l, int lo, const Cell<NP>& mask,
+                                            Cell<NP>& prev, uint32_t& pm, Range& from)
 {
     set_init(prev);
     pm = 2000;
+    from = Range();
     if (l >= 0) {
-        load_u16<NP>(a.cols + ((size_t)p * a.H + l) * a.cs + lo, active, prev);
+        load_u16<NP>(a.cols + ((size_t)p * a.H + l) * a.cs + lo, lo < a.cs, prev);
         pm = a.colmins[p * a.H + l];
+        const int ncol = a.col - a.dir;  // the neighbour column (the plan labels only real columns)
+        if (VAR && ncol >= 0 && ncol < a.W) {
+            from.s = a.vstart[(size_t)l * a.W + ncol];
+            from.c = a.vcap[(size_t)l * a.W + ncol];
+        }
     } else if (l <= -2) {
         const int c = -2 - l;
-        load_u16<NP>(a.seeds + (size_t)c * a.cs + lo, active, prev);  // seed table of path 1
+        load_u16<NP>(a.seeds + (size_t)c * a.cs + lo, lo < a.cs, prev);  // seed table of path 1
         pm = a.seedmins[c];
+        if (VAR) {
+            from.s = a.seedstart[c];
+            from.c = a.seedcap[c];
+        }
     }
-    if (MASKED) {
+    if (VAR) {
+        clip_to_cap<NP>(prev, from.c, lo / (2 * NP));
+    } else if (MASKED) {
 #pragma unroll
         for (int k = 0; k < NP; k++) prev.v[k] = (prev.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
     }
 }
 
+// Path step of a border visit from a state of another cell (range `from`) to the visited cell (start s, cap c).
+template <int NP, bool MASKED, bool VAR>
+__device__ __forceinline__ uint32_t step_from(Cell<NP> prev, Range from, uint32_t pm, int s, int c, int lane,
+                                              const Cell<NP>& raw, const ReplayArgs& a, const Cell<NP>& mask, Cell<NP>& lc)
+{
+    uint32_t el, er;
+    reframe<NP, VAR>(prev, from, s, c, lane, el, er);
+    return path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc, el, er);
+}
+
 // Paths 0, 1 and 3 of every border visit, one wavefront per visit, all in parallel: their neighbours are
 // fixed by the plan (saved neighbour-column states, pass +1 seeds, or the margin slot for path 3), so
 // they do not chain. Each visit's sum of the three path costs goes to `partial`.
-template <int NP, bool MASKED>
+template <int NP, bool MASKED, bool VAR>
 __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int i = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (i >= a.nvis) return;
     const int y = a.visits[4 * i], l0 = a.visits[4 * i + 2], l1 = a.visits[4 * i + 3];
+    const size_t cell = (size_t)y * a.W + a.col;
+    const int s = VAR ? (int)a.vstart[cell] : 0, c = VAR ? (int)a.vcap[cell] : a.cap;
     Cell<NP> mask;
     bool active;
-    make_mask<NP>(lane, a.cap, mask, active);
+    make_mask<NP>(lane, c, mask, active);
     const int lo = lane * 2 * NP;
     Cell<NP> raw, prev, lc0, lc1, lc3;
-    load_raw<NP>(a.raw + ((size_t)y * a.W + a.col) * a.cs + lo, active, raw);
+    load_raw<NP>(a.raw + cell * a.cs + lo, lo < a.cs, raw);
     uint32_t pm;
-    label_state<NP, MASKED>(a, 0, l0, lo, active, mask, prev, pm);
-    path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc0);
-    label_state<NP, MASKED>(a, 1, l1, lo, active, mask, prev, pm);
-    path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc1);
+    Range from;
+    label_state<NP, MASKED, VAR>(a, 0, l0, lo, mask, prev, pm, from);
+    step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc0);
+    label_state<NP, MASKED, VAR>(a, 1, l1, lo, mask, prev, pm, from);
+    step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc1);
     set_init(prev);
     path_step<NP, MASKED>(prev, 2000, raw, a.p1x2, a.p2, mask, lc3);
     Cell<NP> sum;
@@ -988,7 +1155,7 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 // per cell, by sgm_replay_add), so it only reads data fixed before it starts and can prefetch deeply.
 // Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's seed-table
 // entry (half 0 as the pass leaves it).
-template <int NP, bool MASKED>
+template <int NP, bool MASKED, bool VAR>
 __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
 {
     constexpr int PF = 8;
@@ -999,28 +1166,36 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
     make_mask<NP>(lane, a.cap, mask, active);
     Cell<NP> h0, h1;
     uint32_t m0 = 2000, m1 = 2000;
+    // VAR: the cells whose states the two halves hold, as start | cap << 16 (kept in registers)
+    uint32_t q0 = 0, q1 = 0;
     set_init(h0);
     set_init(h1);
     if (a.dir < 0) {
-        load_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, active, h0);  // path 2's seed
-        if (MASKED) {
+        load_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, lo < cs, h0);  // path 2's seed
+        if (VAR) {
+            q0 = (uint32_t)a.seedstart[1 * a.W + a.col] | ((uint32_t)a.seedcap[1 * a.W + a.col] << 16);
+            clip_to_cap<NP>(h0, (int)(q0 >> 16), lane);
+        } else if (MASKED) {
 #pragma unroll
             for (int k = 0; k < NP; k++) h0.v[k] = (h0.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
         }
         m0 = a.seedmins[1 * a.W + a.col];
     }
-    const int lol = active ? lo : 0;
+    const int lol = lo < cs ? lo : 0;
     const int nv = a.nvis;
     RawW<NP> rb[PF];
     Cell<NP> pb[PF];
-    int cb[PF];  // half of each prefetched visit
+    int cb[PF];          // half of each prefetched visit
+    uint32_t rs[PF];     // VAR: start | cap << 16 of each prefetched visit's cell
     if (nv > 0) {
 #pragma unroll
         for (int j = 0; j < PF; j++) {
             const int ij = min(j, nv - 1);
+            const size_t cj = (size_t)vis[4 * ij] * a.W + a.col;
             cb[j] = vis[4 * ij + 1];
-            ld_raw_w<NP>(a.raw + ((size_t)vis[4 * ij] * a.W + a.col) * cs + lol, rb[j]);
+            ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
             ld_u16<NP>(a.partial + (size_t)ij * cs + lol, pb[j]);
+            rs[j] = VAR ? ((uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16)) : 0u;
         }
     }
     auto body = [&](int i, int j) {
@@ -1028,18 +1203,42 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
         const Cell<NP> part = pb[j];
+        int s = 0, c = a.cap;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            s = (int)(r & 0xffffu);
+            c = (int)(r >> 16);
+            make_mask<NP>(lane, c, mask, active);
+        }
         const int ip = min(i + PF, nv - 1);
+        const size_t cpn = (size_t)vis[4 * ip] * a.W + a.col;
         cb[j] = vis[4 * ip + 1];
-        ld_raw_w<NP>(a.raw + ((size_t)vis[4 * ip] * a.W + a.col) * cs + lol, rb[j]);
+        ld_raw_w<NP>(a.raw + cpn * cs + lol, rb[j]);
         ld_u16<NP>(a.partial + (size_t)ip * cs + lol, pb[j]);
-        Cell<NP> lc, sum;
-        uint32_t mn;
-        if (cur) mn = path_step<NP, MASKED>(h0, m0, raw, a.p1x2, a.p2, mask, lc);
-        else mn = path_step<NP, MASKED>(h1, m1, raw, a.p1x2, a.p2, mask, lc);
+        if (VAR) rs[j] = (uint32_t)a.vstart[cpn] | ((uint32_t)a.vcap[cpn] << 16);
+        Cell<NP> lc, sum, prev;
+        uint32_t mn, pm, q = 0;
+#pragma unroll
+        for (int k = 0; k < NP; k++) prev.v[k] = cur ? h0.v[k] : h1.v[k];  // the other half's state
+        pm = cur ? m0 : m1;
+        q = cur ? q0 : q1;
+        Range from;
+        from.s = (int)(q & 0xffffu);
+        from.c = (int)(q >> 16);
+        mn = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc);
 #pragma unroll
         for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
         store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
-        if (cur) { h1 = lc; m1 = mn; } else { h0 = lc; m0 = mn; }
+        const uint32_t qn = VAR ? ((uint32_t)s | ((uint32_t)c << 16)) : 0u;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            h1.v[k] = cur ? lc.v[k] : h1.v[k];
+            h0.v[k] = cur ? h0.v[k] : lc.v[k];
+        }
+        m1 = cur ? mn : m1;
+        m0 = cur ? m0 : mn;
+        q1 = cur ? qn : q1;
+        q0 = cur ? q0 : qn;
     };
     int i0 = 0;
     for (; i0 + PF <= nv; i0 += PF) {
@@ -1051,44 +1250,67 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
         if (i0 + j < nv) body(i0 + j, j);
     if (a.dir > 0) {
         // path 2: the chain's half-0 state; paths 1 and 3: recomputed for the last visit that wrote half 0
-        store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, active, h0);
-        if (lane == 0) a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
+        store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, lo < cs, h0);
+        if (lane == 0) {
+            a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
+            if (VAR) {
+                a.seedstart[1 * a.W + a.col] = (uint16_t)(q0 & 0xffffu);
+                a.seedcap[1 * a.W + a.col] = (uint16_t)(q0 >> 16);
+            }
+        }
         int last = -1;
         for (int i = a.nvis - 1; i >= 0; i--)
             if (vis[4 * i + 1] == 0) { last = i; break; }
         Cell<NP> s1, s3;
         uint32_t sm1 = 2000, sm3 = 2000;
+        Range rl;
         set_init(s1);
         set_init(s3);
         if (last >= 0) {
             const int yl = vis[4 * last];
+            const size_t cl = (size_t)yl * a.W + a.col;
+            int s = 0, c = a.cap;
+            if (VAR) {
+                s = a.vstart[cl];
+                c = a.vcap[cl];
+                make_mask<NP>(lane, c, mask, active);
+                rl.s = s;
+                rl.c = c;
+            }
             Cell<NP> r, prev;
             uint32_t pm;
-            load_raw<NP>(a.raw + ((size_t)yl * a.W + a.col) * cs + lo, active, r);
-            label_state<NP, MASKED>(a, 1, vis[4 * last + 3], lo, active, mask, prev, pm);
-            sm1 = path_step<NP, MASKED>(prev, pm, r, a.p1x2, a.p2, mask, s1);
+            Range from;
+            load_raw<NP>(a.raw + cl * cs + lo, lo < cs, r);
+            label_state<NP, MASKED, VAR>(a, 1, vis[4 * last + 3], lo, mask, prev, pm, from);
+            sm1 = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, r, a, mask, s1);
             set_init(prev);
             sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, a.p2, mask, s3);
         }
-        store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, active, s1);
-        store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, active, s3);
+        store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, lo < cs, s1);
+        store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, lo < cs, s3);
         if (lane == 0) {
             a.seedmins[0 * a.W + a.col] = (uint16_t)sm1;
             a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
+            if (VAR) {
+                a.seedstart[0 * a.W + a.col] = (uint16_t)rl.s;
+                a.seedcap[0 * a.W + a.col] = (uint16_t)rl.c;
+                a.seedstart[2 * a.W + a.col] = (uint16_t)rl.s;
+                a.seedcap[2 * a.W + a.col] = (uint16_t)rl.c;
+            }
         }
     }
 }
 
 // Adds every border visit's four path costs to its cell (one wavefront per row of the border column;
 // `cellvis` lists each row's visits, at most two per pass).
-template <int NP>
+template <int NP, bool VAR>
 __global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int* cellvis)
 {
     const int lane = threadIdx.x & 63;
     const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
     const int lo = lane * 2 * NP;
-    const bool active = lo < a.cap;
+    const bool active = lo < (VAR ? (int)a.vcap[(size_t)y * a.W + a.col] : a.cap);
     const size_t cell = ((size_t)y * a.W + a.col) * a.cs + lo;
     Cell<NP> agg, p;
     load_u16<NP>(a.agg + cell, active, agg);
@@ -1178,6 +1400,27 @@ __global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R,
     }
 }
 
+// Jagged per-pixel arrays (cap[i] entries of pixel i at off[i], raster order: DenseLayer Raw/Agg) <-> the
+// cell-major volume with stride cs; entries of the volume past a pixel's cap get `fill`.
+template <typename T>
+__global__ void unpack_kernel(const T* src, const uint64_t* off, const uint16_t* cap, T* dst, int cs, size_t cells, T fill)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    dst[i] = j < cap[c] ? src[off[c] + j] : fill;
+}
+template <typename T>
+__global__ void pack_kernel(const T* src, const uint64_t* off, const uint16_t* cap, T* dst, int cs, size_t cells)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    if (j < cap[c]) dst[off[c] + j] = src[i];
+}
+
 template <typename T>
 __global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t cells, int n, T fill)
 {
@@ -1192,7 +1435,7 @@ __global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t c
 inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((total + block - 1) / block, 1u << 20); }
 
 // ------------------------------------------------------------------ launchers
-template <int NP, bool MASKED>
+template <int NP, bool MASKED, bool VAR>
 hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                    hipStream_t s, hipEvent_t* marks)
 {
@@ -1207,31 +1450,37 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     a.seeds = b.seeds;
     a.seedmins = b.seedmins;
     a.seed_row = b.seed_row;
+    a.vstart = b.vstart;
+    a.vcap = b.vcap;
+    a.seedstart = b.seedstart;
+    a.seedcap = b.seedcap;
     RowArgs ra;
     ra.raw = b.raw; ra.raw_out = b.raw_out; ra.cl = b.cl; ra.cr = b.cr; ra.agg = b.agg;
     ra.W = g.W; ra.H = g.H; ra.cs = g.cs; ra.cap = g.cap; ra.D = g.D;
     ra.p1x2 = a.p1x2; ra.p2 = a.p2;
     ra.disp = o.disp; ra.sub = o.sub; ra.dr = b.dr; ra.dr0 = b.dr0; ra.write_agg = o.write_agg;
+    ra.vstart = b.vstart; ra.vcap = b.vcap; ra.vcount = b.vcount;
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
     const int nh = g.H, nv = g.W, nd = g.W + g.H - 1;
     if (b.seed_row < 0) {  // pass +1 never writes half 0: every seed is the initial fill
         fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
         fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedmins, (size_t)3 * g.W, 2000);
+        if (VAR) fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedcap, (size_t)3 * g.W, 0);  // empty ranges
     }
     // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
     auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
         constexpr int DIR = decltype(dirc)::value;
-        if (NP > 2 || !o.bands) {
-            sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+        if (NP > 2 || VAR || !o.bands) {
+            sgm_sweep<NP, DIR, 1, false, MASKED, VAR><<<blocks(nd), kBlock, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 2, false, MASKED, VAR><<<blocks(nv), kBlock, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 3, false, MASKED, VAR><<<blocks(nd), kBlock, 0, s>>>(a);
             mark();
             return;
         }
-        if constexpr (NP <= 2) {
+        if constexpr (NP <= 2 && !VAR) {
             BandArgs ba;
             ba.raw = b.raw; ba.agg = b.agg; ba.W = g.W; ba.H = g.H; ba.cs = g.cs; ba.cap = g.cap;
             ba.p1x2 = a.p1x2; ba.p2 = a.p2;
@@ -1259,17 +1508,17 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             mark();  // one launch for the three paths: the next two stages are empty
             mark();
             mark();
-        } else {
-            sgm_sweep<NP, DIR, 1, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
-            sgm_sweep<NP, DIR, 2, false, MASKED><<<blocks(nv), kBlock, 0, s>>>(a);
-            sgm_sweep<NP, DIR, 3, false, MASKED><<<blocks(nd), kBlock, 0, s>>>(a);
         }
     };
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
-    if (b.cl) sgm_row_fwd<NP, MASKED, true><<<blocks(nh), kBlock, 0, s>>>(ra);
-    else sgm_row_fwd<NP, MASKED, false><<<blocks(nh), kBlock, 0, s>>>(ra);
+    if constexpr (VAR) {
+        sgm_row_fwd<NP, MASKED, false, true><<<blocks(nh), kBlock, 0, s>>>(ra);
+    } else {
+        if (b.cl) sgm_row_fwd<NP, MASKED, true, false><<<blocks(nh), kBlock, 0, s>>>(ra);
+        else sgm_row_fwd<NP, MASKED, false, false><<<blocks(nh), kBlock, 0, s>>>(ra);
+    }
     mark();
     vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
     ReplayArgs r;
@@ -1278,9 +1527,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     r.seeds = b.seeds; r.seedmins = b.seedmins;
     r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
     r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
-    if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r, r.visits);
-    sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
+    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED, VAR><<<1, 64, 0, s>>>(r, r.visits);
+    sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
     mark();
     if (agg_pass1) {
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
@@ -1291,15 +1541,15 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
     vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
-    sgm_row_bwd_wta<NP, MASKED><<<blocks(nh), kBlock, 0, s>>>(ra);
+    sgm_row_bwd_wta<NP, MASKED, VAR><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
-    if (r.nvis > 0) sgm_replay_paths<NP, MASKED><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED><<<1, 64, 0, s>>>(r, r.visits);
-    sgm_replay_add<NP><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED, VAR><<<1, 64, 0, s>>>(r, r.visits);
+    sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
     mark();
-    wta_finish<NP><<<g.H, 64, 0, s>>>(ra, o.tol, o.valid);
+    wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
     mark();
     return hipGetLastError();
 }
@@ -1362,11 +1612,15 @@ hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& 
                       hipStream_t s, hipEvent_t* m)
 {
     const bool full = g.cap == 64 * g.dpl;
+    if (b.vstart) {  // per-pixel ranges: one packed pair per lane, per-cell masks
+        if (g.dpl != 2) return hipErrorInvalidValue;
+        return run_sgm<1, true, true>(g, b, o, p1, p2, agg_pass1, s, m);
+    }
     switch (g.dpl) {
-        case 2: return full ? run_sgm<1, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<1, true>(g, b, o, p1, p2, agg_pass1, s, m);
-        case 4: return full ? run_sgm<2, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<2, true>(g, b, o, p1, p2, agg_pass1, s, m);
-        case 8: return full ? run_sgm<4, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<4, true>(g, b, o, p1, p2, agg_pass1, s, m);
-        case 16: return full ? run_sgm<8, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<8, true>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 2: return full ? run_sgm<1, false, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<1, true, false>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 4: return full ? run_sgm<2, false, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<2, true, false>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 8: return full ? run_sgm<4, false, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<4, true, false>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 16: return full ? run_sgm<8, false, false>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<8, true, false>(g, b, o, p1, p2, agg_pass1, s, m);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1375,6 +1629,22 @@ hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, i
 {
     const size_t total = (size_t)cells * dst_cs;
     restride_kernel<uint16_t><<<grid_for(total, 256), 256, 0, s>>>(src, src_cs, dst, dst_cs, (size_t)cells, n, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
+                            hipStream_t s)
+{
+    const size_t total = cells * (size_t)cs;
+    unpack_kernel<uint8_t><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(src, off, cap, dst, cs, cells, (uint8_t)255);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_u16(const uint16_t* src, const uint64_t* off, const uint16_t* cap, uint16_t* dst, int cs, size_t cells,
+                           hipStream_t s)
+{
+    const size_t total = cells * (size_t)cs;
+    pack_kernel<uint16_t><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(src, off, cap, dst, cs, cells);
     return hipGetLastError();
 }
 

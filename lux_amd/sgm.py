@@ -20,7 +20,7 @@ COST_CENSUS, COST_AD = 0, 1
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
-           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
+           "lux_sgm_compute_layer_ranges", "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_poll", "lux_sgm_server_wait",
            "lux_sgm_server_cancel", "lux_sgm_server_info_get")
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -97,6 +97,7 @@ def lib():
         L.lux_sgm_rectified.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_rectified_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_compute_layer_ranges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
         L.lux_sgm_plan_visits.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32]
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -228,15 +229,19 @@ def rectified_device(ctx, p, d_left, d_right, d_disp, d_sub, d_valid, d_agg=0, s
 
 
 class DenseLayer:
-    """Mirror of DenseLayer (DenseStereoSgm.cs:13-26) for a uniform plane range (layer-0 form of Init).
+    """Mirror of DenseLayer (DenseStereoSgm.cs:13-26).
 
-    Set ``Raw`` (uint8 [H, W, Cap]) and call ``compute()``; ``Agg`` (uint16 [H, W, Cap]), ``AggPass1``
-    (with ``KeepPass1``) and ``Wta`` (plane index start+arg, the argument of ``Depth = Planes[...]``,
-    :225-231) are filled from the device.
+    Uniform range (the layer-0 form of Init): set ``Raw`` (uint8 [H, W, Cap]) and call ``compute()``;
+    ``Agg`` (uint16 [H, W, Cap]), ``AggPass1`` (with ``KeepPass1``) and ``Wta`` (plane index start+arg, the
+    argument of ``Depth = Planes[...]``, :225-231) are filled from the device.
+    Per-pixel ranges (as RangeFromPrevious leaves them, :91-140): call ``set_ranges(start, count)`` first;
+    ``Raw`` is then the jagged volume (``Cap[i]`` bytes per pixel, concatenated in raster order) and ``Agg`` /
+    ``AggPass1`` come back the same way.
     """
 
     def __init__(self, ctx, W, H, planes, *, align=8, p1=1, p2=0, tile=4, xsw=False):
         self.ctx, self.W, self.H, self.Align, self.Tile, self.Xsw = ctx, W, H, align, tile, xsw
+        self.Planes = planes
         self.P1, self.P2 = p1, p2
         cap = ((planes + align - 1) // align) * align
         self.Start = np.zeros(W * H, np.uint16)
@@ -250,7 +255,18 @@ class DenseLayer:
         self.Threads = 1
         self.KeepPass1 = False
 
+    def set_ranges(self, start, count):
+        self.Start = np.ascontiguousarray(start, np.uint16).reshape(-1)
+        self.Count = np.ascontiguousarray(count, np.uint16).reshape(-1)
+        self.Cap = (((self.Count.astype(np.int64) + self.Align - 1) // self.Align) * self.Align).astype(np.uint16)
+        self.MinLo = int(self.Start.min())
+        self.MaxHi = int((self.Start.astype(np.int64) + self.Count).max() + self.Align - 1) // self.Align * self.Align
+        self._ranged = True
+        return self
+
     def compute(self):
+        if getattr(self, "_ranged", False):
+            return self._compute_ranges()
         cap = int(self.Cap[0])
         raw = np.ascontiguousarray(self.Raw, np.uint8).reshape(self.H, self.W, cap)
         p = LayerParams()
@@ -261,6 +277,22 @@ class DenseLayer:
         self.Wta = np.empty((self.H, self.W), np.uint16)
         _check(lib().lux_sgm_compute_layer(self.ctx.handle, ctypes.byref(p), _p(raw), _p(self.Agg), _p(self.AggPass1),
                                            _p(self.Wta)), self.ctx.handle)
+        return self
+
+    def _compute_ranges(self):
+        total = int(self.Cap.astype(np.int64).sum())
+        raw = np.ascontiguousarray(self.Raw, np.uint8).reshape(-1)
+        if raw.size != total:
+            raise ValueError(f"Raw holds {raw.size} bytes, the ranges need {total}")
+        p = LayerParams()
+        p.width, p.height, p.planes, p.align = self.W, self.H, self.Planes, self.Align
+        p.p1, p.p2, p.tile, p.xs_clamp_w = self.P1, self.P2, self.Tile, int(bool(self.Xsw))
+        self.Agg = np.empty(max(total, 1), np.uint16)[:total]
+        self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
+        self.Wta = np.empty((self.H, self.W), np.uint16)
+        _check(lib().lux_sgm_compute_layer_ranges(self.ctx.handle, ctypes.byref(p), _p(self.Start), _p(self.Count),
+                                                  _p(raw), _p(self.Agg), _p(self.AggPass1), _p(self.Wta)),
+               self.ctx.handle)
         return self
 
 

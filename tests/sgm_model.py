@@ -85,6 +85,81 @@ def label_eval(raw, T, p1, p2, xsw=False, return_pass1=False, return_visits=Fals
     return out[0] if len(out) == 1 else tuple(out)
 
 
+def step_ranged(raw, prev, pmin, sp, s, p1, p2):
+    """One path step for a cell whose planes start at s, from a predecessor vector `prev` stored for planes
+    [sp, sp + len(prev)) (2000 outside it, the line buffer after the stale cleanup)."""
+    cap = raw.size
+    full = lambda d: prev[d - sp] if 0 <= d - sp < prev.size else INIT  # noqa: E731
+    lm = np.array([full(s + j - 1) for j in range(cap)], np.int64)
+    l0 = np.array([full(s + j) for j in range(cap)], np.int64)
+    lp = np.array([full(s + j + 1) for j in range(cap)], np.int64)
+    p2p = min(p2 + pmin, 65535)
+    m = np.minimum(np.minimum(np.minimum(lm + p1, 65535), l0), np.minimum(np.minimum(lp + p1, 65535), p2p))
+    lc = np.maximum(np.minimum(raw.astype(np.int64) + m, 65535) - pmin, 0)
+    return lc, (int(lc.min()) if cap else 0xFFFF)
+
+
+def label_eval_ranges(raws, start, count, align, T, p1, p2, xsw=False):
+    """Per-pixel ranges: raws[i] is pixel i's raw vector (cap_i bytes). A slot holds, per path, the state of
+    the last visit that wrote it in that visit's own plane frame. Returns (agg list, pass-1 agg list, wta)."""
+    H, W = start.shape
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    init = (np.zeros(0, np.int64), INIT, 0)
+    slots = {}
+    agg = [np.zeros(int(caps[y, x]), np.int64) for y in range(H) for x in range(W)]
+    agg1 = None
+    last_dir = 1
+    for d, x, y, cur in schedule(W, H, T, xsw):
+        if d != last_dir:
+            agg1 = [a.copy() for a in agg]
+            last_dir = d
+        other = 1 - cur
+        slot = x + 1
+        i = y * W + x
+        s = int(start[y, x])
+        states = []
+        for p in range(4):
+            ns = slot + (0 if p == 2 else (d if p == 3 else -d))
+            half = cur if p == 0 else other
+            st = slots.get((half, ns))
+            prev, pmin, sp = init if (st is None or ns in (0, W + 1)) else st[p]
+            lc, mn = step_ranged(raws[i], prev, pmin, sp, s, p1, p2)
+            states.append((lc, mn, s))
+            agg[i] += lc
+        slots[(cur, slot)] = states
+    if agg1 is None:
+        agg1 = [a.copy() for a in agg]
+    wta = np.zeros((H, W), np.uint16)
+    for y in range(H):
+        for x in range(W):
+            a = np.minimum(agg[y * W + x], 65535)
+            c = int(count[y, x])
+            wta[y, x] = int(start[y, x]) + (int(np.argmin(a[:c])) if c else 0)
+    pack = lambda v: np.concatenate([np.minimum(a, 65535) for a in v]).astype(np.uint16)  # noqa: E731
+    return pack(agg), pack(agg1), wta
+
+
+def random_ranges(rng, W, H, planes, cmax, align, smooth=True):
+    """Per-pixel (start, count) whose padding stays inside the line-buffer segments (start - MinLo + cap <=
+    MaxHi): a smooth field with jumps of both parities, one pixel at plane 0 and one reaching the top."""
+    count = rng.integers(0, cmax + 1, (H, W)).astype(np.int64)
+    count[rng.random((H, W)) < 0.7] = cmax
+    caps = ((count + align - 1) // align) * align
+    top = planes - ((cmax + align - 1) // align) * align
+    if smooth:
+        base = np.cumsum(rng.integers(-3, 4, (H, W)), axis=1) + np.cumsum(rng.integers(-2, 3, (H, 1)), axis=0)
+        base = base - base.min()
+        start = np.clip(base, 0, top)
+    else:
+        start = rng.integers(0, top + 1, (H, W))
+    start = np.minimum(start, planes - caps)
+    start[0, 0], count[0, 0] = 0, cmax
+    ys, xs = H - 1, W - 1
+    capm = ((cmax + align - 1) // align) * align
+    start[ys, xs], count[ys, xs] = planes - capm, capm
+    return start.astype(np.uint16), count.astype(np.uint16)
+
+
 def census_bruteforce(img, radius):
     H, W = img.shape
     out = np.zeros((H, W), np.uint64)

@@ -100,3 +100,45 @@ def test_full_module_properties(ctx):
     err = np.abs(a["disp"].astype(np.int64) - truth)
     assert (err[v] <= 1).mean() > 0.95
     assert np.abs(a["sub"] - a["disp"]).max() <= 0.5
+
+
+# ------------------------------------------------------------------ per-pixel plane ranges
+import sgm_model as M  # noqa: E402
+
+RANGE_LAYERS = [(9, 7, 4, 48, 20, 8, True), (13, 5, 2, 40, 13, 8, False), (33, 12, 16, 64, 24, 8, True),
+                (10, 10, 8, 32, 9, 2, False), (1, 6, 2, 24, 7, 8, False), (70, 40, 64, 128, 40, 8, True),
+                (130, 70, 64, 256, 128, 8, True), (64, 33, 8, 200, 99, 2, False), (40, 9, 4, 96, 1, 8, True)]
+
+
+@pytest.mark.parametrize("W,H,T,planes,cmax,align,smooth", RANGE_LAYERS)
+def test_layer_ranges(W, H, T, planes, cmax, align, smooth):
+    rng = np.random.default_rng(W * 1000 + H * 10 + T + cmax)
+    start, count = M.random_ranges(rng, W, H, planes, cmax, align, smooth)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    raws = []
+    for i in range(W * H):
+        r = rng.integers(0, 49, int(caps.flat[i])).astype(np.uint8)
+        r[int(count.flat[i]):] = 255
+        raws.append(r)
+    packed = np.concatenate(raws)
+    print(f"ranges W={W} H={H} T={T} planes={planes} cmax={cmax} align={align}", flush=True)
+    with sgm.Context(0) as c:
+        for p1, p2 in ((8, 96), (2500, 3000)):
+            L = sgm.DenseLayer(c, W, H, planes, align=align, p1=p1, p2=p2, tile=T).set_ranges(start, count)
+            L.Raw = packed
+            L.KeepPass1 = True
+            L.compute()
+            agg, agg1, wta = o.layer(packed, W, H, start, count, T=T, align=align, p1=p1, p2=p2, keep_pass1=True)
+            np.testing.assert_array_equal(L.AggPass1, agg1)
+            np.testing.assert_array_equal(L.Agg, agg)
+            np.testing.assert_array_equal(L.Wta, wta)
+
+
+def test_layer_ranges_refuses_padding_overflow():
+    start = np.array([[0, 0, 0, 0], [0, 5, 5, 0], [0, 5, 5, 0]], np.uint16)
+    count = np.array([[8, 8, 8, 8], [8, 3, 3, 8], [8, 3, 3, 8]], np.uint16)
+    with sgm.Context(0) as c:
+        L = sgm.DenseLayer(c, 4, 3, 16, align=8, tile=2).set_ranges(start, count)
+        L.Raw = np.zeros(int(L.Cap.astype(np.int64).sum()), np.uint8)
+        with pytest.raises(sgm.NotSupported):
+            L.compute()

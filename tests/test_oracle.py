@@ -102,6 +102,51 @@ def test_vector_form_equals_scalar_form_variable_ranges():
     assert n >= 20
 
 
+RANGE_CASES = [(9, 7, 4, 48, 20, 8, True), (13, 5, 2, 40, 13, 8, False), (6, 11, 4, 64, 24, 8, True),
+               (10, 10, 8, 32, 9, 2, False), (5, 3, 2, 16, 5, 2, True), (1, 6, 2, 24, 7, 8, False)]
+
+
+@pytest.mark.parametrize("W,H,T,planes,cmax,align,smooth", RANGE_CASES)
+def test_variable_ranges_match_frame_model(W, H, T, planes, cmax, align, smooth):
+    """Per-pixel ranges whose padding stays inside the segments: the oracle's literal line buffer (LIdx with
+    MinLo, stale cleanup, Rd) equals the frame model (a neighbour's plane d is its stored value, else 2000)."""
+    rng = np.random.default_rng(W * 1000 + H * 10 + T)
+    start, count = M.random_ranges(rng, W, H, planes, cmax, align, smooth)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    minlo = int(start.min())
+    maxhi = int(((start.astype(np.int64) + count).max() + align - 1) // align * align)
+    assert (start.astype(np.int64) - minlo + caps <= maxhi).all()
+    raws = []
+    for i in range(W * H):
+        r = rng.integers(0, 40, int(caps.flat[i])).astype(np.uint8)
+        r[int(count.flat[i]):] = 255
+        raws.append(r)
+    packed = np.concatenate(raws) if raws else np.zeros(0, np.uint8)
+    for p1, p2 in ((8, 96), (0, 0), (3000, 3500)):
+        agg, agg1, wta = o.layer(packed, W, H, start, count, T=T, align=align, p1=p1, p2=p2, keep_pass1=True)
+        ragg, ragg1, rwta = M.label_eval_ranges(raws, start, count, align, T, p1, p2)
+        np.testing.assert_array_equal(agg1, ragg1)
+        np.testing.assert_array_equal(agg, ragg)
+        np.testing.assert_array_equal(wta, rwta)
+
+
+def test_variable_range_padding_overflow_differs_from_frame_model():
+    """The case the device path refuses: a range's padding (count 3 -> cap 8) runs past MaxHi, so the
+    reference writes into the next path's segment and the frame model no longer describes it."""
+    W, H, T, align = 4, 3, 2, 8
+    start = np.array([[0, 0, 0, 0], [0, 5, 5, 0], [0, 5, 5, 0]], np.uint16)
+    count = np.array([[8, 8, 8, 8], [8, 3, 3, 8], [8, 3, 3, 8]], np.uint16)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    assert int((start.astype(np.int64) + caps).max()) > 8  # MaxHi = 8, MinLo = 0
+    rng = np.random.default_rng(1)
+    raws = [rng.integers(0, 9, int(c)).astype(np.uint8) for c in caps.flat]
+    for r, c in zip(raws, count.flat):
+        r[int(c):] = 255
+    agg, _, _ = o.layer(np.concatenate(raws), W, H, start, count, T=T, align=align, p1=8, p2=96)
+    ragg, _, _ = M.label_eval_ranges(raws, start, count, align, T, 8, 96)
+    assert not np.array_equal(agg, ragg)
+
+
 def _f32(bits):
     return np.array([bits], np.uint32).view(np.float32)[0]
 
