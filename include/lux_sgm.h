@@ -123,6 +123,25 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
 
+/* Coarse-to-fine rectified request: `levels` levels of halved resolution (2x2 mean), the coarsest searching
+ * all of its planes, every finer level searching 2*radius planes starting at 2*d - radius (clamped) around
+ * twice the disparity d found below it. It is the rectified counterpart of the reference's layer pyramid
+ * (DenseStereoPyramid.Run, DenseStereoSgm.cs:328-353, where RangeFromPrevious :91-140 seeds each layer's
+ * per-pixel range from the previous layer). base describes the full-resolution pair; base.disparities is the
+ * plane count of the finest level (level k has base.disparities >> (levels-1-k)). 2*radius must be a
+ * multiple of base.align and at most 128. Outputs are the finest level's: disp = start + first-minimum
+ * index, sub = disp + parabola offset (each may be NULL). No left/right check. */
+typedef struct {
+    lux_sgm_rect_params base;
+    int32_t levels;      /* 1..4 */
+    int32_t radius[3];   /* levels 1 .. levels-1, coarse to fine */
+} lux_sgm_pyramid_params;
+int lux_sgm_pyramid_bytes(const lux_sgm_pyramid_params* p, uint64_t* bytes);
+int lux_sgm_pyramid(lux_sgm_ctx* ctx, const lux_sgm_pyramid_params* p, const void* left, const void* right,
+                    uint16_t* disp, float* sub);
+int lux_sgm_pyramid_device(lux_sgm_ctx* ctx, const lux_sgm_pyramid_params* p, const void* d_left, const void* d_right,
+                           uint16_t* d_disp, float* d_sub, void* stream);
+
 /* Context options (no counterpart in the reference: execution strategy only, results are identical).
  * LUX_SGM_OPT_FUSED_BANDS: 1 runs the three vertical/diagonal paths of each pass fused in band tiles that
  * synchronise through device flags (up to 256 planes); 0 (default) runs one sweep kernel per path. */

@@ -423,6 +423,57 @@ int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const u
     return lux::check_band_error(ws, s, &c->err);
 }
 
+int lux_sgm_pyramid_bytes(const lux_sgm_pyramid_params* p, uint64_t* bytes)
+{
+    if (!p || !bytes) return LUX_SGM_EINVAL;
+    std::vector<lux::BufferNeed> need;
+    const size_t imgb = (size_t)p->base.height * p->base.stride * (p->base.bits > 8 ? 2 : 1);
+    int rc = lux::pyramid_buffers(p, imgb, &need, nullptr);
+    if (rc) return rc;
+    uint64_t t = 0;
+    for (auto& b : need) t += b.bytes;
+    *bytes = t;
+    return LUX_SGM_OK;
+}
+
+int lux_sgm_pyramid_device(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void* d_left, const void* d_right,
+                           uint16_t* d_disp, float* d_sub, void* stream)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!d_left || !d_right) return fail(c, LUX_SGM_EINVAL, "null image");
+    (void)hipSetDevice(c->device);
+    return lux::enqueue_pyramid(c->ws, p, d_left, d_right, d_disp, d_sub, stream ? (hipStream_t)stream : c->stream,
+                                &c->err);
+}
+
+int lux_sgm_pyramid(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void* left, const void* right, uint16_t* disp,
+                    float* sub)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = lux::check_pyramid(p, &c->err);
+    if (rc) return rc;
+    if (!left || !right) return fail(c, LUX_SGM_EINVAL, "null image");
+    (void)hipSetDevice(c->device);
+    hipStream_t s = c->stream;
+    const size_t px = (size_t)p->base.width * p->base.height;
+    const size_t imgb = (size_t)p->base.height * p->base.stride * (p->base.bits > 8 ? 2 : 1);
+    lux::Workspace& ws = c->ws;
+    void *dl, *dr, *dd, *ds;
+    if ((rc = ws.get("img_l", imgb, &dl, &c->err)) || (rc = ws.get("img_r", imgb, &dr, &c->err)) ||
+        (rc = ws.get("out_disp", px * 2, &dd, &c->err)) || (rc = ws.get("out_sub", px * 4, &ds, &c->err)))
+        return rc;
+    hipError_t e = hipMemcpyAsync(dl, left, imgb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, right, imgb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e, "upload images");
+    if ((rc = lux::enqueue_pyramid(ws, p, dl, dr, (uint16_t*)dd, sub ? (float*)ds : nullptr, s, &c->err))) return rc;
+    if (disp) e = hipMemcpyAsync(disp, dd, px * 2, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && sub) e = hipMemcpyAsync(sub, ds, px * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "download results");
+}
+
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start,
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta)

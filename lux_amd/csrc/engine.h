@@ -76,7 +76,7 @@ struct EngineOpts {
 // The named device buffers of one request: prepare() allocates exactly these, and the server admits a request
 // by the growth they need in a slot's workspace. host_img_bytes > 0 adds the staging of host images and outputs.
 struct BufferNeed {
-    const char* name;
+    std::string name;
     size_t bytes;
 };
 std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes = 0);
@@ -100,6 +100,16 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
 // device-to-device copies only. ev (if not null) receives 2 + kSgmMarks events for the stage times.
 int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const void* dl, const void* dr, const RectOut& o,
                  const EngineOpts& opt, hipStream_t s, hipEvent_t* ev, std::string* err);
+
+// Coarse-to-fine requests (lux_sgm_pyramid_params): level geometry, checks, buffers and the enqueue.
+struct PyrLevel {
+    int W = 0, H = 0, stride = 0, D = 0, radius = 0;  // radius 0: the coarsest level, all D planes
+};
+std::vector<PyrLevel> pyramid_levels(const lux_sgm_pyramid_params* p);
+int check_pyramid(const lux_sgm_pyramid_params* p, std::string* err);
+int pyramid_buffers(const lux_sgm_pyramid_params* p, size_t host_img_bytes, std::vector<BufferNeed>* out, std::string* err);
+int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* dl, const void* dr, uint16_t* disp,
+                    float* sub, hipStream_t s, std::string* err);
 
 // After s has drained: a band tile that gave up waiting for its neighbours leaves an error word set.
 int check_band_error(Workspace& ws, hipStream_t s, std::string* err);

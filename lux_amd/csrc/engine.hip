@@ -193,7 +193,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     std::map<std::string, void*> ptr;
     for (const BufferNeed& nb : request_buffers(g, *plan, census)) {
         void* q;
-        if ((rc = ws.get(nb.name, nb.bytes, &q, err))) return rc;
+        if ((rc = ws.get(nb.name.c_str(), nb.bytes, &q, err))) return rc;
         ptr[nb.name] = q;
     }
     void *raw = ptr["raw"], *agg = ptr["agg"], *seeds = ptr["seeds"], *seedmins = ptr["seedmins"], *cols = ptr["cols"],
@@ -286,6 +286,163 @@ int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const
     if (o.agg) {
         e = launch_restride_u16(w.agg, g.cs, o.agg, g.cap, g.W * g.H, g.cap, s);
         if (e != hipSuccess) return hip_status(e, "agg copy", err);
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ coarse-to-fine
+static const char* kPyrImg[2][4] = {{"pyr_l0", "pyr_l1", "pyr_l2", "pyr_l3"}, {"pyr_r0", "pyr_r1", "pyr_r2", "pyr_r3"}};
+static const char* kPyrDisp[4] = {"pyr_d0", "pyr_d1", "pyr_d2", "pyr_d3"};
+
+std::vector<PyrLevel> pyramid_levels(const lux_sgm_pyramid_params* p)
+{
+    const int L = p->levels;
+    std::vector<PyrLevel> v(L);
+    int W = p->base.width, H = p->base.height;
+    for (int k = L - 1; k >= 0; k--) {
+        v[k].W = W;
+        v[k].H = H;
+        v[k].stride = k == L - 1 ? p->base.stride : W;
+        v[k].D = p->base.disparities >> (L - 1 - k);
+        v[k].radius = k == 0 ? 0 : p->radius[k - 1];
+        W = (W + 1) / 2;
+        H = (H + 1) / 2;
+    }
+    return v;
+}
+
+static lux_sgm_rect_params level_params(const lux_sgm_pyramid_params* p, const PyrLevel& l)
+{
+    lux_sgm_rect_params q = p->base;
+    q.width = l.W;
+    q.height = l.H;
+    q.stride = l.stride;
+    q.disparities = l.radius ? 2 * l.radius : l.D;
+    return q;
+}
+
+int check_pyramid(const lux_sgm_pyramid_params* p, std::string* err)
+{
+    if (!p) return fail(LUX_SGM_EINVAL, "null params", err);
+    int rc = check_rect(&p->base, err);
+    if (rc) return rc;
+    if (p->levels < 1 || p->levels > 4) return fail(LUX_SGM_EINVAL, "levels must be 1..4", err);
+    if (p->base.cost != LUX_SGM_COST_CENSUS) return fail(LUX_SGM_EUNSUPPORTED, "coarse-to-fine takes the census cost", err);
+    const auto lv = pyramid_levels(p);
+    if (lv[0].W < 2 || lv[0].H < 2 || lv[0].D < 1) return fail(LUX_SGM_EINVAL, "the coarsest level is empty", err);
+    for (int k = 1; k < p->levels; k++) {
+        const int r = lv[k].radius;
+        if (r <= 0 || (2 * r) % p->base.align || 2 * r > 128 || 2 * r > lv[k].D)
+            return fail(LUX_SGM_EINVAL, "each radius needs 2*radius a multiple of align, <= 128 and <= the level's planes", err);
+    }
+    return 0;
+}
+
+int pyramid_buffers(const lux_sgm_pyramid_params* p, size_t host_img_bytes, std::vector<BufferNeed>* out, std::string* err)
+{
+    int rc = check_pyramid(p, err);
+    if (rc) return rc;
+    const auto lv = pyramid_levels(p);
+    std::map<std::string, size_t> need;
+    auto put = [&](const std::string& name, size_t n) {
+        size_t& v = need[name];
+        v = std::max(v, n);
+    };
+    const size_t esz = p->base.bits > 8 ? 2 : 1;
+    for (int k = 0; k < p->levels; k++) {
+        const PyrLevel& l = lv[k];
+        const lux_sgm_rect_params q = level_params(p, l);
+        Geometry g = geometry(l.W, l.H, q.disparities, q.align);
+        std::shared_ptr<Plan> plan;
+        if ((rc = plan_for(l.W, l.H, q.tile, q.xs_clamp_w, &plan, err))) return rc;
+        for (const BufferNeed& b : request_buffers(g, *plan, true)) put(b.name, b.bytes);
+        const size_t cells = (size_t)l.W * l.H;
+        if (l.radius) {
+            put("vstart", cells * 2);
+            put("vcap", cells * 2);
+            put("vcount", cells * 2);
+            put("seedstart", (size_t)3 * l.W * 2);
+            put("seedcap", (size_t)3 * l.W * 2);
+        }
+        if (k < p->levels - 1) {
+            put(kPyrImg[0][k], cells * esz);
+            put(kPyrImg[1][k], cells * esz);
+            put(kPyrDisp[k], cells * 2);
+        }
+    }
+    if (host_img_bytes) {
+        const size_t cells = (size_t)p->base.width * p->base.height;
+        put("img_l", host_img_bytes);
+        put("img_r", host_img_bytes);
+        put("out_disp", cells * 2);
+        put("out_sub", cells * 4);
+    }
+    out->clear();
+    for (auto& kv : need) out->push_back({kv.first, kv.second});
+    return 0;
+}
+
+int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* dl, const void* dr, uint16_t* disp,
+                    float* sub, hipStream_t s, std::string* err)
+{
+    std::vector<BufferNeed> need;
+    int rc = pyramid_buffers(p, 0, &need, err);
+    if (rc) return rc;
+    // every buffer at its largest size first: nothing is reallocated once work is queued
+    std::map<std::string, void*> ptr;
+    for (const BufferNeed& b : need) {
+        void* q;
+        if ((rc = ws.get(b.name.c_str(), b.bytes, &q, err))) return rc;
+        ptr[b.name] = q;
+    }
+    const auto lv = pyramid_levels(p);
+    const int L = p->levels;
+    const bool img16 = p->base.bits > 8;
+    const void* imgs[2][4];
+    imgs[0][L - 1] = dl;
+    imgs[1][L - 1] = dr;
+    for (int k = L - 2; k >= 0; k--)
+        for (int e = 0; e < 2; e++) {
+            imgs[e][k] = ptr[kPyrImg[e][k]];
+            hipError_t h = launch_downsample2(imgs[e][k + 1], img16, lv[k + 1].W, lv[k + 1].H, lv[k + 1].stride,
+                                              (void*)imgs[e][k], lv[k].W, lv[k].H, s);
+            if (h != hipSuccess) return hip_status(h, "downsample", err);
+        }
+    for (int k = 0; k < L; k++) {
+        const PyrLevel& l = lv[k];
+        const lux_sgm_rect_params q = level_params(p, l);
+        Geometry g = geometry(l.W, l.H, q.disparities, q.align);
+        Prepared w;
+        if ((rc = prepare(ws, g, q.tile, q.xs_clamp_w, true, s, &w, err))) return rc;
+        uint16_t* ldisp = k == L - 1 ? disp : (uint16_t*)ptr[kPyrDisp[k]];
+        float* lsub = k == L - 1 ? sub : nullptr;
+        if (!l.radius) {  // the coarsest level: the whole plane range, as a rectified pair
+            RectOut o;
+            o.disp = ldisp;
+            o.sub = lsub;
+            EngineOpts opt;
+            if ((rc = enqueue_rect(ws, w, &q, imgs[0][k], imgs[1][k], o, opt, s, nullptr, err))) return rc;
+            continue;
+        }
+        uint16_t *vs = (uint16_t*)ptr["vstart"], *vc = (uint16_t*)ptr["vcap"], *vn = (uint16_t*)ptr["vcount"];
+        hipError_t h = launch_ranges_from_coarse((const uint16_t*)ptr[kPyrDisp[k - 1]], lv[k - 1].W, lv[k - 1].H, l.W, l.H,
+                                                 l.radius, l.D, vs, vc, vn, s);
+        if (h == hipSuccess) h = launch_census(imgs[0][k], img16, l.W, l.H, l.stride, q.census_radius, (uint64_t*)w.b.cl, s);
+        if (h == hipSuccess) h = launch_census(imgs[1][k], img16, l.W, l.H, l.stride, q.census_radius, (uint64_t*)w.b.cr, s);
+        if (h == hipSuccess) h = launch_cost_census_ranges(w.b.cl, w.b.cr, l.W, l.H, g.cs, vs, vn, w.raw, s);
+        if (h != hipSuccess) return hip_status(h, "level ranges and costs", err);
+        w.b.cl = w.b.cr = nullptr;
+        w.b.vstart = vs;
+        w.b.vcap = vc;
+        w.b.vcount = vn;
+        w.b.seedstart = (uint16_t*)ptr["seedstart"];
+        w.b.seedcap = (uint16_t*)ptr["seedcap"];
+        SgmOutputs so;
+        so.disp = ldisp ? ldisp : w.b.disp_tmp;
+        so.sub = lsub;
+        so.tol = 0;
+        h = launch_sgm(g, w.b, so, q.p1, q.p2, nullptr, s);
+        if (h != hipSuccess) return hip_status(h, "level aggregation", err);
     }
     return 0;
 }

@@ -77,6 +77,12 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 int band_tiles(int W, int H);
 // Repack [H][W][cs] <-> [H][W][cap] on the device.
 hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, int dst_cs, int cells, int n, hipStream_t s);
+// Coarse-to-fine levels: 2x2 mean downsampling, per-pixel ranges from the coarser disparity, range costs.
+hipError_t launch_downsample2(const void* src, bool img16, int W, int H, int stride, void* dst, int W2, int H2, hipStream_t s);
+hipError_t launch_ranges_from_coarse(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
+                                     uint16_t* vcap, uint16_t* vcount, hipStream_t s);
+hipError_t launch_cost_census_ranges(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
+                                     const uint16_t* vcount, uint8_t* raw, hipStream_t s);
 // Jagged per-pixel arrays (cap[i] entries at off[i]) <-> the cell-major volume of stride cs (255 past a cap).
 hipError_t launch_unpack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
                             hipStream_t s);

@@ -1400,6 +1400,54 @@ __global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R,
     }
 }
 
+// ------------------------------------------------------------------ coarse-to-fine levels
+// One level down: the rounded mean of each 2x2 block (the last row / column repeats on odd sizes).
+template <typename T>
+__global__ void downsample2_kernel(const T* src, int W, int H, int stride, T* dst, int W2, int H2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W2 || y >= H2) return;
+    const int x0 = min(2 * x, W - 1), x1 = min(2 * x + 1, W - 1), y0 = min(2 * y, H - 1), y1 = min(2 * y + 1, H - 1);
+    const uint32_t sum = (uint32_t)src[(size_t)y0 * stride + x0] + src[(size_t)y0 * stride + x1] +
+                         src[(size_t)y1 * stride + x0] + src[(size_t)y1 * stride + x1];
+    dst[(size_t)y * W2 + x] = (T)((sum + 2) >> 2);
+}
+
+// The plane range of every pixel of a level from the disparity of the level below (half resolution, half
+// the planes): start = clamp(2 d - r, 0, D - 2r), count = cap = 2r (a multiple of the alignment, so no
+// range's padding reaches the end of its line-buffer segment).
+__global__ void ranges_from_coarse_kernel(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
+                                          uint16_t* vcap, uint16_t* vcount)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)W * H) return;
+    const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+    const int d = dc[(size_t)min(y >> 1, Hc - 1) * Wc + min(x >> 1, Wc - 1)];
+    const int st = max(0, min(2 * d - r, D - 2 * r));
+    vstart[i] = (uint16_t)st;
+    vcap[i] = (uint16_t)(2 * r);
+    vcount[i] = (uint16_t)(2 * r);
+}
+
+// Census costs of per-pixel ranges: raw[y][x][j] = popcount(cl(x) ^ cr(max(x - (start + j), 0))) for j < count,
+// 255 past it (DenseStereo.cs:380); one thread per byte of the cell-major volume.
+__global__ void cost_census_ranges_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
+                                          const uint16_t* vcount, uint8_t* raw)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)W * H;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    const int y = (int)(c / W), x = (int)(c - (size_t)y * W);
+    uint8_t v = 255;
+    if (j < vcount[c]) {
+        const int xr = max(x - ((int)vstart[c] + j), 0);
+        v = (uint8_t)__popcll(cl[c] ^ cr[(size_t)y * W + xr]);
+    }
+    raw[i] = v;
+}
+
 // Jagged per-pixel arrays (cap[i] entries of pixel i at off[i], raster order: DenseLayer Raw/Agg) <-> the
 // cell-major volume with stride cs; entries of the volume past a pixel's cap get `fill`.
 template <typename T>
@@ -1629,6 +1677,30 @@ hipError_t launch_restride_u16(const uint16_t* src, int src_cs, uint16_t* dst, i
 {
     const size_t total = (size_t)cells * dst_cs;
     restride_kernel<uint16_t><<<grid_for(total, 256), 256, 0, s>>>(src, src_cs, dst, dst_cs, (size_t)cells, n, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample2(const void* src, bool img16, int W, int H, int stride, void* dst, int W2, int H2, hipStream_t s)
+{
+    dim3 blk(32, 8), grd((W2 + 31) / 32, (H2 + 7) / 8);
+    if (img16) downsample2_kernel<uint16_t><<<grd, blk, 0, s>>>((const uint16_t*)src, W, H, stride, (uint16_t*)dst, W2, H2);
+    else downsample2_kernel<uint8_t><<<grd, blk, 0, s>>>((const uint8_t*)src, W, H, stride, (uint8_t*)dst, W2, H2);
+    return hipGetLastError();
+}
+
+hipError_t launch_ranges_from_coarse(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
+                                     uint16_t* vcap, uint16_t* vcount, hipStream_t s)
+{
+    const size_t n = (size_t)W * H;
+    ranges_from_coarse_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dc, Wc, Hc, W, H, r, D, vstart, vcap, vcount);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_census_ranges(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
+                                     const uint16_t* vcount, uint8_t* raw, hipStream_t s)
+{
+    const size_t n = (size_t)W * H * cs;
+    cost_census_ranges_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cl, cr, W, H, cs, vstart, vcount, raw);
     return hipGetLastError();
 }
 

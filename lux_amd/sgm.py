@@ -20,7 +20,8 @@ COST_CENSUS, COST_AD = 0, 1
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
-           "lux_sgm_compute_layer_ranges", "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
+           "lux_sgm_compute_layer_ranges", "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
+           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_poll", "lux_sgm_server_wait",
            "lux_sgm_server_cancel", "lux_sgm_server_info_get")
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -66,6 +67,10 @@ class ServerInfo(ctypes.Structure):
                 ("bytes_peak", ctypes.c_uint64)]
 
 
+class PyramidParams(ctypes.Structure):
+    _fields_ = [("base", RectParams), ("levels", ctypes.c_int32), ("radius", ctypes.c_int32 * 3)]
+
+
 class LayerParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("width", "height", "planes", "align", "p1", "p2", "tile", "xs_clamp_w")]
 
@@ -103,6 +108,9 @@ def lib():
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
         L.lux_sgm_stage_times.argtypes = [vp, vp, i32]
         L.lux_sgm_set_option.argtypes = [vp, i32, i32]
+        L.lux_sgm_pyramid_bytes.argtypes = [vp, ctypes.POINTER(u64)]
+        L.lux_sgm_pyramid.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_pyramid_device.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_batch.argtypes = [vp, i32, vp, vp]
         L.lux_sgm_batch_tune.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32]
         L.lux_sgm_server_create.argtypes = [ctypes.c_int, u64, i32, ctypes.POINTER(vp)]
@@ -384,3 +392,38 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def pyramid_params(W, H, D, radii, **kw):
+    """Coarse-to-fine request: full-resolution pair W x H, D planes at the finest level, one radius per finer level."""
+    p = PyramidParams()
+    p.base = rect_params(W, H, D, **kw)
+    p.levels = len(radii) + 1
+    for i, r in enumerate(radii):
+        p.radius[i] = r
+    return p
+
+
+def pyramid_bytes(p):
+    out = ctypes.c_uint64()
+    _check(lib().lux_sgm_pyramid_bytes(ctypes.byref(p), ctypes.byref(out)))
+    return out.value
+
+
+def pyramid(ctx, left, right, D, radii, **kw):
+    """Coarse-to-fine rectified request from host arrays. Returns dict(disp, sub) of the finest level."""
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    H, W = left.shape
+    bits = kw.pop("bits", 8 if left.dtype == np.uint8 else 10)
+    p = pyramid_params(W, H, D, radii, bits=bits, **kw)
+    disp = np.empty((H, W), np.uint16)
+    sub = np.empty((H, W), np.float32)
+    _check(lib().lux_sgm_pyramid(ctx.handle, ctypes.byref(p), _p(left), _p(right), _p(disp), _p(sub)), ctx.handle)
+    return dict(disp=disp, sub=sub)
+
+
+def pyramid_device(ctx, p, d_left, d_right, d_disp, d_sub=0, stream=0):
+    _check(lib().lux_sgm_pyramid_device(ctx.handle, ctypes.byref(p), ctypes.c_void_p(d_left), ctypes.c_void_p(d_right),
+                                        ctypes.c_void_p(d_disp or None), ctypes.c_void_p(d_sub or None),
+                                        ctypes.c_void_p(stream or None)), ctx.handle)

@@ -141,3 +141,62 @@ def rectified(left, right, D, *, cost="census", radius=3, bits=10, adcap=28, ali
     if want_agg:
         out["agg"] = agg
     return out
+
+
+def downsample2(img):
+    """2x2 rounded mean, the last row / column repeated on odd sizes (one coarse-to-fine level)."""
+    H, W = img.shape
+    H2, W2 = (H + 1) // 2, (W + 1) // 2
+    ys0 = np.minimum(2 * np.arange(H2), H - 1)
+    ys1 = np.minimum(2 * np.arange(H2) + 1, H - 1)
+    xs0 = np.minimum(2 * np.arange(W2), W - 1)
+    xs1 = np.minimum(2 * np.arange(W2) + 1, W - 1)
+    a = img.astype(np.uint32)
+    s = a[ys0][:, xs0] + a[ys0][:, xs1] + a[ys1][:, xs0] + a[ys1][:, xs1]
+    return ((s + 2) >> 2).astype(img.dtype)
+
+
+def cost_census_ranges(cl, cr, start, n):
+    """raw[y, x, j] = popcount(cl(x) ^ cr(max(x - (start + j), 0))) for j < n (the census cost of a range)."""
+    H, W = cl.shape
+    xr = np.maximum(np.arange(W)[None, :, None] - (start.astype(np.int64)[:, :, None] + np.arange(n)[None, None, :]), 0)
+    crs = cr[np.arange(H)[:, None, None], xr]
+    x = np.bitwise_xor(cl[:, :, None], crs).view(np.uint8).reshape(H, W, n, 8)
+    return np.unpackbits(x, axis=3).sum(axis=3).astype(np.uint8)
+
+
+def pyramid(left, right, D, radii, *, radius=3, bits=10, align=8, p1=8, p2=96, T=64):
+    """Coarse-to-fine rectified request from the oracle's parts: the coarsest level as `rectified`, every finer
+    level as `layer` over per-pixel ranges start = clip(2 d - r, 0, D_k - 2r), count = 2r, with the census
+    cost of those planes. Returns (disp, sub) of the finest level."""
+    levels = len(radii) + 1
+    imgs = [(np.ascontiguousarray(left), np.ascontiguousarray(right))]
+    for _ in range(levels - 1):
+        imgs.insert(0, (downsample2(imgs[0][0]), downsample2(imgs[0][1])))
+    l0, r0 = imgs[0]
+    out = rectified(l0, r0, D >> (levels - 1), cost="census", radius=radius, bits=bits, align=align, p1=p1, p2=p2, T=T)
+    disp, sub = out["disp"], out["sub"]
+    for k in range(1, levels):
+        lk, rk = imgs[k]
+        H, W = lk.shape
+        Dk, r = D >> (levels - 1 - k), radii[k - 1]
+        yc = np.minimum(np.arange(H) // 2, disp.shape[0] - 1)
+        xc = np.minimum(np.arange(W) // 2, disp.shape[1] - 1)
+        dprev = disp.astype(np.int64)[yc][:, xc]
+        start = np.clip(2 * dprev - r, 0, Dk - 2 * r).astype(np.int64)
+        n = 2 * r
+        raw = cost_census_ranges(census(lk, radius), census(rk, radius), start, n)
+        count = np.full((H, W), n, np.uint16)
+        agg, _, wta = layer(raw.reshape(-1), W, H, start.astype(np.uint16), count, T=T, align=align, p1=p1, p2=p2)
+        agg = agg.reshape(H, W, n).astype(np.int64)
+        arg = wta.astype(np.int64) - start
+        disp = wta
+        sub = (start + arg).astype(np.float32)
+        a = np.take_along_axis(agg, np.clip(arg - 1, 0, n - 1)[..., None], 2)[..., 0]
+        c = np.take_along_axis(agg, arg[..., None], 2)[..., 0]
+        b = np.take_along_axis(agg, np.clip(arg + 1, 0, n - 1)[..., None], 2)[..., 0]
+        den = a - 2 * c + b
+        ok = (arg > 0) & (arg < n - 1) & (den > 0)
+        off = (np.float32(0.5) * (a - b).astype(np.float32)) / np.where(ok, den, 1).astype(np.float32)
+        sub = np.where(ok, sub + off.astype(np.float32), sub).astype(np.float32)
+    return disp, sub

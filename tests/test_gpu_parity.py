@@ -142,3 +142,18 @@ def test_layer_ranges_refuses_padding_overflow():
         L.Raw = np.zeros(int(L.Cap.astype(np.int64).sum()), np.uint8)
         with pytest.raises(sgm.NotSupported):
             L.compute()
+
+
+# ------------------------------------------------------------------ coarse-to-fine requests
+@pytest.mark.parametrize("W,H,D,radii,bits,fg,T", [(160, 120, 64, (8, 4), 10, 0.1, 16), (130, 66, 64, (8,), 8, 0.0, 8),
+                                                  (257, 131, 128, (16, 8), 10, 0.1, 64),
+                                                  (96, 80, 32, (4, 4, 4), 10, 0.0, 4)])
+def test_pyramid_matches_oracle(W, H, D, radii, bits, fg, T):
+    left, right, _ = scene.make_pair(W, H, D - 1, planes=4, sigma=2.0, seed=W + H, bits=bits, fg_frac=fg)
+    print(f"pyramid W={W} H={H} D={D} radii={radii} bits={bits} T={T}", flush=True)
+    with sgm.Context(0) as c:
+        r = sgm.pyramid(c, left, right, D, radii, bits=bits, radius=3, p1=8, p2=96, tile=T)
+    disp, sub = o.pyramid(left, right, D, list(radii), radius=3, bits=bits, p1=8, p2=96, T=T)
+    np.testing.assert_array_equal(r["disp"], disp)
+    np.testing.assert_allclose(r["sub"], sub, rtol=0, atol=SUB_TOL)
+    np.testing.assert_array_equal(r["sub"], sub)

@@ -271,3 +271,44 @@ def test_known_shift_recovered():
     out2 = o.rectified(L, R, 16, radius=2, bits=8, p1=10, p2=120, T=4, threads=3)
     for k in ("disp", "sub", "valid"):
         np.testing.assert_array_equal(out[k], out2[k])
+
+
+def test_downsample2_bruteforce():
+    rng = np.random.default_rng(4)
+    for H, W in ((5, 7), (6, 8), (1, 3), (9, 1)):
+        img = rng.integers(0, 1024, (H, W)).astype(np.uint16)
+        out = o.downsample2(img)
+        for y in range((H + 1) // 2):
+            for x in range((W + 1) // 2):
+                ys, xs = [min(2 * y, H - 1), min(2 * y + 1, H - 1)], [min(2 * x, W - 1), min(2 * x + 1, W - 1)]
+                s = sum(int(img[a, b]) for a in ys for b in xs)
+                assert out[y, x] == (s + 2) >> 2
+
+
+def test_cost_census_ranges_bruteforce():
+    rng = np.random.default_rng(5)
+    H, W, n = 4, 9, 6
+    img_l = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    img_r = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    cl, cr = o.census(img_l, 2), o.census(img_r, 2)
+    start = rng.integers(0, 12, (H, W))
+    raw = o.cost_census_ranges(cl, cr, start, n)
+    for y in range(H):
+        for x in range(W):
+            for j in range(n):
+                want = bin(int(cl[y, x]) ^ int(cr[y, max(x - (int(start[y, x]) + j), 0)])).count("1")
+                assert raw[y, x, j] == want
+
+
+def test_pyramid_recovers_a_known_shift():
+    """A pure shift by d at full resolution is d/2 one level down: every level must find it (away from the
+    left border, where the right image has no match)."""
+    rng = np.random.default_rng(6)
+    H, W, d = 48, 96, 12
+    base = rng.integers(0, 1024, (H, W + d)).astype(np.uint16)
+    left = o.box3(base, 1023)[:, :W].copy()        # left(x) = right(x - d)
+    right = o.box3(base, 1023)[:, d:].copy()
+    disp, sub = o.pyramid(left, right, 32, [4], radius=2, bits=10, T=8)
+    inner = disp[4:-4, 24:-4]
+    assert (inner == d).mean() > 0.98
+    assert np.abs(sub[4:-4, 24:-4] - d).max() < 1.0
