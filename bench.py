@@ -13,7 +13,9 @@ Two more legs go into the same line:
   `served`:  a mixed stream of depth requests (sizes uniform over 640x480 D64, 1040x780 D96, 2080x1560 D128,
              4160x3120 D256; host images in, host results out) through the request loop under a 96 GB budget:
              latency p50/p99 at Poisson arrivals of 40 req/s, and throughput at saturation against the plain
-             drop-in running the same requests in order.
+             drop-in running the same requests in order;
+  `coarse_to_fine`: 3-level requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8) in bursts of 16 through the
+             request loop under a 192 GB budget, latency p50/p99 against plain sequential execution.
 """
 import argparse
 import json
@@ -226,6 +228,63 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
             "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
 
 
+PYR_W, PYR_H, PYR_D, PYR_RADII = 4160, 3120, 256, (16, 8)
+
+
+def bench_pyramid(np, sgm, scene, n_req, burst, budget_gb, slots):
+    """Coarse-to-fine queue: 3-level requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8) arriving in
+    bursts; latency from a burst's arrival to each request's completion, served vs the same requests one after
+    another (plain sequential execution)."""
+    scenes = [scene.make_pair(PYR_W, PYR_H, PYR_D - 1, planes=6, sigma=4.0, seed=300 + j, bits=BITS, fg_frac=0.1)[:2]
+              for j in range(2)]
+    pp = sgm.pyramid_params(PYR_W, PYR_H, PYR_D, PYR_RADII, bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+    cells_per = (PYR_W // 4) * (PYR_H // 4) * (PYR_D // 4) + (PYR_W // 2) * (PYR_H // 2) * 2 * PYR_RADII[0] + \
+        PYR_W * PYR_H * 2 * PYR_RADII[1]
+    outs = [(np.empty((PYR_H, PYR_W), np.uint16), np.empty((PYR_H, PYR_W), np.float32)) for _ in range(burst)]
+    # plain sequential: each burst's requests one after another through lux_sgm_pyramid
+    ctx = sgm.Context(0)
+    sgm.pyramid(ctx, *scenes[0], PYR_D, PYR_RADII, bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+    plain_lat, svc = [], []
+    t0 = time.perf_counter()
+    for b in range(n_req // burst):
+        arrive = time.perf_counter()
+        for i in range(burst):
+            a = time.perf_counter()
+            sgm.pyramid(ctx, *scenes[(b + i) % 2], PYR_D, PYR_RADII, bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+            now = time.perf_counter()
+            svc.append((now - a) * 1e3)
+            plain_lat.append((now - arrive) * 1e3)
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+    srv = sgm.Server(0, int(budget_gb * 1e9), slots=slots)
+    for t in [srv.submit_pyramid(pp, *scenes[0]) for _ in range(slots)]:  # warm every slot
+        srv.wait(t)
+    lat = []
+    t0 = time.perf_counter()
+    for b in range(n_req // burst):
+        arrive = time.perf_counter()
+        ts = [srv.submit_pyramid(pp, *scenes[(b + i) % 2], disp=outs[i][0], sub=outs[i][1]) for i in range(burst)]
+        for t in ts:
+            srv.wait(t)
+            lat.append((time.perf_counter() - arrive) * 1e3)
+    served_s = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
+    n = (n_req // burst) * burst
+    pct = lambda v, q: round(float(np.percentile(v, q)), 3)  # noqa: E731
+    return {"workload": f"{n} coarse-to-fine requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8), "
+                        f"uint16 10-bit, census 7x7, 10% foreground, bursts of {burst}; host images in, host results out",
+            "requests": n, "burst": burst, "slots": slots, "budget_gb": budget_gb, "cells_per_request": cells_per,
+            "served": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "wall_s": round(served_s, 3),
+                       "req_per_s": round(n / served_s, 3), "gcells_s": round(n * cells_per * 8 / served_s / 1e9, 3)},
+            "plain_sequential": {"p50_ms": pct(plain_lat, 50), "p99_ms": pct(plain_lat, 99), "wall_s": round(plain_s, 3),
+                                 "req_per_s": round(n / plain_s, 3), "service_ms_p50": pct(svc, 50),
+                                 "gcells_s": round(n * cells_per * 8 / plain_s / 1e9, 3)},
+            "served_vs_plain_throughput": round(plain_s / served_s, 3),
+            "latency": "burst arrival -> request complete (results in host memory)",
+            "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,7 +293,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
-    ap.add_argument("--forms", default="batch,serve", help="extra legs: batch, serve (comma list; '' for none)")
+    ap.add_argument("--forms", default="batch,serve,pyramid",
+                    help="extra legs: batch, serve, pyramid (comma list; '' for none)")
+    ap.add_argument("--pyr-requests", type=int, default=128)
+    ap.add_argument("--pyr-burst", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -355,6 +417,11 @@ def main():
         print("bench: served leg", file=sys.stderr, flush=True)
         served = bench_served(np, sgm, scene, args.serve_requests, args.serve_rate, 96, args.serve_slots, 16)
 
+    pyramid = None
+    if "pyramid" in forms:
+        print("bench: coarse-to-fine leg", file=sys.stderr, flush=True)
+        pyramid = bench_pyramid(np, sgm, scene, args.pyr_requests, args.pyr_burst, 192, args.serve_slots)
+
     out = {
         "metric": "SGM Gcells/s (W·H·D·paths) + p50/p99 depth-request latency, served vs plain",
         "value": round(gcells, 3),
@@ -387,6 +454,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "coarse_to_fine": pyramid,
     }
     print(json.dumps(out))
 

@@ -247,6 +247,10 @@ const char* lux_sgm_server_last_error(const lux_sgm_server* srv);
  * larger than the whole budget fails here with LUX_SGM_ENOMEM. */
 int lux_sgm_server_submit(lux_sgm_server* srv, const lux_sgm_rect_params* p, int32_t mem, const void* left,
                           const void* right, uint16_t* disp, float* sub, uint8_t* valid, uint64_t* ticket);
+/* Queue one coarse-to-fine request (lux_sgm_pyramid_params); the same memory rules as lux_sgm_server_submit,
+ * outputs disp and sub of the finest level. */
+int lux_sgm_server_submit_pyramid(lux_sgm_server* srv, const lux_sgm_pyramid_params* p, int32_t mem, const void* left,
+                                  const void* right, uint16_t* disp, float* sub, uint64_t* ticket);
 /* State of a request (LUX_SGM_REQ_*), without waiting. */
 int lux_sgm_server_poll(lux_sgm_server* srv, uint64_t ticket, int32_t* state);
 /* Wait up to timeout_ms (< 0: no limit) for a request; on completion copy host outputs, fill stats (may be

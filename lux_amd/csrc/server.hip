@@ -78,7 +78,9 @@ struct PinnedPool {
 
 struct Request {
     uint64_t ticket = 0;
-    lux_sgm_rect_params p{};
+    lux_sgm_rect_params p{};       // the pair (for a coarse-to-fine request: its full-resolution level)
+    bool pyramid = false;
+    lux_sgm_pyramid_params pp{};
     int mem = LUX_SGM_MEM_HOST;
     const void* left = nullptr;   // device mode
     const void* right = nullptr;
@@ -199,17 +201,21 @@ int launch(lux_sgm_server* s, int k, Request& r)
         o.sub = r.sub;
         o.valid = r.valid;
     }
-    lux::Geometry g = lux::geometry(p.width, p.height, p.disparities, p.align);
-    lux::Prepared w;
-    if ((rc = lux::prepare(ws, g, p.tile, p.xs_clamp_w, p.cost == LUX_SGM_COST_CENSUS, st, &w, &err))) return rc;
-    lux::EngineOpts opt;
-    if ((rc = lux::enqueue_rect(ws, w, &p, dl, dr, o, opt, st, nullptr, &err))) return rc;
+    if (r.pyramid) {
+        if ((rc = lux::enqueue_pyramid(ws, &r.pp, dl, dr, o.disp, o.sub, st, &err))) return rc;
+    } else {
+        lux::Geometry g = lux::geometry(p.width, p.height, p.disparities, p.align);
+        lux::Prepared w;
+        if ((rc = lux::prepare(ws, g, p.tile, p.xs_clamp_w, p.cost == LUX_SGM_COST_CENSUS, st, &w, &err))) return rc;
+        lux::EngineOpts opt;
+        if ((rc = lux::enqueue_rect(ws, w, &p, dl, dr, o, opt, st, nullptr, &err))) return rc;
+    }
     if (r.mem == LUX_SGM_MEM_HOST) {
         char* so = (char*)r.stage_out;
         hipError_t e = hipSuccess;
         if (r.disp) e = hipMemcpyAsync(so, od, px * 2, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess && r.sub) e = hipMemcpyAsync(so + px * 2, os, px * 4, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess && r.valid) e = hipMemcpyAsync(so + px * 6, ov, px, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && r.valid && !r.pyramid) e = hipMemcpyAsync(so + px * 6, ov, px, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return lux::hip_status(e, "download results", &err);
     }
     auto* a = new CallbackArg{s, k};
@@ -341,6 +347,8 @@ void dispatcher(lux_sgm_server* s)
     }
 }
 
+int submit(lux_sgm_server* s, std::shared_ptr<Request> r, const void* left, const void* right, uint64_t* ticket);
+
 }  // namespace
 
 extern "C" {
@@ -433,7 +441,6 @@ int lux_sgm_server_submit(lux_sgm_server* s, const lux_sgm_rect_params* p, int32
     std::string err;
     int rc = lux::check_rect(p, &err);
     if (rc) return fail(s, rc, err);
-    if (!left || !right) return fail(s, LUX_SGM_EINVAL, "null image");
     if (mem != LUX_SGM_MEM_HOST && mem != LUX_SGM_MEM_DEVICE) return fail(s, LUX_SGM_EINVAL, "unknown memory kind");
     auto r = std::make_shared<Request>();
     r->p = *p;
@@ -441,14 +448,50 @@ int lux_sgm_server_submit(lux_sgm_server* s, const lux_sgm_rect_params* p, int32
     r->disp = disp;
     r->sub = sub;
     r->valid = valid;
-    const size_t px = (size_t)p->width * p->height, imgb = image_bytes(*p);
-    {
-        lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
-        std::shared_ptr<lux::Plan> plan;
-        if ((rc = lux::plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, &err))) return fail(s, rc, err);
-        r->bufs = lux::request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS, mem == LUX_SGM_MEM_HOST ? imgb : 0);
-        for (const auto& b : r->bufs) r->need += b.bytes;
+    const size_t imgb = image_bytes(*p);
+    lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
+    std::shared_ptr<lux::Plan> plan;
+    if ((rc = lux::plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, &err))) return fail(s, rc, err);
+    r->bufs = lux::request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS, mem == LUX_SGM_MEM_HOST ? imgb : 0);
+    return submit(s, r, left, right, ticket);
+}
+
+int lux_sgm_server_submit_pyramid(lux_sgm_server* s, const lux_sgm_pyramid_params* p, int32_t mem, const void* left,
+                                  const void* right, uint16_t* disp, float* sub, uint64_t* ticket)
+{
+    if (!s || !ticket) return LUX_SGM_EINVAL;
+    std::string err;
+    int rc = lux::check_pyramid(p, &err);
+    if (rc) return fail(s, rc, err);
+    if (mem != LUX_SGM_MEM_HOST && mem != LUX_SGM_MEM_DEVICE) return fail(s, LUX_SGM_EINVAL, "unknown memory kind");
+    auto r = std::make_shared<Request>();
+    r->p = p->base;
+    r->pyramid = true;
+    r->pp = *p;
+    r->mem = mem;
+    r->disp = disp;
+    r->sub = sub;
+    if ((rc = lux::pyramid_buffers(p, mem == LUX_SGM_MEM_HOST ? image_bytes(p->base) : 0, &r->bufs, &err)))
+        return fail(s, rc, err);
+    if (mem == LUX_SGM_MEM_HOST) {  // the host form stages outputs in out_disp / out_sub (valid unused)
+        const size_t px = (size_t)p->base.width * p->base.height;
+        r->bufs.push_back({"out_valid", px});
     }
+    return submit(s, r, left, right, ticket);
+}
+
+}  // extern "C"
+
+namespace {
+
+int submit(lux_sgm_server* s, std::shared_ptr<Request> r, const void* left, const void* right, uint64_t* ticket)
+{
+    int rc = 0;
+    const lux_sgm_rect_params* p = &r->p;
+    const int mem = r->mem;
+    if (!left || !right) return fail(s, LUX_SGM_EINVAL, "null image");
+    const size_t px = (size_t)p->width * p->height, imgb = image_bytes(*p);
+    for (const auto& b : r->bufs) r->need += b.bytes;
     if (s->budget.limit && r->need > s->budget.limit)
         return fail(s, LUX_SGM_ENOMEM, "request larger than the server's HBM budget");
     if (mem == LUX_SGM_MEM_HOST) {
@@ -477,8 +520,13 @@ int lux_sgm_server_submit(lux_sgm_server* s, const lux_sgm_rect_params* p, int32
         *ticket = r->ticket;
     }
     s->wake.notify_all();
+    (void)rc;
     return LUX_SGM_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int lux_sgm_server_poll(lux_sgm_server* s, uint64_t ticket, int32_t* state)
 {

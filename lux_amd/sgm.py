@@ -22,7 +22,8 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
            "lux_sgm_compute_layer_ranges", "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
-           "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_poll", "lux_sgm_server_wait",
+           "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
+           "lux_sgm_server_wait",
            "lux_sgm_server_cancel", "lux_sgm_server_info_get")
 MEM_HOST, MEM_DEVICE = 0, 1
 REQ_PENDING, REQ_RUNNING, REQ_DONE = 0, 1, 2
@@ -118,6 +119,7 @@ def lib():
         L.lux_sgm_server_last_error.argtypes = [vp]
         L.lux_sgm_server_last_error.restype = ctypes.c_char_p
         L.lux_sgm_server_submit.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, ctypes.POINTER(u64)]
+        L.lux_sgm_server_submit_pyramid.argtypes = [vp, vp, i32, vp, vp, vp, vp, ctypes.POINTER(u64)]
         L.lux_sgm_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
         L.lux_sgm_server_wait.argtypes = [vp, u64, i32, vp]
         L.lux_sgm_server_cancel.argtypes = [vp, u64]
@@ -352,6 +354,21 @@ class Server:
         vp = lambda x: ctypes.c_void_p(x or None)  # noqa: E731
         _check_srv(lib().lux_sgm_server_submit(self.handle, ctypes.byref(p), MEM_DEVICE, vp(d_left), vp(d_right),
                                                vp(d_disp), vp(d_sub), vp(d_valid), ctypes.byref(t)), self.handle)
+        return t.value
+
+    def submit_pyramid(self, pp, left, right, *, disp=None, sub=None):
+        """A coarse-to-fine request from host arrays (PyramidParams from pyramid_params)."""
+        t = ctypes.c_uint64()
+        _check_srv(lib().lux_sgm_server_submit_pyramid(self.handle, ctypes.byref(pp), MEM_HOST, _p(left), _p(right),
+                                                       _p(disp), _p(sub), ctypes.byref(t)), self.handle)
+        self._keep[t.value] = (disp, sub)
+        return t.value
+
+    def submit_pyramid_device(self, pp, d_left, d_right, d_disp=0, d_sub=0):
+        t = ctypes.c_uint64()
+        vp = lambda x: ctypes.c_void_p(x or None)  # noqa: E731
+        _check_srv(lib().lux_sgm_server_submit_pyramid(self.handle, ctypes.byref(pp), MEM_DEVICE, vp(d_left),
+                                                       vp(d_right), vp(d_disp), vp(d_sub), ctypes.byref(t)), self.handle)
         return t.value
 
     def poll(self, ticket):

@@ -201,3 +201,32 @@ def test_server_cancel_pending(cases):
             assert e.value.code == sgm.ECANCELED
         else:
             srv.wait(last, timeout_ms=60000)
+
+
+def test_server_pyramids_mixed_with_pairs(cases):
+    """Coarse-to-fine requests in the same queue as plain pairs: each equals the same request run alone."""
+    pyr = []
+    for i, (W, H, D, radii) in enumerate([(160, 120, 64, (8, 4)), (130, 66, 64, (8,)), (96, 80, 32, (4, 4))]):
+        left, right, _ = scene.make_pair(W, H, D - 1, planes=4, sigma=2.0, seed=50 + i, bits=10, fg_frac=0.1)
+        with sgm.Context(0) as c:
+            alone = sgm.pyramid(c, left, right, D, radii, bits=10, p1=8, p2=96, tile=16)
+        pp = sgm.pyramid_params(W, H, D, radii, bits=10, p1=8, p2=96, tile=16)
+        pyr.append((pp, left, right, alone))
+    with sgm.Server(0, 0, slots=3) as srv:
+        tickets, outs = [], []
+        for rep in range(2):
+            for k, (pp, left, right, alone) in enumerate(pyr):
+                d = np.full(left.shape, 0xffff, np.uint16)
+                s = np.full(left.shape, -1, np.float32)
+                tickets.append(srv.submit_pyramid(pp, left, right, disp=d, sub=s))
+                outs.append((k, d, s))
+                c = cases[(rep + k) % len(cases)]
+                tickets.append(srv.submit(c["p"], c["left"], c["right"]))
+                outs.append(None)
+        for t, o_ in zip(tickets, outs):
+            srv.wait(t, timeout_ms=60000)
+            if o_ is not None:
+                k, d, s = o_
+                np.testing.assert_array_equal(d, pyr[k][3]["disp"])
+                np.testing.assert_array_equal(s, pyr[k][3]["sub"])
+        assert srv.info()["failed"] == 0
