@@ -123,6 +123,22 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
 
+/* Image-adaptive P2 (DenseStereoSgm.cs:267-274): every path step takes P2 = (int)(P1 * p2_scale * E),
+ * E = fast exp2 of -S, S = (|d0| + |d2|) + |d1|, d_c = (g_c - q_c) * weights[c], g the RGBA guidance of the cell
+ * and q that of the cell whose line-buffer state the step reads (0 where nothing was written). */
+typedef struct {
+    const uint8_t* guidance;   /* [H*W*4] RGBA8 at layer resolution (DenseLayer.Guidance, DenseStereo.cs:163-177) */
+    float weights[4];          /* StereoParams.Guidance (DenseStereo.cs:17-21) */
+    float p2_scale;            /* StereoParams.P2Scale */
+} lux_sgm_guidance;
+
+/* DenseLayer.Compute in full: per-pixel ranges when start/count are given (else the uniform range of p),
+ * adaptive P2 when g is given (else the constant p->p2). raw/agg/agg_pass1 are jagged with ranges, [H*W*cap]
+ * without; wta as in the two calls above. */
+int lux_sgm_compute_layer_ex(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
+                             const uint16_t* count, const lux_sgm_guidance* g, const uint8_t* raw, uint16_t* agg,
+                             uint16_t* agg_pass1, uint16_t* wta);
+
 /* Coarse-to-fine rectified request: `levels` levels of halved resolution (2x2 mean), the coarsest searching
  * all of its planes, every finer level searching 2*radius planes starting at 2*d - radius (clamped) around
  * twice the disparity d found below it. It is the rectified counterpart of the reference's layer pyramid

@@ -376,53 +376,6 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
     return lux::check_band_error(ws, s, &c->err);
 }
 
-int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
-                          uint16_t* agg_pass1, uint16_t* wta)
-{
-    if (!c) return LUX_SGM_EINVAL;
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (!raw) return fail(c, LUX_SGM_EINVAL, "null argument");
-    int rc = lux::check_layer(p, &c->err);
-    if (rc) return rc;
-    (void)hipSetDevice(c->device);
-    hipStream_t s = c->stream;
-    lux::Geometry g = lux::geometry(p->width, p->height, p->planes, p->align);
-    lux::Workspace& ws = c->ws;
-    lux::Prepared w;
-    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err))) return rc;
-    const size_t cells = (size_t)g.W * g.H;
-    void *stage, *a1 = nullptr, *dw = nullptr;
-    if ((rc = ws.get("layer_stage", cells * g.cap * 2, &stage, &c->err))) return rc;
-    if (agg_pass1 && (rc = ws.get("layer_pass1", cells * g.cs * 2, &a1, &c->err))) return rc;
-    if ((rc = ws.get("layer_wta", cells * 2, &dw, &c->err))) return rc;
-    hipError_t e = hipMemcpyAsync(stage, raw, cells * g.cap, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = lux::launch_restride_u8((const uint8_t*)stage, g.cap, w.raw, g.cs, (int)cells, g.cap, 255, s);
-    if (e != hipSuccess) return hip_fail(c, e, "upload raw");
-    lux::SgmOutputs o;
-    o.disp = (uint16_t*)dw;
-    o.tol = 0;
-    o.write_agg = agg != nullptr;
-    o.bands = c->opt_bands;
-    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
-    if (e != hipSuccess) return hip_fail(c, e, "aggregation");
-    if (wta) {
-        e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return hip_fail(c, e, "wta");
-    }
-    uint16_t* outs[2] = {agg, agg_pass1};
-    const uint16_t* srcs[2] = {w.agg, (const uint16_t*)a1};
-    for (int k = 0; k < 2; k++) {
-        if (!outs[k]) continue;
-        e = lux::launch_restride_u16(srcs[k], g.cs, (uint16_t*)stage, g.cap, (int)cells, g.cap, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(outs[k], stage, cells * g.cap * 2, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(c, e, "download aggregate");
-    }
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
-    return lux::check_band_error(ws, s, &c->err);
-}
-
 int lux_sgm_pyramid_bytes(const lux_sgm_pyramid_params* p, uint64_t* bytes)
 {
     if (!p || !bytes) return LUX_SGM_EINVAL;
@@ -474,74 +427,123 @@ int lux_sgm_pyramid(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void*
     return e == hipSuccess ? 0 : hip_fail(c, e, "download results");
 }
 
+int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
+                          uint16_t* agg_pass1, uint16_t* wta)
+{
+    return lux_sgm_compute_layer_ex(c, p, nullptr, nullptr, nullptr, raw, agg, agg_pass1, wta);
+}
+
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start,
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta)
 {
+    if (!start || !count) return fail(c, LUX_SGM_EINVAL, "null ranges");
+    return lux_sgm_compute_layer_ex(c, p, start, count, nullptr, raw, agg, agg_pass1, wta);
+}
+
+int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start, const uint16_t* count,
+                             const lux_sgm_guidance* gd, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
+                             uint16_t* wta)
+{
     if (!c) return LUX_SGM_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!raw || !start || !count) return fail(c, LUX_SGM_EINVAL, "null argument");
+    if (!raw) return fail(c, LUX_SGM_EINVAL, "null argument");
+    if ((start == nullptr) != (count == nullptr)) return fail(c, LUX_SGM_EINVAL, "start and count go together");
+    if (gd && !gd->guidance) return fail(c, LUX_SGM_EINVAL, "null guidance image");
     int rc = lux::check_layer(p, &c->err);
     if (rc) return rc;
+    const bool ranged = start != nullptr;
     const int W = p->width, H = p->height, align = p->align;
     const size_t cells = (size_t)W * H;
     // ranges as DenseLayer.Init leaves them (:53-58): cap = roundup(count, align); MinLo = min start,
-    // MaxHi = roundup(max(start + count), align) (:46, :51, :136)
-    std::vector<uint16_t> cap(cells);
-    std::vector<uint64_t> off(cells);
+    // MaxHi = roundup(max(start + count), align) (:46, :51, :136). Uniform: start 0, count = planes.
+    std::vector<uint16_t> cap;
+    std::vector<uint64_t> off;
     uint64_t total = 0;
-    int minlo = 1 << 30, maxhi = 0, maxcap = 0, maxcount = 0;
-    for (size_t i = 0; i < cells; i++) {
-        if ((int)start[i] + (int)count[i] > p->planes) return fail(c, LUX_SGM_EINVAL, "a range passes the last plane");
-        const int cp = ((count[i] + align - 1) / align) * align;
-        cap[i] = (uint16_t)cp;
-        off[i] = total;
-        total += (uint64_t)cp;
-        minlo = std::min(minlo, (int)start[i]);
-        maxhi = std::max(maxhi, (int)start[i] + (int)count[i]);
-        maxcap = std::max(maxcap, cp);
-        maxcount = std::max(maxcount, (int)count[i]);
+    int maxcount = p->planes;
+    if (ranged) {
+        cap.resize(cells);
+        off.resize(cells);
+        int minlo = 1 << 30, maxhi = 0, maxcap = 0;
+        maxcount = 0;
+        for (size_t i = 0; i < cells; i++) {
+            if ((int)start[i] + (int)count[i] > p->planes) return fail(c, LUX_SGM_EINVAL, "a range passes the last plane");
+            const int cp = ((count[i] + align - 1) / align) * align;
+            cap[i] = (uint16_t)cp;
+            off[i] = total;
+            total += (uint64_t)cp;
+            minlo = std::min(minlo, (int)start[i]);
+            maxhi = std::max(maxhi, (int)start[i] + (int)count[i]);
+            maxcap = std::max(maxcap, cp);
+            maxcount = std::max(maxcount, (int)count[i]);
+        }
+        maxhi = ((maxhi + align - 1) / align) * align;
+        if (maxcap == 0) return fail(c, LUX_SGM_EINVAL, "every range is empty");
+        if (maxcap > 128) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 128 lanes");
+        for (size_t i = 0; i < cells; i++)
+            if ((int)start[i] - minlo + (int)cap[i] > maxhi)
+                return fail(c, LUX_SGM_EUNSUPPORTED,
+                            "a range's padding reaches the end of its line-buffer segment (start - MinLo + cap > MaxHi): "
+                            "the reference then writes into the next path's segment, which the device path does not model");
     }
-    maxhi = ((maxhi + align - 1) / align) * align;
-    if (maxcap == 0) return fail(c, LUX_SGM_EINVAL, "every range is empty");
-    if (maxcap > 128) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 128 lanes");
-    for (size_t i = 0; i < cells; i++)
-        if ((int)start[i] - minlo + (int)cap[i] > maxhi)
-            return fail(c, LUX_SGM_EUNSUPPORTED,
-                        "a range's padding reaches the end of its line-buffer segment (start - MinLo + cap > MaxHi): "
-                        "the reference then writes into the next path's segment, which the device path does not model");
     (void)hipSetDevice(c->device);
     hipStream_t s = c->stream;
     lux::Geometry g = lux::geometry(W, H, maxcount, align);
+    if (!ranged) total = cells * g.cap;
     lux::Workspace& ws = c->ws;
     lux::Prepared w;
     if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err))) return rc;
-    void *vs, *vc, *vn, *ss, *sc, *vo, *stage, *a1 = nullptr, *dw;
-    if ((rc = ws.get("vstart", cells * 2, &vs, &c->err)) || (rc = ws.get("vcap", cells * 2, &vc, &c->err)) ||
-        (rc = ws.get("vcount", cells * 2, &vn, &c->err)) || (rc = ws.get("seedstart", (size_t)3 * W * 2, &ss, &c->err)) ||
-        (rc = ws.get("seedcap", (size_t)3 * W * 2, &sc, &c->err)) || (rc = ws.get("voff", cells * 8, &vo, &c->err)) ||
-        (rc = ws.get("layer_stage", std::max<size_t>(total * 2, 2), &stage, &c->err)) ||
+    void *stage, *a1 = nullptr, *dw;
+    if ((rc = ws.get("layer_stage", std::max<size_t>(total * 2, 2), &stage, &c->err)) ||
         (rc = ws.get("layer_wta", cells * 2, &dw, &c->err)))
         return rc;
     if (agg_pass1 && (rc = ws.get("layer_pass1", cells * g.cs * 2, &a1, &c->err))) return rc;
-    hipError_t e = hipMemcpyAsync(vs, start, cells * 2, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(vc, cap.data(), cells * 2, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(vn, count, cells * 2, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(vo, off.data(), cells * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && total) e = hipMemcpyAsync(stage, raw, total, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = lux::launch_unpack_u8((const uint8_t*)stage, (const uint64_t*)vo, (const uint16_t*)vc, w.raw, g.cs, cells, s);
-    if (e != hipSuccess) return hip_fail(c, e, "upload ranges and raw");
-    w.b.vstart = (const uint16_t*)vs;
-    w.b.vcap = (const uint16_t*)vc;
-    w.b.vcount = (const uint16_t*)vn;
-    w.b.seedstart = (uint16_t*)ss;
-    w.b.seedcap = (uint16_t*)sc;
+    hipError_t e = hipSuccess;
+    void* vo = nullptr;
+    if (ranged) {
+        void *vs, *vc, *vn, *ss, *sc;
+        if ((rc = ws.get("vstart", cells * 2, &vs, &c->err)) || (rc = ws.get("vcap", cells * 2, &vc, &c->err)) ||
+            (rc = ws.get("vcount", cells * 2, &vn, &c->err)) || (rc = ws.get("seedstart", (size_t)3 * W * 2, &ss, &c->err)) ||
+            (rc = ws.get("seedcap", (size_t)3 * W * 2, &sc, &c->err)) || (rc = ws.get("voff", cells * 8, &vo, &c->err)))
+            return rc;
+        e = hipMemcpyAsync(vs, start, cells * 2, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(vc, cap.data(), cells * 2, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(vn, count, cells * 2, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(vo, off.data(), cells * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && total) e = hipMemcpyAsync(stage, raw, total, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = lux::launch_unpack_u8((const uint8_t*)stage, (const uint64_t*)vo, (const uint16_t*)vc, w.raw, g.cs, cells, s);
+        w.b.vstart = (const uint16_t*)vs;
+        w.b.vcap = (const uint16_t*)vc;
+        w.b.vcount = (const uint16_t*)vn;
+        w.b.seedstart = (uint16_t*)ss;
+        w.b.seedcap = (uint16_t*)sc;
+    } else {
+        e = hipMemcpyAsync(stage, raw, cells * g.cap, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = lux::launch_restride_u8((const uint8_t*)stage, g.cap, w.raw, g.cs, (int)cells, g.cap, 255, s);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "upload raw");
+    if (gd) {  // adaptive P2: the guidance image and the seed table's guidance
+        void *gi, *sg;
+        if ((rc = ws.get("guidance", cells * 4, &gi, &c->err)) ||
+            (rc = ws.get("seedguid", (size_t)3 * W * 4, &sg, &c->err)))
+            return rc;
+        e = hipMemcpyAsync(gi, gd->guidance, cells * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(c, e, "upload guidance");
+        w.b.ap.guid = (const uint32_t*)gi;
+        w.b.ap.w0 = gd->weights[0];
+        w.b.ap.w1 = gd->weights[1];
+        w.b.ap.w2 = gd->weights[2];
+        w.b.ap.base = (float)p->p1 * gd->p2_scale;  // p2Base (:253), one float multiply
+        w.b.ap.on = 1;
+        w.b.seedguid = (uint32_t*)sg;
+    }
     lux::SgmOutputs o;
     o.disp = (uint16_t*)dw;
     o.tol = 0;
     o.write_agg = agg != nullptr;
-    o.bands = 0;
+    o.bands = ranged || gd ? 0 : c->opt_bands;
     e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
     if (e != hipSuccess) return hip_fail(c, e, "aggregation");
     if (wta) {
@@ -552,14 +554,15 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, 
     const uint16_t* srcs[2] = {w.agg, (const uint16_t*)a1};
     for (int k = 0; k < 2; k++) {
         if (!outs[k] || !total) continue;
-        e = lux::launch_pack_u16(srcs[k], (const uint64_t*)vo, (const uint16_t*)vc, (uint16_t*)stage, g.cs, cells, s);
+        if (ranged) e = lux::launch_pack_u16(srcs[k], (const uint64_t*)vo, w.b.vcap, (uint16_t*)stage, g.cs, cells, s);
+        else e = lux::launch_restride_u16(srcs[k], g.cs, (uint16_t*)stage, g.cap, (int)cells, g.cap, s);
         if (e == hipSuccess) e = hipMemcpyAsync(outs[k], stage, total * 2, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_fail(c, e, "download aggregate");
     }
     e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(c, e, "compute_layer_ranges");
-    return 0;
+    if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
+    return lux::check_band_error(ws, s, &c->err);
 }
 
 int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)

@@ -253,6 +253,9 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
     w->b.seed_row = plan->seed_row;
     w->b.disp_tmp = (uint16_t*)dtmp;
+    // constant P2: the kernels' guidance loads read word 0 of an allocation that always exists
+    w->b.ap = AdaptP2();
+    w->b.ap.guid = (const uint32_t*)sync;
     return 0;
 }
 

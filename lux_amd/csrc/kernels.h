@@ -13,6 +13,17 @@ struct Geometry {
 };
 Geometry geometry(int W, int H, int D, int align);
 
+// Image-adaptive P2 (DenseStereoSgm.cs:267-274): per path step, P2 = (int)(base * fastexp2(-S)) with
+// S = (|d0| + |d2|) + |d1|, d_c = (g_c - q_c) * w_c, g the cell's guidance and q the guidance of the cell whose
+// state the step reads (0 for the initial fill). guid: [H][W] RGBA8 as u32 (never null: a one-word dummy
+// when off, so loads need no select).
+struct AdaptP2 {
+    const uint32_t* guid = nullptr;
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f;
+    float base = 0.f;  // (float)P1 * P2Scale
+    int on = 0;
+};
+
 struct SgmBuffers {
     const uint8_t* raw = nullptr;   // [H][W][cs]
     uint8_t* raw_out = nullptr;     // same buffer, written by the first sweep when the census cost is fused
@@ -48,6 +59,8 @@ struct SgmBuffers {
     const uint16_t* vcount = nullptr;
     uint16_t* seedstart = nullptr;  // [3][W]
     uint16_t* seedcap = nullptr;
+    AdaptP2 ap;                     // adaptive P2 (ap.on = 0: every step uses the constant P2)
+    uint32_t* seedguid = nullptr;   // [3][W] guidance of the cell behind each seed-table entry (adaptive P2)
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
 };
 

@@ -1,0 +1,1710 @@
+// gfx950 kernels of the dense-stereo SGM hot path (DenseStereoSgm.cs:181-232 and its cost front end): the
+// device code and the launch sequence of one request, included by the translation units that instantiate it
+// (kernels.hip: constant P2; kernels_adp.hip: image-adaptive P2), so the two compile in parallel.
+//
+// Layout: one cell = the cs planes of one pixel, contiguous (cell-major [y][x][cs]); a 64-lane wavefront
+// owns one cell per step, lane l holding planes [l*DPL, (l+1)*DPL) as DPL/2 packed u16 pairs. Every path
+// step is then one coalesced cs-byte raw load and a cs*2-byte aggregate read-modify-write, neighbour planes
+// d-1 / d+1 come from v_alignbit inside a lane and DPP wave_shr/wave_shl across lanes, and the per-step
+// minimum over planes is a DPP row_shr / row_bcast reduction. All arithmetic is the reference's exact u16
+// saturating form (DenseStereoSgm.cs:288-298) on v_pk_add_u16/v_pk_sub_u16 with clamp and v_pk_min_u16.
+#pragma once
+#include "kernels.h"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace lux {
+
+namespace {
+
+constexpr uint32_t kInit2 = 0x07d007d0u;  // 2000 in both halves: the line buffers' fill value (:161-162)
+constexpr int kBlock = 256;                // four wavefronts per workgroup
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_add_sat(as2(a), as2(b))); }
+__device__ __forceinline__ uint32_t psub(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_sub_sat(as2(a), as2(b))); }
+__device__ __forceinline__ uint32_t pmin(uint32_t a, uint32_t b) { return as1(__builtin_elementwise_min(as2(a), as2(b))); }
+
+// DPP cross-lane moves. `old` is what a lane without a source lane receives.
+__device__ __forceinline__ uint32_t shr1(uint32_t v, uint32_t old)  // lane i <- lane i-1 (wave_shr:1)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t shl1(uint32_t v, uint32_t old)  // lane i <- lane i+1 (wave_shl:1)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v)
+{
+    uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROWMASK, 0xf, false);
+    return v < o ? v : o;
+}
+// Minimum of a u32 over the 64 lanes, returned wave-uniform.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+    v = dpp_min<0x111, 0xf>(v);  // row_shr:1
+    v = dpp_min<0x112, 0xf>(v);  // row_shr:2
+    v = dpp_min<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_min<0x118, 0xf>(v);  // row_shr:8  -> lane 15 of each row holds the row minimum
+    v = dpp_min<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_min<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3 -> lane 63 holds the minimum
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// ------------------------------------------------------------------ lane-local cell data
+template <int NP>
+struct Cell {
+    uint32_t v[NP];
+};
+
+// Planes [lane*2*NP, +2*NP) of a cell: raw bytes widened to packed u16 pairs.
+template <int NP>
+__device__ __forceinline__ void load_raw(const uint8_t* p, bool active, Cell<NP>& r)
+{
+    if constexpr (NP == 1) {
+        uint32_t b = active ? (uint32_t)(*(const uint16_t*)p) : 0u;
+        r.v[0] = __builtin_amdgcn_perm(0u, b, 0x0c010c00u);
+    } else {
+        uint32_t w[NP / 2];
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) w[k] = active ? ((const uint32_t*)p)[k] : 0u;
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) {
+            r.v[2 * k] = __builtin_amdgcn_perm(0u, w[k], 0x0c010c00u);
+            r.v[2 * k + 1] = __builtin_amdgcn_perm(0u, w[k], 0x0c030c02u);
+        }
+    }
+}
+// The same planes as packed bytes, loaded without widening: a prefetch that stays in flight until use.
+template <int NP>
+struct RawW {
+    uint32_t w[NP == 1 ? 1 : NP / 2];
+};
+template <int NP>
+__device__ __forceinline__ void load_raw_w(const uint8_t* p, bool active, RawW<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.w[0] = active ? (uint32_t)(*(const uint16_t*)p) : 0u;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) r.w[k] = active ? ((const uint32_t*)p)[k] : 0u;
+    }
+}
+// Unconditional forms for prefetch rings: the caller passes an address that is always in bounds (idle lanes
+// read lane 0's planes), so the load needs no select that would wait for it.
+template <int NP>
+__device__ __forceinline__ void ld_raw_w(const uint8_t* p, RawW<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.w[0] = (uint32_t)(*(const uint16_t*)p);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) r.w[k] = ((const uint32_t*)p)[k];
+    }
+}
+template <int NP>
+__device__ __forceinline__ void ld_u16(const uint16_t* p, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) r.v[k] = ((const uint32_t*)p)[k];
+}
+// Agent-scope relaxed atomic accesses (global_load / global_store with sc1): the band-boundary states go
+// through these so they are coherent across XCDs without a full L2 writeback / invalidate per tile.
+template <int NP>
+__device__ __forceinline__ void store_u16_agent(uint16_t* p, bool active, const Cell<NP>& r)
+{
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+            __hip_atomic_store((uint32_t*)p + k, r.v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+template <int NP>
+__device__ __forceinline__ void load_u16_agent(const uint16_t* p, bool active, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+        r.v[k] = active ? __hip_atomic_load((const uint32_t*)p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+}
+template <int NP>
+__device__ __forceinline__ void widen(const RawW<NP>& b, Cell<NP>& r)
+{
+    if constexpr (NP == 1) {
+        r.v[0] = __builtin_amdgcn_perm(0u, b.w[0], 0x0c010c00u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) {
+            r.v[2 * k] = __builtin_amdgcn_perm(0u, b.w[k], 0x0c010c00u);
+            r.v[2 * k + 1] = __builtin_amdgcn_perm(0u, b.w[k], 0x0c030c02u);
+        }
+    }
+}
+template <int NP>
+__device__ __forceinline__ void load_u16(const uint16_t* p, bool active, Cell<NP>& r)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) r.v[k] = active ? ((const uint32_t*)p)[k] : 0u;
+}
+template <int NP>
+__device__ __forceinline__ void store_u16(uint16_t* p, bool active, const Cell<NP>& r)
+{
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) ((uint32_t*)p)[k] = r.v[k];
+    }
+}
+
+// One path step of DenseLayer.Pixel (:276-314) for the planes this lane owns.
+//   m  = min(min(sat(Lm+P1), L0), min(sat(Lp+P1), P2p)),  P2p = min(P2 + minPrev, 65535)
+//   lc = satsub(sat(raw + m), minPrev)
+// Lm/Lp of the range ends read the guard words (2000). min(sat(a+P1), sat(b+P1)) = sat(min(a,b)+P1) as
+// saturating addition of a constant is monotone. Planes beyond cap (mask 0) hold 2000 like the guards and
+// are left out of the minimum. Returns the new minPrev (wave-uniform).
+// edge_l / edge_r (both halves equal) are the neighbour planes just below lane 0's first plane and just above
+// lane 63's last: the guard value 2000 unless a realigned range (below) puts a stored plane there.
+template <int NP, bool MASKED>
+__device__ __forceinline__ uint32_t path_step(const Cell<NP>& prev, uint32_t minprev, const Cell<NP>& raw,
+                                              uint32_t p1x2, uint32_t p2, const Cell<NP>& mask, Cell<NP>& lc,
+                                              uint32_t edge_l = kInit2, uint32_t edge_r = kInit2)
+{
+    uint32_t p2p = p2 + minprev;
+    p2p = p2p > 65535u ? 65535u : p2p;
+    const uint32_t p2x2 = p2p | (p2p << 16), minx2 = minprev | (minprev << 16);
+    const uint32_t left = shr1(prev.v[NP - 1], edge_l);   // plane l*DPL-1 in the high half
+    const uint32_t right = shl1(prev.v[0], edge_r);       // plane (l+1)*DPL in the low half
+    uint32_t run = 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        const uint32_t lm = __builtin_amdgcn_alignbit(prev.v[k], k == 0 ? left : prev.v[k - 1], 16);
+        const uint32_t lp = __builtin_amdgcn_alignbit(k == NP - 1 ? right : prev.v[k + 1], prev.v[k], 16);
+        uint32_t m = pmin(padd(pmin(lm, lp), p1x2), prev.v[k]);
+        m = pmin(m, p2x2);
+        uint32_t t = psub(padd(raw.v[k], m), minx2);
+        if constexpr (MASKED) {
+            t = (t & mask.v[k]) | (kInit2 & ~mask.v[k]);
+            run = pmin(run, t | ~mask.v[k]);
+        } else {
+            run = pmin(run, t);
+        }
+        lc.v[k] = t;
+    }
+    const uint32_t lo = run & 0xffffu, hi = run >> 16;
+    return wave_min(lo < hi ? lo : hi);
+}
+
+template <int NP>
+__device__ __forceinline__ void make_mask(int lane, int cap, Cell<NP>& mask, bool& active_any)
+{
+    const int e0 = lane * 2 * NP;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        uint32_t m = 0;
+        if (e0 + 2 * k < cap) m |= 0xffffu;
+        if (e0 + 2 * k + 1 < cap) m |= 0xffff0000u;
+        mask.v[k] = m;
+    }
+    active_any = e0 < cap;
+}
+
+template <int NP>
+__device__ __forceinline__ void set_init(Cell<NP>& c)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) c.v[k] = kInit2;
+}
+
+// The adaptive P2 of one path step, written one rounded operation at a time like the reference (no FMA):
+// E = 2^e by a cubic on the fraction plus an exponent add; note ei = (int)e - 1 for every negative e, so a
+// negative whole e gets f = 1 (:269-272).
+__device__ __forceinline__ uint32_t p2_step(const AdaptP2& q, uint32_t p2, uint32_t g, uint32_t gp)
+{
+    if (!q.on) return p2;
+    const float d0 = __fmul_rn(__fsub_rn((float)(g & 255u), (float)(gp & 255u)), q.w0);
+    const float d1 = __fmul_rn(__fsub_rn((float)((g >> 8) & 255u), (float)((gp >> 8) & 255u)), q.w1);
+    const float d2 = __fmul_rn(__fsub_rn((float)((g >> 16) & 255u), (float)((gp >> 16) & 255u)), q.w2);
+    const float S = __fadd_rn(__fadd_rn(fabsf(d0), fabsf(d2)), fabsf(d1));
+    const float e = fmaxf(fminf(-S, 128.0f), -126.0f);
+    const int ei = (int)e + (e < 0.0f ? -1 : 0);
+    const float f = __fsub_rn(e, (float)ei);
+    const float c3 = __int_as_float(0x3d9fcb52), c2 = __int_as_float(0x3e677e26), c1 = __int_as_float(0x3f322226),
+                c0 = __int_as_float(0x3f7ffb19);
+    const float poly = __fadd_rn(__fmul_rn(__fadd_rn(__fmul_rn(__fadd_rn(__fmul_rn(c3, f), c2), f), c1), f), c0);
+    const float E = __int_as_float((int)((uint32_t)__float_as_int(poly) + ((uint32_t)ei << 23)));
+    return (uint32_t)(int)__fmul_rn(q.base, E);
+}
+
+// ------------------------------------------------------------------ per-pixel plane ranges
+// With per-pixel ranges (DenseLayer Start/Count/Cap, DenseStereoSgm.cs:15-26, :43-58) a cell stores planes
+// [start, start+cap) of its path vector, lane l holding relative planes 2l, 2l+1 (one packed pair per lane,
+// cap <= 128). The line-buffer slot a successor reads (:265, :288-298) then holds those planes and 2000
+// everywhere else: the stale cleanup (:254-264) refills the previous occupant's range before each write, and
+// the guard words sit between the path segments. That holds as long as no pixel's range reaches the last word
+// of its segment (start - MinLo + cap <= MaxHi), which the host checks. So a successor starting at s sees
+// the predecessor (start sp, cap cp) through this realignment: relative plane j of the successor is plane
+// j + delta (delta = s - sp) of the predecessor, 2000 outside [0, cp); edge_l / edge_r are planes -1 and
+// 128 in the successor's frame.
+struct Range {
+    int s = 0, c = 0;  // start, cap of the cell whose state a vector holds (c = 0: the initial fill)
+};
+__device__ __forceinline__ uint32_t rel_plane(uint32_t v, int r, int cp)
+{
+    if (r < 0 || r >= cp) return 2000u;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, r >> 1);
+    return (r & 1) ? (w >> 16) : (w & 0xffffu);
+}
+__device__ __forceinline__ void realign(Cell<1>& st, const Range& from, int s, int lane, uint32_t& edge_l,
+                                        uint32_t& edge_r)
+{
+    const int delta = s - from.s, cp = from.c;
+    const uint32_t v = st.v[0];
+    const int r0 = 2 * lane + delta;
+    uint32_t out;
+    if ((delta & 1) == 0) {
+        const int sl = lane + (delta >> 1);
+        out = (uint32_t)__builtin_amdgcn_ds_bpermute((sl & 63) << 2, (int)v);
+    } else {
+        const int sa = lane + ((delta - 1) >> 1);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute((sa & 63) << 2, (int)v);
+        const uint32_t b = (uint32_t)__builtin_amdgcn_ds_bpermute(((sa + 1) & 63) << 2, (int)v);
+        out = __builtin_amdgcn_alignbit(b, a, 16);
+    }
+    const bool ok0 = r0 >= 0 && r0 < cp, ok1 = r0 + 1 >= 0 && r0 + 1 < cp;
+    st.v[0] = (ok0 ? (out & 0xffffu) : 2000u) | (ok1 ? (out & 0xffff0000u) : (2000u << 16));
+    const uint32_t el = rel_plane(v, delta - 1, cp), er = rel_plane(v, 128 + delta, cp);
+    edge_l = el | (el << 16);
+    edge_r = er | (er << 16);
+}
+// Move a chain's state into the frame of the next cell (start s); a no-op when the start does not change
+// (the planes past the predecessor's cap already hold 2000).
+template <int NP, bool VAR>
+__device__ __forceinline__ void reframe(Cell<NP>& st, Range& from, int s, int c, int lane, uint32_t& edge_l,
+                                        uint32_t& edge_r)
+{
+    edge_l = kInit2;
+    edge_r = kInit2;
+    if constexpr (VAR) {
+        static_assert(NP == 1, "per-pixel ranges take one packed pair per lane");
+        if (s != from.s) realign(*(Cell<1>*)&st, from, s, lane, edge_l, edge_r);
+        from.s = s;
+        from.c = c;
+    }
+}
+// A state loaded from a saved table (seeds, border columns) holds only the lanes of the stored stride: give
+// every plane past the stored cap the 2000 the line buffer has there.
+template <int NP>
+__device__ __forceinline__ void clip_to_cap(Cell<NP>& st, int cap, int lane)
+{
+    Cell<NP> m;
+    bool any;
+    make_mask<NP>(lane, cap, m, any);
+#pragma unroll
+    for (int k = 0; k < NP; k++) st.v[k] = (st.v[k] & m.v[k]) | (kInit2 & ~m.v[k]);
+}
+
+// Plane q of a cell held in lanes (2*NP planes per lane), read wave-uniformly.
+template <int NP>
+__device__ __forceinline__ uint32_t plane_of(const Cell<NP>& c, int q)
+{
+    const int ln = q / (2 * NP), e = q - ln * 2 * NP;
+    uint32_t w = c.v[0];
+#pragma unroll
+    for (int k = 1; k < NP; k++) w = (e >> 1) == k ? c.v[k] : w;
+    w = (uint32_t)__builtin_amdgcn_readlane((int)w, ln);
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// Chains of one path direction: the cells whose predecessor (x-DX, y-DY) lies outside the image start one.
+template <int DX, int DY>
+__device__ __forceinline__ int chain_count(int W, int H) { return DY == 0 ? H : (DX == 0 ? W : W + H - 1); }
+
+template <int DX, int DY>
+__device__ __forceinline__ void chain_start(int c, int W, int H, int& x0, int& y0, int& n)
+{
+    if constexpr (DY == 0) {
+        y0 = c; x0 = DX > 0 ? 0 : W - 1; n = W;
+    } else if constexpr (DX == 0) {
+        x0 = c; y0 = DY > 0 ? 0 : H - 1; n = H;
+    } else {
+        if (c < W) {
+            x0 = c; y0 = DY > 0 ? 0 : H - 1;
+        } else {
+            const int k = c - W + 1;
+            y0 = DY > 0 ? k : H - 1 - k; x0 = DX > 0 ? 0 : W - 1;
+        }
+        const int nx = DX > 0 ? W - x0 : x0 + 1, ny = DY > 0 ? H - y0 : y0 + 1;
+        n = nx < ny ? nx : ny;
+    }
+}
+
+struct SweepArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    uint16_t* seeds;     // [3][W][cs]
+    uint16_t* seedmins;  // [3][W]
+    int seed_row;
+    uint16_t* cols;      // [2][H][cs] (this pass)
+    uint16_t* colmins;   // [2][H]
+    const uint16_t* vstart;  // per-pixel ranges: start and cap of every cell [H][W] (null: uniform)
+    const uint16_t* vcap;
+    uint16_t* seedstart;     // [3][W] start / cap of the cell each seed-table entry belongs to
+    uint16_t* seedcap;
+    AdaptP2 ap;
+    uint32_t* seedguid;      // [3][W] (adaptive P2)
+};
+
+// One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
+// (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
+// 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
+template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
+{
+    constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
+    constexpr int DY = DIR * (P == 0 ? 0 : 1);
+    const int lane = threadIdx.x & 63;
+    const int chain = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    const int W = a.W, H = a.H, cs = a.cs;
+    if (chain >= chain_count<DX, DY>(W, H)) return;
+    int x, y, n;
+    chain_start<DX, DY>(chain, W, H, x, y, n);
+    const int bx = DIR > 0 ? W - 1 : 0;
+    if (P == 2 && x == bx) return;  // the border column's vertical chain is the replay's
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const int lo = lane * 2 * NP;
+
+    Cell<NP> state;
+    uint32_t minprev = 2000, gprev = 0;  // gprev: guidance of the state's cell (0: the initial fill)
+    Range from;
+    set_init(state);
+    if (DIR < 0 && P != 0 && y == H - 1) {
+        // pass -1's first row reads half 0 as pass +1 left it (:28-29, :219)
+        const int sc = x - DX;
+        if (sc >= 0 && sc < W) {
+            load_u16<NP>(a.seeds + ((size_t)(P - 1) * W + sc) * cs + lo, lo < cs, state);
+            if (VAR) {
+                from.s = a.seedstart[(P - 1) * W + sc];
+                from.c = a.seedcap[(P - 1) * W + sc];
+                clip_to_cap<NP>(state, from.c, lane);
+            } else if (MASKED) {
+#pragma unroll
+                for (int k = 0; k < NP; k++) state.v[k] = (state.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+            }
+            minprev = a.seedmins[(P - 1) * W + sc];
+            if (ADP) gprev = a.seedguid[(P - 1) * W + sc];
+        }
+    }
+    // software pipeline: PF cells in flight; raw stays packed until its step. Prefetch addresses are clamped
+    // to the chain's last cell and idle lanes read lane 0's planes, so every load is unconditional.
+    constexpr int PF = 8;
+    const ptrdiff_t step = (ptrdiff_t)DY * W + DX;
+    const size_t cell0 = (size_t)y * W + x;
+    const int lol = (VAR ? lo < cs : active) ? lo : 0;
+    size_t cell = cell0;
+    RawW<NP> rb[PF];
+    Cell<NP> ab[PF];
+    uint32_t rs[VAR ? PF : 1];  // ranges of the prefetched cells: start | cap << 16
+    uint32_t gq[ADP ? PF : 1];  // ADP: guidance of the prefetched cells
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
+        ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
+        if (!FIRST) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16);
+        if constexpr (ADP) gq[j] = a.ap.guid[cj];
+    }
+    auto body = [&](int i, int j) {
+        Cell<NP> raw;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> agg = ab[j];
+        uint32_t el = kInit2, er = kInit2;
+        int cs_ = 0, cc_ = 0;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            cs_ = (int)(r & 0xffffu);
+            cc_ = (int)(r >> 16);
+            reframe<NP, VAR>(state, from, cs_, cc_, lane, el, er);
+            make_mask<NP>(lane, cc_, mask, active);
+        }
+        const uint32_t gc = ADP ? (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[ADP ? j : 0]) : 0u;
+        const uint32_t p2 = ADP ? p2_step(a.ap, a.p2, gc, gprev) : a.p2;
+        const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
+        ld_raw_w<NP>(a.raw + cp * cs + lol, rb[j]);
+        if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cp] | ((uint32_t)a.vcap[cp] << 16);
+        if constexpr (ADP) gq[j] = a.ap.guid[cp];
+        Cell<NP> lc;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
+        if (x != bx) {
+            Cell<NP> out;
+#pragma unroll
+            for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
+            store_u16<NP>(a.agg + cell * cs + lo, active, out);
+            if (DIR > 0 && P != 0 && y == a.seed_row) {
+                store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, lo < cs, lc);
+                if (lane == 0) {
+                    a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
+                    if (VAR) {
+                        a.seedstart[(P - 1) * W + x] = (uint16_t)cs_;
+                        a.seedcap[(P - 1) * W + x] = (uint16_t)cc_;
+                    }
+                    if (ADP) a.seedguid[(P - 1) * W + x] = gc;
+                }
+            }
+        } else if (FIRST) {
+            Cell<NP> z;
+#pragma unroll
+            for (int k = 0; k < NP; k++) z.v[k] = 0;
+            store_u16<NP>(a.agg + cell * cs + lo, active, z);
+        }
+        if (P <= 1 && x == bx - DIR) {  // the border replay reads this column's path 0 / 1 states
+            store_u16<NP>(a.cols + ((size_t)P * H + y) * cs + lo, lo < cs, lc);
+            if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
+        }
+        state = lc;
+        gprev = gc;
+        cell += step;
+        x += DX;
+        y += DY;
+    };
+    int i0 = 0;
+    for (; i0 + PF <= n; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(i0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < n) body(i0 + j, j);
+}
+
+// ------------------------------------------------------------------ horizontal paths (rows)
+struct RowArgs {
+    const uint8_t* raw;     // raw volume (read; written when the cost is fused)
+    uint8_t* raw_out;       // fused census cost: where the raw bytes go
+    const uint64_t* cl;     // fused census cost: census codes of the left / right images
+    const uint64_t* cr;
+    uint16_t* agg;
+    int W, H, cs, cap, D;
+    uint32_t p1x2, p2;
+    uint16_t* cols;         // [2][H][cs] of this pass (slot 0: path 0)
+    uint16_t* colmins;      // [2][H]
+    uint16_t* disp;         // WTA outputs
+    float* sub;
+    uint16_t* dr;           // [H][W] right-image disparity
+    uint32_t* dr0;          // [H] dR(0) without cell (0, 0)
+    int write_agg;          // store the final aggregate too
+    const uint16_t* vstart; // per-pixel ranges [H][W]: start, cap and count (null: uniform)
+    const uint16_t* vcap;
+    const uint16_t* vcount;
+    AdaptP2 ap;
+};
+
+// Census codes along a row. Lane l owns planes d = lo .. lo+2*NP-1 (lo = 2*NP*l) and keeps the window
+// win[e] = cr(max(x - lo - e, 0)) of the right-image codes its planes compare with at the current x. Moving
+// to x+1 the window shifts by one code: win[e] <- win[e-1], and win[0] <- the left neighbour lane's
+// win[2*NP-1] = cr(max(x + 1 - lo, 0)) (DPP wave_shr), lane 0 taking cr(x+1) itself. So a step reads two
+// wave-uniform codes, cl(x) and cr(x+1), instead of 2*NP per lane.
+__device__ __forceinline__ uint64_t shr1_64(uint64_t v)
+{
+    const uint32_t lo = shr1((uint32_t)v, 0u), hi = shr1((uint32_t)(v >> 32), 0u);
+    return ((uint64_t)hi << 32) | lo;
+}
+template <int NP>
+__device__ __forceinline__ void census_advance(uint64_t (&win)[2 * NP], uint64_t fresh, int lane)
+{
+    const uint64_t from = shr1_64(win[2 * NP - 1]);
+#pragma unroll
+    for (int e = 2 * NP - 1; e > 0; e--) win[e] = win[e - 1];
+    win[0] = lane == 0 ? fresh : from;
+}
+// Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
+// repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
+template <int NP>
+__device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2 * NP], int D, int lo, Cell<NP>& raw,
+                                            uint32_t (&bytes)[NP / 2 + (NP == 1)])
+{
+    uint32_t v[2 * NP];
+#pragma unroll
+    for (int e = 0; e < 2 * NP; e++) v[e] = lo + e < D ? (uint32_t)__popcll(cl ^ win[e]) : 255u;
+#pragma unroll
+    for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
+    if constexpr (NP == 1) {
+        bytes[0] = v[0] | (v[1] << 8);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) bytes[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+    }
+}
+
+// Pass +1, path 0 (->): one wavefront per row, x = 0 .. W-1. The first aggregate writer (writes lc instead
+// of adding; the border column W-1 gets 0 and is filled by the replay). COST: computes the census cost of
+// each cell on the way and writes the raw volume for the later sweeps. Saves the W-2 column's states for
+// the border replay.
+template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
+{
+    static_assert(!(COST && VAR), "per-pixel ranges take their raw volume from the cost kernel");
+    constexpr int PF = 8;
+    const int lane = threadIdx.x & 63;
+    const int y = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
+    if (y >= a.H) return;
+    const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const bool store_lane = lo < cs;
+    Cell<NP> state;
+    set_init(state);
+    uint32_t minprev = 2000;
+    const size_t rowcell = (size_t)y * W;
+    const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
+    const uint64_t* clrow = COST ? a.cl + rowcell : nullptr;
+    const int lol = store_lane ? lo : 0;
+    // COST: the lane's window of right codes, and the prefetched wave-uniform codes cl(x), cr(x+1)
+    uint64_t win[2 * NP];
+    uint64_t rl[COST ? PF : 1], rr[COST ? PF : 1];
+    RawW<NP> rb[COST ? 1 : PF];
+    uint32_t rs[VAR ? PF : 1];  // VAR: start | cap << 16 of the prefetched cells
+    Range from;
+    uint32_t gq[ADP ? PF : 1], gprev = 0;  // ADP: guidance of the prefetched cells / of the state's cell
+    if constexpr (COST) {
+        const uint64_t c0 = crow[0];
+#pragma unroll
+        for (int e = 0; e < 2 * NP; e++) win[e] = c0;
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int xj = min(j, W - 1);
+        if constexpr (COST) {
+            rl[j] = clrow[xj];
+            rr[j] = crow[min(j + 1, W - 1)];
+        } else {
+            ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        }
+        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
+        if constexpr (ADP) gq[j] = a.ap.guid[rowcell + xj];
+    }
+    auto body = [&](int x, int j) {
+        Cell<NP> raw;
+        const int xp = min(x + PF, W - 1);
+        uint32_t el = kInit2, er = kInit2;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            reframe<NP, VAR>(state, from, (int)(r & 0xffffu), (int)(r >> 16), lane, el, er);
+            make_mask<NP>(lane, (int)(r >> 16), mask, active);
+            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+        }
+        if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
+            uint32_t bytes[NP / 2 + (NP == 1)];
+            census_cost<NP>(rl[j], win, a.D, lo, raw, bytes);
+            uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
+            if (store_lane) {
+                if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
+                else {
+#pragma unroll
+                    for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bytes[k];
+                }
+            }
+            census_advance<NP>(win, rr[j], lane);
+            rl[j] = clrow[xp];
+            rr[j] = crow[min(x + PF + 1, W - 1)];
+        } else {
+            widen<NP>(rb[j], raw);
+            ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
+        }
+        uint32_t p2 = a.p2;
+        if constexpr (ADP) {
+            const uint32_t gc = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
+            p2 = p2_step(a.ap, a.p2, gc, gprev);
+            gq[j] = a.ap.guid[rowcell + xp];
+            gprev = gc;
+        }
+        Cell<NP> lc;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
+        Cell<NP> out;
+#pragma unroll
+        for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
+        store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
+        if (x == W - 2) {
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, lc);
+            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+        }
+        state = lc;
+    };
+    int x0 = 0;
+    for (; x0 + PF <= W; x0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(x0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (x0 + j < W) body(x0 + j, j);
+}
+
+// Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
+// winner-take-all fused: one wavefront per row, x = W-1 .. 1.
+//  dL(x): first minimum over planes [0, D) of the final aggregate (DenseStereoSgm.cs:225-231), as the
+//         minimum of the keys agg << 16 | d.
+//  sub(x): parabola through agg(d-1), agg(d), agg(d+1) in float32 (correctly rounded division, no
+//         contraction), no offset at the range ends (CrossViewCost.cs:197-202).
+//  dR(xr) = first minimum over d of agg(xr+d, d): a shift register of keys along the planes. After cell x,
+//         register plane p holds min over j >= p of key(x-p+j, j), i.e. dR candidates of xr = x-p, so
+//         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
+//         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
+// Saves the column-1 states for the border replay.
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
+{
+    constexpr int PF = 8, E = 2 * NP;
+    const int lane = threadIdx.x & 63;
+    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (y >= a.H) return;
+    const int W = a.W, cs = a.cs, lo = lane * E;
+    int D = a.D;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    Cell<NP> state;
+    set_init(state);
+    uint32_t minprev = 2000;
+    uint32_t R[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
+    const size_t rowcell = (size_t)y * W;
+    const int lol = (VAR ? lo < cs : active) ? lo : 0;
+    RawW<NP> rb[PF];
+    Cell<NP> ab[PF];
+    uint32_t rs[VAR ? PF : 1], rn[VAR ? PF : 1];  // VAR: start | cap << 16, and count, of the prefetched cells
+    Range from;
+    uint32_t gq[ADP ? PF : 1], gprev = 0;  // ADP: guidance of the prefetched cells / of the state's cell
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int xj = max(W - 1 - j, 0);
+        ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        if constexpr (VAR) {
+            rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
+            rn[j] = a.vcount[rowcell + xj];
+        }
+        if constexpr (ADP) gq[j] = a.ap.guid[rowcell + xj];
+    }
+    uint32_t myd = 0, myr = 0;
+    float mys = 0.f;
+    auto body = [&](int x, int j) {
+        Cell<NP> raw;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> agg = ab[j];
+        const int xp = max(x - PF, 0);
+        uint32_t el = kInit2, er = kInit2;
+        int s0 = 0;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            s0 = (int)(r & 0xffffu);
+            D = __builtin_amdgcn_readfirstlane((int)rn[j]);
+            reframe<NP, VAR>(state, from, s0, (int)(r >> 16), lane, el, er);
+            make_mask<NP>(lane, (int)(r >> 16), mask, active);
+            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+            rn[j] = a.vcount[rowcell + xp];
+        }
+        uint32_t p2 = a.p2;
+        if constexpr (ADP) {
+            const uint32_t gc = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
+            p2 = p2_step(a.ap, a.p2, gc, gprev);
+            gq[j] = a.ap.guid[rowcell + xp];
+            gprev = gc;
+        }
+        ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
+        ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        Cell<NP> lc, fin;
+        minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
+#pragma unroll
+        for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
+        if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+        if (x == 1) {
+            store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, lc);
+            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+        }
+        state = lc;
+        // winner-take-all on the final aggregate of cell x
+        uint32_t key[E];
+        uint32_t best = 0xffffffffu;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int d = lo + e;
+            const uint32_t v = (e & 1) ? (fin.v[e >> 1] >> 16) : (fin.v[e >> 1] & 0xffffu);
+            key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+            best = key[e] < best ? key[e] : best;
+        }
+        uint32_t rdone = 0;
+        if constexpr (!VAR) {  // the right-image disparity needs one plane range for the whole row
+            const uint32_t carry = shl1(R[0], 0xffffffffu);
+#pragma unroll
+            for (int e = 0; e < E - 1; e++) R[e] = R[e + 1];
+            R[E - 1] = carry;
+#pragma unroll
+            for (int e = 0; e < E; e++) R[e] = key[e] < R[e] ? key[e] : R[e];
+            rdone = (uint32_t)__builtin_amdgcn_readlane((int)R[0], 0);
+        }
+        best = wave_min(best);
+        const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+        float v = (float)(d + s0);
+        if (d > 0 && d < D - 1) {
+            const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
+            const int den = l - 2 * cc + r;
+            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+        }
+        if (lane == (x & 63)) {
+            myd = (uint32_t)(d + s0);  // the plane index start + arg (:230)
+            mys = v;
+            myr = rdone & 0xffffu;
+        }
+        if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
+            const int xb = x & ~63, l0 = x & 63;
+            const int l1 = min(63, W - 1 - xb);
+            if (lane >= l0 && lane <= l1) {
+                const size_t o = rowcell + xb + lane;
+                a.disp[o] = (uint16_t)myd;
+                if (a.sub) a.sub[o] = mys;
+                if (!VAR) a.dr[o] = (uint16_t)myr;
+            }
+        }
+    };
+    // x = W-1 .. 1: (W-1) steps
+    const int n = W - 1;
+    int i0 = 0;
+    for (; i0 + PF <= n; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < n) body(W - 1 - (i0 + j), j);
+    if constexpr (!VAR) {
+        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
+        if (lane == 0) a.dr0[y] = r1;
+    }
+}
+
+// Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
+// valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol (RealStereo.cs:418-421).
+template <int NP, bool VAR>
+__global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* valid)
+{
+    const int y = blockIdx.x, lane = threadIdx.x;
+    const size_t rowcell = (size_t)y * a.W;
+    const int W = a.W, cs = a.cs, D = VAR ? (int)a.vcount[rowcell] : a.D, lo = lane * 2 * NP;
+    const int s0 = VAR ? (int)a.vstart[rowcell] : 0;
+    const bool active = lo < cs;
+    Cell<NP> c;
+    load_u16<NP>(a.agg + rowcell * cs + lo, active, c);
+    uint32_t best = 0xffffffffu;
+#pragma unroll
+    for (int e = 0; e < 2 * NP; e++) {
+        const int d = lo + e;
+        const uint32_t v = (e & 1) ? (c.v[e >> 1] >> 16) : (c.v[e >> 1] & 0xffffu);
+        const uint32_t key = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+        best = key < best ? key : best;
+    }
+    best = wave_min(best);
+    const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+    float v = (float)(d + s0);
+    if (d > 0 && d < D - 1) {
+        const int l = (int)plane_of<NP>(c, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(c, d + 1);
+        const int den = l - 2 * cc + r;
+        if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+    }
+    if (lane == 0) {
+        a.disp[rowcell] = (uint16_t)(d + s0);
+        if (a.sub) a.sub[rowcell] = v;
+    }
+    if constexpr (VAR) return;  // no right-image disparity with per-pixel ranges
+    const uint32_t k00 = (plane_of<NP>(c, 0) << 16);
+    const uint32_t part = a.dr0[y];
+    if (lane == 0) a.dr[rowcell] = (uint16_t)((k00 < part ? k00 : part) & 0xffffu);
+    __syncthreads();
+    if (valid)
+        for (int x = lane; x < W; x += 64) {
+            const int dd = a.disp[rowcell + x], xr = x - dd;
+            valid[rowcell + x] = (uint8_t)(xr >= 0 && abs(dd - (int)a.dr[rowcell + xr]) <= tol);
+        }
+}
+
+// ------------------------------------------------------------------ fused vertical + diagonal paths (bands)
+// Paths 1, 2, 3 of one pass in one kernel, so each cell's raw cost is read once and its aggregate
+// read-modify-written once for the three. Rows advance in bands of kBandRows; a band is one launch, with one
+// workgroup per kStrip-column strip. A strip computes its three paths row by row from the states the previous
+// band left at its last row (a ping-pong boundary buffer). The diagonal paths also run over halo columns,
+// one fewer per row (kBandRows-1 at the band's first row), so a strip needs no other strip of its band.
+// Waves own kBandG consecutive columns each and swap their edge columns' states through LDS once per row.
+constexpr int kBandRows = 8, kStrip = 16, kBandWaves = 8, kBandG = 4;
+static_assert(kStrip + 2 * kBandRows <= kBandWaves * kBandG, "band halo does not fit the waves");
+
+struct BandArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    int split;                    // bands of kBandRows cover rows [0, split) then [split, H) (in pass order)
+    int strips;                   // tiles per band
+    const uint16_t* bin0;         // [3][W][cs] states of the row before the first band (null: the initial fill)
+    const uint16_t* bin0min;      // [3][W] (u16, the seed table's)
+    uint16_t* bnd;                // [2][3][W][cs] boundary ping-pong: band b writes slot b & 1
+    uint32_t* bndmin;             // [2][3][W]
+    uint16_t* extra;              // the last row of band `extra_band` also goes here (the pass +1 seed table)
+    uint16_t* extramin;
+    int extra_band;
+    uint16_t* cols;               // [2][H][cs] of this pass: slot 1 takes path 1 of the column next to the border
+    uint16_t* colmins;
+    int* ticket;                  // tile order (zeroed before the launch)
+    int* flags;                   // [bands][strips]: epoch when the tile's boundary is written
+    int epoch;
+    int* err;                     // set when a wait gives up
+};
+
+// Band b, in pass order: its first row (distance from the pass's first row) and its row count.
+__device__ __forceinline__ void band_rows(int b, int split, int H, int& first, int& rows)
+{
+    const int nsplit = (split + kBandRows - 1) / kBandRows;
+    if (b < nsplit) {
+        first = b * kBandRows;
+        rows = min(kBandRows, split - first);
+    } else {
+        first = split + (b - nsplit) * kBandRows;
+        rows = min(kBandRows, H - first);
+    }
+}
+
+template <int NP, int DIR, bool MASKED>
+__global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
+{
+    constexpr int G = kBandG, NW = kBandWaves;
+    // path j (0, 1, 2) is the reference's path j+1 of pass DIR; its predecessor column is x - DIR, x, x + DIR
+    constexpr int JL = DIR > 0 ? 0 : 2;  // the path reading column x-1
+    constexpr int JR = DIR > 0 ? 2 : 0;  // the path reading column x+1
+    __shared__ uint32_t edge[2][NW][2][64 * NP];
+    __shared__ uint32_t edgemin[2][NW][2];
+    __shared__ int tile_s;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // tiles run in ticket order, band by band: every tile this one waits for took an earlier ticket, so it
+    // has started and (by induction) finishes: no deadlock whatever the number of resident workgroups
+    if (threadIdx.x == 0) tile_s = atomicAdd(a.ticket, 1);
+    __syncthreads();
+    const int tile = tile_s;
+    const int band = tile / a.strips, strip = tile - band * a.strips;
+    int first, rows;
+    band_rows(band, a.split, a.H, first, rows);
+    const int y0 = DIR > 0 ? first : a.H - 1 - first;
+    const int W = a.W, cs = a.cs, H1 = rows - 1;
+    const int xa = strip * kStrip, xb = min(W, xa + kStrip);
+    const size_t bsz = (size_t)3 * W * cs, bmsz = (size_t)3 * W;
+    const uint16_t* bin = band == 0 ? a.bin0 : a.bnd + ((band - 1) & 1) * bsz;
+    const uint32_t* binmin = band == 0 ? nullptr : a.bndmin + ((band - 1) & 1) * bmsz;
+    if (band > 0 && threadIdx.x == 0) {
+        // wait for the previous band's tiles whose last row this tile's halo reads (strips s-1, s, s+1); their
+        // boundary stores were drained before the flag, and are read below with agent-scope loads
+        for (int d = max(strip - 1, 0); d <= min(strip + 1, a.strips - 1); d++) {
+            const int* f = a.flags + (band - 1) * a.strips + d;
+            int spins = 0;
+            while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1 << 22)) {
+                    atomicOr(a.err, 1);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // columns [xa - kBandRows, ...): the strip, the halo, and one guard column each side that only ever holds
+    // the boundary state its neighbour reads in the band's first row
+    const int cbase = xa - kBandRows + wv * G;
+    const int bx = DIR > 0 ? W - 1 : 0;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    const int lo = lane * 2 * NP, lol = active ? lo : 0;
+
+    // the states of the row before the band ("row -1"): the boundary buffer, or the initial fill
+    Cell<NP> st[G][3];
+    uint32_t sm[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int x = cbase + g;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            set_init(st[g][j]);
+            sm[g][j] = 2000;
+            if (bin && x >= 0 && x < W) {
+                load_u16_agent<NP>(bin + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (MASKED) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) st[g][j].v[q] = (st[g][j].v[q] & mask.v[q]) | (kInit2 & ~mask.v[q]);
+                }
+                sm[g][j] = binmin ? __hip_atomic_load(binmin + j * W + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : (uint32_t)a.bin0min[j * W + x];
+            }
+        }
+    }
+    // the region of path j at band row k (L = halo width left at that row)
+    auto need = [&](int x, int j, int k) -> bool {
+        if (x < 0 || x >= W) return false;
+        const int L = H1 - k;
+        if (j == JL) return x >= xa - L && x < xb;
+        if (j == JR) return x >= xa && x < xb + L;
+        return x >= xa && x < xb && x != bx;
+    };
+    auto need_raw = [&](int x, int k) -> bool {
+        const int L = H1 - k;
+        return x >= 0 && x < W && x >= xa - L && x < xb + L;
+    };
+    auto load_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G]) {
+        const int y = y0 + DIR * k;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need_raw(x, k)) ld_raw_w<NP>(a.raw + ((size_t)y * W + x) * cs + lol, rb[g]);
+            if (x >= xa && x < xb) ld_u16<NP>(a.agg + ((size_t)y * W + x) * cs + lol, ab[g]);
+        }
+    };
+    auto do_row = [&](int k, RawW<NP> (&rb)[G], Cell<NP> (&ab)[G], RawW<NP> (&nrb)[G], Cell<NP> (&nab)[G]) {
+        const int y = y0 + DIR * k;
+        const int par = k & 1;
+        // edge exchange: the previous row's states of this wave's boundary columns
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            edge[par][wv][0][lane * NP + q] = st[G - 1][JL].v[q];
+            edge[par][wv][1][lane * NP + q] = st[0][JR].v[q];
+        }
+        if (lane == 0) {
+            edgemin[par][wv][0] = sm[G - 1][JL];
+            edgemin[par][wv][1] = sm[0][JR];
+        }
+        __syncthreads();
+        if (k + 1 < rows) load_row(k + 1, nrb, nab);
+        // path costs accumulate straight into the strip cells' prefetched aggregates ab[g]
+        // the path reading x-1: columns right to left, so x-1 still holds the previous row
+#pragma unroll
+        for (int g = G - 1; g >= 0; g--) {
+            const int x = cbase + g;
+            if (need(x, JL, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                if (x - 1 < 0) { set_init(prev); pm = 2000; }
+                else if (g > 0) { prev = st[g - 1][JL]; pm = sm[g - 1][JL]; }
+                else {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) prev.v[q] = edge[par][wv - 1][0][lane * NP + q];
+                    pm = edgemin[par][wv - 1][0];
+                }
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][JL] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][JL] = lc;
+                if (x >= xa && x < xb) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+                }
+            }
+        }
+        // the vertical path
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need(x, 1, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                prev = st[g][1];
+                pm = sm[g][1];
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][1] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][1] = lc;
+#pragma unroll
+                for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+            }
+        }
+        // the path reading x+1: columns left to right
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (need(x, JR, k)) {
+                Cell<NP> prev;
+                uint32_t pm;
+                if (x + 1 >= W) { set_init(prev); pm = 2000; }
+                else if (g < G - 1) { prev = st[g + 1][JR]; pm = sm[g + 1][JR]; }
+                else {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) prev.v[q] = edge[par][wv + 1][1][lane * NP + q];
+                    pm = edgemin[par][wv + 1][1];
+                }
+                Cell<NP> lc, raw;
+                widen<NP>(rb[g], raw);
+                sm[g][JR] = path_step<NP, MASKED>(prev, pm, raw, a.p1x2, a.p2, mask, lc);
+                st[g][JR] = lc;
+                if (x >= xa && x < xb) {
+#pragma unroll
+                    for (int q = 0; q < NP; q++) ab[g].v[q] = padd(ab[g].v[q], lc.v[q]);
+                }
+            }
+        }
+        // the strip's own cells: aggregate, seeds, the replay's column, the boundary for the next band
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int x = cbase + g;
+            if (x >= xa && x < xb) {
+                const size_t cell = (size_t)y * W + x;
+                if (x != bx) store_u16<NP>(a.agg + cell * cs + lo, active, ab[g]);
+                if (x == bx - DIR) {
+                    store_u16<NP>(a.cols + ((size_t)a.H + y) * cs + lo, active, st[g][0]);
+                    if (lane == 0) a.colmins[a.H + y] = (uint16_t)sm[g][0];
+                }
+            }
+        }
+    };
+    // one row body: the next row's loads go out at the start of a row and are taken over at its end, a row
+    // of work later
+    RawW<NP> rb[G], nrb[G];
+    Cell<NP> ab[G], nab[G];
+    load_row(0, rb, ab);
+    for (int k = 0; k < rows; k++) {
+        do_row(k, rb, ab, nrb, nab);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            rb[g] = nrb[g];
+            ab[g] = nab[g];
+        }
+    }
+    // the band's last row: the boundary the next band starts from
+    uint16_t* bout = a.bnd + (band & 1) * bsz;
+    uint32_t* boutmin = a.bndmin + (band & 1) * bmsz;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int x = cbase + g;
+        if (x >= xa && x < xb) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                store_u16_agent<NP>(bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                if (lane == 0) __hip_atomic_store(boutmin + j * W + x, sm[g][j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (band == a.extra_band) {
+                    store_u16<NP>(a.extra + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
+                    if (lane == 0) a.extramin[j * W + x] = (uint16_t)sm[g][j];
+                }
+            }
+        }
+    }
+    // publish: every wave drains its stores (the boundary's agent-scope stores are then visible device-wide),
+    // then one flag store
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(a.flags + band * a.strips + strip, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct ReplayArgs {
+    const uint8_t* raw;
+    uint16_t* agg;
+    int W, H, cs, cap;
+    uint32_t p1x2, p2;
+    int dir, col;
+    const int* visits;        // 4 ints per visit: y, half, path-0 label, path-1 label
+    int nvis;
+    const uint16_t* cols;     // [2][H][cs] of this pass
+    const uint16_t* colmins;  // [2][H]
+    uint16_t* seeds;          // [3][W][cs]
+    uint16_t* seedmins;       // [3][W]
+    uint16_t* partial;        // [nvis][cs]: paths 0 + 1 + 3 of each visit
+    const uint16_t* vstart;   // per-pixel ranges [H][W] (null: uniform)
+    const uint16_t* vcap;
+    uint16_t* seedstart;      // [3][W]
+    uint16_t* seedcap;
+    AdaptP2 ap;
+    uint32_t* seedguid;       // [3][W] (adaptive P2)
+};
+
+// The state a path-0 / path-1 neighbour read returns, by plan label: -1 the initial fill, y >= 0 the saved
+// neighbour-column state of row y, -2-c pass +1's half-0 leftover of cell c (path 1 only). `from` receives
+// the range of the cell the state belongs to.
+template <int NP, bool MASKED, bool VAR>
+__device__ __forceinline__ void label_state(const ReplayArgs& a, int p, int l, int lo, const Cell<NP>& mask,
+                                            Cell<NP>& prev, uint32_t& pm, Range& from, uint32_t& gp)
+{
+    set_init(prev);
+    pm = 2000;
+    gp = 0;
+    from = Range();
+    if (l >= 0) {
+        load_u16<NP>(a.cols + ((size_t)p * a.H + l) * a.cs + lo, lo < a.cs, prev);
+        pm = a.colmins[p * a.H + l];
+        const int ncol = a.col - a.dir;  // the neighbour column (the plan labels only real columns)
+        if (ncol >= 0 && ncol < a.W) {
+            if (VAR) {
+                from.s = a.vstart[(size_t)l * a.W + ncol];
+                from.c = a.vcap[(size_t)l * a.W + ncol];
+            }
+            if (a.ap.on) gp = a.ap.guid[(size_t)l * a.W + ncol];
+        }
+    } else if (l <= -2) {
+        const int c = -2 - l;
+        load_u16<NP>(a.seeds + (size_t)c * a.cs + lo, lo < a.cs, prev);  // seed table of path 1
+        pm = a.seedmins[c];
+        if (VAR) {
+            from.s = a.seedstart[c];
+            from.c = a.seedcap[c];
+        }
+        if (a.ap.on) gp = a.seedguid[c];
+    }
+    if (VAR) {
+        clip_to_cap<NP>(prev, from.c, lo / (2 * NP));
+    } else if (MASKED) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) prev.v[k] = (prev.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+    }
+}
+
+// Path step of a border visit from a state of another cell (range `from`) to the visited cell (start s, cap c).
+template <int NP, bool MASKED, bool VAR>
+__device__ __forceinline__ uint32_t step_from(Cell<NP> prev, Range from, uint32_t pm, int s, int c, int lane,
+                                              const Cell<NP>& raw, const ReplayArgs& a, const Cell<NP>& mask, Cell<NP>& lc,
+                                              uint32_t g, uint32_t gp)
+{
+    uint32_t el, er;
+    reframe<NP, VAR>(prev, from, s, c, lane, el, er);
+    return path_step<NP, MASKED>(prev, pm, raw, a.p1x2, p2_step(a.ap, a.p2, g, gp), mask, lc, el, er);
+}
+
+// Paths 0, 1 and 3 of every border visit, one wavefront per visit, all in parallel: their neighbours are
+// fixed by the plan (saved neighbour-column states, pass +1 seeds, or the margin slot for path 3), so
+// they do not chain. Each visit's sum of the three path costs goes to `partial`.
+template <int NP, bool MASKED, bool VAR>
+__global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (i >= a.nvis) return;
+    const int y = a.visits[4 * i], l0 = a.visits[4 * i + 2], l1 = a.visits[4 * i + 3];
+    const size_t cell = (size_t)y * a.W + a.col;
+    const int s = VAR ? (int)a.vstart[cell] : 0, c = VAR ? (int)a.vcap[cell] : a.cap;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, c, mask, active);
+    const int lo = lane * 2 * NP;
+    Cell<NP> raw, prev, lc0, lc1, lc3;
+    load_raw<NP>(a.raw + cell * a.cs + lo, lo < a.cs, raw);
+    const uint32_t g = a.ap.on ? a.ap.guid[cell] : 0u;
+    uint32_t pm, gp;
+    Range from;
+    label_state<NP, MASKED, VAR>(a, 0, l0, lo, mask, prev, pm, from, gp);
+    step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc0, g, gp);
+    label_state<NP, MASKED, VAR>(a, 1, l1, lo, mask, prev, pm, from, gp);
+    step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc1, g, gp);
+    set_init(prev);
+    path_step<NP, MASKED>(prev, 2000, raw, a.p1x2, p2_step(a.ap, a.p2, g, 0u), mask, lc3);  // the margin slot
+    Cell<NP> sum;
+#pragma unroll
+    for (int k = 0; k < NP; k++) sum.v[k] = padd(padd(lc0.v[k], lc1.v[k]), lc3.v[k]);
+    store_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, sum);
+}
+
+// Path 2 of the border column: one chain through the column's visits in schedule order. Each visit reads
+// the state the other line-buffer half holds for the border slot and leaves its own in the current half
+// (:219, :265). The chain adds its path cost to the visit's `partial` (the aggregate is updated afterwards,
+// per cell, by sgm_replay_add), so it only reads data fixed before it starts and can prefetch deeply.
+// Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's seed-table
+// entry (half 0 as the pass leaves it).
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
+{
+    constexpr int PF = 8;
+    const int lane = threadIdx.x;
+    const int cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    Cell<NP> h0, h1;
+    uint32_t m0 = 2000, m1 = 2000;
+    // VAR: the cells whose states the two halves hold, as start | cap << 16 (kept in registers)
+    uint32_t q0 = 0, q1 = 0;
+    uint32_t g0 = 0, g1 = 0;  // guidance of the cells whose states the two halves hold (adaptive P2)
+    set_init(h0);
+    set_init(h1);
+    if (a.dir < 0) {
+        load_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, lo < cs, h0);  // path 2's seed
+        if (VAR) {
+            q0 = (uint32_t)a.seedstart[1 * a.W + a.col] | ((uint32_t)a.seedcap[1 * a.W + a.col] << 16);
+            clip_to_cap<NP>(h0, (int)(q0 >> 16), lane);
+        } else if (MASKED) {
+#pragma unroll
+            for (int k = 0; k < NP; k++) h0.v[k] = (h0.v[k] & mask.v[k]) | (kInit2 & ~mask.v[k]);
+        }
+        m0 = a.seedmins[1 * a.W + a.col];
+        if (ADP) g0 = a.seedguid[1 * a.W + a.col];
+    }
+    const int lol = lo < cs ? lo : 0;
+    const int nv = a.nvis;
+    RawW<NP> rb[PF];
+    Cell<NP> pb[PF];
+    int cb[PF];          // half of each prefetched visit
+    uint32_t rs[PF];     // VAR: start | cap << 16 of each prefetched visit's cell
+    uint32_t gq[ADP ? PF : 1];  // ADP: guidance of each prefetched visit's cell
+    if (nv > 0) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int ij = min(j, nv - 1);
+            const size_t cj = (size_t)vis[4 * ij] * a.W + a.col;
+            cb[j] = vis[4 * ij + 1];
+            ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
+            ld_u16<NP>(a.partial + (size_t)ij * cs + lol, pb[j]);
+            rs[j] = VAR ? ((uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16)) : 0u;
+            if constexpr (ADP) gq[j] = a.ap.guid[cj];
+        }
+    }
+    auto body = [&](int i, int j) {
+        const int cur = cb[j];
+        Cell<NP> raw;
+        widen<NP>(rb[j], raw);
+        const Cell<NP> part = pb[j];
+        int s = 0, c = a.cap;
+        if constexpr (VAR) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            s = (int)(r & 0xffffu);
+            c = (int)(r >> 16);
+            make_mask<NP>(lane, c, mask, active);
+        }
+        const int ip = min(i + PF, nv - 1);
+        const size_t cpn = (size_t)vis[4 * ip] * a.W + a.col;
+        cb[j] = vis[4 * ip + 1];
+        ld_raw_w<NP>(a.raw + cpn * cs + lol, rb[j]);
+        ld_u16<NP>(a.partial + (size_t)ip * cs + lol, pb[j]);
+        if (VAR) rs[j] = (uint32_t)a.vstart[cpn] | ((uint32_t)a.vcap[cpn] << 16);
+        uint32_t g = 0;
+        if constexpr (ADP) {
+            g = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
+            gq[j] = a.ap.guid[cpn];
+        }
+        Cell<NP> lc, sum, prev;
+        uint32_t mn, pm, q = 0;
+#pragma unroll
+        for (int k = 0; k < NP; k++) prev.v[k] = cur ? h0.v[k] : h1.v[k];  // the other half's state
+        pm = cur ? m0 : m1;
+        q = cur ? q0 : q1;
+        const uint32_t gp = cur ? g0 : g1;
+        Range from;
+        from.s = (int)(q & 0xffffu);
+        from.c = (int)(q >> 16);
+        mn = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc, g, gp);
+#pragma unroll
+        for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
+        store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
+        const uint32_t qn = VAR ? ((uint32_t)s | ((uint32_t)c << 16)) : 0u;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            h1.v[k] = cur ? lc.v[k] : h1.v[k];
+            h0.v[k] = cur ? h0.v[k] : lc.v[k];
+        }
+        m1 = cur ? mn : m1;
+        m0 = cur ? m0 : mn;
+        q1 = cur ? qn : q1;
+        q0 = cur ? q0 : qn;
+        g1 = cur ? g : g1;
+        g0 = cur ? g0 : g;
+    };
+    int i0 = 0;
+    for (; i0 + PF <= nv; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(i0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < nv) body(i0 + j, j);
+    if (a.dir > 0) {
+        // path 2: the chain's half-0 state; paths 1 and 3: recomputed for the last visit that wrote half 0
+        store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, lo < cs, h0);
+        if (lane == 0) {
+            a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
+            if (VAR) {
+                a.seedstart[1 * a.W + a.col] = (uint16_t)(q0 & 0xffffu);
+                a.seedcap[1 * a.W + a.col] = (uint16_t)(q0 >> 16);
+            }
+            if (ADP) a.seedguid[1 * a.W + a.col] = g0;
+        }
+        int last = -1;
+        for (int i = a.nvis - 1; i >= 0; i--)
+            if (vis[4 * i + 1] == 0) { last = i; break; }
+        Cell<NP> s1, s3;
+        uint32_t sm1 = 2000, sm3 = 2000;
+        Range rl;
+        uint32_t gl = 0;
+        set_init(s1);
+        set_init(s3);
+        if (last >= 0) {
+            const int yl = vis[4 * last];
+            const size_t cl = (size_t)yl * a.W + a.col;
+            if (ADP) gl = a.ap.guid[cl];
+            int s = 0, c = a.cap;
+            if (VAR) {
+                s = a.vstart[cl];
+                c = a.vcap[cl];
+                make_mask<NP>(lane, c, mask, active);
+                rl.s = s;
+                rl.c = c;
+            }
+            Cell<NP> r, prev;
+            uint32_t pm, gp;
+            Range from;
+            load_raw<NP>(a.raw + cl * cs + lo, lo < cs, r);
+            label_state<NP, MASKED, VAR>(a, 1, vis[4 * last + 3], lo, mask, prev, pm, from, gp);
+            sm1 = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, r, a, mask, s1, gl, gp);
+            set_init(prev);
+            sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, p2_step(a.ap, a.p2, gl, 0u), mask, s3);
+        }
+        store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, lo < cs, s1);
+        store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, lo < cs, s3);
+        if (lane == 0) {
+            a.seedmins[0 * a.W + a.col] = (uint16_t)sm1;
+            a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
+            if (VAR) {
+                a.seedstart[0 * a.W + a.col] = (uint16_t)rl.s;
+                a.seedcap[0 * a.W + a.col] = (uint16_t)rl.c;
+                a.seedstart[2 * a.W + a.col] = (uint16_t)rl.s;
+                a.seedcap[2 * a.W + a.col] = (uint16_t)rl.c;
+            }
+            if (ADP) {
+                a.seedguid[0 * a.W + a.col] = gl;
+                a.seedguid[2 * a.W + a.col] = gl;
+            }
+        }
+    }
+}
+
+// Adds every border visit's four path costs to its cell (one wavefront per row of the border column;
+// `cellvis` lists each row's visits, at most two per pass).
+template <int NP, bool VAR>
+__global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int* cellvis)
+{
+    const int lane = threadIdx.x & 63;
+    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (y >= a.H) return;
+    const int lo = lane * 2 * NP;
+    const bool active = lo < (VAR ? (int)a.vcap[(size_t)y * a.W + a.col] : a.cap);
+    const size_t cell = ((size_t)y * a.W + a.col) * a.cs + lo;
+    Cell<NP> agg, p;
+    load_u16<NP>(a.agg + cell, active, agg);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int i = cellvis[2 * y + j];
+        if (i >= 0) {
+            load_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, p);
+#pragma unroll
+            for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], p.v[k]);
+        }
+    }
+    store_u16<NP>(a.agg + cell, active, agg);
+}
+
+// Seeds of interior cells when pass +1 never writes half 0 (H = 1): the initial fill.
+__global__ void fill_u16(uint16_t* p, size_t n, uint16_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------ census and cost volume
+template <typename T, int R>
+__global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* out)
+{
+    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int c = img[(size_t)y * stride + x];
+    uint64_t code = 0;
+    int bit = 0;
+#pragma unroll
+    for (int dy = -R; dy <= R; dy++) {
+        const int yy = min(max(y + dy, 0), H - 1);
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            const int xx = min(max(x + dx, 0), W - 1);
+            const int v = img[(size_t)yy * stride + xx];
+            code |= (uint64_t)(v < c) << bit;
+            bit++;
+        }
+    }
+    out[(size_t)y * W + x] = code;
+}
+
+// Truncated 3x3 absolute-difference cost, four planes per thread (consecutive threads write consecutive
+// 4-byte groups of a cell); blockIdx.y = row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cost_ad_kernel(const T* L, const T* R, int W, int H, int stride, int D, int cs,
+                                                          int shift, int adcap, uint8_t* raw)
+{
+    const int groups = cs >> 2;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < W * groups) {
+        const int y = blockIdx.y;
+        const int x = t / groups, d0 = (t - x * groups) * 4;
+        int lv[9];
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+                lv[(dy + 1) * 3 + dx + 1] = L[(size_t)min(max(y + dy, 0), H - 1) * stride + min(max(x + dx, 0), W - 1)];
+        uint32_t w = 0;
+        for (int j = 0; j < 4; j++) {
+            const int d = d0 + j;
+            uint32_t v = 255u;
+            if (d < D) {
+                const int xr = max(x - d, 0);
+                int s = 0;
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int b = R[(size_t)min(max(y + dy, 0), H - 1) * stride + min(max(xr + dx, 0), W - 1)];
+                        const int df = abs(lv[(dy + 1) * 3 + dx + 1] - b) >> shift;
+                        s += df < adcap ? df : adcap;
+                    }
+                v = (uint32_t)s;
+            }
+            w |= v << (8 * j);
+        }
+        ((uint32_t*)(raw + (size_t)y * W * cs))[t] = w;
+    }
+}
+
+// ------------------------------------------------------------------ coarse-to-fine levels
+// One level down: the rounded mean of each 2x2 block (the last row / column repeats on odd sizes).
+template <typename T>
+__global__ void downsample2_kernel(const T* src, int W, int H, int stride, T* dst, int W2, int H2)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W2 || y >= H2) return;
+    const int x0 = min(2 * x, W - 1), x1 = min(2 * x + 1, W - 1), y0 = min(2 * y, H - 1), y1 = min(2 * y + 1, H - 1);
+    const uint32_t sum = (uint32_t)src[(size_t)y0 * stride + x0] + src[(size_t)y0 * stride + x1] +
+                         src[(size_t)y1 * stride + x0] + src[(size_t)y1 * stride + x1];
+    dst[(size_t)y * W2 + x] = (T)((sum + 2) >> 2);
+}
+
+// The plane range of every pixel of a level from the disparity of the level below (half resolution, half
+// the planes): start = clamp(2 d - r, 0, D - 2r), count = cap = 2r (a multiple of the alignment, so no
+// range's padding reaches the end of its line-buffer segment).
+__global__ void ranges_from_coarse_kernel(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
+                                          uint16_t* vcap, uint16_t* vcount)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)W * H) return;
+    const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+    const int d = dc[(size_t)min(y >> 1, Hc - 1) * Wc + min(x >> 1, Wc - 1)];
+    const int st = max(0, min(2 * d - r, D - 2 * r));
+    vstart[i] = (uint16_t)st;
+    vcap[i] = (uint16_t)(2 * r);
+    vcount[i] = (uint16_t)(2 * r);
+}
+
+// Census costs of per-pixel ranges: raw[y][x][j] = popcount(cl(x) ^ cr(max(x - (start + j), 0))) for j < count,
+// 255 past it (DenseStereo.cs:380); one thread per byte of the cell-major volume.
+__global__ void cost_census_ranges_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
+                                          const uint16_t* vcount, uint8_t* raw)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)W * H;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    const int y = (int)(c / W), x = (int)(c - (size_t)y * W);
+    uint8_t v = 255;
+    if (j < vcount[c]) {
+        const int xr = max(x - ((int)vstart[c] + j), 0);
+        v = (uint8_t)__popcll(cl[c] ^ cr[(size_t)y * W + xr]);
+    }
+    raw[i] = v;
+}
+
+// Jagged per-pixel arrays (cap[i] entries of pixel i at off[i], raster order: DenseLayer Raw/Agg) <-> the
+// cell-major volume with stride cs; entries of the volume past a pixel's cap get `fill`.
+template <typename T>
+__global__ void unpack_kernel(const T* src, const uint64_t* off, const uint16_t* cap, T* dst, int cs, size_t cells, T fill)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    dst[i] = j < cap[c] ? src[off[c] + j] : fill;
+}
+template <typename T>
+__global__ void pack_kernel(const T* src, const uint64_t* off, const uint16_t* cap, T* dst, int cs, size_t cells)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    const int j = (int)(i - c * cs);
+    if (j < cap[c]) dst[off[c] + j] = src[i];
+}
+
+template <typename T>
+__global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t cells, int n, T fill)
+{
+    const size_t total = cells * (size_t)dcs;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t / dcs;
+        const int d = (int)(t - c * dcs);
+        dst[t] = d < n ? src[c * scs + d] : fill;
+    }
+}
+
+inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((total + block - 1) / block, 1u << 20); }
+
+// ------------------------------------------------------------------ launchers
+template <int NP, bool MASKED, bool VAR, bool ADP>
+hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                   hipStream_t s, hipEvent_t* marks)
+{
+    int mk = 0;
+    auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
+    SweepArgs a;
+    a.raw = b.raw;
+    a.agg = b.agg;
+    a.W = g.W; a.H = g.H; a.cs = g.cs; a.cap = g.cap;
+    a.p1x2 = (uint32_t)p1 | ((uint32_t)p1 << 16);
+    a.p2 = (uint32_t)p2;
+    a.seeds = b.seeds;
+    a.seedmins = b.seedmins;
+    a.seed_row = b.seed_row;
+    a.vstart = b.vstart;
+    a.vcap = b.vcap;
+    a.seedstart = b.seedstart;
+    a.seedcap = b.seedcap;
+    a.ap = b.ap;
+    a.seedguid = b.seedguid;
+    RowArgs ra;
+    ra.raw = b.raw; ra.raw_out = b.raw_out; ra.cl = b.cl; ra.cr = b.cr; ra.agg = b.agg;
+    ra.W = g.W; ra.H = g.H; ra.cs = g.cs; ra.cap = g.cap; ra.D = g.D;
+    ra.p1x2 = a.p1x2; ra.p2 = a.p2;
+    ra.disp = o.disp; ra.sub = o.sub; ra.dr = b.dr; ra.dr0 = b.dr0; ra.write_agg = o.write_agg;
+    ra.vstart = b.vstart; ra.vcap = b.vcap; ra.vcount = b.vcount;
+    ra.ap = b.ap;
+    const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
+    auto blocks = [&](int chains) { return (chains + 3) / 4; };
+    const int nh = g.H, nv = g.W, nd = g.W + g.H - 1;
+    if (b.seed_row < 0) {  // pass +1 never writes half 0: every seed is the initial fill
+        fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
+        fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedmins, (size_t)3 * g.W, 2000);
+        if (VAR) fill_u16<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedcap, (size_t)3 * g.W, 0);  // empty ranges
+        if (b.ap.on) fill_u32<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedguid, (size_t)3 * g.W, 0u);
+    }
+    // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
+    auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
+        constexpr int DIR = decltype(dirc)::value;
+        if (NP > 2 || VAR || ADP || !o.bands) {  // the fused bands take neither ranges nor adaptive P2
+            sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, s>>>(a);
+            mark();
+            sgm_sweep<NP, DIR, 2, false, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, s>>>(a);
+            mark();
+            sgm_sweep<NP, DIR, 3, false, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, s>>>(a);
+            mark();
+            return;
+        }
+        if constexpr (NP <= 2 && !VAR && !ADP) {
+            BandArgs ba;
+            ba.raw = b.raw; ba.agg = b.agg; ba.W = g.W; ba.H = g.H; ba.cs = g.cs; ba.cap = g.cap;
+            ba.p1x2 = a.p1x2; ba.p2 = a.p2;
+            ba.cols = cols; ba.colmins = colmins;
+            ba.strips = (g.W + kStrip - 1) / kStrip;
+            // pass +1 bands end at the seed row, whose states become the seed table (half 0 as pass +1 leaves
+            // it for the interior cells): rows [0, seed_row] in bands of kBandRows, then the rest
+            ba.split = DIR > 0 && b.seed_row >= 0 ? b.seed_row + 1 : g.H;
+            const int nbands = (ba.split + kBandRows - 1) / kBandRows + (g.H - ba.split + kBandRows - 1) / kBandRows;
+            ba.bin0 = DIR > 0 ? nullptr : b.seeds;
+            ba.bin0min = DIR > 0 ? nullptr : b.seedmins;
+            ba.bnd = b.bnd;
+            ba.bndmin = b.bndmin;
+            ba.extra = b.seeds;
+            ba.extramin = b.seedmins;
+            ba.extra_band = DIR > 0 && b.seed_row >= 0 ? (ba.split + kBandRows - 1) / kBandRows - 1 : -1;
+            ba.ticket = b.ticket + (DIR > 0 ? 0 : 1);
+            ba.flags = b.flags + (DIR > 0 ? 0 : (size_t)b.max_tiles);
+            ba.epoch = b.epoch;
+            ba.err = b.err;
+            fill_u32<<<1, 1, 0, s>>>((uint32_t*)ba.ticket, 1, 0u);
+            if (b.zero_flags)
+                fill_u32<<<grid_for((size_t)b.max_tiles, 256), 256, 0, s>>>((uint32_t*)ba.flags, (size_t)b.max_tiles, 0u);
+            sgm_band<NP, DIR, MASKED><<<nbands * ba.strips, kBandWaves * 64, 0, s>>>(ba);
+            mark();  // one launch for the three paths: the next two stages are empty
+            mark();
+            mark();
+        }
+    };
+    // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
+    a.cols = b.cols; a.colmins = b.colmins;
+    ra.cols = b.cols; ra.colmins = b.colmins;
+    if constexpr (VAR) {
+        sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+    } else {
+        if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        else sgm_row_fwd<NP, MASKED, false, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+    }
+    mark();
+    vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
+    ReplayArgs r;
+    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
+    r.p1x2 = a.p1x2; r.p2 = a.p2;
+    r.seeds = b.seeds; r.seedmins = b.seedmins;
+    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
+    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
+    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
+    r.ap = b.ap; r.seedguid = b.seedguid;
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
+    sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
+    mark();
+    if (agg_pass1) {
+        hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    // pass -1: up-left, up, up-right (starting from pass +1's half-0 leftovers), then <- with the
+    // winner-take-all fused (the last contribution)
+    a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
+    ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
+    vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
+    sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+    mark();
+    r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
+    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
+    if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
+    sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
+    sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
+    mark();
+    wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
+    mark();
+    return hipGetLastError();
+}
+
+}  // namespace
+
+template <bool ADP>
+hipError_t launch_sgm_t(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                        hipStream_t s, hipEvent_t* m)
+{
+    const bool full = g.cap == 64 * g.dpl;
+    if (b.vstart) {  // per-pixel ranges: one packed pair per lane, per-cell masks
+        if (g.dpl != 2) return hipErrorInvalidValue;
+        return run_sgm<1, true, true, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+    }
+    switch (g.dpl) {
+        case 2: return full ? run_sgm<1, false, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<1, true, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 4: return full ? run_sgm<2, false, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<2, true, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 8: return full ? run_sgm<4, false, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<4, true, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+        case 16: return full ? run_sgm<8, false, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m) : run_sgm<8, true, false, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lux

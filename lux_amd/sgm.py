@@ -20,7 +20,7 @@ COST_CENSUS, COST_AD = 0, 1
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
-           "lux_sgm_compute_layer_ranges", "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
+           "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
            "lux_sgm_server_wait",
@@ -72,6 +72,11 @@ class PyramidParams(ctypes.Structure):
     _fields_ = [("base", RectParams), ("levels", ctypes.c_int32), ("radius", ctypes.c_int32 * 3)]
 
 
+class Guidance(ctypes.Structure):
+    """lux_sgm_guidance: adaptive P2 from an RGBA8 guidance image (DenseStereoSgm.cs:267-274)."""
+    _fields_ = [("guidance", ctypes.c_void_p), ("weights", ctypes.c_float * 4), ("p2_scale", ctypes.c_float)]
+
+
 class LayerParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("width", "height", "planes", "align", "p1", "p2", "tile", "xs_clamp_w")]
 
@@ -104,6 +109,7 @@ def lib():
         L.lux_sgm_rectified_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer.argtypes = [vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer_ranges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
         L.lux_sgm_plan_visits.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32]
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -265,6 +271,25 @@ class DenseLayer:
         self.Threads = 1
         self.KeepPass1 = False
 
+    def set_guidance(self, guidance, weights, p2_scale):
+        """Adaptive P2: Guidance (uint8 [H, W, 4]), P.Guidance weights and P.P2Scale (StereoParams,
+        DenseStereo.cs:15-31); every path step then takes P2 = (int)(P1 * P2Scale * fastexp2(-S))."""
+        self.Guidance = np.ascontiguousarray(guidance, np.uint8).reshape(self.H, self.W, 4)
+        self.GuidanceWeights = tuple(float(w) for w in weights)
+        self.P2Scale = float(p2_scale)
+        return self
+
+    def _guidance_arg(self):
+        g = getattr(self, "Guidance", None)
+        if g is None:
+            return None
+        gd = Guidance()
+        gd.guidance = g.ctypes.data
+        for k in range(4):
+            gd.weights[k] = self.GuidanceWeights[k]
+        gd.p2_scale = self.P2Scale
+        return ctypes.byref(gd)
+
     def set_ranges(self, start, count):
         self.Start = np.ascontiguousarray(start, np.uint16).reshape(-1)
         self.Count = np.ascontiguousarray(count, np.uint16).reshape(-1)
@@ -285,8 +310,8 @@ class DenseLayer:
         self.Agg = np.empty((self.H, self.W, cap), np.uint16)
         self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
         self.Wta = np.empty((self.H, self.W), np.uint16)
-        _check(lib().lux_sgm_compute_layer(self.ctx.handle, ctypes.byref(p), _p(raw), _p(self.Agg), _p(self.AggPass1),
-                                           _p(self.Wta)), self.ctx.handle)
+        _check(lib().lux_sgm_compute_layer_ex(self.ctx.handle, ctypes.byref(p), None, None, self._guidance_arg(), _p(raw),
+                                              _p(self.Agg), _p(self.AggPass1), _p(self.Wta)), self.ctx.handle)
         return self
 
     def _compute_ranges(self):
@@ -300,9 +325,9 @@ class DenseLayer:
         self.Agg = np.empty(max(total, 1), np.uint16)[:total]
         self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
         self.Wta = np.empty((self.H, self.W), np.uint16)
-        _check(lib().lux_sgm_compute_layer_ranges(self.ctx.handle, ctypes.byref(p), _p(self.Start), _p(self.Count),
-                                                  _p(raw), _p(self.Agg), _p(self.AggPass1), _p(self.Wta)),
-               self.ctx.handle)
+        _check(lib().lux_sgm_compute_layer_ex(self.ctx.handle, ctypes.byref(p), _p(self.Start), _p(self.Count),
+                                              self._guidance_arg(), _p(raw), _p(self.Agg), _p(self.AggPass1),
+                                              _p(self.Wta)), self.ctx.handle)
         return self
 
 

@@ -157,3 +157,57 @@ def test_pyramid_matches_oracle(W, H, D, radii, bits, fg, T):
     np.testing.assert_array_equal(r["disp"], disp)
     np.testing.assert_allclose(r["sub"], sub, rtol=0, atol=SUB_TOL)
     np.testing.assert_array_equal(r["sub"], sub)
+
+
+# ------------------------------------------------------------------ adaptive P2 (guidance)
+C_G = float(np.array([0x3ef6384f], np.uint32).view(np.float32)[0])
+L16_WEIGHTS = (C_G / 6, C_G / 16, C_G / 16, 0.0)  # StereoParams.L16Pyramid guidance weights (DenseStereo.cs:17-21)
+
+
+@pytest.mark.parametrize("W,H,T,planes,p1,scale,weights", [
+    (21, 11, 4, 24, 1, 500.0, L16_WEIGHTS), (33, 17, 8, 64, 8, 12.0, (0.3, 0.02, 0.05, 0.7)),
+    (9, 13, 2, 16, 10, 12.0, (2.0, 2.0, 2.0, 0.0)), (70, 40, 64, 256, 3, 40.0, L16_WEIGHTS)])
+def test_layer_adaptive_p2(ctx, W, H, T, planes, p1, scale, weights):
+    rng = np.random.default_rng(W * H + planes)
+    cap = ((planes + 7) // 8) * 8
+    raw = rng.integers(0, 49, (H, W, cap)).astype(np.uint8)
+    raw[:, :, planes:] = 255
+    guid = rng.integers(0, 256, (H, W, 4)).astype(np.uint8)
+    guid[H // 2:, : W // 2] = guid[0, 0]  # flat regions too (S = 0)
+    print(f"adaptive W={W} H={H} T={T} planes={planes} p1={p1} scale={scale}", flush=True)
+    L = sgm.DenseLayer(ctx, W, H, planes, p1=p1, p2=0, tile=T).set_guidance(guid, weights, scale)
+    L.Raw = raw
+    L.KeepPass1 = True
+    L.compute()
+    start = np.zeros(W * H, np.uint16)
+    count = np.full(W * H, planes, np.uint16)
+    agg, agg1, wta = o.layer(raw.reshape(-1), W, H, start, count, T=T, p1=p1, guidance=guid, gw=weights, p2_scale=scale,
+                             adaptive=True, keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1.reshape(H, W, cap))
+    np.testing.assert_array_equal(L.Agg, agg.reshape(H, W, cap))
+    np.testing.assert_array_equal(L.Wta, wta)
+
+
+def test_layer_ranges_adaptive_p2():
+    W, H, T, planes, cmax, align = 40, 23, 8, 96, 40, 8
+    rng = np.random.default_rng(11)
+    start, count = M.random_ranges(rng, W, H, planes, cmax, align, True)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    raws = []
+    for i in range(W * H):
+        r = rng.integers(0, 49, int(caps.flat[i])).astype(np.uint8)
+        r[int(count.flat[i]):] = 255
+        raws.append(r)
+    packed = np.concatenate(raws)
+    guid = rng.integers(0, 256, (H, W, 4)).astype(np.uint8)
+    with sgm.Context(0) as c:
+        L = sgm.DenseLayer(c, W, H, planes, align=align, p1=1, p2=0, tile=T).set_ranges(start, count)
+        L.set_guidance(guid, L16_WEIGHTS, 500.0)
+        L.Raw = packed
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(packed, W, H, start, count, T=T, align=align, p1=1, guidance=guid, gw=L16_WEIGHTS,
+                             p2_scale=500.0, adaptive=True, keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1)
+    np.testing.assert_array_equal(L.Agg, agg)
+    np.testing.assert_array_equal(L.Wta, wta)
