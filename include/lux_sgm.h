@@ -123,6 +123,13 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
 
+/* The skip mask of a layer (DenseLayer.SkipMask, DenseStereoSgm.cs:64-87): out[H*W] = 0 for computed pixels, 0xff
+ * for skipped ones. Mode 0: all computed; mode 2: per 64x64 tile, a generator seeded 5489 picks one pixel of every
+ * 2x2 cell; other modes: LUX_SGM_EUNSUPPORTED (the reference's NotSupportedException, :69). Host only. */
+int lux_sgm_skip_mask(int32_t mode, int32_t w, int32_t h, uint8_t* out);
+/* The n-th output (n >= 1) of the 32-bit Mersenne twister seeded with `seed` (Mt19937, FundamentalMatrixFilter.cs:7-30). */
+uint32_t lux_sgm_mt19937_draw(uint32_t seed, uint32_t n);
+
 /* Image-adaptive P2 (DenseStereoSgm.cs:267-274): every path step takes P2 = (int)(P1 * p2_scale * E),
  * E = fast exp2 of -S, S = (|d0| + |d2|) + |d1|, d_c = (g_c - q_c) * weights[c], g the RGBA guidance of the cell
  * and q that of the cell whose line-buffer state the step reads (0 where nothing was written). */
@@ -132,12 +139,13 @@ typedef struct {
     float p2_scale;            /* StereoParams.P2Scale */
 } lux_sgm_guidance;
 
-/* DenseLayer.Compute in full: per-pixel ranges when start/count are given (else the uniform range of p),
- * adaptive P2 when g is given (else the constant p->p2). raw/agg/agg_pass1 are jagged with ranges, [H*W*cap]
- * without; wta as in the two calls above. */
+/* DenseLayer.Compute in full: per-pixel ranges when start/count are given (else the uniform range of p), a
+ * skip mask when skip is given ([H*W], non-zero = skipped: such pixels aggregate raw 0 over their count and 255
+ * over the padding, :242, whatever their bytes in raw), adaptive P2 when g is given (else the constant p->p2).
+ * raw/agg/agg_pass1 are jagged with ranges, [H*W*cap] without; wta as in the two calls above. */
 int lux_sgm_compute_layer_ex(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
-                             const uint16_t* count, const lux_sgm_guidance* g, const uint8_t* raw, uint16_t* agg,
-                             uint16_t* agg_pass1, uint16_t* wta);
+                             const uint16_t* count, const uint8_t* skip, const lux_sgm_guidance* g, const uint8_t* raw,
+                             uint16_t* agg, uint16_t* agg_pass1, uint16_t* wta);
 
 /* Coarse-to-fine rectified request: `levels` levels of halved resolution (2x2 mean), the coarsest searching
  * all of its planes, every finer level searching 2*radius planes starting at 2*d - radius (clamped) around

@@ -430,7 +430,7 @@ int lux_sgm_pyramid(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void*
 int lux_sgm_compute_layer(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint8_t* raw, uint16_t* agg,
                           uint16_t* agg_pass1, uint16_t* wta)
 {
-    return lux_sgm_compute_layer_ex(c, p, nullptr, nullptr, nullptr, raw, agg, agg_pass1, wta);
+    return lux_sgm_compute_layer_ex(c, p, nullptr, nullptr, nullptr, nullptr, raw, agg, agg_pass1, wta);
 }
 
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start,
@@ -438,12 +438,12 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* c, const lux_sgm_layer_params* p, 
                                  uint16_t* wta)
 {
     if (!start || !count) return fail(c, LUX_SGM_EINVAL, "null ranges");
-    return lux_sgm_compute_layer_ex(c, p, start, count, nullptr, raw, agg, agg_pass1, wta);
+    return lux_sgm_compute_layer_ex(c, p, start, count, nullptr, nullptr, raw, agg, agg_pass1, wta);
 }
 
 int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, const uint16_t* start, const uint16_t* count,
-                             const lux_sgm_guidance* gd, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
-                             uint16_t* wta)
+                             const uint8_t* skip, const lux_sgm_guidance* gd, const uint8_t* raw, uint16_t* agg,
+                             uint16_t* agg_pass1, uint16_t* wta)
 {
     if (!c) return LUX_SGM_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -522,6 +522,12 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         e = hipMemcpyAsync(stage, raw, cells * g.cap, hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
             e = lux::launch_restride_u8((const uint8_t*)stage, g.cap, w.raw, g.cs, (int)cells, g.cap, 255, s);
+    }
+    if (e == hipSuccess && skip) {  // skipped pixels: raw 0 in range, 255 padding, whatever the caller's bytes
+        void* sk;
+        if ((rc = ws.get("skip", cells, &sk, &c->err))) return rc;
+        e = hipMemcpyAsync(sk, skip, cells, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = lux::launch_apply_skip(w.raw, (const uint8_t*)sk, w.b.vcount, maxcount, g.cs, cells, s);
     }
     if (e != hipSuccess) return hip_fail(c, e, "upload raw");
     if (gd) {  // adaptive P2: the guidance image and the seed table's guidance

@@ -98,6 +98,25 @@ hipError_t launch_cost_census_ranges(const uint64_t* cl, const uint64_t* cr, int
     return hipGetLastError();
 }
 
+// Skipped pixels (DenseLayer.Skip != 0) aggregate raw 0 over their planes and 255 over the padding (:242).
+__global__ void apply_skip_kernel(uint8_t* raw, const uint8_t* skip, const uint16_t* vcount, int count, int cs, size_t cells)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * cs) return;
+    const size_t c = i / cs;
+    if (!skip[c]) return;
+    const int j = (int)(i - c * cs);
+    raw[i] = j < (vcount ? (int)vcount[c] : count) ? 0 : 255;
+}
+
+hipError_t launch_apply_skip(uint8_t* raw, const uint8_t* skip, const uint16_t* vcount, int count, int cs, size_t cells,
+                             hipStream_t s)
+{
+    const size_t n = cells * (size_t)cs;
+    apply_skip_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(raw, skip, vcount, count, cs, cells);
+    return hipGetLastError();
+}
+
 hipError_t launch_unpack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
                             hipStream_t s)
 {

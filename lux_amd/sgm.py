@@ -20,7 +20,8 @@ COST_CENSUS, COST_AD = 0, 1
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
-           "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
+           "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_skip_mask", "lux_sgm_mt19937_draw",
+           "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
            "lux_sgm_server_wait",
@@ -109,7 +110,10 @@ def lib():
         L.lux_sgm_rectified_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer.argtypes = [vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer_ranges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
-        L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_skip_mask.argtypes = [i32, i32, i32, vp]
+        L.lux_sgm_mt19937_draw.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.lux_sgm_mt19937_draw.restype = ctypes.c_uint32
         L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
         L.lux_sgm_plan_visits.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32]
         L.lux_sgm_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -290,6 +294,13 @@ class DenseLayer:
         gd.p2_scale = self.P2Scale
         return ctypes.byref(gd)
 
+    def _skip_arg(self):
+        sk = getattr(self, "Skip", None)
+        if sk is None:
+            return None
+        self.Skip = np.ascontiguousarray(sk, np.uint8).reshape(-1)
+        return _p(self.Skip)
+
     def set_ranges(self, start, count):
         self.Start = np.ascontiguousarray(start, np.uint16).reshape(-1)
         self.Count = np.ascontiguousarray(count, np.uint16).reshape(-1)
@@ -310,8 +321,9 @@ class DenseLayer:
         self.Agg = np.empty((self.H, self.W, cap), np.uint16)
         self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
         self.Wta = np.empty((self.H, self.W), np.uint16)
-        _check(lib().lux_sgm_compute_layer_ex(self.ctx.handle, ctypes.byref(p), None, None, self._guidance_arg(), _p(raw),
-                                              _p(self.Agg), _p(self.AggPass1), _p(self.Wta)), self.ctx.handle)
+        _check(lib().lux_sgm_compute_layer_ex(self.ctx.handle, ctypes.byref(p), None, None, self._skip_arg(),
+                                              self._guidance_arg(), _p(raw), _p(self.Agg), _p(self.AggPass1),
+                                              _p(self.Wta)), self.ctx.handle)
         return self
 
     def _compute_ranges(self):
@@ -326,8 +338,8 @@ class DenseLayer:
         self.AggPass1 = np.empty_like(self.Agg) if self.KeepPass1 else None
         self.Wta = np.empty((self.H, self.W), np.uint16)
         _check(lib().lux_sgm_compute_layer_ex(self.ctx.handle, ctypes.byref(p), _p(self.Start), _p(self.Count),
-                                              self._guidance_arg(), _p(raw), _p(self.Agg), _p(self.AggPass1),
-                                              _p(self.Wta)), self.ctx.handle)
+                                              self._skip_arg(), self._guidance_arg(), _p(raw), _p(self.Agg),
+                                              _p(self.AggPass1), _p(self.Wta)), self.ctx.handle)
         return self
 
 
@@ -469,3 +481,15 @@ def pyramid_device(ctx, p, d_left, d_right, d_disp, d_sub=0, stream=0):
     _check(lib().lux_sgm_pyramid_device(ctx.handle, ctypes.byref(p), ctypes.c_void_p(d_left), ctypes.c_void_p(d_right),
                                         ctypes.c_void_p(d_disp or None), ctypes.c_void_p(d_sub or None),
                                         ctypes.c_void_p(stream or None)), ctx.handle)
+
+
+def skip_mask(mode, W, H):
+    """DenseLayer.SkipMask (DenseStereoSgm.cs:64-87): uint8 [H, W], 0 computed, 0xff skipped."""
+    out = np.empty((H, W), np.uint8)
+    _check(lib().lux_sgm_skip_mask(mode, W, H, _p(out)))
+    return out
+
+
+def mt19937_draw(seed, n):
+    """The n-th output of the 32-bit Mersenne twister seeded with `seed`."""
+    return int(lib().lux_sgm_mt19937_draw(seed, n))

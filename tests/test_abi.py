@@ -87,3 +87,41 @@ def test_planner_large_shapes_and_odd_tiles():
     assert n_plus == 480                                  # LUX_SGM_XSW=1 removes pass +1's double visits
     with pytest.raises(sgm.NotSupported):
         sgm.plan_info(64, 48, 5)
+
+
+def test_mt19937_known_answers():
+    """std::mt19937's defining values (C++ [rand.predef]: the 10000th output of the default seed 5489 is
+    4123659995) and numpy's legacy MT19937 stream (same seeding) at several positions."""
+    assert sgm.mt19937_draw(5489, 1) == 3499211612
+    assert sgm.mt19937_draw(5489, 10000) == 4123659995
+    stream = np.random.RandomState(7).randint(0, 2 ** 32, size=1300, dtype=np.uint32)
+    for n in (1, 2, 623, 624, 625, 1248, 1300):
+        assert sgm.mt19937_draw(7, n) == int(stream[n - 1])
+
+
+def _skip_mask_model(w, h):
+    """DenseLayer.SkipMask mode 2 written from its description with numpy's MT19937 (legacy seeding)."""
+    m = np.full((h, w), 0xFF, np.uint8)
+    nx = max(1, w // 64 + (1 if 64 < 2 * (w % 64) else 0))
+    ny = max(1, h // 64 + (1 if 64 < 2 * (h % 64) else 0))
+    for j in range(ny):
+        for i in range(nx):
+            x0, y0 = 64 * i, 64 * j
+            x1 = min(w, x0 + 64 * (2 if i == nx - 1 else 1))
+            y1 = min(h, y0 + 64 * (2 if j == ny - 1 else 1))
+            draws = iter(np.random.RandomState(5489).randint(0, 2 ** 32, size=2 * 64 * 64, dtype=np.uint32))
+            for y in range(y0, y1, 2):
+                for x in range(x0, x1, 2):
+                    dy, dx = int(next(draws)) & 1, int(next(draws)) & 1
+                    if x + dx < w and y + dy < h:
+                        m[y + dy, x + dx] = 0
+    return m
+
+
+@pytest.mark.parametrize("w,h", [(65, 49), (130, 98), (260, 195), (1, 1), (64, 64), (97, 31), (200, 150)])
+def test_skip_mask_mode2(w, h):
+    got = sgm.skip_mask(2, w, h)
+    np.testing.assert_array_equal(got, _skip_mask_model(w, h))
+    assert (sgm.skip_mask(0, w, h) == 0).all()
+    with pytest.raises(sgm.NotSupported):
+        sgm.skip_mask(1, w, h)

@@ -211,3 +211,34 @@ def test_layer_ranges_adaptive_p2():
     np.testing.assert_array_equal(L.AggPass1, agg1)
     np.testing.assert_array_equal(L.Agg, agg)
     np.testing.assert_array_equal(L.Wta, wta)
+
+
+# ------------------------------------------------------------------ skip masks
+@pytest.mark.parametrize("ranged", [False, True])
+def test_layer_skip_mask(ranged):
+    W, H, T, planes, align = 130, 98, 8, 48, 8
+    rng = np.random.default_rng(17)
+    skip = sgm.skip_mask(2, W, H)
+    if ranged:
+        start, count = M.random_ranges(rng, W, H, planes, 24, align, True)
+    else:
+        start, count = np.zeros((H, W), np.uint16), np.full((H, W), planes, np.uint16)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    raws = []
+    for i in range(W * H):
+        r = rng.integers(0, 49, int(caps.flat[i])).astype(np.uint8)
+        r[int(count.flat[i]):] = 255
+        raws.append(r)
+    packed = np.concatenate(raws)
+    with sgm.Context(0) as c:
+        L = sgm.DenseLayer(c, W, H, planes, align=align, p1=8, p2=96, tile=T)
+        if ranged:
+            L.set_ranges(start, count)
+        L.Skip = skip
+        L.Raw = packed if ranged else packed.reshape(H, W, -1)
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(packed, W, H, start, count, T=T, align=align, p1=8, p2=96, skip=skip, keep_pass1=True)
+    np.testing.assert_array_equal(np.asarray(L.AggPass1).reshape(-1), agg1)
+    np.testing.assert_array_equal(np.asarray(L.Agg).reshape(-1), agg)
+    np.testing.assert_array_equal(L.Wta, wta)
