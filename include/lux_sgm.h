@@ -123,6 +123,15 @@ int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
 
+/* The 3x3 box filter the pyramid applies to its inputs (DenseStereo.BoxFilter3, DenseStereo.cs:93-159): per
+ * channel, the in-image window's sum as float times 1/(cols*rows), rounded to nearest even and saturated to the
+ * pixel range. channels 1 (uint8 or uint16 with `bits`) or 4 (RGBA8); stride in pixels; dst is dense
+ * (width*channels per row). _device: device pointers, enqueued on `stream` (NULL = the context's). */
+int lux_sgm_box3_device(lux_sgm_ctx* ctx, const void* d_src, int32_t bits, int32_t channels, int32_t width, int32_t height,
+                        int32_t stride, void* d_dst, void* stream);
+int lux_sgm_box3(lux_sgm_ctx* ctx, const void* src, int32_t bits, int32_t channels, int32_t width, int32_t height,
+                 int32_t stride, void* dst);
+
 /* The skip mask of a layer (DenseLayer.SkipMask, DenseStereoSgm.cs:64-87): out[H*W] = 0 for computed pixels, 0xff
  * for skipped ones. Mode 0: all computed; mode 2: per 64x64 tile, a generator seeded 5489 picks one pixel of every
  * 2x2 cell; other modes: LUX_SGM_EUNSUPPORTED (the reference's NotSupportedException, :69). Host only. */

@@ -376,6 +376,45 @@ int lux_sgm_rectified(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* 
     return lux::check_band_error(ws, s, &c->err);
 }
 
+int lux_sgm_box3_device(lux_sgm_ctx* c, const void* d_src, int32_t bits, int32_t channels, int32_t width, int32_t height,
+                        int32_t stride, void* d_dst, void* stream)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!d_src || !d_dst || width <= 0 || height <= 0 || stride < width) return fail(c, LUX_SGM_EINVAL, "bad image");
+    if (!(bits == 8 || (bits >= 9 && bits <= 16))) return fail(c, LUX_SGM_EINVAL, "bits must be 8 or 9..16");
+    if (channels != 1 && channels != 4) return fail(c, LUX_SGM_EINVAL, "channels must be 1 or 4");
+    if (channels == 4 && bits != 8) return fail(c, LUX_SGM_EINVAL, "RGBA images are 8-bit");
+    (void)hipSetDevice(c->device);
+    hipError_t e = lux::launch_box3(d_src, bits > 8, width, height, stride, channels, (1 << bits) - 1, d_dst,
+                                    stream ? (hipStream_t)stream : c->stream);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "box3");
+}
+
+int lux_sgm_box3(lux_sgm_ctx* c, const void* src, int32_t bits, int32_t channels, int32_t width, int32_t height,
+                 int32_t stride, void* dst)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    if (!src || !dst || width <= 0 || height <= 0 || stride < width) return fail(c, LUX_SGM_EINVAL, "bad image");
+    const size_t esz = bits > 8 ? 2 : 1, inb = (size_t)height * stride * channels * esz,
+                 outb = (size_t)height * width * channels * esz;
+    void *di, *dd;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        int rc;
+        if ((rc = c->ws.get("box_in", inb, &di, &c->err)) || (rc = c->ws.get("box_out", outb, &dd, &c->err))) return rc;
+        (void)hipSetDevice(c->device);
+        hipError_t e = hipMemcpyAsync(di, src, inb, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "upload image");
+    }
+    int rc = lux_sgm_box3_device(c, di, bits, channels, width, height, stride, dd, nullptr);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipError_t e = hipMemcpyAsync(dst, dd, outb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "download image");
+}
+
 int lux_sgm_pyramid_bytes(const lux_sgm_pyramid_params* p, uint64_t* bytes)
 {
     if (!p || !bytes) return LUX_SGM_EINVAL;

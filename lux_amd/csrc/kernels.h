@@ -96,6 +96,8 @@ hipError_t launch_ranges_from_coarse(const uint16_t* dc, int Wc, int Hc, int W, 
                                      uint16_t* vcap, uint16_t* vcount, hipStream_t s);
 hipError_t launch_cost_census_ranges(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
                                      const uint16_t* vcount, uint8_t* raw, hipStream_t s);
+// 3x3 box filter (1 or 4 interleaved channels; stride in pixels; output dense W*ch per row).
+hipError_t launch_box3(const void* src, bool img16, int W, int H, int stride, int ch, int maxval, void* dst, hipStream_t s);
 // Skipped pixels: raw 0 over their count (vcount[c], or `count` when null), 255 past it.
 hipError_t launch_apply_skip(uint8_t* raw, const uint8_t* skip, const uint16_t* vcount, int count, int cs, size_t cells,
                              hipStream_t s);

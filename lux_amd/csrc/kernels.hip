@@ -98,6 +98,32 @@ hipError_t launch_cost_census_ranges(const uint64_t* cl, const uint64_t* cr, int
     return hipGetLastError();
 }
 
+// The 3x3 box filter of the matcher's inputs (DenseStereo.BoxFilter3, DenseStereo.cs:93-159), per channel:
+// (float)sum over the in-image window * (1 / (cols * rows)), rounded to nearest even, saturated to maxval.
+template <typename T>
+__global__ void box3_kernel(const T* src, int W, int H, int stride, int ch, int maxval, T* dst)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int x0 = max(x - 1, 0), x1 = min(x + 1, W - 1), y0 = max(y - 1, 0), y1 = min(y + 1, H - 1);
+    const float inv = __fdiv_rn(1.0f, __fmul_rn((float)(x1 - x0 + 1), (float)(y1 - y0 + 1)));
+    for (int c = 0; c < ch; c++) {
+        int sum = 0;
+        for (int yy = y0; yy <= y1; yy++)
+            for (int xx = x0; xx <= x1; xx++) sum += src[(size_t)yy * stride * ch + (size_t)xx * ch + c];
+        const int r = __float2int_rn(__fmul_rn((float)sum, inv));
+        dst[((size_t)y * W + x) * ch + c] = (T)min(max(r, 0), maxval);
+    }
+}
+
+hipError_t launch_box3(const void* src, bool img16, int W, int H, int stride, int ch, int maxval, void* dst, hipStream_t s)
+{
+    dim3 blk(32, 8), grd((W + 31) / 32, (H + 7) / 8);
+    if (img16) box3_kernel<uint16_t><<<grd, blk, 0, s>>>((const uint16_t*)src, W, H, stride, ch, maxval, (uint16_t*)dst);
+    else box3_kernel<uint8_t><<<grd, blk, 0, s>>>((const uint8_t*)src, W, H, stride, ch, maxval, (uint8_t*)dst);
+    return hipGetLastError();
+}
+
 // Skipped pixels (DenseLayer.Skip != 0) aggregate raw 0 over their planes and 255 over the padding (:242).
 __global__ void apply_skip_kernel(uint8_t* raw, const uint8_t* skip, const uint16_t* vcount, int count, int cs, size_t cells)
 {

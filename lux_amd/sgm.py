@@ -21,6 +21,7 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
            "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_skip_mask", "lux_sgm_mt19937_draw",
+           "lux_sgm_box3", "lux_sgm_box3_device",
            "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
@@ -112,6 +113,8 @@ def lib():
         L.lux_sgm_compute_layer_ranges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_skip_mask.argtypes = [i32, i32, i32, vp]
+        L.lux_sgm_box3.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp]
+        L.lux_sgm_box3_device.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp]
         L.lux_sgm_mt19937_draw.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.lux_sgm_mt19937_draw.restype = ctypes.c_uint32
         L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
@@ -493,3 +496,14 @@ def skip_mask(mode, W, H):
 def mt19937_draw(seed, n):
     """The n-th output of the 32-bit Mersenne twister seeded with `seed`."""
     return int(lib().lux_sgm_mt19937_draw(seed, n))
+
+
+def box3(ctx, img, bits=None):
+    """DenseStereo.BoxFilter3 on the device: uint8/uint16 [H, W] or RGBA8 [H, W, 4]."""
+    img = np.ascontiguousarray(img)
+    ch = img.shape[2] if img.ndim == 3 else 1
+    bits = bits or (8 if img.dtype == np.uint8 else 16)
+    out = np.empty_like(img)
+    H, W = img.shape[:2]
+    _check(lib().lux_sgm_box3(ctx.handle, _p(img), bits, ch, W, H, W, _p(out)), ctx.handle)
+    return out

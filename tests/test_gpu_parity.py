@@ -242,3 +242,44 @@ def test_layer_skip_mask(ranged):
     np.testing.assert_array_equal(np.asarray(L.AggPass1).reshape(-1), agg1)
     np.testing.assert_array_equal(np.asarray(L.Agg).reshape(-1), agg)
     np.testing.assert_array_equal(L.Wta, wta)
+
+
+# ------------------------------------------------------------------ Lumen dumps as the wire format
+def test_layer_through_lumen_dump(tmp_path):
+    from lux_amd import lumen
+
+    W, H, T, planes, align = 37, 21, 4, 64, 8
+    rng = np.random.default_rng(23)
+    start, count = M.random_ranges(rng, W, H, planes, 24, align, True)
+    skip = sgm.skip_mask(2, W, H)
+    caps = lumen.caps_of(count.reshape(-1), align)
+    raw = rng.integers(0, 49, int(caps.sum())).astype(np.uint8)
+    src = str(tmp_path / "in")
+    lumen.write_costvol(src, W, H, start, count, align, np.zeros(int(caps.sum()), np.uint16), raw=raw, skip=skip)
+    with sgm.Context(0) as c:
+        L = lumen.layer_from_dump(c, src, planes, align=align, p1=8, p2=96, tile=T)
+        L.KeepPass1 = True
+        L.compute()
+    out = str(tmp_path / "gpu")
+    lumen.dump_layer(out, L)
+    d = lumen.read_costvol(src)
+    agg, agg1, _ = o.layer(d["raw"], W, H, start, count, T=T, align=align, p1=8, p2=96, skip=skip, keep_pass1=True)
+    ref = str(tmp_path / "ref")
+    lumen.write_costvol(ref, W, H, start, count, align, agg, raw=d["raw"], skip=skip, agg_pass1=agg1)
+    for suffix in ("_offsets.u32", "_costvol.bin", "_costvol_pass1.bin"):
+        assert open(out + suffix, "rb").read() == open(ref + suffix, "rb").read(), suffix
+
+
+# ------------------------------------------------------------------ box filter
+@pytest.mark.parametrize("shape,dtype,bits", [((97, 131), np.uint8, 8), ((64, 48), np.uint16, 10), ((1, 5), np.uint8, 8),
+                                              ((7, 1), np.uint16, 16), ((33, 20, 4), np.uint8, 8)])
+def test_box3(shape, dtype, bits):
+    rng = np.random.default_rng(sum(shape))
+    img = rng.integers(0, 1 << bits, shape).astype(dtype)
+    with sgm.Context(0) as c:
+        got = sgm.box3(c, img, bits)
+    if img.ndim == 3:
+        want = np.stack([o.box3(np.ascontiguousarray(img[:, :, k]), 255) for k in range(4)], axis=2)
+    else:
+        want = o.box3(img, (1 << bits) - 1)
+    np.testing.assert_array_equal(got, want)
