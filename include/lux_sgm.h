@@ -132,6 +132,107 @@ int lux_sgm_box3_device(lux_sgm_ctx* ctx, const void* d_src, int32_t bits, int32
 int lux_sgm_box3(lux_sgm_ctx* ctx, const void* src, int32_t bits, int32_t channels, int32_t width, int32_t height,
                  int32_t stride, void* dst);
 
+/* ---- Plane-sweep matching cost of the L16 dense stereo (DenseStereo.cs:17-381) ---- */
+#define LUX_SGM_MAX_VIEWS 16
+/* One other camera of the cost context (DenseStereo.CostContext, BuildContext :190-200). */
+typedef struct lux_sgm_view {
+    const uint8_t* rgba;   /* box-filtered RGBA8 image (BoxFilter3), `height` dense rows of `width` pixels, >= 4 x 4 */
+    int32_t width, height;
+    float warp[16];        /* Warp(reference, camera) = Mat4D.FlowMatrix, column-major (:186) */
+    uint16_t wq[4];        /* channel weights: {8160, 680, 680, 0} colour, {12240, 0, 0, 0} gray (:197) */
+} lux_sgm_view;
+typedef struct lux_sgm_sweep {
+    const uint8_t* ref;    /* the box-filtered reference image, dense RGBA8 */
+    int32_t ref_width, ref_height;
+    int32_t n_views;       /* 1 .. LUX_SGM_MAX_VIEWS */
+    const lux_sgm_view* views;
+    const float* planes;   /* DenseStereo.Planes: metric depths, far .. near */
+    int32_t n_planes;
+    uint8_t caps[4];       /* StereoParams.Caps, {2, 6, 6, 0} */
+} lux_sgm_sweep;
+
+/* The raw cost volume of one layer as DenseLayer.Compute produces it before the sweeps (DenseStereoSgm.cs:186-205):
+ * MatchingCost + Normalise (DenseStereo.cs:217-381) for every computed pixel. Layer pixel (x, y) reads reference pixel
+ * (min(scale*x + scale/2, ref_width-1), min(scale*y + scale/2, ref_height-1)). start/count: the per-pixel plane ranges
+ * (both null: every pixel searches all planes); skip: 0 computed, else skipped (may be null). raw: packed pixel by
+ * pixel, roundup(count, align) bytes each, raster order; skipped pixels get 0 over their planes and 255 over the
+ * padding (the bytes Pixel uses for them, :242). Bit-exact with the reference, the vector and scalar forms alike. */
+int lux_sgm_sweep_cost(lux_sgm_ctx* ctx, const lux_sgm_sweep* sweep, int32_t width, int32_t height, int32_t scale,
+                       int32_t align, const uint16_t* start, const uint16_t* count, const uint8_t* skip, uint8_t* raw);
+
+/* RangeFromPrevious (DenseStereoSgm.cs:91-140): the plane range of every pixel of a width x height layer from the
+ * previous layer's depth [ph*pw] and skip mask (may be null): the plane index of each previous pixel (rcpss against
+ * the midpoints of neighbouring planes), its min / max over the 4 x 4 window of computed pixels around the resampled
+ * position, widened by `margin` below and one plane above. Writes start / count (16 bits, as the reference stores
+ * them) and MinLo / MaxHi (MaxHi not rounded to the alignment). */
+int lux_sgm_range_from_previous(lux_sgm_ctx* ctx, const float* planes, int32_t n, const float* prev_depth,
+                                const uint8_t* prev_skip, int32_t pw, int32_t ph, int32_t width, int32_t height,
+                                int32_t margin, uint16_t* start, uint16_t* count, int32_t* min_lo, int32_t* max_hi);
+
+/* One layer of the L16 dense stereo: StereoParams of the layer plus its size. */
+typedef struct lux_sgm_dense_params {
+    int32_t width, height;  /* (ref_width + scale - 1) / scale, (ref_height + scale - 1) / scale */
+    int32_t scale;          /* StereoParams.Scale: 32, 16, 8, 4, 2, 1 */
+    int32_t tile;           /* StereoParams.Tile: 4, 8, 16, 32, 64, 64 */
+    int32_t align;          /* 8, or 2 in the pyramid's low-memory mode */
+    int32_t p1;             /* StereoParams.P1 (1) */
+    float p2_scale;         /* StereoParams.P2Scale (500) */
+    float guidance[4];      /* StereoParams.Guidance: the adaptive-P2 weights */
+} lux_sgm_dense_params;
+
+/* DenseLayer.Compute (DenseStereoSgm.cs:181-232) of a layer whose ranges and skip mask are set (Init, :34-62):
+ * the plane-sweep cost of every computed pixel, the image-adaptive eight-path aggregation (the guidance image is the
+ * reference subsampled at the layer's scale, :39), the first-minimum WTA and depth = Planes[start + arg] [H*W].
+ * start/count both null: every pixel searches all planes (layer 0). skip may be null. agg / agg_pass1 (packed like
+ * the raw volume) and wta (the plane index) may be null. Returns LUX_SGM_EUNSUPPORTED where the ranged aggregation
+ * is (ranges above 128 lanes, padding past the line-buffer segment) and for more than 1024 planes at layer 0. */
+int lux_sgm_dense_layer(lux_sgm_ctx* ctx, const lux_sgm_sweep* sweep, const lux_sgm_dense_params* params,
+                        const uint16_t* start, const uint16_t* count, const uint8_t* skip, float* depth, uint16_t* agg,
+                        uint16_t* agg_pass1, uint16_t* wta);
+
+/* DenseStereoPyramid.Run (DenseStereoSgm.cs:328-355) on the device: the images are box-filtered once, then each
+ * level is a DenseLayer Init (ranges from the previous level's depth and skip mask, the skip mask of its mode) and
+ * Compute. From level 1 on, the memory estimate (DenseLayer.MemoryEstimate, :142-156) picks align 8 if it stays under
+ * `budget`, else align 2 with the align-2 planes; above the budget with both it fails (LUX_SGM_EUNSUPPORTED, the
+ * reference's NotSupportedException). */
+typedef struct lux_sgm_dense_pyramid_params {
+    int32_t levels;         /* 6 */
+    int32_t scale[8];       /* 32, 16, 8, 4, 2, 1 */
+    int32_t tile[8];        /* 4, 8, 16, 32, 64, 64 */
+    int32_t skip_mode[8];   /* 0, 0, 0, 0, 2, 2 */
+    int32_t p1;             /* 1 */
+    float p2_scale;         /* 500 */
+    float guidance[4];      /* c/6, c/16, c/16, 0 with c = 0x3ef6384f */
+    int32_t margin;         /* 1 */
+    uint8_t caps[4];        /* 2, 6, 6, 0 */
+    uint64_t budget;        /* 1 GiB */
+} lux_sgm_dense_pyramid_params;
+/* StereoParams.L16Pyramid() and DenseStereoPyramid.Budget. */
+void lux_sgm_dense_pyramid_defaults(lux_sgm_dense_pyramid_params* params);
+/* images[0] is the reference; every image is RGBA8 with dense rows (not yet filtered). gray[k] selects the gray
+ * weights for camera k (may be null: colour). warps: (n_images - 1) x 16 floats, Warp(reference, camera k) for
+ * k = 1.. planes8 / planes2: DenseStereo.Planes(near, far, calibs, density, 8 / 2) (planes2 may be null when the
+ * budget is never exceeded). depths[i] (may be null): level i's depth, ((W0 + s - 1) / s) x ((H0 + s - 1) / s) for its
+ * scale s. aligns[i] (may be null) receives each level's alignment. */
+int lux_sgm_dense_pyramid(lux_sgm_ctx* ctx, const lux_sgm_dense_pyramid_params* params, int32_t n_images,
+                          const uint8_t* const* images, const int32_t* widths, const int32_t* heights, const uint8_t* gray,
+                          const float* warps, const float* planes8, int32_t n8, const float* planes2, int32_t n2,
+                          float* const* depths, int32_t* aligns);
+
+/* DenseUpsampleLayer.Run (DenseUpsampleLayer.cs:35-48): the finest layer's depth prev_depth [ph*pw] float to the
+ * guide's resolution. The RGBA8 guide [height*width*4] is shifted by half a pixel with the 4-tap cubic, resized to
+ * pw x ph (16.16 nearest), and steers a bilateral upsample (sigma, 12 in the reference) of the depth's inverse (rcpss);
+ * depth [height*width] is the inverse of the result. prev_conf (may be null; then conf is not written) goes through
+ * the same bilateral weights with no inverse into conf. width/height must be 2*set_w / 2*set_h, else EINVAL
+ * "Incorrect output depth size!". Every float operation is the reference's, in its order, and rcpss is Intel's: the
+ * outputs are the reference's bit for bit. _device: device pointers, enqueued on `stream`. */
+int lux_sgm_upsample_layer(lux_sgm_ctx* ctx, const float* prev_depth, const float* prev_conf, int32_t pw, int32_t ph,
+                           const uint8_t* guide, int32_t width, int32_t height, int32_t set_w, int32_t set_h, float sigma,
+                           float* depth, float* conf);
+int lux_sgm_upsample_layer_device(lux_sgm_ctx* ctx, const float* d_prev_depth, const float* d_prev_conf, int32_t pw,
+                                  int32_t ph, const uint8_t* d_guide, int32_t width, int32_t height, int32_t set_w,
+                                  int32_t set_h, float sigma, float* d_depth, float* d_conf, void* stream);
+
 /* The skip mask of a layer (DenseLayer.SkipMask, DenseStereoSgm.cs:64-87): out[H*W] = 0 for computed pixels, 0xff
  * for skipped ones. Mode 0: all computed; mode 2: per 64x64 tile, a generator seeded 5489 picks one pixel of every
  * 2x2 cell; other modes: LUX_SGM_EUNSUPPORTED (the reference's NotSupportedException, :69). Host only. */

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -12,51 +13,17 @@
 #include <vector>
 
 #include "../../include/lux_sgm.h"
+#include "ctx.h"
 #include "engine.h"
 
 namespace {
 
 thread_local std::string g_create_error;
 
-constexpr int kMaxBranches = 8;
-
-struct GraphEntry {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    uint64_t gen = 0;
-};
-
-}  // namespace
-
-struct lux_sgm_ctx {
-    int device = 0;
-    std::string err;
-    std::mutex mu;
-    hipStream_t stream = nullptr;
-    lux::Budget budget;
-    lux::Workspace ws;                                   // single requests and layers
-    std::vector<std::unique_ptr<lux::Workspace>> bws;     // one per pair of a batch
-    hipStream_t side[kMaxBranches] = {};
-    hipEvent_t fork = nullptr, join[kMaxBranches] = {};
-    std::map<std::string, GraphEntry> graphs;             // batch descriptor -> instantiated graph
-    std::map<std::string, lux_sgm_schedule> tuned;        // batch shapes -> schedule
-    uint64_t gen = 0;                                     // bumped when a batch workspace is reallocated
-    int opt_bands = 0;
-    // stage timing (lux_sgm_set_profiling): events recorded between the launches of one request
-    bool profiling = false;
-    hipEvent_t ev[16] = {};
-    int nev = 0;
-};
-
-namespace {
-
-int fail(lux_sgm_ctx* c, int code, const std::string& msg)
-{
-    if (c) c->err = msg;
-    return code;
-}
-
-int hip_fail(lux_sgm_ctx* c, hipError_t e, const char* where) { return lux::hip_status(e, where, c ? &c->err : nullptr); }
+using lux::fail;
+using lux::GraphEntry;
+using lux::hip_fail;
+using lux::kMaxBranches;
 
 int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, const void* dr, const lux::RectOut& o,
                 hipStream_t s)
@@ -413,6 +380,61 @@ int lux_sgm_box3(lux_sgm_ctx* c, const void* src, int32_t bits, int32_t channels
     hipError_t e = hipMemcpyAsync(dst, dd, outb, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e == hipSuccess ? 0 : hip_fail(c, e, "download image");
+}
+
+static int check_upsample(lux_sgm_ctx* c, int pw, int ph, int W, int H, int setW, int setH)
+{
+    if (pw <= 0 || ph <= 0 || W <= 0 || H <= 0) return fail(c, LUX_SGM_EINVAL, "bad sizes");
+    if (W != setW * 2 || H != setH * 2) return fail(c, LUX_SGM_EINVAL, "Incorrect output depth size!");  // (:46-47)
+    if ((int64_t)W * 65536 > INT32_MAX || (int64_t)H * 65536 > INT32_MAX) return fail(c, LUX_SGM_EUNSUPPORTED, "image too large");
+    return 0;
+}
+
+int lux_sgm_upsample_layer_device(lux_sgm_ctx* c, const float* d_prev_depth, const float* d_prev_conf, int32_t pw, int32_t ph,
+                                  const uint8_t* d_guide, int32_t width, int32_t height, int32_t set_w, int32_t set_h,
+                                  float sigma, float* d_depth, float* d_conf, void* stream)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_upsample(c, pw, ph, width, height, set_w, set_h);
+    if (rc) return rc;
+    if (!d_prev_depth || !d_guide || !d_depth) return fail(c, LUX_SGM_EINVAL, "null argument");
+    if (d_prev_conf && !d_conf) return fail(c, LUX_SGM_EINVAL, "a previous confidence needs a confidence output");
+    (void)hipSetDevice(c->device);
+    return lux::enqueue_upsample(c->ws, d_prev_depth, d_prev_conf, pw, ph, d_guide, width, height, sigma, d_depth, d_conf,
+                                 stream ? (hipStream_t)stream : c->stream, &c->err);
+}
+
+int lux_sgm_upsample_layer(lux_sgm_ctx* c, const float* prev_depth, const float* prev_conf, int32_t pw, int32_t ph,
+                           const uint8_t* guide, int32_t width, int32_t height, int32_t set_w, int32_t set_h, float sigma,
+                           float* depth, float* conf)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_upsample(c, pw, ph, width, height, set_w, set_h);
+    if (rc) return rc;
+    if (!prev_depth || !guide || !depth) return fail(c, LUX_SGM_EINVAL, "null argument");
+    if (prev_conf && !conf) return fail(c, LUX_SGM_EINVAL, "a previous confidence needs a confidence output");
+    (void)hipSetDevice(c->device);
+    hipStream_t s = c->stream;
+    const size_t n = (size_t)pw * ph, m = (size_t)width * height;
+    void *dp, *dg, *dd, *dpc = nullptr, *dc = nullptr;
+    if ((rc = c->ws.get("ups_prev", n * 4, &dp, &c->err)) || (rc = c->ws.get("ups_guide", m * 4, &dg, &c->err)) ||
+        (rc = c->ws.get("ups_out", m * 4, &dd, &c->err)))
+        return rc;
+    if (prev_conf && ((rc = c->ws.get("ups_prev_conf", n * 4, &dpc, &c->err)) || (rc = c->ws.get("ups_conf", m * 4, &dc, &c->err))))
+        return rc;
+    hipError_t e = hipMemcpyAsync(dp, prev_depth, n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dg, guide, m * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && prev_conf) e = hipMemcpyAsync(dpc, prev_conf, n * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e, "upload");
+    if ((rc = lux::enqueue_upsample(c->ws, (const float*)dp, (const float*)dpc, pw, ph, (const uint8_t*)dg, width, height, sigma,
+                                    (float*)dd, (float*)dc, s, &c->err)))
+        return rc;
+    e = hipMemcpyAsync(depth, dd, m * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && prev_conf) e = hipMemcpyAsync(conf, dc, m * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "download depth");
 }
 
 int lux_sgm_pyramid_bytes(const lux_sgm_pyramid_params* p, uint64_t* bytes)

@@ -1,0 +1,57 @@
+// The context behind lux_sgm_ctx* and the error helpers of the C ABI, shared by the translation units that
+// implement entry points (api.hip, sweep.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lux_sgm.h"
+#include "engine.h"
+
+namespace lux {
+
+constexpr int kMaxBranches = 8;
+
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+};
+
+}  // namespace lux
+
+struct lux_sgm_ctx {
+    int device = 0;
+    std::string err;
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    lux::Budget budget;
+    lux::Workspace ws;                                   // single requests and layers
+    std::vector<std::unique_ptr<lux::Workspace>> bws;     // one per pair of a batch
+    hipStream_t side[lux::kMaxBranches] = {};
+    hipEvent_t fork = nullptr, join[lux::kMaxBranches] = {};
+    std::map<std::string, lux::GraphEntry> graphs;        // batch descriptor -> instantiated graph
+    std::map<std::string, lux_sgm_schedule> tuned;        // batch shapes -> schedule
+    uint64_t gen = 0;                                     // bumped when a batch workspace is reallocated
+    int opt_bands = 0;
+    // stage timing (lux_sgm_set_profiling): events recorded between the launches of one request
+    bool profiling = false;
+    hipEvent_t ev[16] = {};
+    int nev = 0;
+};
+
+namespace lux {
+
+inline int fail(lux_sgm_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+inline int hip_fail(lux_sgm_ctx* c, hipError_t e, const char* where) { return hip_status(e, where, c ? &c->err : nullptr); }
+
+}  // namespace lux

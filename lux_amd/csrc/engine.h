@@ -111,6 +111,11 @@ int pyramid_buffers(const lux_sgm_pyramid_params* p, size_t host_img_bytes, std:
 int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* dl, const void* dr, uint16_t* disp,
                     float* sub, hipStream_t s, std::string* err);
 
+// DenseUpsampleLayer (upsample.hip): prev depth (and confidence, may be null) pw x ph and RGBA8 guide W x H (device)
+// -> depth (and confidence when both confidence pointers are set) W x H.
+int enqueue_upsample(Workspace& ws, const float* d_prev, const float* d_prev_conf, int pw, int ph, const uint8_t* d_guide,
+                     int W, int H, float sigma, float* d_depth, float* d_conf, hipStream_t s, std::string* err);
+
 // After s has drained: a band tile that gave up waiting for its neighbours leaves an error word set.
 int check_band_error(Workspace& ws, hipStream_t s, std::string* err);
 

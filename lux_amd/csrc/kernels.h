@@ -104,6 +104,8 @@ hipError_t launch_apply_skip(uint8_t* raw, const uint8_t* skip, const uint16_t* 
 // Jagged per-pixel arrays (cap[i] entries at off[i]) <-> the cell-major volume of stride cs (255 past a cap).
 hipError_t launch_unpack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
                             hipStream_t s);
+hipError_t launch_pack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
+                          hipStream_t s);
 hipError_t launch_pack_u16(const uint16_t* src, const uint64_t* off, const uint16_t* cap, uint16_t* dst, int cs, size_t cells,
                            hipStream_t s);
 hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,

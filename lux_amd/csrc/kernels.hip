@@ -159,6 +159,14 @@ hipError_t launch_pack_u16(const uint16_t* src, const uint64_t* off, const uint1
     return hipGetLastError();
 }
 
+hipError_t launch_pack_u8(const uint8_t* src, const uint64_t* off, const uint16_t* cap, uint8_t* dst, int cs, size_t cells,
+                          hipStream_t s)
+{
+    const size_t total = cells * (size_t)cs;
+    pack_kernel<uint8_t><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(src, off, cap, dst, cs, cells);
+    return hipGetLastError();
+}
+
 hipError_t launch_restride_u8(const uint8_t* src, int src_cs, uint8_t* dst, int dst_cs, int cells, int n, uint8_t fill,
                               hipStream_t s)
 {

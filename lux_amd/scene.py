@@ -113,3 +113,53 @@ def sha256(*arrays):
     for a in arrays:
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def plane_list(near, far, n):
+    """n metric plane depths evenly spaced in inverse depth, far .. near, the last one exactly `near` (the spacing of
+    DenseStereo.Planes, DenseStereo.cs:41-88, without its reciprocal approximation)."""
+    inv = np.linspace(1.0 / far, 1.0 / near, n)
+    p = (1.0 / inv).astype(np.float32)
+    p[-1] = np.float32(near)
+    return p
+
+
+def multiview(W0, H0, *, fbs=(4000.0, -6000.0, 8000.0, -2000.0), gray=(False, False, True, False), z_bg=2000.0,
+              z_fg=1000.0, rects=2, seed=0):
+    """A reference camera and len(fbs) rectified cameras looking at a textured fronto-parallel background at z_bg with
+    `rects` nearer rectangles at z_fg. Camera k sees reference pixel x at x - fbs[k]/z + M (M = a margin that keeps
+    every projection of the planes inside its wider image); its warp is the flow matrix of that map (column-major:
+    m0 = m5 = m10 = m15 = 1, m8 = M, m12 = -fb). Images are RGBA8 noise (numpy PCG64, `seed`), not yet box-filtered;
+    a gray camera's image repeats channel 0. Returns (images, gray flags incl. the reference, warps [k][16], depth)."""
+    rng = np.random.default_rng(seed)
+    M = int(max(abs(f) for f in fbs) / min(z_bg, z_fg)) + 8
+    Wt = W0 + 2 * M
+    bg = rng.integers(0, 256, (H0, Wt, 4), dtype=np.uint8)
+    fg = rng.integers(0, 256, (H0, W0, 4), dtype=np.uint8)
+    depth = np.full((H0, W0), np.float32(z_bg), np.float32)
+    ref = bg[:, M:M + W0].copy()
+    boxes = []
+    for _ in range(rects):
+        w, h = int(rng.integers(W0 // 6, W0 // 3)), int(rng.integers(H0 // 6, H0 // 3))
+        x0, y0 = int(rng.integers(M, W0 - w - M)), int(rng.integers(0, H0 - h))
+        boxes.append((x0, y0, w, h))
+        ref[y0:y0 + h, x0:x0 + w] = fg[y0:y0 + h, x0:x0 + w]
+        depth[y0:y0 + h, x0:x0 + w] = z_fg
+    images, warps = [ref], []
+    for fb, g in zip(fbs, gray):
+        sb, sf = int(round(fb / z_bg)), int(round(fb / z_fg))
+        img = np.empty((H0, Wt, 4), np.uint8)
+        # background: camera pixel x' shows reference-plane pixel x = x' + sb - M of the wide texture
+        xs = np.clip(np.arange(Wt) + sb, 0, Wt - 1)
+        img[:] = bg[:, xs]
+        for x0, y0, w, h in boxes:
+            img[y0:y0 + h, x0 - sf + M:x0 - sf + M + w] = fg[y0:y0 + h, x0:x0 + w]
+        if g:
+            img[:, :, 1] = img[:, :, 2] = img[:, :, 3] = img[:, :, 0]
+        images.append(img)
+        m = np.zeros(16, np.float32)
+        m[0] = m[5] = m[10] = m[15] = 1.0
+        m[8] = M
+        m[12] = -fb
+        warps.append(m)
+    return images, [False] + list(gray), warps, depth

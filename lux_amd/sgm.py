@@ -21,7 +21,9 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
            "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_skip_mask", "lux_sgm_mt19937_draw",
-           "lux_sgm_box3", "lux_sgm_box3_device",
+           "lux_sgm_box3", "lux_sgm_box3_device", "lux_sgm_upsample_layer", "lux_sgm_upsample_layer_device",
+           "lux_sgm_sweep_cost", "lux_sgm_range_from_previous", "lux_sgm_dense_layer", "lux_sgm_dense_pyramid",
+           "lux_sgm_dense_pyramid_defaults",
            "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
@@ -114,6 +116,15 @@ def lib():
         L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_skip_mask.argtypes = [i32, i32, i32, vp]
         L.lux_sgm_box3.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp]
+        L.lux_sgm_sweep_cost.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]
+        L.lux_sgm_range_from_previous.argtypes = [vp, vp, i32, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp]
+        L.lux_sgm_dense_layer.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_dense_pyramid.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp]
+        L.lux_sgm_dense_pyramid_defaults.argtypes = [vp]
+        L.lux_sgm_dense_pyramid_defaults.restype = None
+        L.lux_sgm_upsample_layer.argtypes = [vp, vp, vp, i32, i32, vp, i32, i32, i32, i32, ctypes.c_float, vp, vp]
+        L.lux_sgm_upsample_layer_device.argtypes = [vp, vp, vp, i32, i32, vp, i32, i32, i32, i32, ctypes.c_float, vp, vp,
+                                                    vp]
         L.lux_sgm_box3_device.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp]
         L.lux_sgm_mt19937_draw.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.lux_sgm_mt19937_draw.restype = ctypes.c_uint32
@@ -507,3 +518,176 @@ def box3(ctx, img, bits=None):
     H, W = img.shape[:2]
     _check(lib().lux_sgm_box3(ctx.handle, _p(img), bits, ch, W, H, W, _p(out)), ctx.handle)
     return out
+
+
+def upsample_layer(ctx, prev_depth, guide, set_w, set_h, sigma=12.0, prev_conf=None):
+    """DenseUpsampleLayer.Run (DenseUpsampleLayer.cs:35-48) on the device: float [ph, pw] depth + RGBA8 guide
+    [H, W, 4] -> float [H, W] depth, or (depth, confidence) when ``prev_conf`` [ph, pw] is given."""
+    prev_depth = np.ascontiguousarray(prev_depth, np.float32)
+    guide = np.ascontiguousarray(guide, np.uint8)
+    H, W = guide.shape[:2]
+    out = np.empty((H, W), np.float32)
+    conf = None
+    if prev_conf is not None:
+        prev_conf = np.ascontiguousarray(prev_conf, np.float32)
+        assert prev_conf.shape == prev_depth.shape
+        conf = np.empty((H, W), np.float32)
+    _check(lib().lux_sgm_upsample_layer(ctx.handle, _p(prev_depth), None if prev_conf is None else _p(prev_conf),
+                                        prev_depth.shape[1], prev_depth.shape[0], _p(guide), W, H, set_w, set_h, sigma,
+                                        _p(out), None if conf is None else _p(conf)), ctx.handle)
+    return out if conf is None else (out, conf)
+
+
+def upsample_layer_device(ctx, d_prev, pw, ph, d_guide, W, H, set_w, set_h, d_depth, sigma=12.0, d_prev_conf=0, d_conf=0,
+                          stream=0):
+    """Device-pointer form of upsample_layer: pointers are ints (torch data_ptr()); enqueue only."""
+    _check(lib().lux_sgm_upsample_layer_device(ctx.handle, ctypes.c_void_p(d_prev), ctypes.c_void_p(d_prev_conf or None), pw,
+                                               ph, ctypes.c_void_p(d_guide), W, H, set_w, set_h, sigma,
+                                               ctypes.c_void_p(d_depth), ctypes.c_void_p(d_conf or None),
+                                               ctypes.c_void_p(stream or None)), ctx.handle)
+
+
+class View(ctypes.Structure):
+    """lux_sgm_view: one other camera of DenseStereo.CostContext."""
+    _fields_ = [("rgba", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("warp", ctypes.c_float * 16), ("wq", ctypes.c_uint16 * 4)]
+
+
+class Sweep(ctypes.Structure):
+    """lux_sgm_sweep: the cost context of a layer (reference, cameras, planes, caps)."""
+    _fields_ = [("ref", ctypes.c_void_p), ("ref_width", ctypes.c_int32), ("ref_height", ctypes.c_int32),
+                ("n_views", ctypes.c_int32), ("views", ctypes.c_void_p), ("planes", ctypes.c_void_p),
+                ("n_planes", ctypes.c_int32), ("caps", ctypes.c_uint8 * 4)]
+
+
+class CostContext:
+    """Mirror of DenseStereo.CostContext (DenseStereo.cs:180-200): ``Ref`` and ``Others`` (box-filtered RGBA8
+    [h, w, 4]), ``Warps`` (16 floats each, column-major), ``Wq`` and ``Planes``; holds the arrays the C struct
+    points at."""
+
+    def __init__(self, ref, others, warps, wq, planes, caps=(2, 6, 6, 0)):
+        self.Ref = np.ascontiguousarray(ref, np.uint8)
+        self.Others = [np.ascontiguousarray(o, np.uint8) for o in others]
+        self.Warps = [np.ascontiguousarray(w, np.float32).reshape(16) for w in warps]
+        self.Wq = [tuple(int(v) for v in q) for q in wq]
+        self.Planes = np.ascontiguousarray(planes, np.float32)
+        self.Caps = tuple(caps)
+        self._views = (View * len(self.Others))()
+        for k, (img, w, q) in enumerate(zip(self.Others, self.Warps, self.Wq)):
+            v = self._views[k]
+            v.rgba = img.ctypes.data
+            v.height, v.width = img.shape[:2]
+            v.warp = (ctypes.c_float * 16)(*w)
+            v.wq = (ctypes.c_uint16 * 4)(*q)
+        self.sweep = Sweep(self.Ref.ctypes.data, self.Ref.shape[1], self.Ref.shape[0], len(self.Others),
+                           ctypes.addressof(self._views), self.Planes.ctypes.data, len(self.Planes),
+                           (ctypes.c_uint8 * 4)(*self.Caps))
+
+
+def sweep_cost(ctx, cc, W, H, scale, align=8, start=None, count=None, skip=None):
+    """The raw volume of a layer (MatchingCost + Normalise per computed pixel), packed: roundup(count, align) bytes
+    per pixel in raster order."""
+    n = len(cc.Planes)
+    if start is not None:
+        start = np.ascontiguousarray(start, np.uint16).reshape(-1)
+        count = np.ascontiguousarray(count, np.uint16).reshape(-1)
+        total = int((((count.astype(np.int64) + align - 1) // align) * align).sum())
+    else:
+        total = W * H * (-(-n // align) * align)
+    skip = None if skip is None else np.ascontiguousarray(skip, np.uint8).reshape(-1)
+    raw = np.empty(max(total, 1), np.uint8)
+    _check(lib().lux_sgm_sweep_cost(ctx.handle, ctypes.byref(cc.sweep), W, H, scale, align, _p(start), _p(count), _p(skip),
+                                    _p(raw)), ctx.handle)
+    return raw[:total]
+
+
+class DenseParams(ctypes.Structure):
+    """lux_sgm_dense_params: StereoParams of one layer plus its size."""
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("scale", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("align", ctypes.c_int32), ("p1", ctypes.c_int32), ("p2_scale", ctypes.c_float),
+                ("guidance", ctypes.c_float * 4)]
+
+
+# StereoParams.Guidance (DenseStereo.cs:21): c/6, c/16, c/16, 0 with c = 0x3ef6384f, each one float division
+_C_G = np.array([0x3ef6384f], np.uint32).view(np.float32)[0]
+L16_GUIDANCE = (float(_C_G / np.float32(6)), float(_C_G / np.float32(16)), float(_C_G / np.float32(16)), 0.0)
+# StereoParams.L16Pyramid (DenseStereo.cs:24-31): scales and tiles of the six layers, skip mode 2 on the last two
+L16_SCALES = (32, 16, 8, 4, 2, 1)
+L16_TILES = (4, 8, 16, 32, 64, 64)
+
+
+def dense_params(ref_w, ref_h, scale, tile, align=8, p1=1, p2_scale=500.0, guidance=L16_GUIDANCE):
+    return DenseParams(-(-ref_w // scale), -(-ref_h // scale), scale, tile, align, p1, p2_scale,
+                       (ctypes.c_float * 4)(*guidance))
+
+
+def range_from_previous(ctx, planes, prev_depth, prev_skip, w, h, margin=1):
+    """RangeFromPrevious on the device: (start, count, MinLo, MaxHi) like oracle.range_from_previous."""
+    planes = np.ascontiguousarray(planes, np.float32)
+    prev_depth = np.ascontiguousarray(prev_depth, np.float32)
+    ph, pw = prev_depth.shape
+    prev_skip = None if prev_skip is None else np.ascontiguousarray(prev_skip, np.uint8)
+    start = np.empty((h, w), np.uint16)
+    count = np.empty((h, w), np.uint16)
+    lo, hi = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().lux_sgm_range_from_previous(ctx.handle, _p(planes), len(planes), _p(prev_depth), _p(prev_skip), pw, ph, w,
+                                             h, margin, _p(start), _p(count), ctypes.byref(lo), ctypes.byref(hi)),
+           ctx.handle)
+    return start, count, lo.value, hi.value
+
+
+def dense_layer(ctx, cc, params, start=None, count=None, skip=None, want_agg=False, keep_pass1=False):
+    """DenseLayer.Compute on the device from the images: returns dict(depth, wta, agg, agg_pass1) (agg packed)."""
+    W, H, align = params.width, params.height, params.align
+    n = len(cc.Planes)
+    if start is not None:
+        start = np.ascontiguousarray(start, np.uint16).reshape(-1)
+        count = np.ascontiguousarray(count, np.uint16).reshape(-1)
+        total = int((((count.astype(np.int64) + align - 1) // align) * align).sum())
+    else:
+        total = W * H * (-(-n // align) * align)
+    skip = None if skip is None else np.ascontiguousarray(skip, np.uint8).reshape(-1)
+    depth = np.empty((H, W), np.float32)
+    wta = np.empty((H, W), np.uint16)
+    agg = np.empty(max(total, 1), np.uint16) if want_agg else None
+    a1 = np.empty(max(total, 1), np.uint16) if keep_pass1 else None
+    _check(lib().lux_sgm_dense_layer(ctx.handle, ctypes.byref(cc.sweep), ctypes.byref(params), _p(start), _p(count), _p(skip),
+                                     _p(depth), _p(agg), _p(a1), _p(wta)), ctx.handle)
+    return dict(depth=depth, wta=wta, agg=None if agg is None else agg[:total],
+                agg_pass1=None if a1 is None else a1[:total])
+
+
+class DensePyramidParams(ctypes.Structure):
+    """lux_sgm_dense_pyramid_params: StereoParams.L16Pyramid() and the budget of DenseStereoPyramid."""
+    _fields_ = [("levels", ctypes.c_int32), ("scale", ctypes.c_int32 * 8), ("tile", ctypes.c_int32 * 8),
+                ("skip_mode", ctypes.c_int32 * 8), ("p1", ctypes.c_int32), ("p2_scale", ctypes.c_float),
+                ("guidance", ctypes.c_float * 4), ("margin", ctypes.c_int32), ("caps", ctypes.c_uint8 * 4),
+                ("budget", ctypes.c_uint64)]
+
+
+def dense_pyramid_defaults():
+    p = DensePyramidParams()
+    lib().lux_sgm_dense_pyramid_defaults(ctypes.byref(p))
+    return p
+
+
+def dense_pyramid(ctx, images, gray, warps, planes8, planes2=None, params=None):
+    """DenseStereoPyramid.Run on the device: unfiltered RGBA8 images (reference first), per-camera gray flags, the
+    other cameras' warps, the plane lists of align 8 and 2. Returns (list of per-level depth arrays, aligns)."""
+    p = params or dense_pyramid_defaults()
+    imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+    n = len(imgs)
+    ptrs = (ctypes.c_void_p * n)(*[im.ctypes.data for im in imgs])
+    ws = (ctypes.c_int32 * n)(*[im.shape[1] for im in imgs])
+    hs = (ctypes.c_int32 * n)(*[im.shape[0] for im in imgs])
+    g = None if gray is None else np.ascontiguousarray(np.asarray(gray, np.uint8))
+    wr = np.ascontiguousarray(np.asarray(warps, np.float32).reshape(n - 1, 16))
+    p8 = np.ascontiguousarray(planes8, np.float32)
+    p2 = None if planes2 is None else np.ascontiguousarray(planes2, np.float32)
+    H0, W0 = imgs[0].shape[:2]
+    depths = [np.empty((-(-H0 // p.scale[i]), -(-W0 // p.scale[i])), np.float32) for i in range(p.levels)]
+    dptr = (ctypes.c_void_p * p.levels)(*[d.ctypes.data for d in depths])
+    aligns = (ctypes.c_int32 * p.levels)()
+    _check(lib().lux_sgm_dense_pyramid(ctx.handle, ctypes.byref(p), n, ptrs, ws, hs, _p(g), _p(wr), _p(p8), len(p8), _p(p2),
+                                       0 if p2 is None else len(p2), dptr, aligns), ctx.handle)
+    return depths, list(aligns)

@@ -9,10 +9,13 @@
  */
 #include "lux_oracle.h"
 
+#include <immintrin.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+
+#include "rcp_table.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -437,4 +440,471 @@ int lo_rectified(const void* left, const void* right, int img16, int bits, int W
     free(start);
     free(count);
     return rc;
+}
+
+
+/* ---------------------------------------------------------------- DenseUpsampleLayer */
+uint32_t lo_rcp_bits(uint32_t b) /* IntelApprox.cs:20-28 */
+{
+    const uint32_t sign = b & 0x80000000u, e = (b >> 23) & 0xffu, m = b & 0x7fffffu;
+    if (e == 0xffu) return m ? (b | 0x400000u) : sign;
+    if (e == 0) return sign | 0x7f800000u;
+    const uint32_t r = kRcpTable[m >> 11];
+    const int re = (int)((r >> 23) & 0xffu) - ((int)e - 127);
+    if (re <= 0) return sign;
+    return sign | ((uint32_t)re << 23) | (r & 0x7fffffu);
+}
+
+uint32_t lo_rcpss_hw(uint32_t b)
+{
+    float x = f_of_bits(b);
+    return bits_of_f(_mm_cvtss_f32(_mm_rcp_ss(_mm_set_ss(x))));
+}
+
+static float lo_rcp(float x) { return f_of_bits(lo_rcp_bits(bits_of_f(x))); }
+
+/* The cubic tap (:58-63): Catmull-Rom written with the constants 9, -15, 6 and -3, 15, -24, 12, times 1/6. */
+static float cubic_tap(float x)
+{
+    const float sixth = f_of_bits(0x3e2aaaab);
+    float x2 = x * x, x3 = x2 * x;
+    if (x < 1.0f) { float a = x3 * 9.0f; float b = x2 * -15.0f; b = b + 6.0f; b = b + a; return b * sixth; }
+    if (x < 2.0f) { float a = x3 * -3.0f; float b = x2 * 15.0f; float c = x * -24.0f; c = c + 12.0f; c = c + b; c = c + a; return c * sixth; }
+    return 0.0f;
+}
+
+int lo_image_shift(const uint8_t* src, int W, int H, float sx, float sy, uint8_t* dst) /* :70-104 */
+{
+    if (!src || !dst || W <= 0 || H <= 0) return -1;
+    const int ix = (int)floorf(sx), iy = (int)floorf(sy);
+    const float tx = sx - (float)ix, ty = sy - (float)iy;
+    const float kx[4] = {cubic_tap(tx + 1.0f), cubic_tap(tx), cubic_tap(1.0f - tx), cubic_tap(2.0f - tx)};
+    const float ky[4] = {cubic_tap(ty + 1.0f), cubic_tap(ty), cubic_tap(1.0f - ty), cubic_tap(2.0f - ty)};
+    float* row = (float*)malloc(sizeof(float) * (size_t)(W + 4) * 4);
+    for (int y = 0; y < H; y++) {
+        int rows[4];
+        for (int i = 0; i < 4; i++) rows[i] = clampi(y + iy - 1 + i, 0, H - 1);
+        for (int c = -2; c < W + 2; c++) { /* columns over all four source rows, clamp to edge */
+            const int cx = clampi(c, 0, W - 1);
+            for (int ch = 0; ch < 4; ch++) {
+                float acc = (float)src[((size_t)rows[0] * W + cx) * 4 + ch] * ky[0];
+                for (int i = 1; i < 4; i++) acc = (float)src[((size_t)rows[i] * W + cx) * 4 + ch] * ky[i] + acc;
+                row[(c + 2) * 4 + ch] = acc;
+            }
+        }
+        for (int x = 0; x < W; x++) {
+            const int b = (x + ix - 1 + 2) * 4;
+            for (int ch = 0; ch < 4; ch++) {
+                const float v0 = row[b + ch], v1 = row[b + 4 + ch], v2 = row[b + 8 + ch], v3 = row[b + 12 + ch];
+                const float s = (v3 * kx[3] + v2 * kx[2]) + (v1 * kx[1] + v0 * kx[0]);
+                const int q = (int)rintf(s); /* round to nearest even */
+                dst[((size_t)y * W + x) * 4 + ch] = (uint8_t)clampi(q, 0, 255);
+            }
+        }
+    }
+    free(row);
+    return 0;
+}
+
+int lo_nearest_resize(const uint8_t* src, int sw, int sh, int W, int H, uint8_t* dst) /* :108-128 */
+{
+    if (!src || !dst || W <= 0 || H <= 0 || sw <= 0 || sh <= 0) return -1;
+    const int stepY = (int)((double)sh * 65536.0 / (double)H), stepX = (int)((double)sw * 65536.0 / (double)W);
+    int yacc = 0;
+    for (int y = 0; y < H; y++) {
+        const int syy = yacc >> 16;
+        int xacc = 0;
+        for (int x = 0; x < W; x++) {
+            const int sxx = xacc >> 16;
+            memcpy(dst + ((size_t)y * W + x) * 4, src + ((size_t)syy * sw + sxx) * 4, 4);
+            xacc += stepX;
+        }
+        yacc += stepY;
+    }
+    return 0;
+}
+
+/* The bilateral weight's fast 2^t (:144-152): n = (int)t + (bits(t) >> 31), so -0 takes n = -1 and f = 1. */
+static float exp2_fast_signbit(float t)
+{
+    const float c1 = f_of_bits(0x3d9fcb52), c2 = f_of_bits(0x3e677e26), c3 = f_of_bits(0x3f322226), c4 = f_of_bits(0x3f7ffb19);
+    t = t < 128.0f ? t : 128.0f;
+    t = t > -126.0f ? t : -126.0f;
+    const int n = (int)t + ((int32_t)bits_of_f(t) >> 31);
+    const float f = t - (float)n;
+    const float p = ((f * c1 + c2) * f + c3) * f + c4;
+    return f_of_bits(bits_of_f(p) + ((uint32_t)n << 23));
+}
+
+static float spatial_w(float f, int i) /* :213-218 */
+{
+    float d = fabsf(f - (float)(i * 2));
+    d = d < 1.0f ? d : 1.0f;
+    return (int)d == 0 ? 1.0f : f_of_bits(0x3eaaaaab);
+}
+
+int lo_bilateral_upsample(const float* lo, int lw, int lh, const uint8_t* guide, int W, int H, const uint8_t* g2, float sigma,
+                          float* out) /* :156-210 */
+{
+    if (!lo || !guide || !g2 || !out || lw <= 0 || lh <= 0 || W <= 0 || H <= 0) return -1;
+    const float k = 0.5f / (sigma * sigma), eps = f_of_bits(0x322bcc77);
+    const int wm1 = lw - 1, hm1 = lh - 1;
+    for (int y = 0; y < H; y++) {
+        const int iy0 = (y - 1) / 2; /* C division, toward zero */
+        const float fy = (float)y + -0.5f;
+        for (int x = 0; x < W; x++) {
+            const uint8_t* gp = guide + ((size_t)y * W + x) * 4;
+            const float hr = gp[0], hg = gp[1], hb = gp[2];
+            const int ix0 = (x - 1) / 2;
+            const float fx = (float)x + -0.5f;
+            const int ibx = clampi(ix0, 0, wm1), ix1 = clampi(ix0 + 1, 0, wm1);
+            const float wx0 = spatial_w(fx, ibx), wx1 = spatial_w(fx, ix1);
+            float accV = 0.0f, accS = 0.0f;
+            for (int j = 0; j < 2; j++) {
+                const int iy = clampi(iy0 + j, 0, hm1);
+                const float wy = spatial_w(fy, iy);
+                const uint8_t* q0 = g2 + ((size_t)iy * lw + ibx) * 4;
+                float d0 = hr - q0[0], d1 = hg - q0[1], d2 = hb - q0[2];
+                float sq = d1 * d1 + (d2 * d2 + d0 * d0);
+                const float w0 = (exp2_fast_signbit(-(sq * k)) + eps) * (wy * wx0);
+                const uint8_t* q1 = g2 + ((size_t)iy * lw + ix1) * 4;
+                d0 = hr - q1[0]; d1 = hg - q1[1]; d2 = hb - q1[2];
+                sq = d1 * d1 + (d2 * d2 + d0 * d0);
+                const float w1 = (exp2_fast_signbit(-(sq * k)) + eps) * (wy * wx1);
+                const float v0 = lo[(size_t)iy * lw + ibx] * w0, v1 = lo[(size_t)iy * lw + ix1] * w1;
+                accV = v1 + (v0 + accV);
+                accS = w1 + (w0 + accS);
+            }
+            out[(size_t)y * W + x] = accV / accS;
+        }
+    }
+    return 0;
+}
+
+int lo_upsample_layer(const float* prev_depth, int pw, int ph, const uint8_t* guide, int W, int H, int setW, int setH,
+                      float sigma, float* depth) /* :38-52 */
+{
+    if (!prev_depth || !guide || !depth || pw <= 0 || ph <= 0 || W <= 0 || H <= 0) return -1;
+    if (W != setW * 2 || H != setH * 2) return -3;
+    uint8_t* shifted = (uint8_t*)malloc((size_t)W * H * 4);
+    uint8_t* g2 = (uint8_t*)malloc((size_t)pw * ph * 4);
+    float* inv = (float*)malloc(sizeof(float) * (size_t)pw * ph);
+    lo_image_shift(guide, W, H, 0.5f, 0.5f, shifted);
+    lo_nearest_resize(shifted, W, H, pw, ph, g2);
+    for (size_t i = 0; i < (size_t)pw * ph; i++) inv[i] = lo_rcp(prev_depth[i]);
+    lo_bilateral_upsample(inv, pw, ph, guide, W, H, g2, sigma, depth);
+    for (size_t i = 0; i < (size_t)W * H; i++) depth[i] = lo_rcp(depth[i]);
+    free(shifted);
+    free(g2);
+    free(inv);
+    return 0;
+}
+
+
+/* ======================================================================================================
+ * Plane-sweep matching cost, DenseStereo.cs:217-381, and the layer set-up around it (DenseStereoSgm.cs).
+ * ====================================================================================================== */
+
+/* float -> int as the reference's runtime converts (saturating; NaN -> 0). */
+static int f2i_sat(float f)
+{
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f < -2147483648.0f) return INT32_MIN;
+    return (int)f;
+}
+
+static inline int avg_u8(int a, int b) { return (a + b + 1) >> 1; } /* pavgb */
+
+/* One camera's cost of one plane with the 4x4 target block at (ix-1, iy-1) (:276-291, :332-367): rows averaged with
+ * the next row for a half-pixel y, then pixels with the next pixel for a half-pixel x; capped differences against the
+ * reference block summed per channel over the 3x3 pixels; weighted and clamped to 65535. */
+static int block_cost(const lo_view* v, const uint8_t refb[9][4], const uint8_t caps[4], int ix, int iy, int hx, int hy)
+{
+    int S[4] = {0, 0, 0, 0};
+    for (int q = 0; q < 3; q++) {
+        const uint8_t* r0 = v->rgba + ((size_t)(iy - 1 + q) * v->w + (ix - 1)) * 4;
+        const uint8_t* r1 = r0 + (size_t)v->w * 4;
+        for (int p = 0; p < 3; p++)
+            for (int ch = 0; ch < 4; ch++) {
+                int a = r0[p * 4 + ch], b = r0[(p + 1) * 4 + ch];
+                if (hy) {
+                    a = avg_u8(a, r1[p * 4 + ch]);
+                    b = avg_u8(b, r1[(p + 1) * 4 + ch]);
+                }
+                const int t = hx ? avg_u8(a, b) : a;
+                const int rb = refb[q * 3 + p][ch];
+                int d = t > rb ? t - rb : rb - t;
+                if (d > caps[ch]) d = caps[ch];
+                S[ch] += d;
+            }
+    }
+    const int cY = (S[0] * v->wq[0] + 16) >> 5, cU = (S[1] * v->wq[1] + 16) >> 5, cV = (S[2] * v->wq[2] + 16) >> 5,
+              cA = (S[3] * v->wq[3] + 16) >> 5;
+    const int ck = (cY + cU) + (cV + cA);
+    return ck < 65535 ? ck : 65535;
+}
+
+int lo_matching_cost(const lo_sweep* s, int x, int y, int start, int count, uint16_t* cost)
+{
+    if (!s || !cost || start < 0 || count < 0 || start + count > s->n) return -1;
+    for (int k = 0; k < s->nv; k++)
+        if (s->views[k].w < 4 || s->views[k].h < 4) return -1;
+    uint8_t refb[9][4];
+    for (int r = -1; r <= 1; r++)
+        for (int c = -1; c <= 1; c++) {
+            const int yy = clampi(y + r, 0, s->rh - 1), xx = clampi(x + c, 0, s->rw - 1);
+            memcpy(refb[(r + 1) * 3 + (c + 1)], s->ref + ((size_t)yy * s->rw + xx) * 4, 4);
+        }
+    memset(cost, 0, sizeof(uint16_t) * (size_t)count);
+    const float xf = (float)x, yf = (float)y;
+    for (int k = 0; k < s->nv; k++) {
+        const lo_view* v = &s->views[k];
+        const float* m = v->warp;
+        const int x0 = 1, y0 = 1, x1 = v->w - 2, y1 = v->h - 2; /* rect + {1, 1, -2, -2} */
+        int lix = 0, liy = 0, lhx = 0, lhy = 0, last = 0, have = 0;
+        for (int d = 0; d < count; d++) {
+            const float z = s->planes[start + d];
+            const float xz = xf * z, yz = yf * z;
+            const float px = ((z * m[8] + m[12]) + xz * m[0]) + yz * m[4];
+            const float py = ((z * m[9] + m[13]) + xz * m[1]) + yz * m[5];
+            const float pz = ((z * m[10] + m[14]) + xz * m[2]) + yz * m[6];
+            const float inv = 1.0f / pz;
+            const float fx = px * inv + 0.25f, fy = py * inv + 0.25f;
+            int ix, iy, hx, hy;
+            if (d == 0) { /* the first plane is clamped into the rect */
+                const float cx = fminf(fmaxf(fx, (float)x0), (float)(x1 - 1)), cy = fminf(fmaxf(fy, (float)y0), (float)(y1 - 1));
+                ix = f2i_sat(cx);
+                iy = f2i_sat(cy);
+                hx = f2i_sat(cx + cx) & 1;
+                hy = f2i_sat(cy + cy) & 1;
+            } else {
+                ix = f2i_sat(fx);
+                iy = f2i_sat(fy);
+                hx = f2i_sat(fx + fx) & 1;
+                hy = f2i_sat(fy + fy) & 1;
+                /* the same position as the last costed plane, or outside the rect: the last cost again */
+                if ((have && ix == lix && iy == liy && hx == lhx && hy == lhy) || ix < x0 || ix >= x1 || iy < y0 || iy >= y1) {
+                    cost[d] = (uint16_t)(cost[d] + last);
+                    continue;
+                }
+            }
+            const int c = block_cost(v, (const uint8_t(*)[4])refb, s->caps, ix, iy, hx, hy);
+            cost[d] = (uint16_t)(cost[d] + c);
+            lix = ix; liy = iy; lhx = hx; lhy = hy; last = c; have = 1;
+        }
+    }
+    return 0;
+}
+
+void lo_normalise(const uint16_t* cost, int count, int cap, int nv, uint8_t* raw)
+{
+    const float f = f_of_bits(0x3d17b426) / (float)nv;
+    for (int d = 0; d < count; d++) raw[d] = (uint8_t)f2i_sat((float)cost[d] * f);
+    for (int d = count; d < cap; d++) raw[d] = 255;
+}
+
+int lo_layer_sweep_cost(const lo_sweep* s, int W, int H, int scale, int align, const uint16_t* start, const uint16_t* count,
+                        const uint8_t* skip, uint8_t* raw, int threads)
+{
+    if (!s || !start || !count || !raw || W <= 0 || H <= 0 || scale <= 0 || align <= 0 || s->nv <= 0) return -1;
+    const size_t npx = (size_t)W * H;
+    size_t* off = (size_t*)malloc(sizeof(size_t) * (npx + 1));
+    off[0] = 0;
+    for (size_t i = 0; i < npx; i++) off[i + 1] = off[i] + (size_t)((count[i] + align - 1) / align) * align;
+    const int half = scale / 2;
+    int bad = 0;
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1) reduction(| : bad)
+    for (int y = 0; y < H; y++) {
+        uint16_t* cost = (uint16_t*)malloc(sizeof(uint16_t) * (size_t)(s->n + 1));
+        for (int x = 0; x < W; x++) {
+            const size_t i = (size_t)y * W + x;
+            const int cap = (int)(off[i + 1] - off[i]), cnt = count[i];
+            uint8_t* r = raw + off[i];
+            if (skip && skip[i]) {
+                memset(r, 0, (size_t)cnt);
+                memset(r + cnt, 255, (size_t)(cap - cnt));
+                continue;
+            }
+            const int xi = scale * x + half < s->rw - 1 ? scale * x + half : s->rw - 1;
+            const int yi = scale * y + half < s->rh - 1 ? scale * y + half : s->rh - 1;
+            if (lo_matching_cost(s, xi, yi, start[i], cnt, cost)) { bad = 1; continue; }
+            lo_normalise(cost, cnt, cap, s->nv, r);
+        }
+        free(cost);
+    }
+    free(off);
+    return bad ? -1 : 0;
+}
+
+int lo_guidance(const uint8_t* ref, int rw, int rh, int w, int h, int scale, uint8_t* out)
+{
+    if (!ref || !out || w <= 0 || h <= 0 || scale <= 0) return -1;
+    if (rw == w && rh == h) {
+        memcpy(out, ref, (size_t)w * h * 4);
+        return 0;
+    }
+    const int half = scale / 2;
+    for (int y = 0; y < h; y++) {
+        const int sy = scale * y + half < rh - 1 ? scale * y + half : rh - 1;
+        for (int x = 0; x < w; x++) {
+            const int sx = scale * x + half < rw - 1 ? scale * x + half : rw - 1;
+            memcpy(out + ((size_t)y * w + x) * 4, ref + ((size_t)sy * rw + sx) * 4, 4);
+        }
+    }
+    return 0;
+}
+
+int lo_range_from_previous(const float* planes, int n, const float* prev_depth, const uint8_t* prev_skip, int pw, int ph,
+                           int w, int h, int margin, uint16_t* start, uint16_t* count, int* min_lo, int* max_hi)
+{
+    if (!planes || !prev_depth || !start || !count || n <= 0 || pw <= 0 || ph <= 0 || w <= 0 || h <= 0) return -1;
+    /* plane boundaries in inverse depth: midpoints of neighbouring planes' rcpss, the last one FLT_MAX */
+    float* b = (float*)malloc(sizeof(float) * (size_t)n);
+    for (int i = 0; i < n - 1; i++) b[i] = (lo_rcp(planes[i + 1]) + lo_rcp(planes[i])) * 0.5f;
+    b[n - 1] = f_of_bits(0x7f7fffff);
+    const size_t np = (size_t)pw * ph;
+    uint16_t* idx = (uint16_t*)malloc(2 * np);
+    for (size_t i = 0; i < np; i++) {
+        const float v = lo_rcp(prev_depth[i]);
+        int c = 0;
+        while (c < n && b[c] <= v) c++; /* the leading run of boundaries at or below v */
+        idx[i] = (uint16_t)c;
+    }
+    /* 4-wide min/max over x (-1..+2, clamped) of computed pixels, then over y (-1..+2, clamped) */
+    uint16_t *hmin = (uint16_t*)malloc(2 * np), *hmax = (uint16_t*)malloc(2 * np), *vmin = (uint16_t*)malloc(2 * np),
+             *vmax = (uint16_t*)malloc(2 * np);
+    for (int y = 0; y < ph; y++)
+        for (int x = 0; x < pw; x++) {
+            int mn = 0xffff, mx = 0;
+            for (int k = -1; k <= 2; k++) {
+                const int xx = clampi(x + k, 0, pw - 1);
+                if (prev_skip && prev_skip[(size_t)y * pw + xx]) continue;
+                const int v = idx[(size_t)y * pw + xx];
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+            hmin[(size_t)y * pw + x] = (uint16_t)mn;
+            hmax[(size_t)y * pw + x] = (uint16_t)mx;
+        }
+    for (int y = 0; y < ph; y++)
+        for (int x = 0; x < pw; x++) {
+            int mn = 0xffff, mx = 0;
+            for (int k = -1; k <= 2; k++) {
+                const int yy = clampi(y + k, 0, ph - 1);
+                const int a = hmin[(size_t)yy * pw + x], c = hmax[(size_t)yy * pw + x];
+                if (a < mn) mn = a;
+                if (c > mx) mx = c;
+            }
+            vmin[(size_t)y * pw + x] = (uint16_t)mn;
+            vmax[(size_t)y * pw + x] = (uint16_t)mx;
+        }
+    const float fy = (float)(ph - 1) / (float)(h - 1), fx = (float)(pw - 1) / (float)(w - 1);
+    int mhi = 0, mlo = INT32_MAX;
+    for (int y = 0; y < h; y++) {
+        const int py = f2i_sat((float)y * fy);
+        for (int x = 0; x < w; x++) {
+            const int px = f2i_sat((float)x * fx);
+            int lo = vmin[(size_t)py * pw + px] - margin, hi = vmax[(size_t)py * pw + px] + 1;
+            if (lo < 0) lo = 0;
+            if (hi > n - 1) hi = n - 1;
+            const size_t i = (size_t)y * w + x;
+            start[i] = (uint16_t)lo;
+            count[i] = (uint16_t)(hi - lo);
+            if ((hi & 0xffff) > mhi) mhi = hi & 0xffff;
+            if ((lo & 0xffff) < mlo) mlo = lo & 0xffff;
+        }
+    }
+    if (min_lo) *min_lo = mlo;
+    if (max_hi) *max_hi = mhi;
+    free(b); free(idx); free(hmin); free(hmax); free(vmin); free(vmax);
+    return 0;
+}
+
+int lo_layer_depth(const float* planes, int n, int W, int H, int align, const uint16_t* start, const uint16_t* count,
+                   const uint16_t* agg, float* depth)
+{
+    if (!planes || !start || !count || !agg || !depth || align <= 0) return -1;
+    size_t off = 0;
+    for (size_t i = 0; i < (size_t)W * H; i++) {
+        const int cnt = count[i], cap = ((cnt + align - 1) / align) * align;
+        int best = 0xffff, arg = 0;
+        for (int k = 0; k < cnt; k++)
+            if (agg[off + k] < best) { best = agg[off + k]; arg = k; }
+        const int pi = (uint16_t)(start[i] + arg);
+        depth[i] = pi < n ? planes[pi] : 0.0f;
+        off += (size_t)cap;
+    }
+    return 0;
+}
+
+
+/* ---- SkipMask and its generator ---- */
+/* MT19937 (Matsumoto & Nishimura 1998): 624 words of state, the twist and the tempering of the published algorithm. */
+typedef struct {
+    uint32_t mt[624];
+    int i;
+} lo_mt;
+
+static void mt_seed(lo_mt* g, uint32_t seed)
+{
+    g->mt[0] = seed;
+    for (int i = 1; i < 624; i++) g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->i = 624;
+}
+
+static uint32_t mt_next(lo_mt* g)
+{
+    if (g->i >= 624) {
+        for (int k = 0; k < 624; k++) {
+            const uint32_t y = (g->mt[k] & 0x80000000u) | (g->mt[(k + 1) % 624] & 0x7fffffffu);
+            g->mt[k] = g->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->i = 0;
+    }
+    uint32_t y = g->mt[g->i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+uint32_t lo_mt19937(uint32_t seed, int n)
+{
+    lo_mt g;
+    mt_seed(&g, seed);
+    uint32_t v = 0;
+    for (int k = 0; k < n; k++) v = mt_next(&g);
+    return v;
+}
+
+int lo_skip_mask(int mode, int w, int h, uint8_t* out)
+{
+    if (w <= 0 || h <= 0 || !out) return -1;
+    if (mode == 0) {
+        memset(out, 0, (size_t)w * h);
+        return 0;
+    }
+    if (mode != 2) return -2;
+    memset(out, 0xff, (size_t)w * h);
+    int nx = w / 64 + (64 < 2 * (w % 64) ? 1 : 0), ny = h / 64 + (64 < 2 * (h % 64) ? 1 : 0);
+    if (nx < 1) nx = 1;
+    if (ny < 1) ny = 1;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int x0 = 64 * i, y0 = 64 * j;
+            const int x1 = w < x0 + 64 * (i == nx - 1 ? 2 : 1) ? w : x0 + 64 * (i == nx - 1 ? 2 : 1);
+            const int y1 = h < y0 + 64 * (j == ny - 1 ? 2 : 1) ? h : y0 + 64 * (j == ny - 1 ? 2 : 1);
+            lo_mt g;
+            mt_seed(&g, 5489u);
+            for (int y = y0; y < y1; y += 2)
+                for (int x = x0; x < x1; x += 2) {
+                    const int dy = (int)(mt_next(&g) & 1u), dx = (int)(mt_next(&g) & 1u);
+                    const int yy = y + dy, xx = x + dx;
+                    if (xx < w && yy < h) out[(size_t)yy * w + xx] = 0;
+                }
+        }
+    return 0;
 }

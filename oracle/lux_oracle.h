@@ -89,6 +89,72 @@ int lo_rectified(const void* left, const void* right, int img16, int bits, int W
                  int cost, int radius, int adcap, int D, int align, int p1, int p2, int T, int xsw, int tol,
                  int threads, uint16_t* disp, float* sub, uint8_t* valid, uint16_t* agg, double* times_ms);
 
+/* ---- DenseUpsampleLayer (Lux.Engine/Pipeline/Registration/DenseUpsampleLayer.cs) ---- */
+/* rcpss / rcpps as an Intel CPU computes them (IntelApprox.cs:20-28): a 4096-entry table on the top 12 mantissa
+ * bits scaled by the input's power of two; +-0 and denormals -> +-inf, +-inf -> +-0, NaN -> quiet NaN, a denormal
+ * result -> +-0. */
+uint32_t lo_rcp_bits(uint32_t b);
+/* The local CPU's own rcpss (for pinning lo_rcp_bits on an Intel host). */
+uint32_t lo_rcpss_hw(uint32_t b);
+/* ImageShift<2, vec4x8ui>(src, {sx, sy}) (:64-104): RGBA8 W x H, separable 4-tap cubic, clamp-to-edge, RNE, saturate. */
+int lo_image_shift(const uint8_t* src, int W, int H, float sx, float sy, uint8_t* dst);
+/* NearestResize (:108-128): 16.16 steps, RGBA8 sw x sh -> W x H. */
+int lo_nearest_resize(const uint8_t* src, int sw, int sh, int W, int H, uint8_t* dst);
+/* BilateralUpsampleFromCollapse<2, float, vec4x8ui> (:156-210): lo lw x lh -> out (guide's W x H). */
+int lo_bilateral_upsample(const float* lo, int lw, int lh, const uint8_t* guide, int W, int H, const uint8_t* g2, float sigma,
+                          float* out);
+/* The whole step (:38-52): shift the guide by (0.5, 0.5), resize to lw x lh, invert the depth, upsample, invert.
+ * Returns -3 when the guide is not (setW*2) x (setH*2) ("Incorrect output depth size!"). */
+int lo_upsample_layer(const float* prev_depth, int pw, int ph, const uint8_t* guide, int W, int H, int setW, int setH,
+                      float sigma, float* depth);
+
+/* ---- Plane-sweep matching cost (Lux.Engine/Pipeline/Registration/DenseStereo.cs) ---- */
+/* One other camera of the cost context (BuildContext :190-200): its box-filtered RGBA8 image (dense rows), the
+ * reference->camera flow matrix (column-major floats, Warp :186) and the channel weights Wq. */
+typedef struct {
+    const uint8_t* rgba;
+    int w, h;
+    float warp[16];
+    uint16_t wq[4];
+} lo_view;
+typedef struct {
+    const uint8_t* ref;   /* filtered[0], dense RGBA8 rw x rh */
+    int rw, rh;
+    const lo_view* views;
+    int nv;
+    const float* planes;  /* metric plane depths, far .. near */
+    int n;
+    uint8_t caps[4];      /* per-channel caps of the absolute differences ({2, 6, 6, 0}) */
+} lo_sweep;
+/* MatchingCost (:217-292, its scalar form :295-373, equal to it) for reference image pixel (x, y), planes
+ * [start, start+count): u16 costs, wrapping sum over the cameras. A first-plane projection that is NaN is clamped
+ * like any other (maxNum/minNum); the reference raises there. Returns -1 on bad arguments. */
+int lo_matching_cost(const lo_sweep* s, int x, int y, int start, int count, uint16_t* cost);
+/* Normalise (:376-381): raw[d] = (byte)(int)(cost[d] * (0x3d17b426 / nv)), planes [count, cap) = 255. */
+void lo_normalise(const uint16_t* cost, int count, int cap, int nv, uint8_t* raw);
+/* The raw volume of a layer (DenseLayer.Compute :186-205): layer pixel (x, y) reads image pixel
+ * (min(scale*x + scale/2, rw-1), min(scale*y + scale/2, rh-1)); raw is packed pixel by pixel, roundup(count, align)
+ * bytes each; skipped pixels (skip != 0) get 0 over their planes and 255 over the padding, as Pixel uses them (:242). */
+int lo_layer_sweep_cost(const lo_sweep* s, int W, int H, int scale, int align, const uint16_t* start, const uint16_t* count,
+                        const uint8_t* skip, uint8_t* raw, int threads);
+/* Guidance (:162-175): the reference image itself when it is w x h, else the pixel at
+ * (min(scale*x + scale/2, rw-1), min(scale*y + scale/2, rh-1)). */
+int lo_guidance(const uint8_t* ref, int rw, int rh, int w, int h, int scale, uint8_t* out);
+/* RangeFromPrevious (DenseStereoSgm.cs:91-140): the plane range of every pixel of a w x h layer from the previous
+ * layer's depth (pw x ph) and skip mask. Writes start/count and returns MinLo / MaxHi (MaxHi not rounded). */
+int lo_range_from_previous(const float* planes, int n, const float* prev_depth, const uint8_t* prev_skip, int pw, int ph,
+                           int w, int h, int margin, uint16_t* start, uint16_t* count, int* min_lo, int* max_hi);
+/* The layer's depth (Compute :225-231): Planes[start + first minimum of agg over count]. agg packed as raw. */
+int lo_layer_depth(const float* planes, int n, int W, int H, int align, const uint16_t* start, const uint16_t* count,
+                   const uint16_t* agg, float* depth);
+
+/* DenseLayer.SkipMask (DenseStereoSgm.cs:64-87): mode 0 all computed; mode 2 one computed pixel per 2x2 cell of
+ * each 64 x 64 tile, drawn from std::mt19937 seeded 5489 per tile (the last tile of a row / column takes the
+ * remainder). 0 computed, 0xff skipped. Returns -2 for another mode. */
+int lo_skip_mask(int mode, int w, int h, uint8_t* out);
+/* The n-th output of std::mt19937 seeded with `seed` (the textbook generator). */
+uint32_t lo_mt19937(uint32_t seed, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -200,3 +200,245 @@ def pyramid(left, right, D, radii, *, radius=3, bits=10, align=8, p1=8, p2=96, T
         off = (np.float32(0.5) * (a - b).astype(np.float32)) / np.where(ok, den, 1).astype(np.float32)
         sub = np.where(ok, sub + off.astype(np.float32), sub).astype(np.float32)
     return disp, sub
+
+
+def _upsample_lib():
+    L = lib()
+    if not getattr(L, "_ups", False):
+        vp, i32, u32, f32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float
+        L.lo_rcp_bits.argtypes, L.lo_rcp_bits.restype = [u32], u32
+        L.lo_rcpss_hw.argtypes, L.lo_rcpss_hw.restype = [u32], u32
+        L.lo_image_shift.argtypes = [vp, i32, i32, f32, f32, vp]
+        L.lo_nearest_resize.argtypes = [vp, i32, i32, i32, i32, vp]
+        L.lo_bilateral_upsample.argtypes = [vp, i32, i32, vp, i32, i32, vp, f32, vp]
+        L.lo_upsample_layer.argtypes = [vp, i32, i32, vp, i32, i32, i32, i32, f32, vp]
+        L._ups = True
+    return L
+
+
+def rcp_bits(b):
+    return int(_upsample_lib().lo_rcp_bits(int(b) & 0xFFFFFFFF))
+
+
+def rcpss_hw(b):
+    return int(_upsample_lib().lo_rcpss_hw(int(b) & 0xFFFFFFFF))
+
+
+def image_shift(img, sx, sy):
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape[:2]
+    out = np.empty_like(img)
+    assert _upsample_lib().lo_image_shift(_p(img), W, H, sx, sy, _p(out)) == 0
+    return out
+
+
+def nearest_resize(img, W, H):
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.empty((H, W, 4), np.uint8)
+    assert _upsample_lib().lo_nearest_resize(_p(img), img.shape[1], img.shape[0], W, H, _p(out)) == 0
+    return out
+
+
+def bilateral_upsample(lo_img, guide, g2, sigma):
+    lo_img = np.ascontiguousarray(lo_img, np.float32)
+    guide = np.ascontiguousarray(guide, np.uint8)
+    g2 = np.ascontiguousarray(g2, np.uint8)
+    H, W = guide.shape[:2]
+    out = np.empty((H, W), np.float32)
+    assert _upsample_lib().lo_bilateral_upsample(_p(lo_img), lo_img.shape[1], lo_img.shape[0], _p(guide), W, H, _p(g2),
+                                                 sigma, _p(out)) == 0
+    return out
+
+
+def upsample_layer(prev_depth, guide, set_w, set_h, sigma=12.0):
+    """DenseUpsampleLayer.Run (DenseUpsampleLayer.cs:38-52): the finest layer's depth to the guide's resolution."""
+    prev_depth = np.ascontiguousarray(prev_depth, np.float32)
+    guide = np.ascontiguousarray(guide, np.uint8)
+    H, W = guide.shape[:2]
+    out = np.empty((H, W), np.float32)
+    rc = _upsample_lib().lo_upsample_layer(_p(prev_depth), prev_depth.shape[1], prev_depth.shape[0], _p(guide), W, H,
+                                           set_w, set_h, sigma, _p(out))
+    if rc != 0:
+        raise RuntimeError(f"lo_upsample_layer failed: {rc}")
+    return out
+
+
+# ---------------------------------------------------------------- plane-sweep matching cost (DenseStereo.cs)
+class View(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("w", ctypes.c_int), ("h", ctypes.c_int), ("warp", ctypes.c_float * 16),
+                ("wq", ctypes.c_uint16 * 4)]
+
+
+class Sweep(ctypes.Structure):
+    _fields_ = [("ref", ctypes.c_void_p), ("rw", ctypes.c_int), ("rh", ctypes.c_int), ("views", ctypes.c_void_p),
+                ("nv", ctypes.c_int), ("planes", ctypes.c_void_p), ("n", ctypes.c_int), ("caps", ctypes.c_uint8 * 4)]
+
+
+def _sweep_lib():
+    L = _upsample_lib()
+    if not getattr(L, "_sweep", False):
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        L.lo_matching_cost.argtypes = [vp, i32, i32, i32, i32, vp]
+        L.lo_normalise.argtypes = [vp, i32, i32, i32, vp]
+        L.lo_normalise.restype = None
+        L.lo_layer_sweep_cost.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp, i32]
+        L.lo_guidance.argtypes = [vp, i32, i32, i32, i32, i32, vp]
+        L.lo_range_from_previous.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp]
+        L.lo_layer_depth.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
+        L.lo_skip_mask.argtypes = [i32, i32, i32, vp]
+        L.lo_mt19937.argtypes, L.lo_mt19937.restype = [ctypes.c_uint32, i32], ctypes.c_uint32
+        L._sweep = True
+    return L
+
+
+class SweepContext:
+    """The cost context of a layer (DenseStereo.CostContext): the filtered reference, the other cameras' filtered
+    images, warps (16 floats, column-major) and weights, the planes and the caps. Keeps its arrays alive."""
+
+    def __init__(self, ref, others, warps, wqs, planes, caps=(2, 6, 6, 0)):
+        self.ref = np.ascontiguousarray(ref, np.uint8)
+        self.others = [np.ascontiguousarray(o, np.uint8) for o in others]
+        self.planes = np.ascontiguousarray(planes, np.float32)
+        self.views = (View * len(others))()
+        for k, (img, w, q) in enumerate(zip(self.others, warps, wqs)):
+            self.views[k].rgba = img.ctypes.data
+            self.views[k].h, self.views[k].w = img.shape[:2]
+            self.views[k].warp = (ctypes.c_float * 16)(*np.asarray(w, np.float32).reshape(16))
+            self.views[k].wq = (ctypes.c_uint16 * 4)(*[int(v) for v in q])
+        self.s = Sweep(self.ref.ctypes.data, self.ref.shape[1], self.ref.shape[0], ctypes.addressof(self.views),
+                       len(others), self.planes.ctypes.data, len(self.planes), (ctypes.c_uint8 * 4)(*caps))
+
+
+def matching_cost(sc, x, y, start, count):
+    out = np.zeros(max(count, 1), np.uint16)
+    rc = _sweep_lib().lo_matching_cost(ctypes.byref(sc.s), x, y, start, count, _p(out))
+    assert rc == 0, rc
+    return out[:count]
+
+
+def normalise(cost, count, cap, nv):
+    cost = np.ascontiguousarray(cost, np.uint16)
+    raw = np.zeros(max(cap, 1), np.uint8)
+    _sweep_lib().lo_normalise(_p(cost), count, cap, nv, _p(raw))
+    return raw[:cap]
+
+
+def layer_sweep_cost(sc, W, H, scale, align, start, count, skip=None, threads=8):
+    """The packed raw volume of a layer: roundup(count, align) bytes per pixel in raster order."""
+    start = np.ascontiguousarray(start, np.uint16).reshape(-1)
+    count = np.ascontiguousarray(count, np.uint16).reshape(-1)
+    skip = None if skip is None else np.ascontiguousarray(skip, np.uint8).reshape(-1)
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    raw = np.zeros(max(int(caps.sum()), 1), np.uint8)
+    rc = _sweep_lib().lo_layer_sweep_cost(ctypes.byref(sc.s), W, H, scale, align, _p(start), _p(count), _p(skip), _p(raw),
+                                          threads)
+    assert rc == 0, rc
+    return raw[:int(caps.sum())]
+
+
+def guidance(ref, w, h, scale):
+    ref = np.ascontiguousarray(ref, np.uint8)
+    out = np.empty((h, w, 4), np.uint8)
+    assert _sweep_lib().lo_guidance(_p(ref), ref.shape[1], ref.shape[0], w, h, scale, _p(out)) == 0
+    return out
+
+
+def range_from_previous(planes, prev_depth, prev_skip, w, h, margin=1):
+    planes = np.ascontiguousarray(planes, np.float32)
+    prev_depth = np.ascontiguousarray(prev_depth, np.float32)
+    ph, pw = prev_depth.shape
+    prev_skip = None if prev_skip is None else np.ascontiguousarray(prev_skip, np.uint8)
+    start = np.empty((h, w), np.uint16)
+    count = np.empty((h, w), np.uint16)
+    lo, hi = ctypes.c_int(), ctypes.c_int()
+    rc = _sweep_lib().lo_range_from_previous(_p(planes), len(planes), _p(prev_depth), _p(prev_skip), pw, ph, w, h, margin,
+                                             _p(start), _p(count), ctypes.byref(lo), ctypes.byref(hi))
+    assert rc == 0, rc
+    return start, count, lo.value, hi.value
+
+
+def layer_depth(planes, W, H, align, start, count, agg):
+    planes = np.ascontiguousarray(planes, np.float32)
+    start = np.ascontiguousarray(start, np.uint16).reshape(-1)
+    count = np.ascontiguousarray(count, np.uint16).reshape(-1)
+    agg = np.ascontiguousarray(agg, np.uint16).reshape(-1)
+    out = np.empty((H, W), np.float32)
+    assert _sweep_lib().lo_layer_depth(_p(planes), len(planes), W, H, align, _p(start), _p(count), _p(agg), _p(out)) == 0
+    return out
+
+
+def dense_layer(sc, W, H, scale, tile, align, start=None, count=None, skip=None, p1=1, p2_scale=500.0, gw=(0, 0, 0, 0),
+                keep_pass1=False):
+    """DenseLayer.Compute from the images: plane-sweep cost, adaptive eight-path aggregation over the subsampled
+    reference as guidance, WTA, depth. Returns dict(depth, wta, agg, agg_pass1, raw)."""
+    n = len(sc.planes)
+    st = np.zeros((H, W), np.uint16) if start is None else np.asarray(start, np.uint16).reshape(H, W)
+    cn = np.full((H, W), n, np.uint16) if count is None else np.asarray(count, np.uint16).reshape(H, W)
+    raw = layer_sweep_cost(sc, W, H, scale, align, st, cn, skip)
+    g = guidance(sc.ref, W, H, scale)
+    agg, agg1, wta = layer(raw, W, H, st, cn, T=tile, align=align, p1=p1, skip=skip, guidance=g, gw=gw, p2_scale=p2_scale,
+                           adaptive=True, keep_pass1=keep_pass1)
+    depth = layer_depth(sc.planes, W, H, align, st, cn, agg)
+    return dict(depth=depth, wta=wta, agg=agg, agg_pass1=agg1, raw=raw)
+
+
+def skip_mask(mode, w, h):
+    out = np.empty((h, w), np.uint8)
+    rc = _sweep_lib().lo_skip_mask(mode, w, h, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def mt19937(seed, n):
+    return int(_sweep_lib().lo_mt19937(seed, n))
+
+
+def box3_rgba(img):
+    """BoxFilter3 of an RGBA8 image: the single-channel filter on each channel (the sums never mix channels)."""
+    return np.stack([box3(np.ascontiguousarray(img[:, :, c]), 255) for c in range(4)], axis=2)
+
+
+def memory_estimate(W, H, start, count, skip, align, maxhi, image_bytes):
+    """DenseLayer.MemoryEstimate (DenseStereoSgm.cs:142-156) from the layer's ranges (before rounding MaxHi)."""
+    cap = ((np.asarray(count, np.int64).reshape(-1) + align - 1) // align) * align
+    sk = np.asarray(skip).reshape(-1) != 0
+    cost = int((8 + cap * np.where(sk, 2, 3)).sum())
+    mh = -(-maxhi // align) * align
+    w, h = W, H
+    return cost + w * h * 4 + 20 * (2 * w + 4) + 2 * ((4 * mh + 5) * (2 * w + 4) + 8 * w + 16) + image_bytes + w * h * 8
+
+
+L16 = dict(scales=(32, 16, 8, 4, 2, 1), tiles=(4, 8, 16, 32, 64, 64), skip_modes=(0, 0, 0, 0, 2, 2))
+
+
+def dense_pyramid(images, gray, warps, planes8, planes2=None, scales=L16["scales"], tiles=L16["tiles"],
+                  skip_modes=L16["skip_modes"], p1=1, p2_scale=500.0, gw=(0, 0, 0, 0), margin=1, budget=1 << 30):
+    """DenseStereoPyramid.Run (DenseStereoSgm.cs:331-355) composed from the oracle's parts. Returns a list of per-level
+    dicts (depth, align, start, count, skip, minlo, maxhi)."""
+    filtered = [box3_rgba(np.ascontiguousarray(im, np.uint8)) for im in images]
+    H0, W0 = filtered[0].shape[:2]
+    image_bytes = sum(f.shape[0] * f.shape[1] * 4 for f in filtered)
+    wqs = [(12240, 0, 0, 0) if (gray is not None and gray[k]) else (8160, 680, 680, 0) for k in range(1, len(images))]
+    out = []
+    prev = None
+    for i, (sc_, tile, sm) in enumerate(zip(scales, tiles, skip_modes)):
+        W, H = -(-W0 // sc_), -(-H0 // sc_)
+        skip = skip_mask(sm, W, H)
+        align, planes = 8, np.asarray(planes8, np.float32)
+        start = count = None
+        minlo, maxhi = 0, len(planes)
+        if prev is not None:
+            start, count, minlo, maxhi = range_from_previous(planes8, prev["depth"], prev["skip"], W, H, margin)
+            if memory_estimate(W, H, start, count, skip, 8, maxhi, image_bytes) >= budget:
+                if planes2 is None:
+                    raise RuntimeError("align-2 planes needed")
+                start, count, minlo, maxhi = range_from_previous(planes2, prev["depth"], prev["skip"], W, H, margin)
+                if memory_estimate(W, H, start, count, skip, 2, maxhi, image_bytes) >= budget:
+                    raise RuntimeError("memory budget exceeded")
+                align, planes = 2, np.asarray(planes2, np.float32)
+        sc = SweepContext(filtered[0], filtered[1:], warps, wqs, planes)
+        r = dense_layer(sc, W, H, sc_, tile, align, start, count, skip, p1=p1, p2_scale=p2_scale, gw=gw)
+        lvl = dict(depth=r["depth"], align=align, start=start, count=count, skip=skip, minlo=minlo, maxhi=maxhi, W=W, H=H)
+        out.append(lvl)
+        prev = lvl
+    return out

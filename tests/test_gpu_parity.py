@@ -283,3 +283,218 @@ def test_box3(shape, dtype, bits):
     else:
         want = o.box3(img, (1 << bits) - 1)
     np.testing.assert_array_equal(got, want)
+
+
+# ------------------------------------------------------------------ DenseUpsampleLayer
+def _upsample_inputs(pw, ph, W, H, seed):
+    rng = np.random.default_rng(seed)
+    depth = rng.uniform(0.5, 40.0, (ph, pw)).astype(np.float32)
+    depth[rng.random((ph, pw)) < 0.02] = 0.0          # holes: rcp(0) = inf, as the reference's inverse leaves them
+    base = rng.integers(0, 256, (H // 4 + 2, W // 4 + 2, 4)).astype(np.uint8)
+    guide = np.repeat(np.repeat(base, 4, 0), 4, 1)[:H, :W].copy()   # blocky, so the range weights vary widely
+    guide[rng.random((H, W)) < 0.1] = rng.integers(0, 256, 4)
+    conf = rng.uniform(0.0, 1.0, (ph, pw)).astype(np.float32)
+    return depth, guide, conf
+
+
+def _upsample_conf_oracle(conf, guide, pw, ph, sigma):
+    g2 = o.nearest_resize(o.image_shift(guide, 0.5, 0.5), pw, ph)
+    return o.bilateral_upsample(conf, guide, g2, sigma)
+
+
+@pytest.mark.parametrize("pw,ph,W,H,sigma", [(130, 98, 260, 196, 12.0), (65, 49, 130, 98, 12.0), (1, 1, 2, 2, 12.0),
+                                            (97, 31, 194, 62, 3.0), (160, 120, 320, 240, 0.75), (40, 30, 100, 60, 12.0)])
+def test_upsample_layer(pw, ph, W, H, sigma):
+    """Bit-exact against the oracle; the last case has a guide of another aspect than the depth (the resize steps
+    differ per axis)."""
+    depth, guide, conf = _upsample_inputs(pw, ph, W, H, pw * 7 + ph)
+    with sgm.Context(0) as c:
+        got = sgm.upsample_layer(c, depth, guide, W // 2, H // 2, sigma)
+        got2, gconf = sgm.upsample_layer(c, depth, guide, W // 2, H // 2, sigma, prev_conf=conf)
+    want = o.upsample_layer(depth, guide, W // 2, H // 2, sigma)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    np.testing.assert_array_equal(got2.view(np.uint32), want.view(np.uint32))
+    np.testing.assert_array_equal(gconf.view(np.uint32), _upsample_conf_oracle(conf, guide, pw, ph, sigma).view(np.uint32))
+
+
+def test_upsample_layer_device_and_size_rule():
+    import torch
+
+    pw, ph, W, H = 520, 390, 1040, 780
+    depth, guide, conf = _upsample_inputs(pw, ph, W, H, 11)
+    want = o.upsample_layer(depth, guide, W // 2, H // 2)
+    want_c = _upsample_conf_oracle(conf, guide, pw, ph, 12.0)
+    with sgm.Context(0) as c:
+        d_prev = torch.from_numpy(depth).cuda()
+        d_conf = torch.from_numpy(conf).cuda()
+        d_guide = torch.from_numpy(guide).cuda()
+        d_out = torch.empty((H, W), dtype=torch.float32, device="cuda")
+        d_oc = torch.empty((H, W), dtype=torch.float32, device="cuda")
+        for with_conf in (False, True):   # the second call reuses the held buffers
+            sgm.upsample_layer_device(c, d_prev.data_ptr(), pw, ph, d_guide.data_ptr(), W, H, W // 2, H // 2,
+                                      d_out.data_ptr(), d_prev_conf=d_conf.data_ptr() if with_conf else 0,
+                                      d_conf=d_oc.data_ptr() if with_conf else 0)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(d_out.cpu().numpy().view(np.uint32), want.view(np.uint32))
+        np.testing.assert_array_equal(d_oc.cpu().numpy().view(np.uint32), want_c.view(np.uint32))
+        with pytest.raises(sgm.SgmError) as e:
+            sgm.upsample_layer(c, depth, guide, W // 2 + 1, H // 2)
+        assert e.value.code == sgm.EINVAL and "Incorrect output depth size" in str(e.value)
+
+
+# ------------------------------------------------------------------ plane-sweep matching cost
+from test_sweep_oracle import COLOUR, GRAY, baseline_warp, planes as sweep_planes, textured, translation_warp  # noqa: E402
+
+
+def _sweep_case(seed, H0, W0, nv, n, near=500.0, far=5000.0):
+    rng = np.random.default_rng(seed)
+    ref = textured(H0, W0, seed)
+    others, warps = [], []
+    for k in range(nv):
+        others.append(textured(H0 + 2 * (k % 3), W0 - 4 * (k % 4), seed * 10 + k))
+        m = baseline_warp(float(rng.uniform(2000, 12000)))
+        m[1], m[4] = rng.uniform(-0.02, 0.02, 2)
+        m[2], m[6] = rng.uniform(-1e-4, 1e-4, 2)
+        m[13] = rng.uniform(-200, 200)
+        warps.append(m)
+    wqs = [COLOUR if k % 2 == 0 else GRAY for k in range(nv)]
+    return ref, others, warps, wqs, sweep_planes(n, near, far)
+
+
+@pytest.mark.parametrize("seed,H0,W0,nv,n,scale,align,mode", [
+    (1, 64, 96, 1, 24, 4, 8, "uniform"), (2, 96, 128, 3, 40, 8, 8, "ranged"), (3, 80, 100, 5, 200, 16, 8, "uniform"),
+    (4, 60, 90, 2, 64, 2, 2, "ranged"), (5, 70, 70, 4, 100, 1, 8, "ranged-skip"), (6, 40, 50, 16, 16, 4, 8, "uniform"),
+    (7, 50, 64, 2, 700, 8, 8, "uniform")])
+def test_sweep_cost_matches_oracle(seed, H0, W0, nv, n, scale, align, mode):
+    ref, others, warps, wqs, pl = _sweep_case(seed, H0, W0, nv, n)
+    W, H = -(-W0 // scale), -(-H0 // scale)
+    start = count = skip = None
+    rng = np.random.default_rng(seed + 100)
+    if mode.startswith("ranged"):
+        start = rng.integers(0, n - 1, (H, W)).astype(np.uint16)
+        count = rng.integers(0, np.minimum(n - start, 70) + 1).astype(np.uint16)
+    if mode.endswith("skip"):
+        skip = sgm.skip_mask(2, W, H)
+    cc = sgm.CostContext(ref, others, warps, wqs, pl)
+    with sgm.Context(0) as c:
+        got = sgm.sweep_cost(c, cc, W, H, scale, align, start, count, skip)
+    sc = o.SweepContext(ref, others, warps, wqs, pl)
+    st = np.zeros((H, W), np.uint16) if start is None else start
+    cn = np.full((H, W), n, np.uint16) if count is None else count
+    want = o.layer_sweep_cost(sc, W, H, scale, align, st, cn, skip)
+    np.testing.assert_array_equal(got, want)
+
+
+def _scene_views(seed, H0, W0, nv, n, near=800.0, far=6000.0):
+    """A reference image and cameras looking at a fronto-parallel scene at a few depths (shifts by f*B/z),
+    box-filtered like DenseStereoPyramid does, with rectified-pair warps of several baselines."""
+    rng = np.random.default_rng(seed)
+    base = textured(H0, W0 + 64, seed)
+    ref = base[:, :W0].copy()
+    others, warps = [], []
+    for k in range(nv):
+        fb = float(rng.uniform(3000, 9000))
+        shift = int(round(fb / 2500.0))
+        img = base[:H0, shift:shift + W0].copy() if k % 2 == 0 else base[:H0, :W0].copy()
+        others.append(img)
+        m = baseline_warp(-fb if k % 2 == 0 else fb * 0.3)
+        warps.append(m)
+    box = lambda im: np.stack([o.box3(np.ascontiguousarray(im[:, :, c]), 255) for c in range(4)], axis=2)
+    ref = box(ref)
+    others = [box(im) for im in others]
+    wqs = [COLOUR if k % 3 else GRAY for k in range(nv)]
+    return ref, others, warps, wqs, sweep_planes(n, near, far)
+
+
+@pytest.mark.parametrize("seed,H0,W0,nv,n,scale,tile", [(1, 192, 256, 3, 64, 8, 4), (2, 100, 140, 2, 200, 4, 8),
+                                                        (3, 64, 96, 5, 24, 1, 16)])
+def test_dense_layer_uniform(seed, H0, W0, nv, n, scale, tile):
+    ref, others, warps, wqs, pl = _scene_views(seed, H0, W0, nv, n)
+    cc = sgm.CostContext(ref, others, warps, wqs, pl)
+    p = sgm.dense_params(W0, H0, scale, tile)
+    with sgm.Context(0) as c:
+        got = sgm.dense_layer(c, cc, p, want_agg=True, keep_pass1=True)
+    sc = o.SweepContext(ref, others, warps, wqs, pl)
+    want = o.dense_layer(sc, p.width, p.height, scale, tile, 8, gw=sgm.L16_GUIDANCE, keep_pass1=True)
+    np.testing.assert_array_equal(got["agg_pass1"], want["agg_pass1"])
+    np.testing.assert_array_equal(got["agg"], want["agg"])
+    np.testing.assert_array_equal(got["wta"], want["wta"])
+    np.testing.assert_array_equal(got["depth"].view(np.uint32), want["depth"].view(np.uint32))
+
+
+@pytest.mark.parametrize("seed,H0,W0,nv,n,scale,tile,align,skipmode", [(4, 96, 128, 3, 120, 2, 8, 8, 0),
+                                                                      (5, 80, 120, 2, 90, 1, 16, 8, 2),
+                                                                      (6, 72, 100, 4, 60, 2, 4, 2, 2)])
+def test_dense_layer_ranged(seed, H0, W0, nv, n, scale, tile, align, skipmode):
+    ref, others, warps, wqs, pl = _scene_views(seed, H0, W0, nv, n)
+    p = sgm.dense_params(W0, H0, scale, tile, align)
+    W, H = p.width, p.height
+    rng = np.random.default_rng(seed)
+    count = rng.integers(0, 40, (H, W)).astype(np.uint16)
+    start = rng.integers(7, n - count + 1).astype(np.uint16)         # MinLo >= 7: no padding past the segment
+    skip = sgm.skip_mask(skipmode, W, H)
+    cc = sgm.CostContext(ref, others, warps, wqs, pl)
+    with sgm.Context(0) as c:
+        got = sgm.dense_layer(c, cc, p, start, count, skip, want_agg=True)
+    sc = o.SweepContext(ref, others, warps, wqs, pl)
+    want = o.dense_layer(sc, W, H, scale, tile, align, start, count, skip, gw=sgm.L16_GUIDANCE)
+    np.testing.assert_array_equal(got["agg"], want["agg"])
+    np.testing.assert_array_equal(got["wta"], want["wta"])
+    np.testing.assert_array_equal(got["depth"].view(np.uint32), want["depth"].view(np.uint32))
+
+
+@pytest.mark.parametrize("seed,pw,ph,w,h,n,skipped", [(1, 16, 12, 32, 24, 64, False), (2, 65, 49, 130, 98, 300, True),
+                                                       (3, 9, 7, 9, 7, 20, True), (4, 40, 30, 81, 60, 1000, True)])
+def test_range_from_previous(seed, pw, ph, w, h, n, skipped):
+    rng = np.random.default_rng(seed)
+    pl = sweep_planes(n)
+    prev = pl[rng.integers(0, n, (ph, pw))].astype(np.float32)
+    jitter = rng.random((ph, pw)) < 0.5
+    prev[jitter] *= rng.uniform(0.97, 1.03, int(jitter.sum())).astype(np.float32)   # between planes
+    prev[0, 0] = 0.0                                                                  # rcpss(0) = inf
+    pskip = sgm.skip_mask(2, pw, ph) if skipped else None
+    with sgm.Context(0) as c:
+        got = sgm.range_from_previous(c, pl, prev, pskip, w, h)
+    want = o.range_from_previous(pl, prev, pskip, w, h)
+    np.testing.assert_array_equal(got[0], want[0])
+    np.testing.assert_array_equal(got[1], want[1])
+    assert got[2:] == want[2:]
+
+
+@pytest.mark.parametrize("W0,H0,seed", [(256, 192, 1), (208, 152, 2)])
+def test_dense_pyramid_matches_oracle(W0, H0, seed):
+    """DenseStereoPyramid.Run end to end: six levels of depth, bit for bit, and the memory mode of each level."""
+    imgs, gray, warps, _ = scene.multiview(W0, H0, seed=seed)
+    p8, p2 = scene.plane_list(500.0, 20000.0, 48), scene.plane_list(500.0, 20000.0, 46)
+    want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
+    for lv in want[1:]:   # the scene keeps every range clear of the unsupported padding case
+        assert lv["minlo"] >= 7
+    with sgm.Context(0) as c:
+        depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2)
+    assert aligns == [lv["align"] for lv in want]
+    for i, (d, lv) in enumerate(zip(depths, want)):
+        np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+
+
+def test_dense_pyramid_memory_modes():
+    """A budget between the last level's align-8 and align-2 estimates switches that level to align 2 with the
+    align-2 planes (the smaller levels fit at align 8); a budget below both fails like the reference."""
+    imgs, gray, warps, _ = scene.multiview(128, 96, seed=3)
+    p8, p2 = scene.plane_list(500.0, 20000.0, 48), scene.plane_list(500.0, 20000.0, 46)
+    base = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
+    image_bytes = sum(im.shape[0] * im.shape[1] * 4 for im in imgs)
+    lv = base[-1]
+    est8 = o.memory_estimate(lv["W"], lv["H"], lv["start"], lv["count"], lv["skip"], 8, lv["maxhi"], image_bytes)
+    budget = est8      # the last level at align 8 does not fit (>=), so it takes align 2
+    want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE, budget=budget)
+    assert [w["align"] for w in want] == [8, 8, 8, 8, 8, 2]
+    p = sgm.dense_pyramid_defaults()
+    p.budget = budget
+    with sgm.Context(0) as c:
+        depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2, params=p)
+        assert aligns == [w["align"] for w in want]
+        for d, w in zip(depths, want):
+            np.testing.assert_array_equal(d.view(np.uint32), w["depth"].view(np.uint32))
+        p.budget = 1
+        with pytest.raises(sgm.NotSupported):
+            sgm.dense_pyramid(c, imgs, gray, warps, p8, p2, params=p)
