@@ -285,6 +285,87 @@ def bench_pyramid(np, sgm, scene, n_req, burst, budget_gb, slots):
             "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
 
 
+L16_W, L16_H, L16_N8, L16_N2 = 2080, 1560, 304, 302
+
+
+def bench_l16(np, torch, sgm, scene, reps, cpu_threads):
+    """The L16 dense stereo end to end (DenseStereoPyramid.Run, six levels from 65x49 to 2080x1560, then the
+    upsample layer to 4160x3120): one reference and four cameras of a rendered two-depth scene, 304 planes. The
+    device call takes host images (box filter, levels, depth downloads included); the upsample is timed on
+    device-resident buffers. The oracle runs the same pyramid on the host cores (cost row-parallel, aggregation
+    serial as the reference) and its depths are compared bit for bit."""
+    imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
+    p8, p2 = scene.plane_list(500.0, 20000.0, L16_N8), scene.plane_list(500.0, 20000.0, L16_N2)
+    ctx = sgm.Context(0)
+    depths, aligns = sgm.dense_pyramid(ctx, imgs, gray, warps, p8, p2)
+    host_times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        depths, aligns = sgm.dense_pyramid(ctx, imgs, gray, warps, p8, p2)
+        host_times.append((time.perf_counter() - t) * 1e3)
+    # images resident in HBM, depths left there: the call's wall time (it synchronises before returning)
+    d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+    d_depths = [torch.empty(d.shape, dtype=torch.float32, device="cuda") for d in depths]
+    shapes = [im.shape[:2] for im in imgs]
+    args = ([t.data_ptr() for t in d_imgs], shapes, gray, warps, p8, p2, [t.data_ptr() for t in d_depths])
+    sgm.dense_pyramid_device(ctx, *args)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        sgm.dense_pyramid_device(ctx, *args)
+        times.append((time.perf_counter() - t) * 1e3)
+    same = all(np.array_equal(d.cpu().numpy().view(np.uint32), h.view(np.uint32)) for d, h in zip(d_depths, depths))
+    # the upsample layer on device buffers, timed with events on the stream it runs on
+    guide = np.repeat(np.repeat(imgs[0], 2, 0), 2, 1)
+    st = torch.cuda.Stream()
+    d_prev = torch.from_numpy(depths[-1]).cuda()
+    d_guide = torch.from_numpy(guide).cuda()
+    d_out = torch.empty((2 * L16_H, 2 * L16_W), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(st):
+        call = lambda: sgm.upsample_layer_device(ctx, d_prev.data_ptr(), L16_W, L16_H, d_guide.data_ptr(), 2 * L16_W,  # noqa: E731
+                                                 2 * L16_H, L16_W, L16_H, d_out.data_ptr(), stream=st.cuda_stream)
+        call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(20):
+            call()
+        e1.record(st)
+    torch.cuda.synchronize()
+    ups_ms = e0.elapsed_time(e1) / 20
+    ups_bytes = (2 * L16_W) * (2 * L16_H) * (4 + 4) + L16_W * L16_H * (4 + 4)
+    ctx.close()
+    out = {"workload": f"DenseStereoPyramid.Run at {L16_W}x{L16_H} (6 levels, scales 32..1, 1 reference + 4 cameras, "
+                       f"{L16_N8} planes, adaptive P2, skip masks on the last two), then the upsample layer to "
+                       f"{2 * L16_W}x{2 * L16_H}; rendered two-depth scene",
+           "levels": [{"scale": s, "align": a, "shape": list(d.shape)} for s, a, d in zip(sgm.L16_SCALES, aligns, depths)],
+           "gpu_ms_p50": round(float(np.percentile(times, 50)), 3), "gpu_ms_min": round(min(times), 3),
+           "gpu_includes": "images resident in HBM: box filter, six levels (ranges, cost, aggregation, depth)",
+           "host_buffers_ms_p50": round(float(np.percentile(host_times, 50)), 3),
+           "device_equals_host_form": same,
+           "upsample_ms": round(ups_ms, 4),
+           "upsample_gb_s": round(ups_bytes / (ups_ms / 1e3) / 1e9, 1)}
+    if cpu_threads:
+        t = time.perf_counter()
+        want = o_pyramid(imgs, gray, warps, p8, p2, cpu_threads)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value_s": round(cpu_s, 3), "cores": cpu_threads, "kind": "port",
+                               "sample": "the whole pyramid at full size (the same scene and planes)"}
+        out["gpu_vs_cpu"] = round(cpu_s * 1e3 / out["gpu_ms_p50"], 1)
+        out["identical_to_oracle"] = all(np.array_equal(d.view(np.uint32), w["depth"].view(np.uint32))
+                                         for d, w in zip(depths, want))
+    return out
+
+
+def o_pyramid(imgs, gray, warps, p8, p2, threads):
+    """The oracle's pyramid (CPU baseline and checker of the L16 leg)."""
+    from oracle import oracle as o
+    from lux_amd import sgm
+
+    return o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE, threads=threads)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,8 +374,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
-    ap.add_argument("--forms", default="batch,serve,pyramid",
-                    help="extra legs: batch, serve, pyramid (comma list; '' for none)")
+    ap.add_argument("--forms", default="batch,serve,pyramid,l16",
+                    help="extra legs: batch, serve, pyramid, l16 (comma list; '' for none)")
     ap.add_argument("--pyr-requests", type=int, default=128)
     ap.add_argument("--pyr-burst", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=500)
@@ -422,6 +503,11 @@ def main():
         print("bench: coarse-to-fine leg", file=sys.stderr, flush=True)
         pyramid = bench_pyramid(np, sgm, scene, args.pyr_requests, args.pyr_burst, 192, args.serve_slots)
 
+    l16 = None
+    if "l16" in forms:
+        print("bench: L16 dense pyramid leg", file=sys.stderr, flush=True)
+        l16 = bench_l16(np, torch, sgm, scene, 5, 0 if args.no_cpu else min(16, os.cpu_count() or 1))
+
     out = {
         "metric": "SGM Gcells/s (W·H·D·paths) + p50/p99 depth-request latency, served vs plain",
         "value": round(gcells, 3),
@@ -455,6 +541,7 @@ def main():
         "batched": batched,
         "served": served,
         "coarse_to_fine": pyramid,
+        "l16_pyramid": l16,
     }
     print(json.dumps(out))
 

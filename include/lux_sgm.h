@@ -218,6 +218,12 @@ int lux_sgm_dense_pyramid(lux_sgm_ctx* ctx, const lux_sgm_dense_pyramid_params* 
                           const uint8_t* const* images, const int32_t* widths, const int32_t* heights, const uint8_t* gray,
                           const float* warps, const float* planes8, int32_t n8, const float* planes2, int32_t n2,
                           float* const* depths, int32_t* aligns);
+/* The same with the images and depths in device memory (every other argument stays on the host); the images are
+ * read, not changed. Synchronises the context's stream before returning. */
+int lux_sgm_dense_pyramid_device(lux_sgm_ctx* ctx, const lux_sgm_dense_pyramid_params* params, int32_t n_images,
+                                 const uint8_t* const* d_images, const int32_t* widths, const int32_t* heights,
+                                 const uint8_t* gray, const float* warps, const float* planes8, int32_t n8,
+                                 const float* planes2, int32_t n2, float* const* d_depths, int32_t* aligns);
 
 /* DenseUpsampleLayer.Run (DenseUpsampleLayer.cs:35-48): the finest layer's depth prev_depth [ph*pw] float to the
  * guide's resolution. The RGBA8 guide [height*width*4] is shifted by half a pixel with the 4-tap cubic, resized to

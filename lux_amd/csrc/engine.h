@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -38,6 +39,7 @@ struct Workspace {
     int device = 0;
     Budget* budget = nullptr;
     std::map<std::string, DevBuf> bufs;
+    std::set<std::string> filled;    // buffers whose contents outlive a call (cleared when reallocated or released)
     const Plan* plan = nullptr;      // whose visit lists are in "visits"
     void* sync_zeroed = nullptr;     // band flag array known to hold no live epoch
     int epoch = 0;

@@ -43,6 +43,7 @@ int Workspace::get(const char* name, size_t bytes, void** out, std::string* err)
         if (b.p) (void)hipFree(b.p);
         b.p = nullptr;
         b.n = 0;
+        filled.erase(name);
         hipError_t e = hipMalloc(&b.p, bytes);
         if (e != hipSuccess) {
             b.p = nullptr;
@@ -71,6 +72,7 @@ void Workspace::release()
         if (budget) budget->release(kv.second.n);
     }
     bufs.clear();
+    filled.clear();
     plan = nullptr;
     sync_zeroed = nullptr;
 }

@@ -93,23 +93,29 @@ __device__ __forceinline__ uint32_t block_cost(const uint32_t* img, int iw, int 
     return ck < 65535u ? ck : 65535u;
 }
 
-// raw[cell*cs + j], j < cs: the normalised cost for j < count (0 for a skipped pixel), 255 past count.
-// vstart / vcount null: every pixel searches planes [0, ucount).
+// raw[cell*cs + j], j < cap: the normalised cost for j < count (0 for a skipped pixel), 255 past count (bytes past a
+// pixel's cap are masked by the sweeps and left unwritten). vstart / vcount / vcap null: every pixel searches planes
+// [0, ucount) with cap ucap.
 template <int G>
 __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W, int H, int scale, const uint16_t* vstart,
-                                                         const uint16_t* vcount, int ucount, const uint8_t* skip, uint8_t* raw,
-                                                         int cs)
+                                                         const uint16_t* vcount, const uint16_t* vcap, int ucount, int ucap,
+                                                         const uint8_t* skip, const uint32_t* list, const uint32_t* nlist,
+                                                         uint8_t* raw, int cs)
 {
     const int lane = threadIdx.x & 63, li = lane & (G - 1);
-    const size_t cell = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / G) + (lane / G);
+    const size_t seg = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / G) + (lane / G);
     const size_t cells = (size_t)W * H;
-    const bool live = cell < cells;
-    int start = 0, count = 0;
+    // with a list, segment i takes the i-th computed cell; else cell i
+    const size_t nseg = list ? (size_t)*nlist : cells;
+    const bool live = seg < nseg;
+    const size_t cell = live ? (list ? (size_t)list[seg] : seg) : 0;
+    int start = 0, count = 0, cap = 0;
     bool skipped = false;
     int x = 0, y = 0;
     if (live) {
         start = vstart ? (int)vstart[cell] : 0;
         count = vcount ? (int)vcount[cell] : ucount;
+        cap = vcap ? (int)vcap[cell] : ucap;
         skipped = skip && skip[cell];
         y = (int)(cell / W);
         x = (int)(cell - (size_t)y * W);
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W
     const u16x2 cape = as_u16x2(a.cap_ev), capo = as_u16x2(a.cap_od);
     const float xf = (float)xi, yf = (float)yi;
     // every lane runs the wave's longest row of chunks, so the shuffles below see whole segments
-    int need = live ? cs : 0;
+    int need = live ? cap : 0;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) need = max(need, __shfl_xor(need, o));
     const bool eval = live && !skipped;
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W
             carry[k] = (uint32_t)__shfl((int)got, G - 1, G);
             tot += got;
         }
-        if (live && d < cs) {
+        if (live && d < cap) {
             uint8_t out = 255;
             if (d < count) out = skipped ? 0 : (uint8_t)f2i_sat(__fmul_rn((float)(tot & 0xffffu), a.fnorm));
             raw[cell * cs + d] = out;
@@ -218,48 +224,157 @@ __global__ void range_index_kernel(const float* prev, size_t np, const float* b,
 }
 
 // Statistics of a layer's ranges, as 64-bit words: 0 MinLo, 1 max hi, 2 max cap, 3 max count, 4 max(start + cap),
-// 5 a range past the last plane, 6 sum of caps of computed pixels, 7 sum of caps of skipped pixels.
-constexpr int kStats = 8;
-__global__ void stats_init_kernel(unsigned long long* st)
+// 5 a range past the last plane, 6 sum of caps of computed pixels, 7 sum of caps of skipped pixels, 8 computed pixels.
+constexpr int kStats = 9;
+constexpr int kRangeBlock = 256;
+// Reduces the per-block partials of range_window_kernel (min for word 0, max for 1-5, sums for 6-8) into st.
+__global__ __launch_bounds__(1024) void stats_reduce_kernel(const unsigned long long* part, int nblocks, unsigned long long* st)
 {
-    if (threadIdx.x < kStats) st[threadIdx.x] = threadIdx.x == 0 ? ~0ull : 0ull;
+    __shared__ unsigned long long red[16][kStats];
+    unsigned long long r[kStats];
+    r[0] = ~0ull;
+#pragma unroll
+    for (int k = 1; k < kStats; k++) r[k] = 0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        const unsigned long long* q = part + (size_t)b * kStats;
+        r[0] = min(r[0], q[0]);
+#pragma unroll
+        for (int k = 1; k < 6; k++) r[k] = max(r[k], q[k]);
+#pragma unroll
+        for (int k = 6; k < kStats; k++) r[k] += q[k];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        r[0] = min(r[0], (unsigned long long)__shfl_xor((long long)r[0], o));
+#pragma unroll
+        for (int k = 1; k < 6; k++) r[k] = max(r[k], (unsigned long long)__shfl_xor((long long)r[k], o));
+#pragma unroll
+        for (int k = 6; k < kStats; k++) r[k] += (unsigned long long)__shfl_xor((long long)r[k], o);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < kStats; k++) red[wv][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int w = 1; w < nw; w++) {
+            r[0] = min(r[0], red[w][0]);
+            for (int k = 1; k < 6; k++) r[k] = max(r[k], red[w][k]);
+            for (int k = 6; k < kStats; k++) r[k] += red[w][k];
+        }
+        for (int k = 0; k < kStats; k++) st[k] = r[k];
+    }
 }
 
-// RangeFromPrevious, the rest (:105-139): the 4 x 4 window (-1..+2) of computed previous pixels around
-// ((int)(x * fx), (int)(y * fy)); start = max(0, min - margin), count = min(n - 1, max + 1) - start, in 16 bits
-// as the reference stores them; cap = roundup(count, align). The window minimum of the two separable passes is the
-// minimum over the window, so one pass does.
-__global__ void range_window_kernel(const uint16_t* idx, const uint8_t* pskip, int pw, int ph, int W, int H, float fx, float fy,
-                                    int margin, int n, int align, const uint8_t* skip, uint16_t* start, uint16_t* count,
-                                    uint16_t* cap, unsigned long long* st)
+// RangeFromPrevious, the window (:105-126): min / max of the computed pixels' indices over x-1..x+2 (clamped),
+// 0xffff / 0 when none is computed; then over y-1..y+2 of those. Packed min | max << 16.
+__global__ void range_hwin_kernel(const uint16_t* idx, const uint8_t* pskip, int pw, int ph, uint32_t* hmm)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= W) return;
-    const int py = f2i_sat(__fmul_rn((float)y, fy)), px = f2i_sat(__fmul_rn((float)x, fx));
+    if (x >= pw) return;
     int mn = 0xffff, mx = 0;
-    for (int j = -1; j <= 2; j++) {
-        const int yy = min(max(py + j, 0), ph - 1);
-        for (int k = -1; k <= 2; k++) {
-            const int xx = min(max(px + k, 0), pw - 1);
-            const size_t i = (size_t)yy * pw + xx;
-            if (pskip && pskip[i]) continue;
-            mn = min(mn, (int)idx[i]);
-            mx = max(mx, (int)idx[i]);
-        }
+#pragma unroll
+    for (int k = -1; k <= 2; k++) {
+        const size_t i = (size_t)y * pw + min(max(x + k, 0), pw - 1);
+        if (pskip && pskip[i]) continue;
+        mn = min(mn, (int)idx[i]);
+        mx = max(mx, (int)idx[i]);
     }
+    hmm[(size_t)y * pw + x] = (uint32_t)mn | ((uint32_t)mx << 16);
+}
+__global__ void range_vwin_kernel(const uint32_t* hmm, int pw, int ph, uint32_t* vmm)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= pw) return;
+    int mn = 0xffff, mx = 0;
+#pragma unroll
+    for (int k = -1; k <= 2; k++) {
+        const uint32_t v = hmm[(size_t)min(max(y + k, 0), ph - 1) * pw + x];
+        mn = min(mn, (int)(v & 0xffffu));
+        mx = max(mx, (int)(v >> 16));
+    }
+    vmm[(size_t)y * pw + x] = (uint32_t)mn | ((uint32_t)mx << 16);
+}
+
+// RangeFromPrevious, the ranges (:127-139): the window at ((int)(x * fx), (int)(y * fy)); start = max(0, min - margin),
+// count = min(n - 1, max + 1) - start, in 16 bits as the reference stores them; cap = roundup(count, align).
+__global__ __launch_bounds__(kRangeBlock) void range_window_kernel(const uint32_t* vmm, int pw, int ph, int W, int H, float fx,
+                                                                   float fy, int margin, int n, int align, const uint8_t* skip,
+                                                                   uint16_t* start, uint16_t* count, uint16_t* cap,
+                                                                   unsigned long long* part)
+{
+    const int x0 = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const bool live = x0 < W;
+    const int x = live ? x0 : W - 1;  // lanes past the row repeat its last pixel for the reductions, write nothing
+    const int py = f2i_sat(__fmul_rn((float)y, fy)), px = f2i_sat(__fmul_rn((float)x, fx));
+    const uint32_t v = vmm[(size_t)py * pw + px];
+    const int mn = (int)(v & 0xffffu), mx = (int)(v >> 16);
     const int lo = max(0, mn - margin), hi = min(n - 1, mx + 1);
     const int s = lo & 0xffff, c = (hi - lo) & 0xffff, cp = ((c + align - 1) / align) * align;
     const size_t o = (size_t)y * W + x;
-    start[o] = (uint16_t)s;
-    count[o] = (uint16_t)c;
-    cap[o] = (uint16_t)cp;
-    atomicMin(&st[0], (unsigned long long)s);
-    atomicMax(&st[1], (unsigned long long)(hi & 0xffff));
-    atomicMax(&st[2], (unsigned long long)cp);
-    atomicMax(&st[3], (unsigned long long)c);
-    atomicMax(&st[4], (unsigned long long)(s + cp));
-    if (s + c > n || cp > 0xffff) atomicMax(&st[5], 1ull);
-    atomicAdd(&st[(skip && skip[o]) ? 7 : 6], (unsigned long long)cp);
+    if (live) {
+        start[o] = (uint16_t)s;
+        count[o] = (uint16_t)c;
+        cap[o] = (uint16_t)cp;
+    }
+    // one atomic per wave and statistic: reduce over the wave first
+    const bool sk = skip && skip[o];
+    int r0 = s, r1 = hi & 0xffff, r2 = cp, r3 = c, r4 = s + cp, r5 = (s + c > n || cp > 0xffff) ? 1 : 0;
+    uint32_t r6 = live && !sk ? (uint32_t)cp : 0u, r7 = live && sk ? (uint32_t)cp : 0u, r8 = live && !sk ? 1u : 0u;
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+        r8 += (uint32_t)__shfl_xor((int)r8, k);
+        r0 = min(r0, __shfl_xor(r0, k));
+        r1 = max(r1, __shfl_xor(r1, k));
+        r2 = max(r2, __shfl_xor(r2, k));
+        r3 = max(r3, __shfl_xor(r3, k));
+        r4 = max(r4, __shfl_xor(r4, k));
+        r5 = max(r5, __shfl_xor(r5, k));
+        r6 += (uint32_t)__shfl_xor((int)r6, k);
+        r7 += (uint32_t)__shfl_xor((int)r7, k);
+    }
+    // the block's partial (no atomics: a second kernel reduces the partials)
+    __shared__ uint32_t red[kRangeBlock / 64][kStats];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = (uint32_t)r0; red[wv][1] = (uint32_t)r1; red[wv][2] = (uint32_t)r2;
+        red[wv][3] = (uint32_t)r3; red[wv][4] = (uint32_t)r4; red[wv][5] = (uint32_t)r5;
+        red[wv][6] = r6; red[wv][7] = r7; red[wv][8] = r8;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long q[kStats];
+        for (int k = 0; k < kStats; k++) q[k] = red[0][k];
+        for (int w = 1; w < kRangeBlock / 64; w++) {
+            q[0] = min(q[0], (unsigned long long)red[w][0]);
+            for (int k = 1; k < 6; k++) q[k] = max(q[k], (unsigned long long)red[w][k]);
+            for (int k = 6; k < kStats; k++) q[k] += red[w][k];
+        }
+        unsigned long long* o = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kStats;
+        for (int k = 0; k < kStats; k++) o[k] = q[k];
+    }
+}
+
+// Skipped pixels get their raw bytes here (0 over the planes, 255 over the padding, as Pixel uses them, :242); the
+// computed ones are listed for the cost kernel (in no particular order: each is costed on its own).
+__global__ void skip_list_kernel(const uint8_t* skip, const uint16_t* vcount, const uint16_t* vcap, size_t cells, uint8_t* raw,
+                                 int cs, uint32_t* list, uint32_t* nlist)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < cells;
+    const bool sk = live && skip[i];
+    if (sk) {
+        const int cnt = vcount[i], cp = vcap[i];
+        uint8_t* r = raw + i * cs;
+        for (int j = 0; j < cp; j++) r[j] = j < cnt ? 0 : 255;
+    }
+    const bool comp = live && !sk;
+    const uint64_t m = __ballot(comp);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(nlist, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0);
+    if (comp) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
 }
 
 // The layer's depth (Compute :225-231): Planes[start + first minimum], the WTA index the sweeps leave.
@@ -349,19 +464,40 @@ int upload_sweep(Workspace& ws, const lux_sgm_sweep* s, bool device, hipStream_t
     return 0;
 }
 
+// G (lanes per pixel) follows the mean cap: wide enough that a typical pixel needs one chunk, narrow enough that
+// the lanes of short ranges are not idle; a wave loops over its widest pixel's chunks.
 hipError_t launch_sweep_cost(const SweepDev& a, int W, int H, int scale, const uint16_t* vstart, const uint16_t* vcount,
-                             int ucount, const uint8_t* skip, uint8_t* raw, int cs, hipStream_t s)
+                             const uint16_t* vcap, int ucount, int ucap, double mean_cap, const uint8_t* skip, uint8_t* raw,
+                             int cs, hipStream_t s, Workspace* ws = nullptr, uint64_t computed = 0)
 {
     const size_t cells = (size_t)W * H;
     int G = 8;
-    while (G < 64 && G < cs) G <<= 1;
+    while (G < 64 && G < mean_cap) G <<= 1;
+    // ranged layers with a skip mask: skipped pixels are written by a list kernel, the cost kernel runs over the
+    // computed ones only (`computed` of them)
+    const uint32_t* list = nullptr;
+    const uint32_t* nlist = nullptr;
+    size_t nseg = cells;
+    if (ws && skip && vcount && vcap) {
+        void *dl, *dn;
+        std::string e2;
+        if (ws->get("sw_list", cells * 4, &dl, &e2) || ws->get("sw_nlist", 256, &dn, &e2)) return hipErrorOutOfMemory;
+        hipError_t e = launch_fill_u32((uint32_t*)dn, 1, 0u, s);
+        if (e != hipSuccess) return e;
+        skip_list_kernel<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(skip, vcount, vcap, cells, raw, cs, (uint32_t*)dl,
+                                                                        (uint32_t*)dn);
+        list = (const uint32_t*)dl;
+        nlist = (const uint32_t*)dn;
+        nseg = computed;
+        skip = nullptr;  // every listed cell is computed
+    }
     const size_t per_block = 4 * (64 / G);
-    const unsigned blocks = (unsigned)((cells + per_block - 1) / per_block);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, (nseg + per_block - 1) / per_block);
     switch (G) {
-        case 8: sweep_cost_kernel<8><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, ucount, skip, raw, cs); break;
-        case 16: sweep_cost_kernel<16><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, ucount, skip, raw, cs); break;
-        case 32: sweep_cost_kernel<32><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, ucount, skip, raw, cs); break;
-        default: sweep_cost_kernel<64><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, ucount, skip, raw, cs); break;
+        case 8: sweep_cost_kernel<8><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
+        case 16: sweep_cost_kernel<16><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
+        case 32: sweep_cost_kernel<32><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
+        default: sweep_cost_kernel<64><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
     }
     return hipGetLastError();
 }
@@ -382,7 +518,7 @@ static std::vector<float> range_boundaries(const float* planes, int n, int* mono
 // Host copies of a layer's range statistics.
 struct RangeStats {
     int minlo = 0, maxhi = 0, maxcap = 0, maxcount = 0, maxend = 0, bad = 0;
-    uint64_t cap_computed = 0, cap_skipped = 0;
+    uint64_t cap_computed = 0, cap_skipped = 0, computed = 0;
 };
 
 // RangeFromPrevious on the device for a W x H layer from the previous layer's depth and skip (device), writing
@@ -398,20 +534,30 @@ int ranges_from_previous(Workspace& ws, const float* planes, int n, const float*
     }
     int mono = 0;
     std::vector<float> b = range_boundaries(planes, n, &mono);
-    void *db, *didx, *dst;
+    void *db, *didx, *dst, *dh, *dv, *dpart;
     int rc;
     const size_t np = (size_t)pw * ph;
+    const int gx = (W + kRangeBlock - 1) / kRangeBlock, nblocks = gx * H;
     if ((rc = ws.get("rg_bounds", (size_t)n * 4, &db, err)) || (rc = ws.get("rg_idx", np * 2, &didx, err)) ||
-        (rc = ws.get("rg_stats", kStats * 8, &dst, err)))
+        (rc = ws.get("rg_stats", kStats * 8, &dst, err)) || (rc = ws.get("rg_hwin", np * 4, &dh, err)) ||
+        (rc = ws.get("rg_vwin", np * 4, &dv, err)) || (rc = ws.get("rg_part", (size_t)nblocks * kStats * 8, &dpart, err)))
         return rc;
     hipError_t e = hipMemcpyAsync(db, b.data(), (size_t)n * 4, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_status(e, "upload boundaries", err);
     range_index_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(d_prev, np, (const float*)db, n, mono, (uint16_t*)didx);
-    stats_init_kernel<<<1, 64, 0, s>>>((unsigned long long*)dst);
     const float fy = (float)(ph - 1) / (float)(H - 1), fx = (float)(pw - 1) / (float)(W - 1);
-    range_window_kernel<<<dim3((W + 127) / 128, H), 128, 0, s>>>((const uint16_t*)didx, d_pskip, pw, ph, W, H, fx, fy, margin, n,
-                                                                 align, d_skip, d_start, d_count, d_cap,
-                                                                 (unsigned long long*)dst);
+    // (int)(y * fy) stays inside the previous layer: y * fy <= (H-1) * (ph-1)/(H-1) up to rounding, checked here
+    auto f2i = [](float f) { return f != f ? 0 : (f >= 2147483648.0f ? INT_MAX : (f < -2147483648.0f ? INT_MIN : (int)f)); };
+    const int pyl = f2i((float)(H - 1) * fy), pxl = f2i((float)(W - 1) * fx);
+    if (pyl > ph - 1 || pxl > pw - 1) {
+        if (err) *err = "resampled position past the previous layer";
+        return LUX_SGM_EINVAL;
+    }
+    range_hwin_kernel<<<dim3((pw + 127) / 128, ph), 128, 0, s>>>((const uint16_t*)didx, d_pskip, pw, ph, (uint32_t*)dh);
+    range_vwin_kernel<<<dim3((pw + 127) / 128, ph), 128, 0, s>>>((const uint32_t*)dh, pw, ph, (uint32_t*)dv);
+    range_window_kernel<<<dim3(gx, H), kRangeBlock, 0, s>>>((const uint32_t*)dv, pw, ph, W, H, fx, fy, margin, n, align, d_skip,
+                                                            d_start, d_count, d_cap, (unsigned long long*)dpart);
+    stats_reduce_kernel<<<1, 1024, 0, s>>>((const unsigned long long*)dpart, nblocks, (unsigned long long*)dst);
     unsigned long long st[kStats];
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(st, dst, sizeof st, hipMemcpyDeviceToHost, s);
@@ -425,6 +571,7 @@ int ranges_from_previous(Workspace& ws, const float* planes, int n, const float*
     out->bad = (int)st[5];
     out->cap_computed = st[6];
     out->cap_skipped = st[7];
+    out->computed = st[8];
     return 0;
 }
 
@@ -468,7 +615,10 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         w->b.seedstart = (uint16_t*)ss;
         w->b.seedcap = (uint16_t*)sc;
     }
-    hipError_t e = launch_sweep_cost(a, W, H, p.scale, d_start, d_count, a.n, d_skip, w->raw, g.cs, s);
+    const int ucap = ((a.n + p.align - 1) / p.align) * p.align;
+    const double mean_cap = ranged ? (double)(rs.cap_computed + rs.cap_skipped) / ((double)W * H) : (double)ucap;
+    hipError_t e = launch_sweep_cost(a, W, H, p.scale, d_start, d_count, d_cap, a.n, ucap, mean_cap, d_skip, w->raw, g.cs, s,
+                                     &ws, rs.computed);
     if (e != hipSuccess) return hip_status(e, "sweep cost", err);
     void *gi, *sg;
     if ((rc = ws.get("guidance", (size_t)W * H * 4, &gi, err)) || (rc = ws.get("seedguid", (size_t)3 * W * 4, &sg, err)))
@@ -566,8 +716,9 @@ int lux_sgm_sweep_cost(lux_sgm_ctx* c, const lux_sgm_sweep* sw, int32_t width, i
         e = hipMemcpyAsync(sk, skip, cells, hipMemcpyHostToDevice, s);
     }
     if (e == hipSuccess)
-        e = lux::launch_sweep_cost(a, width, height, scale, (const uint16_t*)vs, (const uint16_t*)vn, n, (const uint8_t*)sk,
-                                   (uint8_t*)dense, cs, s);
+        e = lux::launch_sweep_cost(a, width, height, scale, (const uint16_t*)vs, (const uint16_t*)vn,
+                                   start ? (const uint16_t*)vc : nullptr, n, ((n + align - 1) / align) * align,
+                                   (double)total / (double)cells, (const uint8_t*)sk, (uint8_t*)dense, cs, s);
     if (e == hipSuccess && total)
         e = lux::launch_pack_u8((const uint8_t*)dense, (const uint64_t*)vo, (const uint16_t*)vc, (uint8_t*)packed, cs, cells, s);
     if (e == hipSuccess && total) e = hipMemcpyAsync(raw, packed, total, hipMemcpyDeviceToHost, s);
@@ -641,6 +792,8 @@ int lux_sgm_dense_layer(lux_sgm_ctx* c, const lux_sgm_sweep* sw, const lux_sgm_d
         rs.maxcap = std::max(rs.maxcap, cp);
         rs.maxcount = std::max(rs.maxcount, cn);
         rs.maxend = std::max(rs.maxend, st + cp);
+        rs.cap_computed += (uint64_t)cp;
+        rs.computed += (skip && skip[i]) ? 0u : 1u;
     }
     (void)hipSetDevice(c->device);
     hipStream_t s = c->stream;
@@ -719,13 +872,15 @@ void lux_sgm_dense_pyramid_defaults(lux_sgm_dense_pyramid_params* p)
     p->budget = 0x40000000ull;
 }
 
-int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_t n_images,
-                          const uint8_t* const* images, const int32_t* widths, const int32_t* heights, const uint8_t* gray,
-                          const float* warps, const float* planes8, int32_t n8, const float* planes2, int32_t n2,
-                          float* const* depths, int32_t* aligns)
+}  // extern "C"
+
+namespace {
+
+// DenseStereoPyramid.Run; `device`: images and depths are device pointers, else host pointers.
+int dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_t n_images, const uint8_t* const* images,
+                  const int32_t* widths, const int32_t* heights, const uint8_t* gray, const float* warps, const float* planes8,
+                  int32_t n8, const float* planes2, int32_t n2, float* const* depths, int32_t* aligns, bool device)
 {
-    if (!c) return LUX_SGM_EINVAL;
-    std::lock_guard<std::mutex> lk(c->mu);
     if (!pp || pp->levels < 1 || pp->levels > 8) return fail(c, LUX_SGM_EINVAL, "1 to 8 levels");
     if (n_images < 2 || n_images > lux::kMaxViews + 1 || !images || !widths || !heights || !warps)
         return fail(c, LUX_SGM_EINVAL, "a reference and 1 to 16 other cameras");
@@ -753,8 +908,13 @@ int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp
     for (int k = 0; k < n_images; k++) {
         const size_t bytes = (size_t)widths[k] * heights[k] * 4;
         if ((rc = ws.get(names[k], bytes, &filt[k], &c->err))) return rc;
-        hipError_t e = hipMemcpyAsync(src, images[k], bytes, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = lux::launch_box3(src, false, widths[k], heights[k], widths[k], 4, 255, filt[k], s);
+        hipError_t e = hipSuccess;
+        const void* in = images[k];
+        if (!device) {
+            e = hipMemcpyAsync(src, images[k], bytes, hipMemcpyHostToDevice, s);
+            in = src;
+        }
+        if (e == hipSuccess) e = lux::launch_box3(in, false, widths[k], heights[k], widths[k], 4, 255, filt[k], s);
         if (e != hipSuccess) return hip_fail(c, e, "box filter");
         imgs += (uint64_t)heights[k] * widths[k] * 4;
     }
@@ -815,15 +975,21 @@ int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp
         const int W = p.width, H = p.height;
         const size_t cells = (size_t)W * H;
         if (W > 10240) return fail(c, LUX_SGM_EUNSUPPORTED, "width above 10240");
-        std::vector<uint8_t> skip(cells);
-        if ((rc = lux_sgm_skip_mask(pp->skip_mode[i], W, H, skip.data()))) return fail(c, rc, "skip mask");
+        // the skip mask of a mode and size never changes: made once per context and kept on the device
+        const std::string skname = "py_skip" + std::to_string(pp->skip_mode[i]) + "_" + std::to_string(W) + "x" +
+                                   std::to_string(H);
         void *dsk, *dd, *dw, *ds8, *dc8, *dcap8;
-        if ((rc = ws.get(nm("skip").c_str(), cells, &dsk, &c->err)) || (rc = ws.get(nm("depth").c_str(), cells * 4, &dd, &c->err)) ||
+        if ((rc = ws.get(skname.c_str(), cells, &dsk, &c->err)) || (rc = ws.get(nm("depth").c_str(), cells * 4, &dd, &c->err)) ||
             (rc = ws.get("py_wta", cells * 2, &dw, &c->err)) || (rc = ws.get("py_start8", cells * 2, &ds8, &c->err)) ||
             (rc = ws.get("py_count8", cells * 2, &dc8, &c->err)) || (rc = ws.get("py_cap8", cells * 2, &dcap8, &c->err)))
             return rc;
-        e = hipMemcpyAsync(dsk, skip.data(), cells, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return hip_fail(c, e, "upload skip mask");
+        if (!ws.filled.count(skname)) {
+            std::vector<uint8_t> skip(cells);
+            if ((rc = lux_sgm_skip_mask(pp->skip_mode[i], W, H, skip.data()))) return fail(c, rc, "skip mask");
+            e = hipMemcpy(dsk, skip.data(), cells, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(c, e, "upload skip mask");
+            ws.filled.insert(skname);
+        }
         lux::RangeStats rs;
         const lux::SweepDev* a = &a8;
         const uint16_t *d_start = nullptr, *d_count = nullptr, *d_cap = nullptr;
@@ -872,7 +1038,7 @@ int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp
         }
         if (aligns) aligns[i] = align;
         if (depths && depths[i]) {
-            e = hipMemcpyAsync(depths[i], dd, cells * 4, hipMemcpyDeviceToHost, s);
+            e = hipMemcpyAsync(depths[i], dd, cells * 4, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s);
             if (e != hipSuccess) return hip_fail(c, e, "download depth");
         }
         e = hipStreamSynchronize(s);
@@ -883,6 +1049,32 @@ int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp
         ph = H;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lux_sgm_dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_t n_images,
+                          const uint8_t* const* images, const int32_t* widths, const int32_t* heights, const uint8_t* gray,
+                          const float* warps, const float* planes8, int32_t n8, const float* planes2, int32_t n2,
+                          float* const* depths, int32_t* aligns)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return dense_pyramid(c, pp, n_images, images, widths, heights, gray, warps, planes8, n8, planes2, n2, depths, aligns,
+                         false);
+}
+
+int lux_sgm_dense_pyramid_device(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_t n_images,
+                                 const uint8_t* const* d_images, const int32_t* widths, const int32_t* heights,
+                                 const uint8_t* gray, const float* warps, const float* planes8, int32_t n8,
+                                 const float* planes2, int32_t n2, float* const* d_depths, int32_t* aligns)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return dense_pyramid(c, pp, n_images, d_images, widths, heights, gray, warps, planes8, n8, planes2, n2, d_depths, aligns,
+                         true);
 }
 
 }  // extern "C"

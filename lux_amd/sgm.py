@@ -23,7 +23,7 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_skip_mask", "lux_sgm_mt19937_draw",
            "lux_sgm_box3", "lux_sgm_box3_device", "lux_sgm_upsample_layer", "lux_sgm_upsample_layer_device",
            "lux_sgm_sweep_cost", "lux_sgm_range_from_previous", "lux_sgm_dense_layer", "lux_sgm_dense_pyramid",
-           "lux_sgm_dense_pyramid_defaults",
+           "lux_sgm_dense_pyramid_defaults", "lux_sgm_dense_pyramid_device",
            "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
@@ -120,6 +120,7 @@ def lib():
         L.lux_sgm_range_from_previous.argtypes = [vp, vp, i32, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp]
         L.lux_sgm_dense_layer.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_dense_pyramid.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp]
+        L.lux_sgm_dense_pyramid_device.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp]
         L.lux_sgm_dense_pyramid_defaults.argtypes = [vp]
         L.lux_sgm_dense_pyramid_defaults.restype = None
         L.lux_sgm_upsample_layer.argtypes = [vp, vp, vp, i32, i32, vp, i32, i32, i32, i32, ctypes.c_float, vp, vp]
@@ -691,3 +692,21 @@ def dense_pyramid(ctx, images, gray, warps, planes8, planes2=None, params=None):
     _check(lib().lux_sgm_dense_pyramid(ctx.handle, ctypes.byref(p), n, ptrs, ws, hs, _p(g), _p(wr), _p(p8), len(p8), _p(p2),
                                        0 if p2 is None else len(p2), dptr, aligns), ctx.handle)
     return depths, list(aligns)
+
+
+def dense_pyramid_device(ctx, d_images, shapes, gray, warps, planes8, planes2, d_depths, params=None):
+    """Device-pointer form: d_images / d_depths are device addresses (ints), shapes [(h, w)] per image."""
+    p = params or dense_pyramid_defaults()
+    n = len(d_images)
+    ptrs = (ctypes.c_void_p * n)(*d_images)
+    ws = (ctypes.c_int32 * n)(*[w for _, w in shapes])
+    hs = (ctypes.c_int32 * n)(*[h for h, _ in shapes])
+    g = None if gray is None else np.ascontiguousarray(np.asarray(gray, np.uint8))
+    wr = np.ascontiguousarray(np.asarray(warps, np.float32).reshape(n - 1, 16))
+    p8 = np.ascontiguousarray(planes8, np.float32)
+    p2 = None if planes2 is None else np.ascontiguousarray(planes2, np.float32)
+    dptr = (ctypes.c_void_p * p.levels)(*d_depths)
+    aligns = (ctypes.c_int32 * p.levels)()
+    _check(lib().lux_sgm_dense_pyramid_device(ctx.handle, ctypes.byref(p), n, ptrs, ws, hs, _p(g), _p(wr), _p(p8), len(p8),
+                                              _p(p2), 0 if p2 is None else len(p2), dptr, aligns), ctx.handle)
+    return list(aligns)

@@ -368,13 +368,13 @@ def layer_depth(planes, W, H, align, start, count, agg):
 
 
 def dense_layer(sc, W, H, scale, tile, align, start=None, count=None, skip=None, p1=1, p2_scale=500.0, gw=(0, 0, 0, 0),
-                keep_pass1=False):
+                keep_pass1=False, threads=8):
     """DenseLayer.Compute from the images: plane-sweep cost, adaptive eight-path aggregation over the subsampled
     reference as guidance, WTA, depth. Returns dict(depth, wta, agg, agg_pass1, raw)."""
     n = len(sc.planes)
     st = np.zeros((H, W), np.uint16) if start is None else np.asarray(start, np.uint16).reshape(H, W)
     cn = np.full((H, W), n, np.uint16) if count is None else np.asarray(count, np.uint16).reshape(H, W)
-    raw = layer_sweep_cost(sc, W, H, scale, align, st, cn, skip)
+    raw = layer_sweep_cost(sc, W, H, scale, align, st, cn, skip, threads=threads)
     g = guidance(sc.ref, W, H, scale)
     agg, agg1, wta = layer(raw, W, H, st, cn, T=tile, align=align, p1=p1, skip=skip, guidance=g, gw=gw, p2_scale=p2_scale,
                            adaptive=True, keep_pass1=keep_pass1)
@@ -412,7 +412,8 @@ L16 = dict(scales=(32, 16, 8, 4, 2, 1), tiles=(4, 8, 16, 32, 64, 64), skip_modes
 
 
 def dense_pyramid(images, gray, warps, planes8, planes2=None, scales=L16["scales"], tiles=L16["tiles"],
-                  skip_modes=L16["skip_modes"], p1=1, p2_scale=500.0, gw=(0, 0, 0, 0), margin=1, budget=1 << 30):
+                  skip_modes=L16["skip_modes"], p1=1, p2_scale=500.0, gw=(0, 0, 0, 0), margin=1, budget=1 << 30,
+                  threads=8):
     """DenseStereoPyramid.Run (DenseStereoSgm.cs:331-355) composed from the oracle's parts. Returns a list of per-level
     dicts (depth, align, start, count, skip, minlo, maxhi)."""
     filtered = [box3_rgba(np.ascontiguousarray(im, np.uint8)) for im in images]
@@ -437,7 +438,7 @@ def dense_pyramid(images, gray, warps, planes8, planes2=None, scales=L16["scales
                     raise RuntimeError("memory budget exceeded")
                 align, planes = 2, np.asarray(planes2, np.float32)
         sc = SweepContext(filtered[0], filtered[1:], warps, wqs, planes)
-        r = dense_layer(sc, W, H, sc_, tile, align, start, count, skip, p1=p1, p2_scale=p2_scale, gw=gw)
+        r = dense_layer(sc, W, H, sc_, tile, align, start, count, skip, p1=p1, p2_scale=p2_scale, gw=gw, threads=threads)
         lvl = dict(depth=r["depth"], align=align, start=start, count=count, skip=skip, minlo=minlo, maxhi=maxhi, W=W, H=H)
         out.append(lvl)
         prev = lvl

@@ -469,11 +469,19 @@ def test_dense_pyramid_matches_oracle(W0, H0, seed):
     want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
     for lv in want[1:]:   # the scene keeps every range clear of the unsupported padding case
         assert lv["minlo"] >= 7
+    import torch
+
     with sgm.Context(0) as c:
         depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2)
-    assert aligns == [lv["align"] for lv in want]
-    for i, (d, lv) in enumerate(zip(depths, want)):
+        d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+        d_depths = [torch.full(d.shape, -1.0, dtype=torch.float32, device="cuda") for d in depths]
+        aligns_d = sgm.dense_pyramid_device(c, [t.data_ptr() for t in d_imgs], [im.shape[:2] for im in imgs], gray, warps,
+                                            p8, p2, [t.data_ptr() for t in d_depths])
+        torch.cuda.synchronize()
+    assert aligns == [lv["align"] for lv in want] == aligns_d
+    for i, (d, dd, lv) in enumerate(zip(depths, d_depths, want)):
         np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+        np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
 
 
 def test_dense_pyramid_memory_modes():
