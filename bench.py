@@ -383,6 +383,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--profile-run", action="store_true",
                     help="for counter runs: a few requests, then a 1 GiB device copy as the byte-count calibration")
+    ap.add_argument("--profile-l16", action="store_true",
+                    help="for trace runs: --steps L16 pyramids (device form, images in HBM) and nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is a design note.",
@@ -396,6 +398,23 @@ def main():
     import torch
 
     from lux_amd import scene, sgm
+
+    if args.profile_l16:
+        imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
+        p8, p2 = scene.plane_list(500.0, 20000.0, L16_N8), scene.plane_list(500.0, 20000.0, L16_N2)
+        ctx = sgm.Context(0)
+        d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+        d_depths = [torch.empty((-(-L16_H // sc), -(-L16_W // sc)), dtype=torch.float32, device="cuda")
+                    for sc in sgm.L16_SCALES]
+        a = ([t.data_ptr() for t in d_imgs], [im.shape[:2] for im in imgs], gray, warps, p8, p2,
+             [t.data_ptr() for t in d_depths])
+        for _ in range(args.warmup + args.steps):
+            t = time.perf_counter()
+            sgm.dense_pyramid_device(ctx, *a)
+            print(f"l16 pyramid {(time.perf_counter() - t) * 1e3:.3f} ms", file=sys.stderr, flush=True)
+        ctx.close()
+        print(json.dumps({"profile_l16": True, "pyramids": args.warmup + args.steps}))
+        return
 
     left, right, truth = scene.make_pair(W, H, D - 1, planes=6, sigma=4.0, seed=1, bits=BITS)
     digest = scene.sha256(left, right)

@@ -5,8 +5,12 @@
  *
  * Reference files restated (paths relative to the mprovenz/lux checkout):
  *   Lux.Engine/Pipeline/Registration/DenseStereoSgm.cs   DenseLayer (Init ranges, line buffers, Compute, Pixel, Rd, WTA)
- *   Lux.Engine/Pipeline/Registration/DenseStereo.cs      BoxFilter3 (:93-159), out-of-range plane rule (:273-274),
- *                                                        padding lanes = 255 (:376-381)
+ *   Lux.Engine/Pipeline/Registration/DenseStereo.cs      BoxFilter3 (:93-159), Guidance (:162-175), MatchingCost and
+ *                                                        its scalar form (:217-373), Normalise (:376-381)
+ *   Lux.Engine/Pipeline/Registration/DenseStereoSgm.cs   also SkipMask (:64-87), RangeFromPrevious (:91-140),
+ *                                                        the layer's depth (:225-231)
+ *   Lux.Engine/Pipeline/Registration/DenseUpsampleLayer.cs  ImageShift, NearestResize, BilateralUpsampleFromCollapse
+ *   Lux.Engine/Pipeline/Registration/IntelApprox.cs      rcpss (through the table read from the instruction)
  *   Lux.Engine/Pipeline/Parallax/RealStereo.cs:229-233,418-421  two-direction check pattern (L/R check)
  *   Lux.Engine/Pipeline/DepthRefine/CrossViewCost.cs:197-202    parabola vertex (subpixel)
  * Census and the truncated-AD cost of a rectified pair are project-defined front ends (the reference
