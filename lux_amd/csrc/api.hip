@@ -83,16 +83,7 @@ std::vector<int> assign_branches(int n, const lux_sgm_pair* pairs, int branches)
     return br;
 }
 
-int ensure_batch_streams(lux_sgm_ctx* c)
-{
-    if (c->fork) return 0;
-    hipError_t e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
-    for (int k = 0; e == hipSuccess && k < kMaxBranches; k++) {
-        e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
-    }
-    return e == hipSuccess ? 0 : hip_fail(c, e, "batch streams");
-}
+int ensure_batch_streams(lux_sgm_ctx* c) { return lux::ensure_side_streams(c); }
 
 // Prepare every pair's workspace (allocation and plan upload happen here, outside any capture).
 int prepare_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<lux::Prepared>* prep)
@@ -457,8 +448,9 @@ int lux_sgm_pyramid_device(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, cons
     std::lock_guard<std::mutex> lk(c->mu);
     if (!d_left || !d_right) return fail(c, LUX_SGM_EINVAL, "null image");
     (void)hipSetDevice(c->device);
+    const lux::Conc conc = lux::conc_of(c);
     return lux::enqueue_pyramid(c->ws, p, d_left, d_right, d_disp, d_sub, stream ? (hipStream_t)stream : c->stream,
-                                &c->err);
+                                &c->err, &conc);
 }
 
 int lux_sgm_pyramid(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void* left, const void* right, uint16_t* disp,
@@ -481,7 +473,8 @@ int lux_sgm_pyramid(lux_sgm_ctx* c, const lux_sgm_pyramid_params* p, const void*
     hipError_t e = hipMemcpyAsync(dl, left, imgb, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, right, imgb, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(c, e, "upload images");
-    if ((rc = lux::enqueue_pyramid(ws, p, dl, dr, (uint16_t*)dd, sub ? (float*)ds : nullptr, s, &c->err))) return rc;
+    const lux::Conc conc = lux::conc_of(c);
+    if ((rc = lux::enqueue_pyramid(ws, p, dl, dr, (uint16_t*)dd, sub ? (float*)ds : nullptr, s, &c->err, &conc))) return rc;
     if (disp) e = hipMemcpyAsync(disp, dd, px * 2, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && sub) e = hipMemcpyAsync(sub, ds, px * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -606,6 +599,8 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         w.b.ap.on = 1;
         w.b.seedguid = (uint32_t*)sg;
     }
+    const lux::Conc conc = lux::conc_of(c);
+    if (ranged) lux::try_concurrency(ws, &w, &conc);  // ranged layers are latency-bound: their sweeps run side by side
     lux::SgmOutputs o;
     o.disp = (uint16_t*)dw;
     o.tol = 0;

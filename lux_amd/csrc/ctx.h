@@ -46,6 +46,32 @@ struct lux_sgm_ctx {
 
 namespace lux {
 
+// The context's side streams and events, made on first use (default priority, non-blocking).
+inline int ensure_side_streams(lux_sgm_ctx* c)
+{
+    if (c->fork) return 0;
+    hipError_t e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    for (int k = 0; e == hipSuccess && k < kMaxBranches; k++) {
+        e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
+    }
+    return e == hipSuccess ? 0 : hip_status(e, "side streams", &c->err);
+}
+
+// The context's side streams and events as the concurrent sweeps take them (the batch form uses the same ones; calls
+// on a context are serialised by its mutex). Without side streams the sweeps stay one after another.
+inline Conc conc_of(lux_sgm_ctx* c)
+{
+    Conc k;
+    if (ensure_side_streams(c)) return k;
+    for (int i = 0; i < 3; i++) {
+        k.side[i] = c->side[i];
+        k.join[i] = c->join[i];
+    }
+    k.fork = c->fork;
+    return k;
+}
+
 inline int fail(lux_sgm_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;

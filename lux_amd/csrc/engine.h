@@ -61,6 +61,15 @@ struct Prepared {
     uint16_t* agg = nullptr;
 };
 
+// Side streams and events for the concurrent sweeps of ranged layers (SgmBuffers::aggp): the caller's, reused per call.
+struct Conc {
+    hipStream_t side[3] = {};
+    hipEvent_t fork = nullptr, join[3] = {};
+};
+// Gives w the three path-cost volumes and the streams when c is set and the volumes fit the budget; otherwise the
+// sweeps stay one after another. Returns whether it did.
+bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c);
+
 struct RectOut {
     uint16_t* disp = nullptr;
     float* sub = nullptr;
@@ -111,7 +120,7 @@ std::vector<PyrLevel> pyramid_levels(const lux_sgm_pyramid_params* p);
 int check_pyramid(const lux_sgm_pyramid_params* p, std::string* err);
 int pyramid_buffers(const lux_sgm_pyramid_params* p, size_t host_img_bytes, std::vector<BufferNeed>* out, std::string* err);
 int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* dl, const void* dr, uint16_t* disp,
-                    float* sub, hipStream_t s, std::string* err);
+                    float* sub, hipStream_t s, std::string* err, const Conc* conc = nullptr);
 
 // DenseUpsampleLayer (upsample.hip): prev depth (and confidence, may be null) pw x ph and RGBA8 guide W x H (device)
 // -> depth (and confidence when both confidence pointers are set) W x H.

@@ -388,7 +388,7 @@ int pyramid_buffers(const lux_sgm_pyramid_params* p, size_t host_img_bytes, std:
 }
 
 int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* dl, const void* dr, uint16_t* disp,
-                    float* sub, hipStream_t s, std::string* err)
+                    float* sub, hipStream_t s, std::string* err, const Conc* conc)
 {
     std::vector<BufferNeed> need;
     int rc = pyramid_buffers(p, 0, &need, err);
@@ -399,6 +399,19 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         void* q;
         if ((rc = ws.get(b.name.c_str(), b.bytes, &q, err))) return rc;
         ptr[b.name] = q;
+    }
+    if (conc && conc->side[0]) {  // the concurrent sweeps' volumes of the ranged levels, at the largest level's size
+        size_t mx = 0;
+        for (const PyrLevel& l : pyramid_levels(p))
+            if (l.radius) {
+                const lux_sgm_rect_params q = level_params(p, l);
+                const Geometry g = geometry(l.W, l.H, q.disparities, q.align);
+                mx = std::max(mx, (size_t)g.W * g.H * g.cs * 2);
+            }
+        std::string e2;
+        void* q;
+        for (const char* nm : {"aggp0", "aggp1", "aggp2"})
+            if (ws.get(nm, mx, &q, &e2)) conc = nullptr;  // over the budget: the sweeps stay one after another
     }
     const auto lv = pyramid_levels(p);
     const int L = p->levels;
@@ -442,6 +455,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         w.b.vcount = vn;
         w.b.seedstart = (uint16_t*)ptr["seedstart"];
         w.b.seedcap = (uint16_t*)ptr["seedcap"];
+        try_concurrency(ws, &w, conc);  // sized above: no reallocation under queued work
         SgmOutputs so;
         so.disp = ldisp ? ldisp : w.b.disp_tmp;
         so.sub = lsub;
@@ -466,6 +480,28 @@ int check_band_error(Workspace& ws, hipStream_t s, std::string* err)
         return fail(LUX_SGM_EDEVICE, "a band tile timed out waiting for its neighbours", err);
     }
     return 0;
+}
+
+}  // namespace lux
+
+namespace lux {
+
+bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
+{
+    if (!c || !c->side[0]) return false;
+    const size_t bytes = (size_t)w->g.W * w->g.H * w->g.cs * 2;
+    static const char* names[3] = {"aggp0", "aggp1", "aggp2"};
+    void* p[3];
+    std::string err;
+    for (int k = 0; k < 3; k++)
+        if (ws.get(names[k], bytes, &p[k], &err)) return false;
+    for (int k = 0; k < 3; k++) {
+        w->b.aggp[k] = (uint16_t*)p[k];
+        w->b.conc[k] = c->side[k];
+        w->b.conc_join[k] = c->join[k];
+    }
+    w->b.conc_fork = c->fork;
+    return true;
 }
 
 }  // namespace lux

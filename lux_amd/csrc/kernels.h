@@ -61,6 +61,11 @@ struct SgmBuffers {
     uint16_t* seedcap = nullptr;
     AdaptP2 ap;                     // adaptive P2 (ap.on = 0: every step uses the constant P2)
     uint32_t* seedguid = nullptr;   // [3][W] guidance of the cell behind each seed-table entry (adaptive P2)
+    // Concurrent paths (set all or none): the sweeps of a pass run on side streams, each writing its path cost
+    // into its own volume [H][W][cs], summed into agg once per pass. For latency-bound (ranged) layers.
+    uint16_t* aggp[3] = {};
+    hipStream_t conc[3] = {};
+    hipEvent_t conc_fork = nullptr, conc_join[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
 };
 

@@ -373,7 +373,17 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     int x, y, n;
     chain_start<DX, DY>(chain, W, H, x, y, n);
     const int bx = DIR > 0 ? W - 1 : 0;
-    if (P == 2 && x == bx) return;  // the border column's vertical chain is the replay's
+    if (P == 2 && x == bx) {  // the border column's vertical chain is the replay's
+        if (FIRST) {          // a path-cost volume of its own still needs its zeros there
+            const int lo = lane * 2 * NP;
+            Cell<NP> z;
+#pragma unroll
+            for (int k = 0; k < NP; k++) z.v[k] = 0;
+            for (int i = 0; i < n; i++)
+                store_u16<NP>(a.agg + ((size_t)(y + i * DY) * W + (x + i * DX)) * cs + lo, lo < cs, z);
+        }
+        return;
+    }
     Cell<NP> mask;
     bool active;
     make_mask<NP>(lane, a.cap, mask, active);
@@ -1415,6 +1425,13 @@ __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+// agg = sat(agg + a1 + a2 + a3) on u16 pairs: the path costs of concurrently run sweeps. Saturating addition of
+// non-negative values is min(sum, 65535) in any order, so this equals adding them one sweep after another.
+__global__ void agg_combine3(uint32_t* agg, const uint32_t* a1, const uint32_t* a2, const uint32_t* a3, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        agg[i] = padd(padd(agg[i], a1[i]), padd(a2[i], a3[i]));
+}
 
 // ------------------------------------------------------------------ census and cost volume
 template <typename T, int R>
@@ -1601,8 +1618,34 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if (b.ap.on) fill_u32<<<grid_for((size_t)3 * g.W, 256), 256, 0, s>>>(b.seedguid, (size_t)3 * g.W, 0u);
     }
     // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
+    // (not with the census cost fused into the row path: the sweeps read the costs it writes)
+    const bool conc = b.aggp[0] && b.conc[0] && !o.bands && !b.cl;
+    bool forked = false;  // pass +1 forks before its row path, so that runs beside the sweeps too
+    const size_t aggn = (size_t)g.W * g.H * g.cs / 2;  // u16 pairs of one volume (cs is a multiple of 4)
     auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
         constexpr int DIR = decltype(dirc)::value;
+        if (conc) {  // the three sweeps side by side, each into its own volume, then one sum into agg
+            if (!forked) (void)hipEventRecord(b.conc_fork, s);
+            forked = false;
+            SweepArgs ak[3] = {a, a, a};
+            for (int k = 0; k < 3; k++) {
+                ak[k].agg = b.aggp[k];
+                (void)hipStreamWaitEvent(b.conc[k], b.conc_fork, 0);
+            }
+            sgm_sweep<NP, DIR, 1, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
+            sgm_sweep<NP, DIR, 2, true, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
+            sgm_sweep<NP, DIR, 3, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
+            for (int k = 0; k < 3; k++) {
+                (void)hipEventRecord(b.conc_join[k], b.conc[k]);
+                (void)hipStreamWaitEvent(s, b.conc_join[k], 0);
+            }
+            agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
+                                                           (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
+            mark();
+            mark();
+            mark();
+            return;
+        }
         if (NP > 2 || VAR || ADP || !o.bands) {  // the fused bands take neither ranges nor adaptive P2
             sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, s>>>(a);
             mark();
@@ -1645,6 +1688,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
+    if (conc) {
+        (void)hipEventRecord(b.conc_fork, s);
+        forked = true;
+    }
     if constexpr (VAR) {
         sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     } else {

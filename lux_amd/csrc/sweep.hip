@@ -582,7 +582,7 @@ int ranges_from_previous(Workspace& ws, const float* planes, int n, const float*
 int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_params& p, const uint16_t* d_start,
                        const uint16_t* d_count, const uint16_t* d_cap, const RangeStats& rs, const uint8_t* d_skip,
                        bool keep_agg, uint16_t* agg_pass1, uint16_t* d_wta, float* d_depth, Prepared* w, hipStream_t s,
-                       std::string* err)
+                       std::string* err, const Conc* conc = nullptr)
 {
     const int W = p.width, H = p.height;
     const bool ranged = d_start != nullptr;
@@ -631,6 +631,7 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
     w->b.ap.base = (float)p.p1 * p.p2_scale;  // p2Base (:253), one float multiply
     w->b.ap.on = 1;
     w->b.seedguid = (uint32_t*)sg;
+    if (ranged) try_concurrency(ws, w, conc);  // ranged layers are latency-bound: their sweeps run side by side
     SgmOutputs o;
     o.disp = d_wta;
     o.tol = 0;
@@ -819,9 +820,10 @@ int lux_sgm_dense_layer(lux_sgm_ctx* c, const lux_sgm_sweep* sw, const lux_sgm_d
     const lux::Geometry g = lux::geometry(W, H, start ? rs.maxcount : n, align);
     if (agg_pass1 && (rc = ws.get("layer_pass1", cells * g.cs * 2, &a1, &c->err))) return rc;
     lux::Prepared w;
+    const lux::Conc conc = lux::conc_of(c);
     if ((rc = lux::dense_layer_device(ws, a, *p, (const uint16_t*)vs, (const uint16_t*)vn, (const uint16_t*)vc, rs,
                                       (const uint8_t*)sk, agg != nullptr, (uint16_t*)a1, (uint16_t*)dw, (float*)dd, &w, s,
-                                      &c->err)))
+                                      &c->err, &conc)))
         return rc;
     e = hipMemcpyAsync(depth, dd, cells * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && wta) e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
@@ -1030,8 +1032,9 @@ int dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_
         }
         p.align = align;
         lux::Prepared w;
+        const lux::Conc conc = lux::conc_of(c);
         rc = lux::dense_layer_device(ws, *a, p, d_start, d_count, d_cap, rs, (const uint8_t*)dsk, false, nullptr,
-                                     (uint16_t*)dw, (float*)dd, &w, s, &c->err);
+                                     (uint16_t*)dw, (float*)dd, &w, s, &c->err, &conc);
         if (rc) {
             c->err = "level " + std::to_string(i) + ": " + c->err;
             return rc;
