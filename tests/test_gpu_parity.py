@@ -506,3 +506,37 @@ def test_dense_pyramid_memory_modes():
         p.budget = 1
         with pytest.raises(sgm.NotSupported):
             sgm.dense_pyramid(c, imgs, gray, warps, p8, p2, params=p)
+
+
+def test_plane_sweep_entry_points_refuse_bad_input():
+    """Argument errors come back as EINVAL before any device work; the shapes the device does not model as
+    EUNSUPPORTED (the reference raises or reads past its buffers there)."""
+    ref, others, warps, wqs, pl = _sweep_case(9, 40, 48, 2, 16)
+    with sgm.Context(0) as c:
+        cc = sgm.CostContext(ref, others, warps, wqs, pl)
+        with pytest.raises(sgm.SgmError) as e:   # a range past the last plane
+            sgm.sweep_cost(c, cc, 12, 10, 4, 8, np.full((10, 12), 10), np.full((10, 12), 7))
+        assert e.value.code == sgm.EINVAL
+        tiny = sgm.CostContext(ref, [others[0][:3, :3]], warps[:1], wqs[:1], pl)
+        with pytest.raises(sgm.SgmError) as e:   # a camera image below 4 x 4
+            sgm.sweep_cost(c, tiny, 12, 10, 4)
+        assert e.value.code == sgm.EINVAL
+        many = sgm.CostContext(ref, [others[0]] * 17, [warps[0]] * 17, [wqs[0]] * 17, pl)
+        with pytest.raises(sgm.SgmError) as e:   # more than 16 cameras
+            sgm.sweep_cost(c, many, 12, 10, 4)
+        assert e.value.code == sgm.EINVAL
+        p = sgm.dense_params(48, 40, 4, 4)
+        p.width += 1
+        with pytest.raises(sgm.SgmError) as e:   # a layer size that is not (ref + scale - 1) / scale
+            sgm.dense_layer(c, cc, p)
+        assert e.value.code == sgm.EINVAL
+        big = sgm.CostContext(ref, others, warps, wqs, sweep_planes(1030))
+        with pytest.raises(sgm.NotSupported):   # layer 0 above 1024 planes
+            sgm.dense_layer(c, big, sgm.dense_params(48, 40, 4, 4))
+        # padding that reaches the end of the line-buffer segment: MinLo 0, a range ending at MaxHi with padding
+        p = sgm.dense_params(48, 40, 4, 4)
+        start = np.zeros((10, 12), np.uint16)
+        count = np.full((10, 12), 3, np.uint16)
+        start[5, 5], count[5, 5] = 13, 3    # [13, 16) with cap 8: 13 - 0 + 8 > roundup(16, 8)
+        with pytest.raises(sgm.NotSupported):
+            sgm.dense_layer(c, cc, p, start, count)
