@@ -63,6 +63,7 @@ struct SgmBuffers {
     uint32_t* seedguid = nullptr;   // [3][W] guidance of the cell behind each seed-table entry (adaptive P2)
     // Concurrent paths (set all or none): the sweeps of a pass run on side streams, each writing its path cost
     // into its own volume [H][W][cs], summed into agg once per pass. For latency-bound (ranged) layers.
+    int seg = 1;                    // narrow per-pixel ranges (caps <= 32) may run several chains per wavefront
     uint16_t* aggp[3] = {};
     hipStream_t conc[3] = {};
     hipEvent_t conc_fork = nullptr, conc_join[3] = {};

@@ -845,6 +845,327 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
         }
 }
 
+// ------------------------------------------------------------------ segmented chains (narrow per-pixel ranges)
+// With per-pixel ranges whose caps are at most 2*SEG planes (SEG = 8 or 16), one wavefront runs 64/SEG chains, SEG
+// lanes each (one packed plane pair per lane, as the VAR kernels above). Everything the unsegmented kernels keep
+// wave-uniform (the cell's range, minPrev, the chain position) is per segment here; the lane shifts of the path step
+// and its minimum stay inside a segment (DPP row shifts and mirrors: a segment never straddles a row of 16 lanes),
+// and realignment and plane reads use ds_bpermute inside the segment. The data layout is the VAR one, so the border
+// replays and the column-0 finish are shared.
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_shr1(uint32_t v, uint32_t edge, int li)  // lane i <- lane i-1; first: edge
+{
+    const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    return li == 0 ? edge : t;
+}
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_shl1(uint32_t v, uint32_t edge, int li)  // lane i <- lane i+1; last: edge
+{
+    const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, 0x101, 0xf, 0xf, false);  // row_shl:1
+    return li == SEG - 1 ? edge : t;
+}
+// Minimum over the segment, in every lane of it: quads, then half-rows (8 lanes), then rows (16 lanes).
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_min(uint32_t v)
+{
+    static_assert(SEG == 8 || SEG == 16, "segments of 8 or 16 lanes");
+    uint32_t o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    v = v < o ? v : o;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    v = v < o ? v : o;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    v = v < o ? v : o;
+    if constexpr (SEG == 16) {
+        o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false);  // row_mirror
+        v = v < o ? v : o;
+    }
+    return v;
+}
+// Lane `src` (0 .. SEG-1) of this lane's segment.
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_get(uint32_t v, int src, int lane)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(((lane & ~(SEG - 1)) + (src & (SEG - 1))) << 2, (int)v);
+}
+// Plane r of the predecessor (start from.s, cap cp) in the successor's frame: 2000 outside [0, cp).
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_rel_plane(uint32_t v, int r, int cp, int lane)
+{
+    const uint32_t w = seg_get<SEG>(v, r >> 1, lane);
+    const uint32_t p = (r & 1) ? (w >> 16) : (w & 0xffffu);
+    return (r >= 0 && r < cp) ? p : 2000u;
+}
+// realign() inside a segment, for every segment at once (delta may differ per segment; 0 is the identity).
+template <int SEG>
+__device__ __forceinline__ void seg_reframe(uint32_t& st, Range& from, int s, int c, int li, int lane, uint32_t& edge_l,
+                                            uint32_t& edge_r)
+{
+    const int delta = s - from.s, cp = from.c;
+    const uint32_t v = st;
+    const uint32_t even = seg_get<SEG>(v, li + (delta >> 1), lane);
+    const int sa = li + ((delta - 1) >> 1);
+    const uint32_t a = seg_get<SEG>(v, sa, lane), b = seg_get<SEG>(v, sa + 1, lane);
+    const uint32_t out = (delta & 1) ? __builtin_amdgcn_alignbit(b, a, 16) : even;
+    const int r0 = 2 * li + delta;
+    const bool ok0 = r0 >= 0 && r0 < cp, ok1 = r0 + 1 >= 0 && r0 + 1 < cp;
+    st = (ok0 ? (out & 0xffffu) : 2000u) | (ok1 ? (out & 0xffff0000u) : (2000u << 16));
+    const uint32_t el = seg_rel_plane<SEG>(v, delta - 1, cp, lane), er = seg_rel_plane<SEG>(v, 2 * SEG + delta, cp, lane);
+    edge_l = el | (el << 16);
+    edge_r = er | (er << 16);
+    from.s = s;
+    from.c = c;
+}
+// path_step<1, MASKED> inside a segment; returns the segment's new minPrev (in every lane of it).
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_path_step(uint32_t prev, uint32_t minprev, uint32_t raw, uint32_t p1x2, uint32_t p2,
+                                                  uint32_t mask, uint32_t& lc, uint32_t edge_l, uint32_t edge_r, int li)
+{
+    uint32_t p2p = p2 + minprev;
+    p2p = p2p > 65535u ? 65535u : p2p;
+    const uint32_t p2x2 = p2p | (p2p << 16), minx2 = minprev | (minprev << 16);
+    const uint32_t left = seg_shr1<SEG>(prev, edge_l, li), right = seg_shl1<SEG>(prev, edge_r, li);
+    const uint32_t lm = __builtin_amdgcn_alignbit(prev, left, 16), lp = __builtin_amdgcn_alignbit(right, prev, 16);
+    uint32_t m = pmin(padd(pmin(lm, lp), p1x2), prev);
+    m = pmin(m, p2x2);
+    uint32_t t = psub(padd(raw, m), minx2);
+    t = (t & mask) | (kInit2 & ~mask);
+    const uint32_t run = t | ~mask;
+    lc = t;
+    const uint32_t lo = run & 0xffffu, hi = run >> 16;
+    return seg_min<SEG>(lo < hi ? lo : hi);
+}
+__device__ __forceinline__ uint32_t seg_mask(int li, int cap)
+{
+    return (2 * li < cap ? 0xffffu : 0u) | (2 * li + 1 < cap ? 0xffff0000u : 0u);
+}
+
+// sgm_sweep<1, DIR, P, FIRST, true, true, false> with 64/SEG chains per wavefront.
+template <int SEG, int DIR, int P, bool FIRST>
+__global__ __launch_bounds__(kBlock) void sgm_sweep_seg(SweepArgs a)
+{
+    constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
+    constexpr int DY = DIR * (P == 0 ? 0 : 1);
+    constexpr int PF = 8;
+    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
+    const int chain = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    const int W = a.W, H = a.H, cs = a.cs;
+    const int nch = chain_count<DX, DY>(W, H);
+    int x = 0, y = 0, n = 0;
+    if (chain < nch) chain_start<DX, DY>(chain, W, H, x, y, n);
+    int nmax = n;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    if (nmax == 0) return;
+    const int bx = DIR > 0 ? W - 1 : 0;
+    const bool replays = P == 2 && x == bx;  // the border column's vertical chain is the replay's
+    const int lo = li * 2;
+    const int lol = lo < cs ? lo : 0;
+    uint32_t state = kInit2, minprev = 2000;
+    Range from;
+    if (DIR < 0 && P != 0 && n > 0 && y == H - 1) {  // pass -1's first row reads half 0 as pass +1 left it
+        const int sc = x - DX;
+        if (sc >= 0 && sc < W) {
+            state = lo < cs ? *(const uint32_t*)(a.seeds + ((size_t)(P - 1) * W + sc) * cs + lo) : 0u;
+            from.s = a.seedstart[(P - 1) * W + sc];
+            from.c = a.seedcap[(P - 1) * W + sc];
+            const uint32_t m = seg_mask(li, from.c);
+            state = (state & m) | (kInit2 & ~m);
+            minprev = a.seedmins[(P - 1) * W + sc];
+        }
+    }
+    const ptrdiff_t step = (ptrdiff_t)DY * W + DX;
+    const size_t cell0 = n > 0 ? (size_t)y * W + x : 0;
+    const int nl = n > 0 ? n - 1 : 0;
+    uint32_t rb[PF], ab[PF], rs[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const size_t cj = cell0 + (ptrdiff_t)min(j, nl) * step;
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + cj * cs + lol));
+        ab[j] = FIRST ? 0u : *(const uint32_t*)(a.agg + cj * cs + lol);
+        rs[j] = (uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16);
+    }
+    size_t cell = cell0;
+    auto body = [&](int i, int j) {
+        const bool live = i < n;
+        const uint32_t raw = __builtin_amdgcn_perm(0u, rb[j], 0x0c010c00u);
+        const uint32_t agg = ab[j];
+        const int cs_ = (int)(rs[j] & 0xffffu), cc_ = (int)(rs[j] >> 16);
+        uint32_t el, er;
+        seg_reframe<SEG>(state, from, cs_, cc_, li, lane, el, er);
+        const uint32_t mask = seg_mask(li, cc_);
+        const bool active = 2 * li < cc_;
+        const size_t cp = cell0 + (ptrdiff_t)min(i + PF, nl) * step;
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + cp * cs + lol));
+        if (!FIRST) ab[j] = *(const uint32_t*)(a.agg + cp * cs + lol);
+        rs[j] = (uint32_t)a.vstart[cp] | ((uint32_t)a.vcap[cp] << 16);
+        uint32_t lc;
+        minprev = seg_path_step<SEG>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er, li);
+        if (live && !replays) {
+            if (x != bx) {
+                if (active) *(uint32_t*)(a.agg + cell * cs + lo) = FIRST ? lc : padd(agg, lc);
+                if (DIR > 0 && P != 0 && y == a.seed_row) {
+                    if (lo < cs) *(uint32_t*)(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo) = lc;
+                    if (li == 0) {
+                        a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
+                        a.seedstart[(P - 1) * W + x] = (uint16_t)cs_;
+                        a.seedcap[(P - 1) * W + x] = (uint16_t)cc_;
+                    }
+                }
+            } else if (FIRST && active) {
+                *(uint32_t*)(a.agg + cell * cs + lo) = 0u;
+            }
+            if (P <= 1 && x == bx - DIR) {  // the border replay reads this column's path 0 / 1 states
+                if (lo < cs) *(uint32_t*)(a.cols + ((size_t)P * H + y) * cs + lo) = lc;
+                if (li == 0) a.colmins[P * H + y] = (uint16_t)minprev;
+            }
+        } else if (live && FIRST && lo < cs) {  // the replay's chain: its own volume still needs the zeros
+            *(uint32_t*)(a.agg + cell * cs + lo) = 0u;
+        }
+        state = lc;
+        cell += step;
+        x += DX;
+        y += DY;
+    };
+    int i0 = 0;
+    for (; i0 + PF <= nmax; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(i0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < nmax) body(i0 + j, j);
+}
+
+// sgm_row_fwd<1, true, false, true, false> with 64/SEG rows per wavefront.
+template <int SEG>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd_seg(RowArgs a)
+{
+    constexpr int PF = 8;
+    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
+    const int yr = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    if (__builtin_amdgcn_readfirstlane(yr - lane / SEG) >= a.H) return;  // the wave's first row: whole waves only
+    const bool row_ok = yr < a.H;
+    const int y = row_ok ? yr : a.H - 1;  // idle segments repeat the last row and store nothing
+    const int W = a.W, cs = a.cs, lo = li * 2;
+    const int lol = lo < cs ? lo : 0;
+    uint32_t state = kInit2, minprev = 2000;
+    Range from;
+    const size_t rowcell = (size_t)y * W;
+    uint32_t rb[PF], rs[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int xj = min(j, W - 1);
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + (rowcell + xj) * cs + lol));
+        rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
+    }
+    auto body = [&](int x, int j) {
+        const int xp = min(x + PF, W - 1);
+        const int s_ = (int)(rs[j] & 0xffffu), c_ = (int)(rs[j] >> 16);
+        uint32_t el, er;
+        seg_reframe<SEG>(state, from, s_, c_, li, lane, el, er);
+        const uint32_t mask = seg_mask(li, c_);
+        const bool active = 2 * li < c_;
+        const uint32_t raw = __builtin_amdgcn_perm(0u, rb[j], 0x0c010c00u);
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + (rowcell + xp) * cs + lol));
+        rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+        uint32_t lc;
+        minprev = seg_path_step<SEG>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er, li);
+        if (row_ok) {
+            if (active) *(uint32_t*)(a.agg + (rowcell + x) * cs + lo) = x != W - 1 ? lc : 0u;
+            if (x == W - 2) {
+                if (lo < cs) *(uint32_t*)(a.cols + (size_t)y * cs + lo) = lc;
+                if (li == 0) a.colmins[y] = (uint16_t)minprev;
+            }
+        }
+        state = lc;
+    };
+    int x0 = 0;
+    for (; x0 + PF <= W; x0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(x0 + j, j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (x0 + j < W) body(x0 + j, j);
+}
+
+// sgm_row_bwd_wta<1, true, true, false> with 64/SEG rows per wavefront: the last contribution and the
+// winner-take-all of every cell but column 0 (the first minimum over the cell's count, the parabola).
+template <int SEG>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_seg(RowArgs a)
+{
+    constexpr int PF = 8;
+    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
+    const int yr = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    if (__builtin_amdgcn_readfirstlane(yr - lane / SEG) >= a.H) return;  // the wave's first row: whole waves only
+    const bool row_ok = yr < a.H;
+    const int y = row_ok ? yr : a.H - 1;
+    const int W = a.W, cs = a.cs, lo = li * 2;
+    const int lol = lo < cs ? lo : 0;
+    uint32_t state = kInit2, minprev = 2000;
+    Range from;
+    const size_t rowcell = (size_t)y * W;
+    uint32_t rb[PF], ab[PF], rs[PF], rn[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int xj = max(W - 1 - j, 0);
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + (rowcell + xj) * cs + lol));
+        ab[j] = *(const uint32_t*)(a.agg + (rowcell + xj) * cs + lol);
+        rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
+        rn[j] = a.vcount[rowcell + xj];
+    }
+    auto body = [&](int x, int j) {
+        const int xp = max(x - PF, 0);
+        const int s0 = (int)(rs[j] & 0xffffu), c_ = (int)(rs[j] >> 16), D = (int)rn[j];
+        uint32_t el, er;
+        seg_reframe<SEG>(state, from, s0, c_, li, lane, el, er);
+        const uint32_t mask = seg_mask(li, c_);
+        const bool active = 2 * li < c_;
+        const uint32_t raw = __builtin_amdgcn_perm(0u, rb[j], 0x0c010c00u);
+        const uint32_t agg = ab[j];
+        rb[j] = (uint32_t)(*(const uint16_t*)(a.raw + (rowcell + xp) * cs + lol));
+        ab[j] = *(const uint32_t*)(a.agg + (rowcell + xp) * cs + lol);
+        rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
+        rn[j] = a.vcount[rowcell + xp];
+        uint32_t lc;
+        minprev = seg_path_step<SEG>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er, li);
+        const uint32_t fin = padd(agg, lc);
+        if (row_ok) {
+            if (a.write_agg && active) *(uint32_t*)(a.agg + (rowcell + x) * cs + lo) = fin;
+            if (x == 1) {
+                if (lo < cs) *(uint32_t*)(a.cols + (size_t)y * cs + lo) = lc;
+                if (li == 0) a.colmins[y] = (uint16_t)minprev;
+            }
+        }
+        state = lc;
+        const uint32_t k0 = lo < D ? (((fin & 0xffffu) << 16) | (uint32_t)lo) : 0xffffffffu;
+        const uint32_t k1 = lo + 1 < D ? ((fin & 0xffff0000u) | (uint32_t)(lo + 1)) : 0xffffffffu;
+        const uint32_t best = seg_min<SEG>(k0 < k1 ? k0 : k1);
+        const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+        float v = (float)(d + s0);
+        const uint32_t wl = seg_get<SEG>(fin, (d - 1) >> 1, lane), wr = seg_get<SEG>(fin, (d + 1) >> 1, lane);
+        if (d > 0 && d < D - 1) {
+            const int l = (int)(((d - 1) & 1) ? (wl >> 16) : (wl & 0xffffu)), cc = (int)(best >> 16);
+            const int r = (int)(((d + 1) & 1) ? (wr >> 16) : (wr & 0xffffu));
+            const int den = l - 2 * cc + r;
+            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
+        }
+        if (row_ok && li == 0) {
+            a.disp[rowcell + x] = (uint16_t)(d + s0);  // the plane index start + arg (:230)
+            if (a.sub) a.sub[rowcell + x] = v;
+        }
+    };
+    const int n = W - 1;  // x = W-1 .. 1
+    int i0 = 0;
+    for (; i0 + PF <= n; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j);
+    }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (i0 + j < n) body(W - 1 - (i0 + j), j);
+}
+
 // ------------------------------------------------------------------ fused vertical + diagonal paths (bands)
 // Paths 1, 2, 3 of one pass in one kernel, so each cell's raw cost is read once and its aggregate
 // read-modify-written once for the three. Rows advance in bands of kBandRows; a band is one launch, with one
@@ -1580,10 +1901,12 @@ __global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t c
 inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((total + block - 1) / block, 1u << 20); }
 
 // ------------------------------------------------------------------ launchers
-template <int NP, bool MASKED, bool VAR, bool ADP>
+template <int NP, bool MASKED, bool VAR, bool ADP, int SEG = 0>
 hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                    hipStream_t s, hipEvent_t* marks)
 {
+    static_assert(SEG == 0 || (NP == 1 && MASKED && VAR && !ADP), "segmented chains: narrow per-pixel ranges only");
+    auto sblocks = [](int chains) { return (unsigned)(((size_t)chains * (SEG ? SEG : 64) + kBlock - 1) / kBlock); };
     int mk = 0;
     auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
     SweepArgs a;
@@ -1632,9 +1955,15 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
                 ak[k].agg = b.aggp[k];
                 (void)hipStreamWaitEvent(b.conc[k], b.conc_fork, 0);
             }
-            sgm_sweep<NP, DIR, 1, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
-            sgm_sweep<NP, DIR, 2, true, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
-            sgm_sweep<NP, DIR, 3, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
+            if constexpr (SEG > 0) {
+                sgm_sweep_seg<SEG, DIR, 1, true><<<sblocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
+                sgm_sweep_seg<SEG, DIR, 2, true><<<sblocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
+                sgm_sweep_seg<SEG, DIR, 3, true><<<sblocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
+            } else {
+                sgm_sweep<NP, DIR, 1, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
+                sgm_sweep<NP, DIR, 2, true, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
+                sgm_sweep<NP, DIR, 3, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
+            }
             for (int k = 0; k < 3; k++) {
                 (void)hipEventRecord(b.conc_join[k], b.conc[k]);
                 (void)hipStreamWaitEvent(s, b.conc_join[k], 0);
@@ -1643,6 +1972,15 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
                                                            (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
             mark();
             mark();
+            mark();
+            return;
+        }
+        if constexpr (SEG > 0) {
+            sgm_sweep_seg<SEG, DIR, 1, false><<<sblocks(nd), kBlock, 0, s>>>(a);
+            mark();
+            sgm_sweep_seg<SEG, DIR, 2, false><<<sblocks(nv), kBlock, 0, s>>>(a);
+            mark();
+            sgm_sweep_seg<SEG, DIR, 3, false><<<sblocks(nd), kBlock, 0, s>>>(a);
             mark();
             return;
         }
@@ -1692,7 +2030,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         (void)hipEventRecord(b.conc_fork, s);
         forked = true;
     }
-    if constexpr (VAR) {
+    if constexpr (SEG > 0) {
+        sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
+    } else if constexpr (VAR) {
         sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     } else {
         if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
@@ -1721,7 +2061,8 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
     vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
-    sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+    if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
+    else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
@@ -1743,6 +2084,10 @@ hipError_t launch_sgm_t(const Geometry& g, const SgmBuffers& b, const SgmOutputs
     const bool full = g.cap == 64 * g.dpl;
     if (b.vstart) {  // per-pixel ranges: one packed pair per lane, per-cell masks
         if (g.dpl != 2) return hipErrorInvalidValue;
+        if constexpr (!ADP) {  // narrow ranges: several chains per wavefront
+            if (b.seg && g.cap <= 16) return run_sgm<1, true, true, false, 8>(g, b, o, p1, p2, agg_pass1, s, m);
+            if (b.seg && g.cap <= 32) return run_sgm<1, true, true, false, 16>(g, b, o, p1, p2, agg_pass1, s, m);
+        }
         return run_sgm<1, true, true, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
     }
     switch (g.dpl) {
