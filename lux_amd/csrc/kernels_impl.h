@@ -901,6 +901,11 @@ __device__ __forceinline__ void seg_reframe(uint32_t& st, Range& from, int s, in
                                             uint32_t& edge_r)
 {
     const int delta = s - from.s, cp = from.c;
+    if (!__ballot(delta != 0)) {  // no segment moves its start: the identity (planes past cp already hold 2000)
+        edge_l = edge_r = kInit2;
+        from.c = c;
+        return;
+    }
     const uint32_t v = st;
     const uint32_t even = seg_get<SEG>(v, li + (delta >> 1), lane);
     const int sa = li + ((delta - 1) >> 1);
