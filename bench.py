@@ -385,6 +385,8 @@ def main():
                     help="for counter runs: a few requests, then a 1 GiB device copy as the byte-count calibration")
     ap.add_argument("--profile-l16", action="store_true",
                     help="for trace runs: --steps L16 pyramids (device form, images in HBM) and nothing else")
+    ap.add_argument("--profile-c2f", action="store_true",
+                    help="for trace runs: --steps coarse-to-fine requests (plain, host buffers) and nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is a design note.",
@@ -398,6 +400,17 @@ def main():
     import torch
 
     from lux_amd import scene, sgm
+
+    if args.profile_c2f:
+        left, right, _ = scene.make_pair(PYR_W, PYR_H, PYR_D - 1, planes=6, sigma=4.0, seed=300, bits=BITS, fg_frac=0.1)
+        ctx = sgm.Context(0)
+        for _ in range(args.warmup + args.steps):
+            t = time.perf_counter()
+            sgm.pyramid(ctx, left, right, PYR_D, PYR_RADII, bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+            print(f"c2f request {(time.perf_counter() - t) * 1e3:.3f} ms", file=sys.stderr, flush=True)
+        ctx.close()
+        print(json.dumps({"profile_c2f": True, "requests": args.warmup + args.steps}))
+        return
 
     if args.profile_l16:
         imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
