@@ -130,3 +130,32 @@ def test_upsample_layer_size_rule_and_flat_depth():
     np.testing.assert_allclose(out, 1500.0, rtol=1e-3)  # rcp twice (~2^-12 each) of a flat depth
     with pytest.raises(RuntimeError):
         o.upsample_layer(depth, guide, 8, 5)  # "Incorrect output depth size!"
+
+
+def test_bilateral_follows_the_one_matching_tap():
+    """A structural pin that does not restate the arithmetic: every low-res pixel has its own colour, 255 apart from
+    the others' on some channel, so for an output pixel whose guide colour equals exactly one of its four taps the
+    range weights of the other three are 2^-126 + 1e-8 against 1, and the result is that tap's value to float32
+    precision. Which tap it is follows (x-1)/2, (x-1)/2 + 1 (clamped) and the same in y; a wrong index, a swapped
+    axis or a dropped spatial weight moves it to another tap."""
+    lw, lh = 6, 5
+    rng = np.random.default_rng(3)
+    lo = rng.uniform(1.0, 9.0, (lh, lw)).astype(np.float32)
+    colours = np.zeros((lh, lw, 4), np.uint8)
+    for j in range(lh):
+        for i in range(lw):  # corners of the RGB cube, distinct for neighbouring pixels (parities of i and j)
+            colours[j, i] = (255 * (i % 2), 255 * (j % 2), 255 * ((i // 2 + j // 2) % 2), 0)
+    W, H = 2 * lw, 2 * lh
+    guide = np.zeros((H, W, 4), np.uint8)
+    want = np.zeros((H, W), np.float32)
+    for y in range(H):
+        iy0 = (y - 1) // 2 if y >= 1 else 0   # C division toward zero: (y-1)/2 is 0 for y = 0
+        for x in range(W):
+            ix0 = (x - 1) // 2 if x >= 1 else 0
+            taps = [(min(max(iy0 + dj, 0), lh - 1), min(max(ix0 + di, 0), lw - 1)) for dj in (0, 1) for di in (0, 1)]
+            ty, tx = taps[(x + 2 * y) % 4]    # pick one of the four taps per pixel
+            guide[y, x] = colours[ty, tx]
+            want[y, x] = lo[ty, tx]
+    out = o.bilateral_upsample(lo, guide, colours, 12.0)
+    # every chosen tap is the only one of the four with that colour, except where clamping repeats a tap
+    np.testing.assert_allclose(out, want, rtol=1e-6)
