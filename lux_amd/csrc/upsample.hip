@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -68,27 +69,35 @@ __device__ __forceinline__ float spatial_w(float f, int i)
     return (int)d == 0 ? 1.0f : __int_as_float(0x3eaaaaab);
 }
 
-__device__ __forceinline__ float range_w(float hr, float hg, float hb, const uint8_t* q, float k, float wyx)
+__device__ __forceinline__ float range_w(float hr, float hg, float hb, uint32_t q, float k, float wyx)
 {
-    const float d0 = __fsub_rn(hr, (float)q[0]), d1 = __fsub_rn(hg, (float)q[1]), d2 = __fsub_rn(hb, (float)q[2]);
+    const float d0 = __fsub_rn(hr, (float)(q & 255u)), d1 = __fsub_rn(hg, (float)((q >> 8) & 255u)),
+                d2 = __fsub_rn(hb, (float)((q >> 16) & 255u));
     const float s = __fadd_rn(__fmul_rn(d1, d1), __fadd_rn(__fmul_rn(d2, d2), __fmul_rn(d0, d0)));
     return __fmul_rn(__fadd_rn(exp2_fast_signbit(-__fmul_rn(s, k)), __int_as_float(0x322bcc77)), wyx);
 }
 
-// BilateralUpsampleFromCollapse<2, float, vec4x8ui> (:156-210) of InverseDepth(prev) (:133-140), one output pixel
-// per thread, then InverseDepth again: the low-res inverse is taken as the taps are read. With `conf` the previous
-// confidence goes through the same weights in the same order (:49-50, no inverse).
+// InverseDepth (:133-140) of the previous depth, once per low-res pixel.
+__global__ void inverse_kernel(const float* src, float* dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = rcp_intel(src[i]);
+}
+
+// BilateralUpsampleFromCollapse<2, float, vec4x8ui> (:156-210) of the inverse depth, one output pixel per thread,
+// then InverseDepth again. With `conf` the previous confidence goes through the same weights in the same order
+// (:49-50, no inverse). Guide and g2 pixels are read as one 32-bit word each (the buffers are 4-byte aligned).
 template <bool CONF>
-__global__ void bilateral_kernel(const float* lo, const float* conf, int lw, int lh, const uint8_t* guide, int W, int H,
-                                 const uint8_t* g2, float k, float* out, float* conf_out)
+__global__ void bilateral_kernel(const float* inv, const float* conf, int lw, int lh, const uint32_t* guide, int W, int H,
+                                 const uint32_t* g2, float k, float* out, float* conf_out)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
     const int wm1 = lw - 1, hm1 = lh - 1;
     const int iy0 = (y - 1) / 2;
     const float fy = __fadd_rn((float)y, -0.5f);
-    const uint8_t* gp = guide + ((size_t)y * W + x) * 4;
-    const float hr = gp[0], hg = gp[1], hb = gp[2];
+    const uint32_t gv = guide[(size_t)y * W + x];
+    const float hr = (float)(gv & 255u), hg = (float)((gv >> 8) & 255u), hb = (float)((gv >> 16) & 255u);
     const int ix0 = (x - 1) / 2;
     const float fx = __fadd_rn((float)x, -0.5f);
     const int ibx = min(max(ix0, 0), wm1), ix1 = min(max(ix0 + 1, 0), wm1);
@@ -99,9 +108,9 @@ __global__ void bilateral_kernel(const float* lo, const float* conf, int lw, int
         const int iy = min(max(iy0 + j, 0), hm1);
         const float wy = spatial_w(fy, iy);
         const size_t i0 = (size_t)iy * lw + ibx, i1 = (size_t)iy * lw + ix1;
-        const float w0 = range_w(hr, hg, hb, g2 + i0 * 4, k, __fmul_rn(wy, wx0));
-        const float w1 = range_w(hr, hg, hb, g2 + i1 * 4, k, __fmul_rn(wy, wx1));
-        const float v0 = __fmul_rn(rcp_intel(lo[i0]), w0), v1 = __fmul_rn(rcp_intel(lo[i1]), w1);
+        const float w0 = range_w(hr, hg, hb, g2[i0], k, __fmul_rn(wy, wx0));
+        const float w1 = range_w(hr, hg, hb, g2[i1], k, __fmul_rn(wy, wx1));
+        const float v0 = __fmul_rn(inv[i0], w0), v1 = __fmul_rn(inv[i1], w1);
         av = __fadd_rn(v1, __fadd_rn(v0, av));
         as = __fadd_rn(w1, __fadd_rn(w0, as));
         if (CONF) ac = __fadd_rn(__fmul_rn(conf[i1], w1), __fadd_rn(__fmul_rn(conf[i0], w0), ac));
@@ -155,20 +164,28 @@ Taps taps_for(float sx, float sy)
 int enqueue_upsample(Workspace& ws, const float* d_prev, const float* d_prev_conf, int pw, int ph, const uint8_t* d_guide,
                      int W, int H, float sigma, float* d_depth, float* d_conf, hipStream_t s, std::string* err)
 {
-    void* g2;
+    void *g2, *inv;
     int rc;
-    if ((rc = ws.get("ups_g2", (size_t)pw * ph * 4, &g2, err))) return rc;
+    if ((rc = ws.get("ups_g2", (size_t)pw * ph * 4, &g2, err)) || (rc = ws.get("ups_inv", (size_t)pw * ph * 4, &inv, err)))
+        return rc;
+    if (((uintptr_t)d_guide & 3u) != 0) {
+        if (err) *err = "the guide must be 4-byte aligned";
+        return LUX_SGM_EINVAL;
+    }
     const Taps t = taps_for(0.5f, 0.5f);  // ImageShift<2>(guide, {0.5f, 0.5f}) (:41)
     const int step_y = (int)((double)H * 65536.0 / (double)ph), step_x = (int)((double)W * 65536.0 / (double)pw);
     shifted_resize_kernel<<<dim3((pw * 4 + 255) / 256, ph), 256, 0, s>>>(d_guide, W, H, t, pw, step_x, step_y, (uint8_t*)g2);
+    const size_t np = (size_t)pw * ph;
+    inverse_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(d_prev, (float*)inv, np);
     const float k = 0.5f / (sigma * sigma);
     const dim3 grid((W + 255) / 256, H);
+    const uint32_t* gd = (const uint32_t*)d_guide;
     if (d_prev_conf && d_conf)
-        bilateral_kernel<true><<<grid, 256, 0, s>>>(d_prev, d_prev_conf, pw, ph, d_guide, W, H, (const uint8_t*)g2, k, d_depth,
-                                                    d_conf);
+        bilateral_kernel<true><<<grid, 256, 0, s>>>((const float*)inv, d_prev_conf, pw, ph, gd, W, H, (const uint32_t*)g2, k,
+                                                    d_depth, d_conf);
     else
-        bilateral_kernel<false><<<grid, 256, 0, s>>>(d_prev, nullptr, pw, ph, d_guide, W, H, (const uint8_t*)g2, k, d_depth,
-                                                     nullptr);
+        bilateral_kernel<false><<<grid, 256, 0, s>>>((const float*)inv, nullptr, pw, ph, gd, W, H, (const uint32_t*)g2, k,
+                                                     d_depth, nullptr);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_status(e, "upsample", err);
 }
