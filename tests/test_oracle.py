@@ -69,7 +69,7 @@ def test_invariants_constant_raw_and_zero_penalties(W, H, T):
     assert nvis.min() >= 2 and nvis.max() <= (4 if W == 1 else 3)
 
 
-def test_double_visit_counts_match_survey():
+def test_double_visit_counts_640x480_t4():
     # every cell is visited >= once per pass; 640x480 with T=4 double-visits 120 cells per pass
     vis = {}
     for d, x, y, cur in M.schedule(640, 480, 4):
