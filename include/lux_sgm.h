@@ -114,11 +114,13 @@ int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const
  *   agg       out, jagged like raw, uint16 (DenseLayer.Agg[i])                       (may be NULL)
  *   agg_pass1 out, the same after pass +1 (DenseLayer.AggPass1)                      (may be NULL)
  *   wta       [H*W] out: start + first-minimum index over [0, count) (:225-231)       (may be NULL)
- * p->planes is the layer's plane count (start + count <= planes). The device path keeps each path vector in
- * its cell's own plane frame and realigns it to the next cell's start; that is the reference's line-buffer
+ * p->planes is the layer's plane count (start + count <= planes). The path sweeps keep each path vector in
+ * its cell's own plane frame and realign it to the next cell's start; that is the reference's line-buffer
  * result as long as no range's padding reaches the last word of its path segment: start - MinLo + cap <=
  * MaxHi for every pixel (MinLo = min start, MaxHi = roundup(max(start + count), align)). A layer that breaks
- * it, or has a cap above 128, gets LUX_SGM_EUNSUPPORTED. */
+ * it (the reference then writes into the next segment and reads those words back), has a cap above 128 or an
+ * odd tile runs on the line-buffer-literal path, which holds the reference's line buffers word for word; caps
+ * above 256 get LUX_SGM_EUNSUPPORTED. */
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
@@ -184,8 +186,9 @@ typedef struct lux_sgm_dense_params {
  * the plane-sweep cost of every computed pixel, the image-adaptive eight-path aggregation (the guidance image is the
  * reference subsampled at the layer's scale, :39), the first-minimum WTA and depth = Planes[start + arg] [H*W].
  * start/count both null: every pixel searches all planes (layer 0). skip may be null. agg / agg_pass1 (packed like
- * the raw volume) and wta (the plane index) may be null. Returns LUX_SGM_EUNSUPPORTED where the ranged aggregation
- * is (ranges above 128 lanes, padding past the line-buffer segment) and for more than 1024 planes at layer 0. */
+ * the raw volume) and wta (the plane index) may be null. Layers whose padding passes the line-buffer segment, with
+ * caps of 129..256 lanes or an odd tile run on the line-buffer-literal path (lux_sgm_compute_layer_ranges).
+ * Returns LUX_SGM_EUNSUPPORTED for ranges above 256 lanes and for more than 1024 planes at layer 0. */
 int lux_sgm_dense_layer(lux_sgm_ctx* ctx, const lux_sgm_sweep* sweep, const lux_sgm_dense_params* params,
                         const uint16_t* start, const uint16_t* count, const uint8_t* skip, float* depth, uint16_t* agg,
                         uint16_t* agg_pass1, uint16_t* wta);
@@ -286,6 +289,10 @@ int lux_sgm_pyramid_device(lux_sgm_ctx* ctx, const lux_sgm_pyramid_params* p, co
  * LUX_SGM_OPT_FUSED_BANDS: 1 runs the three vertical/diagonal paths of each pass fused in band tiles that
  * synchronise through device flags (up to 256 planes); 0 (default) runs one sweep kernel per path. */
 #define LUX_SGM_OPT_FUSED_BANDS 1
+/* LUX_SGM_OPT_LITERAL: 1 runs every layer of at most 256 lanes (lux_sgm_compute_layer*, lux_sgm_dense_layer,
+ * the dense pyramid) on the line-buffer-literal path: one wavefront per row follows the reference's tile
+ * schedule against its own line buffers. Slower; layers that need it take it whatever this option says. */
+#define LUX_SGM_OPT_LITERAL 2
 int lux_sgm_set_option(lux_sgm_ctx* ctx, int32_t option, int32_t value);
 
 /* Stage timing, the per-layer timing log of the reference's pyramid driver (DenseStereoSgm.cs:347-349)

@@ -515,6 +515,8 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
     std::vector<uint64_t> off;
     uint64_t total = 0;
     int maxcount = p->planes;
+    bool spill = false;  // some range's padding passes its line-buffer segment (start - MinLo + cap > MaxHi)
+    int lminlo = 0, lmaxhi = 0, lmaxcap = 0;
     if (ranged) {
         cap.resize(cells);
         off.resize(cells);
@@ -533,20 +535,28 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         }
         maxhi = ((maxhi + align - 1) / align) * align;
         if (maxcap == 0) return fail(c, LUX_SGM_EINVAL, "every range is empty");
-        if (maxcap > 128) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 128 lanes");
+        if (maxcap > 256) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 256 lanes");
         for (size_t i = 0; i < cells; i++)
-            if ((int)start[i] - minlo + (int)cap[i] > maxhi)
-                return fail(c, LUX_SGM_EUNSUPPORTED,
-                            "a range's padding reaches the end of its line-buffer segment (start - MinLo + cap > MaxHi): "
-                            "the reference then writes into the next path's segment, which the device path does not model");
+            if ((int)start[i] - minlo + (int)cap[i] > maxhi) spill = true;
+        lminlo = minlo;
+        lmaxhi = maxhi;
+        lmaxcap = maxcap;
     }
     (void)hipSetDevice(c->device);
     hipStream_t s = c->stream;
     lux::Geometry g = lux::geometry(W, H, maxcount, align);
-    if (!ranged) total = cells * g.cap;
+    if (!ranged) {
+        total = cells * g.cap;
+        lmaxhi = lmaxcap = g.cap;
+    }
     lux::Workspace& ws = c->ws;
+    // The path sweeps take even tiles, caps up to 128 with ranges and ranges whose padding stays inside its
+    // line-buffer segment; the line-buffer-literal path takes the rest up to 256 lanes (and everything when forced).
+    const bool literal = lmaxcap <= 256 && (ws.force_literal || spill || (p->tile & 1) || (ranged && lmaxcap > 128));
     lux::Prepared w;
-    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err))) return rc;
+    if (literal) rc = lux::prepare_literal(ws, g, &w, &c->err);
+    else rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err);
+    if (rc) return rc;
     void *stage, *a1 = nullptr, *dw;
     if ((rc = ws.get("layer_stage", std::max<size_t>(total * 2, 2), &stage, &c->err)) ||
         (rc = ws.get("layer_wta", cells * 2, &dw, &c->err)))
@@ -599,15 +609,38 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         w.b.ap.on = 1;
         w.b.seedguid = (uint32_t*)sg;
     }
-    const lux::Conc conc = lux::conc_of(c);
-    if (ranged) lux::try_concurrency(ws, &w, &conc);  // ranged layers are latency-bound: their sweeps run side by side
-    lux::SgmOutputs o;
-    o.disp = (uint16_t*)dw;
-    o.tol = 0;
-    o.write_agg = agg != nullptr;
-    o.bands = ranged || gd ? 0 : c->opt_bands;
-    e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
-    if (e != hipSuccess) return hip_fail(c, e, "aggregation");
+    if (literal) {
+        lux::LiteralLayer l;
+        l.W = W;
+        l.H = H;
+        l.T = p->tile;
+        l.align = align;
+        l.xsw = p->xs_clamp_w;
+        l.minlo = lminlo;
+        l.maxhi = lmaxhi;
+        l.start = w.b.vstart;
+        l.cap = w.b.vcap;
+        l.count = w.b.vcount;
+        l.ucap = g.cap;
+        l.ucount = p->planes;
+        l.raw = w.raw;
+        l.agg = w.agg;
+        l.cs = g.cs;
+        l.ap = w.b.ap;
+        l.p1 = p->p1;
+        l.p2 = p->p2;
+        if ((rc = lux::run_literal(ws, l, lmaxcap, (uint16_t*)a1, (uint16_t*)dw, s, &c->err))) return rc;
+    } else {
+        const lux::Conc conc = lux::conc_of(c);
+        if (ranged) lux::try_concurrency(ws, &w, &conc);  // ranged layers are latency-bound: their sweeps run side by side
+        lux::SgmOutputs o;
+        o.disp = (uint16_t*)dw;
+        o.tol = 0;
+        o.write_agg = agg != nullptr;
+        o.bands = ranged || gd ? 0 : c->opt_bands;
+        e = lux::launch_sgm(g, w.b, o, p->p1, p->p2, (uint16_t*)a1, s);
+        if (e != hipSuccess) return hip_fail(c, e, "aggregation");
+    }
     if (wta) {
         e = hipMemcpyAsync(wta, dw, cells * 2, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(c, e, "wta");
@@ -624,7 +657,7 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
     }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "compute_layer");
-    return lux::check_band_error(ws, s, &c->err);
+    return literal ? 0 : lux::check_band_error(ws, s, &c->err);
 }
 
 int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)
@@ -633,6 +666,7 @@ int lux_sgm_set_option(lux_sgm_ctx* c, int32_t option, int32_t value)
     std::lock_guard<std::mutex> lk(c->mu);
     switch (option) {
         case LUX_SGM_OPT_FUSED_BANDS: c->opt_bands = value != 0; return LUX_SGM_OK;
+        case LUX_SGM_OPT_LITERAL: c->ws.force_literal = value != 0; return LUX_SGM_OK;
         default: return fail(c, LUX_SGM_EINVAL, "unknown option");
     }
 }

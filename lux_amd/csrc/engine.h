@@ -43,6 +43,7 @@ struct Workspace {
     const Plan* plan = nullptr;      // whose visit lists are in "visits"
     void* sync_zeroed = nullptr;     // band flag array known to hold no live epoch
     int epoch = 0;
+    int force_literal = 0;           // LUX_SGM_OPT_LITERAL: layers of <= 256 lanes take the line-buffer-literal path
 
     Workspace() = default;
     Workspace(const Workspace&) = delete;
@@ -106,6 +107,14 @@ int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::stri
 // Allocate the workspace for geometry g and upload the plan's visit lists (copies from host memory on s).
 // Nothing here may be captured into a graph; enqueue_* below may.
 int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err);
+
+// The line-buffer-literal path (launch_literal). prepare_literal gives w the raw and aggregate volumes of g (the
+// workspace buffers prepare() uses, so a layer's cost can be written into them first). run_literal runs one layer
+// whose largest cap is maxcap and synchronises s: LUX_SGM_EUNSUPPORTED above 256 lanes or when the device cannot
+// hold the wavefronts the schedule's waits need, LUX_SGM_EDEVICE when a wait gave up.
+int prepare_literal(Workspace& ws, const Geometry& g, Prepared* w, std::string* err);
+int run_literal(Workspace& ws, LiteralLayer l, int maxcap, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s,
+                std::string* err);
 
 // Enqueue one rectified request on s: cost volume, the eight paths, WTA, subpixel, L/R. Kernel launches and
 // device-to-device copies only. ev (if not null) receives 2 + kSgmMarks events for the stage times.

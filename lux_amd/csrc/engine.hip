@@ -504,4 +504,41 @@ bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
     return true;
 }
 
+int prepare_literal(Workspace& ws, const Geometry& g, Prepared* w, std::string* err)
+{
+    const size_t cells = (size_t)g.W * g.H;
+    void *raw, *agg;
+    int rc;
+    if ((rc = ws.get("raw", cells * g.cs, &raw, err)) || (rc = ws.get("agg", cells * g.cs * 2, &agg, err))) return rc;
+    w->g = g;
+    w->raw = (uint8_t*)raw;
+    w->agg = (uint16_t*)agg;
+    w->b = SgmBuffers{};
+    w->b.raw = w->raw;
+    w->b.raw_out = w->raw;
+    w->b.agg = w->agg;
+    return 0;
+}
+
+int run_literal(Workspace& ws, LiteralLayer l, int maxcap, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s,
+                std::string* err)
+{
+    if (maxcap > 256) return fail(LUX_SGM_EUNSUPPORTED, "the line-buffer-literal path takes at most 256 lanes per cell", err);
+    l.npl = maxcap <= 128 ? 2 : 4;
+    static int resident[2][2] = {{-1, -1}, {-1, -1}};  // [npl 2/4][adaptive]; the same device model throughout
+    int& res = resident[l.npl == 4][l.ap.on != 0];
+    if (res < 0) res = literal_resident(l.npl, l.ap.on != 0);
+    if (res < std::min(literal_depth(l.W, l.T), l.H))
+        return fail(LUX_SGM_EUNSUPPORTED, "the device holds too few wavefronts for the line-buffer-literal schedule", err);
+    void* scr;
+    int rc = ws.get("literal", literal_scratch_bytes(l.W, l.H, l.maxhi), &scr, err);
+    if (rc) return rc;
+    hipError_t e = launch_literal(l, scr, agg_pass1, wta, s);
+    uint32_t bad = 0;
+    if (e == hipSuccess) e = literal_error(scr, l.W, l.H, l.maxhi, s, &bad);
+    if (e != hipSuccess) return hip_status(e, "line-buffer-literal layer", err);
+    if (bad) return fail(LUX_SGM_EDEVICE, "a line-buffer-literal wait gave up", err);
+    return 0;
+}
+
 }  // namespace lux

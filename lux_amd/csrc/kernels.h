@@ -90,6 +90,31 @@ struct SgmOutputs {
 constexpr int kSgmMarks = 11;
 hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
                       hipStream_t s, hipEvent_t* marks = nullptr);
+// Line-buffer-literal aggregation (literal.hip): DenseLayer.Compute's two passes over the reference's own line
+// buffers, word for word, for the layers the path sweeps do not model (padding that spills past its segment,
+// odd tiles, caps up to 256), then the first-minimum WTA.
+struct LiteralLayer {
+    int W = 0, H = 0, T = 0, align = 0, xsw = 0;
+    int minlo = 0, maxhi = 0;          // MinLo, MaxHi rounded to align
+    const uint16_t* start = nullptr;   // null: every cell [0, ucap) with ucount planes
+    const uint16_t* cap = nullptr;
+    const uint16_t* count = nullptr;
+    int ucap = 0, ucount = 0;
+    const uint8_t* raw = nullptr;      // [H][W][cs], skipped cells' bytes already 0 / 255
+    uint16_t* agg = nullptr;           // [H][W][cs]
+    int cs = 0;
+    int npl = 2;                       // planes per lane: 2 (caps <= 128) or 4 (caps <= 256)
+    AdaptP2 ap;
+    int p1 = 0, p2 = 0;
+};
+size_t literal_scratch_bytes(int W, int H, int maxhi);
+// Rows whose wavefronts must be resident at once for every wait to end, and how many the device holds.
+int literal_depth(int W, int T);
+int literal_resident(int npl, bool adp);
+hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s);
+// Synchronises s and reads the error word (non-zero: a wait gave up).
+hipError_t literal_error(void* scratch, int W, int H, int maxhi, hipStream_t s, uint32_t* err);
+
 // Fill n u32 with v (a kernel, not a memset: safe inside captured graphs).
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // Band tiles of a W x H layer (the flag array size per pass).

@@ -1,0 +1,464 @@
+// Line-buffer-literal aggregation of one DenseLayer (DenseStereoSgm.cs:158-166 buffers, :206-224 schedule,
+// Pixel :234-317): the reference's two line-buffer halves, path segments, guard words and margin slots
+// held word for word in HBM, and every visit of the sheared T x T tile schedule run against them. It serves
+// the layers the per-direction sweeps do not model:
+//   - padding that spills: a range with start - MinLo + cap > MaxHi writes its last planes past its path
+//     segment, into the guard word and the low words of the next segment (the next slot's lead word and
+//     path-0 segment for path 3), and its stale cleanup later refills the same words (:254-264, :280-313);
+//     successors whose ranges touch those words read them (and Rd's zero past the end of the buffer, :324);
+//   - odd tiles T, whose halves do not alternate row by row across band boundaries (:219);
+//   - ranges of up to 256 planes.
+//
+// Schedule. The reference runs visits in the order of the key (band j, tile i, tile row r), row r of tile i
+// covering x in [iT - r, iT + T - r) (mirrored in pass -1), clamped to the image, and tile gwN-1 visiting the
+// last column (the first in pass -1) a second time when (gwN-1)T - r >= W. Row y = jT + r (in pass order)
+// therefore meets its cells in column order, tile i = (x~ + r) / T for column position x~, then possibly the
+// second visit. A visit reads and writes line-buffer slots within two columns of its own, in its own half
+// and the other (rows y-2 .. y+2 use them), so it only has to follow, in the reference's order, the visits
+// of those rows within that reach. One wavefront runs one row's visits in order; before each it waits until
+// each of rows y-2, y-1, y+1, y+2 has completed every visit with a smaller key within three columns (a
+// prefix of that row's visits, so one progress counter per row says it). Rows of an earlier band come
+// wholly before, those of a later band wholly after.
+//
+// The second visit of the border column is the exception: it reads the buffers as every earlier visit of its
+// band left them, and the last writer near the border can be any row of the band (the rows below it whose
+// border cells fall in the previous tile), so it waits on every row of its band.
+//
+// Progress. Waits only point to smaller keys, and a row never waits on a row of a later band, so a wait for a
+// later row reaches at most T rows ahead. Rows are taken by ticket, so with T + 4 wavefronts resident (the
+// host checks occupancy) every wait ends. A bounded spin sets the error word instead of hanging, and later
+// waits are skipped, so the grid always drains.
+//
+// Memory. All line-buffer, minimum, range and guidance words go through agent-scope accesses (global
+// load/store sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
+// (s_waitcnt) before its progress store. One wavefront per row, planes across lanes (NPL per lane).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+#include "p2_dev.h"
+
+namespace lux {
+namespace {
+
+constexpr int kLitReach = 3;   // columns either side whose visits may share line-buffer words with a visit
+constexpr int kLitGrid = 2048; // persistent single-wavefront workgroups per pass (rows go by ticket)
+
+struct LitArgs {
+    int W, H, T, gwN, rev_plus, align;
+    int minlo, M, stride;            // M = MaxHi rounded to the alignment; stride = 4M + 5 (:160)
+    long halfL, Llen;
+    int halfM, halfR;
+    uint16_t* L;
+    uint16_t* Min;
+    uint16_t* RS;
+    uint16_t* RC;
+    uint32_t* Pix;                   // guidance of the slot's cell, RGBA8 (0 for the initial fill)
+    const uint16_t* start;           // null: every cell [0, ucap)
+    const uint16_t* cap;
+    int ucap;
+    const uint8_t* raw;              // [H][W][cs]
+    uint16_t* agg;                   // [H][W][cs]
+    int cs;
+    AdaptP2 ap;
+    int p1, p2;
+    uint32_t* progress;              // [H] visits completed, per row in pass order
+    uint32_t* ticket;
+    uint32_t* err;
+};
+
+__device__ __forceinline__ uint16_t ald(const uint16_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ast(uint16_t* p, uint16_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ald32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ast32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Whether tile row r ends with the second visit of the last column (first column in pass -1): the last
+// tile's row starts past the image (:213-216); LUX_SGM_XSW's clamp to W removes it in pass +1.
+__device__ __forceinline__ int lit_rev(const LitArgs& a, int dir, int r)
+{
+    return (dir < 0 || a.rev_plus) && (a.gwN - 1) * a.T - r >= a.W;
+}
+
+// Visits of row2 (pass order) whose key precedes (band j, tile i, tile row r).
+__device__ __forceinline__ int lit_before(const LitArgs& a, int dir, int j, int i, int r, int row2)
+{
+    const int jj = row2 / a.T, rr = row2 - jj * a.T;
+    const int rev = lit_rev(a, dir, rr);
+    if (jj < j) return a.W + rev;
+    if (jj > j) return 0;
+    if (rr < r && i == a.gwN - 1) return a.W + rev;
+    const long lim = (long)(rr < r ? i + 1 : i) * a.T - rr;
+    return (int)std::min(std::max(lim, 0L), (long)a.W);
+}
+
+// Rd (:318-324): past the end of the buffer the reference reads zero pages.
+__device__ __forceinline__ int lit_rd(const LitArgs& a, long i) { return i < a.Llen ? (int)ald(a.L + i) : 0; }
+
+__device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+        const uint32_t lo = std::min(v & 0xffffu, w & 0xffffu), hi = std::min(v >> 16, w >> 16);
+        v = lo | (hi << 16);
+    }
+    return v;
+}
+
+// One Pixel call (:234-317) at (x, y).
+template <int DIR, int NPL, bool ADP>
+__device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth)
+{
+    const int lane = threadIdx.x;
+    const int slot = x + 1;
+    const size_t cell = (size_t)y * a.W + x;
+    const int start = a.start ? (int)a.start[cell] : 0;
+    const int cap = a.start ? (int)a.cap[cell] : a.ucap;
+    const int q = start - a.minlo;
+    const uint32_t g = ADP ? a.ap.guid[cell] : 0u;
+    const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
+    const size_t cbase = cell * (size_t)a.cs;
+
+    // every read first: the visit never reads a word it writes (path 0 reads another slot of its half, paths
+    // 1-3 the other half, and path 3's spill into the next slot comes after path 0's read in pass -1)
+    int minPrev[4], p2p[4];
+    int lm[4][NPL], l0[4][NPL], lp[4][NPL];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
+        const int half = p == 0 ? cur : oth;
+        minPrev[p] = ald(a.Min + half * a.halfM + ns * 4 + p);
+        int P2 = a.p2;
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, ald32(a.Pix + half * a.halfR + ns));
+        p2p[p] = std::min(P2 + minPrev[p], 65535);
+        const long nb = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
+#pragma unroll
+        for (int h = 0; h < NPL; h++) {
+            const int k = lane + 64 * h;
+            lm[p][h] = l0[p][h] = lp[p][h] = 0;
+            if (k < cap) {
+                lm[p][h] = lit_rd(a, nb + k - 1);
+                l0[p][h] = lit_rd(a, nb + k);
+                lp[p][h] = lit_rd(a, nb + k + 1);
+            }
+        }
+    }
+    int rb[NPL], ag[NPL];
+#pragma unroll
+    for (int h = 0; h < NPL; h++) {
+        const int k = lane + 64 * h;
+        rb[h] = k < cap ? (int)a.raw[cbase + k] : 0;
+        ag[h] = k < cap ? (int)ald(a.agg + cbase + k) : 0;
+    }
+
+    // the recurrence (:288-298 / :302-312)
+    int lc[4][NPL];
+    uint32_t mins01 = 0xffffffffu, mins23 = 0xffffffffu;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        uint32_t run = 0xffffu;
+#pragma unroll
+        for (int h = 0; h < NPL; h++) {
+            const int m = std::min(std::min(std::min(lm[p][h] + a.p1, 65535), l0[p][h]),
+                                   std::min(std::min(lp[p][h] + a.p1, 65535), p2p[p]));
+            const int v = std::max(std::min(rb[h] + m, 65535) - minPrev[p], 0);
+            lc[p][h] = v;
+            if (lane + 64 * h < cap) run = std::min(run, (uint32_t)v);
+        }
+        if (p == 0) mins01 = (mins01 & 0xffff0000u) | run;
+        if (p == 1) mins01 = (mins01 & 0xffffu) | (run << 16);
+        if (p == 2) mins23 = (mins23 & 0xffff0000u) | run;
+        if (p == 3) mins23 = (mins23 & 0xffffu) | (run << 16);
+    }
+    mins01 = wave_pmin(mins01);
+    mins23 = wave_pmin(mins23);
+
+    // Pixel's stores in its order: cleanup(0), write(0), cleanup(1), write(1), ... Each is an interval of
+    // words; one that runs past its segment continues into the next (the guard word, then the next segment,
+    // or for path 3 the next slot's lead word and path-0 segment). A word is stored once, by the last
+    // interval that covers it, so no two stores of a visit race.
+    const long B = (long)cur * a.halfL + (long)slot * a.stride;
+    const long q0 = sp - a.minlo;
+    long lo[8], hi[8];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const long bc = B + 1 + (long)p * (a.M + 1);
+        lo[2 * p] = bc + std::max(q0, 0L);
+        hi[2 * p] = bc + std::min(q0 + cp, a.Llen - bc);  // k1 = min(cp, L.Length - baseCur - q0) (:262)
+        lo[2 * p + 1] = bc + q;
+        hi[2 * p + 1] = bc + q + cap;
+    }
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+#pragma unroll
+        for (int h = 0; h < NPL; h++) {
+            const long w = lo[o] + lane + 64 * h;
+            if (w >= hi[o] || w >= a.Llen) continue;
+            bool last = true;
+#pragma unroll
+            for (int o2 = o + 1; o2 < 8; o2++) last = last && !(w >= lo[o2] && w < hi[o2]);
+            if (last) ast(a.L + w, (uint16_t)((o & 1) ? lc[o >> 1][h] : 2000));
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NPL; h++) {
+        const int k = lane + 64 * h;
+        if (k < cap)
+            ast(a.agg + cbase + k,
+                (uint16_t)std::min(ag[h] + lc[0][h] + lc[1][h] + lc[2][h] + lc[3][h], 65535));
+    }
+    if (lane < 4) {
+        const uint32_t mm = lane < 2 ? mins01 : mins23;
+        const int rm = (lane & 1) ? (int)(mm >> 16) : (int)(mm & 0xffffu);
+        ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));  // :275, :314
+    }
+    if (lane == 0) {
+        ast(a.RS + cur * a.halfR + slot, (uint16_t)start);  // :316
+        ast(a.RC + cur * a.halfR + slot, (uint16_t)cap);
+        if constexpr (ADP) ast32(a.Pix + cur * a.halfR + slot, g);  // :239
+    }
+}
+
+// Lane 0 waits until row2 has completed `need` visits; a spin past the bound (or another wavefront's) sets the
+// error word, after which no wavefront waits again.
+__device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, int& seen, bool& dead)
+{
+    if (dead || seen >= need) return;
+    int spins = 0;
+    for (;;) {
+        seen = (int)ald32(a.progress + row2);
+        if (seen >= need) return;
+        if (ald32(a.err) != 0u || ++spins > (1 << 22)) {
+            atomicOr(a.err, 1u);
+            dead = true;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ int lit_need(const LitArgs& a, int dir, int j, int i, int r, int c, int row2)
+{
+    int mp = std::min(c + kLitReach, a.W - 1);
+    if (lit_rev(a, dir, row2 % a.T) && c + kLitReach >= a.W - 1) mp = a.W;
+    return std::min(lit_before(a, dir, j, i, r, row2), mp + 1);
+}
+
+template <int DIR, int NPL, bool ADP>
+__global__ __launch_bounds__(64) void lit_pass(LitArgs a)
+{
+    const int lane = threadIdx.x;
+    const int W = a.W, T = a.T;
+    bool dead = false;  // lane 0: the error word is set, waits are skipped so the grid drains
+    for (;;) {
+        int row = 0;
+        if (lane == 0) row = (int)atomicAdd(a.ticket, 1u);
+        row = __shfl(row, 0);
+        if (row >= a.H) break;
+        const int j = row / T, r = row - j * T;
+        const int y = DIR > 0 ? row : a.H - 1 - row;
+        const int cur = (r & 1) == 0 ? 1 : 0, oth = 1 - cur;  // :219
+        const int nvis = W + lit_rev(a, DIR, r);
+        int seen[4] = {0, 0, 0, 0};  // lane 0: progress last read of rows row-2, row-1, row+1, row+2
+        for (int pos = 0; pos < nvis; pos++) {
+            const int c = pos < W ? pos : W - 1;
+            const int i = pos < W ? (pos + r) / T : a.gwN - 1;
+            if (lane == 0 && pos < W) {
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const int row2 = row + (d < 2 ? d - 2 : d - 1);
+                    if (row2 >= 0 && row2 < a.H) lit_wait(a, row2, lit_need(a, DIR, j, i, r, c, row2), seen[d], dead);
+                }
+            } else if (lane == 0) {
+                // the second visit of the border column reads the buffers as the band's earlier visits left them,
+                // whichever rows made those: it waits on every row of its band (and rows row-2, row-1 before it)
+                const int b1 = std::min(a.H, j * T + T);
+                for (int row2 = std::max(0, std::min(row - 2, j * T)); row2 < b1; row2++) {
+                    int s = 0;
+                    if (row2 != row) lit_wait(a, row2, lit_need(a, DIR, j, i, r, c, row2), s, dead);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth);
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) ast32(a.progress + row, (uint32_t)(pos + 1));
+        }
+    }
+}
+
+// First-minimum WTA (:225-231): the plane index start + arg.
+__global__ void lit_wta(const uint16_t* agg, int cs, const uint16_t* start, const uint16_t* count, int ucount, size_t cells,
+                        uint16_t* wta)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const int n = start ? (int)count[i] : ucount;
+    const uint16_t* p = agg + i * (size_t)cs;
+    int best = 0xffff, arg = 0;
+    for (int k = 0; k < n; k++) {
+        const int v = p[k];
+        if (v < best) {
+            best = v;
+            arg = k;
+        }
+    }
+    wta[i] = (uint16_t)((start ? (int)start[i] : 0) + arg);
+}
+
+__global__ void lit_fill16(uint16_t* p, size_t n, uint16_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void lit_fill32(uint32_t* p, size_t n, uint32_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+struct LitLayout {
+    size_t L, Min, RS, RC, Pix, progress, ctl, total;  // byte offsets; ctl = ticket, err
+};
+LitLayout lit_layout(int W, int H, int M)
+{
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t slots = (size_t)W + 2, stride = (size_t)4 * M + 5;
+    LitLayout l;
+    size_t o = 0;
+    l.L = o;
+    o += up(2 * slots * stride * 2 + 4);
+    l.Min = o;
+    o += up(2 * slots * 4 * 2);
+    l.RS = o;
+    o += up(2 * slots * 2);
+    l.RC = o;
+    o += up(2 * slots * 2);
+    l.Pix = o;
+    o += up(2 * slots * 4);
+    l.progress = o;
+    o += up((size_t)H * 4);
+    l.ctl = o;
+    o += 256;
+    l.total = o;
+    return l;
+}
+
+template <int DIR, int NPL, bool ADP>
+hipError_t lit_launch(const LitArgs& a, int grid, hipStream_t s)
+{
+    lit_pass<DIR, NPL, ADP><<<grid, 64, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int NPL, bool ADP>
+hipError_t lit_pass_dispatch(int dir, const LitArgs& a, int grid, hipStream_t s)
+{
+    return dir > 0 ? lit_launch<1, NPL, ADP>(a, grid, s) : lit_launch<-1, NPL, ADP>(a, grid, s);
+}
+
+hipError_t lit_run_pass(int dir, int npl, bool adp, const LitArgs& a, int grid, hipStream_t s)
+{
+    if (npl == 2) return adp ? lit_pass_dispatch<2, true>(dir, a, grid, s) : lit_pass_dispatch<2, false>(dir, a, grid, s);
+    return adp ? lit_pass_dispatch<4, true>(dir, a, grid, s) : lit_pass_dispatch<4, false>(dir, a, grid, s);
+}
+
+}  // namespace
+
+size_t literal_scratch_bytes(int W, int H, int maxhi) { return lit_layout(W, H, maxhi).total; }
+
+int literal_depth(int W, int T) { return T + 4; }
+
+int literal_resident(int npl, bool adp)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const void* k = npl == 2 ? (adp ? (const void*)lit_pass<1, 2, true> : (const void*)lit_pass<1, 2, false>)
+                             : (adp ? (const void*)lit_pass<1, 4, true> : (const void*)lit_pass<1, 4, false>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64, 0) != hipSuccess) return 0;
+    return per * cus;
+}
+
+hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s)
+{
+    const LitLayout lay = lit_layout(l.W, l.H, l.maxhi);
+    char* base = (char*)scratch;
+    LitArgs a{};
+    a.W = l.W;
+    a.H = l.H;
+    a.T = l.T;
+    a.gwN = (l.W + 2 * l.T - 1) / l.T;
+    a.rev_plus = !l.xsw;
+    a.align = l.align;
+    a.minlo = l.minlo;
+    a.M = l.maxhi;
+    a.stride = 4 * l.maxhi + 5;
+    a.halfL = (long)(l.W + 2) * a.stride;
+    a.Llen = 2 * a.halfL;
+    a.halfM = (l.W + 2) * 4;
+    a.halfR = l.W + 2;
+    a.L = (uint16_t*)(base + lay.L);
+    a.Min = (uint16_t*)(base + lay.Min);
+    a.RS = (uint16_t*)(base + lay.RS);
+    a.RC = (uint16_t*)(base + lay.RC);
+    a.Pix = (uint32_t*)(base + lay.Pix);
+    a.progress = (uint32_t*)(base + lay.progress);
+    a.ticket = (uint32_t*)(base + lay.ctl);
+    a.err = a.ticket + 1;
+    a.start = l.start;
+    a.cap = l.cap;
+    a.ucap = l.ucap;
+    a.raw = l.raw;
+    a.agg = l.agg;
+    a.cs = l.cs;
+    a.ap = l.ap;
+    a.p1 = l.p1;
+    a.p2 = l.p2;
+    const size_t cells = (size_t)l.W * l.H;
+    // AllocLineBuffers (:158-165): L and Min 2000, range starts 0, range caps Align, guidance 0; Agg 0 (:56)
+    const unsigned fb = 1024;
+    lit_fill32<<<fb, 256, 0, s>>>((uint32_t*)a.L, (size_t)(a.Llen + 1) / 2, 0x07d007d0u);
+    lit_fill32<<<fb, 256, 0, s>>>((uint32_t*)a.Min, (size_t)a.halfM, 0x07d007d0u);
+    lit_fill16<<<fb, 256, 0, s>>>(a.RS, (size_t)2 * a.halfR, 0);
+    lit_fill16<<<fb, 256, 0, s>>>(a.RC, (size_t)2 * a.halfR, (uint16_t)l.align);
+    lit_fill32<<<fb, 256, 0, s>>>(a.Pix, (size_t)2 * a.halfR, 0u);
+    lit_fill16<<<fb, 256, 0, s>>>(l.agg, cells * l.cs, 0);
+    lit_fill32<<<1, 64, 0, s>>>(a.ticket, 2, 0u);
+    hipError_t e = hipGetLastError();
+    const int grid = std::min(l.H, kLitGrid);
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        lit_fill32<<<fb, 256, 0, s>>>(a.progress, (size_t)l.H, 0u);
+        lit_fill32<<<1, 64, 0, s>>>(a.ticket, 1, 0u);  // the error word (ticket + 1) stays set across passes
+        e = lit_run_pass(pass == 0 ? 1 : -1, l.npl, l.ap.on != 0, a, grid, s);
+        if (e == hipSuccess && pass == 0 && agg_pass1)
+            e = hipMemcpyAsync(agg_pass1, l.agg, cells * l.cs * 2, hipMemcpyDeviceToDevice, s);
+    }
+    if (e == hipSuccess && wta) {
+        lit_wta<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(l.agg, l.cs, l.start, l.count, l.ucount, cells, wta);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t literal_error(void* scratch, int W, int H, int maxhi, hipStream_t s, uint32_t* err)
+{
+    const LitLayout lay = lit_layout(W, H, maxhi);
+    hipError_t e = hipMemcpyAsync(err, (char*)scratch + lay.ctl + 4, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+}  // namespace lux

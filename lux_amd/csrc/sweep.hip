@@ -590,21 +590,23 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         if (err) *err = msg;
         return code;
     };
+    const int ucap = ((a.n + p.align - 1) / p.align) * p.align;
+    bool spill = false;  // some range's padding passes its line-buffer segment (start - MinLo + cap > MaxHi)
+    int maxhi_r = ucap, maxcap = ucap;
     if (ranged) {
-        const int maxhi_r = ((rs.maxhi + p.align - 1) / p.align) * p.align;
+        maxhi_r = ((rs.maxhi + p.align - 1) / p.align) * p.align;
+        maxcap = rs.maxcap;
         if (rs.bad) return refuse(LUX_SGM_EINVAL, "a range passes the last plane");
         if (rs.maxcap == 0) return refuse(LUX_SGM_EINVAL, "every range is empty");
-        if (rs.maxcap > 128) return refuse(LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 128 lanes");
-        if (rs.maxend - rs.minlo > maxhi_r)
-            return refuse(LUX_SGM_EUNSUPPORTED,
-                          "a range's padding reaches the end of its line-buffer segment (start - MinLo + cap > MaxHi): "
-                          "the reference then writes into the next path's segment, which the device path does not model");
+        if (rs.maxcap > 256) return refuse(LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 256 lanes");
+        spill = rs.maxend - rs.minlo > maxhi_r;
     }
-    if (!ranged && ((a.n + p.align - 1) / p.align) * p.align > 1024)
-        return refuse(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
+    if (!ranged && ucap > 1024) return refuse(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
+    // the path sweeps, or the line-buffer-literal path for spilling padding, caps of 129..256 and odd tiles
+    const bool literal = maxcap <= 256 && (ws.force_literal || spill || (p.tile & 1) || (ranged && maxcap > 128));
     const Geometry g = geometry(W, H, ranged ? rs.maxcount : a.n, p.align);
-    int rc;
-    if ((rc = prepare(ws, g, p.tile, 0, false, s, w, err))) return rc;
+    int rc = literal ? prepare_literal(ws, g, w, err) : prepare(ws, g, p.tile, 0, false, s, w, err);
+    if (rc) return rc;
     if (ranged) {
         void *ss, *sc;
         if ((rc = ws.get("seedstart", (size_t)3 * W * 2, &ss, err)) || (rc = ws.get("seedcap", (size_t)3 * W * 2, &sc, err)))
@@ -615,7 +617,6 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         w->b.seedstart = (uint16_t*)ss;
         w->b.seedcap = (uint16_t*)sc;
     }
-    const int ucap = ((a.n + p.align - 1) / p.align) * p.align;
     const double mean_cap = ranged ? (double)(rs.cap_computed + rs.cap_skipped) / ((double)W * H) : (double)ucap;
     hipError_t e = launch_sweep_cost(a, W, H, p.scale, d_start, d_count, d_cap, a.n, ucap, mean_cap, d_skip, w->raw, g.cs, s,
                                      &ws, rs.computed);
@@ -631,14 +632,36 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
     w->b.ap.base = (float)p.p1 * p.p2_scale;  // p2Base (:253), one float multiply
     w->b.ap.on = 1;
     w->b.seedguid = (uint32_t*)sg;
-    if (ranged) try_concurrency(ws, w, conc);  // ranged layers are latency-bound: their sweeps run side by side
-    SgmOutputs o;
-    o.disp = d_wta;
-    o.tol = 0;
-    o.write_agg = keep_agg;
-    o.bands = 0;
-    e = launch_sgm(g, w->b, o, p.p1, 0, agg_pass1, s);
-    if (e != hipSuccess) return hip_status(e, "aggregation", err);
+    if (literal) {
+        LiteralLayer l;
+        l.W = W;
+        l.H = H;
+        l.T = p.tile;
+        l.align = p.align;
+        l.minlo = ranged ? rs.minlo : 0;
+        l.maxhi = maxhi_r;
+        l.start = d_start;
+        l.cap = d_cap;
+        l.count = d_count;
+        l.ucap = ucap;
+        l.ucount = a.n;
+        l.raw = w->raw;
+        l.agg = w->agg;
+        l.cs = g.cs;
+        l.ap = w->b.ap;
+        l.p1 = p.p1;
+        l.p2 = 0;
+        if ((rc = run_literal(ws, l, maxcap, agg_pass1, d_wta, s, err))) return rc;
+    } else {
+        if (ranged) try_concurrency(ws, w, conc);  // ranged layers are latency-bound: their sweeps run side by side
+        SgmOutputs o;
+        o.disp = d_wta;
+        o.tol = 0;
+        o.write_agg = keep_agg;
+        o.bands = 0;
+        e = launch_sgm(g, w->b, o, p.p1, 0, agg_pass1, s);
+        if (e != hipSuccess) return hip_status(e, "aggregation", err);
+    }
     const size_t cells = (size_t)W * H;
     depth_kernel<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(d_wta, a.planes, a.n, cells, d_depth);
     e = hipGetLastError();

@@ -32,6 +32,7 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
 MEM_HOST, MEM_DEVICE = 0, 1
 REQ_PENDING, REQ_RUNNING, REQ_DONE = 0, 1, 2
 OPT_FUSED_BANDS = 1
+OPT_LITERAL = 2
 STAGES = ("census", "rows+", "vd+1", "vd+2", "vd+3", "replay+", "vd-1", "vd-2", "vd-3", "rows-", "replay-", "finish")
 
 

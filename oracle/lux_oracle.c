@@ -264,56 +264,85 @@ static void Pixel(Layer* s, int x, int y, int dir, int cur, int other)
     s->RangeC[cur * s->halfR + slot] = (uint16_t)cap;
 }
 
-int lo_layer_compute(const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed,
-                     uint16_t* agg_pass1_packed, uint16_t* wta)
+/* Init (:34-62) and AllocLineBuffers (:158-165) of one layer. */
+static int layer_init(Layer* s, const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed)
 {
     if (!p || !raw_packed || !agg_packed || !p->start || !p->count || p->W <= 0 || p->H <= 0 || p->T <= 0 ||
         p->align <= 0 || p->p1 < 0 || p->p1 > 65535)
         return -1;
     if (p->p2_const < 0 && !p->guidance) return -1;
-    Layer s;
-    memset(&s, 0, sizeof s);
-    s.p = p;
-    s.W = p->W;
-    s.H = p->H;
-    int W = s.W, H = s.H, npx = W * H, align = p->align;
-    s.cap = (uint16_t*)malloc(sizeof(uint16_t) * npx);
-    s.off = (uint64_t*)malloc(sizeof(uint64_t) * (npx + 1));
+    memset(s, 0, sizeof *s);
+    s->p = p;
+    s->W = p->W;
+    s->H = p->H;
+    int W = s->W, H = s->H, npx = W * H, align = p->align;
+    s->cap = (uint16_t*)malloc(sizeof(uint16_t) * npx);
+    s->off = (uint64_t*)malloc(sizeof(uint64_t) * (npx + 1));
     /* ranges: Init (:43-58); MinLo = min start, MaxHi = roundup(max(start+count), align) (:46, :51, :136) */
     int minLo = 65535, maxHi = 0, maxCap = 0;
     uint64_t o = 0;
     for (int i = 0; i < npx; i++) {
         int cap = ((p->count[i] + align - 1) / align) * align;
-        s.cap[i] = (uint16_t)cap;
-        s.off[i] = o;
+        s->cap[i] = (uint16_t)cap;
+        s->off[i] = o;
         o += (uint64_t)cap;
         minLo = mini(minLo, p->start[i]);
         maxHi = maxi(maxHi, p->start[i] + p->count[i]);
         maxCap = maxi(maxCap, cap);
     }
-    s.off[npx] = o;
-    s.MinLo = minLo;
-    s.MaxHi = ((maxHi + align - 1) / align) * align;
-    s.raw = raw_packed;
-    s.agg = agg_packed;
+    s->off[npx] = o;
+    s->MinLo = minLo;
+    s->MaxHi = ((maxHi + align - 1) / align) * align;
+    s->raw = raw_packed;
+    s->agg = agg_packed;
     memset(agg_packed, 0, sizeof(uint16_t) * o);
     /* AllocLineBuffers (:158-165) */
     int slots = W + 2;
-    s.slotStride = 4 * s.MaxHi + 5;
-    s.halfL = slots * s.slotStride;
-    s.Llen = 2L * s.halfL;
-    s.L = (uint16_t*)malloc(sizeof(uint16_t) * s.Llen);
-    for (long i = 0; i < s.Llen; i++) s.L[i] = 2000;
-    s.halfM = slots * 4;
-    s.Min = (uint16_t*)malloc(sizeof(uint16_t) * 2 * s.halfM);
-    for (int i = 0; i < 2 * s.halfM; i++) s.Min[i] = 2000;
-    s.halfR = slots;
-    s.RangeS = (uint16_t*)calloc(2 * slots, sizeof(uint16_t));
-    s.RangeC = (uint16_t*)malloc(sizeof(uint16_t) * 2 * slots);
-    for (int i = 0; i < 2 * slots; i++) s.RangeC[i] = (uint16_t)align;
-    s.Pix = (float*)calloc((size_t)2 * slots * 4, sizeof(float));
-    s.rawtmp = (uint8_t*)malloc((size_t)maxCap + 8);
-    int T = p->T;
+    s->slotStride = 4 * s->MaxHi + 5;
+    s->halfL = slots * s->slotStride;
+    s->Llen = 2L * s->halfL;
+    s->L = (uint16_t*)malloc(sizeof(uint16_t) * s->Llen);
+    for (long i = 0; i < s->Llen; i++) s->L[i] = 2000;
+    s->halfM = slots * 4;
+    s->Min = (uint16_t*)malloc(sizeof(uint16_t) * 2 * s->halfM);
+    for (int i = 0; i < 2 * s->halfM; i++) s->Min[i] = 2000;
+    s->halfR = slots;
+    s->RangeS = (uint16_t*)calloc(2 * slots, sizeof(uint16_t));
+    s->RangeC = (uint16_t*)malloc(sizeof(uint16_t) * 2 * slots);
+    for (int i = 0; i < 2 * slots; i++) s->RangeC[i] = (uint16_t)align;
+    s->Pix = (float*)calloc((size_t)2 * slots * 4, sizeof(float));
+    s->rawtmp = (uint8_t*)malloc((size_t)maxCap + 8);
+    (void)H;
+    return 0;
+}
+
+/* first-minimum WTA (:225-231); the plane index start+arg */
+static void layer_wta(const Layer* s, uint16_t* wta)
+{
+    const lo_layer_params* p = s->p;
+    for (int i = 0; i < s->W * s->H; i++) {
+        const uint16_t* a = s->agg + s->off[i];
+        int cnt = p->count[i], best = 0xffff, arg = 0;
+        for (int k = 0; k < cnt; k++)
+            if (a[k] < best) { best = a[k]; arg = k; }
+        wta[i] = (uint16_t)(p->start[i] + arg);
+    }
+}
+
+static int layer_finish(Layer* s, uint16_t* wta)
+{
+    if (!s->fault && wta) layer_wta(s, wta);
+    int fault = s->fault;
+    free(s->cap); free(s->off); free(s->L); free(s->Min); free(s->RangeS); free(s->RangeC); free(s->Pix); free(s->rawtmp);
+    return fault ? -2 : 0;
+}
+
+int lo_layer_compute(const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed,
+                     uint16_t* agg_pass1_packed, uint16_t* wta)
+{
+    Layer s;
+    if (layer_init(&s, p, raw_packed, agg_packed)) return -1;
+    int W = s.W, H = s.H, T = p->T;
     /* Compute (:206-224): pass +1 then pass -1 over the sheared T x T wavefront tiles */
     for (int pass = 0; pass < 2 && !s.fault; pass++) {
         int dir = pass == 0 ? 1 : -1;
@@ -332,19 +361,24 @@ int lo_layer_compute(const lo_layer_params* p, const uint8_t* raw_packed, uint16
                     for (int x = xs; x != xe && !s.fault; x += dir) Pixel(&s, x, y, dir, cur, other);
                 }
             }
-        if (dir > 0 && agg_pass1_packed) memcpy(agg_pass1_packed, agg_packed, sizeof(uint16_t) * o); /* :223 */
+        if (dir > 0 && agg_pass1_packed) memcpy(agg_pass1_packed, agg_packed, sizeof(uint16_t) * s.off[W * H]); /* :223 */
     }
-    if (!s.fault && wta) /* first-minimum WTA (:225-231); the plane index start+arg */
-        for (int i = 0; i < npx; i++) {
-            const uint16_t* a = agg_packed + s.off[i];
-            int cnt = p->count[i], best = 0xffff, arg = 0;
-            for (int k = 0; k < cnt; k++)
-                if (a[k] < best) { best = a[k]; arg = k; }
-            wta[i] = (uint16_t)(p->start[i] + arg);
-        }
-    int fault = s.fault;
-    free(s.cap); free(s.off); free(s.L); free(s.Min); free(s.RangeS); free(s.RangeC); free(s.Pix); free(s.rawtmp);
-    return fault ? -2 : 0;
+    return layer_finish(&s, wta);
+}
+
+int lo_layer_compute_visits(const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed,
+                            uint16_t* agg_pass1_packed, uint16_t* wta, const int32_t* visits, int32_t n_plus,
+                            int32_t n_minus)
+{
+    Layer s;
+    if (!visits || n_plus < 0 || n_minus < 0 || layer_init(&s, p, raw_packed, agg_packed)) return -1;
+    for (int k = 0; k < n_plus + n_minus && !s.fault; k++) {
+        const int32_t* v = visits + 3 * (size_t)k;
+        if (v[0] < 0 || v[0] >= s.W || v[1] < 0 || v[1] >= s.H || (v[2] & ~1)) { s.fault = 1; break; }
+        Pixel(&s, v[0], v[1], k < n_plus ? 1 : -1, v[2], 1 - v[2]);
+        if (k + 1 == n_plus && agg_pass1_packed) memcpy(agg_pass1_packed, agg_packed, sizeof(uint16_t) * s.off[s.W * s.H]);
+    }
+    return layer_finish(&s, wta);
 }
 
 /* ---------------------------------------------------------------- WTA + subpixel + L/R */

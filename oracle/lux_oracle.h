@@ -67,6 +67,12 @@ typedef struct {
 /* Returns 0 on success, -1 on bad arguments, -2 when the reference would fault (out-of-range write). */
 int lo_layer_compute(const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed,
                      uint16_t* agg_pass1_packed /* may be NULL */, uint16_t* wta /* W*H plane index */);
+/* The same layer with its visits in a caller-given order: visits[3k..3k+2] = (x, y, cur half), the first n_plus
+ * in pass +1, the rest in pass -1. In the reference's order (lo_layer_compute's loop) it gives the same results;
+ * the tests use it to run orders a parallel schedule allows. Returns -2 on a visit outside the image. */
+int lo_layer_compute_visits(const lo_layer_params* p, const uint8_t* raw_packed, uint16_t* agg_packed,
+                            uint16_t* agg_pass1_packed, uint16_t* wta, const int32_t* visits, int32_t n_plus,
+                            int32_t n_minus);
 
 /* The image-adaptive P2 of one path step (DenseStereoSgm.cs:267-273): S = (|d0|+|d2|)+|d1| with
  * d = (g - q)*gw, e = clamp(-S, -126, 128), E = fast exp2(e) (cubic + exponent add), trunc(p2Base*E). */

@@ -25,6 +25,7 @@ def lib():
         L.lo_cost_ad.argtypes = [vp, vp, i, i, i, i, i, i, i, i, vp, i]
         L.lo_box3.argtypes = [vp, i, i, i, i, vp]
         L.lo_layer_compute.argtypes = [vp, vp, vp, vp, vp]
+        L.lo_layer_compute_visits.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32]
         L.lo_wta_sub_lr.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.lo_p2_adaptive.argtypes = [f, vp, vp, vp]
         L.lo_set_force_scalar.argtypes = [i]
@@ -82,9 +83,10 @@ def box3(img, maxval):
 
 
 def layer(raw_packed, W, H, start, count, T=64, align=8, p1=10, p2=120, xsw=False, skip=None, guidance=None,
-          gw=(0.0, 0.0, 0.0, 0.0), p2_scale=0.0, adaptive=False, keep_pass1=False):
+          gw=(0.0, 0.0, 0.0, 0.0), p2_scale=0.0, adaptive=False, keep_pass1=False, visits=None, n_plus=0):
     """DenseLayer.Compute restated. raw_packed: uint8 flat, cap entries per pixel (raster order).
-    Returns (agg_packed uint16 flat, agg_pass1 or None, wta uint16 [H, W])."""
+    visits (optional, int32 [n, 3] of x, y, cur half; the first n_plus in pass +1): run Pixel in that order
+    instead of the reference's tile loop. Returns (agg_packed uint16 flat, agg_pass1 or None, wta uint16 [H, W])."""
     start = np.ascontiguousarray(start, np.uint16).reshape(-1)
     count = np.ascontiguousarray(count, np.uint16).reshape(-1)
     caps = ((count.astype(np.int64) + align - 1) // align) * align
@@ -103,7 +105,12 @@ def layer(raw_packed, W, H, start, count, T=64, align=8, p1=10, p2=120, xsw=Fals
     g = None if guidance is None else np.ascontiguousarray(guidance, np.uint8)
     sk = None if skip is None else np.ascontiguousarray(skip, np.uint8).reshape(-1)
     lp.guidance, lp.start, lp.count, lp.skip = _p(g), _p(start), _p(count), _p(sk)
-    rc = lib().lo_layer_compute(ctypes.byref(lp), _p(raw_packed), _p(agg), _p(agg1), _p(wta))
+    if visits is None:
+        rc = lib().lo_layer_compute(ctypes.byref(lp), _p(raw_packed), _p(agg), _p(agg1), _p(wta))
+    else:
+        v = np.ascontiguousarray(visits, np.int32).reshape(-1, 3)
+        rc = lib().lo_layer_compute_visits(ctypes.byref(lp), _p(raw_packed), _p(agg), _p(agg1), _p(wta), _p(v),
+                                           int(n_plus), int(v.shape[0] - n_plus))
     if rc != 0:
         raise RuntimeError(f"lo_layer_compute failed: {rc}")
     return agg, agg1, wta.reshape(H, W)

@@ -231,3 +231,115 @@ def wta_sub_lr_model(agg, D, tol=1):
             xr = x - dl[x]
             valid[y, x] = 1 if (xr >= 0 and abs(dl[x] - dr[xr]) <= tol) else 0
     return disp, sub, valid
+
+
+# ------------------------------------------------------------------ the line-buffer-literal schedule
+# A restatement of literal.hip's row sequences and waits (lit_rev, lit_before, the reach of 3 columns), so the
+# tests can check on the CPU that every order the waits allow gives the reference's results.
+LIT_REACH = 3
+
+
+def literal_rows(W, H, T, xsw, d):
+    """Per row in pass order: the list of (x, y, cur, key) the kernel's wavefront visits, key = (j, i, r)."""
+    gwN = (W + 2 * T - 1) // T
+    rows = []
+    for row in range(H):
+        j, r = divmod(row, T)
+        y = row if d > 0 else H - 1 - row
+        cur = 1 if r % 2 == 0 else 0
+        rev = (d < 0 or not xsw) and (gwN - 1) * T - r >= W
+        seq = []
+        for pos in range(W + int(rev)):
+            c = pos if pos < W else W - 1
+            i = (pos + r) // T if pos < W else gwN - 1
+            seq.append((c if d > 0 else W - 1 - c, y, cur, (j, i, r)))
+        rows.append(seq)
+    return rows
+
+
+def literal_need(W, H, T, xsw, d, row, pos, row2):
+    """How many of row2's visits the kernel waits for before row's visit pos."""
+    gwN = (W + 2 * T - 1) // T
+    j, r = divmod(row, T)
+    jj, rr = divmod(row2, T)
+    c = pos if pos < W else W - 1
+    i = (pos + r) // T if pos < W else gwN - 1
+    rev2 = int((d < 0 or not xsw) and (gwN - 1) * T - rr >= W)
+    if jj < j:
+        before = W + rev2
+    elif jj > j:
+        before = 0
+    elif rr < r and i == gwN - 1:
+        before = W + rev2
+    else:
+        before = min(max(((i + 1) if rr < r else i) * T - rr, 0), W)
+    mp = min(c + LIT_REACH, W - 1)
+    if rev2 and c + LIT_REACH >= W - 1:
+        mp = W
+    return min(before, mp + 1)
+
+
+def literal_depth(W, T):
+    return T + 4
+
+
+def literal_wait_rows(W, H, T, row, pos):
+    """The rows a visit waits on: rows row-2 .. row+2, and for the second visit of the border column (pos W) every
+    row of its band, since the buffers it reads were last written by whichever rows of the band came before."""
+    rows = {r2 for r2 in (row - 2, row - 1, row + 1, row + 2) if 0 <= r2 < H}
+    if pos >= W:
+        j = row // T
+        rows |= set(range(j * T, min(H, j * T + T)))
+        rows.discard(row)
+    return sorted(rows)
+
+
+def literal_interleave(W, H, T, xsw, rng, resident):
+    """One order of both passes that the kernel's waits allow: `resident` wavefronts take rows by ticket and a
+    random runnable one advances. Returns the visits as (x, y, cur) rows, and n_plus. Raises on a deadlock."""
+    out = []
+    n_plus = 0
+    for d in (1, -1):
+        rows = literal_rows(W, H, T, xsw, d)
+        prog = [0] * H
+        nxt = 0
+        active = []
+        while nxt < H and len(active) < resident:
+            active.append(nxt)
+            nxt += 1
+        while active:
+            ready = []
+            for row in active:
+                pos = prog[row]
+                if all(prog[r2] >= literal_need(W, H, T, xsw, d, row, pos, r2)
+                       for r2 in literal_wait_rows(W, H, T, row, pos)):
+                    ready.append(row)
+            if not ready:
+                raise RuntimeError(f"deadlock: W={W} H={H} T={T} d={d} resident={resident} active={active}")
+            row = ready[int(rng.integers(len(ready)))]
+            x, y, cur, _ = rows[row][prog[row]]
+            out.append((x, y, cur))
+            prog[row] += 1
+            if prog[row] == len(rows[row]):
+                active.remove(row)
+                if nxt < H:
+                    active.append(nxt)
+                    nxt += 1
+        if d > 0:
+            n_plus = len(out)
+    return np.array(out, np.int32), n_plus
+
+
+def spill_ranges(rng, W, H, planes, cmax, align):
+    """Per-pixel (start, count) whose padding spills past the line-buffer segments: MinLo = 0, many ranges ending
+    at the top plane with counts that are not multiples of the alignment, and ranges starting at plane 0 that read
+    the guard and low words the spills write."""
+    count = rng.integers(1, cmax + 1, (H, W)).astype(np.int64)
+    start = (rng.random((H, W)) * (planes - count + 1)).astype(np.int64)
+    u = rng.random((H, W))
+    start[u < 0.3] = planes - count[u < 0.3]
+    start[u > 0.8] = 0
+    start[0, 0] = 0
+    count[H - 1, W - 1] = max(1, cmax - (cmax % align == 0))
+    start[H - 1, W - 1] = planes - count[H - 1, W - 1]
+    return start.astype(np.uint16), count.astype(np.uint16)

@@ -134,14 +134,95 @@ def test_layer_ranges(W, H, T, planes, cmax, align, smooth):
             np.testing.assert_array_equal(L.Wta, wta)
 
 
-def test_layer_ranges_refuses_padding_overflow():
+def _ranged_raw(rng, count, align):
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    raws = []
+    for i in range(count.size):
+        r = rng.integers(0, 49, int(caps.flat[i])).astype(np.uint8)
+        r[int(count.flat[i]):] = 255
+        raws.append(r)
+    return np.concatenate(raws)
+
+
+def _spills(start, count, align):
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    maxhi = -(-int((start.astype(np.int64) + count).max()) // align) * align
+    return int((start.astype(np.int64) - int(start.min()) + caps > maxhi).sum())
+
+
+def test_layer_ranges_padding_overflow_small():
+    # MinLo 0 and ranges [5, 8) with cap 8: 5 - 0 + 8 > 8, the padding runs into the next segment
     start = np.array([[0, 0, 0, 0], [0, 5, 5, 0], [0, 5, 5, 0]], np.uint16)
     count = np.array([[8, 8, 8, 8], [8, 3, 3, 8], [8, 3, 3, 8]], np.uint16)
+    raw = _ranged_raw(np.random.default_rng(3), count, 8)
     with sgm.Context(0) as c:
-        L = sgm.DenseLayer(c, 4, 3, 16, align=8, tile=2).set_ranges(start, count)
-        L.Raw = np.zeros(int(L.Cap.astype(np.int64).sum()), np.uint8)
-        with pytest.raises(sgm.NotSupported):
-            L.compute()
+        L = sgm.DenseLayer(c, 4, 3, 16, align=8, p1=8, p2=96, tile=2).set_ranges(start, count)
+        L.Raw = raw
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(raw, 4, 3, start, count, T=2, align=8, p1=8, p2=96, keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1)
+    np.testing.assert_array_equal(L.Agg, agg)
+    np.testing.assert_array_equal(L.Wta, wta)
+
+
+# W, H, T, planes, cmax, align, adaptive: padding past the segments, odd tiles, caps above 128 (4 planes a lane)
+SPILL_LAYERS = [(9, 7, 4, 40, 13, 8, False), (13, 11, 3, 24, 11, 8, False), (16, 12, 5, 30, 9, 2, True),
+                (70, 40, 64, 96, 45, 8, True), (130, 70, 16, 160, 77, 8, False), (200, 90, 8, 300, 130, 8, True),
+                (41, 33, 7, 64, 61, 8, False), (300, 120, 64, 120, 40, 8, True)]
+
+
+@pytest.mark.parametrize("W,H,T,planes,cmax,align,adaptive", SPILL_LAYERS)
+def test_layer_ranges_spilling_padding(W, H, T, planes, cmax, align, adaptive):
+    rng = np.random.default_rng(W * 1000 + H + T)
+    start, count = M.spill_ranges(rng, W, H, planes, cmax, align)
+    assert _spills(start, count, align) > 0
+    raw = _ranged_raw(rng, count, align)
+    guid = rng.integers(0, 256, (H, W, 4)).astype(np.uint8) if adaptive else None
+    print(f"spill W={W} H={H} T={T} planes={planes} cmax={cmax} align={align} adaptive={adaptive}", flush=True)
+    kw = dict(T=T, align=align, keep_pass1=True)
+    if adaptive:
+        kw.update(p1=1, guidance=guid, gw=L16_WEIGHTS, p2_scale=500.0, adaptive=True)
+    else:
+        kw.update(p1=8, p2=96)
+    with sgm.Context(0) as c:
+        L = sgm.DenseLayer(c, W, H, planes, align=align, p1=kw["p1"], p2=kw.get("p2", 0), tile=T)
+        L.set_ranges(start, count)
+        if adaptive:
+            L.set_guidance(guid, L16_WEIGHTS, 500.0)
+        L.Raw = raw
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(raw, W, H, start, count, **kw)
+    np.testing.assert_array_equal(L.AggPass1, agg1)
+    np.testing.assert_array_equal(L.Agg, agg)
+    np.testing.assert_array_equal(L.Wta, wta)
+
+
+@pytest.mark.parametrize("W,H,T,planes", [(9, 13, 3, 16), (33, 12, 5, 64), (70, 40, 63, 24), (17, 17, 9, 200)])
+def test_layer_odd_tiles(ctx, W, H, T, planes):
+    _layer_case(ctx, W, H, planes, T, 10, 120, seed=W + H + T)
+    _layer_case(ctx, W, H, planes, T, 10, 120, seed=W + H + T + 1, xsw=True)
+
+
+def test_layer_literal_path_forced():
+    """The line-buffer-literal path on layers the path sweeps also take (LUX_SGM_OPT_LITERAL): the same results."""
+    with sgm.Context(0) as c:
+        c.set_option(sgm.OPT_LITERAL, 1)
+        for W, H, T, planes in ((7, 9, 4, 16), (130, 70, 64, 96), (64, 33, 8, 256)):
+            _layer_case(c, W, H, planes, T, 8, 96, seed=W * H)
+        rng = np.random.default_rng(8)
+        W, H, T, planes, cmax, align = 90, 50, 16, 128, 60, 8
+        start, count = M.random_ranges(rng, W, H, planes, cmax, align, True)
+        raw = _ranged_raw(rng, count, align)
+        L = sgm.DenseLayer(c, W, H, planes, align=align, p1=8, p2=96, tile=T).set_ranges(start, count)
+        L.Raw = raw
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(raw, W, H, start, count, T=T, align=align, p1=8, p2=96, keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1)
+    np.testing.assert_array_equal(L.Agg, agg)
+    np.testing.assert_array_equal(L.Wta, wta)
 
 
 # ------------------------------------------------------------------ coarse-to-fine requests
@@ -443,6 +524,34 @@ def test_dense_layer_ranged(seed, H0, W0, nv, n, scale, tile, align, skipmode):
     np.testing.assert_array_equal(got["depth"].view(np.uint32), want["depth"].view(np.uint32))
 
 
+@pytest.mark.parametrize("seed,H0,W0,nv,n,scale,tile,align,skipmode", [(7, 96, 128, 3, 120, 2, 8, 8, 0),
+                                                                      (8, 80, 120, 2, 90, 1, 15, 8, 2),
+                                                                      (9, 72, 100, 4, 60, 2, 4, 2, 2)])
+def test_dense_layer_spilling_padding(seed, H0, W0, nv, n, scale, tile, align, skipmode):
+    """MinLo 0 and ranges ending at the top plane with padding: the line-buffer-literal path, bit for bit."""
+    ref, others, warps, wqs, pl = _scene_views(seed, H0, W0, nv, n)
+    p = sgm.dense_params(W0, H0, scale, tile, align)
+    W, H = p.width, p.height
+    rng = np.random.default_rng(seed)
+    count = rng.integers(1, 40, (H, W)).astype(np.int64)
+    start = rng.integers(0, n - count + 1)
+    top = rng.random((H, W)) < 0.3
+    start[top] = n - count[top]
+    start[0, 0] = 0
+    start, count = start.astype(np.uint16), count.astype(np.uint16)
+    assert _spills(start, count, align) > 0
+    skip = sgm.skip_mask(skipmode, W, H)
+    cc = sgm.CostContext(ref, others, warps, wqs, pl)
+    print(f"dense spill W={W} H={H} n={n} tile={tile} align={align}", flush=True)
+    with sgm.Context(0) as c:
+        got = sgm.dense_layer(c, cc, p, start, count, skip, want_agg=True)
+    sc = o.SweepContext(ref, others, warps, wqs, pl)
+    want = o.dense_layer(sc, W, H, scale, tile, align, start, count, skip, gw=sgm.L16_GUIDANCE)
+    np.testing.assert_array_equal(got["agg"], want["agg"])
+    np.testing.assert_array_equal(got["wta"], want["wta"])
+    np.testing.assert_array_equal(got["depth"].view(np.uint32), want["depth"].view(np.uint32))
+
+
 @pytest.mark.parametrize("seed,pw,ph,w,h,n,skipped", [(1, 16, 12, 32, 24, 64, False), (2, 65, 49, 130, 98, 300, True),
                                                        (3, 9, 7, 9, 7, 20, True), (4, 40, 30, 81, 60, 1000, True)])
 def test_range_from_previous(seed, pw, ph, w, h, n, skipped):
@@ -471,6 +580,31 @@ def test_dense_pyramid_matches_oracle(W0, H0, seed):
         assert lv["minlo"] >= 7
     import torch
 
+    with sgm.Context(0) as c:
+        depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2)
+        d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+        d_depths = [torch.full(d.shape, -1.0, dtype=torch.float32, device="cuda") for d in depths]
+        aligns_d = sgm.dense_pyramid_device(c, [t.data_ptr() for t in d_imgs], [im.shape[:2] for im in imgs], gray, warps,
+                                            p8, p2, [t.data_ptr() for t in d_depths])
+        torch.cuda.synchronize()
+    assert aligns == [lv["align"] for lv in want] == aligns_d
+    for i, (d, dd, lv) in enumerate(zip(depths, d_depths, want)):
+        np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+        np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+
+
+@pytest.mark.parametrize("W0,H0,seed,near,far", [(256, 192, 1, 1000.0, 2200.0), (208, 152, 2, 1000.0, 2200.0),
+                                                 (256, 192, 1, 1500.0, 2250.0)])
+def test_dense_pyramid_spilling_padding(W0, H0, seed, near, far):
+    """A plane range whose far end sits on the background: MinLo 0 on the finer levels and ranges ending at the top
+    plane, so padding runs past the line-buffer segments as in real captures. Six levels, bit for bit."""
+    imgs, gray, warps, _ = scene.multiview(W0, H0, seed=seed)
+    p8, p2 = scene.plane_list(near, far, 48), scene.plane_list(near, far, 46)
+    want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
+    assert sum(_spills(lv["start"], lv["count"], lv["align"]) for lv in want[1:]) > 0
+    import torch
+
+    print(f"dense pyramid spill {W0}x{H0} planes {near}..{far}", flush=True)
     with sgm.Context(0) as c:
         depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2)
         d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
@@ -533,10 +667,10 @@ def test_plane_sweep_entry_points_refuse_bad_input():
         big = sgm.CostContext(ref, others, warps, wqs, sweep_planes(1030))
         with pytest.raises(sgm.NotSupported):   # layer 0 above 1024 planes
             sgm.dense_layer(c, big, sgm.dense_params(48, 40, 4, 4))
-        # padding that reaches the end of the line-buffer segment: MinLo 0, a range ending at MaxHi with padding
+        # ranges above 256 lanes
         p = sgm.dense_params(48, 40, 4, 4)
         start = np.zeros((10, 12), np.uint16)
         count = np.full((10, 12), 3, np.uint16)
-        start[5, 5], count[5, 5] = 13, 3    # [13, 16) with cap 8: 13 - 0 + 8 > roundup(16, 8)
+        count[5, 5] = 300
         with pytest.raises(sgm.NotSupported):
-            sgm.dense_layer(c, cc, p, start, count)
+            sgm.dense_layer(c, sgm.CostContext(ref, others, warps, wqs, sweep_planes(320)), p, start, count)
