@@ -120,7 +120,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const
  * MaxHi for every pixel (MinLo = min start, MaxHi = roundup(max(start + count), align)). A layer that breaks
  * it (the reference then writes into the next segment and reads those words back), has a cap above 128 or an
  * odd tile runs on the line-buffer-literal path, which holds the reference's line buffers word for word; caps
- * above 256 get LUX_SGM_EUNSUPPORTED. */
+ * above 1024 get LUX_SGM_EUNSUPPORTED. */
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
                                  const uint16_t* count, const uint8_t* raw, uint16_t* agg, uint16_t* agg_pass1,
                                  uint16_t* wta);
@@ -187,8 +187,8 @@ typedef struct lux_sgm_dense_params {
  * reference subsampled at the layer's scale, :39), the first-minimum WTA and depth = Planes[start + arg] [H*W].
  * start/count both null: every pixel searches all planes (layer 0). skip may be null. agg / agg_pass1 (packed like
  * the raw volume) and wta (the plane index) may be null. Layers whose padding passes the line-buffer segment, with
- * caps of 129..256 lanes or an odd tile run on the line-buffer-literal path (lux_sgm_compute_layer_ranges).
- * Returns LUX_SGM_EUNSUPPORTED for ranges above 256 lanes and for more than 1024 planes at layer 0. */
+ * caps of 129..1024 lanes or an odd tile run on the line-buffer-literal path (lux_sgm_compute_layer_ranges).
+ * Returns LUX_SGM_EUNSUPPORTED for ranges above 1024 lanes and for more than 1024 planes at layer 0. */
 int lux_sgm_dense_layer(lux_sgm_ctx* ctx, const lux_sgm_sweep* sweep, const lux_sgm_dense_params* params,
                         const uint16_t* start, const uint16_t* count, const uint8_t* skip, float* depth, uint16_t* agg,
                         uint16_t* agg_pass1, uint16_t* wta);
@@ -289,7 +289,7 @@ int lux_sgm_pyramid_device(lux_sgm_ctx* ctx, const lux_sgm_pyramid_params* p, co
  * LUX_SGM_OPT_FUSED_BANDS: 1 runs the three vertical/diagonal paths of each pass fused in band tiles that
  * synchronise through device flags (up to 256 planes); 0 (default) runs one sweep kernel per path. */
 #define LUX_SGM_OPT_FUSED_BANDS 1
-/* LUX_SGM_OPT_LITERAL: 1 runs every layer of at most 256 lanes (lux_sgm_compute_layer*, lux_sgm_dense_layer,
+/* LUX_SGM_OPT_LITERAL: 1 runs every layer of at most 1024 lanes (lux_sgm_compute_layer*, lux_sgm_dense_layer,
  * the dense pyramid) on the line-buffer-literal path: one wavefront per row follows the reference's tile
  * schedule against its own line buffers. Slower; layers that need it take it whatever this option says. */
 #define LUX_SGM_OPT_LITERAL 2

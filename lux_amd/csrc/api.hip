@@ -535,7 +535,7 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         }
         maxhi = ((maxhi + align - 1) / align) * align;
         if (maxcap == 0) return fail(c, LUX_SGM_EINVAL, "every range is empty");
-        if (maxcap > 256) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 256 lanes");
+        if (maxcap > 1024) return fail(c, LUX_SGM_EUNSUPPORTED, "per-pixel ranges above 1024 lanes");
         for (size_t i = 0; i < cells; i++)
             if ((int)start[i] - minlo + (int)cap[i] > maxhi) spill = true;
         lminlo = minlo;
@@ -551,8 +551,8 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
     }
     lux::Workspace& ws = c->ws;
     // The path sweeps take even tiles, caps up to 128 with ranges and ranges whose padding stays inside its
-    // line-buffer segment; the line-buffer-literal path takes the rest up to 256 lanes (and everything when forced).
-    const bool literal = lmaxcap <= 256 && (ws.force_literal || spill || (p->tile & 1) || (ranged && lmaxcap > 128));
+    // line-buffer segment; the line-buffer-literal path takes the rest up to 1024 lanes (and everything when forced).
+    const bool literal = lmaxcap <= 1024 && (ws.force_literal || spill || (p->tile & 1) || (ranged && lmaxcap > 128));
     lux::Prepared w;
     if (literal) rc = lux::prepare_literal(ws, g, &w, &c->err);
     else rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err);

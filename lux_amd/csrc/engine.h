@@ -110,7 +110,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
 
 // The line-buffer-literal path (launch_literal). prepare_literal gives w the raw and aggregate volumes of g (the
 // workspace buffers prepare() uses, so a layer's cost can be written into them first). run_literal runs one layer
-// whose largest cap is maxcap and synchronises s: LUX_SGM_EUNSUPPORTED above 256 lanes or when the device cannot
+// whose largest cap is maxcap and synchronises s: LUX_SGM_EUNSUPPORTED above 1024 lanes or when the device cannot
 // hold the wavefronts the schedule's waits need, LUX_SGM_EDEVICE when a wait gave up.
 int prepare_literal(Workspace& ws, const Geometry& g, Prepared* w, std::string* err);
 int run_literal(Workspace& ws, LiteralLayer l, int maxcap, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s,

@@ -92,7 +92,7 @@ hipError_t launch_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& 
                       hipStream_t s, hipEvent_t* marks = nullptr);
 // Line-buffer-literal aggregation (literal.hip): DenseLayer.Compute's two passes over the reference's own line
 // buffers, word for word, for the layers the path sweeps do not model (padding that spills past its segment,
-// odd tiles, caps up to 256), then the first-minimum WTA.
+// odd tiles, caps up to 1024), then the first-minimum WTA.
 struct LiteralLayer {
     int W = 0, H = 0, T = 0, align = 0, xsw = 0;
     int minlo = 0, maxhi = 0;          // MinLo, MaxHi rounded to align
@@ -103,7 +103,7 @@ struct LiteralLayer {
     const uint8_t* raw = nullptr;      // [H][W][cs], skipped cells' bytes already 0 / 255
     uint16_t* agg = nullptr;           // [H][W][cs]
     int cs = 0;
-    int npl = 2;                       // planes per lane: 2 (caps <= 128) or 4 (caps <= 256)
+    int npl = 2;                       // planes per lane: 2 (caps <= 128), 4 (<= 256), 0 (chunks, <= 1024)
     AdaptP2 ap;
     int p1 = 0, p2 = 0;
 };

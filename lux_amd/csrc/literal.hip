@@ -7,7 +7,7 @@
 //     path-0 segment for path 3), and its stale cleanup later refills the same words (:254-264, :280-313);
 //     successors whose ranges touch those words read them (and Rd's zero past the end of the buffer, :324);
 //   - odd tiles T, whose halves do not alternate row by row across band boundaries (:219);
-//   - ranges of up to 256 planes.
+//   - ranges of 129 to 1024 planes.
 //
 // Schedule. The reference runs visits in the order of the key (band j, tile i, tile row r), row r of tile i
 // covering x in [iT - r, iT + T - r) (mirrored in pass -1), clamped to the image, and tile gwN-1 visiting the
@@ -31,7 +31,8 @@
 //
 // Memory. All line-buffer, minimum, range and guidance words go through agent-scope accesses (global
 // load/store sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
-// (s_waitcnt) before its progress store. One wavefront per row, planes across lanes (NPL per lane).
+// (s_waitcnt) before its progress store. One wavefront per row, planes across lanes (NPL per lane; NPL 0:
+// 64-plane chunks in turn, caps up to 1024).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -232,6 +233,95 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     }
 }
 
+// The same visit for caps up to 1024 (NPL 0): 64-plane chunks one after another, the path costs kept in LDS for
+// the store phase.
+constexpr int kLitMaxCap = 1024;
+
+template <int DIR, bool ADP>
+__device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, uint16_t (*lcs)[kLitMaxCap])
+{
+    const int lane = threadIdx.x;
+    const int slot = x + 1;
+    const size_t cell = (size_t)y * a.W + x;
+    const int start = a.start ? (int)a.start[cell] : 0;
+    const int cap = a.start ? (int)a.cap[cell] : a.ucap;
+    const int q = start - a.minlo;
+    const uint32_t g = ADP ? a.ap.guid[cell] : 0u;
+    const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
+    const size_t cbase = cell * (size_t)a.cs;
+    int minPrev[4], p2p[4];
+    long nb[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
+        const int half = p == 0 ? cur : oth;
+        minPrev[p] = ald(a.Min + half * a.halfM + ns * 4 + p);
+        int P2 = a.p2;
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, ald32(a.Pix + half * a.halfR + ns));
+        p2p[p] = std::min(P2 + minPrev[p], 65535);
+        nb[p] = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
+    }
+    uint32_t run[4] = {0xffffu, 0xffffu, 0xffffu, 0xffffu};
+    for (int k = lane; k < ((cap + 63) & ~63); k += 64) {
+        const bool act = k < cap;
+        int lm[4], l0[4], lp[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            lm[p] = act ? lit_rd(a, nb[p] + k - 1) : 0;
+            l0[p] = act ? lit_rd(a, nb[p] + k) : 0;
+            lp[p] = act ? lit_rd(a, nb[p] + k + 1) : 0;
+        }
+        const int rb = act ? (int)a.raw[cbase + k] : 0;
+        int sum = act ? (int)ald(a.agg + cbase + k) : 0;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int m = std::min(std::min(std::min(lm[p] + a.p1, 65535), l0[p]),
+                                   std::min(std::min(lp[p] + a.p1, 65535), p2p[p]));
+            const int v = std::max(std::min(rb + m, 65535) - minPrev[p], 0);
+            if (act) {
+                lcs[p][k] = (uint16_t)v;
+                run[p] = std::min(run[p], (uint32_t)v);
+                sum += v;
+            }
+        }
+        if (act) ast(a.agg + cbase + k, (uint16_t)std::min(sum, 65535));
+    }
+    const uint32_t mins01 = wave_pmin(run[0] | (run[1] << 16)), mins23 = wave_pmin(run[2] | (run[3] << 16));
+    __syncthreads();
+    const long B = (long)cur * a.halfL + (long)slot * a.stride;
+    const long q0 = sp - a.minlo;
+    long lo[8], hi[8];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const long bc = B + 1 + (long)p * (a.M + 1);
+        lo[2 * p] = bc + std::max(q0, 0L);
+        hi[2 * p] = bc + std::min(q0 + cp, a.Llen - bc);
+        lo[2 * p + 1] = bc + q;
+        hi[2 * p + 1] = bc + q + cap;
+    }
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+        for (long k = lane; k < hi[o] - lo[o]; k += 64) {
+            const long w = lo[o] + k;
+            if (w >= a.Llen) break;
+            bool last = true;
+#pragma unroll
+            for (int o2 = o + 1; o2 < 8; o2++) last = last && !(w >= lo[o2] && w < hi[o2]);
+            if (last) ast(a.L + w, (o & 1) ? lcs[o >> 1][k] : (uint16_t)2000);
+        }
+    }
+    if (lane < 4) {
+        const uint32_t mm = lane < 2 ? mins01 : mins23;
+        const int rm = (lane & 1) ? (int)(mm >> 16) : (int)(mm & 0xffffu);
+        ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));
+    }
+    if (lane == 0) {
+        ast(a.RS + cur * a.halfR + slot, (uint16_t)start);
+        ast(a.RC + cur * a.halfR + slot, (uint16_t)cap);
+        if constexpr (ADP) ast32(a.Pix + cur * a.halfR + slot, g);
+    }
+}
+
 // Lane 0 waits until row2 has completed `need` visits; a spin past the bound (or another wavefront's) sets the
 // error word, after which no wavefront waits again.
 __device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, int& seen, bool& dead)
@@ -262,6 +352,7 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
 {
     const int lane = threadIdx.x;
     const int W = a.W, T = a.T;
+    __shared__ uint16_t lcs[NPL == 0 ? 4 : 1][NPL == 0 ? kLitMaxCap : 1];
     bool dead = false;  // lane 0: the error word is set, waits are skipped so the grid drains
     for (;;) {
         int row = 0;
@@ -292,7 +383,8 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth);
+            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, lcs);
+            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth);
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane == 0) ast32(a.progress + row, (uint32_t)(pos + 1));
@@ -372,7 +464,8 @@ hipError_t lit_pass_dispatch(int dir, const LitArgs& a, int grid, hipStream_t s)
 hipError_t lit_run_pass(int dir, int npl, bool adp, const LitArgs& a, int grid, hipStream_t s)
 {
     if (npl == 2) return adp ? lit_pass_dispatch<2, true>(dir, a, grid, s) : lit_pass_dispatch<2, false>(dir, a, grid, s);
-    return adp ? lit_pass_dispatch<4, true>(dir, a, grid, s) : lit_pass_dispatch<4, false>(dir, a, grid, s);
+    if (npl == 4) return adp ? lit_pass_dispatch<4, true>(dir, a, grid, s) : lit_pass_dispatch<4, false>(dir, a, grid, s);
+    return adp ? lit_pass_dispatch<0, true>(dir, a, grid, s) : lit_pass_dispatch<0, false>(dir, a, grid, s);
 }
 
 }  // namespace
@@ -386,8 +479,9 @@ int literal_resident(int npl, bool adp)
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    const void* k = npl == 2 ? (adp ? (const void*)lit_pass<1, 2, true> : (const void*)lit_pass<1, 2, false>)
-                             : (adp ? (const void*)lit_pass<1, 4, true> : (const void*)lit_pass<1, 4, false>);
+    const void* k = npl == 2   ? (adp ? (const void*)lit_pass<1, 2, true> : (const void*)lit_pass<1, 2, false>)
+                    : npl == 4 ? (adp ? (const void*)lit_pass<1, 4, true> : (const void*)lit_pass<1, 4, false>)
+                               : (adp ? (const void*)lit_pass<1, 0, true> : (const void*)lit_pass<1, 0, false>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64, 0) != hipSuccess) return 0;
     return per * cus;
 }

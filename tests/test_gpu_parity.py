@@ -169,7 +169,8 @@ def test_layer_ranges_padding_overflow_small():
 # W, H, T, planes, cmax, align, adaptive: padding past the segments, odd tiles, caps above 128 (4 planes a lane)
 SPILL_LAYERS = [(9, 7, 4, 40, 13, 8, False), (13, 11, 3, 24, 11, 8, False), (16, 12, 5, 30, 9, 2, True),
                 (70, 40, 64, 96, 45, 8, True), (130, 70, 16, 160, 77, 8, False), (200, 90, 8, 300, 130, 8, True),
-                (41, 33, 7, 64, 61, 8, False), (300, 120, 64, 120, 40, 8, True)]
+                (41, 33, 7, 64, 61, 8, False), (300, 120, 64, 120, 40, 8, True), (60, 30, 8, 800, 700, 8, False),
+                (33, 20, 5, 400, 290, 8, True)]
 
 
 @pytest.mark.parametrize("W,H,T,planes,cmax,align,adaptive", SPILL_LAYERS)
@@ -616,6 +617,20 @@ def test_dense_pyramid_spilling_padding(W0, H0, seed, near, far):
     for i, (d, dd, lv) in enumerate(zip(depths, d_depths, want)):
         np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
         np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+
+
+def test_dense_pyramid_wide_ranges():
+    """304 planes over a narrow depth span: ranges up to 288 planes at the depth edges (64-plane chunks)."""
+    imgs, gray, warps, _ = scene.multiview(256, 192, seed=7)
+    p8, p2 = scene.plane_list(1000.0, 2200.0, 304), scene.plane_list(1000.0, 2200.0, 302)
+    want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
+    assert max(int(lv["count"].max()) for lv in want[1:]) > 256
+    print("dense pyramid wide ranges 256x192, 304 planes", flush=True)
+    with sgm.Context(0) as c:
+        depths, aligns = sgm.dense_pyramid(c, imgs, gray, warps, p8, p2)
+    assert aligns == [lv["align"] for lv in want]
+    for i, (d, lv) in enumerate(zip(depths, want)):
+        np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
 
 
 def test_dense_pyramid_memory_modes():
