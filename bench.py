@@ -286,16 +286,19 @@ def bench_pyramid(np, sgm, scene, n_req, burst, budget_gb, slots):
 
 
 L16_W, L16_H, L16_N8, L16_N2 = 2080, 1560, 304, 302
+L16_SPAN = (500.0, 20000.0)        # plane depths: ranges up to 88 planes, every level on the path sweeps
+L16_SPAN_WIDE = (1000.0, 2200.0)   # the same planes over the scene's own depth span: ranges up to 288 planes at the
+                                   # depth edges, levels 1-5 on the line-buffer-literal path
 
 
-def bench_l16(np, torch, sgm, scene, reps, cpu_threads):
+def bench_l16(np, torch, sgm, scene, reps, cpu_threads, span=L16_SPAN, upsample=True):
     """The L16 dense stereo end to end (DenseStereoPyramid.Run, six levels from 65x49 to 2080x1560, then the
-    upsample layer to 4160x3120): one reference and four cameras of a rendered two-depth scene, 304 planes. The
-    device call takes host images (box filter, levels, depth downloads included); the upsample is timed on
-    device-resident buffers. The oracle runs the same pyramid on the host cores (cost row-parallel, aggregation
-    serial as the reference) and its depths are compared bit for bit."""
+    upsample layer to 4160x3120): one reference and four cameras of a rendered two-depth scene, 304 planes over the
+    depth span `span`. The device call takes host images (box filter, levels, depth downloads included); the
+    upsample is timed on device-resident buffers. The oracle runs the same pyramid on the host cores (cost
+    row-parallel, aggregation serial as the reference) and its depths are compared bit for bit."""
     imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
-    p8, p2 = scene.plane_list(500.0, 20000.0, L16_N8), scene.plane_list(500.0, 20000.0, L16_N2)
+    p8, p2 = scene.plane_list(span[0], span[1], L16_N8), scene.plane_list(span[0], span[1], L16_N2)
     ctx = sgm.Context(0)
     depths, aligns = sgm.dense_pyramid(ctx, imgs, gray, warps, p8, p2)
     host_times = []
@@ -316,6 +319,35 @@ def bench_l16(np, torch, sgm, scene, reps, cpu_threads):
         sgm.dense_pyramid_device(ctx, *args)
         times.append((time.perf_counter() - t) * 1e3)
     same = all(np.array_equal(d.cpu().numpy().view(np.uint32), h.view(np.uint32)) for d, h in zip(d_depths, depths))
+    out = {"workload": f"DenseStereoPyramid.Run at {L16_W}x{L16_H} (6 levels, scales 32..1, 1 reference + 4 cameras, "
+                       f"{L16_N8} planes over z {span[0]:g}..{span[1]:g}, adaptive P2, skip masks on the last two)"
+                       + (f", then the upsample layer to {2 * L16_W}x{2 * L16_H}" if upsample else "")
+                       + "; rendered two-depth scene",
+           "levels": [{"scale": s, "align": a, "shape": list(d.shape)} for s, a, d in zip(sgm.L16_SCALES, aligns, depths)],
+           "gpu_ms_p50": round(float(np.percentile(times, 50)), 3), "gpu_ms_min": round(min(times), 3),
+           "gpu_includes": "images resident in HBM: box filter, six levels (ranges, cost, aggregation, depth)",
+           "host_buffers_ms_p50": round(float(np.percentile(host_times, 50)), 3),
+           "device_equals_host_form": same}
+    if cpu_threads:
+        t = time.perf_counter()
+        want = o_pyramid(imgs, gray, warps, p8, p2, cpu_threads)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value_s": round(cpu_s, 3), "cores": cpu_threads, "kind": "port",
+                               "sample": "the whole pyramid at full size (the same scene and planes)"}
+        out["gpu_vs_cpu"] = round(cpu_s * 1e3 / out["gpu_ms_p50"], 1)
+        out["identical_to_oracle"] = all(np.array_equal(d.view(np.uint32), w["depth"].view(np.uint32))
+                                         for d, w in zip(depths, want))
+        for lv, w in zip(out["levels"], want):
+            if w["count"] is None:   # layer 0: every pixel searches all planes
+                continue
+            a = w["align"]
+            caps = (w["count"].astype(np.int64) + a - 1) // a * a
+            maxhi = -(-int(w["maxhi"]) // a) * a
+            lv["max_cap"] = int(caps.max())
+            lv["spilling_cells"] = int((w["start"].astype(np.int64) - int(w["minlo"]) + caps > maxhi).sum())
+    if not upsample:
+        ctx.close()
+        return out
     # the upsample layer on device buffers, timed with events on the stream it runs on
     guide = np.repeat(np.repeat(imgs[0], 2, 0), 2, 1)
     st = torch.cuda.Stream()
@@ -336,25 +368,8 @@ def bench_l16(np, torch, sgm, scene, reps, cpu_threads):
     ups_ms = e0.elapsed_time(e1) / 20
     ups_bytes = (2 * L16_W) * (2 * L16_H) * (4 + 4) + L16_W * L16_H * (4 + 4)
     ctx.close()
-    out = {"workload": f"DenseStereoPyramid.Run at {L16_W}x{L16_H} (6 levels, scales 32..1, 1 reference + 4 cameras, "
-                       f"{L16_N8} planes, adaptive P2, skip masks on the last two), then the upsample layer to "
-                       f"{2 * L16_W}x{2 * L16_H}; rendered two-depth scene",
-           "levels": [{"scale": s, "align": a, "shape": list(d.shape)} for s, a, d in zip(sgm.L16_SCALES, aligns, depths)],
-           "gpu_ms_p50": round(float(np.percentile(times, 50)), 3), "gpu_ms_min": round(min(times), 3),
-           "gpu_includes": "images resident in HBM: box filter, six levels (ranges, cost, aggregation, depth)",
-           "host_buffers_ms_p50": round(float(np.percentile(host_times, 50)), 3),
-           "device_equals_host_form": same,
-           "upsample_ms": round(ups_ms, 4),
-           "upsample_gb_s": round(ups_bytes / (ups_ms / 1e3) / 1e9, 1)}
-    if cpu_threads:
-        t = time.perf_counter()
-        want = o_pyramid(imgs, gray, warps, p8, p2, cpu_threads)
-        cpu_s = time.perf_counter() - t
-        out["cpu_baseline"] = {"value_s": round(cpu_s, 3), "cores": cpu_threads, "kind": "port",
-                               "sample": "the whole pyramid at full size (the same scene and planes)"}
-        out["gpu_vs_cpu"] = round(cpu_s * 1e3 / out["gpu_ms_p50"], 1)
-        out["identical_to_oracle"] = all(np.array_equal(d.view(np.uint32), w["depth"].view(np.uint32))
-                                         for d, w in zip(depths, want))
+    out["upsample_ms"] = round(ups_ms, 4)
+    out["upsample_gb_s"] = round(ups_bytes / (ups_ms / 1e3) / 1e9, 1)
     return out
 
 
@@ -374,8 +389,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
-    ap.add_argument("--forms", default="batch,serve,pyramid,l16",
-                    help="extra legs: batch, serve, pyramid, l16 (comma list; '' for none)")
+    ap.add_argument("--forms", default="batch,serve,pyramid,l16,l16wide",
+                    help="extra legs: batch, serve, pyramid, l16, l16wide (comma list; '' for none)")
+    ap.add_argument("--l16-span", default="default", choices=["default", "wide"],
+                    help="--profile-l16: the plane span (wide: ranges up to 288 planes, the line-buffer-literal path)")
     ap.add_argument("--pyr-requests", type=int, default=128)
     ap.add_argument("--pyr-burst", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=500)
@@ -414,7 +431,8 @@ def main():
 
     if args.profile_l16:
         imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
-        p8, p2 = scene.plane_list(500.0, 20000.0, L16_N8), scene.plane_list(500.0, 20000.0, L16_N2)
+        span = L16_SPAN_WIDE if args.l16_span == "wide" else L16_SPAN
+        p8, p2 = scene.plane_list(span[0], span[1], L16_N8), scene.plane_list(span[0], span[1], L16_N2)
         ctx = sgm.Context(0)
         d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
         d_depths = [torch.empty((-(-L16_H // sc), -(-L16_W // sc)), dtype=torch.float32, device="cuda")
@@ -539,6 +557,11 @@ def main():
     if "l16" in forms:
         print("bench: L16 dense pyramid leg", file=sys.stderr, flush=True)
         l16 = bench_l16(np, torch, sgm, scene, 5, 0 if args.no_cpu else min(16, os.cpu_count() or 1))
+    l16w = None
+    if "l16wide" in forms:
+        print("bench: L16 dense pyramid leg, wide ranges", file=sys.stderr, flush=True)
+        l16w = bench_l16(np, torch, sgm, scene, 3, 0 if args.no_cpu else min(16, os.cpu_count() or 1),
+                         span=L16_SPAN_WIDE, upsample=False)
 
     out = {
         "metric": "SGM Gcells/s (W·H·D·paths) + p50/p99 depth-request latency, served vs plain",
@@ -574,6 +597,7 @@ def main():
         "served": served,
         "coarse_to_fine": pyramid,
         "l16_pyramid": l16,
+        "l16_pyramid_wide_ranges": l16w,
     }
     print(json.dumps(out))
 

@@ -16,7 +16,7 @@
 // second visit. A visit reads and writes line-buffer slots within two columns of its own, in its own half
 // and the other (rows y-2 .. y+2 use them), so it only has to follow, in the reference's order, the visits
 // of those rows within that reach. One wavefront runs one row's visits in order; before each it waits until
-// each of rows y-2, y-1, y+1, y+2 has completed every visit with a smaller key within three columns (a
+// each of rows y-2, y-1, y+1, y+2 has completed every visit with a smaller key within two columns (a
 // prefix of that row's visits, so one progress counter per row says it). Rows of an earlier band come
 // wholly before, those of a later band wholly after.
 //
@@ -29,10 +29,11 @@
 // host checks occupancy) every wait ends. A bounded spin sets the error word instead of hanging, and later
 // waits are skipped, so the grid always drains.
 //
-// Memory. All line-buffer, minimum, range and guidance words go through agent-scope accesses (global
+// Memory. All line-buffer, minimum, range, guidance and progress words go through agent-scope accesses (global
 // load/store sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
-// (s_waitcnt) before its progress store. One wavefront per row, planes across lanes (NPL per lane; NPL 0:
-// 64-plane chunks in turn, caps up to 1024).
+// (s_waitcnt) before its progress store. The aggregate of a cell is read and written by its own row's wavefront
+// only, with plain accesses (the second visit of the border column takes the first visit's sums from registers).
+// One wavefront per row, planes across lanes (NPL per lane; NPL 0: 64-plane chunks in turn, caps up to 1024).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +44,7 @@
 namespace lux {
 namespace {
 
-constexpr int kLitReach = 3;   // columns either side whose visits may share line-buffer words with a visit
+constexpr int kLitReach = 2;   // columns either side whose visits may share line-buffer words with a visit
 constexpr int kLitGrid = 2048; // persistent single-wavefront workgroups per pass (rows go by ticket)
 
 struct LitArgs {
@@ -120,8 +121,10 @@ __device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
 }
 
 // One Pixel call (:234-317) at (x, y).
+// re: the second visit of the border column, whose aggregate the first visit (the one before) left in `last`.
 template <int DIR, int NPL, bool ADP>
-__device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth)
+__device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth, bool re,
+                                          int (&last)[NPL])
 {
     const int lane = threadIdx.x;
     const int slot = x + 1;
@@ -162,7 +165,7 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     for (int h = 0; h < NPL; h++) {
         const int k = lane + 64 * h;
         rb[h] = k < cap ? (int)a.raw[cbase + k] : 0;
-        ag[h] = k < cap ? (int)ald(a.agg + cbase + k) : 0;
+        ag[h] = k < cap ? (re ? last[h] : (int)a.agg[cbase + k]) : 0;
     }
 
     // the recurrence (:288-298 / :302-312)
@@ -217,13 +220,12 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
 #pragma unroll
     for (int h = 0; h < NPL; h++) {
         const int k = lane + 64 * h;
-        if (k < cap)
-            ast(a.agg + cbase + k,
-                (uint16_t)std::min(ag[h] + lc[0][h] + lc[1][h] + lc[2][h] + lc[3][h], 65535));
+        last[h] = std::min(ag[h] + lc[0][h] + lc[1][h] + lc[2][h] + lc[3][h], 65535);
+        if (k < cap) a.agg[cbase + k] = (uint16_t)last[h];
     }
     if (lane < 4) {
-        const uint32_t mm = lane < 2 ? mins01 : mins23;
-        const int rm = (lane & 1) ? (int)(mm >> 16) : (int)(mm & 0xffffu);
+        const uint32_t pair = lane < 2 ? mins01 : mins23;
+        const int rm = (lane & 1) ? (int)(pair >> 16) : (int)(pair & 0xffffu);
         ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));  // :275, :314
     }
     if (lane == 0) {
@@ -238,7 +240,8 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
 constexpr int kLitMaxCap = 1024;
 
 template <int DIR, bool ADP>
-__device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, uint16_t (*lcs)[kLitMaxCap])
+__device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, bool re,
+                                              uint16_t (*lcs)[kLitMaxCap], uint16_t* aggl)
 {
     const int lane = threadIdx.x;
     const int slot = x + 1;
@@ -272,7 +275,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
             lp[p] = act ? lit_rd(a, nb[p] + k + 1) : 0;
         }
         const int rb = act ? (int)a.raw[cbase + k] : 0;
-        int sum = act ? (int)ald(a.agg + cbase + k) : 0;
+        int sum = act ? (re ? (int)aggl[k] : (int)a.agg[cbase + k]) : 0;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int m = std::min(std::min(std::min(lm[p] + a.p1, 65535), l0[p]),
@@ -284,7 +287,11 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
                 sum += v;
             }
         }
-        if (act) ast(a.agg + cbase + k, (uint16_t)std::min(sum, 65535));
+        if (act) {
+            const uint16_t v = (uint16_t)std::min(sum, 65535);
+            a.agg[cbase + k] = v;
+            aggl[k] = v;
+        }
     }
     const uint32_t mins01 = wave_pmin(run[0] | (run[1] << 16)), mins23 = wave_pmin(run[2] | (run[3] << 16));
     __syncthreads();
@@ -311,8 +318,8 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
         }
     }
     if (lane < 4) {
-        const uint32_t mm = lane < 2 ? mins01 : mins23;
-        const int rm = (lane & 1) ? (int)(mm >> 16) : (int)(mm & 0xffffu);
+        const uint32_t pair = lane < 2 ? mins01 : mins23;
+        const int rm = (lane & 1) ? (int)(pair >> 16) : (int)(pair & 0xffffu);
         ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));
     }
     if (lane == 0) {
@@ -331,7 +338,7 @@ __device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, i
     for (;;) {
         seen = (int)ald32(a.progress + row2);
         if (seen >= need) return;
-        if (ald32(a.err) != 0u || ++spins > (1 << 22)) {
+        if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > (1 << 22)) {
             atomicOr(a.err, 1u);
             dead = true;
             return;
@@ -353,6 +360,8 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
     const int lane = threadIdx.x;
     const int W = a.W, T = a.T;
     __shared__ uint16_t lcs[NPL == 0 ? 4 : 1][NPL == 0 ? kLitMaxCap : 1];
+    __shared__ uint16_t aggl[NPL == 0 ? kLitMaxCap : 1];
+    int last[NPL == 0 ? 1 : NPL];
     bool dead = false;  // lane 0: the error word is set, waits are skipped so the grid drains
     for (;;) {
         int row = 0;
@@ -383,8 +392,8 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, lcs);
-            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth);
+            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, lcs, aggl);
+            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, last);
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane == 0) ast32(a.progress + row, (uint32_t)(pos + 1));

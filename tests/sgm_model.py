@@ -236,7 +236,7 @@ def wta_sub_lr_model(agg, D, tol=1):
 # ------------------------------------------------------------------ the line-buffer-literal schedule
 # A restatement of literal.hip's row sequences and waits (lit_rev, lit_before, the reach of 3 columns), so the
 # tests can check on the CPU that every order the waits allow gives the reference's results.
-LIT_REACH = 3
+LIT_REACH = 2
 
 
 def literal_rows(W, H, T, xsw, d):

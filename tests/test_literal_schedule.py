@@ -77,14 +77,14 @@ def test_allowed_orders_adaptive_p2():
 
 
 def test_a_shorter_reach_breaks_the_order(monkeypatch):
-    # the check has teeth: waits that ignore the visits of the neighbouring columns let orders through that do
-    # not give the reference's results on a spilling layer
+    # the check has teeth: waits that reach one column less than the kernel's let orders through that do not give
+    # the reference's results on a spilling layer
     W, H, T, planes, cmax, align = 13, 11, 4, 24, 11, 8
     rng = np.random.default_rng(W * 100 + H + T)
     start, count, raw, _ = _layer_inputs(rng, W, H, planes, cmax, align)
     kw = dict(T=T, align=align, p1=8, p2=96, keep_pass1=True)
     agg, _, _ = o.layer(raw, W, H, start, count, **kw)
-    monkeypatch.setattr(M, "LIT_REACH", -2)
+    monkeypatch.setattr(M, "LIT_REACH", 1)
     differs = 0
     for seed in range(6):
         visits, n_plus = M.literal_interleave(W, H, T, False, np.random.default_rng(seed), 10 ** 6)
