@@ -41,6 +41,20 @@ def _layer_inputs(rng, W, H, planes, cmax, align):
     return start, count, np.concatenate(raws), spill
 
 
+def test_visit_order_entry_in_the_reference_order():
+    # lo_layer_compute_visits given the reference's own order is lo_layer_compute
+    W, H, T, planes, cmax, align = 11, 9, 4, 32, 13, 8
+    rng = np.random.default_rng(3)
+    start, count, raw, _ = _layer_inputs(rng, W, H, planes, cmax, align)
+    kw = dict(T=T, align=align, p1=8, p2=96, keep_pass1=True)
+    visits = [(x, y, cur) for d, x, y, cur in M.schedule(W, H, T)]
+    n_plus = sum(1 for d, *_ in M.schedule(W, H, T) if d > 0)
+    a = o.layer(raw, W, H, start, count, **kw)
+    b = o.layer(raw, W, H, start, count, visits=np.array(visits, np.int32), n_plus=n_plus, **kw)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+
+
 SPILL_LAYERS = [(9, 7, 4, False, 40, 13, 8), (13, 11, 4, True, 24, 11, 8), (16, 12, 3, False, 30, 9, 2),
                 (12, 17, 8, False, 64, 21, 8), (21, 9, 2, False, 16, 7, 8), (6, 14, 5, True, 48, 30, 8)]
 
