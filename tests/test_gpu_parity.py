@@ -682,10 +682,10 @@ def test_plane_sweep_entry_points_refuse_bad_input():
         big = sgm.CostContext(ref, others, warps, wqs, sweep_planes(1030))
         with pytest.raises(sgm.NotSupported):   # layer 0 above 1024 planes
             sgm.dense_layer(c, big, sgm.dense_params(48, 40, 4, 4))
-        # ranges above 256 lanes
+        # ranges above 1024 lanes
         p = sgm.dense_params(48, 40, 4, 4)
         start = np.zeros((10, 12), np.uint16)
         count = np.full((10, 12), 3, np.uint16)
-        count[5, 5] = 300
+        count[5, 5] = 1030
         with pytest.raises(sgm.NotSupported):
-            sgm.dense_layer(c, sgm.CostContext(ref, others, warps, wqs, sweep_planes(320)), p, start, count)
+            sgm.dense_layer(c, sgm.CostContext(ref, others, warps, wqs, sweep_planes(1040)), p, start, count)
