@@ -120,19 +120,34 @@ __device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
     return v;
 }
 
+// The cell's own words, which no other row writes: loaded one visit ahead (LitCell), so a visit's line-buffer
+// reads are its only round trip after its waits.
+struct LitCell {
+    int start, cap;
+    uint32_t g;
+};
+template <bool ADP>
+__device__ __forceinline__ LitCell lit_cell(const LitArgs& a, size_t cell)
+{
+    LitCell c;
+    c.start = a.start ? (int)a.start[cell] : 0;
+    c.cap = a.start ? (int)a.cap[cell] : a.ucap;
+    c.g = ADP ? a.ap.guid[cell] : 0u;
+    return c;
+}
+
 // One Pixel call (:234-317) at (x, y).
 // re: the second visit of the border column, whose aggregate the first visit (the one before) left in `last`.
 template <int DIR, int NPL, bool ADP>
-__device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth, bool re,
+__device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth, bool re, const LitCell& cc,
                                           int (&last)[NPL])
 {
     const int lane = threadIdx.x;
     const int slot = x + 1;
     const size_t cell = (size_t)y * a.W + x;
-    const int start = a.start ? (int)a.start[cell] : 0;
-    const int cap = a.start ? (int)a.cap[cell] : a.ucap;
+    const int start = cc.start, cap = cc.cap;
     const int q = start - a.minlo;
-    const uint32_t g = ADP ? a.ap.guid[cell] : 0u;
+    const uint32_t g = cc.g;
     const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
 
@@ -240,16 +255,15 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
 constexpr int kLitMaxCap = 1024;
 
 template <int DIR, bool ADP>
-__device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, bool re,
+__device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, bool re, const LitCell& cc,
                                               uint16_t (*lcs)[kLitMaxCap], uint16_t* aggl)
 {
     const int lane = threadIdx.x;
     const int slot = x + 1;
     const size_t cell = (size_t)y * a.W + x;
-    const int start = a.start ? (int)a.start[cell] : 0;
-    const int cap = a.start ? (int)a.cap[cell] : a.ucap;
+    const int start = cc.start, cap = cc.cap;
     const int q = start - a.minlo;
-    const uint32_t g = ADP ? a.ap.guid[cell] : 0u;
+    const uint32_t g = cc.g;
     const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
     int minPrev[4], p2p[4];
@@ -373,9 +387,15 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
         const int cur = (r & 1) == 0 ? 1 : 0, oth = 1 - cur;  // :219
         const int nvis = W + lit_rev(a, DIR, r);
         int seen[4] = {0, 0, 0, 0};  // lane 0: progress last read of rows row-2, row-1, row+1, row+2
+        LitCell next = lit_cell<ADP>(a, (size_t)y * W + (DIR > 0 ? 0 : W - 1));
         for (int pos = 0; pos < nvis; pos++) {
             const int c = pos < W ? pos : W - 1;
             const int i = pos < W ? (pos + r) / T : a.gwN - 1;
+            const LitCell cc = next;
+            if (pos + 1 < nvis) {  // in flight during this visit's waits
+                const int c1 = pos + 1 < W ? pos + 1 : W - 1;
+                next = lit_cell<ADP>(a, (size_t)y * W + (DIR > 0 ? c1 : W - 1 - c1));
+            }
             if (lane == 0 && pos < W) {
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
@@ -392,8 +412,8 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, lcs, aggl);
-            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, last);
+            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, lcs, aggl);
+            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, last);
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane == 0) ast32(a.progress + row, (uint32_t)(pos + 1));
