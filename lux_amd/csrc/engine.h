@@ -43,7 +43,8 @@ struct Workspace {
     const Plan* plan = nullptr;      // whose visit lists are in "visits"
     void* sync_zeroed = nullptr;     // band flag array known to hold no live epoch
     int epoch = 0;
-    int force_literal = 0;           // LUX_SGM_OPT_LITERAL: layers of <= 256 lanes take the line-buffer-literal path
+    int force_literal = 0;           // LUX_SGM_OPT_LITERAL: layers of <= 1024 lanes take the line-buffer-literal path
+    int lit_resident[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // resident literal wavefronts [npl 2/4/0][adaptive]
 
     Workspace() = default;
     Workspace(const Workspace&) = delete;

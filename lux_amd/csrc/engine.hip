@@ -525,8 +525,7 @@ int run_literal(Workspace& ws, LiteralLayer l, int maxcap, uint16_t* agg_pass1, 
 {
     if (maxcap > 1024) return fail(LUX_SGM_EUNSUPPORTED, "the line-buffer-literal path takes at most 1024 lanes per cell", err);
     l.npl = maxcap <= 128 ? 2 : maxcap <= 256 ? 4 : 0;
-    static int resident[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // [npl 2/4/0][adaptive]; one device model throughout
-    int& res = resident[l.npl == 2 ? 0 : l.npl == 4 ? 1 : 2][l.ap.on != 0];
+    int& res = ws.lit_resident[l.npl == 2 ? 0 : l.npl == 4 ? 1 : 2][l.ap.on != 0];  // per context, so per device
     if (res < 0) res = literal_resident(l.npl, l.ap.on != 0);
     if (res < std::min(literal_depth(l.W, l.T), l.H))
         return fail(LUX_SGM_EUNSUPPORTED, "the device holds too few wavefronts for the line-buffer-literal schedule", err);

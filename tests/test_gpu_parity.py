@@ -200,6 +200,30 @@ def test_layer_ranges_spilling_padding(W, H, T, planes, cmax, align, adaptive):
     np.testing.assert_array_equal(L.Wta, wta)
 
 
+@pytest.mark.parametrize("W,H,T,xsw,skipped", [(37, 29, 4, True, False), (65, 50, 8, False, True), (50, 37, 5, True, True)])
+def test_layer_spilling_padding_xsw_and_skip(W, H, T, xsw, skipped):
+    """Spilling padding with the LUX_SGM_XSW clamp (no second visit in pass +1) and with a mode-2 skip mask."""
+    planes, cmax, align = 48, 21, 8
+    rng = np.random.default_rng(W * H + T)
+    start, count = M.spill_ranges(rng, W, H, planes, cmax, align)
+    assert _spills(start, count, align) > 0
+    raw = _ranged_raw(rng, count, align)
+    skip = sgm.skip_mask(2, W, H) if skipped else None
+    print(f"spill xsw={xsw} skip={skipped} W={W} H={H} T={T}", flush=True)
+    with sgm.Context(0) as c:
+        L = sgm.DenseLayer(c, W, H, planes, align=align, p1=8, p2=96, tile=T, xsw=xsw).set_ranges(start, count)
+        if skipped:
+            L.Skip = skip
+        L.Raw = raw
+        L.KeepPass1 = True
+        L.compute()
+    agg, agg1, wta = o.layer(raw, W, H, start, count, T=T, align=align, p1=8, p2=96, xsw=xsw, skip=skip,
+                             keep_pass1=True)
+    np.testing.assert_array_equal(L.AggPass1, agg1)
+    np.testing.assert_array_equal(L.Agg, agg)
+    np.testing.assert_array_equal(L.Wta, wta)
+
+
 @pytest.mark.parametrize("W,H,T,planes", [(9, 13, 3, 16), (33, 12, 5, 64), (70, 40, 63, 24), (17, 17, 9, 200)])
 def test_layer_odd_tiles(ctx, W, H, T, planes):
     _layer_case(ctx, W, H, planes, T, 10, 120, seed=W + H + T)
