@@ -557,9 +557,11 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
     const uint64_t* clrow = COST ? a.cl + rowcell : nullptr;
     const int lol = store_lane ? lo : 0;
-    // COST: the lane's window of right codes, and the prefetched wave-uniform codes cl(x), cr(x+1)
+    // COST: the lane's window of right codes, and the wave-uniform codes cl(x), cr(x+1) of 64 cells at a time in
+    // VGPRs (lane k: cell x0 + k), read out per cell with v_readlane, the next 64 in flight. (Eight cells of codes
+    // prefetched in SGPRs spill into VGPR lanes, and a spill waits for its scalar load every cell.)
     uint64_t win[2 * NP];
-    uint64_t rl[COST ? PF : 1], rr[COST ? PF : 1];
+    uint64_t bl = 0, br = 0, nbl = 0, nbr = 0;
     RawW<NP> rb[COST ? 1 : PF];
     uint32_t rs[VAR ? PF : 1];  // VAR: start | cap << 16 of the prefetched cells
     Range from;
@@ -568,16 +570,15 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
         const uint64_t c0 = crow[0];
 #pragma unroll
         for (int e = 0; e < 2 * NP; e++) win[e] = c0;
+        bl = clrow[min(lane, W - 1)];
+        br = crow[min(lane + 1, W - 1)];
+        nbl = clrow[min(64 + lane, W - 1)];
+        nbr = crow[min(65 + lane, W - 1)];
     }
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = min(j, W - 1);
-        if constexpr (COST) {
-            rl[j] = clrow[xj];
-            rr[j] = crow[min(j + 1, W - 1)];
-        } else {
-            ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
-        }
+        if constexpr (!COST) ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         if constexpr (VAR) rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
         if constexpr (ADP) gq[j] = a.ap.guid[rowcell + xj];
     }
@@ -592,8 +593,13 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
             rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
         }
         if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
+            const int q = x & 63;
+            const uint64_t cl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bl >> 32), q) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bl, q);
+            const uint64_t cr1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(br >> 32), q) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)br, q);
             uint32_t bytes[NP / 2 + (NP == 1)];
-            census_cost<NP>(rl[j], win, a.D, lo, raw, bytes);
+            census_cost<NP>(cl, win, a.D, lo, raw, bytes);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
             if (store_lane) {
                 if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
@@ -602,9 +608,13 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
                     for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bytes[k];
                 }
             }
-            census_advance<NP>(win, rr[j], lane);
-            rl[j] = clrow[xp];
-            rr[j] = crow[min(x + PF + 1, W - 1)];
+            census_advance<NP>(win, cr1, lane);
+            if (q == 63) {  // the next 64 cells' codes, and the 64 after them into flight
+                bl = nbl;
+                br = nbr;
+                nbl = clrow[min(x + 65 + lane, W - 1)];
+                nbr = crow[min(x + 66 + lane, W - 1)];
+            }
         } else {
             widen<NP>(rb[j], raw);
             ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
