@@ -339,6 +339,32 @@ struct SweepArgs {
     uint32_t* seedguid;      // [3][W] (adaptive P2)
 };
 
+// Per-cell words of a chain (the cell's range, its guidance), 64 chain positions in one VGPR (lane k: position
+// i0 + k, one vector load), read out per step with v_readlane, the next 64 in flight: one load per 64 cells and
+// word instead of one per cell (the row and sweep loops issue about a third of the vector loads they did).
+struct ChainWords {
+    uint32_t cur = 0, nxt = 0;
+    template <typename F>
+    __device__ __forceinline__ void init(const F& load, int lane)
+    {
+        cur = load(lane);
+        nxt = load(64 + lane);
+    }
+    __device__ __forceinline__ uint32_t at(int pos) const
+    {
+        return (uint32_t)__builtin_amdgcn_readlane((int)cur, pos & 63);
+    }
+    // after the last use of position pos: at the end of a block, move on and put the block after next in flight
+    template <typename F>
+    __device__ __forceinline__ void advance(int pos, const F& load, int lane)
+    {
+        if ((pos & 63) == 63) {
+            cur = nxt;
+            nxt = load(pos + 65 + lane);
+        }
+    }
+};
+
 // One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
 // (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
 // 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
@@ -400,15 +426,20 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     size_t cell = cell0;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
-    uint32_t rs[VAR ? PF : 1];  // ranges of the prefetched cells: start | cap << 16
-    uint32_t gq[ADP ? PF : 1];  // ADP: guidance of the prefetched cells
+    // VAR / ADP: the chain's range (start | cap << 16) and guidance words, 64 positions a VGPR (ChainWords)
+    const auto rs_at = [&](int pos) {
+        const size_t c = cell0 + (ptrdiff_t)min(pos, n - 1) * step;
+        return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
+    };
+    const auto gq_at = [&](int pos) { return a.ap.guid[cell0 + (ptrdiff_t)min(pos, n - 1) * step]; };
+    ChainWords rs, gq;
+    if constexpr (VAR) rs.init(rs_at, lane);
+    if constexpr (ADP) gq.init(gq_at, lane);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
         ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
         if (!FIRST) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
-        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16);
-        if constexpr (ADP) gq[j] = a.ap.guid[cj];
     }
     auto body = [&](int i, int j) {
         Cell<NP> raw;
@@ -417,19 +448,19 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
         uint32_t el = kInit2, er = kInit2;
         int cs_ = 0, cc_ = 0;
         if constexpr (VAR) {
-            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            const uint32_t r = rs.at(i);
             cs_ = (int)(r & 0xffffu);
             cc_ = (int)(r >> 16);
             reframe<NP, VAR>(state, from, cs_, cc_, lane, el, er);
             make_mask<NP>(lane, cc_, mask, active);
         }
-        const uint32_t gc = ADP ? (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[ADP ? j : 0]) : 0u;
+        const uint32_t gc = ADP ? gq.at(i) : 0u;
         const uint32_t p2 = ADP ? p2_step(a.ap, a.p2, gc, gprev) : a.p2;
         const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
         ld_raw_w<NP>(a.raw + cp * cs + lol, rb[j]);
         if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
-        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[cp] | ((uint32_t)a.vcap[cp] << 16);
-        if constexpr (ADP) gq[j] = a.ap.guid[cp];
+        if constexpr (VAR) rs.advance(i, rs_at, lane);
+        if constexpr (ADP) gq.advance(i, gq_at, lane);
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
         if (x != bx) {
@@ -563,9 +594,17 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     uint64_t win[2 * NP];
     uint64_t bl = 0, br = 0, nbl = 0, nbr = 0;
     RawW<NP> rb[COST ? 1 : PF];
-    uint32_t rs[VAR ? PF : 1];  // VAR: start | cap << 16 of the prefetched cells
     Range from;
-    uint32_t gq[ADP ? PF : 1], gprev = 0;  // ADP: guidance of the prefetched cells / of the state's cell
+    uint32_t gprev = 0;  // ADP: guidance of the state's cell
+    // VAR / ADP: the row's range (start | cap << 16) and guidance words, 64 cells a VGPR (ChainWords)
+    const auto rs_at = [&](int pos) {
+        const size_t c = rowcell + min(pos, W - 1);
+        return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
+    };
+    const auto gq_at = [&](int pos) { return a.ap.guid[rowcell + min(pos, W - 1)]; };
+    ChainWords rs, gq;
+    if constexpr (VAR) rs.init(rs_at, lane);
+    if constexpr (ADP) gq.init(gq_at, lane);
     if constexpr (COST) {
         const uint64_t c0 = crow[0];
 #pragma unroll
@@ -579,18 +618,16 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     for (int j = 0; j < PF; j++) {
         const int xj = min(j, W - 1);
         if constexpr (!COST) ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
-        if constexpr (VAR) rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
-        if constexpr (ADP) gq[j] = a.ap.guid[rowcell + xj];
     }
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         const int xp = min(x + PF, W - 1);
         uint32_t el = kInit2, er = kInit2;
         if constexpr (VAR) {
-            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            const uint32_t r = rs.at(x);
+            rs.advance(x, rs_at, lane);
             reframe<NP, VAR>(state, from, (int)(r & 0xffffu), (int)(r >> 16), lane, el, er);
             make_mask<NP>(lane, (int)(r >> 16), mask, active);
-            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
         }
         if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
             const int q = x & 63;
@@ -621,9 +658,9 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
         }
         uint32_t p2 = a.p2;
         if constexpr (ADP) {
-            const uint32_t gc = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
+            const uint32_t gc = gq.at(x);
+            gq.advance(x, gq_at, lane);
             p2 = p2_step(a.ap, a.p2, gc, gprev);
-            gq[j] = a.ap.guid[rowcell + xp];
             gprev = gc;
         }
         Cell<NP> lc;
@@ -681,19 +718,27 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
     const int lol = (VAR ? lo < cs : active) ? lo : 0;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
-    uint32_t rs[VAR ? PF : 1], rn[VAR ? PF : 1];  // VAR: start | cap << 16, and count, of the prefetched cells
     Range from;
-    uint32_t gq[ADP ? PF : 1], gprev = 0;  // ADP: guidance of the prefetched cells / of the state's cell
+    uint32_t gprev = 0;  // ADP: guidance of the state's cell
+    // VAR / ADP: range (start | cap << 16), count and guidance words by position W-1-x, 64 a VGPR (ChainWords)
+    const auto cell_at = [&](int pos) { return rowcell + max(W - 1 - pos, 0); };
+    const auto rs_at = [&](int pos) {
+        const size_t c = cell_at(pos);
+        return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
+    };
+    const auto rn_at = [&](int pos) { return (uint32_t)a.vcount[cell_at(pos)]; };
+    const auto gq_at = [&](int pos) { return a.ap.guid[cell_at(pos)]; };
+    ChainWords rs, rn, gq;
+    if constexpr (VAR) {
+        rs.init(rs_at, lane);
+        rn.init(rn_at, lane);
+    }
+    if constexpr (ADP) gq.init(gq_at, lane);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
-        if constexpr (VAR) {
-            rs[j] = (uint32_t)a.vstart[rowcell + xj] | ((uint32_t)a.vcap[rowcell + xj] << 16);
-            rn[j] = a.vcount[rowcell + xj];
-        }
-        if constexpr (ADP) gq[j] = a.ap.guid[rowcell + xj];
     }
     uint32_t myd = 0, myr = 0;
     float mys = 0.f;
@@ -705,19 +750,19 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
         uint32_t el = kInit2, er = kInit2;
         int s0 = 0;
         if constexpr (VAR) {
-            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            const uint32_t r = rs.at(W - 1 - x);
+            D = (int)rn.at(W - 1 - x);
+            rs.advance(W - 1 - x, rs_at, lane);
+            rn.advance(W - 1 - x, rn_at, lane);
             s0 = (int)(r & 0xffffu);
-            D = __builtin_amdgcn_readfirstlane((int)rn[j]);
             reframe<NP, VAR>(state, from, s0, (int)(r >> 16), lane, el, er);
             make_mask<NP>(lane, (int)(r >> 16), mask, active);
-            rs[j] = (uint32_t)a.vstart[rowcell + xp] | ((uint32_t)a.vcap[rowcell + xp] << 16);
-            rn[j] = a.vcount[rowcell + xp];
         }
         uint32_t p2 = a.p2;
         if constexpr (ADP) {
-            const uint32_t gc = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
+            const uint32_t gc = gq.at(W - 1 - x);
+            gq.advance(W - 1 - x, gq_at, lane);
             p2 = p2_step(a.ap, a.p2, gc, gprev);
-            gq[j] = a.ap.guid[rowcell + xp];
             gprev = gc;
         }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
