@@ -550,9 +550,9 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         lmaxhi = lmaxcap = g.cap;
     }
     lux::Workspace& ws = c->ws;
-    // The path sweeps take even tiles, caps up to 128 with ranges and ranges whose padding stays inside its
+    // The path sweeps take even tiles, caps up to 512 with ranges and ranges whose padding stays inside its
     // line-buffer segment; the line-buffer-literal path takes the rest up to 1024 lanes (and everything when forced).
-    const bool literal = lmaxcap <= 1024 && (ws.force_literal || spill || (p->tile & 1) || (ranged && lmaxcap > 128));
+    const bool literal = lmaxcap <= 1024 && (ws.force_literal || spill || (p->tile & 1) || (ranged && lmaxcap > 512));
     lux::Prepared w;
     if (literal) rc = lux::prepare_literal(ws, g, &w, &c->err);
     else rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, false, s, &w, &c->err);

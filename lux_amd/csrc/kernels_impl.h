@@ -259,6 +259,54 @@ __device__ __forceinline__ void realign(Cell<1>& st, const Range& from, int s, i
     edge_l = el | (el << 16);
     edge_r = er | (er << 16);
 }
+// NP pairs a lane (P = 2 NP planes; lane l holds planes P*l + 2k, P*l + 2k + 1 in v[k]): plane r of a state,
+// read wave-uniformly, 2000 outside [0, cp).
+template <int NP>
+__device__ __forceinline__ uint32_t rel_plane_np(const Cell<NP>& c, int r, int cp)
+{
+    if (r < 0 || r >= cp) return 2000u;
+    const int ln = r / (2 * NP), e = r - ln * 2 * NP;
+    uint32_t w = c.v[0];
+#pragma unroll
+    for (int k = 1; k < NP; k++) w = (e >> 1) == k ? c.v[k] : w;
+    w = (uint32_t)__builtin_amdgcn_readlane((int)w, ln);
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+// The realignment with NP pairs a lane: relative plane j of the new frame is plane j + delta of the old one.
+// With delta = P*dl + de (0 <= de < P), lane l's pairs come from lanes l+dl and l+dl+1 (2 NP ds_bpermute), the
+// pair offset de/2 selects among them and an odd de shifts by one plane (v_alignbit).
+template <int NP>
+__device__ __forceinline__ void realign_np(Cell<NP>& st, const Range& from, int s, int lane, uint32_t& edge_l,
+                                           uint32_t& edge_r)
+{
+    constexpr int P = 2 * NP;
+    const int delta = s - from.s, cp = from.c;
+    const Cell<NP> v = st;
+    const int dl = delta >= 0 ? delta / P : -((-delta + P - 1) / P);
+    const int de = delta - dl * P, h = de >> 1;
+    const int sa = ((lane + dl) & 63) << 2, sb = ((lane + dl + 1) & 63) << 2;
+    uint32_t S[2 * NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        S[k] = (uint32_t)__builtin_amdgcn_ds_bpermute(sa, (int)v.v[k]);
+        S[NP + k] = (uint32_t)__builtin_amdgcn_ds_bpermute(sb, (int)v.v[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        uint32_t p0 = S[0], p1 = S[1];
+#pragma unroll
+        for (int j = 1; j < 2 * NP; j++) p0 = k + h == j ? S[j] : p0;
+#pragma unroll
+        for (int j = 2; j < 2 * NP; j++) p1 = k + h + 1 == j ? S[j] : p1;
+        const uint32_t out = (de & 1) ? __builtin_amdgcn_alignbit(p1, p0, 16) : p0;
+        const int r0 = P * lane + 2 * k + delta;
+        const bool ok0 = r0 >= 0 && r0 < cp, ok1 = r0 + 1 >= 0 && r0 + 1 < cp;
+        st.v[k] = (ok0 ? (out & 0xffffu) : 2000u) | (ok1 ? (out & 0xffff0000u) : (2000u << 16));
+    }
+    const uint32_t el = rel_plane_np<NP>(v, delta - 1, cp), er = rel_plane_np<NP>(v, 64 * P + delta, cp);
+    edge_l = el | (el << 16);
+    edge_r = er | (er << 16);
+}
 // Move a chain's state into the frame of the next cell (start s); a no-op when the start does not change
 // (the planes past the predecessor's cap already hold 2000).
 template <int NP, bool VAR>
@@ -268,8 +316,11 @@ __device__ __forceinline__ void reframe(Cell<NP>& st, Range& from, int s, int c,
     edge_l = kInit2;
     edge_r = kInit2;
     if constexpr (VAR) {
-        static_assert(NP == 1, "per-pixel ranges take one packed pair per lane");
-        if (s != from.s) realign(*(Cell<1>*)&st, from, s, lane, edge_l, edge_r);
+        if constexpr (NP == 1) {
+            if (s != from.s) realign(*(Cell<1>*)&st, from, s, lane, edge_l, edge_r);
+        } else {
+            if (s != from.s) realign_np<NP>(st, from, s, lane, edge_l, edge_r);
+        }
         from.s = s;
         from.c = c;
     }
@@ -2123,7 +2174,9 @@ hipError_t launch_sgm_t(const Geometry& g, const SgmBuffers& b, const SgmOutputs
                         hipStream_t s, hipEvent_t* m)
 {
     const bool full = g.cap == 64 * g.dpl;
-    if (b.vstart) {  // per-pixel ranges: one packed pair per lane, per-cell masks
+    if (b.vstart) {  // per-pixel ranges: 1, 2 or 4 packed pairs a lane (caps up to 128, 256, 512), per-cell masks
+        if (g.dpl == 4) return run_sgm<2, true, true, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
+        if (g.dpl == 8) return run_sgm<4, true, true, ADP>(g, b, o, p1, p2, agg_pass1, s, m);
         if (g.dpl != 2) return hipErrorInvalidValue;
         if constexpr (!ADP) {  // narrow ranges: several chains per wavefront
             if (b.seg && g.cap <= 16) return run_sgm<1, true, true, false, 8>(g, b, o, p1, p2, agg_pass1, s, m);

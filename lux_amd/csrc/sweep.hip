@@ -602,8 +602,8 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         spill = rs.maxend - rs.minlo > maxhi_r;
     }
     if (!ranged && ucap > 1024) return refuse(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell");
-    // the path sweeps, or the line-buffer-literal path for spilling padding, caps of 129..1024 and odd tiles
-    const bool literal = maxcap <= 1024 && (ws.force_literal || spill || (p.tile & 1) || (ranged && maxcap > 128));
+    // the path sweeps, or the line-buffer-literal path for spilling padding, caps of 513..1024 and odd tiles
+    const bool literal = maxcap <= 1024 && (ws.force_literal || spill || (p.tile & 1) || (ranged && maxcap > 512));
     const Geometry g = geometry(W, H, ranged ? rs.maxcount : a.n, p.align);
     int rc = literal ? prepare_literal(ws, g, w, err) : prepare(ws, g, p.tile, 0, false, s, w, err);
     if (rc) return rc;

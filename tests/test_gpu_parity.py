@@ -107,7 +107,9 @@ import sgm_model as M  # noqa: E402
 
 RANGE_LAYERS = [(9, 7, 4, 48, 20, 8, True), (13, 5, 2, 40, 13, 8, False), (33, 12, 16, 64, 24, 8, True),
                 (10, 10, 8, 32, 9, 2, False), (1, 6, 2, 24, 7, 8, False), (70, 40, 64, 128, 40, 8, True),
-                (130, 70, 64, 256, 128, 8, True), (64, 33, 8, 200, 99, 2, False), (40, 9, 4, 96, 1, 8, True)]
+                (130, 70, 64, 256, 128, 8, True), (64, 33, 8, 200, 99, 2, False), (40, 9, 4, 96, 1, 8, True),
+                (130, 70, 64, 300, 200, 8, True), (70, 40, 16, 512, 400, 8, False), (64, 33, 8, 300, 250, 2, True),
+                (33, 21, 4, 520, 512, 8, False)]
 
 
 @pytest.mark.parametrize("W,H,T,planes,cmax,align,smooth", RANGE_LAYERS)
@@ -294,8 +296,8 @@ def test_layer_adaptive_p2(ctx, W, H, T, planes, p1, scale, weights):
     np.testing.assert_array_equal(L.Wta, wta)
 
 
-def test_layer_ranges_adaptive_p2():
-    W, H, T, planes, cmax, align = 40, 23, 8, 96, 40, 8
+@pytest.mark.parametrize("W,H,T,planes,cmax,align", [(40, 23, 8, 96, 40, 8), (50, 31, 8, 400, 300, 8)])
+def test_layer_ranges_adaptive_p2(W, H, T, planes, cmax, align):
     rng = np.random.default_rng(11)
     start, count = M.random_ranges(rng, W, H, planes, cmax, align, True)
     caps = ((count.astype(np.int64) + align - 1) // align) * align
