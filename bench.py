@@ -288,7 +288,7 @@ def bench_pyramid(np, sgm, scene, n_req, burst, budget_gb, slots):
 L16_W, L16_H, L16_N8, L16_N2 = 2080, 1560, 304, 302
 L16_SPAN = (500.0, 20000.0)        # plane depths: ranges up to 88 planes, every level on the path sweeps
 L16_SPAN_WIDE = (1000.0, 2200.0)   # the same planes over the scene's own depth span: ranges up to 288 planes at the
-                                   # depth edges, levels 1-5 on the line-buffer-literal path
+                                   # depth edges, levels 1-5 on the sweeps with four packed pairs a lane
 
 
 def bench_l16(np, torch, sgm, scene, reps, cpu_threads, span=L16_SPAN, upsample=True):
@@ -392,7 +392,9 @@ def main():
     ap.add_argument("--forms", default="batch,serve,pyramid,l16,l16wide",
                     help="extra legs: batch, serve, pyramid, l16, l16wide (comma list; '' for none)")
     ap.add_argument("--l16-span", default="default", choices=["default", "wide"],
-                    help="--profile-l16: the plane span (wide: ranges up to 288 planes, the line-buffer-literal path)")
+                    help="--profile-l16: the plane span (wide: ranges up to 288 planes, four packed pairs a lane)")
+    ap.add_argument("--force-literal", action="store_true",
+                    help="--profile-l16: every layer on the line-buffer-literal path (LUX_SGM_OPT_LITERAL)")
     ap.add_argument("--pyr-requests", type=int, default=128)
     ap.add_argument("--pyr-burst", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=500)
@@ -434,6 +436,8 @@ def main():
         span = L16_SPAN_WIDE if args.l16_span == "wide" else L16_SPAN
         p8, p2 = scene.plane_list(span[0], span[1], L16_N8), scene.plane_list(span[0], span[1], L16_N2)
         ctx = sgm.Context(0)
+        if args.force_literal:
+            ctx.set_option(sgm.OPT_LITERAL, 1)
         d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
         d_depths = [torch.empty((-(-L16_H // sc), -(-L16_W // sc)), dtype=torch.float32, device="cuda")
                     for sc in sgm.L16_SCALES]

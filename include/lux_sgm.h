@@ -118,7 +118,7 @@ int lux_sgm_compute_layer(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const
  * its cell's own plane frame and realign it to the next cell's start; that is the reference's line-buffer
  * result as long as no range's padding reaches the last word of its path segment: start - MinLo + cap <=
  * MaxHi for every pixel (MinLo = min start, MaxHi = roundup(max(start + count), align)). A layer that breaks
- * it (the reference then writes into the next segment and reads those words back), has a cap above 128 or an
+ * it (the reference then writes into the next segment and reads those words back), has a cap above 512 or an
  * odd tile runs on the line-buffer-literal path, which holds the reference's line buffers word for word; caps
  * above 1024 get LUX_SGM_EUNSUPPORTED. */
 int lux_sgm_compute_layer_ranges(lux_sgm_ctx* ctx, const lux_sgm_layer_params* p, const uint16_t* start,
@@ -187,7 +187,7 @@ typedef struct lux_sgm_dense_params {
  * reference subsampled at the layer's scale, :39), the first-minimum WTA and depth = Planes[start + arg] [H*W].
  * start/count both null: every pixel searches all planes (layer 0). skip may be null. agg / agg_pass1 (packed like
  * the raw volume) and wta (the plane index) may be null. Layers whose padding passes the line-buffer segment, with
- * caps of 129..1024 lanes or an odd tile run on the line-buffer-literal path (lux_sgm_compute_layer_ranges).
+ * caps of 513..1024 lanes or an odd tile run on the line-buffer-literal path (lux_sgm_compute_layer_ranges).
  * Returns LUX_SGM_EUNSUPPORTED for ranges above 1024 lanes and for more than 1024 planes at layer 0. */
 int lux_sgm_dense_layer(lux_sgm_ctx* ctx, const lux_sgm_sweep* sweep, const lux_sgm_dense_params* params,
                         const uint16_t* start, const uint16_t* count, const uint8_t* skip, float* depth, uint16_t* agg,

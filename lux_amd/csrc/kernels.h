@@ -53,7 +53,7 @@ struct SgmBuffers {
     int seed_row = -1;
     uint16_t* disp_tmp = nullptr;   // [H][W] disparity when the caller wants none
     // per-pixel plane ranges (null: every pixel has planes [0, cap)): start, cap and count of every cell, and
-    // the start / cap of the cell behind each seed-table entry. Needs cap <= 128 (dpl 2); no fused bands.
+    // the start / cap of the cell behind each seed-table entry. Needs cap <= 512 (dpl 2, 4 or 8); no fused bands.
     const uint16_t* vstart = nullptr;
     const uint16_t* vcap = nullptr;
     const uint16_t* vcount = nullptr;

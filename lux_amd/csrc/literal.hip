@@ -7,7 +7,7 @@
 //     path-0 segment for path 3), and its stale cleanup later refills the same words (:254-264, :280-313);
 //     successors whose ranges touch those words read them (and Rd's zero past the end of the buffer, :324);
 //   - odd tiles T, whose halves do not alternate row by row across band boundaries (:219);
-//   - ranges of 129 to 1024 planes.
+//   - ranges of 513 to 1024 planes.
 //
 // Schedule. The reference runs visits in the order of the key (band j, tile i, tile row r), row r of tile i
 // covering x in [iT - r, iT + T - r) (mirrored in pass -1), clamped to the image, and tile gwN-1 visiting the
