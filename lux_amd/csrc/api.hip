@@ -32,6 +32,8 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
     lux::Prepared w;
     int rc = lux::prepare(c->ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, s, &w, &c->err);
     if (rc) return rc;
+    const lux::Conc conc = lux::conc_of(c);
+    lux::attach_chain(&w, &conc);
     lux::EngineOpts opt;
     opt.bands = c->opt_bands;
     rc = lux::enqueue_rect(c->ws, w, p, dl, dr, o, opt, s, c->profiling ? c->ev : nullptr, &c->err);
@@ -260,6 +262,7 @@ int lux_sgm_destroy(lux_sgm_ctx* c)
         if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
     }
     if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->chain_fork) (void)hipEventDestroy(c->chain_fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LUX_SGM_OK;
@@ -633,6 +636,7 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
     } else {
         const lux::Conc conc = lux::conc_of(c);
         if (ranged) lux::try_concurrency(ws, &w, &conc);  // ranged layers are latency-bound: their sweeps run side by side
+        lux::attach_chain(&w, &conc);
         lux::SgmOutputs o;
         o.disp = (uint16_t*)dw;
         o.tol = 0;

@@ -34,6 +34,7 @@ struct lux_sgm_ctx {
     std::vector<std::unique_ptr<lux::Workspace>> bws;     // one per pair of a batch
     hipStream_t side[lux::kMaxBranches] = {};
     hipEvent_t fork = nullptr, join[lux::kMaxBranches] = {};
+    hipEvent_t chain_fork = nullptr;                     // the border chains' fork (they run on side[3])
     std::map<std::string, lux::GraphEntry> graphs;        // batch descriptor -> instantiated graph
     std::map<std::string, lux_sgm_schedule> tuned;        // batch shapes -> schedule
     uint64_t gen = 0;                                     // bumped when a batch workspace is reallocated
@@ -51,6 +52,7 @@ inline int ensure_side_streams(lux_sgm_ctx* c)
 {
     if (c->fork) return 0;
     hipError_t e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->chain_fork, hipEventDisableTiming);
     for (int k = 0; e == hipSuccess && k < kMaxBranches; k++) {
         e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
@@ -58,8 +60,9 @@ inline int ensure_side_streams(lux_sgm_ctx* c)
     return e == hipSuccess ? 0 : hip_status(e, "side streams", &c->err);
 }
 
-// The context's side streams and events as the concurrent sweeps take them (the batch form uses the same ones; calls
-// on a context are serialised by its mutex). Without side streams the sweeps stay one after another.
+// The context's side streams and events as the concurrent sweeps (side 0-2) and the border chains (side 3) take them
+// (the batch form uses the same ones; calls on a context are serialised by its mutex). Without side streams the
+// sweeps and chains stay one after another.
 inline Conc conc_of(lux_sgm_ctx* c)
 {
     Conc k;
@@ -69,6 +72,9 @@ inline Conc conc_of(lux_sgm_ctx* c)
         k.join[i] = c->join[i];
     }
     k.fork = c->fork;
+    k.chain = c->side[3];
+    k.chain_fork = c->chain_fork;
+    k.chain_join = c->join[3];
     return k;
 }
 

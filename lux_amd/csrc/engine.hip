@@ -124,6 +124,7 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
         {"cols", (size_t)4 * g.H * g.cs * 2},
         {"colmins", (size_t)4 * g.H * 2},
         {"replay_partial", maxvis * g.cs * 2},
+        {"replay_chain", maxvis * g.cs * 2},
         {"wta_dr", cells * 2},
         {"wta_dr0", (size_t)g.H * 4},
         {"band_boundary", (size_t)6 * g.W * g.cs * 2},
@@ -247,6 +248,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     w->b.cols = (uint16_t*)cols;
     w->b.colmins = (uint16_t*)colmins;
     w->b.partial = (uint16_t*)partial;
+    w->b.chainp = (uint16_t*)ptr["replay_chain"];
     w->b.visits_plus = (const int*)vis;
     w->b.n_plus = (int)plan->plus.size() / 4;
     w->b.visits_minus = (const int*)vis + plan->plus.size();
@@ -456,6 +458,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         w.b.seedstart = (uint16_t*)ptr["seedstart"];
         w.b.seedcap = (uint16_t*)ptr["seedcap"];
         try_concurrency(ws, &w, conc);  // sized above: no reallocation under queued work
+        attach_chain(&w, conc);
         SgmOutputs so;
         so.disp = ldisp ? ldisp : w.b.disp_tmp;
         so.sub = lsub;
@@ -502,6 +505,14 @@ bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
     }
     w->b.conc_fork = c->fork;
     return true;
+}
+
+void attach_chain(Prepared* w, const Conc* c)
+{
+    if (!c || !c->chain || !c->chain_fork || !c->chain_join) return;
+    w->b.chain = c->chain;
+    w->b.chain_fork = c->chain_fork;
+    w->b.chain_join = c->chain_join;
 }
 
 int prepare_literal(Workspace& ws, const Geometry& g, Prepared* w, std::string* err)

@@ -36,7 +36,8 @@ struct SgmBuffers {
     uint16_t* seedmins = nullptr;   // [3][W]
     uint16_t* cols = nullptr;       // [2 passes][2 paths][H][cs]  neighbour-column states for the replays
     uint16_t* colmins = nullptr;    // [2][2][H]
-    uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch
+    uint16_t* partial = nullptr;    // [max(visits)][cs]  replay scratch: paths 0, 1 and 3 of each border visit
+    uint16_t* chainp = nullptr;     // [max(visits)][cs]  path 2 of each border visit (the border chain's)
     uint16_t* bnd = nullptr;        // [2][3][W][cs]  band boundary states (ping-pong)
     uint32_t* bndmin = nullptr;     // [2][3][W]
     int* ticket = nullptr;          // [2] tile tickets of the two band launches
@@ -68,6 +69,10 @@ struct SgmBuffers {
     hipStream_t conc[3] = {};
     hipEvent_t conc_fork = nullptr, conc_join[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
+    // The border chains on a side stream (set all or none; none: they run in order on the request's stream). They
+    // read only the cost volume, the ranges, the guidance and pass +1's seeds, so they run beside the sweeps.
+    hipStream_t chain = nullptr;
+    hipEvent_t chain_fork = nullptr, chain_join = nullptr;
 };
 
 // Census codes of one image (T = uint8_t or uint16_t pixels).

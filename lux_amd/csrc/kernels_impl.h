@@ -393,25 +393,31 @@ struct SweepArgs {
 // Per-cell words of a chain (the cell's range, its guidance), 64 chain positions in one VGPR (lane k: position
 // i0 + k, one vector load), read out per step with v_readlane, the next 64 in flight: one load per 64 cells and
 // word instead of one per cell (the row and sweep loops issue about a third of the vector loads they did).
+// Block k (positions 64k .. 64k+63) sits in register b[k & 1]; when block k is done, that register is reloaded
+// with block k+2 in place (a load into a register that is then copied elsewhere would wait for the load at once).
 struct ChainWords {
-    uint32_t cur = 0, nxt = 0;
+    uint32_t b0 = 0, b1 = 0;
     template <typename F>
     __device__ __forceinline__ void init(const F& load, int lane)
     {
-        cur = load(lane);
-        nxt = load(64 + lane);
+        b0 = load(lane);
+        b1 = load(64 + lane);
     }
     __device__ __forceinline__ uint32_t at(int pos) const
     {
-        return (uint32_t)__builtin_amdgcn_readlane((int)cur, pos & 63);
+        // read both and pick the scalar (a select between the registers becomes a dynamically indexed array)
+        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, pos & 63);
+        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)b1, pos & 63);
+        return (pos & 64) ? r1 : r0;
     }
-    // after the last use of position pos: at the end of a block, move on and put the block after next in flight
+    // after the last use of position pos: at the end of its block, reload that block's register with the block
+    // after next
     template <typename F>
     __device__ __forceinline__ void advance(int pos, const F& load, int lane)
     {
         if ((pos & 63) == 63) {
-            cur = nxt;
-            nxt = load(pos + 65 + lane);
+            if (pos & 64) b1 = load(pos + 65 + lane);
+            else b0 = load(pos + 65 + lane);
         }
     }
 };
@@ -643,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     // VGPRs (lane k: cell x0 + k), read out per cell with v_readlane, the next 64 in flight. (Eight cells of codes
     // prefetched in SGPRs spill into VGPR lanes, and a spill waits for its scalar load every cell.)
     uint64_t win[2 * NP];
-    uint64_t bl = 0, br = 0, nbl = 0, nbr = 0;
+    uint64_t bl0 = 0, br0 = 0, bl1 = 0, br1 = 0;  // cells 64k .. 64k+63 in the registers of parity k & 1
     RawW<NP> rb[COST ? 1 : PF];
     Range from;
     uint32_t gprev = 0;  // ADP: guidance of the state's cell
@@ -660,10 +666,10 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
         const uint64_t c0 = crow[0];
 #pragma unroll
         for (int e = 0; e < 2 * NP; e++) win[e] = c0;
-        bl = clrow[min(lane, W - 1)];
-        br = crow[min(lane + 1, W - 1)];
-        nbl = clrow[min(64 + lane, W - 1)];
-        nbr = crow[min(65 + lane, W - 1)];
+        bl0 = clrow[min(lane, W - 1)];
+        br0 = crow[min(lane + 1, W - 1)];
+        bl1 = clrow[min(64 + lane, W - 1)];
+        br1 = crow[min(65 + lane, W - 1)];
     }
 #pragma unroll
     for (int j = 0; j < PF; j++) {
@@ -682,10 +688,12 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
         }
         if constexpr (COST) {  // the cost of this cell, and its raw bytes for the later sweeps
             const int q = x & 63;
-            const uint64_t cl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bl >> 32), q) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bl, q);
-            const uint64_t cr1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(br >> 32), q) << 32) |
-                                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)br, q);
+            const auto rl64 = [&](uint64_t v) {
+                return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), q) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, q);
+            };
+            const uint64_t cl0 = rl64(bl0), cl1 = rl64(bl1), cr0 = rl64(br0), cr1_ = rl64(br1);
+            const uint64_t cl = (x & 64) ? cl1 : cl0, cr1 = (x & 64) ? cr1_ : cr0;
             uint32_t bytes[NP / 2 + (NP == 1)];
             census_cost<NP>(cl, win, a.D, lo, raw, bytes);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
@@ -697,11 +705,14 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
                 }
             }
             census_advance<NP>(win, cr1, lane);
-            if (q == 63) {  // the next 64 cells' codes, and the 64 after them into flight
-                bl = nbl;
-                br = nbr;
-                nbl = clrow[min(x + 65 + lane, W - 1)];
-                nbr = crow[min(x + 66 + lane, W - 1)];
+            if (q == 63) {  // this block's registers take the block after next, loaded in place
+                if (x & 64) {
+                    bl1 = clrow[min(x + 65 + lane, W - 1)];
+                    br1 = crow[min(x + 66 + lane, W - 1)];
+                } else {
+                    bl0 = clrow[min(x + 65 + lane, W - 1)];
+                    br0 = crow[min(x + 66 + lane, W - 1)];
+                }
             }
         } else {
             widen<NP>(rb[j], raw);
@@ -1514,7 +1525,7 @@ __global__ __launch_bounds__(kBandWaves * 64) void sgm_band(BandArgs a)
             for (int j = 0; j < 3; j++) {
                 store_u16_agent<NP>(bout + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
                 if (lane == 0) __hip_atomic_store(boutmin + j * W + x, sm[g][j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (band == a.extra_band) {
+                if (band == a.extra_band && x != bx) {  // the border cell's entries are the replays'
                     store_u16<NP>(a.extra + ((size_t)j * W + x) * cs + lo, active, st[g][j]);
                     if (lane == 0) a.extramin[j * W + x] = (uint16_t)sm[g][j];
                 }
@@ -1541,6 +1552,7 @@ struct ReplayArgs {
     uint16_t* seeds;          // [3][W][cs]
     uint16_t* seedmins;       // [3][W]
     uint16_t* partial;        // [nvis][cs]: paths 0 + 1 + 3 of each visit
+    uint16_t* chainp;         // [nvis][cs]: path 2 of each visit (the chain's)
     const uint16_t* vstart;   // per-pixel ranges [H][W] (null: uniform)
     const uint16_t* vcap;
     uint16_t* seedstart;      // [3][W]
@@ -1635,10 +1647,10 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 
 // Path 2 of the border column: one chain through the column's visits in schedule order. Each visit reads
 // the state the other line-buffer half holds for the border slot and leaves its own in the current half
-// (:219, :265). The chain adds its path cost to the visit's `partial` (the aggregate is updated afterwards,
-// per cell, by sgm_replay_add), so it only reads data fixed before it starts and can prefetch deeply.
-// Pass -1 starts from pass +1's half-0 leftover. Pass +1 ends by writing the border cell's seed-table
-// entry (half 0 as the pass leaves it).
+// (:219, :265). The chain writes its path cost to `chainp` (the aggregate is updated afterwards, per cell, by
+// sgm_replay_add). It reads only the cost volume, the ranges and guidance, its visit list and, in pass -1, the
+// border cell's path-2 seed (pass +1's half-0 leftover), so it prefetches deeply and runs beside the sweeps.
+// Pass +1 ends by writing the border cell's path-2 seed-table entry (half 0 as the pass leaves it).
 template <int NP, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
 {
@@ -1670,46 +1682,50 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
     const int lol = lo < cs ? lo : 0;
     const int nv = a.nvis;
     RawW<NP> rb[PF];
-    Cell<NP> pb[PF];
-    int cb[PF];          // half of each prefetched visit
-    uint32_t rs[PF];     // VAR: start | cap << 16 of each prefetched visit's cell
-    uint32_t gq[ADP ? PF : 1];  // ADP: guidance of each prefetched visit's cell
+    // the visit list's words 64 visits a VGPR (ChainWords): the rows of the visits being prefetched (PF ahead),
+    // and the half, range and guidance of the visit being run
+    const auto cell_of = [&](int pos) { return (size_t)vis[4 * min(pos, nv - 1)] * a.W + a.col; };
+    const auto vrow = [&](int pos) { return (uint32_t)vis[4 * min(pos, nv - 1)]; };
+    const auto vcur = [&](int pos) { return (uint32_t)vis[4 * min(pos, nv - 1) + 1]; };
+    const auto vrs = [&](int pos) {
+        const size_t c = cell_of(pos);
+        return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
+    };
+    const auto vg = [&](int pos) { return a.ap.guid[cell_of(pos)]; };
+    ChainWords pr, cw, rw, gw;
     if (nv > 0) {
+        pr.init(vrow, lane);
+        cw.init(vcur, lane);
+        if constexpr (VAR) rw.init(vrs, lane);
+        if constexpr (ADP) gw.init(vg, lane);
 #pragma unroll
         for (int j = 0; j < PF; j++) {
-            const int ij = min(j, nv - 1);
-            const size_t cj = (size_t)vis[4 * ij] * a.W + a.col;
-            cb[j] = vis[4 * ij + 1];
+            const size_t cj = (size_t)pr.at(j) * a.W + a.col;
             ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
-            ld_u16<NP>(a.partial + (size_t)ij * cs + lol, pb[j]);
-            rs[j] = VAR ? ((uint32_t)a.vstart[cj] | ((uint32_t)a.vcap[cj] << 16)) : 0u;
-            if constexpr (ADP) gq[j] = a.ap.guid[cj];
         }
     }
     auto body = [&](int i, int j) {
-        const int cur = cb[j];
+        const int cur = (int)cw.at(i);
+        cw.advance(i, vcur, lane);
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
-        const Cell<NP> part = pb[j];
         int s = 0, c = a.cap;
         if constexpr (VAR) {
-            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)rs[j]);
+            const uint32_t r = rw.at(i);
+            rw.advance(i, vrs, lane);
             s = (int)(r & 0xffffu);
             c = (int)(r >> 16);
             make_mask<NP>(lane, c, mask, active);
         }
-        const int ip = min(i + PF, nv - 1);
-        const size_t cpn = (size_t)vis[4 * ip] * a.W + a.col;
-        cb[j] = vis[4 * ip + 1];
+        const size_t cpn = (size_t)pr.at(i + PF) * a.W + a.col;
+        pr.advance(i + PF, vrow, lane);
         ld_raw_w<NP>(a.raw + cpn * cs + lol, rb[j]);
-        ld_u16<NP>(a.partial + (size_t)ip * cs + lol, pb[j]);
-        if (VAR) rs[j] = (uint32_t)a.vstart[cpn] | ((uint32_t)a.vcap[cpn] << 16);
         uint32_t g = 0;
         if constexpr (ADP) {
-            g = (uint32_t)__builtin_amdgcn_readfirstlane((int)gq[j]);
-            gq[j] = a.ap.guid[cpn];
+            g = gw.at(i);
+            gw.advance(i, vg, lane);
         }
-        Cell<NP> lc, sum, prev;
+        Cell<NP> lc, prev;
         uint32_t mn, pm, q = 0;
 #pragma unroll
         for (int k = 0; k < NP; k++) prev.v[k] = cur ? h0.v[k] : h1.v[k];  // the other half's state
@@ -1720,9 +1736,7 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
         from.s = (int)(q & 0xffffu);
         from.c = (int)(q >> 16);
         mn = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, raw, a, mask, lc, g, gp);
-#pragma unroll
-        for (int k = 0; k < NP; k++) sum.v[k] = padd(part.v[k], lc.v[k]);
-        store_u16<NP>(a.partial + (size_t)i * cs + lo, active, sum);
+        store_u16<NP>(a.chainp + (size_t)i * cs + lo, active, lc);
         const uint32_t qn = VAR ? ((uint32_t)s | ((uint32_t)c << 16)) : 0u;
 #pragma unroll
         for (int k = 0; k < NP; k++) {
@@ -1745,7 +1759,7 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
     for (int j = 0; j < PF; j++)
         if (i0 + j < nv) body(i0 + j, j);
     if (a.dir > 0) {
-        // path 2: the chain's half-0 state; paths 1 and 3: recomputed for the last visit that wrote half 0
+        // path 2's seed: the chain's half-0 state (paths 1 and 3: sgm_replay_seeds)
         store_u16<NP>(a.seeds + ((size_t)1 * a.W + a.col) * cs + lo, lo < cs, h0);
         if (lane == 0) {
             a.seedmins[1 * a.W + a.col] = (uint16_t)m0;
@@ -1755,51 +1769,63 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
             }
             if (ADP) a.seedguid[1 * a.W + a.col] = g0;
         }
-        int last = -1;
-        for (int i = a.nvis - 1; i >= 0; i--)
-            if (vis[4 * i + 1] == 0) { last = i; break; }
-        Cell<NP> s1, s3;
-        uint32_t sm1 = 2000, sm3 = 2000;
-        Range rl;
-        uint32_t gl = 0;
-        set_init(s1);
-        set_init(s3);
-        if (last >= 0) {
-            const int yl = vis[4 * last];
-            const size_t cl = (size_t)yl * a.W + a.col;
-            if (ADP) gl = a.ap.guid[cl];
-            int s = 0, c = a.cap;
-            if (VAR) {
-                s = a.vstart[cl];
-                c = a.vcap[cl];
-                make_mask<NP>(lane, c, mask, active);
-                rl.s = s;
-                rl.c = c;
-            }
-            Cell<NP> r, prev;
-            uint32_t pm, gp;
-            Range from;
-            load_raw<NP>(a.raw + cl * cs + lo, lo < cs, r);
-            label_state<NP, MASKED, VAR>(a, 1, vis[4 * last + 3], lo, mask, prev, pm, from, gp);
-            sm1 = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, r, a, mask, s1, gl, gp);
-            set_init(prev);
-            sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, p2_step(a.ap, a.p2, gl, 0u), mask, s3);
+    }
+}
+
+// Pass +1's seed-table entries of paths 1 and 3 for the border cell: recomputed for the last visit that wrote
+// half 0. Path 1 reads a saved neighbour-column state, so this runs after the sweeps (one wavefront).
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(64) void sgm_replay_seeds(ReplayArgs a, const int* __restrict__ vis)
+{
+    const int lane = threadIdx.x;
+    const int cs = a.cs, lo = lane * 2 * NP;
+    Cell<NP> mask;
+    bool active;
+    make_mask<NP>(lane, a.cap, mask, active);
+    int last = -1;
+    for (int i = a.nvis - 1; i >= 0; i--)
+        if (vis[4 * i + 1] == 0) { last = i; break; }
+    Cell<NP> s1, s3;
+    uint32_t sm1 = 2000, sm3 = 2000;
+    Range rl;
+    uint32_t gl = 0;
+    set_init(s1);
+    set_init(s3);
+    if (last >= 0) {
+        const int yl = vis[4 * last];
+        const size_t cl = (size_t)yl * a.W + a.col;
+        if (ADP) gl = a.ap.guid[cl];
+        int s = 0, c = a.cap;
+        if (VAR) {
+            s = a.vstart[cl];
+            c = a.vcap[cl];
+            make_mask<NP>(lane, c, mask, active);
+            rl.s = s;
+            rl.c = c;
         }
-        store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, lo < cs, s1);
-        store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, lo < cs, s3);
-        if (lane == 0) {
-            a.seedmins[0 * a.W + a.col] = (uint16_t)sm1;
-            a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
-            if (VAR) {
-                a.seedstart[0 * a.W + a.col] = (uint16_t)rl.s;
-                a.seedcap[0 * a.W + a.col] = (uint16_t)rl.c;
-                a.seedstart[2 * a.W + a.col] = (uint16_t)rl.s;
-                a.seedcap[2 * a.W + a.col] = (uint16_t)rl.c;
-            }
-            if (ADP) {
-                a.seedguid[0 * a.W + a.col] = gl;
-                a.seedguid[2 * a.W + a.col] = gl;
-            }
+        Cell<NP> r, prev;
+        uint32_t pm, gp;
+        Range from;
+        load_raw<NP>(a.raw + cl * cs + lo, lo < cs, r);
+        label_state<NP, MASKED, VAR>(a, 1, vis[4 * last + 3], lo, mask, prev, pm, from, gp);
+        sm1 = step_from<NP, MASKED, VAR>(prev, from, pm, s, c, lane, r, a, mask, s1, gl, gp);
+        set_init(prev);
+        sm3 = path_step<NP, MASKED>(prev, 2000, r, a.p1x2, p2_step(a.ap, a.p2, gl, 0u), mask, s3);
+    }
+    store_u16<NP>(a.seeds + ((size_t)0 * a.W + a.col) * cs + lo, lo < cs, s1);
+    store_u16<NP>(a.seeds + ((size_t)2 * a.W + a.col) * cs + lo, lo < cs, s3);
+    if (lane == 0) {
+        a.seedmins[0 * a.W + a.col] = (uint16_t)sm1;
+        a.seedmins[2 * a.W + a.col] = (uint16_t)sm3;
+        if (VAR) {
+            a.seedstart[0 * a.W + a.col] = (uint16_t)rl.s;
+            a.seedcap[0 * a.W + a.col] = (uint16_t)rl.c;
+            a.seedstart[2 * a.W + a.col] = (uint16_t)rl.s;
+            a.seedcap[2 * a.W + a.col] = (uint16_t)rl.c;
+        }
+        if (ADP) {
+            a.seedguid[0 * a.W + a.col] = gl;
+            a.seedguid[2 * a.W + a.col] = gl;
         }
     }
 }
@@ -1822,6 +1848,9 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int
         const int i = cellvis[2 * y + j];
         if (i >= 0) {
             load_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, p);
+#pragma unroll
+            for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], p.v[k]);
+            load_u16<NP>(a.chainp + (size_t)i * a.cs + lo, active, p);
 #pragma unroll
             for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], p.v[k]);
         }
@@ -2115,9 +2144,36 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             mark();
         }
     };
+    // The border chains (path 2 of the replayed column) run on b.chain when it is given: each forks from s once
+    // its inputs are final (pass +1: the cost volume; pass -1: also pass +1's seeds) and s waits for it just
+    // before the visits' costs are added.
+    if (!b.chainp) return hipErrorInvalidValue;
+    const bool side = b.chain && b.chain_fork && b.chain_join;
+    auto start_chain = [&](const ReplayArgs& rr) {
+        hipStream_t cst = s;
+        if (side) {
+            (void)hipEventRecord(b.chain_fork, s);
+            (void)hipStreamWaitEvent(b.chain, b.chain_fork, 0);
+            cst = b.chain;
+        }
+        sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, cst>>>(rr, rr.visits);
+        if (side) (void)hipEventRecord(b.chain_join, b.chain);
+    };
+    auto end_chain = [&]() {
+        if (side) (void)hipStreamWaitEvent(s, b.chain_join, 0);
+    };
+    ReplayArgs r;
+    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
+    r.p1x2 = a.p1x2; r.p2 = a.p2;
+    r.seeds = b.seeds; r.seedmins = b.seedmins;
+    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
+    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial; r.chainp = b.chainp;
+    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
+    r.ap = b.ap; r.seedguid = b.seedguid;
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
+    if (!b.cl) start_chain(r);  // else the row path writes the cost volume first
     if (conc) {
         (void)hipEventRecord(b.conc_fork, s);
         forked = true;
@@ -2130,18 +2186,12 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
         else sgm_row_fwd<NP, MASKED, false, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     }
+    if (b.cl) start_chain(r);
     mark();
     vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
-    ReplayArgs r;
-    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
-    r.p1x2 = a.p1x2; r.p2 = a.p2;
-    r.seeds = b.seeds; r.seedmins = b.seedmins;
-    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
-    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial;
-    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
-    r.ap = b.ap; r.seedguid = b.seedguid;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
+    sgm_replay_seeds<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
+    end_chain();
     sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
     mark();
     if (agg_pass1) {
@@ -2150,16 +2200,17 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     }
     // pass -1: up-left, up, up-right (starting from pass +1's half-0 leftovers), then <- with the
     // winner-take-all fused (the last contribution)
+    r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
+    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz;
+    start_chain(r);
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
     vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
     if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
     else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
-    r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
-    r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz; r.partial = b.partial;
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
-    sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
+    end_chain();
     sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
     mark();
     wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);

@@ -654,6 +654,7 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         if ((rc = run_literal(ws, l, maxcap, agg_pass1, d_wta, s, err))) return rc;
     } else {
         if (ranged) try_concurrency(ws, w, conc);  // ranged layers are latency-bound: their sweeps run side by side
+        attach_chain(w, conc);
         SgmOutputs o;
         o.disp = d_wta;
         o.tol = 0;
