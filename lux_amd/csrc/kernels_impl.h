@@ -426,12 +426,10 @@ struct ChainWords {
 // (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
 // 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
 template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
-__global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
+__device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int lane)
 {
     constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
     constexpr int DY = DIR * (P == 0 ? 0 : 1);
-    const int lane = threadIdx.x & 63;
-    const int chain = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     const int W = a.W, H = a.H, cs = a.cs;
     if (chain >= chain_count<DX, DY>(W, H)) return;
     int x, y, n;
@@ -561,6 +559,11 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(i0 + j, j);
 }
+template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
+{
+    sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(a, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
+}
 
 // ------------------------------------------------------------------ horizontal paths (rows)
 struct RowArgs {
@@ -626,12 +629,10 @@ __device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2
 // each cell on the way and writes the raw volume for the later sweeps. Saves the W-2 column's states for
 // the border replay.
 template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
-__global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
+__device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
 {
     static_assert(!(COST && VAR), "per-pixel ranges take their raw volume from the cost kernel");
     constexpr int PF = 8;
-    const int lane = threadIdx.x & 63;
-    const int y = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
     if (y >= a.H) return;
     const int W = a.W, cs = a.cs, lo = lane * 2 * NP;
     Cell<NP> mask;
@@ -745,6 +746,12 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (x0 + j < W) body(x0 + j, j);
+}
+template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
+{
+    row_fwd_wave<NP, MASKED, COST, VAR, ADP>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
 
 // Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
@@ -1652,10 +1659,9 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 // border cell's path-2 seed (pass +1's half-0 leftover), so it prefetches deeply and runs beside the sweeps.
 // Pass +1 ends by writing the border cell's path-2 seed-table entry (half 0 as the pass leaves it).
 template <int NP, bool MASKED, bool VAR, bool ADP>
-__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
+__device__ __forceinline__ void replay_chain_wave(const ReplayArgs& a, const int* __restrict__ vis, int lane)
 {
     constexpr int PF = 8;
-    const int lane = threadIdx.x;
     const int cs = a.cs, lo = lane * 2 * NP;
     Cell<NP> mask;
     bool active;
@@ -1770,6 +1776,49 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
             if (ADP) a.seedguid[1 * a.W + a.col] = g0;
         }
     }
+}
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* __restrict__ vis)
+{
+    replay_chain_wave<NP, MASKED, VAR, ADP>(a, vis, threadIdx.x);
+}
+
+// One pass's independent chains in one launch, for latency-bound (ranged) layers: the border chain (block 0,
+// wave 0: the longest serial path starts first), pass +1's row path, then paths 2, 1 and 3, each path into
+// its own volume (FIRST). One launch instead of one stream per path: streams beyond the device's hardware
+// queues share a queue and run one after another.
+struct PassArgs {
+    SweepArgs s[3];   // paths 1, 2, 3
+    RowArgs r;        // pass +1's row path
+    ReplayArgs c;     // the border chain
+    int n1, n2, nrow; // chains of paths 1, 2 and of the row path (0 in pass -1)
+};
+template <int NP, int DIR, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
+{
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 64) replay_chain_wave<NP, MASKED, VAR, ADP>(p.c, p.c.visits, lane);
+        return;
+    }
+    int w = __builtin_amdgcn_readfirstlane((int)(((blockIdx.x - 1) * kBlock + threadIdx.x) >> 6));
+    if (DIR > 0) {
+        if (w < p.nrow) {
+            row_fwd_wave<NP, MASKED, false, VAR, ADP>(p.r, w, lane);
+            return;
+        }
+        w -= p.nrow;
+    }
+    if (w < p.n2) {
+        sweep_wave<NP, DIR, 2, true, MASKED, VAR, ADP>(p.s[1], w, lane);
+        return;
+    }
+    w -= p.n2;
+    if (w < p.n1) {
+        sweep_wave<NP, DIR, 1, true, MASKED, VAR, ADP>(p.s[0], w, lane);
+        return;
+    }
+    sweep_wave<NP, DIR, 3, true, MASKED, VAR, ADP>(p.s[2], w - p.n1, lane);  // returns past the last chain
 }
 
 // Pass +1's seed-table entries of paths 1 and 3 for the border cell: recomputed for the last visit that wrote
@@ -2149,6 +2198,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // before the visits' costs are added.
     if (!b.chainp) return hipErrorInvalidValue;
     const bool side = b.chain && b.chain_fork && b.chain_join;
+    bool chain_live = false;
     auto start_chain = [&](const ReplayArgs& rr) {
         hipStream_t cst = s;
         if (side) {
@@ -2158,9 +2208,34 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         }
         sgm_replay_chain<NP, MASKED, VAR, ADP><<<1, 64, 0, cst>>>(rr, rr.visits);
         if (side) (void)hipEventRecord(b.chain_join, b.chain);
+        chain_live = side;
     };
     auto end_chain = [&]() {
-        if (side) (void)hipStreamWaitEvent(s, b.chain_join, 0);
+        if (chain_live) (void)hipStreamWaitEvent(s, b.chain_join, 0);
+        chain_live = false;
+    };
+    // Ranged layers given path volumes: each pass's border chain, row path (pass +1) and paths 1-3 in one launch
+    // (sgm_pass), then the volumes summed into agg.
+    constexpr bool kFusable = SEG == 0 && VAR;
+    const bool fused = kFusable && conc;
+    auto launch_pass = [&](auto dirc, const ReplayArgs& rr) {
+        constexpr int DIR = decltype(dirc)::value;
+        if constexpr (kFusable) {
+            PassArgs p;
+            for (int k = 0; k < 3; k++) {
+                p.s[k] = a;
+                p.s[k].agg = b.aggp[k];
+            }
+            p.r = ra;
+            p.c = rr;
+            p.n1 = nd;
+            p.n2 = nv;
+            p.nrow = DIR > 0 ? nh : 0;
+            const int waves = p.nrow + nv + 2 * nd;
+            sgm_pass<NP, DIR, MASKED, VAR, ADP><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
+            agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
+                                                           (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
+        }
     };
     ReplayArgs r;
     r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
@@ -2173,22 +2248,30 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
-    if (!b.cl) start_chain(r);  // else the row path writes the cost volume first
-    if (conc) {
-        (void)hipEventRecord(b.conc_fork, s);
-        forked = true;
-    }
-    if constexpr (SEG > 0) {
-        sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
-    } else if constexpr (VAR) {
-        sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+    if (fused) {
+        launch_pass(std::integral_constant<int, 1>(), r);
+        mark();  // the row path and paths 1-3 in one launch: the next three stages are empty
+        mark();
+        mark();
+        mark();
     } else {
-        if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
-        else sgm_row_fwd<NP, MASKED, false, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        if (!b.cl) start_chain(r);  // else the row path writes the cost volume first
+        if (conc) {
+            (void)hipEventRecord(b.conc_fork, s);
+            forked = true;
+        }
+        if constexpr (SEG > 0) {
+            sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
+        } else if constexpr (VAR) {
+            sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        } else {
+            if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        }
+        if (b.cl) start_chain(r);
+        mark();
+        vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
     }
-    if (b.cl) start_chain(r);
-    mark();
-    vertical_diagonal(std::integral_constant<int, 1>(), b.cols, b.colmins);
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
     sgm_replay_seeds<NP, MASKED, VAR, ADP><<<1, 64, 0, s>>>(r, r.visits);
     end_chain();
@@ -2202,10 +2285,17 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // winner-take-all fused (the last contribution)
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz;
-    start_chain(r);
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
-    vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
+    if (fused) {
+        launch_pass(std::integral_constant<int, -1>(), r);
+        mark();
+        mark();
+        mark();
+    } else {
+        start_chain(r);
+        vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
+    }
     if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
     else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
     mark();
