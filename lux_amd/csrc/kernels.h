@@ -65,7 +65,8 @@ struct SgmBuffers {
     // Concurrent paths (set all or none): the sweeps of a pass run on side streams, each writing its path cost
     // into its own volume [H][W][cs], summed into agg once per pass. For latency-bound (ranged) layers.
     int seg = 1;                    // narrow per-pixel ranges (caps <= 32) may run several chains per wavefront
-    uint16_t* aggp[3] = {};
+    // With a fourth volume (ranged layers), each pass runs as one launch and pass -1's row path writes aggp[3].
+    uint16_t* aggp[4] = {};
     hipStream_t conc[3] = {};
     hipEvent_t conc_fork = nullptr, conc_join[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)

@@ -764,13 +764,12 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 //         register plane p holds min over j >= p of key(x-p+j, j), i.e. dR candidates of xr = x-p, so
 //         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
 //         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
-// Saves the column-1 states for the border replay.
-template <int NP, bool MASKED, bool VAR, bool ADP>
-__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
+// Saves the column-1 states for the border replay. WTA = false: the path alone, its cost written to a.agg as
+// a volume of its own (FIRST; 0 in the border column), the sum and the winner-take-all left to sgm_finish.
+template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA>
+__device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 {
     constexpr int PF = 8, E = 2 * NP;
-    const int lane = threadIdx.x & 63;
-    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
     const int W = a.W, cs = a.cs, lo = lane * E;
     int D = a.D;
@@ -807,14 +806,14 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
-        ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        if (WTA) ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
     }
     uint32_t myd = 0, myr = 0;
     float mys = 0.f;
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
-        const Cell<NP> agg = ab[j];
+        const Cell<NP> agg = WTA ? ab[j] : Cell<NP>{};
         const int xp = max(x - PF, 0);
         uint32_t el = kInit2, er = kInit2;
         int s0 = 0;
@@ -835,17 +834,21 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
             gprev = gc;
         }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
-        ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        if (WTA) ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
         Cell<NP> lc, fin;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
-#pragma unroll
-        for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
-        if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
         if (x == 1) {
             store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, lc);
             if (lane == 0) a.colmins[y] = (uint16_t)minprev;
         }
         state = lc;
+        if constexpr (!WTA) {
+            store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, lc);
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
+        if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
         // winner-take-all on the final aggregate of cell x
         uint32_t key[E];
         uint32_t best = 0xffffffffu;
@@ -900,10 +903,20 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(W - 1 - (i0 + j), j);
-    if constexpr (!VAR) {
+    if constexpr (!WTA) {
+        Cell<NP> z;
+#pragma unroll
+        for (int k = 0; k < NP; k++) z.v[k] = 0;
+        store_u16<NP>(a.agg + rowcell * cs + lo, lo < cs, z);  // the border column: the replays'
+    } else if constexpr (!VAR) {
         const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
         if (lane == 0) a.dr0[y] = r1;
     }
+}
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
+{
+    row_bwd_wave<NP, MASKED, VAR, ADP, true>(a, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
 }
 
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
@@ -948,6 +961,69 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
             const int dd = a.disp[rowcell + x], xr = x - dd;
             valid[rowcell + x] = (uint8_t)(xr >= 0 && abs(dd - (int)a.dr[rowcell + xr]) <= tol);
         }
+}
+
+// Ranged layers after the fused passes: the final aggregate agg + the four pass -1 path volumes (saturating, so
+// the order of the additions does not matter), stored when asked, and its first-minimum winner-take-all and
+// subpixel (as sgm_row_bwd_wta and wta_finish compute them) for every cell, border columns included. One
+// wavefront per 64 cells of a row; the results are stored coalesced.
+template <int NP>
+__global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* __restrict__ v0,
+                                                     const uint16_t* __restrict__ v1, const uint16_t* __restrict__ v2,
+                                                     const uint16_t* __restrict__ v3)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
+    const int nb = (a.W + 63) >> 6;
+    const int y = wv / nb;
+    if (y >= a.H) return;
+    const int xb = (wv - y * nb) << 6, n = min(64, a.W - xb);
+    const size_t c0 = (size_t)y * a.W + xb;
+    const int cs = a.cs, lo = lane * 2 * NP, lol = lo < cs ? lo : 0;
+    // the 64 cells' ranges, one a lane, read out with v_readlane
+    const uint32_t rs = lane < n ? ((uint32_t)a.vstart[c0 + lane] | ((uint32_t)a.vcap[c0 + lane] << 16)) : 0u;
+    const uint32_t rn = lane < n ? (uint32_t)a.vcount[c0 + lane] : 0u;
+    uint32_t myd = 0;
+    float mys = 0.f;
+#pragma unroll 4
+    for (int j = 0; j < n; j++) {
+        const size_t off = (c0 + j) * cs + lol;
+        Cell<NP> t, u0, u1, u2, u3;
+        ld_u16<NP>(a.agg + off, t);
+        ld_u16<NP>(v0 + off, u0);
+        ld_u16<NP>(v1 + off, u1);
+        ld_u16<NP>(v2 + off, u2);
+        ld_u16<NP>(v3 + off, u3);
+#pragma unroll
+        for (int k = 0; k < NP; k++) t.v[k] = padd(padd(t.v[k], u0.v[k]), padd(padd(u1.v[k], u2.v[k]), u3.v[k]));
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rs, j);
+        const int s0 = (int)(r & 0xffffu), cap = (int)(r >> 16), D = __builtin_amdgcn_readlane((int)rn, j);
+        if (a.write_agg) store_u16<NP>(a.agg + (c0 + j) * cs + lo, lo < cap, t);
+        uint32_t best = 0xffffffffu;
+#pragma unroll
+        for (int e = 0; e < 2 * NP; e++) {
+            const int d = lo + e;
+            const uint32_t v = (e & 1) ? (t.v[e >> 1] >> 16) : (t.v[e >> 1] & 0xffffu);
+            const uint32_t key = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+            best = key < best ? key : best;
+        }
+        best = wave_min(best);
+        const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
+        float v = (float)(d + s0);
+        if (d > 0 && d < D - 1) {
+            const int l = (int)plane_of<NP>(t, d - 1), cc = (int)(best >> 16), rr = (int)plane_of<NP>(t, d + 1);
+            const int den = l - 2 * cc + rr;
+            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - rr)), (float)den);
+        }
+        if (lane == j) {
+            myd = (uint32_t)(d + s0);
+            mys = v;
+        }
+    }
+    if (lane < n) {
+        a.disp[c0 + lane] = (uint16_t)myd;
+        if (a.sub) a.sub[c0 + lane] = mys;
+    }
 }
 
 // ------------------------------------------------------------------ segmented chains (narrow per-pixel ranges)
@@ -1784,14 +1860,15 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
 }
 
 // One pass's independent chains in one launch, for latency-bound (ranged) layers: the border chain (block 0,
-// wave 0: the longest serial path starts first), pass +1's row path, then paths 2, 1 and 3, each path into
-// its own volume (FIRST). One launch instead of one stream per path: streams beyond the device's hardware
+// wave 0: the longest serial path starts first), the row path, then paths 2, 1 and 3. Pass +1's row path is
+// the first aggregate writer; every other path writes its own volume (FIRST), summed by agg_combine3 (pass +1)
+// or sgm_finish (pass -1). One launch instead of one stream per path: streams beyond the device's hardware
 // queues share a queue and run one after another.
 struct PassArgs {
     SweepArgs s[3];   // paths 1, 2, 3
-    RowArgs r;        // pass +1's row path
+    RowArgs r;        // the row path
     ReplayArgs c;     // the border chain
-    int n1, n2, nrow; // chains of paths 1, 2 and of the row path (0 in pass -1)
+    int n1, n2, nrow; // chains of paths 1, 2 and of the row path
 };
 template <int NP, int DIR, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
@@ -1802,13 +1879,12 @@ __global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
         return;
     }
     int w = __builtin_amdgcn_readfirstlane((int)(((blockIdx.x - 1) * kBlock + threadIdx.x) >> 6));
-    if (DIR > 0) {
-        if (w < p.nrow) {
-            row_fwd_wave<NP, MASKED, false, VAR, ADP>(p.r, w, lane);
-            return;
-        }
-        w -= p.nrow;
+    if (w < p.nrow) {
+        if (DIR > 0) row_fwd_wave<NP, MASKED, false, VAR, ADP>(p.r, w, lane);
+        else row_bwd_wave<NP, MASKED, VAR, ADP, false>(p.r, w, lane);  // into its own volume
+        return;
     }
+    w -= p.nrow;
     if (w < p.n2) {
         sweep_wave<NP, DIR, 2, true, MASKED, VAR, ADP>(p.s[1], w, lane);
         return;
@@ -2217,7 +2293,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // Ranged layers given path volumes: each pass's border chain, row path (pass +1) and paths 1-3 in one launch
     // (sgm_pass), then the volumes summed into agg.
     constexpr bool kFusable = SEG == 0 && VAR;
-    const bool fused = kFusable && conc;
+    const bool fused = kFusable && conc && b.aggp[3];
     auto launch_pass = [&](auto dirc, const ReplayArgs& rr) {
         constexpr int DIR = decltype(dirc)::value;
         if constexpr (kFusable) {
@@ -2227,14 +2303,16 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
                 p.s[k].agg = b.aggp[k];
             }
             p.r = ra;
+            if (DIR < 0) p.r.agg = b.aggp[3];
             p.c = rr;
             p.n1 = nd;
             p.n2 = nv;
-            p.nrow = DIR > 0 ? nh : 0;
+            p.nrow = nh;
             const int waves = p.nrow + nv + 2 * nd;
             sgm_pass<NP, DIR, MASKED, VAR, ADP><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
-            agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
-                                                           (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
+            if (DIR > 0)
+                agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
+                                                               (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
         }
     };
     ReplayArgs r;
@@ -2296,14 +2374,23 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         start_chain(r);
         vertical_diagonal(std::integral_constant<int, -1>(), b.cols + colsz, b.colmins + colminsz);
     }
-    if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
-    else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
-    mark();
+    if (fused) {
+        mark();  // the row path ran in the pass launch
+    } else {
+        if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
+        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        mark();
+    }
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
     end_chain();
     sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_minus);
     mark();
-    wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
+    if (fused) {  // the sum of pass -1's volumes and the winner-take-all of every cell
+        const size_t waves = (size_t)g.H * ((g.W + 63) / 64);
+        sgm_finish<NP><<<(unsigned)((waves + 3) / 4), kBlock, 0, s>>>(ra, b.aggp[0], b.aggp[1], b.aggp[2], b.aggp[3]);
+    } else {
+        wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
+    }
     mark();
     return hipGetLastError();
 }
