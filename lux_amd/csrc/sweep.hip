@@ -356,25 +356,52 @@ __global__ __launch_bounds__(kRangeBlock) void range_window_kernel(const uint32_
 }
 
 // Skipped pixels get their raw bytes here (0 over the planes, 255 over the padding, as Pixel uses them, :242); the
-// computed ones are listed for the cost kernel (in no particular order: each is costed on its own).
-__global__ void skip_list_kernel(const uint8_t* skip, const uint16_t* vcount, const uint16_t* vcap, size_t cells, uint8_t* raw,
-                                 int cs, uint32_t* list, uint32_t* nlist)
+// computed ones are listed for the cost kernel (in no particular order: each is costed on its own). One wavefront
+// per kSkipGroups x 64 cells: each skipped cell's bytes are written by the whole wavefront, four bytes a lane (cs is
+// a multiple of 4; the cells' counts and caps are read 64 at a time and taken out with v_readlane), and the
+// computed cells take one list reservation per wavefront.
+constexpr int kSkipGroups = 4;
+__global__ __launch_bounds__(256) void skip_list_kernel(const uint8_t* skip, const uint16_t* vcount, const uint16_t* vcap,
+                                                        size_t cells, uint8_t* raw, int cs, uint32_t* list, uint32_t* nlist)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < cells;
-    const bool sk = live && skip[i];
-    if (sk) {
-        const int cnt = vcount[i], cp = vcap[i];
-        uint8_t* r = raw + i * cs;
-        for (int j = 0; j < cp; j++) r[j] = j < cnt ? 0 : 255;
-    }
-    const bool comp = live && !sk;
-    const uint64_t m = __ballot(comp);
     const int lane = threadIdx.x & 63;
+    const size_t c0 = (((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64 * kSkipGroups);
+    if (c0 >= cells) return;
+    uint64_t comp[kSkipGroups];
+    uint32_t total = 0;
+#pragma unroll
+    for (int g = 0; g < kSkipGroups; g++) {
+        const size_t i = c0 + (size_t)g * 64 + lane;
+        const bool live = i < cells;
+        const bool sk = live && skip[i];
+        const int mycnt = live ? vcount[i] : 0, mycap = live ? vcap[i] : 0;
+        comp[g] = __ballot(live && !sk);
+        total += (uint32_t)__popcll(comp[g]);
+        for (uint64_t sm = __ballot(sk); sm; sm &= sm - 1) {
+            const int b = __builtin_ctzll(sm);
+            const size_t c = c0 + (size_t)g * 64 + (size_t)b;
+            const int cnt = __builtin_amdgcn_readlane(mycnt, b), cp = __builtin_amdgcn_readlane(mycap, b);
+            for (int j = lane * 4; j < cp; j += 256) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) w |= (j + e < cnt ? 0u : 0xffu) << (8 * e);
+                if (j + 4 <= cp) {
+                    *(uint32_t*)(raw + c * cs + j) = w;
+                } else {
+                    for (int e = 0; j + e < cp; e++) raw[c * cs + j + e] = (uint8_t)(w >> (8 * e));
+                }
+            }
+        }
+    }
     uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(nlist, (uint32_t)__popcll(m));
+    if (lane == 0 && total) base = atomicAdd(nlist, total);
     base = (uint32_t)__shfl((int)base, 0);
-    if (comp) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
+#pragma unroll
+    for (int g = 0; g < kSkipGroups; g++) {
+        const uint64_t m = comp[g];
+        if ((m >> lane) & 1ull) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)(c0 + (size_t)g * 64 + lane);
+        base += (uint32_t)__popcll(m);
+    }
 }
 
 // The layer's depth (Compute :225-231): Planes[start + first minimum], the WTA index the sweeps leave.
@@ -484,8 +511,9 @@ hipError_t launch_sweep_cost(const SweepDev& a, int W, int H, int scale, const u
         if (ws->get("sw_list", cells * 4, &dl, &e2) || ws->get("sw_nlist", 256, &dn, &e2)) return hipErrorOutOfMemory;
         hipError_t e = launch_fill_u32((uint32_t*)dn, 1, 0u, s);
         if (e != hipSuccess) return e;
-        skip_list_kernel<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(skip, vcount, vcap, cells, raw, cs, (uint32_t*)dl,
-                                                                        (uint32_t*)dn);
+        const size_t waves = (cells + 64 * kSkipGroups - 1) / (64 * kSkipGroups);
+        skip_list_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(skip, vcount, vcap, cells, raw, cs, (uint32_t*)dl,
+                                                                     (uint32_t*)dn);
         list = (const uint32_t*)dl;
         nlist = (const uint32_t*)dn;
         nseg = computed;
