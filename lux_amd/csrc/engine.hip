@@ -412,7 +412,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
             }
         std::string e2;
         void* q;
-        for (const char* nm : {"aggp0", "aggp1", "aggp2", "aggp3"})
+        for (const char* nm : {"aggp0", "aggp1", "aggp2"})
             if (ws.get(nm, mx, &q, &e2)) conc = nullptr;  // over the budget: the sweeps stay one after another
     }
     const auto lv = pyramid_levels(p);
@@ -493,16 +493,16 @@ bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
 {
     if (!c || !c->side[0]) return false;
     const size_t bytes = (size_t)w->g.W * w->g.H * w->g.cs * 2;
-    static const char* names[4] = {"aggp0", "aggp1", "aggp2", "aggp3"};
-    void* p[4];
+    static const char* names[3] = {"aggp0", "aggp1", "aggp2"};
+    void* p[3];
     std::string err;
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 3; k++)
         if (ws.get(names[k], bytes, &p[k], &err)) return false;
     for (int k = 0; k < 3; k++) {
+        w->b.aggp[k] = (uint16_t*)p[k];
         w->b.conc[k] = c->side[k];
         w->b.conc_join[k] = c->join[k];
     }
-    for (int k = 0; k < 4; k++) w->b.aggp[k] = (uint16_t*)p[k];
     w->b.conc_fork = c->fork;
     return true;
 }

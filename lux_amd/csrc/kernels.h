@@ -65,8 +65,8 @@ struct SgmBuffers {
     // Concurrent paths (set all or none): the sweeps of a pass run on side streams, each writing its path cost
     // into its own volume [H][W][cs], summed into agg once per pass. For latency-bound (ranged) layers.
     int seg = 1;                    // narrow per-pixel ranges (caps <= 32) may run several chains per wavefront
-    // With a fourth volume (ranged layers), each pass runs as one launch and pass -1's row path writes aggp[3].
-    uint16_t* aggp[4] = {};
+    // Ranged layers given the volumes run each pass as one launch; pass -1 adds its paths 1-3 to pass +1's volumes.
+    uint16_t* aggp[3] = {};
     hipStream_t conc[3] = {};
     hipEvent_t conc_fork = nullptr, conc_join[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)

@@ -764,8 +764,8 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 //         register plane p holds min over j >= p of key(x-p+j, j), i.e. dR candidates of xr = x-p, so
 //         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
 //         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
-// Saves the column-1 states for the border replay. WTA = false: the path alone, its cost written to a.agg as
-// a volume of its own (FIRST; 0 in the border column), the sum and the winner-take-all left to sgm_finish.
+// Saves the column-1 states for the border replay. WTA = false: the path alone, its cost added to a.agg (the
+// rest of the sum and the winner-take-all are sgm_finish's).
 template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA>
 __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 {
@@ -806,14 +806,14 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
-        if (WTA) ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
     }
     uint32_t myd = 0, myr = 0;
     float mys = 0.f;
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
-        const Cell<NP> agg = WTA ? ab[j] : Cell<NP>{};
+        const Cell<NP> agg = ab[j];
         const int xp = max(x - PF, 0);
         uint32_t el = kInit2, er = kInit2;
         int s0 = 0;
@@ -834,7 +834,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             gprev = gc;
         }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
-        if (WTA) ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
         Cell<NP> lc, fin;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
         if (x == 1) {
@@ -842,13 +842,10 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             if (lane == 0) a.colmins[y] = (uint16_t)minprev;
         }
         state = lc;
-        if constexpr (!WTA) {
-            store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, lc);
-            return;
-        }
 #pragma unroll
         for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
-        if (a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+        if (!WTA || a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+        if constexpr (!WTA) return;
         // winner-take-all on the final aggregate of cell x
         uint32_t key[E];
         uint32_t best = 0xffffffffu;
@@ -903,12 +900,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(W - 1 - (i0 + j), j);
-    if constexpr (!WTA) {
-        Cell<NP> z;
-#pragma unroll
-        for (int k = 0; k < NP; k++) z.v[k] = 0;
-        store_u16<NP>(a.agg + rowcell * cs + lo, lo < cs, z);  // the border column: the replays'
-    } else if constexpr (!VAR) {
+    if constexpr (WTA && !VAR) {
         const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
         if (lane == 0) a.dr0[y] = r1;
     }
@@ -963,14 +955,13 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
         }
 }
 
-// Ranged layers after the fused passes: the final aggregate agg + the four pass -1 path volumes (saturating, so
-// the order of the additions does not matter), stored when asked, and its first-minimum winner-take-all and
-// subpixel (as sgm_row_bwd_wta and wta_finish compute them) for every cell, border columns included. One
-// wavefront per 64 cells of a row; the results are stored coalesced.
+// Ranged layers after the fused passes: the final aggregate, agg + the three path volumes (saturating, so the
+// order of the additions does not matter), stored when asked, and its first-minimum winner-take-all and subpixel
+// (as sgm_row_bwd_wta and wta_finish compute them) for every cell, border columns included. One wavefront per 64
+// cells of a row; the results are stored coalesced.
 template <int NP>
 __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* __restrict__ v0,
-                                                     const uint16_t* __restrict__ v1, const uint16_t* __restrict__ v2,
-                                                     const uint16_t* __restrict__ v3)
+                                                     const uint16_t* __restrict__ v1, const uint16_t* __restrict__ v2)
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
@@ -985,17 +976,15 @@ __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* 
     const uint32_t rn = lane < n ? (uint32_t)a.vcount[c0 + lane] : 0u;
     uint32_t myd = 0;
     float mys = 0.f;
-#pragma unroll 4
     for (int j = 0; j < n; j++) {
         const size_t off = (c0 + j) * cs + lol;
-        Cell<NP> t, u0, u1, u2, u3;
+        Cell<NP> t, u0, u1, u2;
         ld_u16<NP>(a.agg + off, t);
         ld_u16<NP>(v0 + off, u0);
         ld_u16<NP>(v1 + off, u1);
         ld_u16<NP>(v2 + off, u2);
-        ld_u16<NP>(v3 + off, u3);
 #pragma unroll
-        for (int k = 0; k < NP; k++) t.v[k] = padd(padd(t.v[k], u0.v[k]), padd(padd(u1.v[k], u2.v[k]), u3.v[k]));
+        for (int k = 0; k < NP; k++) t.v[k] = padd(padd(t.v[k], u0.v[k]), padd(u1.v[k], u2.v[k]));
         const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rs, j);
         const int s0 = (int)(r & 0xffffu), cap = (int)(r >> 16), D = __builtin_amdgcn_readlane((int)rn, j);
         if (a.write_agg) store_u16<NP>(a.agg + (c0 + j) * cs + lo, lo < cap, t);
@@ -1861,9 +1850,10 @@ __global__ __launch_bounds__(64) void sgm_replay_chain(ReplayArgs a, const int* 
 
 // One pass's independent chains in one launch, for latency-bound (ranged) layers: the border chain (block 0,
 // wave 0: the longest serial path starts first), the row path, then paths 2, 1 and 3. Pass +1's row path is
-// the first aggregate writer; every other path writes its own volume (FIRST), summed by agg_combine3 (pass +1)
-// or sgm_finish (pass -1). One launch instead of one stream per path: streams beyond the device's hardware
-// queues share a queue and run one after another.
+// the first aggregate writer and its paths 1-3 write a volume each (FIRST); pass -1 adds its row path to agg and
+// its paths 1-3 to those volumes (the border columns, which each pass leaves to its replays, keep the other
+// pass's costs), and sgm_finish sums them. One launch instead of one stream per path: streams beyond the
+// device's hardware queues share a queue and run one after another.
 struct PassArgs {
     SweepArgs s[3];   // paths 1, 2, 3
     RowArgs r;        // the row path
@@ -1886,15 +1876,15 @@ __global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
     }
     w -= p.nrow;
     if (w < p.n2) {
-        sweep_wave<NP, DIR, 2, true, MASKED, VAR, ADP>(p.s[1], w, lane);
+        sweep_wave<NP, DIR, 2, (DIR > 0), MASKED, VAR, ADP>(p.s[1], w, lane);
         return;
     }
     w -= p.n2;
     if (w < p.n1) {
-        sweep_wave<NP, DIR, 1, true, MASKED, VAR, ADP>(p.s[0], w, lane);
+        sweep_wave<NP, DIR, 1, (DIR > 0), MASKED, VAR, ADP>(p.s[0], w, lane);
         return;
     }
-    sweep_wave<NP, DIR, 3, true, MASKED, VAR, ADP>(p.s[2], w - p.n1, lane);  // returns past the last chain
+    sweep_wave<NP, DIR, 3, (DIR > 0), MASKED, VAR, ADP>(p.s[2], w - p.n1, lane);  // returns past the last chain
 }
 
 // Pass +1's seed-table entries of paths 1 and 3 for the border cell: recomputed for the last visit that wrote
@@ -1991,6 +1981,13 @@ __global__ void fill_u16(uint16_t* p, size_t n, uint16_t v)
 __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+// out = sat(a0 + a1 + a2 + a3) on u16 pairs (pass +1's aggregate when its paths are still in their volumes).
+__global__ void agg_sum4(uint32_t* out, const uint32_t* a0, const uint32_t* a1, const uint32_t* a2, const uint32_t* a3,
+                         size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = padd(padd(a0[i], a1[i]), padd(a2[i], a3[i]));
 }
 // agg = sat(agg + a1 + a2 + a3) on u16 pairs: the path costs of concurrently run sweeps. Saturating addition of
 // non-negative values is min(sum, 65535) in any order, so this equals adding them one sweep after another.
@@ -2293,7 +2290,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // Ranged layers given path volumes: each pass's border chain, row path (pass +1) and paths 1-3 in one launch
     // (sgm_pass), then the volumes summed into agg.
     constexpr bool kFusable = SEG == 0 && VAR;
-    const bool fused = kFusable && conc && b.aggp[3];
+    const bool fused = kFusable && conc;
     auto launch_pass = [&](auto dirc, const ReplayArgs& rr) {
         constexpr int DIR = decltype(dirc)::value;
         if constexpr (kFusable) {
@@ -2303,16 +2300,12 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
                 p.s[k].agg = b.aggp[k];
             }
             p.r = ra;
-            if (DIR < 0) p.r.agg = b.aggp[3];
             p.c = rr;
             p.n1 = nd;
             p.n2 = nv;
             p.nrow = nh;
             const int waves = p.nrow + nv + 2 * nd;
             sgm_pass<NP, DIR, MASKED, VAR, ADP><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
-            if (DIR > 0)
-                agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
-                                                               (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
         }
     };
     ReplayArgs r;
@@ -2355,7 +2348,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     end_chain();
     sgm_replay_add<NP, VAR><<<blocks(g.H), kBlock, 0, s>>>(r, b.cellvis_plus);
     mark();
-    if (agg_pass1) {
+    if (agg_pass1 && fused) {  // pass +1's paths 1-3 are still in their volumes
+        agg_sum4<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)agg_pass1, (const uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
+                                                     (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
+    } else if (agg_pass1) {
         hipError_t e = hipMemcpyAsync(agg_pass1, b.agg, (size_t)g.W * g.H * g.cs * 2, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
     }
@@ -2387,7 +2383,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     mark();
     if (fused) {  // the sum of pass -1's volumes and the winner-take-all of every cell
         const size_t waves = (size_t)g.H * ((g.W + 63) / 64);
-        sgm_finish<NP><<<(unsigned)((waves + 3) / 4), kBlock, 0, s>>>(ra, b.aggp[0], b.aggp[1], b.aggp[2], b.aggp[3]);
+        sgm_finish<NP><<<(unsigned)((waves + 3) / 4), kBlock, 0, s>>>(ra, b.aggp[0], b.aggp[1], b.aggp[2]);
     } else {
         wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
     }
