@@ -116,9 +116,36 @@ __global__ void box3_kernel(const T* src, int W, int H, int stride, int ch, int 
     }
 }
 
+// The same filter on four interleaved 8-bit channels: one 32-bit load per tap, the four sums from its bytes, and the
+// reciprocal of the window's size (9, 6 or 4 pixels away from tiny images) as a constant: the literal quotients are
+// the correctly rounded ones the division gives.
+__global__ void box3_rgba8_kernel(const uint32_t* src, int W, int H, int stride, int maxval, uint32_t* dst)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int x0 = max(x - 1, 0), x1 = min(x + 1, W - 1), y0 = max(y - 1, 0), y1 = min(y + 1, H - 1);
+    const int n = (x1 - x0 + 1) * (y1 - y0 + 1);
+    const float inv = n == 9 ? 1.0f / 9.0f : n == 6 ? 1.0f / 6.0f : n == 4 ? 0.25f : __fdiv_rn(1.0f, (float)n);
+    int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int yy = y0; yy <= y1; yy++)
+        for (int xx = x0; xx <= x1; xx++) {
+            const uint32_t v = src[(size_t)yy * stride + xx];
+            s0 += (int)(v & 0xffu);
+            s1 += (int)((v >> 8) & 0xffu);
+            s2 += (int)((v >> 16) & 0xffu);
+            s3 += (int)(v >> 24);
+        }
+    const auto q = [&](int sum) { return (uint32_t)min(max(__float2int_rn(__fmul_rn((float)sum, inv)), 0), maxval); };
+    dst[(size_t)y * W + x] = q(s0) | (q(s1) << 8) | (q(s2) << 16) | (q(s3) << 24);
+}
+
 hipError_t launch_box3(const void* src, bool img16, int W, int H, int stride, int ch, int maxval, void* dst, hipStream_t s)
 {
     dim3 blk(32, 8), grd((W + 31) / 32, (H + 7) / 8);
+    if (!img16 && ch == 4 && ((uintptr_t)src & 3) == 0 && ((uintptr_t)dst & 3) == 0) {
+        box3_rgba8_kernel<<<grd, blk, 0, s>>>((const uint32_t*)src, W, H, stride, maxval, (uint32_t*)dst);
+        return hipGetLastError();
+    }
     if (img16) box3_kernel<uint16_t><<<grd, blk, 0, s>>>((const uint16_t*)src, W, H, stride, ch, maxval, (uint16_t*)dst);
     else box3_kernel<uint8_t><<<grd, blk, 0, s>>>((const uint8_t*)src, W, H, stride, ch, maxval, (uint8_t*)dst);
     return hipGetLastError();

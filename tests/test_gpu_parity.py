@@ -380,7 +380,8 @@ def test_layer_through_lumen_dump(tmp_path):
 
 # ------------------------------------------------------------------ box filter
 @pytest.mark.parametrize("shape,dtype,bits", [((97, 131), np.uint8, 8), ((64, 48), np.uint16, 10), ((1, 5), np.uint8, 8),
-                                              ((7, 1), np.uint16, 16), ((33, 20, 4), np.uint8, 8)])
+                                              ((7, 1), np.uint16, 16), ((33, 20, 4), np.uint8, 8), ((1, 1, 4), np.uint8, 8),
+                                              ((1, 5, 4), np.uint8, 8), ((2, 3, 4), np.uint8, 8), ((130, 97, 4), np.uint8, 8)])
 def test_box3(shape, dtype, bits):
     rng = np.random.default_rng(sum(shape))
     img = rng.integers(0, 1 << bits, shape).astype(dtype)
