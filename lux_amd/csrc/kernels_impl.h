@@ -422,6 +422,41 @@ struct ChainWords {
     }
 };
 
+// The adaptive P2 of a chain's steps, 64 positions a VGPR. Step i's P2 depends only on the guidance words of
+// positions i and i-1 (of the chain's start state for i = 0), so when a block's words are in (loaded a block
+// earlier, by ChainWords) the 64 lanes compute the block's P2 at once (lane j: position 64k+j, its predecessor's
+// word from lane j-1 by DPP, lane 0 taking the previous block's last word) and each step reads its own with
+// v_readlane: the same arithmetic as p2_step per step, a 64th of the instructions.
+struct ChainP2 {
+    ChainWords g;      // the guidance words (g.at(pos) is position pos's word)
+    uint32_t p2 = 0;   // P2 of the current block's positions
+    template <typename F>
+    __device__ __forceinline__ void init(const F& load, int lane, const AdaptP2& q, uint32_t p2c, uint32_t gprev0)
+    {
+        g.init(load, lane);
+        p2 = p2_step(q, p2c, g.b0, shr1(g.b0, gprev0));
+    }
+    __device__ __forceinline__ uint32_t at(int pos) const
+    {
+        return (uint32_t)__builtin_amdgcn_readlane((int)p2, pos & 63);
+    }
+    // after the last use of position pos: at the end of its block, the next block's P2 from its words, then the
+    // freed register takes the block after next
+    template <typename F>
+    __device__ __forceinline__ void advance(int pos, const F& load, int lane, const AdaptP2& q, uint32_t p2c)
+    {
+        if ((pos & 63) == 63) {
+            if (pos & 64) {
+                p2 = p2_step(q, p2c, g.b0, shr1(g.b0, (uint32_t)__builtin_amdgcn_readlane((int)g.b1, 63)));
+                g.b1 = load(pos + 65 + lane);
+            } else {
+                p2 = p2_step(q, p2c, g.b1, shr1(g.b1, (uint32_t)__builtin_amdgcn_readlane((int)g.b0, 63)));
+                g.b0 = load(pos + 65 + lane);
+            }
+        }
+    }
+};
+
 // One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
 // (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
 // 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
@@ -452,7 +487,7 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
     const int lo = lane * 2 * NP;
 
     Cell<NP> state;
-    uint32_t minprev = 2000, gprev = 0;  // gprev: guidance of the state's cell (0: the initial fill)
+    uint32_t minprev = 2000, gprev = 0;  // gprev: guidance of the start state's cell (0: the initial fill)
     Range from;
     set_init(state);
     if (DIR < 0 && P != 0 && y == H - 1) {
@@ -487,9 +522,10 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
         return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
     };
     const auto gq_at = [&](int pos) { return a.ap.guid[cell0 + (ptrdiff_t)min(pos, n - 1) * step]; };
-    ChainWords rs, gq;
+    ChainWords rs;
+    ChainP2 gq;
     if constexpr (VAR) rs.init(rs_at, lane);
-    if constexpr (ADP) gq.init(gq_at, lane);
+    if constexpr (ADP) gq.init(gq_at, lane, a.ap, a.p2, gprev);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
@@ -509,13 +545,13 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
             reframe<NP, VAR>(state, from, cs_, cc_, lane, el, er);
             make_mask<NP>(lane, cc_, mask, active);
         }
-        const uint32_t gc = ADP ? gq.at(i) : 0u;
-        const uint32_t p2 = ADP ? p2_step(a.ap, a.p2, gc, gprev) : a.p2;
+        const uint32_t gc = ADP ? gq.g.at(i) : 0u;
+        const uint32_t p2 = ADP ? gq.at(i) : a.p2;
         const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
         ld_raw_w<NP>(a.raw + cp * cs + lol, rb[j]);
         if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
         if constexpr (VAR) rs.advance(i, rs_at, lane);
-        if constexpr (ADP) gq.advance(i, gq_at, lane);
+        if constexpr (ADP) gq.advance(i, gq_at, lane, a.ap, a.p2);
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
         if (x != bx) {
@@ -545,7 +581,6 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
             if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
         }
         state = lc;
-        gprev = gc;
         cell += step;
         x += DX;
         y += DY;
@@ -653,16 +688,16 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
     uint64_t bl0 = 0, br0 = 0, bl1 = 0, br1 = 0;  // cells 64k .. 64k+63 in the registers of parity k & 1
     RawW<NP> rb[COST ? 1 : PF];
     Range from;
-    uint32_t gprev = 0;  // ADP: guidance of the state's cell
     // VAR / ADP: the row's range (start | cap << 16) and guidance words, 64 cells a VGPR (ChainWords)
     const auto rs_at = [&](int pos) {
         const size_t c = rowcell + min(pos, W - 1);
         return (uint32_t)a.vstart[c] | ((uint32_t)a.vcap[c] << 16);
     };
     const auto gq_at = [&](int pos) { return a.ap.guid[rowcell + min(pos, W - 1)]; };
-    ChainWords rs, gq;
+    ChainWords rs;
+    ChainP2 gq;
     if constexpr (VAR) rs.init(rs_at, lane);
-    if constexpr (ADP) gq.init(gq_at, lane);
+    if constexpr (ADP) gq.init(gq_at, lane, a.ap, a.p2, 0u);
     if constexpr (COST) {
         const uint64_t c0 = crow[0];
 #pragma unroll
@@ -721,10 +756,8 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
         }
         uint32_t p2 = a.p2;
         if constexpr (ADP) {
-            const uint32_t gc = gq.at(x);
-            gq.advance(x, gq_at, lane);
-            p2 = p2_step(a.ap, a.p2, gc, gprev);
-            gprev = gc;
+            p2 = gq.at(x);
+            gq.advance(x, gq_at, lane, a.ap, a.p2);
         }
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
@@ -787,7 +820,6 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
     Range from;
-    uint32_t gprev = 0;  // ADP: guidance of the state's cell
     // VAR / ADP: range (start | cap << 16), count and guidance words by position W-1-x, 64 a VGPR (ChainWords)
     const auto cell_at = [&](int pos) { return rowcell + max(W - 1 - pos, 0); };
     const auto rs_at = [&](int pos) {
@@ -796,12 +828,13 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     };
     const auto rn_at = [&](int pos) { return (uint32_t)a.vcount[cell_at(pos)]; };
     const auto gq_at = [&](int pos) { return a.ap.guid[cell_at(pos)]; };
-    ChainWords rs, rn, gq;
+    ChainWords rs, rn;
+    ChainP2 gq;
     if constexpr (VAR) {
         rs.init(rs_at, lane);
         rn.init(rn_at, lane);
     }
-    if constexpr (ADP) gq.init(gq_at, lane);
+    if constexpr (ADP) gq.init(gq_at, lane, a.ap, a.p2, 0u);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
@@ -828,10 +861,8 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         }
         uint32_t p2 = a.p2;
         if constexpr (ADP) {
-            const uint32_t gc = gq.at(W - 1 - x);
-            gq.advance(W - 1 - x, gq_at, lane);
-            p2 = p2_step(a.ap, a.p2, gc, gprev);
-            gprev = gc;
+            p2 = gq.at(W - 1 - x);
+            gq.advance(W - 1 - x, gq_at, lane, a.ap, a.p2);
         }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
         ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
