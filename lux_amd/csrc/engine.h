@@ -40,8 +40,8 @@ struct Workspace {
     Budget* budget = nullptr;
     std::map<std::string, DevBuf> bufs;
     std::set<std::string> filled;    // buffers whose contents outlive a call (cleared when reallocated or released)
-    const Plan* plan = nullptr;      // whose visit lists are in "visits"
     void* sync_zeroed = nullptr;     // band flag array known to hold no live epoch
+    std::map<std::string, std::vector<float>> uploaded;  // host copies of float buffers uploaded once (when filled)
     int epoch = 0;
     int force_literal = 0;           // LUX_SGM_OPT_LITERAL: layers of <= 1024 lanes take the line-buffer-literal path
     int lit_resident[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // resident literal wavefronts [npl 2/4/0][adaptive]

@@ -51,7 +51,6 @@ int Workspace::get(const char* name, size_t bytes, void** out, std::string* err)
             return hip_status(e, "hipMalloc", err);
         }
         b.n = bytes;
-        if (name == std::string("visits")) plan = nullptr;
     }
     *out = b.p;
     return 0;
@@ -73,7 +72,6 @@ void Workspace::release()
     }
     bufs.clear();
     filled.clear();
-    plan = nullptr;
     sync_zeroed = nullptr;
 }
 
@@ -111,6 +109,14 @@ int check_layer(const lux_sgm_layer_params* p, std::string* err)
     return 0;
 }
 
+// Each plan's visit lists have a buffer of their own, uploaded once: a pyramid's levels (or a server's request
+// sizes) each keep theirs instead of re-uploading them per request.
+static std::string visits_name(const Plan& plan)
+{
+    return "visits " + std::to_string(plan.W) + "x" + std::to_string(plan.H) + " T" + std::to_string(plan.T) +
+           (plan.xsw ? " xsw" : "");
+}
+
 std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes)
 {
     const size_t cells = (size_t)g.W * g.H;
@@ -130,7 +136,7 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
         {"band_boundary", (size_t)6 * g.W * g.cs * 2},
         {"band_boundary_min", (size_t)6 * g.W * 4},
         {"band_sync", (size_t)(4 + 2 * band_tiles(g.W, g.H)) * sizeof(int)},
-        {"visits", (nv + 4) * sizeof(int)},
+        {visits_name(plan), (nv + 4) * sizeof(int)},
         {"disp_tmp", cells * 2},
     };
     if (census) {
@@ -200,7 +206,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
         ptr[nb.name] = q;
     }
     void *raw = ptr["raw"], *agg = ptr["agg"], *seeds = ptr["seeds"], *seedmins = ptr["seedmins"], *cols = ptr["cols"],
-         *colmins = ptr["colmins"], *vis = ptr["visits"], *partial = ptr["replay_partial"], *drb = ptr["wta_dr"],
+         *colmins = ptr["colmins"], *vis = ptr[visits_name(*plan)], *partial = ptr["replay_partial"], *drb = ptr["wta_dr"],
          *dr0 = ptr["wta_dr0"], *bnd = ptr["band_boundary"], *bndmin = ptr["band_boundary_min"], *sync = ptr["band_sync"];
     const int tiles = band_tiles(g.W, g.H);
     if (sync != ws.sync_zeroed) {  // a new buffer: no flag may hold a live epoch
@@ -209,7 +215,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
         ws.sync_zeroed = sync;
     }
     if (++ws.epoch <= 0) ws.epoch = 1;
-    if (ws.plan != plan.get()) {  // upload the visit lists once per plan
+    if (!ws.filled.count(visits_name(*plan))) {  // upload the visit lists once per plan
         const size_t np = plan->plus.size(), nm = plan->minus.size();
         hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), np * sizeof(int), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
@@ -221,7 +227,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
             e = hipMemcpyAsync((int*)vis + np + nm + 2 * (size_t)g.H, plan->cellvis_minus.data(),
                                plan->cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return hip_status(e, "upload plan", err);
-        ws.plan = plan.get();
+        ws.filled.insert(visits_name(*plan));
     }
     if (census) {
         w->b.cl = (const uint64_t*)ptr["census_l"];

@@ -552,6 +552,20 @@ struct RangeStats {
 // RangeFromPrevious on the device for a W x H layer from the previous layer's depth and skip (device), writing
 // start/count/cap (device) and returning the statistics (synchronises the stream). `tag` keeps the buffers of
 // two uses apart (the pyramid asks for align 8 and align 2 of one layer).
+// Copies n floats into the workspace buffer `name` (at dst) unless it already holds them: the plane lists and their
+// boundaries repeat across a pyramid's levels and its calls.
+static hipError_t upload_once(Workspace& ws, const char* name, void* dst, const float* src, int n, hipStream_t s)
+{
+    std::vector<float>& held = ws.uploaded[name];
+    if (ws.filled.count(name) && held.size() == (size_t)n && std::memcmp(held.data(), src, (size_t)n * 4) == 0)
+        return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    held.assign(src, src + n);
+    ws.filled.insert(name);
+    return hipSuccess;
+}
+
 int ranges_from_previous(Workspace& ws, const float* planes, int n, const float* d_prev, const uint8_t* d_pskip, int pw, int ph,
                          int W, int H, int margin, int align, const uint8_t* d_skip, uint16_t* d_start, uint16_t* d_count,
                          uint16_t* d_cap, RangeStats* out, hipStream_t s, std::string* err)
@@ -570,7 +584,7 @@ int ranges_from_previous(Workspace& ws, const float* planes, int n, const float*
         (rc = ws.get("rg_stats", kStats * 8, &dst, err)) || (rc = ws.get("rg_hwin", np * 4, &dh, err)) ||
         (rc = ws.get("rg_vwin", np * 4, &dv, err)) || (rc = ws.get("rg_part", (size_t)nblocks * kStats * 8, &dpart, err)))
         return rc;
-    hipError_t e = hipMemcpyAsync(db, b.data(), (size_t)n * 4, hipMemcpyHostToDevice, s);
+    hipError_t e = upload_once(ws, "rg_bounds", db, b.data(), n, s);
     if (e != hipSuccess) return hip_status(e, "upload boundaries", err);
     range_index_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(d_prev, np, (const float*)db, n, mono, (uint16_t*)didx);
     const float fy = (float)(ph - 1) / (float)(H - 1), fx = (float)(pw - 1) / (float)(W - 1);
@@ -988,10 +1002,10 @@ int dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_
     }
     void *dp8, *dp2 = nullptr;
     if ((rc = ws.get("py_planes8", (size_t)n8 * 4, &dp8, &c->err))) return rc;
-    hipError_t e = hipMemcpyAsync(dp8, planes8, (size_t)n8 * 4, hipMemcpyHostToDevice, s);
+    hipError_t e = upload_once(ws, "py_planes8", dp8, planes8, n8, s);
     if (e == hipSuccess && planes2) {
         if ((rc = ws.get("py_planes2", (size_t)n2 * 4, &dp2, &c->err))) return rc;
-        e = hipMemcpyAsync(dp2, planes2, (size_t)n2 * 4, hipMemcpyHostToDevice, s);
+        e = upload_once(ws, "py_planes2", dp2, planes2, n2, s);
     }
     if (e != hipSuccess) return hip_fail(c, e, "upload planes");
     lux_sgm_sweep sw{};
@@ -1096,14 +1110,14 @@ int dense_pyramid(lux_sgm_ctx* c, const lux_sgm_dense_pyramid_params* pp, int32_
             e = hipMemcpyAsync(depths[i], dd, cells * 4, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s);
             if (e != hipSuccess) return hip_fail(c, e, "download depth");
         }
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(c, e, "dense pyramid");
         d_prev = (const float*)dd;
         d_pskip = (const uint8_t*)dsk;
         pw = W;
         ph = H;
     }
-    return 0;
+    // one wait at the end: each level's range statistics already wait for the level before
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "dense pyramid");
 }
 
 }  // namespace
