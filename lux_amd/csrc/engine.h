@@ -51,6 +51,10 @@ struct Workspace {
     Workspace& operator=(const Workspace&) = delete;
     ~Workspace() { release(); }
     int get(const char* name, size_t bytes, void** out, std::string* err);
+    // A small pinned host buffer (grow-only) for read-backs the host waits for.
+    int pinned(size_t bytes, void** out, std::string* err);
+    void* host_pinned = nullptr;
+    size_t host_pinned_n = 0;
     uint64_t held() const;
     void release();
 };

@@ -63,6 +63,23 @@ uint64_t Workspace::held() const
     return s;
 }
 
+int Workspace::pinned(size_t bytes, void** out, std::string* err)
+{
+    if (host_pinned_n < bytes) {
+        if (host_pinned) (void)hipHostFree(host_pinned);
+        host_pinned = nullptr;
+        host_pinned_n = 0;
+        hipError_t e = hipHostMalloc(&host_pinned, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            host_pinned = nullptr;
+            return hip_status(e, "hipHostMalloc", err);
+        }
+        host_pinned_n = bytes;
+    }
+    *out = host_pinned;
+    return 0;
+}
+
 void Workspace::release()
 {
     if (!bufs.empty()) (void)hipSetDevice(device);
@@ -73,6 +90,9 @@ void Workspace::release()
     bufs.clear();
     filled.clear();
     sync_zeroed = nullptr;
+    if (host_pinned) (void)hipHostFree(host_pinned);
+    host_pinned = nullptr;
+    host_pinned_n = 0;
 }
 
 int check_rect(const lux_sgm_rect_params* p, std::string* err)

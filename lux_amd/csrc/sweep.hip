@@ -601,10 +601,13 @@ int ranges_from_previous(Workspace& ws, const float* planes, int n, const float*
                                                             d_start, d_count, d_cap, (unsigned long long*)dpart);
     stats_reduce_kernel<<<1, 1024, 0, s>>>((const unsigned long long*)dpart, nblocks, (unsigned long long*)dst);
     unsigned long long st[kStats];
+    void* hst;  // pinned: the copy is a DMA the stream wait covers, not a staged copy that blocks first
+    if ((rc = ws.pinned(sizeof st, &hst, err))) return rc;
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(st, dst, sizeof st, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hst, dst, sizeof st, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e, "ranges from previous", err);
+    std::memcpy(st, hst, sizeof st);
     out->minlo = (int)st[0];
     out->maxhi = (int)st[1];
     out->maxcap = (int)st[2];
