@@ -18,6 +18,7 @@
 #include <climits>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lux_sgm.h"
@@ -96,7 +97,9 @@ __device__ __forceinline__ uint32_t block_cost(const uint32_t* img, int iw, int 
 // raw[cell*cs + j], j < cap: the normalised cost for j < count (0 for a skipped pixel), 255 past count (bytes past a
 // pixel's cap are masked by the sweeps and left unwritten). vstart / vcount / vcap null: every pixel searches planes
 // [0, ucount) with cap ucap.
-template <int G>
+// NV > 0: exactly NV other cameras (the unrolled view loop then keeps only their matrices live; unrolled over all
+// kMaxViews, the sixteen matrices spill from SGPRs into VGPR lanes); NV = 0: a.nv of up to kMaxViews.
+template <int G, int NV>
 __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W, int H, int scale, const uint16_t* vstart,
                                                          const uint16_t* vcount, const uint16_t* vcap, int ucount, int ucap,
                                                          const uint8_t* skip, const uint32_t* list, const uint32_t* nlist,
@@ -138,17 +141,18 @@ __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) need = max(need, __shfl_xor(need, o));
     const bool eval = live && !skipped;
-    uint32_t carry[kMaxViews];
+    constexpr int KV = NV > 0 ? NV : kMaxViews;
+    uint32_t carry[KV];
 #pragma unroll
-    for (int k = 0; k < kMaxViews; k++) carry[k] = 0;
+    for (int k = 0; k < KV; k++) carry[k] = 0;
     for (int c0 = 0; c0 < need; c0 += G) {
         const int d = c0 + li;
         const bool act = eval && d < count;
         const float z = act ? a.planes[start + d] : 1.0f;
         uint32_t tot = 0;
 #pragma unroll
-        for (int k = 0; k < kMaxViews; k++) {
-            if (k >= a.nv) continue;
+        for (int k = 0; k < KV; k++) {
+            if (NV == 0 && k >= a.nv) continue;
             const float* m = a.m[k];
             const float xz = __fmul_rn(xf, z), yz = __fmul_rn(yf, z);
             const float px = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(z, m[6]), m[9]), __fmul_rn(xz, m[0])), __fmul_rn(yz, m[3]));
@@ -521,11 +525,19 @@ hipError_t launch_sweep_cost(const SweepDev& a, int W, int H, int scale, const u
     }
     const size_t per_block = 4 * (64 / G);
     const unsigned blocks = (unsigned)std::max<size_t>(1, (nseg + per_block - 1) / per_block);
+    auto go = [&](auto gc, auto nvc) {
+        sweep_cost_kernel<decltype(gc)::value, decltype(nvc)::value><<<blocks, 256, 0, s>>>(
+            a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs);
+    };
+    auto by_views = [&](auto gc) {
+        if (a.nv == 4) go(gc, std::integral_constant<int, 4>());
+        else go(gc, std::integral_constant<int, 0>());
+    };
     switch (G) {
-        case 8: sweep_cost_kernel<8><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
-        case 16: sweep_cost_kernel<16><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
-        case 32: sweep_cost_kernel<32><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
-        default: sweep_cost_kernel<64><<<blocks, 256, 0, s>>>(a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs); break;
+        case 8: by_views(std::integral_constant<int, 8>()); break;
+        case 16: by_views(std::integral_constant<int, 16>()); break;
+        case 32: by_views(std::integral_constant<int, 32>()); break;
+        default: by_views(std::integral_constant<int, 64>()); break;
     }
     return hipGetLastError();
 }
