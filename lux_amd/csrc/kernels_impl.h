@@ -1001,14 +1001,18 @@ __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* 
     if (y >= a.H) return;
     const int xb = (wv - y * nb) << 6, n = min(64, a.W - xb);
     const size_t c0 = (size_t)y * a.W + xb;
-    const int cs = a.cs, lo = lane * 2 * NP, lol = lo < cs ? lo : 0;
+    const int cs = a.cs, lo = lane * 2 * NP;
     // the 64 cells' ranges, one a lane, read out with v_readlane
     const uint32_t rs = lane < n ? ((uint32_t)a.vstart[c0 + lane] | ((uint32_t)a.vcap[c0 + lane] << 16)) : 0u;
     const uint32_t rn = lane < n ? (uint32_t)a.vcount[c0 + lane] : 0u;
     uint32_t myd = 0;
     float mys = 0.f;
     for (int j = 0; j < n; j++) {
-        const size_t off = (c0 + j) * cs + lol;
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rs, j);
+        const int s0 = (int)(r & 0xffffu), cap = (int)(r >> 16), D = __builtin_amdgcn_readlane((int)rn, j);
+        // lanes past the cell's cap read lane 0's planes (no bytes of their own fetched): none of them is stored
+        // or reaches the minimum
+        const size_t off = (c0 + j) * cs + (lo < cap ? lo : 0);
         Cell<NP> t, u0, u1, u2;
         ld_u16<NP>(a.agg + off, t);
         ld_u16<NP>(v0 + off, u0);
@@ -1016,8 +1020,6 @@ __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* 
         ld_u16<NP>(v2 + off, u2);
 #pragma unroll
         for (int k = 0; k < NP; k++) t.v[k] = padd(padd(t.v[k], u0.v[k]), padd(u1.v[k], u2.v[k]));
-        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rs, j);
-        const int s0 = (int)(r & 0xffffu), cap = (int)(r >> 16), D = __builtin_amdgcn_readlane((int)rn, j);
         if (a.write_agg) store_u16<NP>(a.agg + (c0 + j) * cs + lo, lo < cap, t);
         uint32_t best = 0xffffffffu;
 #pragma unroll
