@@ -74,8 +74,8 @@ struct Conc {
     hipStream_t chain = nullptr;  // the border chains' stream (SgmBuffers::chain)
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
 };
-// Gives w the three path-cost volumes and the streams when c is set and the volumes fit the budget (each pass then
-// runs as one launch); otherwise the sweeps stay one after another. Returns whether it did.
+// Gives w the three path-cost volumes when they fit the budget (ranged layers then run each pass as one launch),
+// and c's side streams when c has them; otherwise the paths run one after another. Returns whether it did.
 bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c);
 // Runs w's border chains on c's chain stream, beside the sweeps (no-op without one).
 void attach_chain(Prepared* w, const Conc* c);

@@ -428,7 +428,8 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         if ((rc = ws.get(b.name.c_str(), b.bytes, &q, err))) return rc;
         ptr[b.name] = q;
     }
-    if (conc && conc->side[0]) {  // the concurrent sweeps' volumes of the ranged levels, at the largest level's size
+    bool volumes = true;
+    {  // the path volumes of the ranged levels (one launch per pass), at the largest level's size
         size_t mx = 0;
         for (const PyrLevel& l : pyramid_levels(p))
             if (l.radius) {
@@ -439,7 +440,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         std::string e2;
         void* q;
         for (const char* nm : {"aggp0", "aggp1", "aggp2"})
-            if (ws.get(nm, mx, &q, &e2)) conc = nullptr;  // over the budget: the sweeps stay one after another
+            if (mx && ws.get(nm, mx, &q, &e2)) volumes = false;  // over the budget: the paths run one after another
     }
     const auto lv = pyramid_levels(p);
     const int L = p->levels;
@@ -483,7 +484,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         w.b.vcount = vn;
         w.b.seedstart = (uint16_t*)ptr["seedstart"];
         w.b.seedcap = (uint16_t*)ptr["seedcap"];
-        try_concurrency(ws, &w, conc);  // sized above: no reallocation under queued work
+        if (volumes) try_concurrency(ws, &w, conc);  // sized above: no reallocation under queued work
         attach_chain(&w, conc);
         SgmOutputs so;
         so.disp = ldisp ? ldisp : w.b.disp_tmp;
@@ -517,19 +518,20 @@ namespace lux {
 
 bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
 {
-    if (!c || !c->side[0]) return false;
     const size_t bytes = (size_t)w->g.W * w->g.H * w->g.cs * 2;
     static const char* names[3] = {"aggp0", "aggp1", "aggp2"};
     void* p[3];
     std::string err;
     for (int k = 0; k < 3; k++)
         if (ws.get(names[k], bytes, &p[k], &err)) return false;
-    for (int k = 0; k < 3; k++) {
-        w->b.aggp[k] = (uint16_t*)p[k];
-        w->b.conc[k] = c->side[k];
-        w->b.conc_join[k] = c->join[k];
+    for (int k = 0; k < 3; k++) w->b.aggp[k] = (uint16_t*)p[k];
+    if (c && c->side[0]) {
+        for (int k = 0; k < 3; k++) {
+            w->b.conc[k] = c->side[k];
+            w->b.conc_join[k] = c->join[k];
+        }
+        w->b.conc_fork = c->fork;
     }
-    w->b.conc_fork = c->fork;
     return true;
 }
 

@@ -62,8 +62,8 @@ struct SgmBuffers {
     uint16_t* seedcap = nullptr;
     AdaptP2 ap;                     // adaptive P2 (ap.on = 0: every step uses the constant P2)
     uint32_t* seedguid = nullptr;   // [3][W] guidance of the cell behind each seed-table entry (adaptive P2)
-    // Concurrent paths (set all or none): the sweeps of a pass run on side streams, each writing its path cost
-    // into its own volume [H][W][cs], summed into agg once per pass. For latency-bound (ranged) layers.
+    // Path volumes [H][W][cs] for latency-bound (ranged) layers: each pass then runs as one launch, paths 1-3 into
+    // the volumes, summed with agg once (the side streams are for the other layers' concurrent sweeps; optional).
     int seg = 1;                    // narrow per-pixel ranges (caps <= 32) may run several chains per wavefront
     // Ranged layers given the volumes run each pass as one launch; pass -1 adds its paths 1-3 to pass +1's volumes.
     uint16_t* aggp[3] = {};

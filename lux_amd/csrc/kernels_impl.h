@@ -1148,14 +1148,15 @@ __device__ __forceinline__ uint32_t seg_mask(int li, int cap)
 }
 
 // sgm_sweep<1, DIR, P, FIRST, true, true, false> with 64/SEG chains per wavefront.
+// t: the lane's index in the path's grid (64/SEG chains per wavefront).
 template <int SEG, int DIR, int P, bool FIRST>
-__global__ __launch_bounds__(kBlock) void sgm_sweep_seg(SweepArgs a)
+__device__ __forceinline__ void sweep_seg_wave(const SweepArgs& a, int t, int lane)
 {
     constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
     constexpr int DY = DIR * (P == 0 ? 0 : 1);
     constexpr int PF = 8;
-    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
-    const int chain = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    const int li = lane & (SEG - 1);
+    const int chain = t / SEG;
     const int W = a.W, H = a.H, cs = a.cs;
     const int nch = chain_count<DX, DY>(W, H);
     int x = 0, y = 0, n = 0;
@@ -1243,14 +1244,19 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep_seg(SweepArgs a)
     for (int j = 0; j < PF; j++)
         if (i0 + j < nmax) body(i0 + j, j);
 }
+template <int SEG, int DIR, int P, bool FIRST>
+__global__ __launch_bounds__(kBlock) void sgm_sweep_seg(SweepArgs a)
+{
+    sweep_seg_wave<SEG, DIR, P, FIRST>(a, (int)(blockIdx.x * kBlock + threadIdx.x), threadIdx.x & 63);
+}
 
 // sgm_row_fwd<1, true, false, true, false> with 64/SEG rows per wavefront.
 template <int SEG>
-__global__ __launch_bounds__(kBlock) void sgm_row_fwd_seg(RowArgs a)
+__device__ __forceinline__ void row_fwd_seg_wave(const RowArgs& a, int t, int lane)
 {
     constexpr int PF = 8;
-    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
-    const int yr = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    const int li = lane & (SEG - 1);
+    const int yr = t / SEG;
     if (__builtin_amdgcn_readfirstlane(yr - lane / SEG) >= a.H) return;  // the wave's first row: whole waves only
     const bool row_ok = yr < a.H;
     const int y = row_ok ? yr : a.H - 1;  // idle segments repeat the last row and store nothing
@@ -1296,15 +1302,21 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd_seg(RowArgs a)
     for (int j = 0; j < PF; j++)
         if (x0 + j < W) body(x0 + j, j);
 }
+template <int SEG>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd_seg(RowArgs a)
+{
+    row_fwd_seg_wave<SEG>(a, (int)(blockIdx.x * kBlock + threadIdx.x), threadIdx.x & 63);
+}
 
 // sgm_row_bwd_wta<1, true, true, false> with 64/SEG rows per wavefront: the last contribution and the
 // winner-take-all of every cell but column 0 (the first minimum over the cell's count, the parabola).
-template <int SEG>
-__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_seg(RowArgs a)
+// WTA = false: the path alone, its cost added to a.agg (sgm_finish takes the winner-take-all).
+template <int SEG, bool WTA>
+__device__ __forceinline__ void row_bwd_seg_wave(const RowArgs& a, int t, int lane)
 {
     constexpr int PF = 8;
-    const int lane = threadIdx.x & 63, li = lane & (SEG - 1);
-    const int yr = (int)((blockIdx.x * kBlock + threadIdx.x) / SEG);
+    const int li = lane & (SEG - 1);
+    const int yr = t / SEG;
     if (__builtin_amdgcn_readfirstlane(yr - lane / SEG) >= a.H) return;  // the wave's first row: whole waves only
     const bool row_ok = yr < a.H;
     const int y = row_ok ? yr : a.H - 1;
@@ -1339,13 +1351,14 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_seg(RowArgs a)
         minprev = seg_path_step<SEG>(state, minprev, raw, a.p1x2, a.p2, mask, lc, el, er, li);
         const uint32_t fin = padd(agg, lc);
         if (row_ok) {
-            if (a.write_agg && active) *(uint32_t*)(a.agg + (rowcell + x) * cs + lo) = fin;
+            if ((!WTA || a.write_agg) && active) *(uint32_t*)(a.agg + (rowcell + x) * cs + lo) = fin;
             if (x == 1) {
                 if (lo < cs) *(uint32_t*)(a.cols + (size_t)y * cs + lo) = lc;
                 if (li == 0) a.colmins[y] = (uint16_t)minprev;
             }
         }
         state = lc;
+        if constexpr (!WTA) return;
         const uint32_t k0 = lo < D ? (((fin & 0xffffu) << 16) | (uint32_t)lo) : 0xffffffffu;
         const uint32_t k1 = lo + 1 < D ? ((fin & 0xffff0000u) | (uint32_t)(lo + 1)) : 0xffffffffu;
         const uint32_t best = seg_min<SEG>(k0 < k1 ? k0 : k1);
@@ -1372,6 +1385,11 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_seg(RowArgs a)
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(W - 1 - (i0 + j), j);
+}
+template <int SEG>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_seg(RowArgs a)
+{
+    row_bwd_seg_wave<SEG, true>(a, (int)(blockIdx.x * kBlock + threadIdx.x), threadIdx.x & 63);
 }
 
 // ------------------------------------------------------------------ fused vertical + diagonal paths (bands)
@@ -1920,6 +1938,37 @@ __global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
     sweep_wave<NP, DIR, 3, (DIR > 0), MASKED, VAR, ADP>(p.s[2], w - p.n1, lane);  // returns past the last chain
 }
 
+// sgm_pass with segmented chains (64/SEG chains per wavefront; each wavefront runs chains of one path): the border
+// chain in block 0, then the row path and paths 2, 1 and 3, as many wavefronts each as its chains fill.
+template <int SEG, int DIR>
+__global__ __launch_bounds__(kBlock) void sgm_pass_seg(PassArgs p)
+{
+    constexpr int CPW = 64 / SEG;
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 64) replay_chain_wave<1, true, true, false>(p.c, p.c.visits, lane);
+        return;
+    }
+    int w = __builtin_amdgcn_readfirstlane((int)(((blockIdx.x - 1) * kBlock + threadIdx.x) >> 6));
+    const int wr = (p.nrow + CPW - 1) / CPW, w2 = (p.n2 + CPW - 1) / CPW, w1 = (p.n1 + CPW - 1) / CPW;
+    if (w < wr) {
+        if (DIR > 0) row_fwd_seg_wave<SEG>(p.r, w * 64 + lane, lane);
+        else row_bwd_seg_wave<SEG, false>(p.r, w * 64 + lane, lane);
+        return;
+    }
+    w -= wr;
+    if (w < w2) {
+        sweep_seg_wave<SEG, DIR, 2, (DIR > 0)>(p.s[1], w * 64 + lane, lane);
+        return;
+    }
+    w -= w2;
+    if (w < w1) {
+        sweep_seg_wave<SEG, DIR, 1, (DIR > 0)>(p.s[0], w * 64 + lane, lane);
+        return;
+    }
+    sweep_seg_wave<SEG, DIR, 3, (DIR > 0)>(p.s[2], (w - w1) * 64 + lane, lane);  // returns past the last chain
+}
+
 // Pass +1's seed-table entries of paths 1 and 3 for the border cell: recomputed for the last visit that wrote
 // half 0. Path 1 reads a saved neighbour-column state, so this runs after the sweeps (one wavefront).
 template <int NP, bool MASKED, bool VAR, bool ADP>
@@ -2320,10 +2369,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if (chain_live) (void)hipStreamWaitEvent(s, b.chain_join, 0);
         chain_live = false;
     };
-    // Ranged layers given path volumes: each pass's border chain, row path (pass +1) and paths 1-3 in one launch
-    // (sgm_pass), then the volumes summed into agg.
-    constexpr bool kFusable = SEG == 0 && VAR;
-    const bool fused = kFusable && conc;
+    // Ranged layers given path volumes: each pass's border chain, row path and paths 1-3 in one launch (sgm_pass,
+    // sgm_pass_seg); pass +1's paths 1-3 fill the volumes, pass -1 adds to them, and sgm_finish sums them with agg.
+    constexpr bool kFusable = VAR;
+    const bool fused = kFusable && b.aggp[0] && !o.bands && !b.cl;  // needs the volumes, not the side streams
     auto launch_pass = [&](auto dirc, const ReplayArgs& rr) {
         constexpr int DIR = decltype(dirc)::value;
         if constexpr (kFusable) {
@@ -2337,8 +2386,14 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             p.n1 = nd;
             p.n2 = nv;
             p.nrow = nh;
-            const int waves = p.nrow + nv + 2 * nd;
-            sgm_pass<NP, DIR, MASKED, VAR, ADP><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
+            if constexpr (SEG > 0) {
+                constexpr int CPW = 64 / (SEG > 0 ? SEG : 64);
+                const int waves = (nh + CPW - 1) / CPW + (nv + CPW - 1) / CPW + 2 * ((nd + CPW - 1) / CPW);
+                sgm_pass_seg<SEG, DIR><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
+            } else {
+                const int waves = p.nrow + nv + 2 * nd;
+                sgm_pass<NP, DIR, MASKED, VAR, ADP><<<1 + (waves + 3) / 4, kBlock, 0, s>>>(p);
+            }
         }
     };
     ReplayArgs r;
