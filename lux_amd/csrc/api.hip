@@ -635,7 +635,7 @@ int lux_sgm_compute_layer_ex(lux_sgm_ctx* c, const lux_sgm_layer_params* p, cons
         if ((rc = lux::run_literal(ws, l, lmaxcap, (uint16_t*)a1, (uint16_t*)dw, s, &c->err))) return rc;
     } else {
         const lux::Conc conc = lux::conc_of(c);
-        if (ranged) lux::try_concurrency(ws, &w, &conc);  // ranged layers are latency-bound: their sweeps run side by side
+        if (ranged) lux::try_path_volumes(ws, &w);  // ranged layers are latency-bound: one launch per pass
         lux::attach_chain(&w, &conc);
         lux::SgmOutputs o;
         o.disp = (uint16_t*)dw;

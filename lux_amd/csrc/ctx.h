@@ -60,18 +60,12 @@ inline int ensure_side_streams(lux_sgm_ctx* c)
     return e == hipSuccess ? 0 : hip_status(e, "side streams", &c->err);
 }
 
-// The context's side streams and events as the concurrent sweeps (side 0-2) and the border chains (side 3) take them
-// (the batch form uses the same ones; calls on a context are serialised by its mutex). Without side streams the
-// sweeps and chains stay one after another.
+// The context's side stream and events as the border chains take them (side 3; the batch form uses the side streams
+// as its branches; calls on a context are serialised by its mutex). Without side streams the chains stay in order.
 inline Conc conc_of(lux_sgm_ctx* c)
 {
     Conc k;
     if (ensure_side_streams(c)) return k;
-    for (int i = 0; i < 3; i++) {
-        k.side[i] = c->side[i];
-        k.join[i] = c->join[i];
-    }
-    k.fork = c->fork;
     k.chain = c->side[3];
     k.chain_fork = c->chain_fork;
     k.chain_join = c->join[3];

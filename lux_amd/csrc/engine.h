@@ -67,16 +67,14 @@ struct Prepared {
     uint16_t* agg = nullptr;
 };
 
-// Side streams and events for the concurrent sweeps of ranged layers (SgmBuffers::aggp): the caller's, reused per call.
+// The caller's side stream and events for the border chains (SgmBuffers::chain), reused per call.
 struct Conc {
-    hipStream_t side[3] = {};
-    hipEvent_t fork = nullptr, join[3] = {};
-    hipStream_t chain = nullptr;  // the border chains' stream (SgmBuffers::chain)
+    hipStream_t chain = nullptr;
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
 };
-// Gives w the three path-cost volumes when they fit the budget (ranged layers then run each pass as one launch),
-// and c's side streams when c has them; otherwise the paths run one after another. Returns whether it did.
-bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c);
+// Gives w the three path-cost volumes when they fit the budget (ranged layers then run each pass as one launch);
+// otherwise the paths run one after another. Returns whether it did.
+bool try_path_volumes(Workspace& ws, Prepared* w);
 // Runs w's border chains on c's chain stream, beside the sweeps (no-op without one).
 void attach_chain(Prepared* w, const Conc* c);
 

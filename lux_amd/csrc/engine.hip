@@ -484,7 +484,7 @@ int enqueue_pyramid(Workspace& ws, const lux_sgm_pyramid_params* p, const void* 
         w.b.vcount = vn;
         w.b.seedstart = (uint16_t*)ptr["seedstart"];
         w.b.seedcap = (uint16_t*)ptr["seedcap"];
-        if (volumes) try_concurrency(ws, &w, conc);  // sized above: no reallocation under queued work
+        if (volumes) try_path_volumes(ws, &w);  // sized above: no reallocation under queued work
         attach_chain(&w, conc);
         SgmOutputs so;
         so.disp = ldisp ? ldisp : w.b.disp_tmp;
@@ -516,7 +516,7 @@ int check_band_error(Workspace& ws, hipStream_t s, std::string* err)
 
 namespace lux {
 
-bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
+bool try_path_volumes(Workspace& ws, Prepared* w)
 {
     const size_t bytes = (size_t)w->g.W * w->g.H * w->g.cs * 2;
     static const char* names[3] = {"aggp0", "aggp1", "aggp2"};
@@ -525,13 +525,6 @@ bool try_concurrency(Workspace& ws, Prepared* w, const Conc* c)
     for (int k = 0; k < 3; k++)
         if (ws.get(names[k], bytes, &p[k], &err)) return false;
     for (int k = 0; k < 3; k++) w->b.aggp[k] = (uint16_t*)p[k];
-    if (c && c->side[0]) {
-        for (int k = 0; k < 3; k++) {
-            w->b.conc[k] = c->side[k];
-            w->b.conc_join[k] = c->join[k];
-        }
-        w->b.conc_fork = c->fork;
-    }
     return true;
 }
 

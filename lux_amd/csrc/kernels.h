@@ -62,13 +62,10 @@ struct SgmBuffers {
     uint16_t* seedcap = nullptr;
     AdaptP2 ap;                     // adaptive P2 (ap.on = 0: every step uses the constant P2)
     uint32_t* seedguid = nullptr;   // [3][W] guidance of the cell behind each seed-table entry (adaptive P2)
-    // Path volumes [H][W][cs] for latency-bound (ranged) layers: each pass then runs as one launch, paths 1-3 into
-    // the volumes, summed with agg once (the side streams are for the other layers' concurrent sweeps; optional).
     int seg = 1;                    // narrow per-pixel ranges (caps <= 32) may run several chains per wavefront
-    // Ranged layers given the volumes run each pass as one launch; pass -1 adds its paths 1-3 to pass +1's volumes.
+    // Path volumes [H][W][cs] for latency-bound (ranged) layers (set all or none): each pass then runs as one
+    // launch, pass +1's paths 1-3 into the volumes, pass -1's added to them, all summed with agg once.
     uint16_t* aggp[3] = {};
-    hipStream_t conc[3] = {};
-    hipEvent_t conc_fork = nullptr, conc_join[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
     // The border chains on a side stream (set all or none; none: they run in order on the request's stream). They
     // read only the cost volume, the ranges, the guidance and pass +1's seeds, so they run beside the sweeps.

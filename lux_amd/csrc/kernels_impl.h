@@ -2267,39 +2267,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     }
     // paths 1-3 of one pass: fused bands (up to 256 planes), else one sweep per path
     // (not with the census cost fused into the row path: the sweeps read the costs it writes)
-    const bool conc = b.aggp[0] && b.conc[0] && !o.bands && !b.cl;
-    bool forked = false;  // pass +1 forks before its row path, so that runs beside the sweeps too
     const size_t aggn = (size_t)g.W * g.H * g.cs / 2;  // u16 pairs of one volume (cs is a multiple of 4)
     auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
         constexpr int DIR = decltype(dirc)::value;
-        if (conc) {  // the three sweeps side by side, each into its own volume, then one sum into agg
-            if (!forked) (void)hipEventRecord(b.conc_fork, s);
-            forked = false;
-            SweepArgs ak[3] = {a, a, a};
-            for (int k = 0; k < 3; k++) {
-                ak[k].agg = b.aggp[k];
-                (void)hipStreamWaitEvent(b.conc[k], b.conc_fork, 0);
-            }
-            if constexpr (SEG > 0) {
-                sgm_sweep_seg<SEG, DIR, 1, true><<<sblocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
-                sgm_sweep_seg<SEG, DIR, 2, true><<<sblocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
-                sgm_sweep_seg<SEG, DIR, 3, true><<<sblocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
-            } else {
-                sgm_sweep<NP, DIR, 1, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[0]>>>(ak[0]);
-                sgm_sweep<NP, DIR, 2, true, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, b.conc[1]>>>(ak[1]);
-                sgm_sweep<NP, DIR, 3, true, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, b.conc[2]>>>(ak[2]);
-            }
-            for (int k = 0; k < 3; k++) {
-                (void)hipEventRecord(b.conc_join[k], b.conc[k]);
-                (void)hipStreamWaitEvent(s, b.conc_join[k], 0);
-            }
-            agg_combine3<<<grid_for(aggn, 256), 256, 0, s>>>((uint32_t*)b.agg, (const uint32_t*)b.aggp[0],
-                                                           (const uint32_t*)b.aggp[1], (const uint32_t*)b.aggp[2], aggn);
-            mark();
-            mark();
-            mark();
-            return;
-        }
         if constexpr (SEG > 0) {
             sgm_sweep_seg<SEG, DIR, 1, false><<<sblocks(nd), kBlock, 0, s>>>(a);
             mark();
@@ -2415,10 +2385,6 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         mark();
     } else {
         if (!b.cl) start_chain(r);  // else the row path writes the cost volume first
-        if (conc) {
-            (void)hipEventRecord(b.conc_fork, s);
-            forked = true;
-        }
         if constexpr (SEG > 0) {
             sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
         } else if constexpr (VAR) {

@@ -710,7 +710,7 @@ int dense_layer_device(Workspace& ws, const SweepDev& a, const lux_sgm_dense_par
         l.p2 = 0;
         if ((rc = run_literal(ws, l, maxcap, agg_pass1, d_wta, s, err))) return rc;
     } else {
-        if (ranged) try_concurrency(ws, w, conc);  // ranged layers are latency-bound: their sweeps run side by side
+        if (ranged) try_path_volumes(ws, w);  // ranged layers are latency-bound: one launch per pass
         attach_chain(w, conc);
         SgmOutputs o;
         o.disp = d_wta;
