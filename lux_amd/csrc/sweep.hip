@@ -175,10 +175,17 @@ __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W
             }
             // the cost of the last lane at or below this one that evaluated, else the carried one
             int v = ok ? li : -1;
+            if constexpr (G == 16) {  // a segment is a DPP row: row_shr, lanes without a source taking -1
+                v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));
+                v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));
+                v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));
+                v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));
+            } else {
 #pragma unroll
-            for (int o = 1; o < G; o <<= 1) {
-                const int t = __shfl_up(v, o, G);
-                if (li >= o) v = max(v, t);
+                for (int o = 1; o < G; o <<= 1) {
+                    const int t = __shfl_up(v, o, G);
+                    if (li >= o) v = max(v, t);
+                }
             }
             uint32_t got = (uint32_t)__shfl((int)cost, v < 0 ? 0 : v, G);
             got = v < 0 ? carry[k] : got;
