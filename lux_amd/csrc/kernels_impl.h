@@ -2071,13 +2071,6 @@ __global__ void agg_sum4(uint32_t* out, const uint32_t* a0, const uint32_t* a1, 
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = padd(padd(a0[i], a1[i]), padd(a2[i], a3[i]));
 }
-// agg = sat(agg + a1 + a2 + a3) on u16 pairs: the path costs of concurrently run sweeps. Saturating addition of
-// non-negative values is min(sum, 65535) in any order, so this equals adding them one sweep after another.
-__global__ void agg_combine3(uint32_t* agg, const uint32_t* a1, const uint32_t* a2, const uint32_t* a3, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        agg[i] = padd(padd(agg[i], a1[i]), padd(a2[i], a3[i]));
-}
 
 // ------------------------------------------------------------------ census and cost volume
 template <typename T, int R>
@@ -2159,7 +2152,7 @@ __global__ void downsample2_kernel(const T* src, int W, int H, int stride, T* ds
 // The plane range of every pixel of a level from the disparity of the level below (half resolution, half
 // the planes): start = clamp(2 d - r, 0, D - 2r), count = cap = 2r (a multiple of the alignment, so no
 // range's padding reaches the end of its line-buffer segment).
-__global__ void ranges_from_coarse_kernel(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
+[[maybe_unused]] __global__ void ranges_from_coarse_kernel(const uint16_t* dc, int Wc, int Hc, int W, int H, int r, int D, uint16_t* vstart,
                                           uint16_t* vcap, uint16_t* vcount)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2174,7 +2167,7 @@ __global__ void ranges_from_coarse_kernel(const uint16_t* dc, int Wc, int Hc, in
 
 // Census costs of per-pixel ranges: raw[y][x][j] = popcount(cl(x) ^ cr(max(x - (start + j), 0))) for j < count,
 // 255 past it (DenseStereo.cs:380); one thread per byte of the cell-major volume.
-__global__ void cost_census_ranges_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
+[[maybe_unused]] __global__ void cost_census_ranges_kernel(const uint64_t* cl, const uint64_t* cr, int W, int H, int cs, const uint16_t* vstart,
                                           const uint16_t* vcount, uint8_t* raw)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
