@@ -841,8 +841,11 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
     }
+    // lane x&63 keeps cell x's winner and the parabola's inputs; the float arithmetic of the subpixel runs once per
+    // 64 cells, lane-parallel, when they are stored (the same operations on the same values as per cell)
     uint32_t myd = 0, myr = 0;
-    float mys = 0.f;
+    int myl = 0, myc = 0, myrr = 0;
+    bool mypar = false;
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
@@ -899,16 +902,15 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         }
         best = wave_min(best);
         const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
-        float v = (float)(d + s0);
-        if (d > 0 && d < D - 1) {
-            const int l = (int)plane_of<NP>(fin, d - 1), cc = (int)(best >> 16), r = (int)plane_of<NP>(fin, d + 1);
-            const int den = l - 2 * cc + r;
-            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - r)), (float)den);
-        }
+        const bool par = d > 0 && d < D - 1;
+        const int l = (int)plane_of<NP>(fin, par ? d - 1 : 0), r = (int)plane_of<NP>(fin, par ? d + 1 : 0);
         if (lane == (x & 63)) {
             myd = (uint32_t)(d + s0);  // the plane index start + arg (:230)
-            mys = v;
             myr = rdone & 0xffffu;
+            mypar = par;
+            myl = l;
+            myc = (int)(best >> 16);
+            myrr = r;
         }
         if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
             const int xb = x & ~63, l0 = x & 63;
@@ -916,7 +918,12 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             if (lane >= l0 && lane <= l1) {
                 const size_t o = rowcell + xb + lane;
                 a.disp[o] = (uint16_t)myd;
-                if (a.sub) a.sub[o] = mys;
+                if (a.sub) {  // the parabola through agg(d-1), agg(d), agg(d+1)
+                    float v = (float)(int)myd;
+                    const int den = myl - 2 * myc + myrr;
+                    if (mypar && den > 0) v = (float)(int)myd + __fdiv_rn(__fmul_rn(0.5f, (float)(myl - myrr)), (float)den);
+                    a.sub[o] = v;
+                }
                 if (!VAR) a.dr[o] = (uint16_t)myr;
             }
         }
