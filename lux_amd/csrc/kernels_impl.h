@@ -1012,8 +1012,11 @@ __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* 
     // the 64 cells' ranges, one a lane, read out with v_readlane
     const uint32_t rs = lane < n ? ((uint32_t)a.vstart[c0 + lane] | ((uint32_t)a.vcap[c0 + lane] << 16)) : 0u;
     const uint32_t rn = lane < n ? (uint32_t)a.vcount[c0 + lane] : 0u;
+    // lane j keeps cell j's winner and the parabola's inputs; the subpixel's float arithmetic runs once at the end,
+    // lane-parallel (the same operations on the same values as per cell)
     uint32_t myd = 0;
-    float mys = 0.f;
+    int myl = 0, myc = 0, myrr = 0;
+    bool mypar = false;
     for (int j = 0; j < n; j++) {
         const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rs, j);
         const int s0 = (int)(r & 0xffffu), cap = (int)(r >> 16), D = __builtin_amdgcn_readlane((int)rn, j);
@@ -1038,20 +1041,24 @@ __global__ __launch_bounds__(kBlock) void sgm_finish(RowArgs a, const uint16_t* 
         }
         best = wave_min(best);
         const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
-        float v = (float)(d + s0);
-        if (d > 0 && d < D - 1) {
-            const int l = (int)plane_of<NP>(t, d - 1), cc = (int)(best >> 16), rr = (int)plane_of<NP>(t, d + 1);
-            const int den = l - 2 * cc + rr;
-            if (den > 0) v = (float)(d + s0) + __fdiv_rn(__fmul_rn(0.5f, (float)(l - rr)), (float)den);
-        }
+        const bool par = d > 0 && d < D - 1;
+        const int l = (int)plane_of<NP>(t, par ? d - 1 : 0), rr = (int)plane_of<NP>(t, par ? d + 1 : 0);
         if (lane == j) {
             myd = (uint32_t)(d + s0);
-            mys = v;
+            mypar = par;
+            myl = l;
+            myc = (int)(best >> 16);
+            myrr = rr;
         }
     }
     if (lane < n) {
         a.disp[c0 + lane] = (uint16_t)myd;
-        if (a.sub) a.sub[c0 + lane] = mys;
+        if (a.sub) {  // the parabola through agg(d-1), agg(d), agg(d+1)
+            float v = (float)(int)myd;
+            const int den = myl - 2 * myc + myrr;
+            if (mypar && den > 0) v = (float)(int)myd + __fdiv_rn(__fmul_rn(0.5f, (float)(myl - myrr)), (float)den);
+            a.sub[c0 + lane] = v;
+        }
     }
 }
 
