@@ -230,3 +230,19 @@ def test_server_pyramids_mixed_with_pairs(cases):
                 np.testing.assert_array_equal(d, pyr[k][3]["disp"])
                 np.testing.assert_array_equal(s, pyr[k][3]["sub"])
         assert srv.info()["failed"] == 0
+
+
+def test_pyramid_without_path_volumes():
+    """A budget of exactly the pyramid's own buffers leaves no room for the three path volumes: the ranged levels then
+    run their paths one launch each (the fallback), with the results of the one-launch passes."""
+    W, H, D, radii = 160, 120, 64, (8, 4)
+    left, right, _ = scene.make_pair(W, H, D - 1, planes=4, sigma=2.0, seed=77, bits=10, fg_frac=0.1)
+    pp = sgm.pyramid_params(W, H, D, radii, bits=10, p1=8, p2=96, tile=16)
+    need = sgm.pyramid_bytes(pp)
+    print(f"pyramid {W}x{H} D{D} radii {radii} budget {need}", flush=True)
+    with sgm.Context(0) as c:
+        full = sgm.pyramid(c, left, right, D, radii, bits=10, p1=8, p2=96, tile=16)
+    with sgm.Context(0, hbm_budget=need) as c:
+        tight = sgm.pyramid(c, left, right, D, radii, bits=10, p1=8, p2=96, tile=16)
+    np.testing.assert_array_equal(tight["disp"], full["disp"])
+    np.testing.assert_array_equal(tight["sub"], full["sub"])
