@@ -597,7 +597,7 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
 template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 {
-    sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(a, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
+    sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
@@ -784,7 +784,7 @@ template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 {
     row_fwd_wave<NP, MASKED, COST, VAR, ADP>(
-        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6)), threadIdx.x & 63);
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
 
 // Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
@@ -946,7 +946,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 template <int NP, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 {
-    row_bwd_wave<NP, MASKED, VAR, ADP, true>(a, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
+    row_bwd_wave<NP, MASKED, VAR, ADP, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
 }
 
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
@@ -2287,11 +2287,13 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             return;
         }
         if (NP > 2 || VAR || ADP || !o.bands) {  // the fused bands take neither ranges nor adaptive P2
-            sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, s>>>(a);
+            // one wavefront a workgroup: the chains spread over the CUs to within one wavefront (four-wave
+            // workgroups left some CUs a whole workgroup more than the rest)
+            sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<nd, 64, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 2, false, MASKED, VAR, ADP><<<blocks(nv), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 2, false, MASKED, VAR, ADP><<<nv, 64, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 3, false, MASKED, VAR, ADP><<<blocks(nd), kBlock, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 3, false, MASKED, VAR, ADP><<<nd, 64, 0, s>>>(a);
             mark();
             return;
         }
@@ -2395,10 +2397,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if constexpr (SEG > 0) {
             sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
         } else if constexpr (VAR) {
-            sgm_row_fwd<NP, MASKED, false, true, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+            sgm_row_fwd<NP, MASKED, false, true, ADP><<<nh, 64, 0, s>>>(ra);
         } else {
-            if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
-            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+            if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<nh, 64, 0, s>>>(ra);  // one wavefront a workgroup
+            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<nh, 64, 0, s>>>(ra);
         }
         if (b.cl) start_chain(r);
         mark();
@@ -2435,7 +2437,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         mark();  // the row path ran in the pass launch
     } else {
         if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
-        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<blocks(nh), kBlock, 0, s>>>(ra);
+        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<nh, 64, 0, s>>>(ra);
         mark();
     }
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
