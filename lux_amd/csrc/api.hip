@@ -128,6 +128,7 @@ int capture_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<
     lux::EngineOpts opt;
     opt.bands = sch.bands;
     opt.replayable = true;
+    opt.shared = true;
     hipStream_t cs = c->stream;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamBeginCapture");

@@ -90,6 +90,9 @@ struct EngineOpts {
     // The launches may be captured into a graph and replayed: band flags are cleared by a kernel before each
     // band launch instead of relying on a fresh epoch per request.
     bool replayable = false;
+    // Other requests share the device at the same time (batch branches, server slots): the chains keep four-wavefront
+    // workgroups instead of one-wavefront ones.
+    bool shared = false;
 };
 
 // The named device buffers of one request: prepare() allocates exactly these, and the server admits a request

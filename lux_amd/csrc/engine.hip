@@ -314,6 +314,7 @@ int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const
     so.write_agg = o.agg != nullptr;
     so.bands = opt.bands;
     w.b.zero_flags = opt.replayable;
+    w.b.solo = !opt.shared;
     e = launch_sgm(g, w.b, so, p->p1, p->p2, nullptr, s, ev ? ev + 2 : nullptr);
     if (e != hipSuccess) return hip_status(e, "aggregation", err);
     if (o.agg) {

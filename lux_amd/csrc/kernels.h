@@ -67,6 +67,10 @@ struct SgmBuffers {
     // launch, pass +1's paths 1-3 into the volumes, pass -1's added to them, all summed with agg once.
     uint16_t* aggp[3] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
+    // The request has the device to itself: each chain's wavefront is a workgroup of its own, so the chains spread
+    // over the CUs to within one wavefront (four-wavefront workgroups left some CUs a whole workgroup more). With
+    // other requests running beside it (0), four-wavefront workgroups.
+    int solo = 1;
     // The border chains on a side stream (set all or none; none: they run in order on the request's stream). They
     // read only the cost volume, the ranges, the guidance and pass +1's seeds, so they run beside the sweeps.
     hipStream_t chain = nullptr;

@@ -2265,6 +2265,9 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     ra.ap = b.ap;
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
+    // one chain a wavefront: one-wavefront workgroups when the request has the device to itself (SgmBuffers::solo)
+    const unsigned wg = b.solo ? 64u : (unsigned)kBlock;
+    auto grid = [&](int chains) { return b.solo ? (unsigned)chains : (unsigned)blocks(chains); };
     const int nh = g.H, nv = g.W, nd = g.W + g.H - 1;
     if (b.seed_row < 0) {  // pass +1 never writes half 0: every seed is the initial fill
         fill_u16<<<grid_for((size_t)3 * g.W * g.cs, 256), 256, 0, s>>>(b.seeds, (size_t)3 * g.W * g.cs, 2000);
@@ -2287,13 +2290,11 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             return;
         }
         if (NP > 2 || VAR || ADP || !o.bands) {  // the fused bands take neither ranges nor adaptive P2
-            // one wavefront a workgroup: the chains spread over the CUs to within one wavefront (four-wave
-            // workgroups left some CUs a whole workgroup more than the rest)
-            sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<nd, 64, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 1, false, MASKED, VAR, ADP><<<grid(nd), wg, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 2, false, MASKED, VAR, ADP><<<nv, 64, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 2, false, MASKED, VAR, ADP><<<grid(nv), wg, 0, s>>>(a);
             mark();
-            sgm_sweep<NP, DIR, 3, false, MASKED, VAR, ADP><<<nd, 64, 0, s>>>(a);
+            sgm_sweep<NP, DIR, 3, false, MASKED, VAR, ADP><<<grid(nd), wg, 0, s>>>(a);
             mark();
             return;
         }
@@ -2397,10 +2398,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if constexpr (SEG > 0) {
             sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
         } else if constexpr (VAR) {
-            sgm_row_fwd<NP, MASKED, false, true, ADP><<<nh, 64, 0, s>>>(ra);
+            sgm_row_fwd<NP, MASKED, false, true, ADP><<<grid(nh), wg, 0, s>>>(ra);
         } else {
-            if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<nh, 64, 0, s>>>(ra);  // one wavefront a workgroup
-            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<nh, 64, 0, s>>>(ra);
+            if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
+            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
         }
         if (b.cl) start_chain(r);
         mark();
@@ -2437,7 +2438,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         mark();  // the row path ran in the pass launch
     } else {
         if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
-        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<nh, 64, 0, s>>>(ra);
+        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
         mark();
     }
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);

@@ -208,6 +208,7 @@ int launch(lux_sgm_server* s, int k, Request& r)
         lux::Prepared w;
         if ((rc = lux::prepare(ws, g, p.tile, p.xs_clamp_w, p.cost == LUX_SGM_COST_CENSUS, st, &w, &err))) return rc;
         lux::EngineOpts opt;
+        opt.shared = true;  // the other slots run beside this request
         if ((rc = lux::enqueue_rect(ws, w, &p, dl, dr, o, opt, st, nullptr, &err))) return rc;
     }
     if (r.mem == LUX_SGM_MEM_HOST) {
