@@ -97,8 +97,8 @@ __device__ __forceinline__ uint32_t block_cost(const uint32_t* img, int iw, int 
 // raw[cell*cs + j], j < cap: the normalised cost for j < count (0 for a skipped pixel), 255 past count (bytes past a
 // pixel's cap are masked by the sweeps and left unwritten). vstart / vcount / vcap null: every pixel searches planes
 // [0, ucount) with cap ucap.
-// NV > 0: exactly NV other cameras (the unrolled view loop then keeps only their matrices live; unrolled over all
-// kMaxViews, the sixteen matrices spill from SGPRs into VGPR lanes); NV = 0: a.nv of up to kMaxViews.
+// NV > 0: exactly NV other cameras, 1 to 4 (the unrolled view loop then keeps only their matrices live; unrolled over
+// all kMaxViews, the sixteen matrices spill from SGPRs into VGPR lanes); NV = 0: a.nv of up to kMaxViews.
 template <int G, int NV>
 __global__ __launch_bounds__(256) void sweep_cost_kernel(const SweepDev a, int W, int H, int scale, const uint16_t* vstart,
                                                          const uint16_t* vcount, const uint16_t* vcap, int ucount, int ucap,
@@ -537,8 +537,13 @@ hipError_t launch_sweep_cost(const SweepDev& a, int W, int H, int scale, const u
             a, W, H, scale, vstart, vcount, vcap, ucount, ucap, skip, list, nlist, raw, cs);
     };
     auto by_views = [&](auto gc) {
-        if (a.nv == 4) go(gc, std::integral_constant<int, 4>());
-        else go(gc, std::integral_constant<int, 0>());
+        switch (a.nv) {
+            case 1: go(gc, std::integral_constant<int, 1>()); break;
+            case 2: go(gc, std::integral_constant<int, 2>()); break;
+            case 3: go(gc, std::integral_constant<int, 3>()); break;
+            case 4: go(gc, std::integral_constant<int, 4>()); break;
+            default: go(gc, std::integral_constant<int, 0>()); break;
+        }
     };
     switch (G) {
         case 8: by_views(std::integral_constant<int, 8>()); break;
