@@ -372,6 +372,34 @@ __device__ __forceinline__ void chain_start(int c, int W, int H, int& x0, int& y
     }
 }
 
+// The diagonal chains (DY != 0, DX != 0) longest first: the full-length ones, then the two of each shorter length (one
+// starting on the first row, one on the first column). A pass launch holds more wavefronts than are resident at once,
+// so the ones that start last are then the short ones. A permutation of [0, W+H-1); indexes past it are kept.
+template <int DX>
+__device__ __forceinline__ int diag_order(int w, int W, int H)
+{
+    if (w >= W + H - 1) return w;
+    int c;
+    if (W >= H) {
+        const int F = W - H + 1;  // first-row starts 0 .. W-H: length H
+        if (w < F) {
+            c = w;
+        } else {
+            const int t = w - F, j = t / 2 + 1;
+            c = (t & 1) ? W - 1 + j : W - H + j;
+        }
+    } else {
+        const int F = H - W + 1;  // the first-row start 0 and the first-column starts 1 .. H-W: length W
+        if (w < F) {
+            c = w == 0 ? 0 : W - 1 + w;
+        } else {
+            const int t = w - F, j = t / 2 + 1;
+            c = (t & 1) ? W - 1 + (H - W) + j : j;
+        }
+    }
+    return (DX < 0 && c < W) ? W - 1 - c : c;  // leftward chains from the first row grow the other way
+}
+
 struct SweepArgs {
     const uint8_t* raw;
     uint16_t* agg;
@@ -1946,10 +1974,11 @@ __global__ __launch_bounds__(kBlock) void sgm_pass(PassArgs p)
     }
     w -= p.n2;
     if (w < p.n1) {
-        sweep_wave<NP, DIR, 1, (DIR > 0), MASKED, VAR, ADP>(p.s[0], w, lane);
+        sweep_wave<NP, DIR, 1, (DIR > 0), MASKED, VAR, ADP>(p.s[0], diag_order<DIR>(w, p.r.W, p.r.H), lane);
         return;
     }
-    sweep_wave<NP, DIR, 3, (DIR > 0), MASKED, VAR, ADP>(p.s[2], w - p.n1, lane);  // returns past the last chain
+    // returns past the last chain
+    sweep_wave<NP, DIR, 3, (DIR > 0), MASKED, VAR, ADP>(p.s[2], diag_order<-DIR>(w - p.n1, p.r.W, p.r.H), lane);
 }
 
 // sgm_pass with segmented chains (64/SEG chains per wavefront; each wavefront runs chains of one path): the border
