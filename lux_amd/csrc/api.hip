@@ -273,7 +273,10 @@ const char* lux_sgm_last_error(const lux_sgm_ctx* c) { return c ? c->err.c_str()
 
 int lux_sgm_rect_bytes(const lux_sgm_rect_params* p, uint64_t* bytes)
 {
-    if (!p || !bytes || p->width <= 0 || p->height <= 0 || p->disparities <= 0 || p->align <= 0) return LUX_SGM_EINVAL;
+    if (!bytes) return LUX_SGM_EINVAL;
+    std::string err;
+    const int rc = lux::check_rect(p, &err);
+    if (rc) return rc;
     const uint64_t px = (uint64_t)p->width * p->height;
     *bytes = lux::rect_workspace_bytes(p) + px * 2 * (p->bits > 8 ? 2 : 1)  // images
              + px * 7;                                                     // outputs

@@ -107,6 +107,8 @@ int check_rect(const lux_sgm_rect_params* p, std::string* err)
     if (p->cost == LUX_SGM_COST_AD && (p->ad_cap < 0 || p->ad_cap > 28))
         return fail(LUX_SGM_EINVAL, "ad_cap must be 0..28", err);
     if (p->disparities <= 0 || p->align <= 0) return fail(LUX_SGM_EINVAL, "disparities/align must be positive", err);
+    if (p->disparities > 65535 || p->align > 65535)  // the uint16 range of the API; keeps the rounding below in int
+        return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
     if (((p->disparities + p->align - 1) / p->align) * p->align > 1024)
         return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
     if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535)
@@ -122,6 +124,8 @@ int check_layer(const lux_sgm_layer_params* p, std::string* err)
     if (p->width <= 0 || p->height <= 0 || p->planes <= 0 || p->align <= 0 || p->tile < 2)
         return fail(LUX_SGM_EINVAL, "bad layer parameters", err);
     if (p->width > 10240) return fail(LUX_SGM_EUNSUPPORTED, "width above 10240", err);
+    if (p->planes > 65535 || p->align > 65535)  // the uint16 range of the API; keeps the rounding below in int
+        return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
     if (((p->planes + p->align - 1) / p->align) * p->align > 1024)
         return fail(LUX_SGM_EUNSUPPORTED, "more than 1024 lanes per cell", err);
     if (p->p1 < 0 || p->p1 > 65535 || p->p2 < 0 || p->p2 > 65535)

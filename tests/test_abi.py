@@ -125,3 +125,62 @@ def test_skip_mask_mode2(w, h):
     assert (sgm.skip_mask(0, w, h) == 0).all()
     with pytest.raises(sgm.NotSupported):
         sgm.skip_mask(1, w, h)
+
+
+def test_rect_bytes_rejects_out_of_range_plane_counts():
+    """Plane counts past the uint16 range are refused before the lane rounding, which would overflow int."""
+    assert sgm.rect_bytes(640, 480, 64, bits=8) > 640 * 480 * 64 * 3
+    for D, align in ((2 ** 31 - 1, 8), (65536, 8), (64, 2 ** 31 - 1)):
+        with pytest.raises(sgm.NotSupported):
+            sgm.rect_bytes(64, 48, D, align=align)
+    with pytest.raises(sgm.NotSupported):
+        sgm.rect_bytes(64, 48, 1025)
+
+
+def _chain_len(c, W, H, DX):
+    """Length of diagonal chain c (chain_start in kernels_impl.h): c < W starts on the first row at x = c, else on the
+    first (DX > 0) or last (DX < 0) column at row c - W + 1."""
+    if c < W:
+        x0, y0 = c, 0
+    else:
+        x0, y0 = (0 if DX > 0 else W - 1), c - W + 1
+    nx = W - x0 if DX > 0 else x0 + 1
+    return min(nx, H - y0)
+
+
+def _diag_order(w, W, H, DX):
+    """diag_order<DX> of kernels_impl.h, restated: the chain wavefront w runs."""
+    if w >= W + H - 1:
+        return w
+    if W >= H:
+        F = W - H + 1
+        if w < F:
+            c = w
+        else:
+            t = w - F
+            j = t // 2 + 1
+            c = W - 1 + j if t & 1 else W - H + j
+    else:
+        F = H - W + 1
+        if w < F:
+            c = 0 if w == 0 else W - 1 + w
+        else:
+            t = w - F
+            j = t // 2 + 1
+            c = W - 1 + (H - W) + j if t & 1 else j
+    return W - 1 - c if (DX < 0 and c < W) else c
+
+
+def test_diag_order_is_a_longest_first_permutation():
+    """Every chain runs once, the full-length ones first and lengths never grow, for all shapes up to 40 x 40 (wider,
+    square and taller than wide) and both horizontal directions."""
+    for W in range(1, 41):
+        for H in range(1, 41):
+            n = W + H - 1
+            for DX in (1, -1):
+                order = [_diag_order(w, W, H, DX) for w in range(n)]
+                assert sorted(order) == list(range(n)), (W, H, DX)
+                lens = [_chain_len(c, W, H, DX) for c in order]
+                assert all(a >= b for a, b in zip(lens, lens[1:])), (W, H, DX, lens)
+                assert lens[0] == min(W, H) and sum(lens) == W * H
+                assert _diag_order(n + 3, W, H, DX) == n + 3

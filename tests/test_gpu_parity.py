@@ -109,7 +109,10 @@ RANGE_LAYERS = [(9, 7, 4, 48, 20, 8, True), (13, 5, 2, 40, 13, 8, False), (33, 1
                 (10, 10, 8, 32, 9, 2, False), (1, 6, 2, 24, 7, 8, False), (70, 40, 64, 128, 40, 8, True),
                 (130, 70, 64, 256, 128, 8, True), (64, 33, 8, 200, 99, 2, False), (40, 9, 4, 96, 1, 8, True),
                 (130, 70, 64, 300, 200, 8, True), (70, 40, 16, 512, 400, 8, False), (64, 33, 8, 300, 250, 2, True),
-                (33, 21, 4, 520, 512, 8, False)]
+                (33, 21, 4, 520, 512, 8, False),
+                # taller than wide and square, caps above 32: the unsegmented pass and both branches of its
+                # longest-first diagonal order
+                (40, 70, 8, 128, 40, 8, True), (48, 48, 8, 128, 40, 8, False)]
 
 
 @pytest.mark.parametrize("W,H,T,planes,cmax,align,smooth", RANGE_LAYERS)
