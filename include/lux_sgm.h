@@ -86,6 +86,18 @@ int lux_sgm_rectified(lux_sgm_ctx* ctx, const lux_sgm_rect_params* p, const void
 int lux_sgm_rectified_device(lux_sgm_ctx* ctx, const lux_sgm_rect_params* p, const void* d_left, const void* d_right,
                              uint16_t* d_disp, float* d_sub, uint8_t* d_valid, uint16_t* d_agg, void* stream);
 
+/* lux_sgm_rectified_device without device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory during the call (code under HIP graph capture, serving loops that forbid hipMalloc). The
+ * request's buffers (cost volume, aggregate, census codes, line states, border visit lists) come from one static
+ * device array of 96 MiB per device, held by the library from the first context on that device until the process
+ * ends; a request that needs more (a 640x480 D=64 census request takes about 68 MiB) fails with LUX_SGM_ENOMEM and
+ * nothing enqueued. The call allocates nothing on the device, creates no stream, and runs the border chains in order
+ * on `stream`. Every call on a device shares that device's array, so calls that overlap on one device race: serialise
+ * them (one stream, or one call at a time). Results are those of lux_sgm_rectified_device, bit for bit. */
+int lux_sgm_rectified_device_noalloc(lux_sgm_ctx* ctx, const lux_sgm_rect_params* p, const void* d_left,
+                                     const void* d_right, uint16_t* d_disp, float* d_sub, uint8_t* d_valid,
+                                     uint16_t* d_agg, void* stream);
+
 /* Parameters of one DenseLayer with a uniform plane range (start 0, count = planes, cap =
  * roundup(planes, align) for every pixel: the layer-0 form of DenseLayer.Init, DenseStereoSgm.cs:43-47). */
 typedef struct {

@@ -231,6 +231,7 @@ int lux_sgm_create(int device, uint64_t hbm_budget, lux_sgm_ctx** out)
         g_create_error = std::string("built for gfx950, device is ") + prop.gcnArchName;
         return LUX_SGM_EUNSUPPORTED;
     }
+    if (lux::noalloc_setup(device, &g_create_error)) return LUX_SGM_EDEVICE;
     auto* c = new lux_sgm_ctx();
     c->device = device;
     c->budget.limit = hbm_budget;

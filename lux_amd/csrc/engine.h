@@ -46,6 +46,12 @@ struct Workspace {
     int force_literal = 0;           // LUX_SGM_OPT_LITERAL: layers of <= 1024 lanes take the line-buffer-literal path
     int lit_resident[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // resident literal wavefronts [npl 2/4/0][adaptive]
 
+    // Arena mode (arena != nullptr): get() hands out 256-byte aligned pieces of [arena, arena + arena_cap) instead of
+    // allocating, and release() frees nothing. The no-allocation entry point gives a fresh workspace the library's
+    // static scratch this way, so a request's buffers need no hipMalloc.
+    uint8_t* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+
     Workspace() = default;
     Workspace(const Workspace&) = delete;
     Workspace& operator=(const Workspace&) = delete;
@@ -150,5 +156,9 @@ int enqueue_upsample(Workspace& ws, const float* d_prev, const float* d_prev_con
 int check_band_error(Workspace& ws, hipStream_t s, std::string* err);
 
 int hip_status(hipError_t e, const char* where, std::string* err);
+
+// Looks up the device address of the no-allocation entry point's scratch array on `device` (current) once per device
+// (noalloc.hip). lux_sgm_create calls it; the entry point calls it too, for a device it has not been called on yet.
+int noalloc_setup(int device, std::string* err);
 
 }  // namespace lux

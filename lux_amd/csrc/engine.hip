@@ -37,6 +37,19 @@ static int fail(int code, const std::string& msg, std::string* err)
 int Workspace::get(const char* name, size_t bytes, void** out, std::string* err)
 {
     DevBuf& b = bufs[name];
+    if (arena) {
+        if (b.n < bytes) {
+            const size_t at = (arena_used + 255) & ~(size_t)255;
+            if (at > arena_cap || bytes > arena_cap - at)
+                return fail(LUX_SGM_ENOMEM, std::string("scratch exhausted placing ") + name, err);
+            b.p = arena + at;
+            b.n = bytes;
+            arena_used = at + bytes;
+            filled.erase(name);
+        }
+        *out = b.p;
+        return 0;
+    }
     if (b.n < bytes) {
         if (budget && !budget->grow(b.n, bytes))
             return fail(LUX_SGM_ENOMEM, std::string("HBM budget exceeded allocating ") + name, err);
@@ -82,6 +95,13 @@ int Workspace::pinned(size_t bytes, void** out, std::string* err)
 
 void Workspace::release()
 {
+    if (arena) {  // the arena is not this workspace's to free
+        bufs.clear();
+        filled.clear();
+        sync_zeroed = nullptr;
+        arena_used = 0;
+        return;
+    }
     if (!bufs.empty()) (void)hipSetDevice(device);
     for (auto& kv : bufs) {
         if (kv.second.p) (void)hipFree(kv.second.p);

@@ -18,7 +18,8 @@ OK, EINVAL, EUNSUPPORTED, ENOMEM, EDEVICE, ECANCELED = 0, -1, -2, -3, -4, -5
 COST_CENSUS, COST_AD = 0, 1
 
 EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last_error", "lux_sgm_rect_bytes",
-           "lux_sgm_rectified", "lux_sgm_rectified_device", "lux_sgm_compute_layer", "lux_sgm_plan_info",
+           "lux_sgm_rectified", "lux_sgm_rectified_device",
+           "lux_sgm_rectified_device_noalloc", "lux_sgm_compute_layer", "lux_sgm_plan_info",
            "lux_sgm_plan_visits", "lux_sgm_set_profiling", "lux_sgm_stage_times", "lux_sgm_set_option",
            "lux_sgm_compute_layer_ranges", "lux_sgm_compute_layer_ex", "lux_sgm_skip_mask", "lux_sgm_mt19937_draw",
            "lux_sgm_box3", "lux_sgm_box3_device", "lux_sgm_upsample_layer", "lux_sgm_upsample_layer_device",
@@ -112,6 +113,7 @@ def lib():
         L.lux_sgm_rect_bytes.argtypes = [vp, ctypes.POINTER(u64)]
         L.lux_sgm_rectified.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_rectified_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lux_sgm_rectified_device_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer.argtypes = [vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer_ranges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_compute_layer_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -262,6 +264,15 @@ def rectified_device(ctx, p, d_left, d_right, d_disp, d_sub, d_valid, d_agg=0, s
                                           ctypes.c_void_p(d_right), ctypes.c_void_p(d_disp), ctypes.c_void_p(d_sub),
                                           ctypes.c_void_p(d_valid), ctypes.c_void_p(d_agg or None),
                                           ctypes.c_void_p(stream or None)), ctx.handle)
+
+
+def rectified_device_noalloc(ctx, p, d_left, d_right, d_disp, d_sub, d_valid, d_agg=0, stream=0):
+    """lux_sgm_rectified_device_noalloc: the same request with its buffers in the library's static scratch."""
+    _check(lib().lux_sgm_rectified_device_noalloc(ctx.handle, ctypes.byref(p), ctypes.c_void_p(d_left),
+                                                  ctypes.c_void_p(d_right), ctypes.c_void_p(d_disp),
+                                                  ctypes.c_void_p(d_sub), ctypes.c_void_p(d_valid),
+                                                  ctypes.c_void_p(d_agg or None), ctypes.c_void_p(stream or None)),
+           ctx.handle)
 
 
 class DenseLayer:
