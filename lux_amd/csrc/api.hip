@@ -30,7 +30,8 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
 {
     lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
     lux::Prepared w;
-    int rc = lux::prepare(c->ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, s, &w, &c->err);
+    int rc = lux::prepare(c->ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, s, &w, &c->err,
+                          lux::path_bytes(p));
     if (rc) return rc;
     const lux::Conc conc = lux::conc_of(c);
     lux::attach_chain(&w, &conc);
@@ -105,7 +106,8 @@ int prepare_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<
         std::map<std::string, void*> before;
         for (auto& kv : ws.bufs) before[kv.first] = kv.second.p;
         lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
-        rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, c->stream, &(*prep)[i], &c->err);
+        rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, c->stream, &(*prep)[i], &c->err,
+                          lux::path_bytes(p));
         if (rc) return rc;
         for (auto& kv : ws.bufs) {
             auto it = before.find(kv.first);

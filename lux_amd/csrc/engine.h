@@ -107,7 +107,13 @@ struct BufferNeed {
     std::string name;
     size_t bytes;
 };
-std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes = 0);
+// pv8: the request runs with byte path volumes (path_bytes).
+std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes = 0,
+                                        bool pv8 = false);
+// A rectified request whose eight path costs all fit a byte (census cost, no padding planes, census maximum + P2 <=
+// 255, at most 256 planes): it runs with one byte volume per path instead of a u16 aggregate read and written by
+// every path, 22 bytes a cell instead of 36, with the same results.
+bool path_bytes(const lux_sgm_rect_params* p);
 // Bytes by which ws must grow to hold `need` (0: it already does).
 uint64_t growth(const Workspace& ws, const std::vector<BufferNeed>& need);
 
@@ -122,7 +128,8 @@ int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::stri
 
 // Allocate the workspace for geometry g and upload the plan's visit lists (copies from host memory on s).
 // Nothing here may be captured into a graph; enqueue_* below may.
-int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err);
+int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err,
+            bool pv8 = false);
 
 // The line-buffer-literal path (launch_literal). prepare_literal gives w the raw and aggregate volumes of g (the
 // workspace buffers prepare() uses, so a layer's cost can be written into them first). run_literal runs one layer

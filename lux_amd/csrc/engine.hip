@@ -161,14 +161,27 @@ static std::string visits_name(const Plan& plan)
            (plan.xsw ? " xsw" : "");
 }
 
-std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes)
+bool path_bytes(const lux_sgm_rect_params* p)
+{
+    // Every path cost lc = satsub(sat(raw + m), minPrev) is at most raw + P2 (m <= minPrev + P2, DenseStereoSgm.cs:288-298),
+    // and the first step of a chain gives lc = raw (the line buffers hold 2000 and minPrev is 2000). A census cost of
+    // radius r is at most (2r+1)^2 - 1, so with no padding planes (cap = count, whose raw would be 255) every lc fits a
+    // byte when that maximum plus P2 does; the eight-path sum then stays below 65535 and needs no saturation.
+    if (p->cost != LUX_SGM_COST_CENSUS) return false;
+    const Geometry g = geometry(p->width, p->height, p->disparities, p->align);
+    if (g.cap != p->disparities || g.dpl > 4) return false;
+    const int r = p->census_radius, cmax = (2 * r + 1) * (2 * r + 1) - 1;
+    return cmax + p->p2 <= 255;
+}
+
+std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, bool census, size_t host_img_bytes, bool pv8)
 {
     const size_t cells = (size_t)g.W * g.H;
     const size_t maxvis = std::max(plan.plus.size(), plan.minus.size()) / 4 + 1;
     const size_t nv = plan.plus.size() + plan.minus.size() + 4 * (size_t)g.H;
     std::vector<BufferNeed> v = {
         {"raw", cells * g.cs},
-        {"agg", cells * g.cs * 2},
+        {"agg", pv8 ? 0 : cells * g.cs * 2},  // with byte path volumes only when the caller asks for the aggregate
         {"seeds", (size_t)3 * g.W * g.cs * 2},
         {"seedmins", (size_t)3 * g.W * 2},
         {"cols", (size_t)4 * g.H * g.cs * 2},
@@ -186,6 +199,10 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
     if (census) {
         v.push_back({"census_l", cells * 8});
         v.push_back({"census_r", cells * 8});
+    }
+    if (pv8) {
+        v.push_back({"path_bytes", 7 * cells * g.cs});
+        v.push_back({"border_agg", (size_t)2 * g.H * g.cs * 2});
     }
     if (host_img_bytes) {
         v.push_back({"img_l", host_img_bytes});
@@ -214,7 +231,7 @@ uint64_t rect_workspace_bytes(const lux_sgm_rect_params* p)
     std::shared_ptr<Plan> plan;
     if (plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, nullptr)) return 0;
     uint64_t s = 0;
-    for (const BufferNeed& b : request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS)) s += b.bytes;
+    for (const BufferNeed& b : request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS, 0, path_bytes(p))) s += b.bytes;
     return s;
 }
 
@@ -238,13 +255,14 @@ int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::stri
     return 0;
 }
 
-int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err)
+int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err,
+            bool pv8)
 {
     std::shared_ptr<Plan> plan;
     int rc = plan_for(g.W, g.H, T, xsw, &plan, err);
     if (rc) return rc;
     std::map<std::string, void*> ptr;
-    for (const BufferNeed& nb : request_buffers(g, *plan, census)) {
+    for (const BufferNeed& nb : request_buffers(g, *plan, census, 0, pv8)) {
         void* q;
         if ((rc = ws.get(nb.name.c_str(), nb.bytes, &q, err))) return rc;
         ptr[nb.name] = q;
@@ -310,6 +328,12 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     // constant P2: the kernels' guidance loads read word 0 of an allocation that always exists
     w->b.ap = AdaptP2();
     w->b.ap.guid = (const uint32_t*)sync;
+    if (pv8) {
+        const size_t vol = (size_t)g.W * g.H * g.cs;
+        for (int k = 0; k < 7; k++) w->b.pv[k] = (uint8_t*)ptr["path_bytes"] + k * vol;
+        w->b.bagg[0] = (uint16_t*)ptr["border_agg"];
+        w->b.bagg[1] = w->b.bagg[0] + (size_t)g.H * g.cs;
+    }
     return 0;
 }
 
@@ -339,6 +363,12 @@ int enqueue_rect(Workspace& ws, Prepared& w, const lux_sgm_rect_params* p, const
     so.bands = opt.bands;
     w.b.zero_flags = opt.replayable;
     w.b.solo = !opt.shared;
+    if (w.b.pv[0] && o.agg) {  // byte path volumes hold no aggregate: the last row path writes it here when asked
+        void* a;
+        int rc = ws.get("agg", (size_t)g.W * g.H * g.cs * 2, &a, err);
+        if (rc) return rc;
+        w.agg = w.b.agg = (uint16_t*)a;
+    }
     e = launch_sgm(g, w.b, so, p->p1, p->p2, nullptr, s, ev ? ev + 2 : nullptr);
     if (e != hipSuccess) return hip_status(e, "aggregation", err);
     if (o.agg) {

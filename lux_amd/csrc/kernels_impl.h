@@ -159,6 +159,40 @@ __device__ __forceinline__ void store_u16(uint16_t* p, bool active, const Cell<N
     }
 }
 
+// A path cost vector as bytes (byte path volumes): the low byte of each u16 plane, planes in order. The caller
+// guarantees the active planes' costs are below 256 (cost + P2 <= 255 bounds every lc, see PathBytes below).
+template <int NP>
+__device__ __forceinline__ void store_bytes(uint8_t* p, bool active, const Cell<NP>& c)
+{
+    if (!active) return;
+    if constexpr (NP == 1) {
+        *(uint16_t*)p = (uint16_t)__builtin_amdgcn_perm(0u, c.v[0], 0x0c0c0200u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) ((uint32_t*)p)[k] = __builtin_amdgcn_perm(c.v[2 * k + 1], c.v[2 * k], 0x06040200u);
+    }
+}
+template <int NP>
+__device__ __forceinline__ void store_zero_bytes(uint8_t* p, bool active)
+{
+    if (!active) return;
+    if constexpr (NP == 1) {
+        *(uint16_t*)p = 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++) ((uint32_t*)p)[k] = 0u;
+    }
+}
+// acc += the bytes of w widened to packed u16 pairs (the inverse layout of store_bytes).
+template <int NP>
+__device__ __forceinline__ void add_bytes(const RawW<NP>& w, Cell<NP>& acc)
+{
+    Cell<NP> t;
+    widen<NP>(w, t);
+#pragma unroll
+    for (int k = 0; k < NP; k++) acc.v[k] += t.v[k];  // no carry between the halves: sums stay far below 65536
+}
+
 // One path step of DenseLayer.Pixel (:276-314) for the planes this lane owns.
 //   m  = min(min(sat(Lm+P1), L0), min(sat(Lp+P1), P2p)),  P2p = min(P2 + minPrev, 65535)
 //   lc = satsub(sat(raw + m), minPrev)
@@ -416,6 +450,7 @@ struct SweepArgs {
     uint16_t* seedcap;
     AdaptP2 ap;
     uint32_t* seedguid;      // [3][W] (adaptive P2)
+    uint8_t* pv = nullptr;   // byte path volume [H][W][cs] of this path (PV sweeps)
 };
 
 // Per-cell words of a chain (the cell's range, its guidance), 64 chain positions in one VGPR (lane k: position
@@ -487,8 +522,9 @@ struct ChainP2 {
 
 // One path direction of one pass. Path P of pass DIR steps by (DX, DY) = DIR * {(1,0),(1,1),(0,1),(-1,1)}[P]
 // (the neighbour slots of :265). Each wavefront runs whole chains. The border column bx (W-1 in pass +1,
-// 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it.
-template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
+// 0 in pass -1) is left to the replay kernel; FIRST writes the aggregate instead of adding to it. PV writes the
+// path's costs as bytes into its own volume a.pv (0 in the border column) and leaves the aggregate alone.
+template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP, bool PV = false>
 __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int lane)
 {
     constexpr int DX = DIR * (P == 0 ? 1 : P == 1 ? 1 : P == 2 ? 0 : -1);
@@ -499,13 +535,16 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
     chain_start<DX, DY>(chain, W, H, x, y, n);
     const int bx = DIR > 0 ? W - 1 : 0;
     if (P == 2 && x == bx) {  // the border column's vertical chain is the replay's
-        if (FIRST) {          // a path-cost volume of its own still needs its zeros there
+        if (FIRST || PV) {    // a path-cost volume of its own still needs its zeros there
             const int lo = lane * 2 * NP;
             Cell<NP> z;
 #pragma unroll
             for (int k = 0; k < NP; k++) z.v[k] = 0;
-            for (int i = 0; i < n; i++)
-                store_u16<NP>(a.agg + ((size_t)(y + i * DY) * W + (x + i * DX)) * cs + lo, lo < cs, z);
+            for (int i = 0; i < n; i++) {
+                const size_t c = (size_t)(y + i * DY) * W + (x + i * DX);
+                if constexpr (PV) store_zero_bytes<NP>(a.pv + c * cs + lo, lo < cs);
+                else store_u16<NP>(a.agg + c * cs + lo, lo < cs, z);
+            }
         }
         return;
     }
@@ -558,7 +597,7 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
     for (int j = 0; j < PF; j++) {
         const size_t cj = cell0 + (ptrdiff_t)min(j, n - 1) * step;
         ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
-        if (!FIRST) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
+        if (!FIRST && !PV) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
     }
     auto body = [&](int i, int j) {
         Cell<NP> raw;
@@ -577,16 +616,20 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
         const uint32_t p2 = ADP ? gq.at(i) : a.p2;
         const size_t cp = cell0 + (ptrdiff_t)min(i + PF, n - 1) * step;
         ld_raw_w<NP>(a.raw + cp * cs + lol, rb[j]);
-        if (!FIRST) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
+        if (!FIRST && !PV) ld_u16<NP>(a.agg + cp * cs + lol, ab[j]);
         if constexpr (VAR) rs.advance(i, rs_at, lane);
         if constexpr (ADP) gq.advance(i, gq_at, lane, a.ap, a.p2);
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
         if (x != bx) {
-            Cell<NP> out;
+            if constexpr (PV) {
+                store_bytes<NP>(a.pv + cell * cs + lo, active, lc);
+            } else {
+                Cell<NP> out;
 #pragma unroll
-            for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
-            store_u16<NP>(a.agg + cell * cs + lo, active, out);
+                for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
+                store_u16<NP>(a.agg + cell * cs + lo, active, out);
+            }
             if (DIR > 0 && P != 0 && y == a.seed_row) {
                 store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, lo < cs, lc);
                 if (lane == 0) {
@@ -598,6 +641,8 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
                     if (ADP) a.seedguid[(P - 1) * W + x] = gc;
                 }
             }
+        } else if constexpr (PV) {
+            store_zero_bytes<NP>(a.pv + cell * cs + lo, active);
         } else if (FIRST) {
             Cell<NP> z;
 #pragma unroll
@@ -627,6 +672,13 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 {
     sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
 }
+// The sweep of a request with byte path volumes: its costs into its own volume a.pv.
+template <int NP, int DIR, int P, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_sweep_pv(SweepArgs a)
+{
+    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                                                              threadIdx.x & 63);
+}
 
 // ------------------------------------------------------------------ horizontal paths (rows)
 struct RowArgs {
@@ -648,6 +700,11 @@ struct RowArgs {
     const uint16_t* vcap;
     const uint16_t* vcount;
     AdaptP2 ap;
+    // byte path volumes (PV): pv[k] is path k of the eight in launch order (-> of pass +1, its three sweeps, pass -1's
+    // three sweeps), each [H][W][cs]; the row path of pass +1 writes pv[0], the one of pass -1 sums them all
+    uint8_t* pv[7] = {};
+    const uint16_t* bagg_plus = nullptr;   // [H][cs] pass +1's border-column (x = W-1) costs from its replay
+    const uint16_t* bagg_minus = nullptr;  // [H][cs] pass -1's border-column (x = 0) costs from its replay
 };
 
 // Census codes along a row. Lane l owns planes d = lo .. lo+2*NP-1 (lo = 2*NP*l) and keeps the window
@@ -690,8 +747,8 @@ __device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2
 // Pass +1, path 0 (->): one wavefront per row, x = 0 .. W-1. The first aggregate writer (writes lc instead
 // of adding; the border column W-1 gets 0 and is filled by the replay). COST: computes the census cost of
 // each cell on the way and writes the raw volume for the later sweeps. Saves the W-2 column's states for
-// the border replay.
-template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
+// the border replay. PV: the costs go to the byte path volume pv[0] instead of the aggregate.
+template <int NP, bool MASKED, bool COST, bool VAR, bool ADP, bool PV = false>
 __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
 {
     static_assert(!(COST && VAR), "per-pixel ranges take their raw volume from the cost kernel");
@@ -792,7 +849,8 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
         Cell<NP> out;
 #pragma unroll
         for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
-        store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
+        if constexpr (PV) store_bytes<NP>(a.pv[0] + (rowcell + x) * cs + lo, active, out);
+        else store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
         if (x == W - 2) {
             store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, lc);
             if (lane == 0) a.colmins[y] = (uint16_t)minprev;
@@ -814,6 +872,13 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
     row_fwd_wave<NP, MASKED, COST, VAR, ADP>(
         a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
+// Pass +1's row path of a request with byte path volumes: census cost fused, costs into pv[0].
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_row_fwd_pv(RowArgs a)
+{
+    row_fwd_wave<NP, MASKED, true, false, false, true>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
+}
 
 // Pass -1, path 0 (<-), the last aggregate contribution to every cell but the border column 0, with the
 // winner-take-all fused: one wavefront per row, x = W-1 .. 1.
@@ -826,11 +891,14 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
 //         plane 0 completes xr = x. Each step shifts it down one plane (v_mov inside a lane, DPP wave_shl
 //         across lanes) and folds in the cell's keys. Plane 1 after x = 1 is dR(0) without cell 0.
 // Saves the column-1 states for the border replay. WTA = false: the path alone, its cost added to a.agg (the
-// rest of the sum and the winner-take-all are sgm_finish's).
-template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA>
+// rest of the sum and the winner-take-all are sgm_finish's). PV: the final aggregate is the sum of the seven byte
+// path volumes, this path's cost and, in column W-1, pass +1's border costs (bagg_plus); agg is only written (when
+// asked for), never read.
+template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA, bool PV = false>
 __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 {
-    constexpr int PF = 8, E = 2 * NP;
+    static_assert(!PV || (WTA && !VAR && !ADP), "byte path volumes: uniform constant-P2 requests");
+    constexpr int PF = 8, E = 2 * NP, NV = PV ? 7 : 0;
     if (y >= a.H) return;
     const int W = a.W, cs = a.cs, lo = lane * E;
     int D = a.D;
@@ -847,6 +915,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     const int lol = (VAR ? lo < cs : active) ? lo : 0;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
+    RawW<NP> vb[PF][NV > 0 ? NV : 1];  // PV: the seven path volumes' bytes of the cells in flight
     Range from;
     // VAR / ADP: range (start | cap << 16), count and guidance words by position W-1-x, 64 a VGPR (ChainWords)
     const auto cell_at = [&](int pos) { return rowcell + max(W - 1 - pos, 0); };
@@ -867,7 +936,12 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
-        ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        if constexpr (PV) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xj) * cs + lol, vb[j][v]);
+        } else {
+            ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+        }
     }
     // lane x&63 keeps cell x's winner and the parabola's inputs; the float arithmetic of the subpixel runs once per
     // 64 cells, lane-parallel, when they are stored (the same operations on the same values as per cell)
@@ -877,7 +951,21 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     auto body = [&](int x, int j) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
-        const Cell<NP> agg = ab[j];
+        Cell<NP> agg;
+        if constexpr (PV) {  // the other seven paths' costs (and pass +1's border-column replay in column W-1)
+#pragma unroll
+            for (int k = 0; k < NP; k++) agg.v[k] = 0u;
+#pragma unroll
+            for (int v = 0; v < NV; v++) add_bytes<NP>(vb[j][v], agg);
+            if (x == W - 1) {
+                Cell<NP> e;
+                load_u16<NP>(a.bagg_plus + (size_t)y * cs + lo, lo < cs, e);
+#pragma unroll
+                for (int k = 0; k < NP; k++) agg.v[k] += e.v[k];
+            }
+        } else {
+            agg = ab[j];
+        }
         const int xp = max(x - PF, 0);
         uint32_t el = kInit2, er = kInit2;
         int s0 = 0;
@@ -896,7 +984,12 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             gq.advance(W - 1 - x, gq_at, lane, a.ap, a.p2);
         }
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
-        ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        if constexpr (PV) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xp) * cs + lol, vb[j][v]);
+        } else {
+            ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+        }
         Cell<NP> lc, fin;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
         if (x == 1) {
@@ -976,6 +1069,13 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 {
     row_bwd_wave<NP, MASKED, VAR, ADP, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
 }
+// Pass -1's row path of a request with byte path volumes: the eight paths' sum and the winner-take-all.
+template <int NP, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_pv(RowArgs a)
+{
+    row_bwd_wave<NP, MASKED, false, false, true, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                                                        threadIdx.x & 63);
+}
 
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
 // valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol (RealStereo.cs:418-421).
@@ -988,7 +1088,18 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
     const int s0 = VAR ? (int)a.vstart[rowcell] : 0;
     const bool active = lo < cs;
     Cell<NP> c;
-    load_u16<NP>(a.agg + rowcell * cs + lo, active, c);
+    if (a.bagg_minus) {  // byte path volumes: column 0 is pass +1's four paths and pass -1's border replay
+        load_u16<NP>(a.bagg_minus + (size_t)y * cs + lo, active, c);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            RawW<NP> w;
+            load_raw_w<NP>(a.pv[v] + rowcell * cs + lo, active, w);
+            add_bytes<NP>(w, c);
+        }
+        if (a.write_agg) store_u16<NP>(a.agg + rowcell * cs + lo, active, c);
+    } else {
+        load_u16<NP>(a.agg + rowcell * cs + lo, active, c);
+    }
     uint32_t best = 0xffffffffu;
 #pragma unroll
     for (int e = 0; e < 2 * NP; e++) {
@@ -1724,6 +1835,7 @@ struct ReplayArgs {
     uint16_t* seedcap;
     AdaptP2 ap;
     uint32_t* seedguid;       // [3][W] (adaptive P2)
+    uint16_t* colagg = nullptr;  // byte path volumes: the border column's visit costs go here, [H][cs], instead of agg
 };
 
 // The state a path-0 / path-1 neighbour read returns, by plan label: -1 the initial fill, y >= 0 the saved
@@ -2082,7 +2194,12 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int
     const bool active = lo < (VAR ? (int)a.vcap[(size_t)y * a.W + a.col] : a.cap);
     const size_t cell = ((size_t)y * a.W + a.col) * a.cs + lo;
     Cell<NP> agg, p;
-    load_u16<NP>(a.agg + cell, active, agg);
+    if (a.colagg) {  // the border column's costs of this pass alone, summed later with the path volumes
+#pragma unroll
+        for (int k = 0; k < NP; k++) agg.v[k] = 0u;
+    } else {
+        load_u16<NP>(a.agg + cell, active, agg);
+    }
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         const int i = cellvis[2 * y + j];
@@ -2095,7 +2212,8 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int
             for (int k = 0; k < NP; k++) agg.v[k] = padd(agg.v[k], p.v[k]);
         }
     }
-    store_u16<NP>(a.agg + cell, active, agg);
+    if (a.colagg) store_u16<NP>(a.colagg + (size_t)y * a.cs + lo, lo < a.cs, agg);
+    else store_u16<NP>(a.agg + cell, active, agg);
 }
 
 // Seeds of interior cells when pass +1 never writes half 0 (H = 1): the initial fill.
@@ -2292,6 +2410,14 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     ra.disp = o.disp; ra.sub = o.sub; ra.dr = b.dr; ra.dr0 = b.dr0; ra.write_agg = o.write_agg;
     ra.vstart = b.vstart; ra.vcap = b.vcap; ra.vcount = b.vcount;
     ra.ap = b.ap;
+    // byte path volumes: uniform constant-P2 census requests of at most 256 planes whose path costs fit a byte
+    // (path_bytes in engine.hip decides and gives the volumes)
+    constexpr bool kPv = !VAR && !ADP && SEG == 0 && NP <= 2;
+    const bool pv = kPv && b.pv[0] != nullptr && b.cl != nullptr;
+    if (pv) {
+        for (int k = 0; k < 7; k++) ra.pv[k] = b.pv[k];
+        ra.bagg_plus = b.bagg[0];
+    }
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
     // one chain a wavefront: one-wavefront workgroups when the request has the device to itself (SgmBuffers::solo)
@@ -2309,6 +2435,21 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     const size_t aggn = (size_t)g.W * g.H * g.cs / 2;  // u16 pairs of one volume (cs is a multiple of 4)
     auto vertical_diagonal = [&](auto dirc, uint16_t* cols, uint16_t* colmins) {
         constexpr int DIR = decltype(dirc)::value;
+        if constexpr (kPv) {
+            if (pv) {  // each path into its own byte volume: 1..3 (pass +1), 4..6 (pass -1)
+                SweepArgs q = a;
+                q.pv = b.pv[DIR > 0 ? 1 : 4];
+                sgm_sweep_pv<NP, DIR, 1, MASKED><<<grid(nd), wg, 0, s>>>(q);
+                mark();
+                q.pv = b.pv[DIR > 0 ? 2 : 5];
+                sgm_sweep_pv<NP, DIR, 2, MASKED><<<grid(nv), wg, 0, s>>>(q);
+                mark();
+                q.pv = b.pv[DIR > 0 ? 3 : 6];
+                sgm_sweep_pv<NP, DIR, 3, MASKED><<<grid(nd), wg, 0, s>>>(q);
+                mark();
+                return;
+            }
+        }
         if constexpr (SEG > 0) {
             sgm_sweep_seg<SEG, DIR, 1, false><<<sblocks(nd), kBlock, 0, s>>>(a);
             mark();
@@ -2413,6 +2554,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial; r.chainp = b.chainp;
     r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
     r.ap = b.ap; r.seedguid = b.seedguid;
+    r.colagg = pv ? b.bagg[0] : nullptr;
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
     a.cols = b.cols; a.colmins = b.colmins;
     ra.cols = b.cols; ra.colmins = b.colmins;
@@ -2428,6 +2570,10 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             sgm_row_fwd_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
         } else if constexpr (VAR) {
             sgm_row_fwd<NP, MASKED, false, true, ADP><<<grid(nh), wg, 0, s>>>(ra);
+        } else if constexpr (kPv) {
+            if (pv) sgm_row_fwd_pv<NP, MASKED><<<grid(nh), wg, 0, s>>>(ra);
+            else if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
+            else sgm_row_fwd<NP, MASKED, false, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
         } else {
             if (b.cl) sgm_row_fwd<NP, MASKED, true, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
             else sgm_row_fwd<NP, MASKED, false, false, ADP><<<grid(nh), wg, 0, s>>>(ra);
@@ -2451,6 +2597,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     // pass -1: up-left, up, up-right (starting from pass +1's half-0 leftovers), then <- with the
     // winner-take-all fused (the last contribution)
     r.dir = -1; r.col = 0; r.visits = b.visits_minus; r.nvis = b.n_minus;
+    r.colagg = pv ? b.bagg[1] : nullptr;
     r.cols = b.cols + colsz; r.colmins = b.colmins + colminsz;
     a.cols = b.cols + colsz; a.colmins = b.colmins + colminsz;
     ra.cols = b.cols + colsz; ra.colmins = b.colmins + colminsz;
@@ -2466,8 +2613,14 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     if (fused) {
         mark();  // the row path ran in the pass launch
     } else {
-        if constexpr (SEG > 0) sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
-        else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
+        if constexpr (SEG > 0) {
+            sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
+        } else if constexpr (kPv) {
+            if (pv) sgm_row_bwd_wta_pv<NP, MASKED><<<grid(nh), wg, 0, s>>>(ra);
+            else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
+        } else {
+            sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
+        }
         mark();
     }
     if (r.nvis > 0) sgm_replay_paths<NP, MASKED, VAR><<<blocks(r.nvis), kBlock, 0, s>>>(r);
@@ -2478,6 +2631,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         const size_t waves = (size_t)g.H * ((g.W + 63) / 64);
         sgm_finish<NP><<<(unsigned)((waves + 3) / 4), kBlock, 0, s>>>(ra, b.aggp[0], b.aggp[1], b.aggp[2]);
     } else {
+        if (pv) ra.bagg_minus = b.bagg[1];  // column 0 from the volumes and pass -1's border replay
         wta_finish<NP, VAR><<<g.H, 64, 0, s>>>(ra, o.tol, VAR ? nullptr : o.valid);
     }
     mark();

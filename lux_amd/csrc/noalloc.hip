@@ -1,7 +1,7 @@
 // The rectified request without device allocation (lux_sgm_rectified_device_noalloc): for callers that cannot call
 // hipMalloc (code under graph capture, serving loops that forbid it). Its buffers are carved out of one file-scope
 // device array of kScratchBytes instead of the context's pooled workspace; the kernels and the enqueue are the ones
-// lux_sgm_rectified_device runs, so the results are the same bit for bit.
+// lux_sgm_rectified_device runs (in their u16-aggregate form), so the results are the same bit for bit.
 //
 // The array exists once per device the library runs on, for the life of the process. Every call on a device uses
 // that device's copy and leaves its contents behind: what has to start at zero (the band flags) is cleared by a fill
@@ -69,9 +69,12 @@ int lux_sgm_rectified_device_noalloc(lux_sgm_ctx* c, const lux_sgm_rect_params* 
     const lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
     std::shared_ptr<lux::Plan> plan;
     if ((rc = lux::plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, &c->err))) return rc;
+    // the u16-aggregate form (3 bytes a cell), not the byte path volumes (8): the scratch holds more requests that way,
+    // and both give the same results
+    const bool census = p->cost == LUX_SGM_COST_CENSUS, pv8 = false;
+    std::vector<lux::BufferNeed> bufs = lux::request_buffers(g, *plan, census, 0, pv8);
     size_t need = 0;
-    for (const lux::BufferNeed& b : lux::request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS))
-        need = ((need + 255) & ~(size_t)255) + b.bytes;
+    for (const lux::BufferNeed& b : bufs) need = ((need + 255) & ~(size_t)255) + b.bytes;
     if (need > kScratchBytes)
         return lux::fail(c, LUX_SGM_ENOMEM,
                          "request needs " + std::to_string(need) + " bytes of scratch; the no-allocation entry point holds " +
@@ -82,7 +85,7 @@ int lux_sgm_rectified_device_noalloc(lux_sgm_ctx* c, const lux_sgm_rect_params* 
     ws.arena = (uint8_t*)base;
     ws.arena_cap = kScratchBytes;
     lux::Prepared w;
-    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, p->cost == LUX_SGM_COST_CENSUS, s, &w, &c->err))) return rc;
+    if ((rc = lux::prepare(ws, g, p->tile, p->xs_clamp_w, census, s, &w, &c->err, pv8))) return rc;
     // no side stream: creating one is not this entry point's to do; the border chains run in order on s
     lux::RectOut o;
     o.disp = d_disp;

@@ -206,7 +206,9 @@ int launch(lux_sgm_server* s, int k, Request& r)
     } else {
         lux::Geometry g = lux::geometry(p.width, p.height, p.disparities, p.align);
         lux::Prepared w;
-        if ((rc = lux::prepare(ws, g, p.tile, p.xs_clamp_w, p.cost == LUX_SGM_COST_CENSUS, st, &w, &err))) return rc;
+        if ((rc = lux::prepare(ws, g, p.tile, p.xs_clamp_w, p.cost == LUX_SGM_COST_CENSUS, st, &w, &err,
+                               lux::path_bytes(&p))))
+            return rc;
         lux::EngineOpts opt;
         opt.shared = true;  // the other slots run beside this request
         if ((rc = lux::enqueue_rect(ws, w, &p, dl, dr, o, opt, st, nullptr, &err))) return rc;
@@ -453,7 +455,8 @@ int lux_sgm_server_submit(lux_sgm_server* s, const lux_sgm_rect_params* p, int32
     lux::Geometry g = lux::geometry(p->width, p->height, p->disparities, p->align);
     std::shared_ptr<lux::Plan> plan;
     if ((rc = lux::plan_for(g.W, g.H, p->tile, p->xs_clamp_w, &plan, &err))) return fail(s, rc, err);
-    r->bufs = lux::request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS, mem == LUX_SGM_MEM_HOST ? imgb : 0);
+    r->bufs = lux::request_buffers(g, *plan, p->cost == LUX_SGM_COST_CENSUS, mem == LUX_SGM_MEM_HOST ? imgb : 0,
+                                   lux::path_bytes(p));
     return submit(s, r, left, right, ticket);
 }
 

@@ -76,6 +76,13 @@ def test_rectified_front_ends(ctx, bits, cost, radius):
     _rect_case(ctx, 97, 53, 32, bits=bits, cost=cost, radius=radius or 3, T=8, seed=bits)
 
 
+@pytest.mark.parametrize("D,p1,p2", [(32, 8, 96), (30, 8, 96), (32, 20, 300), (256, 8, 207), (256, 8, 208)])
+def test_rectified_census_byte_bound(ctx, D, p1, p2):
+    """Census 7x7 (costs up to 48): P2 <= 207 with no padding planes runs with byte path volumes, a padded plane count
+    (30 of cap 32) or P2 >= 208 with the u16 aggregate; every one equals the oracle."""
+    _rect_case(ctx, 75, 41, D, bits=10, cost="census", radius=3, T=16, p1=p1, p2=p2, seed=D + p2)
+
+
 def test_rectified_config0(ctx):
     # 640x480 uint8, census 5x5, D=64, P1=10, P2=120, L/R tolerance 1 (seeded planar scene, 4 planes, sigma 2)
     got, want = _rect_case(ctx, 640, 480, 64, bits=8, cost="census", radius=2, T=64, p1=10, p2=120, planes=4, sigma=2.0)
