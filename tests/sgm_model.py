@@ -343,3 +343,163 @@ def spill_ranges(rng, W, H, planes, cmax, align):
     count[H - 1, W - 1] = max(1, cmax - (cmax % align == 0))
     start[H - 1, W - 1] = planes - count[H - 1, W - 1]
     return start.astype(np.uint16), count.astype(np.uint16)
+
+
+class _RefIndexError(IndexError):
+    """Where the reference's bounds-checked C# array access would throw."""
+
+
+_E = [np.array(b, np.uint32).view(np.float32)[()] for b in (0x3d9fcb52, 0x3e677e26, 0x3f322226, 0x3f7ffb19)]
+
+
+def p2_fast_exp2(p2_base, g, q, gw):
+    """Pixel's adaptive P2 (DenseStereoSgm.cs:266-273) in float32, one rounded operation at a time: the weighted
+    guidance differences, S = (|d0| + |d2|) + |d1|, e = clamp(-S, -126, 128), the cubic in the fraction (highest power
+    first, ExpPoly3..0), the exponent added to its bits, the truncating conversion of p2_base * E."""
+    f32 = np.float32
+    d = [(f32(g[c]) - f32(q[c])) * f32(gw[c]) for c in range(3)]
+    S = (abs(d[0]) + abs(d[2])) + abs(d[1])
+    e = max(min(-S, f32(128.0)), f32(-126.0))
+    ei = int(e) + (-1 if e < 0 else 0)
+    f = e - f32(ei)
+    poly = ((_E[0] * f + _E[1]) * f + _E[2]) * f + _E[3]
+    E = np.array(int(np.array(poly, np.float32).view(np.int32)) + (ei << 23), np.int64).astype(np.uint32).view(np.float32)
+    return int(f32(p2_base) * f32(E))
+
+
+def word_layer(raws, start, count, align, T, p1, p2, xsw=False, skip=None, guidance=None, gw=None, p2_scale=None,
+               past_end=0):
+    """DenseLayer.Compute over the reference's line buffers, word for word, written from DenseStereoSgm.cs without the
+    C oracle: AllocLineBuffers and LIdx (:158-166), the tile schedule (:206-222, via ``schedule``), Pixel's stale
+    cleanup (:254-264), neighbour reads through Rd (:318-324: below 0 -> 2000, past the end -> 0), the scalar
+    recurrence in its 8-lane blocks (:300-313), the range store (:314-316) and the first-minimum WTA (:225-231).
+    P2 is the constant p2, or with ``guidance`` ([H, W, 4] RGBA8), ``gw`` and ``p2_scale`` the adaptive one against the
+    guidance stored with the slot read (_Pix, :239, 0 where nothing was stored). ``past_end`` is the value a read past
+    the buffer's end returns (the reference's 0; another value shows the rule decides results). The vector form
+    (:280-299) is not modelled: its
+    conditions (whole blocks, every access inside the buffer, written range apart from the range read) make it give
+    the scalar form's words.
+
+    raws: list of per-pixel uint8 vectors (cap_i bytes, raster order); start/count: [H, W]; skip: [H, W] non-zero =
+    skipped (raw 0 over the count, 255 over the padding, :242) or None. Returns (agg, agg_pass1, wta, events):
+    agg/agg_pass1 packed like raws (uint16), wta [H, W] (start + arg), and a dict counting what the layer did with
+    the words past a range: writes to a segment's last word ('guard'), into the next path's segment of the slot
+    ('next_segment'), path 3's writes into the next slot's lead word ('next_lead') and path-0 segment
+    ('next_slot_path0'), reads of a word last written by another path or slot ('foreign_read'), reads past the end of
+    the buffer ('past_end'), and reads below its start ('below_start').
+    Raises _RefIndexError where the reference's array accesses go out of bounds (a write past the buffer, or an
+    aggregate lane past cap: the scalar blocks run to roundup(cap, 8) and cap is then not a multiple of 8)."""
+    H, W = start.shape
+    st = [int(v) for v in start.reshape(-1)]
+    cn = [int(v) for v in count.reshape(-1)]
+    caps = [((c + align - 1) // align) * align for c in cn]
+    sk = [0] * (W * H) if skip is None else [int(v) for v in np.asarray(skip).reshape(-1)]
+    min_lo = min(st)
+    max_hi = max(s + c for s, c in zip(st, cn))
+    M = ((max_hi + align - 1) // align) * align  # MaxHi, rounded (:51, :61)
+    stride = 4 * M + 5
+    slots = W + 2
+    half_l, half_m = slots * stride, slots * 4
+    L = [INIT] * (2 * half_l)
+    owner = [None] * (2 * half_l)  # (path, half, slot) of the last write of each word
+    Min = [INIT] * (2 * half_m)
+    Pix = [(0.0, 0.0, 0.0, 0.0)] * (2 * slots)
+    gimg = None if guidance is None else np.asarray(guidance).reshape(W * H, 4)
+    p2_base = None if guidance is None else np.float32(p1) * np.float32(p2_scale)
+    RS = [0] * (2 * slots)
+    RC = [align] * (2 * slots)
+    ev = dict(guard=0, next_segment=0, next_lead=0, next_slot_path0=0, foreign_read=0, past_end=0, below_start=0)
+    agg = [[0] * c for c in caps]
+    agg1 = None
+
+    def lidx(half, slot, path):
+        return half * half_l + slot * stride + 1 + path * (M + 1)
+
+    def rd(i, path, half, slot):
+        if i < 0:
+            ev["below_start"] += 1
+            return INIT
+        if i >= len(L):
+            ev["past_end"] += 1
+            return past_end
+        o = owner[i]
+        if o is not None and o != (path, half, slot):
+            ev["foreign_read"] += 1
+        return L[i]
+
+    last_dir = 1
+    for d, x, y, cur in schedule(W, H, T, xsw):
+        if d != last_dir:
+            agg1 = [list(a) for a in agg]
+            last_dir = d
+        other = 1 - cur
+        slot = x + 1
+        pi = y * W + x
+        g = None if gimg is None else tuple(float(v) for v in gimg[pi])
+        if g is not None:
+            Pix[cur * slots + slot] = g
+        s0, c0, cap = st[pi], cn[pi], caps[pi]
+        if sk[pi]:
+            raw = [0] * c0 + [255] * (cap - c0)
+        else:
+            raw = [int(v) for v in raws[pi]]
+        raw = raw + [0] * 8  # the scalar blocks read up to 7 bytes past cap of the caller's scratch (:245-246)
+        sp, cp = RS[cur * slots + slot], RC[cur * slots + slot]
+        for p in range(4):
+            base = lidx(cur, slot, p)
+            q0 = sp - min_lo
+            k0, k1 = max(0, -q0), min(cp, len(L) - base - q0)
+            for k in range(k0, k1):  # stale cleanup of the slot's previous range (:261-264)
+                L[base + q0 + k] = INIT
+                owner[base + q0 + k] = None
+            ns = slot + (0 if p == 2 else (d if p == 3 else -d))
+            half = cur if p == 0 else other
+            min_prev = Min[half * half_m + ns * 4 + p]
+            P2 = p2 if g is None else p2_fast_exp2(p2_base, g, Pix[half * slots + ns], gw)
+            p2p = min(P2 + min_prev, 65535)
+            if cap == 0:
+                Min[cur * half_m + slot * 4 + p] = 0xFFFF
+                continue
+            nb = lidx(half, ns, p)
+            run = 0xFFFF
+            q = s0 - min_lo
+            for b in range(0, cap, 8):
+                for l in range(8):
+                    qi = q + b + l
+                    lm = rd(nb + qi - 1, p, half, ns)
+                    lp = rd(nb + qi + 1, p, half, ns)
+                    l0 = rd(nb + qi, p, half, ns)
+                    m = min(min(min(lm + p1, 65535), l0), min(min(lp + p1, 65535), p2p))
+                    lc = max(min(raw[b + l] + m, 65535) - min_prev, 0)
+                    wi = base + qi
+                    if wi < 0 or wi >= len(L):
+                        raise _RefIndexError(f"line-buffer write at {wi} of {len(L)}")
+                    if qi == M:
+                        ev["guard"] += 1
+                    elif qi > M:
+                        if p < 3:
+                            ev["next_segment"] += 1
+                        elif qi == M + 1:
+                            ev["next_lead"] += 1
+                        else:
+                            ev["next_slot_path0"] += 1
+                    L[wi] = lc
+                    owner[wi] = (p, cur, slot)
+                    run = min(run, lc)
+                    if b + l >= cap:
+                        raise _RefIndexError(f"aggregate lane {b + l} of cap {cap}")
+                    agg[pi][b + l] = min(agg[pi][b + l] + lc, 65535)
+            Min[cur * half_m + slot * 4 + p] = run
+        RS[cur * slots + slot] = s0
+        RC[cur * slots + slot] = cap
+    if agg1 is None:
+        agg1 = [list(a) for a in agg]
+    wta = np.zeros((H, W), np.uint16)
+    for i in range(W * H):
+        best, arg = 0xFFFF, 0
+        for k in range(cn[i]):
+            if agg[i][k] < best:
+                best, arg = agg[i][k], k
+        wta.flat[i] = st[i] + arg
+    pack = lambda v: np.array([w for a in v for w in a], np.uint16)  # noqa: E731
+    return pack(agg), pack(agg1), wta, ev
