@@ -3,25 +3,33 @@
 reference's schedule -> WTA + subpixel + L/R, on one 4160x3120 uint16 10-bit pair, D=256 (P1=8, P2=96).
 
 Prints ONE JSON line. `value` = Gcells/s = W*H*D*8 paths / time per request, inputs resident in HBM.
-`roofline` = the dominant kernel's algorithmic HBM bytes per launch / its average duration (HIP events on
-the stream the kernels run on). `cpu_baseline` = the CPU oracle (the reference's algorithm restated in C:
-row-parallel cost, serial aggregation) on a bounded band of the same workload, on this host.
+`roofline` = the path's algorithmic HBM bytes (7 per cell: the reference's two-pass decomposition, write raw 1, pass +1
+raw 1 + agg 2, pass -1 raw 1 + agg 2) / the request's time (HIP events on the stream the kernels run on), against the
+8 TB/s peak; `roofline.traffic` = the HBM bytes one request moves, counted by rocprofv3 (FETCH_SIZE, WRITE_SIZE) and
+read from profiles/traffic.json when that file was made from the same kernel sources. `kernel_roofline` = the
+dominant kernel's own bytes over its time. `cpu_baseline` = the CPU oracle (the reference's algorithm restated in C:
+row-parallel cost, serial aggregation) on a bounded band of the same workload, on this host: the median of five runs
+after one warm-up; `cpu_baseline_c0` is the same for the 640x480 uint8 D=64 workload, whole.
 
-Two more legs go into the same line:
+More legs go into the same line:
   `batched`: one capture of 10 pairs (6 x 2080x1560 D=128, 4 x 3120x2340 D=192, independent seeds) as one graph
              launch with the autotuned schedule, against the same pairs run one after another (plain drop-in);
   `served`:  a mixed stream of depth requests (sizes uniform over 640x480 D64, 1040x780 D96, 2080x1560 D128,
              4160x3120 D256; host images in, host results out) through the request loop under a 96 GB budget:
-             latency p50/p99 at Poisson arrivals of 40 req/s, and throughput at saturation against the plain
-             drop-in running the same requests in order;
-  `coarse_to_fine`: 3-level requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8) in bursts of 16 through the
-             request loop under a 192 GB budget, latency p50/p99 against plain sequential execution.
+             latency p50/p99 at Poisson arrivals of 40 req/s, served and plain (the plain drop-in serving the same
+             arrivals in order, one at a time), and throughput at saturation against the plain drop-in in order;
+  `coarse_to_fine`: 2000 3-level requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8) in bursts of 16
+             through the request loop under a 192 GB budget, latency p50/p99 against plain sequential execution;
+  `l16_pyramid*`: DenseStereoPyramid.Run at 2080x1560 (device form), its oracle on the host, and a plane span whose
+             levels spill past their line-buffer segments (the real-capture case).
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import platform
-import subprocess
+import re
 import sys
 import time
 
@@ -32,22 +40,42 @@ W, H, D = 4160, 3120, 256
 P1, P2, TILE, RADIUS, BITS = 8, 96, 64, 3, 10
 HBM_PEAK_GBS = 8000.0
 PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1, agg 2), WTA fused
-# algorithmic HBM bytes per cell of each stage (what the stage must move at minimum)
-# algorithmic HBM bytes per cell of each timed stage:
-#   rows+ (pass +1 path ->, census cost fused): writes raw (1) and agg (2)
-#   vd+k / vd-k (one vertical or diagonal path sweep): reads raw (1), read-modify-writes agg (2 + 2);
-#     with fused bands vd+1 / vd-1 is the one launch for the pass's three paths, still 5 B/cell
-#   rows- (pass -1 path <-, winner-take-all fused): reads raw (1) and agg (2)
-STAGE_BYTES = {"rows+": 3, "rows-": 3}
-for _k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3"):
-    STAGE_BYTES[_k] = 5
+# HBM bytes per cell each timed stage of this design moves (what it must move at minimum):
+#   byte path volumes (census requests whose path costs fit a byte, lux_amd/csrc/engine.hip path_bytes):
+#     rows+ (pass +1 path ->, census cost fused): writes raw (1) and its path volume (1)
+#     vd+k / vd-k (one vertical or diagonal path): reads raw (1), writes its path volume (1)
+#     rows- (pass -1 path <-, the sum of the eight paths and the winner-take-all fused): raw (1) + seven volumes (7)
+#   u16 aggregate (every other request): rows+ raw 1 + agg 2; a sweep raw 1 + agg read 2 + agg write 2; rows- raw 1 +
+#     agg 2
+STAGE_BYTES_PV = {"rows+": 2, "rows-": 8, **{k: 2 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
+STAGE_BYTES_U16 = {"rows+": 3, "rows-": 3, **{k: 5 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
 KERNEL_NAMES = {"rows+": "sgm_row_fwd (pass +1 path ->, census cost fused)",
-                "rows-": "sgm_row_bwd_wta (pass -1 path <-, winner-take-all fused)",
+                "rows-": "sgm_row_bwd_wta (pass -1 path <-, eight-path sum and winner-take-all fused)",
                 "vd+1": "sgm_sweep<pass +1, path 1 (down-right)>", "vd+2": "sgm_sweep<pass +1, path 2 (down)>",
                 "vd+3": "sgm_sweep<pass +1, path 3 (down-left)>", "vd-1": "sgm_sweep<pass -1, path 1 (up-left)>",
                 "vd-2": "sgm_sweep<pass -1, path 2 (up)>", "vd-3": "sgm_sweep<pass -1, path 3 (up-right)>"}
-BAND_NAMES = {"vd+1": "sgm_band<pass +1> (paths 1-3 fused, ticketed band tiles)",
-              "vd-1": "sgm_band<pass -1> (paths 1-3 fused, ticketed band tiles)"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def path_bytes(d, radius, p2, align=8):
+    """Whether a census request runs with byte path volumes: the rule of path_bytes in lux_amd/csrc/engine.hip (no
+    padding planes, census maximum + P2 <= 255, at most 256 planes)."""
+    cap = -(-d // align) * align
+    return cap == d and d <= 256 and (2 * radius + 1) ** 2 - 1 + p2 <= 255
+
+
+def source_stamp():
+    """Digest of the kernel sources a counter profile was made from (the library's .hip/.h/.cpp and its header)."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "lux_amd", "csrc", "*.hip")) +
+                   glob.glob(os.path.join(ROOT, "lux_amd", "csrc", "*.h")) +
+                   glob.glob(os.path.join(ROOT, "lux_amd", "csrc", "*.cpp")) +
+                   [os.path.join(ROOT, "include", "lux_sgm.h")])
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def host_info():
@@ -62,17 +90,153 @@ def host_info():
     return model, os.cpu_count() or 1
 
 
-def cpu_baseline(left, right, rows, threads):
+def cpu_baseline(left, right, rows, threads, d=D, radius=RADIUS, bits=BITS, p1=P1, p2=P2, reps=5):
+    """The oracle on rows 0..rows-1 of the pair: one warm-up, then `reps` timed runs; the median."""
     from oracle import oracle
 
     band_l = left[:rows].copy()
     band_r = right[:rows].copy()
-    t0 = time.perf_counter()
-    out = oracle.rectified(band_l, band_r, D, cost="census", radius=RADIUS, bits=BITS, p1=P1, p2=P2, T=TILE,
-                           tol=1, threads=threads)
-    t = time.perf_counter() - t0
-    cells = W * rows * D
-    return cells * 8 / t / 1e9, t, out["times_ms"]
+    kw = dict(cost="census", radius=radius, bits=bits, p1=p1, p2=p2, T=TILE, tol=1, threads=threads)
+    oracle.rectified(band_l, band_r, d, **kw)
+    ts, parts = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = oracle.rectified(band_l, band_r, d, **kw)
+        ts.append(time.perf_counter() - t0)
+        parts.append(out["times_ms"])
+    import numpy as np
+
+    k = int(np.argsort(ts)[len(ts) // 2])
+    cells = left.shape[1] * rows * d
+    return cells * 8 / ts[k] / 1e9, ts[k], parts[k], sorted(ts)
+
+
+def load_traffic():
+    """The counted HBM bytes of one request (profiles/traffic.json), used only when the file was made from these kernel
+    sources: (data or None, a note saying which)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no counter profile (profiles/traffic.json)"
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError) as e:
+        return None, f"profiles/traffic.json unreadable: {e}"
+    stamp = source_stamp()
+    if t.get("source_sha16") != stamp:
+        return None, (f"profiles/traffic.json counts kernels of sources {t.get('source_sha16')}, these are {stamp}: "
+                      f"not used")
+    return t, f"profiles/traffic.json, rocprofv3 FETCH_SIZE/WRITE_SIZE of these kernel sources ({stamp})"
+
+
+def bench_c0(np, torch, sgm, scene, threads, model, ncpu, reps=20):
+    """The CPU reference path's workload, whole: 640x480 uint8, census 5x5, D=64, P1=10, P2=120, L/R tolerance 1. The
+    oracle on the host (median of five runs after one warm-up) and the device (inputs in HBM, HIP events)."""
+    from oracle import oracle
+
+    w, h, d = 640, 480, 64
+    left, right, _ = scene.make_pair(w, h, d - 1, planes=4, sigma=2.0, seed=0, bits=8)
+    kw = dict(cost="census", radius=2, bits=8, p1=10, p2=120, T=TILE, tol=1, threads=threads)
+    oracle.rectified(left, right, d, **kw)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        want = oracle.rectified(left, right, d, **kw)
+        ts.append(time.perf_counter() - t0)
+    cpu_s = float(np.median(ts))
+    dev = torch.device("cuda:0")
+    dl, dr = torch.from_numpy(left).to(dev), torch.from_numpy(right).to(dev)
+    od = torch.empty((h, w), dtype=torch.int16, device=dev)
+    osub = torch.empty((h, w), dtype=torch.float32, device=dev)
+    ov = torch.empty((h, w), dtype=torch.uint8, device=dev)
+    p = sgm.rect_params(w, h, d, bits=8, cost="census", radius=2, p1=10, p2=120, tile=TILE, tol=1)
+    st = torch.cuda.Stream()
+    ctx = sgm.Context(0)
+    call = lambda: sgm.rectified_device(ctx, p, dl.data_ptr(), dr.data_ptr(), od.data_ptr(), osub.data_ptr(),  # noqa: E731
+                                        ov.data_ptr(), 0, st.cuda_stream)
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        call()
+    e1.record(st)
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / reps
+    same = bool(np.array_equal(od.cpu().numpy().view(np.uint16), want["disp"]) and
+                np.array_equal(ov.cpu().numpy(), want["valid"]) and np.array_equal(osub.cpu().numpy(), want["sub"]))
+    ctx.close()
+    cells = w * h * d
+    return {"workload": "640x480 uint8, census 5x5, D=64, P1=10, P2=120, L/R tolerance 1 (4 planes, sigma 2, seed 0)",
+            "value": round(cells * 8 / cpu_s / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+            "sample": f"the whole pair, median of 5 runs after 1 warm-up: {cpu_s * 1e3:.1f} ms; host {model}, {ncpu} CPUs",
+            "runs_ms": [round(t * 1e3, 2) for t in sorted(ts)],
+            "gpu_ms": round(gpu_ms, 4), "gpu_gcells_s": round(cells * 8 / gpu_ms / 1e6, 2),
+            "gpu_vs_cpu": round(cpu_s * 1e3 / gpu_ms, 1), "identical_to_oracle": same}
+
+
+def _stage_of(kernel):
+    """The bench stage a kernel's dispatches belong to (rocprofv3's demangled names), or None."""
+    m = re.search(r"sgm_sweep(?:_pv)?<(\d+), (-?1), ([123])", kernel)
+    if m:
+        return f"vd{'+' if m.group(2) == '1' else '-'}{m.group(3)}"
+    if re.search(r"sgm_row_fwd(?:_pv)?<", kernel):
+        return "rows+"
+    if re.search(r"sgm_row_bwd_wta(?:_pv)?<", kernel):
+        return "rows-"
+    return None
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_bytes=1 << 30):
+    """profiles/traffic.json from two rocprofv3 --pmc runs of `bench.py --profile-run` (FETCH_SIZE, then WRITE_SIZE, in
+    KiB per dispatch): per kernel the mean bytes of a dispatch, scaled by what each counter reads for the 1 GiB device
+    copy the run ends with, times the dispatches of one request (a request launches the census kernel twice)."""
+    import csv
+
+    def read(d, counter):
+        per = {}
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]) * 1024.0)
+        return per
+
+    fetch, write = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise SystemExit("no copy kernel in the profile: run bench.py --profile-run")
+    rf = copy_bytes / max(fetch[copy[0]][-1], 1.0)
+    wf = copy_bytes / max(write[copy[0]][-1], 1.0)
+    census = [k for k in fetch if "census_kernel" in k]
+    nreq = sum(len(fetch[k]) for k in census) / 2
+    kernels, stages, total = {}, {}, 0.0
+    for k in fetch:
+        if "lux::" not in k:  # the calibration copy and torch's own kernels (the copy's source fill)
+            continue
+        rd = sum(fetch[k]) / nreq * rf
+        wr = sum(write.get(k, [0.0])) / nreq * wf
+        kernels[k] = {"read": round(rd), "write": round(wr), "dispatches_per_request": len(fetch[k]) / nreq}
+        st = _stage_of(k)
+        if st:
+            stages[st] = stages.get(st, 0) + round(rd + wr)
+        total += rd + wr
+    out = {"source_sha16": source_stamp(), "workload": f"{W}x{H} D={D} census 7x7 P1={P1} P2={P2}",
+           "read_factor": round(rf, 4), "write_factor": round(wf, 4), "requests": nreq,
+           "request_bytes": round(total), "stages": stages, "kernels": kernels}
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(out, fh, indent=1)
+    cells = W * H * D
+    print(f"read factor {rf:.4f}, write factor {wf:.4f}, {nreq:g} requests; per request {total / 1e9:.3f} GB = "
+          f"{total / cells:.3f} B/cell")
+    print("| kernel | read GB | write GB | total GB | B/cell |\n|---|---:|---:|---:|---:|")
+    for k, v in sorted(kernels.items(), key=lambda kv: -(kv[1]["read"] + kv[1]["write"])):
+        t = v["read"] + v["write"]
+        name = re.sub(r"^void |lux::|\(anonymous namespace\)::|\(.*$", "", k)
+        print(f"| `{name}` | {v['read'] / 1e9:.3f} | {v['write'] / 1e9:.3f} | {t / 1e9:.3f} | {t / cells:.3f} |")
 
 
 BATCH = [(2080, 1560, 128)] * 6 + [(3120, 2340, 192)] * 4
@@ -163,6 +327,18 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
         sgm.rectified(ctx, *scenes[k][i % 2], SERVE_SIZES[k][2], bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
         plain_lat.append((time.perf_counter() - a) * 1e3)
     plain_s = time.perf_counter() - t0
+    # the plain drop-in under the same Poisson arrivals: one request at a time in arrival order (a FIFO), latency from
+    # arrival to results in host memory, as for the served requests
+    arrivals = np.cumsum(gaps)
+    plain_poi = []
+    t_start = time.perf_counter()
+    for i, k in enumerate(sizes):
+        arrive = t_start + arrivals[i]
+        delay = arrive - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        sgm.rectified(ctx, *scenes[k][i % 2], SERVE_SIZES[k][2], bits=BITS, radius=RADIUS, p1=P1, p2=P2, tile=TILE)
+        plain_poi.append((time.perf_counter() - arrive) * 1e3)
     ctx.close()
 
     srv = sgm.Server(0, int(budget_gb * 1e9), slots=slots)
@@ -199,12 +375,13 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
         th.join()
         return time.perf_counter() - t_start, stats
 
-    for k in scenes:  # warm every slot's workspace for every size
-        run_w = [srv.submit(params[k], *scenes[k][0]) for _ in range(slots)]
+    for k in scenes:  # warm every slot's workspace and the pinned staging of a full window, outputs included
+        keep = [outputs(k) for _ in range(window)]
+        run_w = [srv.submit(params[k], *scenes[k][0], disp=o[0], sub=o[1], valid=o[2]) for o in keep]
         for t in run_w:
             srv.wait(t)
     sat_s, sat_stats = run(None)
-    poi_s, poi_stats = run(np.cumsum(gaps))
+    poi_s, poi_stats = run(arrivals)
     info = srv.info()
     srv.close()
     lat = np.array([st["total_ms"] for st in poi_stats])
@@ -222,8 +399,11 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
                            "p99_ms": round(float(np.percentile(sat_lat, 99)), 3)},
             "plain": {"wall_s": round(plain_s, 3), "gcells_s": round(cells * 8 / plain_s / 1e9, 3),
                       "req_per_s": round(n_req / plain_s, 3),
-                      "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
-                      "p99_ms": round(float(np.percentile(plain_lat, 99)), 3)},
+                      "service_p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+                      "service_p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
+                      "poisson": {"rate_per_s": rate, "p50_ms": round(float(np.percentile(plain_poi, 50)), 3),
+                                  "p99_ms": round(float(np.percentile(plain_poi, 99)), 3),
+                                  "latency": "arrival -> results in host memory, requests served in arrival order"}},
             "served_vs_plain_throughput": round(plain_s / sat_s, 3),
             "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
 
@@ -257,7 +437,8 @@ def bench_pyramid(np, sgm, scene, n_req, burst, budget_gb, slots):
     plain_s = time.perf_counter() - t0
     ctx.close()
     srv = sgm.Server(0, int(budget_gb * 1e9), slots=slots)
-    for t in [srv.submit_pyramid(pp, *scenes[0]) for _ in range(slots)]:  # warm every slot
+    # warm every slot and the pinned staging of a whole burst, outputs included (staging is made at first use)
+    for t in [srv.submit_pyramid(pp, *scenes[0], disp=outs[i][0], sub=outs[i][1]) for i in range(burst)]:
         srv.wait(t)
     lat = []
     t0 = time.perf_counter()
@@ -291,14 +472,18 @@ L16_SPAN_WIDE = (1000.0, 2200.0)   # the same planes over the scene's own depth 
                                    # depth edges, levels 1-5 on the sweeps with four packed pairs a lane
 
 
-def bench_l16(np, torch, sgm, scene, reps, cpu_threads, span=L16_SPAN, upsample=True):
+L16_SPILL = (1000.0, 2200.0, 64, 62)  # 64 planes over the scene's own span: MinLo 0 and ranges ending at the top
+                                      # plane, so thousands of cells a level spill past their line-buffer segment
+
+
+def bench_l16(np, torch, sgm, scene, reps, cpu_threads, span=L16_SPAN, upsample=True, n8=L16_N8, n2=L16_N2):
     """The L16 dense stereo end to end (DenseStereoPyramid.Run, six levels from 65x49 to 2080x1560, then the
     upsample layer to 4160x3120): one reference and four cameras of a rendered two-depth scene, 304 planes over the
     depth span `span`. The device call takes host images (box filter, levels, depth downloads included); the
     upsample is timed on device-resident buffers. The oracle runs the same pyramid on the host cores (cost
     row-parallel, aggregation serial as the reference) and its depths are compared bit for bit."""
     imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
-    p8, p2 = scene.plane_list(span[0], span[1], L16_N8), scene.plane_list(span[0], span[1], L16_N2)
+    p8, p2 = scene.plane_list(span[0], span[1], n8), scene.plane_list(span[0], span[1], n2)
     ctx = sgm.Context(0)
     depths, aligns = sgm.dense_pyramid(ctx, imgs, gray, warps, p8, p2)
     host_times = []
@@ -320,7 +505,7 @@ def bench_l16(np, torch, sgm, scene, reps, cpu_threads, span=L16_SPAN, upsample=
         times.append((time.perf_counter() - t) * 1e3)
     same = all(np.array_equal(d.cpu().numpy().view(np.uint32), h.view(np.uint32)) for d, h in zip(d_depths, depths))
     out = {"workload": f"DenseStereoPyramid.Run at {L16_W}x{L16_H} (6 levels, scales 32..1, 1 reference + 4 cameras, "
-                       f"{L16_N8} planes over z {span[0]:g}..{span[1]:g}, adaptive P2, skip masks on the last two)"
+                       f"{n8} planes over z {span[0]:g}..{span[1]:g}, adaptive P2, skip masks on the last two)"
                        + (f", then the upsample layer to {2 * L16_W}x{2 * L16_H}" if upsample else "")
                        + "; rendered two-depth scene",
            "levels": [{"scale": s, "align": a, "shape": list(d.shape)} for s, a, d in zip(sgm.L16_SCALES, aligns, depths)],
@@ -386,16 +571,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=1560, help="rows of the CPU-baseline band (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=520, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bands", type=int, default=0, help="1: fused band tiles for the vertical/diagonal paths")
-    ap.add_argument("--forms", default="batch,serve,pyramid,l16,l16wide",
-                    help="extra legs: batch, serve, pyramid, l16, l16wide (comma list; '' for none)")
-    ap.add_argument("--l16-span", default="default", choices=["default", "wide"],
-                    help="--profile-l16: the plane span (wide: ranges up to 288 planes, four packed pairs a lane)")
+    ap.add_argument("--forms", default="batch,serve,pyramid,l16,l16wide,l16spill",
+                    help="extra legs: batch, serve, pyramid, l16, l16wide, l16spill (comma list; none for none)")
+    ap.add_argument("--l16-span", default="default", choices=["default", "wide", "spill"],
+                    help="--profile-l16: the plane span (wide: ranges up to 288 planes, four packed pairs a lane; spill: "
+                         "64 planes whose ranges spill past their line-buffer segments)")
     ap.add_argument("--force-literal", action="store_true",
                     help="--profile-l16: every layer on the line-buffer-literal path (LUX_SGM_OPT_LITERAL)")
-    ap.add_argument("--pyr-requests", type=int, default=128)
+    ap.add_argument("--pyr-requests", type=int, default=2000)
     ap.add_argument("--pyr-burst", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-rate", type=float, default=40.0)
@@ -406,7 +592,12 @@ def main():
                     help="for trace runs: --steps L16 pyramids (device form, images in HBM) and nothing else")
     ap.add_argument("--profile-c2f", action="store_true",
                     help="for trace runs: --steps coarse-to-fine requests (plain, host buffers) and nothing else")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs of --profile-run (no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is a design note.",
               file=sys.stderr)
@@ -433,8 +624,9 @@ def main():
 
     if args.profile_l16:
         imgs, gray, warps, _ = scene.multiview(L16_W, L16_H, seed=7)
-        span = L16_SPAN_WIDE if args.l16_span == "wide" else L16_SPAN
-        p8, p2 = scene.plane_list(span[0], span[1], L16_N8), scene.plane_list(span[0], span[1], L16_N2)
+        span, n8, n2 = {"wide": (L16_SPAN_WIDE, L16_N8, L16_N2), "spill": (L16_SPILL[:2], L16_SPILL[2], L16_SPILL[3])}.get(
+            args.l16_span, (L16_SPAN, L16_N8, L16_N2))
+        p8, p2 = scene.plane_list(span[0], span[1], n8), scene.plane_list(span[0], span[1], n2)
         ctx = sgm.Context(0)
         if args.force_literal:
             ctx.set_option(sgm.OPT_LITERAL, 1)
@@ -513,33 +705,31 @@ def main():
             stages.setdefault(k, []).append(v)
     ctx.set_profiling(False)
     avg = {k: float(np.mean(v)) for k, v in stages.items()}
-    dom = max((k for k in avg if k in STAGE_BYTES), key=lambda k: avg[k])
-    names = dict(KERNEL_NAMES, **(BAND_NAMES if args.bands else {}))
+    pv = path_bytes(D, RADIUS, P2)
+    stage_bytes = STAGE_BYTES_PV if pv else STAGE_BYTES_U16
+    dom = max((k for k in avg if k in stage_bytes), key=lambda k: avg[k])
     cells = W * H * D
-    achieved = STAGE_BYTES[dom] * cells / (avg[dom] / 1e3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        try:
-            with open(tf) as fh:
-                traffic = json.load(fh).get("bands" if args.bands else "sweeps", {}).get(dom)
-        except (OSError, ValueError):
-            traffic = None
+    traffic, traffic_note = load_traffic()
+    design_bpc = sum(stage_bytes.values())
 
     d = disp.cpu().numpy().view(np.uint16)
     v = valid.cpu().numpy() > 0
     acc = float((np.abs(d.astype(np.int64) - truth)[v] <= 1).mean()) if v.any() else 0.0
 
-    cpu = None
+    cpu = cpu0 = None
     if not args.no_cpu and args.cpu_rows > 0:
         model, ncpu = host_info()
         threads = min(16, ncpu)
-        val, secs, tms = cpu_baseline(left, right, args.cpu_rows, threads)
+        print(f"bench: CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        val, secs, tms, all_s = cpu_baseline(left, right, args.cpu_rows, threads)
         cpu = {"value": round(val, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+               "runs_s": [round(t, 3) for t in all_s],
                "sample": (f"oracle/lux_oracle.c rectified pipeline on rows 0..{args.cpu_rows - 1} of the same "
-                          f"4160x{H} pair (D=256, census 7x7): {secs:.1f} s (cost {tms[0] / 1e3:.1f} s on {threads} "
-                          f"threads, aggregation {tms[1] / 1e3:.1f} s serial as DenseStereoSgm.cs:171, "
-                          f"WTA {tms[2] / 1e3:.1f} s); host {model}, {ncpu} CPUs")}
+                          f"4160x{H} pair (D=256, census 7x7): median of 5 runs after 1 warm-up, {secs:.2f} s (cost "
+                          f"{tms[0] / 1e3:.2f} s on {threads} threads, aggregation {tms[1] / 1e3:.2f} s serial as "
+                          f"DenseStereoSgm.cs:171, WTA {tms[2] / 1e3:.2f} s); host {model}, {ncpu} CPUs"),
+               "gpu_vs_cpu": round(gcells / val, 1)}
+        cpu0 = bench_c0(np, torch, sgm, scene, threads, model, ncpu)
 
     ctx.close()
     del dl, dr
@@ -561,6 +751,11 @@ def main():
     if "l16" in forms:
         print("bench: L16 dense pyramid leg", file=sys.stderr, flush=True)
         l16 = bench_l16(np, torch, sgm, scene, 5, 0 if args.no_cpu else min(16, os.cpu_count() or 1))
+    l16s = None
+    if "l16spill" in forms:
+        print("bench: L16 dense pyramid leg, spilling padding", file=sys.stderr, flush=True)
+        l16s = bench_l16(np, torch, sgm, scene, 3, 0 if args.no_cpu else min(16, os.cpu_count() or 1),
+                         span=L16_SPILL[:2], upsample=False, n8=L16_SPILL[2], n2=L16_SPILL[3])
     l16w = None
     if "l16wide" in forms:
         print("bench: L16 dense pyramid leg, wide ranges", file=sys.stderr, flush=True)
@@ -582,26 +777,38 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain GPU drop-in, one 4160x3120 uint16 10-bit pair, census 7x7, D=256, 8 paths, "
                                "P1=8, P2=96, WTA + subpixel + L/R check (tol 1), reference schedule T=64",
-                   "vertical_diagonal": "fused bands" if args.bands else "per-path sweeps",
+                   "paths": "byte path volumes" if pv else ("fused bands" if args.bands else "per-path sweeps"),
                    "width": W, "height": H, "disparities": D, "cells": cells, "inputs_sha256": digest},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "form": "plain"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": names.get(dom, dom), "avg_ms": round(avg[dom], 4),
-                     "algorithmic_bytes": STAGE_BYTES[dom] * cells},
-        "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL,
-                          "achieved": round(PATH_BYTES_PER_CELL * cells / (ms / 1e3) / 1e9, 1),
-                          "frac": round(PATH_BYTES_PER_CELL * cells / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(PATH_BYTES_PER_CELL * cells / (ms / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(PATH_BYTES_PER_CELL * cells / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["request_bytes"] if traffic else None,
+                     "kernel": "the whole request (census codes, the eight path kernels, border replays, finish)",
+                     "avg_ms": round(ms, 4), "bytes_per_cell": PATH_BYTES_PER_CELL,
+                     "algorithmic_bytes": PATH_BYTES_PER_CELL * cells, "traffic_source": traffic_note},
+        "kernel_roofline": {"kernel": KERNEL_NAMES.get(dom, dom), "stage": dom, "avg_ms": round(avg[dom], 4),
+                            "bytes_per_cell": stage_bytes[dom],
+                            "achieved": round(stage_bytes[dom] * cells / (avg[dom] / 1e3) / 1e9, 1),
+                            "frac": round(stage_bytes[dom] * cells / (avg[dom] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "traffic": (traffic["stages"].get(dom) if traffic else None),
+                            "algorithmic_bytes": stage_bytes[dom] * cells},
+        "design_bytes": {"form": "byte path volumes" if pv else "u16 aggregate", "bytes_per_cell": design_bpc,
+                         "stages": stage_bytes,
+                         "achieved": round(design_bpc * cells / (ms / 1e3) / 1e9, 1),
+                         "frac": round(design_bpc * cells / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "stage_ms": {k: round(v, 4) for k, v in avg.items()},
         "accuracy": {"valid_frac": round(float(v.mean()), 4), "within1_of_truth_on_valid": round(acc, 4)},
         "wall_s": round(t_wall, 3),
         "cpu_baseline": cpu,
+        "cpu_baseline_c0": cpu0,
         "batched": batched,
         "served": served,
         "coarse_to_fine": pyramid,
         "l16_pyramid": l16,
         "l16_pyramid_wide_ranges": l16w,
+        "l16_pyramid_spilling": l16s,
     }
     print(json.dumps(out))
 
