@@ -254,6 +254,37 @@ int lux_sgm_upsample_layer_device(lux_sgm_ctx* ctx, const float* d_prev_depth, c
                                   int32_t ph, const uint8_t* d_guide, int32_t width, int32_t height, int32_t set_w,
                                   int32_t set_h, float sigma, float* d_depth, float* d_conf, void* stream);
 
+/* ---- Layer set-up arithmetic of the L16 path (host, computed once per capture) ---- */
+/* One camera's calibration as Lux.Engine's CalibData holds it (Triangulator.cs:8-19): intrinsics K (3x3, row-major),
+ * translation T, rotation R (3x3, row-major), in the order and layout of its 84-byte record (CalibData.FromBytes). */
+typedef struct lux_sgm_calib {
+    float K[9];
+    float T[3];
+    float R[9];
+} lux_sgm_calib;
+
+/* DenseStereo.Planes (DenseStereo.cs:41-88): the metric plane depths far .. near, uniform in 1/z, their number
+ * n = trunc((1/near - 1/far) * density * B * K0[0]) (B the largest distance of a camera centre from camera 0's, through
+ * Homography.Sqrt), at most 4096, rounded up to `align`; 1/z from rcpss refined by one Newton step in groups of eight,
+ * the remainder by division, the last plane exactly `near`. *n receives n; planes (cap entries, may be NULL for a size
+ * query) the depths. LUX_SGM_EINVAL with the reference's InvalidOperationException messages ("The near end has to be
+ * smaller than the fard end!", "Both the near and far end have to be positive!", "Baseline too small, check
+ * calibration."), readable from lux_sgm_calib_last_error(). The result is the reference's bit for bit. */
+int lux_sgm_planes(float near_z, float far_z, const lux_sgm_calib* calibs, int32_t n_calibs, float density, int32_t align,
+                   float* planes, int32_t cap, int32_t* n);
+/* Mat4D.FlowMatrix (ViewTransform.cs:120-133), the warp DenseStereo.Warp / BuildContext hand the cost
+ * (DenseStereo.cs:180-199): M = (K_b E_b)(K_a E_a)^-1 in double with the reference's operation order (its products and
+ * adjugate inverse), rounded to float, column-major out[16]: the `warp` of a lux_sgm_view. */
+int lux_sgm_flow_matrix(const lux_sgm_calib* a, const lux_sgm_calib* b, float* out);
+/* Homography.Sqrt (Homography.cs:18): x * rsqrtss(x) with one Newton step, 0 for 0. */
+float lux_sgm_homography_sqrt(float x);
+/* Intel's rcpss and rsqrtss as bit functions (IntelApprox.RcpBits / RsqrtBits, IntelApprox.cs:21-44): the tables of the
+ * reference machine and its special cases, the same on every host. */
+uint32_t lux_sgm_rcpss_bits(uint32_t bits);
+uint32_t lux_sgm_rsqrtss_bits(uint32_t bits);
+/* Message of the last failed lux_sgm_planes call on this thread. */
+const char* lux_sgm_calib_last_error(void);
+
 /* The skip mask of a layer (DenseLayer.SkipMask, DenseStereoSgm.cs:64-87): out[H*W] = 0 for computed pixels, 0xff
  * for skipped ones. Mode 0: all computed; mode 2: per 64x64 tile, a generator seeded 5489 picks one pixel of every
  * 2x2 cell; other modes: LUX_SGM_EUNSUPPORTED (the reference's NotSupportedException, :69). Host only. */

@@ -124,6 +124,27 @@ def plane_list(near, far, n):
     return p
 
 
+def _margin(fbs, z_bg, z_fg):
+    return int(max(abs(f) for f in fbs) / min(z_bg, z_fg)) + 8
+
+
+def multiview_calibs(W0, H0, *, fbs=(4000.0, -6000.0, 8000.0, -2000.0), z_bg=2000.0, z_fg=1000.0, margin=None, f=1000.0):
+    """CalibData records (K, T, R) of the rig `multiview` renders: focal length f, no rotation; the reference camera at
+    the origin with its principal point at the image centre, camera k translated by -fbs[k] / f along x with its
+    principal point `margin` further right. Mat4D.FlowMatrix of (reference, camera k) is then the warp `multiview`
+    returns for camera k."""
+    M = _margin(fbs, z_bg, z_fg) if margin is None else margin
+    I = np.eye(3, dtype=np.float32)
+
+    def K(cx):
+        return np.array([[f, 0, cx], [0, f, H0 / 2], [0, 0, 1]], np.float32)
+
+    cams = [(K(W0 / 2), np.zeros(3, np.float32), I)]
+    for fb in fbs:
+        cams.append((K(W0 / 2 + M), np.array([-fb / f, 0, 0], np.float32), I))
+    return cams
+
+
 def multiview(W0, H0, *, fbs=(4000.0, -6000.0, 8000.0, -2000.0), gray=(False, False, True, False), z_bg=2000.0,
               z_fg=1000.0, rects=2, seed=0):
     """A reference camera and len(fbs) rectified cameras looking at a textured fronto-parallel background at z_bg with
@@ -132,7 +153,7 @@ def multiview(W0, H0, *, fbs=(4000.0, -6000.0, 8000.0, -2000.0), gray=(False, Fa
     m0 = m5 = m10 = m15 = 1, m8 = M, m12 = -fb). Images are RGBA8 noise (numpy PCG64, `seed`), not yet box-filtered;
     a gray camera's image repeats channel 0. Returns (images, gray flags incl. the reference, warps [k][16], depth)."""
     rng = np.random.default_rng(seed)
-    M = int(max(abs(f) for f in fbs) / min(z_bg, z_fg)) + 8
+    M = _margin(fbs, z_bg, z_fg)
     Wt = W0 + 2 * M
     bg = rng.integers(0, 256, (H0, Wt, 4), dtype=np.uint8)
     fg = rng.integers(0, 256, (H0, W0, 4), dtype=np.uint8)

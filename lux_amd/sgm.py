@@ -29,7 +29,8 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
            "lux_sgm_server_wait",
-           "lux_sgm_server_cancel", "lux_sgm_server_info_get")
+           "lux_sgm_server_cancel", "lux_sgm_server_info_get", "lux_sgm_planes", "lux_sgm_flow_matrix",
+           "lux_sgm_homography_sqrt", "lux_sgm_rcpss_bits", "lux_sgm_rsqrtss_bits", "lux_sgm_calib_last_error")
 MEM_HOST, MEM_DEVICE = 0, 1
 REQ_PENDING, REQ_RUNNING, REQ_DONE = 0, 1, 2
 OPT_FUSED_BANDS = 1
@@ -130,6 +131,17 @@ def lib():
         L.lux_sgm_upsample_layer_device.argtypes = [vp, vp, vp, i32, i32, vp, i32, i32, i32, i32, ctypes.c_float, vp, vp,
                                                     vp]
         L.lux_sgm_box3_device.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp]
+        L.lux_sgm_planes.argtypes = [ctypes.c_float, ctypes.c_float, vp, i32, ctypes.c_float, i32, vp, i32,
+                                     ctypes.POINTER(i32)]
+        L.lux_sgm_flow_matrix.argtypes = [vp, vp, vp]
+        L.lux_sgm_homography_sqrt.argtypes = [ctypes.c_float]
+        L.lux_sgm_homography_sqrt.restype = ctypes.c_float
+        L.lux_sgm_rcpss_bits.argtypes = [ctypes.c_uint32]
+        L.lux_sgm_rcpss_bits.restype = ctypes.c_uint32
+        L.lux_sgm_rsqrtss_bits.argtypes = [ctypes.c_uint32]
+        L.lux_sgm_rsqrtss_bits.restype = ctypes.c_uint32
+        L.lux_sgm_calib_last_error.argtypes = []
+        L.lux_sgm_calib_last_error.restype = ctypes.c_char_p
         L.lux_sgm_mt19937_draw.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.lux_sgm_mt19937_draw.restype = ctypes.c_uint32
         L.lux_sgm_plan_info.argtypes = [i32, i32, i32, i32, vp, vp, vp]
@@ -522,6 +534,63 @@ def mt19937_draw(seed, n):
     return int(lib().lux_sgm_mt19937_draw(seed, n))
 
 
+class Calib(ctypes.Structure):
+    """One camera's CalibData (Triangulator.cs:8-19): K (3x3 row-major), T, R (3x3 row-major)."""
+    _fields_ = [("K", ctypes.c_float * 9), ("T", ctypes.c_float * 3), ("R", ctypes.c_float * 9)]
+
+
+def calib(K, T, R):
+    c = Calib()
+    c.K[:] = [float(v) for v in np.asarray(K, np.float32).reshape(9)]
+    c.T[:] = [float(v) for v in np.asarray(T, np.float32).reshape(3)]
+    c.R[:] = [float(v) for v in np.asarray(R, np.float32).reshape(9)]
+    return c
+
+
+def _calibs(cs):
+    return (Calib * len(cs))(*[c if isinstance(c, Calib) else calib(*c) for c in cs])
+
+
+def planes(near_z, far_z, calibs, density, align=1):
+    """DenseStereo.Planes (DenseStereo.cs:41-88): float32 plane depths far .. near. calibs: Calib records or (K, T, R)
+    tuples, camera 0 first. Raises SgmError with the reference's message where it throws."""
+    arr = _calibs(calibs)
+    n = ctypes.c_int32(0)
+    L = lib()
+
+    def call(out, cap):
+        rc = L.lux_sgm_planes(float(near_z), float(far_z), arr, len(arr), float(density), int(align), _p(out), cap,
+                              ctypes.byref(n))
+        if rc:
+            raise SgmError(rc, L.lux_sgm_calib_last_error().decode("utf-8", "replace"))
+
+    call(None, 0)
+    out = np.empty(n.value, np.float32)
+    call(out, n.value)
+    return out
+
+
+def flow_matrix(a, b):
+    """Mat4D.FlowMatrix (ViewTransform.cs:120-133): the column-major float32[16] warp from camera a to camera b."""
+    out = np.empty(16, np.float32)
+    ca, cb = _calibs([a]), _calibs([b])
+    _check(lib().lux_sgm_flow_matrix(ca, cb, _p(out)))
+    return out
+
+
+def homography_sqrt(x):
+    """Homography.Sqrt (Homography.cs:18)."""
+    return np.float32(lib().lux_sgm_homography_sqrt(float(np.float32(x))))
+
+
+def rcpss_bits(b):
+    return int(lib().lux_sgm_rcpss_bits(int(b) & 0xFFFFFFFF))
+
+
+def rsqrtss_bits(b):
+    return int(lib().lux_sgm_rsqrtss_bits(int(b) & 0xFFFFFFFF))
+
+
 def box3(ctx, img, bits=None):
     """DenseStereo.BoxFilter3 on the device: uint8/uint16 [H, W] or RGBA8 [H, W, 4]."""
     img = np.ascontiguousarray(img)
@@ -704,6 +773,18 @@ def dense_pyramid(ctx, images, gray, warps, planes8, planes2=None, params=None):
     _check(lib().lux_sgm_dense_pyramid(ctx.handle, ctypes.byref(p), n, ptrs, ws, hs, _p(g), _p(wr), _p(p8), len(p8), _p(p2),
                                        0 if p2 is None else len(p2), dptr, aligns), ctx.handle)
     return depths, list(aligns)
+
+
+def dense_pyramid_run(ctx, images, gray, calibs, near=200.0, far=640000.0, density=2.0, params=None):
+    """DenseStereoPyramid.Run(images, gray, calibs, near, far) (DenseStereoSgm.cs:331-355) with its set-up: the plane
+    lists of align 8 and 2 (DenseStereo.Planes, at the StereoParams plane density, 2 for module type 0 and 4 otherwise)
+    and the warps Mat4D.FlowMatrix(calibs[0], calibs[k]) (DenseStereo.BuildContext, :189-199), all from the library's
+    host functions. near / far default to DenseStereo.Near / Far. Returns (depths, aligns, planes8, planes2, warps)."""
+    p8 = planes(near, far, calibs, density, 8)
+    p2 = planes(near, far, calibs, density, 2)
+    warps = [flow_matrix(calibs[0], c) for c in calibs[1:]]
+    depths, aligns = dense_pyramid(ctx, images, gray, warps, p8, p2, params)
+    return depths, aligns, p8, p2, warps
 
 
 def dense_pyramid_device(ctx, d_images, shapes, gray, warps, planes8, planes2, d_depths, params=None):

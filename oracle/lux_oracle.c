@@ -478,7 +478,7 @@ int lo_rectified(const void* left, const void* right, int img16, int bits, int W
 
 
 /* ---------------------------------------------------------------- DenseUpsampleLayer */
-uint32_t lo_rcp_bits(uint32_t b) /* IntelApprox.cs:20-28 */
+uint32_t lo_rcp_bits(uint32_t b) /* IntelApprox.cs:21-29 */
 {
     const uint32_t sign = b & 0x80000000u, e = (b >> 23) & 0xffu, m = b & 0x7fffffu;
     if (e == 0xffu) return m ? (b | 0x400000u) : sign;
@@ -941,4 +941,160 @@ int lo_skip_mask(int mode, int w, int h, uint8_t* out)
                 }
         }
     return 0;
+}
+
+/* ---- layer set-up arithmetic of the L16 path (host, once per capture) ---- */
+#include "rsqrt_table.h"
+
+uint32_t lo_rsqrt_bits(uint32_t b) /* IntelApprox.cs:32-44 */
+{
+    const uint32_t sign = b & 0x80000000u, e = (b >> 23) & 0xffu, m = b & 0x7fffffu;
+    if (e == 0xffu && m != 0) return b | 0x400000u; /* NaN -> quiet NaN */
+    if (e == 0) return sign | 0x7f800000u;          /* +-0, denormals -> +-inf */
+    if (sign != 0) return 0xffc00000u;               /* negative -> indefinite QNaN */
+    if (e == 0xffu) return 0;                        /* +inf -> +0 */
+    const int k = (int)e - 127;
+    uint32_t r;
+    int re;
+    if ((k & 1) == 0) { r = kRsqrtOddTable[m >> 11]; re = (int)((r >> 23) & 0xffu) - k / 2; }
+    else { r = kRsqrtEvenTable[m >> 11]; re = (int)((r >> 23) & 0xffu) - (k - 1) / 2; }
+    if (re <= 0) return 0;
+    return ((uint32_t)re << 23) | (r & 0x7fffffu);
+}
+
+uint32_t lo_rsqrtss_hw(uint32_t b)
+{
+    float x = f_of_bits(b);
+    return bits_of_f(_mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x))));
+}
+
+float lo_homography_sqrt(float x) /* Homography.cs:18 */
+{
+    float r = f_of_bits(lo_rsqrt_bits(bits_of_f(x)));
+    float A = x * r, B = A * (-0.5f), C = A * r, D = C + (-3.0f), s = D * B;
+    return x == 0.0f ? 0.0f : s;
+}
+
+static float lo_rcp_nr(float a) /* DenseStereo.cs:36-37 */
+{
+    float x0 = lo_rcp(a);
+    return x0 + x0 * (1.0f - a * x0);
+}
+
+int lo_planes(float near_z, float far_z, const float* calibs, int ncal, float density, int align, float* planes, int cap,
+              int* n_out) /* DenseStereo.cs:41-88; calibs: ncal records of K[9], T[3], R[9] */
+{
+    if (!(near_z < far_z)) return -1;
+    if (!(near_z > 0.0f && far_z > 0.0f)) return -2;
+    if (ncal <= 0 || align <= 0) return -4;
+    float C[64][3];
+    if (ncal > 64) return -4;
+    for (int k = 0; k < ncal; k++) {
+        const float* R = calibs + 21 * k + 12;
+        const float* t = calibs + 21 * k + 9;
+        C[k][0] = (t[0] * R[0] + t[1] * R[3]) + t[2] * R[6];
+        C[k][1] = (t[0] * R[1] + t[1] * R[4]) + t[2] * R[7];
+        C[k][2] = (t[0] * R[2] + t[1] * R[5]) + t[2] * R[8];
+    }
+    float B = 0.0f;
+    for (int k = 0; k < ncal; k++) {
+        float dx = C[0][0] - C[k][0], dy = C[0][1] - C[k][1], dz = C[0][2] - C[k][2];
+        float d2 = dz * dz + (dx * dx + dy * dy);
+        float d = lo_homography_sqrt(d2);
+        if (!(d <= B)) B = d;
+    }
+    float x1_0 = lo_rcp_nr(near_z), x1_1 = lo_rcp_nr(far_z);
+    float inv = x1_0 - x1_1;
+    int n = f2i_sat(((inv * density) * B) * calibs[0]); /* the runtime's saturating (int) */
+    if (n > 0x1000) n = 0x1000;
+    if (n <= 0) return -3;
+    n = ((n + align - 1) / align) * align;
+    *n_out = n;
+    if (!planes) return 0;
+    if (cap < n) return -4;
+    float step = (x1_0 - x1_1) / (float)(n - 1), bas = x1_1;
+    if (n - 1 <= 7) {
+        float v = bas;
+        for (int i = 0; i < n - 1; i++) { planes[i] = 1.0f / v; v += step; }
+    } else {
+        int m = (n - 1) & ~7;
+        float v4[4], d4 = 4.0f * step;
+        for (int l = 0; l < 4; l++) v4[l] = step * (float)l + bas;
+        for (int i = 0; i < m; i += 8) {
+            float hi[4];
+            for (int l = 0; l < 4; l++) hi[l] = v4[l] + d4;
+            for (int l = 0; l < 4; l++) { planes[i + l] = lo_rcp_nr(v4[l]); planes[i + 4 + l] = lo_rcp_nr(hi[l]); }
+            for (int l = 0; l < 4; l++) v4[l] = hi[l] + d4;
+        }
+        float v = (float)m * step + bas;
+        for (int i = m; i < n - 1; i++) { planes[i] = 1.0f / v; v += step; }
+    }
+    planes[n - 1] = near_z;
+    return 0;
+}
+
+static void lo_mul4(const double* a, const double* b, double* o) /* ViewTransform.cs:88-94 */
+{
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            o[4 * i + j] = ((a[4 * i + 3] * b[12 + j] + a[4 * i + 2] * b[8 + j]) + (a[4 * i + 1] * b[4 + j] + a[4 * i] * b[j]));
+}
+
+static void lo_inverse4(const double* m, double* o) /* ViewTransform.cs:96-118 */
+{
+    double m00 = m[0], m01 = m[1], m02 = m[2], m03 = m[3], m10 = m[4], m11 = m[5], m12 = m[6], m13 = m[7], m20 = m[8], m21 = m[9],
+           m22 = m[10], m23 = m[11], m30 = m[12], m31 = m[13], m32 = m[14], m33 = m[15];
+    double adj[16];
+    adj[0] = (m11 * (m22 * m33 - m23 * m32)) + ((m12 * (m23 * m31 - m33 * m21)) + (m13 * (m32 * m21 - m31 * m22)));
+    adj[1] = (m01 * (m23 * m32 - m22 * m33)) + ((m02 * (m33 * m21 - m23 * m31)) + (m03 * (m31 * m22 - m32 * m21)));
+    adj[2] = (((m02 * m13 - m12 * m03) * m31) + ((m12 * m01) * m33 + (m11 * m03) * m32)) - ((m01 * m13) * m32 + (m11 * m02) * m33);
+    adj[3] = ((m23 * (m11 * m02 - m12 * m01)) + (m21 * (m12 * m03 - m02 * m13))) + (m22 * (m01 * m13 - m11 * m03));
+    adj[4] = ((m12 * (m33 * m20 - m23 * m30)) + (m13 * (m22 * m30 - m32 * m20))) + (m10 * (m23 * m32 - m22 * m33));
+    adj[5] = (m00 * (m22 * m33 - m23 * m32)) + ((m02 * (m23 * m30 - m33 * m20)) + (m03 * (m32 * m20 - m22 * m30)));
+    adj[6] = (m33 * (m02 * m10 - m00 * m12)) + ((m30 * (m12 * m03 - m02 * m13)) + (m32 * (m00 * m13 - m03 * m10)));
+    adj[7] = (((m02 * m13 - m12 * m03) * m20) + ((m03 * m10) * m22 + (m00 * m12) * m23)) - (m23 * (m02 * m10) + (m00 * m13) * m22);
+    adj[8] = (m11 * (m23 * m30 - m33 * m20)) + ((m13 * (m31 * m20 - m21 * m30)) + (m10 * (m33 * m21 - m23 * m31)));
+    adj[9] = (m00 * (m23 * m31 - m33 * m21)) + (((m33 * m20 - m23 * m30) * m01) + (m03 * (m21 * m30 - m31 * m20)));
+    adj[10] = (((m01 * m13 - m11 * m03) * m30) + ((m00 * m11) * m33 + (m03 * m10) * m31)) - ((m00 * m13) * m31 + m33 * (m01 * m10));
+    adj[11] = (m23 * (m01 * m10 - m00 * m11)) + ((m20 * (m11 * m03 - m01 * m13)) + (m21 * (m00 * m13 - m03 * m10)));
+    adj[12] = (m11 * (m32 * m20 - m22 * m30)) + ((m12 * (m21 * m30 - m31 * m20)) + (m10 * (m31 * m22 - m32 * m21)));
+    adj[13] = (m00 * (m32 * m21 - m31 * m22)) + (((m22 * m30 - m32 * m20) * m01) + (m02 * (m31 * m20 - m21 * m30)));
+    adj[14] = (m32 * (m01 * m10 - m00 * m11)) + (((m11 * m02 - m12 * m01) * m30) + (m31 * (m00 * m12 - m02 * m10)));
+    adj[15] = (((m12 * m01 - m11 * m02) * m20) + ((m02 * m10) * m21 + (m00 * m11) * m22)) - ((m01 * m10) * m22 + (m00 * m12) * m21);
+    double det = ((adj[12] * m03 + m02 * adj[8]) + (adj[4] * m01 + m00 * adj[0]));
+    double rd = 1.0 / det;
+    for (int k = 0; k < 16; k++) o[k] = adj[k] * rd;
+}
+
+void lo_flow_matrix(const float* a, const float* b, float* out) /* ViewTransform.cs:120-133; records K[9], T[3], R[9] */
+{
+    double Ka[16] = {0}, Ea[16] = {0}, Kb[16] = {0}, Eb[16] = {0}, P1[16], P2[16], I2[16], M[16];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Ka[4 * i + j] = a[3 * i + j];
+            Kb[4 * i + j] = b[3 * i + j];
+            Ea[4 * i + j] = a[12 + 3 * i + j];
+            Eb[4 * i + j] = b[12 + 3 * i + j];
+        }
+    for (int i = 0; i < 3; i++) { Ea[4 * i + 3] = a[9 + i]; Eb[4 * i + 3] = b[9 + i]; }
+    Ka[15] = Kb[15] = Ea[15] = Eb[15] = 1.0;
+    lo_mul4(Kb, Eb, P1);
+    lo_mul4(Ka, Ea, P2);
+    lo_inverse4(P2, I2);
+    lo_mul4(P1, I2, M);
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) out[4 * c + r] = (float)M[4 * r + c];
+}
+
+uint64_t lo_approx_mismatches(int which, uint64_t lo, uint64_t hi) /* test helper: model vs the local instruction */
+{
+    uint64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int64_t i = (int64_t)lo; i < (int64_t)hi; i++) {
+        const uint32_t b = (uint32_t)i;
+        const uint32_t want = which ? lo_rsqrtss_hw(b) : lo_rcpss_hw(b);
+        const uint32_t got = which ? lo_rsqrt_bits(b) : lo_rcp_bits(b);
+        bad += want != got;
+    }
+    return bad;
 }

@@ -10,7 +10,7 @@
  *   Lux.Engine/Pipeline/Registration/DenseStereoSgm.cs   also SkipMask (:64-87), RangeFromPrevious (:91-140),
  *                                                        the layer's depth (:225-231)
  *   Lux.Engine/Pipeline/Registration/DenseUpsampleLayer.cs  ImageShift, NearestResize, BilateralUpsampleFromCollapse
- *   Lux.Engine/Pipeline/Registration/IntelApprox.cs      rcpss (through the table read from the instruction)
+ *   Lux.Engine/Pipeline/Registration/IntelApprox.cs      rcpss, rsqrtss (through the tables read from the instructions)
  *   Lux.Engine/Pipeline/Parallax/RealStereo.cs:229-233,418-421  two-direction check pattern (L/R check)
  *   Lux.Engine/Pipeline/DepthRefine/CrossViewCost.cs:197-202    parabola vertex (subpixel)
  * Census and the truncated-AD cost of a rectified pair are project-defined front ends (the reference
@@ -100,12 +100,26 @@ int lo_rectified(const void* left, const void* right, int img16, int bits, int W
                  int threads, uint16_t* disp, float* sub, uint8_t* valid, uint16_t* agg, double* times_ms);
 
 /* ---- DenseUpsampleLayer (Lux.Engine/Pipeline/Registration/DenseUpsampleLayer.cs) ---- */
-/* rcpss / rcpps as an Intel CPU computes them (IntelApprox.cs:20-28): a 4096-entry table on the top 12 mantissa
+/* rcpss / rcpps as an Intel CPU computes them (IntelApprox.cs:21-29): a 4096-entry table on the top 12 mantissa
  * bits scaled by the input's power of two; +-0 and denormals -> +-inf, +-inf -> +-0, NaN -> quiet NaN, a denormal
  * result -> +-0. */
 uint32_t lo_rcp_bits(uint32_t b);
 /* The local CPU's own rcpss (for pinning lo_rcp_bits on an Intel host). */
 uint32_t lo_rcpss_hw(uint32_t b);
+/* IntelApprox.RsqrtBits (IntelApprox.cs:32-44) and the local CPU's own rsqrtss (for pinning it on an Intel host). */
+uint32_t lo_rsqrt_bits(uint32_t b);
+uint32_t lo_rsqrtss_hw(uint32_t b);
+/* Test helper: how many bit patterns b in [lo, hi) the model (which 0: rcp, 1: rsqrt) maps differently from the local
+ * CPU's instruction. */
+uint64_t lo_approx_mismatches(int which, uint64_t lo, uint64_t hi);
+/* Homography.Sqrt (Homography.cs:18). */
+float lo_homography_sqrt(float x);
+/* DenseStereo.Planes (DenseStereo.cs:41-88); calibs: ncal records of K[9], T[3], R[9] floats. Returns 0, or -1 near >=
+ * far, -2 a non-positive end, -3 baseline too small, -4 bad arguments; *n_out = the plane count. */
+int lo_planes(float near_z, float far_z, const float* calibs, int ncal, float density, int align, float* planes, int cap,
+              int* n_out);
+/* Mat4D.FlowMatrix (ViewTransform.cs:120-133): column-major float out[16] of (K_b E_b)(K_a E_a)^-1. */
+void lo_flow_matrix(const float* a, const float* b, float* out);
 /* ImageShift<2, vec4x8ui>(src, {sx, sy}) (:64-104): RGBA8 W x H, separable 4-tap cubic, clamp-to-edge, RNE, saturate. */
 int lo_image_shift(const uint8_t* src, int W, int H, float sx, float sy, uint8_t* dst);
 /* NearestResize (:108-128): 16.16 steps, RGBA8 sw x sh -> W x H. */

@@ -219,6 +219,13 @@ def _upsample_lib():
         L.lo_nearest_resize.argtypes = [vp, i32, i32, i32, i32, vp]
         L.lo_bilateral_upsample.argtypes = [vp, i32, i32, vp, i32, i32, vp, f32, vp]
         L.lo_upsample_layer.argtypes = [vp, i32, i32, vp, i32, i32, i32, i32, f32, vp]
+        L.lo_rsqrt_bits.argtypes, L.lo_rsqrt_bits.restype = [u32], u32
+        L.lo_rsqrtss_hw.argtypes, L.lo_rsqrtss_hw.restype = [u32], u32
+        L.lo_approx_mismatches.argtypes, L.lo_approx_mismatches.restype = [i32, ctypes.c_uint64, ctypes.c_uint64], \
+            ctypes.c_uint64
+        L.lo_homography_sqrt.argtypes, L.lo_homography_sqrt.restype = [f32], f32
+        L.lo_planes.argtypes, L.lo_planes.restype = [f32, f32, vp, i32, f32, i32, vp, i32, vp], i32
+        L.lo_flow_matrix.argtypes = [vp, vp, vp]
         L._ups = True
     return L
 
@@ -229,6 +236,63 @@ def rcp_bits(b):
 
 def rcpss_hw(b):
     return int(_upsample_lib().lo_rcpss_hw(int(b) & 0xFFFFFFFF))
+
+
+def rsqrt_bits(b):
+    return int(_upsample_lib().lo_rsqrt_bits(int(b) & 0xFFFFFFFF))
+
+
+def rsqrtss_hw(b):
+    return int(_upsample_lib().lo_rsqrtss_hw(int(b) & 0xFFFFFFFF))
+
+
+def approx_mismatches(which, lo, hi):
+    """How many bit patterns in [lo, hi) the rcpss (which="rcp") / rsqrtss ("rsqrt") model maps differently from the
+    local CPU's instruction."""
+    return int(_upsample_lib().lo_approx_mismatches(1 if which == "rsqrt" else 0, int(lo), int(hi)))
+
+
+def homography_sqrt(x):
+    return np.float32(_upsample_lib().lo_homography_sqrt(float(np.float32(x))))
+
+
+def _calib_records(calibs):
+    """calibs: sequence of (K 3x3, T 3, R 3x3) -> float32 [n, 21] records K, T, R (row-major)."""
+    rec = np.empty((len(calibs), 21), np.float32)
+    for k, (K, T, R) in enumerate(calibs):
+        rec[k, :9] = np.asarray(K, np.float32).reshape(9)
+        rec[k, 9:12] = np.asarray(T, np.float32).reshape(3)
+        rec[k, 12:] = np.asarray(R, np.float32).reshape(9)
+    return rec
+
+
+PLANES_ERRORS = {-1: "near >= far", -2: "non-positive depth", -3: "baseline too small", -4: "bad arguments"}
+
+
+def planes(near_z, far_z, calibs, density, align=1):
+    """DenseStereo.Planes (DenseStereo.cs:41-88): the float32 plane depths, far .. near. Raises ValueError with the
+    reason where the reference throws."""
+    rec = _calib_records(calibs)
+    n = ctypes.c_int(0)
+    L = _upsample_lib()
+    rc = L.lo_planes(float(near_z), float(far_z), _p(rec), len(rec), float(density), int(align), None, 0,
+                     ctypes.byref(n))
+    if rc:
+        raise ValueError(PLANES_ERRORS[rc])
+    out = np.empty(n.value, np.float32)
+    rc = L.lo_planes(float(near_z), float(far_z), _p(rec), len(rec), float(density), int(align), _p(out), n.value,
+                     ctypes.byref(n))
+    if rc:
+        raise ValueError(PLANES_ERRORS[rc])
+    return out
+
+
+def flow_matrix(a, b):
+    """Mat4D.FlowMatrix (ViewTransform.cs:120-133) of calibrations a, b ((K, T, R) each): column-major float32[16]."""
+    ra, rb = _calib_records([a]), _calib_records([b])
+    out = np.empty(16, np.float32)
+    _upsample_lib().lo_flow_matrix(_p(ra), _p(rb), _p(out))
+    return out
 
 
 def image_shift(img, sx, sy):

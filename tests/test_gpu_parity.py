@@ -656,6 +656,27 @@ def test_dense_pyramid_spilling_padding(W0, H0, seed, near, far):
         np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
 
 
+@pytest.mark.parametrize("W0,H0,seed,near,far,density", [(256, 192, 1, 500.0, 20000.0, 3.0),
+                                                         (208, 152, 2, 1000.0, 2200.0, 10.0)])
+def test_dense_pyramid_from_calibrations(W0, H0, seed, near, far, density):
+    """DenseStereoPyramid.Run from calibrations: plane lists and warps from the library's DenseStereo.Planes and
+    Mat4D.FlowMatrix equal the oracle's bit for bit, and six levels of depth computed from them equal the oracle's."""
+    imgs, gray, _, _ = scene.multiview(W0, H0, seed=seed)
+    cams = scene.multiview_calibs(W0, H0)
+    p8, p2 = o.planes(near, far, cams, density, 8), o.planes(near, far, cams, density, 2)
+    warps = [o.flow_matrix(cams[0], c) for c in cams[1:]]
+    want = o.dense_pyramid(imgs, gray, warps, p8, p2, gw=sgm.L16_GUIDANCE)
+    print(f"dense pyramid from calibrations {W0}x{H0} planes {near}..{far} ({len(p8)}/{len(p2)})", flush=True)
+    with sgm.Context(0) as c:
+        depths, aligns, g8, g2, gw = sgm.dense_pyramid_run(c, imgs, gray, cams, near, far, density)
+    np.testing.assert_array_equal(g8.view(np.uint32), p8.view(np.uint32))
+    np.testing.assert_array_equal(g2.view(np.uint32), p2.view(np.uint32))
+    np.testing.assert_array_equal(np.asarray(gw).view(np.uint32), np.asarray(warps).view(np.uint32))
+    assert aligns == [lv["align"] for lv in want]
+    for i, (d, lv) in enumerate(zip(depths, want)):
+        np.testing.assert_array_equal(d.view(np.uint32), lv["depth"].view(np.uint32), err_msg=f"level {i}")
+
+
 def test_dense_pyramid_wide_ranges():
     """304 planes over a narrow depth span: ranges up to 288 planes at the depth edges (64-plane chunks)."""
     imgs, gray, warps, _ = scene.multiview(256, 192, seed=7)
