@@ -329,8 +329,11 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     w->b.ap = AdaptP2();
     w->b.ap.guid = (const uint32_t*)sync;
     if (pv8) {
+        // seven volumes one after another (a per-cell interleave of the seven, 7 * cs bytes a cell, measured the
+        // same for the last row path and slower for the sweeps)
         const size_t vol = (size_t)g.W * g.H * g.cs;
         for (int k = 0; k < 7; k++) w->b.pv[k] = (uint8_t*)ptr["path_bytes"] + k * vol;
+        w->b.pvs = g.cs;
         w->b.bagg[0] = (uint16_t*)ptr["border_agg"];
         w->b.bagg[1] = w->b.bagg[0] + (size_t)g.H * g.cs;
     }

@@ -66,11 +66,13 @@ struct SgmBuffers {
     // Path volumes [H][W][cs] for latency-bound (ranged) layers (set all or none): each pass then runs as one
     // launch, pass +1's paths 1-3 into the volumes, pass -1's added to them, all summed with agg once.
     uint16_t* aggp[3] = {};
-    // Byte path volumes [H][W][cs] (set all seven or none; uniform constant-P2 census requests whose every path cost
-    // fits a byte, PathBytes in engine.hip): each path's costs go to a volume of its own, and pass -1's row path sums
+    // Byte path volumes, cell c of volume k at pv[k] + c * pvs (set all seven or none; uniform constant-P2 census
+    // requests whose every path cost fits a byte, PathBytes in engine.hip): each path's costs go to a volume of its
+    // own, and pass -1's row path sums
     // them with its own into the final aggregate, which it then writes only when asked (agg may be null). bagg holds
     // the two border columns' replayed costs, [2][H][cs] (x = W-1 of pass +1, x = 0 of pass -1).
     uint8_t* pv[7] = {};
+    int pvs = 0;                    // byte stride between consecutive cells of a path volume (7 * cs: interleaved)
     uint16_t* bagg[2] = {};
     int zero_flags = 0;             // clear the band flags by a kernel before each band launch (graph replays)
     // The request has the device to itself: each chain's wavefront is a workgroup of its own, so the chains spread

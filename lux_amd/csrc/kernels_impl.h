@@ -193,6 +193,49 @@ __device__ __forceinline__ void add_bytes(const RawW<NP>& w, Cell<NP>& acc)
     for (int k = 0; k < NP; k++) acc.v[k] += t.v[k];  // no carry between the halves: sums stay far below 65536
 }
 
+// Branch-free stores for the step loops. A store inside a condition (a lane mask, a border column, a flush every
+// 64 cells) splits the loop body, and the compiler's wait counting then has to assume the store may be missing on
+// one side: it waits for prefetched loads several steps early (s_waitcnt vmcnt far below the loads in flight). These
+// stores always issue, as raw buffer stores: a store that should not happen gets an offset past the buffer's size,
+// and the hardware's range check drops it.
+constexpr int kRsrcWord3 = 0x00020000;      // raw buffer, 32-bit data format (the gfx9 resource word 3)
+constexpr uint32_t kDrop = 0x80000000u;     // an offset past every buffer these stores address
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ void bstore_u16(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bstore_u32(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bstore_f32(float v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+}
+// A cell's NP packed pairs (u16 planes) at byte offset off (or dropped).
+template <int NP>
+__device__ __forceinline__ void bstore_cell(const Cell<NP>& c, __amdgpu_buffer_rsrc_t r, uint32_t off, bool on)
+{
+#pragma unroll
+    for (int k = 0; k < NP; k++) bstore_u32(c.v[k], r, on ? off + 4u * k : kDrop);
+}
+// A cell's planes as bytes (the layout of store_bytes) at byte offset off (or dropped).
+template <int NP>
+__device__ __forceinline__ void bstore_bytes(const Cell<NP>& c, __amdgpu_buffer_rsrc_t r, uint32_t off, bool on)
+{
+    if constexpr (NP == 1) {
+        bstore_u16(__builtin_amdgcn_perm(0u, c.v[0], 0x0c0c0200u), r, on ? off : kDrop);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NP / 2; k++)
+            bstore_u32(__builtin_amdgcn_perm(c.v[2 * k + 1], c.v[2 * k], 0x06040200u), r, on ? off + 4u * k : kDrop);
+    }
+}
+
 // One path step of DenseLayer.Pixel (:276-314) for the planes this lane owns.
 //   m  = min(min(sat(Lm+P1), L0), min(sat(Lp+P1), P2p)),  P2p = min(P2 + minPrev, 65535)
 //   lc = satsub(sat(raw + m), minPrev)
@@ -450,7 +493,8 @@ struct SweepArgs {
     uint16_t* seedcap;
     AdaptP2 ap;
     uint32_t* seedguid;      // [3][W] (adaptive P2)
-    uint8_t* pv = nullptr;   // byte path volume [H][W][cs] of this path (PV sweeps)
+    uint8_t* pv = nullptr;   // byte path volume of this path (PV sweeps): cell c's planes at pv + c * pvs
+    int pvs = 0;
 };
 
 // Per-cell words of a chain (the cell's range, its guidance), 64 chain positions in one VGPR (lane k: position
@@ -542,7 +586,7 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
             for (int k = 0; k < NP; k++) z.v[k] = 0;
             for (int i = 0; i < n; i++) {
                 const size_t c = (size_t)(y + i * DY) * W + (x + i * DX);
-                if constexpr (PV) store_zero_bytes<NP>(a.pv + c * cs + lo, lo < cs);
+                if constexpr (PV) store_zero_bytes<NP>(a.pv + c * a.pvs + lo, lo < cs);
                 else store_u16<NP>(a.agg + c * cs + lo, lo < cs, z);
             }
         }
@@ -599,6 +643,10 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
         ld_raw_w<NP>(a.raw + cj * cs + lol, rb[j]);
         if (!FIRST && !PV) ld_u16<NP>(a.agg + cj * cs + lol, ab[j]);
     }
+    Cell<NP> seedv, colv;
+    uint32_t seedm = 0, colm = 0, sgd = 0;
+    int xs = 0, yc = 0, scs = 0, scc = 0;
+    bool has_seed = false, has_col = false;
     auto body = [&](int i, int j) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
@@ -621,37 +669,41 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
         if constexpr (ADP) gq.advance(i, gq_at, lane, a.ap, a.p2);
         Cell<NP> lc;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
-        if (x != bx) {
-            if constexpr (PV) {
-                store_bytes<NP>(a.pv + cell * cs + lo, active, lc);
-            } else {
-                Cell<NP> out;
+        const bool bord = x == bx;
+        if constexpr (PV) {
+            // every step stores (the border column its zeros): no memory access of the step loop depends on a
+            // condition, which would cost the prefetches their early issue (see bstore_*)
+            Cell<NP> out;
 #pragma unroll
-                for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
-                store_u16<NP>(a.agg + cell * cs + lo, active, out);
-            }
-            if (DIR > 0 && P != 0 && y == a.seed_row) {
-                store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + x) * cs + lo, lo < cs, lc);
-                if (lane == 0) {
-                    a.seedmins[(P - 1) * W + x] = (uint16_t)minprev;
-                    if (VAR) {
-                        a.seedstart[(P - 1) * W + x] = (uint16_t)cs_;
-                        a.seedcap[(P - 1) * W + x] = (uint16_t)cc_;
-                    }
-                    if (ADP) a.seedguid[(P - 1) * W + x] = gc;
-                }
-            }
-        } else if constexpr (PV) {
-            store_zero_bytes<NP>(a.pv + cell * cs + lo, active);
+            for (int k = 0; k < NP; k++) out.v[k] = bord ? 0u : lc.v[k];
+            store_bytes<NP>(a.pv + cell * a.pvs + lo, MASKED ? active : true, out);
+        } else if (!bord) {
+            Cell<NP> out;
+#pragma unroll
+            for (int k = 0; k < NP; k++) out.v[k] = FIRST ? lc.v[k] : padd(agg.v[k], lc.v[k]);
+            store_u16<NP>(a.agg + cell * cs + lo, active, out);
         } else if (FIRST) {
             Cell<NP> z;
 #pragma unroll
             for (int k = 0; k < NP; k++) z.v[k] = 0;
             store_u16<NP>(a.agg + cell * cs + lo, active, z);
         }
-        if (P <= 1 && x == bx - DIR) {  // the border replay reads this column's path 0 / 1 states
-            store_u16<NP>(a.cols + ((size_t)P * H + y) * cs + lo, lo < cs, lc);
-            if (lane == 0) a.colmins[P * H + y] = (uint16_t)minprev;
+        // the seed-table entry (pass +1 in the seed row) and the border replay's column state (path 0 / 1 next to
+        // the border column): a chain meets each at most once, so they are kept and stored after the loop
+        if (DIR > 0 && P != 0 && y == a.seed_row && !bord) {
+            seedv = lc;
+            seedm = minprev;
+            xs = x;
+            scs = cs_;
+            scc = cc_;
+            sgd = gc;
+            has_seed = true;
+        }
+        if (P <= 1 && x == bx - DIR) {
+            colv = lc;
+            colm = minprev;
+            yc = y;
+            has_col = true;
         }
         state = lc;
         cell += step;
@@ -666,18 +718,34 @@ __device__ __forceinline__ void sweep_wave(const SweepArgs& a, int chain, int la
 #pragma unroll
     for (int j = 0; j < PF; j++)
         if (i0 + j < n) body(i0 + j, j);
+    if (has_seed) {
+        store_u16<NP>(a.seeds + ((size_t)(P - 1) * W + xs) * cs + lo, lo < cs, seedv);
+        if (lane == 0) {
+            a.seedmins[(P - 1) * W + xs] = (uint16_t)seedm;
+            if (VAR) {
+                a.seedstart[(P - 1) * W + xs] = (uint16_t)scs;
+                a.seedcap[(P - 1) * W + xs] = (uint16_t)scc;
+            }
+            if (ADP) a.seedguid[(P - 1) * W + xs] = sgd;
+        }
+    }
+    if (has_col) {  // the border replay reads this column's path 0 / 1 states
+        store_u16<NP>(a.cols + ((size_t)P * H + yc) * cs + lo, lo < cs, colv);
+        if (lane == 0) a.colmins[P * H + yc] = (uint16_t)colm;
+    }
 }
 template <int NP, int DIR, int P, bool FIRST, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 {
-    sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
+    sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
 // The sweep of a request with byte path volumes: its costs into its own volume a.pv.
 template <int NP, int DIR, int P, bool MASKED>
 __global__ __launch_bounds__(kBlock) void sgm_sweep_pv(SweepArgs a)
 {
-    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
-                                                              threadIdx.x & 63);
+    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
@@ -701,8 +769,9 @@ struct RowArgs {
     const uint16_t* vcount;
     AdaptP2 ap;
     // byte path volumes (PV): pv[k] is path k of the eight in launch order (-> of pass +1, its three sweeps, pass -1's
-    // three sweeps), each [H][W][cs]; the row path of pass +1 writes pv[0], the one of pass -1 sums them all
+    // three sweeps), cell c at pv[k] + c * pvs; the row path of pass +1 writes pv[0], the one of pass -1 sums them all
     uint8_t* pv[7] = {};
+    int pvs = 0;            // byte stride between consecutive cells of a path volume
     const uint16_t* bagg_plus = nullptr;   // [H][cs] pass +1's border-column (x = W-1) costs from its replay
     const uint16_t* bagg_minus = nullptr;  // [H][cs] pass -1's border-column (x = 0) costs from its replay
 };
@@ -849,7 +918,7 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
         Cell<NP> out;
 #pragma unroll
         for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
-        if constexpr (PV) store_bytes<NP>(a.pv[0] + (rowcell + x) * cs + lo, active, out);
+        if constexpr (PV) store_bytes<NP>(a.pv[0] + (rowcell + x) * a.pvs + lo, active, out);
         else store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
         if (x == W - 2) {
             store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, lc);
@@ -894,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void sgm_row_fwd_pv(RowArgs a)
 // rest of the sum and the winner-take-all are sgm_finish's). PV: the final aggregate is the sum of the seven byte
 // path volumes, this path's cost and, in column W-1, pass +1's border costs (bagg_plus); agg is only written (when
 // asked for), never read.
-template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA, bool PV = false>
+template <int NP, bool MASKED, bool VAR, bool ADP, bool WTA, bool PV = false, int WAGG = -1>
 __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 {
     static_assert(!PV || (WTA && !VAR && !ADP), "byte path volumes: uniform constant-P2 requests");
@@ -917,6 +986,15 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     Cell<NP> ab[PF];
     RawW<NP> vb[PF][NV > 0 ? NV : 1];  // PV: the seven path volumes' bytes of the cells in flight
     Range from;
+    // The step loop's stores do not depend on a condition (which would cost the prefetches their early issue, as
+    // bstore_* explains): the aggregate is stored by every step or by none (WAGG, or a.write_agg: one branch for the
+    // whole row), and the winner-take-all outputs leave once per block of 64 steps, after the block's loop.
+    const bool wagg = WAGG >= 0 ? WAGG != 0 : (!WTA || a.write_agg);
+    const __amdgpu_buffer_rsrc_t ragg = make_rsrc(wagg ? a.agg + rowcell * cs : nullptr, wagg ? (uint32_t)(W * cs * 2) : 0u);
+    const __amdgpu_buffer_rsrc_t rsub = make_rsrc(WTA && a.sub ? a.sub + rowcell : nullptr, WTA && a.sub ? (uint32_t)(W * 4) : 0u);
+    // PV: pass +1's border-column costs of this row, added at x = W-1 (loaded ahead of the loop)
+    Cell<NP> eplus;
+    if constexpr (PV) load_u16<NP>(a.bagg_plus + (size_t)y * cs + lo, lo < cs, eplus);
     // VAR / ADP: range (start | cap << 16), count and guidance words by position W-1-x, 64 a VGPR (ChainWords)
     const auto cell_at = [&](int pos) { return rowcell + max(W - 1 - pos, 0); };
     const auto rs_at = [&](int pos) {
@@ -938,7 +1016,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
         if constexpr (PV) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xj) * cs + lol, vb[j][v]);
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xj) * a.pvs + lol, vb[j][v]);
         } else {
             ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
         }
@@ -948,7 +1026,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     uint32_t myd = 0, myr = 0;
     int myl = 0, myc = 0, myrr = 0;
     bool mypar = false;
-    auto body = [&](int x, int j) {
+    auto body = [&](int x, int j, int xb) {
         Cell<NP> raw;
         widen<NP>(rb[j], raw);
         Cell<NP> agg;
@@ -958,10 +1036,8 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 #pragma unroll
             for (int v = 0; v < NV; v++) add_bytes<NP>(vb[j][v], agg);
             if (x == W - 1) {
-                Cell<NP> e;
-                load_u16<NP>(a.bagg_plus + (size_t)y * cs + lo, lo < cs, e);
 #pragma unroll
-                for (int k = 0; k < NP; k++) agg.v[k] += e.v[k];
+                for (int k = 0; k < NP; k++) agg.v[k] += eplus.v[k];
             }
         } else {
             agg = ab[j];
@@ -986,20 +1062,19 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
         if constexpr (PV) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xp) * cs + lol, vb[j][v]);
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xp) * a.pvs + lol, vb[j][v]);
         } else {
             ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
         }
         Cell<NP> lc, fin;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
-        if (x == 1) {
-            store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, lc);
-            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
-        }
-        state = lc;
+        state = lc;  // at x = 1, the column-1 state the border replay reads (stored after the loop)
 #pragma unroll
         for (int k = 0; k < NP; k++) fin.v[k] = padd(agg.v[k], lc.v[k]);
-        if (!WTA || a.write_agg) store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, fin);
+        if constexpr (WAGG == 1 || !WTA) bstore_cell<NP>(fin, ragg, (uint32_t)((x * cs + lo) * 2), active);
+        else if constexpr (WAGG < 0) {
+            if (wagg) bstore_cell<NP>(fin, ragg, (uint32_t)((x * cs + lo) * 2), active);
+        }
         if constexpr (!WTA) return;
         // winner-take-all on the final aggregate of cell x
         uint32_t key[E];
@@ -1025,7 +1100,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         const int d = best == 0xffffffffu ? 0 : (int)(best & 0xffffu);  // no planes: index 0 (:227)
         const bool par = d > 0 && d < D - 1;
         const int l = (int)plane_of<NP>(fin, par ? d - 1 : 0), r = (int)plane_of<NP>(fin, par ? d + 1 : 0);
-        if (lane == (x & 63)) {
+        if (lane == x - xb) {
             myd = (uint32_t)(d + s0);  // the plane index start + arg (:230)
             myr = rdone & 0xffffu;
             mypar = par;
@@ -1033,32 +1108,48 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             myc = (int)(best >> 16);
             myrr = r;
         }
-        if ((x & 63) == 0 || x == 1) {  // flush the block of 64 pixels, coalesced
-            const int xb = x & ~63, l0 = x & 63;
-            const int l1 = min(63, W - 1 - xb);
-            if (lane >= l0 && lane <= l1) {
-                const size_t o = rowcell + xb + lane;
-                a.disp[o] = (uint16_t)myd;
-                if (a.sub) {  // the parabola through agg(d-1), agg(d), agg(d+1)
-                    float v = (float)(int)myd;
-                    const int den = myl - 2 * myc + myrr;
-                    if (mypar && den > 0) v = (float)(int)myd + __fdiv_rn(__fmul_rn(0.5f, (float)(myl - myrr)), (float)den);
-                    a.sub[o] = v;
-                }
-                if (!VAR) a.dr[o] = (uint16_t)myr;
-            }
-        }
     };
-    // x = W-1 .. 1: (W-1) steps
+    // the block's 64 pixels xb .. xb+63 (lane l: pixel xb + l; lanes [l0, l1] hold results), coalesced
+    auto flush = [&](int xb, int l0, int l1) {
+        if (lane < l0 || lane > l1) return;
+        const size_t o = rowcell + xb + lane;
+        a.disp[o] = (uint16_t)myd;
+        float v = (float)(int)myd;  // the parabola through agg(d-1), agg(d), agg(d+1)
+        const int den = myl - 2 * myc + myrr;
+        if (mypar && den > 0) v = (float)(int)myd + __fdiv_rn(__fmul_rn(0.5f, (float)(myl - myrr)), (float)den);
+        bstore_f32(v, rsub, (uint32_t)(xb + lane) * 4u);  // dropped when the caller wants no subpixel
+        if (!VAR) a.dr[o] = (uint16_t)myr;
+    };
+    // x = W-1 .. 1: (W-1) steps, in blocks of 64 steps counted from x = W-1 (PF divides 64, so every block starts on
+    // the same prefetch slot); a block's loop holds only the prefetch loads and, when asked, the aggregate stores
     const int n = W - 1;
     int i0 = 0;
+    if constexpr (WTA) {
+        for (; i0 + 64 <= n; i0 += 64) {
+            const int xb = W - 64 - i0;
+            for (int i8 = i0; i8 < i0 + 64; i8 += PF) {
+#pragma unroll
+                for (int j = 0; j < PF; j++) body(W - 1 - (i8 + j), j, xb);
+            }
+            flush(xb, 0, 63);
+        }
+    }
+    // the rest (the last block's fewer than 64 steps, or the whole row without the winner-take-all)
+    const int xb = W - 64 - i0, ilast = i0;
     for (; i0 + PF <= n; i0 += PF) {
 #pragma unroll
-        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j);
+        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j, xb);
     }
 #pragma unroll
     for (int j = 0; j < PF; j++)
-        if (i0 + j < n) body(W - 1 - (i0 + j), j);
+        if (i0 + j < n) body(W - 1 - (i0 + j), j, xb);
+    if constexpr (WTA) {
+        if (n > ilast) flush(xb, 1 - xb, W - 1 - ilast - xb);  // pixels 1 .. W-1-ilast
+    }
+    if (n > 0) {  // the column-1 states for the border replay
+        store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, state);
+        if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+    }
     if constexpr (WTA && !VAR) {
         const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)R[1], 0);
         if (lane == 0) a.dr0[y] = r1;
@@ -1067,14 +1158,16 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 template <int NP, bool MASKED, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta(RowArgs a)
 {
-    row_bwd_wave<NP, MASKED, VAR, ADP, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), threadIdx.x & 63);
+    row_bwd_wave<NP, MASKED, VAR, ADP, true>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
-// Pass -1's row path of a request with byte path volumes: the eight paths' sum and the winner-take-all.
-template <int NP, bool MASKED>
+// Pass -1's row path of a request with byte path volumes: the eight paths' sum and the winner-take-all; WAGG: the
+// final aggregate is stored too.
+template <int NP, bool MASKED, int WAGG>
 __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_pv(RowArgs a)
 {
-    row_bwd_wave<NP, MASKED, false, false, true, true>(a, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
-                                                        threadIdx.x & 63);
+    row_bwd_wave<NP, MASKED, false, false, true, true, WAGG>(
+        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
 
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
@@ -1093,7 +1186,7 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             RawW<NP> w;
-            load_raw_w<NP>(a.pv[v] + rowcell * cs + lo, active, w);
+            load_raw_w<NP>(a.pv[v] + rowcell * a.pvs + lo, active, w);
             add_bytes<NP>(w, c);
         }
         if (a.write_agg) store_u16<NP>(a.agg + rowcell * cs + lo, active, c);
@@ -2416,6 +2509,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     const bool pv = kPv && b.pv[0] != nullptr && b.cl != nullptr;
     if (pv) {
         for (int k = 0; k < 7; k++) ra.pv[k] = b.pv[k];
+        ra.pvs = b.pvs;
         ra.bagg_plus = b.bagg[0];
     }
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
@@ -2438,6 +2532,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if constexpr (kPv) {
             if (pv) {  // each path into its own byte volume: 1..3 (pass +1), 4..6 (pass -1)
                 SweepArgs q = a;
+                q.pvs = b.pvs;
                 q.pv = b.pv[DIR > 0 ? 1 : 4];
                 sgm_sweep_pv<NP, DIR, 1, MASKED><<<grid(nd), wg, 0, s>>>(q);
                 mark();
@@ -2616,7 +2711,8 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         if constexpr (SEG > 0) {
             sgm_row_bwd_wta_seg<SEG><<<sblocks(nh), kBlock, 0, s>>>(ra);
         } else if constexpr (kPv) {
-            if (pv) sgm_row_bwd_wta_pv<NP, MASKED><<<grid(nh), wg, 0, s>>>(ra);
+            if (pv && o.write_agg) sgm_row_bwd_wta_pv<NP, MASKED, 1><<<grid(nh), wg, 0, s>>>(ra);
+            else if (pv) sgm_row_bwd_wta_pv<NP, MASKED, 0><<<grid(nh), wg, 0, s>>>(ra);
             else sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
         } else {
             sgm_row_bwd_wta<NP, MASKED, VAR, ADP><<<grid(nh), wg, 0, s>>>(ra);
