@@ -789,10 +789,12 @@ __device__ __forceinline__ uint64_t shr1_64(uint64_t v)
 template <int NP>
 __device__ __forceinline__ void census_advance(uint64_t (&win)[2 * NP], uint64_t fresh, int lane)
 {
-    const uint64_t from = shr1_64(win[2 * NP - 1]);
+    // wave_shr:1 with lane 0 (which has no source lane) taking `fresh` as the DPP old value: no select
+    const uint64_t last = win[2 * NP - 1];
+    const uint32_t lo = shr1((uint32_t)last, (uint32_t)fresh), hi = shr1((uint32_t)(last >> 32), (uint32_t)(fresh >> 32));
 #pragma unroll
     for (int e = 2 * NP - 1; e > 0; e--) win[e] = win[e - 1];
-    win[0] = lane == 0 ? fresh : from;
+    win[0] = ((uint64_t)hi << 32) | lo;
 }
 // Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
 // repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
@@ -866,7 +868,12 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
         const int xj = min(j, W - 1);
         if constexpr (!COST) ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
     }
-    auto body = [&](int x, int j) {
+    // the step loop's stores do not depend on a condition but for padded lanes (see bstore_*); the column W-2
+    // state for the border replay is kept and stored after the loop
+    const bool st_raw = MASKED ? store_lane : true;
+    Cell<NP> colv;
+    uint32_t colm = 0;
+    auto body = [&](int x, int j, uint64_t& bl, uint64_t& br) {
         Cell<NP> raw;
         const int xp = min(x + PF, W - 1);
         uint32_t el = kInit2, er = kInit2;
@@ -882,12 +889,11 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
                 return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), q) << 32) |
                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, q);
             };
-            const uint64_t cl0 = rl64(bl0), cl1 = rl64(bl1), cr0 = rl64(br0), cr1_ = rl64(br1);
-            const uint64_t cl = (x & 64) ? cl1 : cl0, cr1 = (x & 64) ? cr1_ : cr0;
+            const uint64_t cl = rl64(bl), cr1 = rl64(br);
             uint32_t bytes[NP / 2 + (NP == 1)];
             census_cost<NP>(cl, win, a.D, lo, raw, bytes);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
-            if (store_lane) {
+            if (st_raw) {
                 if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
                 else {
 #pragma unroll
@@ -895,15 +901,6 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
                 }
             }
             census_advance<NP>(win, cr1, lane);
-            if (q == 63) {  // this block's registers take the block after next, loaded in place
-                if (x & 64) {
-                    bl1 = clrow[min(x + 65 + lane, W - 1)];
-                    br1 = crow[min(x + 66 + lane, W - 1)];
-                } else {
-                    bl0 = clrow[min(x + 65 + lane, W - 1)];
-                    br0 = crow[min(x + 66 + lane, W - 1)];
-                }
-            }
         } else {
             widen<NP>(rb[j], raw);
             ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
@@ -918,22 +915,48 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
         Cell<NP> out;
 #pragma unroll
         for (int k = 0; k < NP; k++) out.v[k] = x != W - 1 ? lc.v[k] : 0u;
-        if constexpr (PV) store_bytes<NP>(a.pv[0] + (rowcell + x) * a.pvs + lo, active, out);
-        else store_u16<NP>(a.agg + (rowcell + x) * cs + lo, active, out);
+        if constexpr (PV) store_bytes<NP>(a.pv[0] + (rowcell + x) * a.pvs + lo, MASKED ? active : true, out);
+        else store_u16<NP>(a.agg + (rowcell + x) * cs + lo, MASKED ? active : true, out);
         if (x == W - 2) {
-            store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, lc);
-            if (lane == 0) a.colmins[y] = (uint16_t)minprev;
+            colv = lc;
+            colm = minprev;
         }
         state = lc;
     };
     int x0 = 0;
-    for (; x0 + PF <= W; x0 += PF) {
+    if constexpr (COST) {
+        // blocks of 64 cells alternate between the two register sets, so each step reads its block's codes from
+        // registers fixed at compile time; a set is reloaded in place with the block after next when its block ends
+        auto half = [&](int xa, uint64_t& bl, uint64_t& br) {
+            int i = xa;
+            const int ie = min(xa + 64, W);
+            for (; i + PF <= ie; i += PF) {
 #pragma unroll
-        for (int j = 0; j < PF; j++) body(x0 + j, j);
+                for (int j = 0; j < PF; j++) body(i + j, j, bl, br);
+            }
+#pragma unroll
+            for (int j = 0; j < PF; j++)
+                if (i + j < ie) body(i + j, j, bl, br);
+            bl = clrow[min(xa + 128 + lane, W - 1)];
+            br = crow[min(xa + 129 + lane, W - 1)];
+        };
+        for (; x0 < W; x0 += 128) {
+            half(x0, bl0, br0);
+            if (x0 + 64 < W) half(x0 + 64, bl1, br1);
+        }
+    } else {
+        for (; x0 + PF <= W; x0 += PF) {
+#pragma unroll
+            for (int j = 0; j < PF; j++) body(x0 + j, j, bl0, br0);
+        }
+#pragma unroll
+        for (int j = 0; j < PF; j++)
+            if (x0 + j < W) body(x0 + j, j, bl0, br0);
     }
-#pragma unroll
-    for (int j = 0; j < PF; j++)
-        if (x0 + j < W) body(x0 + j, j);
+    if (W >= 2) {  // the column W-2 states for the border replay
+        store_u16<NP>(a.cols + (size_t)y * cs + lo, store_lane, colv);
+        if (lane == 0) a.colmins[y] = (uint16_t)colm;
+    }
 }
 template <int NP, bool MASKED, bool COST, bool VAR, bool ADP>
 __global__ __launch_bounds__(kBlock) void sgm_row_fwd(RowArgs a)
@@ -982,6 +1005,13 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     for (int e = 0; e < E; e++) R[e] = 0xffffffffu;
     const size_t rowcell = (size_t)y * W;
     const int lol = (VAR ? lo < cs : active) ? lo : 0;
+    // the row's base addresses are wave-uniform (scalar registers) and each load adds a 32-bit lane offset below
+    // W * cs, so the loads take their address as base + offset with no 64-bit arithmetic per lane
+    const uint8_t* rawrow = a.raw + rowcell * cs;
+    const uint16_t* aggrow = a.agg + rowcell * cs;
+    const uint8_t* pvrow[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int v = 0; v < NV; v++) pvrow[v] = a.pv[v] + rowcell * a.pvs;
     RawW<NP> rb[PF];
     Cell<NP> ab[PF];
     RawW<NP> vb[PF][NV > 0 ? NV : 1];  // PV: the seven path volumes' bytes of the cells in flight
@@ -1013,12 +1043,12 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
-        ld_raw_w<NP>(a.raw + (rowcell + xj) * cs + lol, rb[j]);
+        ld_raw_w<NP>(rawrow + (uint32_t)(xj * cs + lol), rb[j]);
         if constexpr (PV) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xj) * a.pvs + lol, vb[j][v]);
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(pvrow[v] + (uint32_t)(xj * a.pvs + lol), vb[j][v]);
         } else {
-            ld_u16<NP>(a.agg + (rowcell + xj) * cs + lol, ab[j]);
+            ld_u16<NP>(aggrow + (uint32_t)(xj * cs + lol), ab[j]);
         }
     }
     // lane x&63 keeps cell x's winner and the parabola's inputs; the float arithmetic of the subpixel runs once per
@@ -1059,12 +1089,12 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             p2 = gq.at(W - 1 - x);
             gq.advance(W - 1 - x, gq_at, lane, a.ap, a.p2);
         }
-        ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
+        ld_raw_w<NP>(rawrow + (uint32_t)(xp * cs + lol), rb[j]);
         if constexpr (PV) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) ld_raw_w<NP>(a.pv[v] + (rowcell + xp) * a.pvs + lol, vb[j][v]);
+            for (int v = 0; v < NV; v++) ld_raw_w<NP>(pvrow[v] + (uint32_t)(xp * a.pvs + lol), vb[j][v]);
         } else {
-            ld_u16<NP>(a.agg + (rowcell + xp) * cs + lol, ab[j]);
+            ld_u16<NP>(aggrow + (uint32_t)(xp * cs + lol), ab[j]);
         }
         Cell<NP> lc, fin;
         minprev = path_step<NP, MASKED>(state, minprev, raw, a.p1x2, p2, mask, lc, el, er);
