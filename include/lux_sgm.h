@@ -375,13 +375,20 @@ typedef struct {
     uint8_t* valid;
 } lux_sgm_pair;
 
-/* A batch schedule: pairs spread over `branches` concurrent graph branches, longest first onto the least
- * loaded branch; `bands` selects the fused-band paths (LUX_SGM_OPT_FUSED_BANDS). ms: measured time of one
- * batch under this schedule (0 when not measured). */
+/* A batch schedule. executor = 0: each pair's launch sequence on one of `branches` concurrent graph branches,
+ * longest first onto the least loaded branch; `bands` selects the fused-band paths (LUX_SGM_OPT_FUSED_BANDS).
+ * executor = 1: the persistent executor, every stage of every pair in one launch (resident wavefronts pull work
+ * items by ticket; an item waits for the stages it reads); executor = 7: the executor in one launch per pass group
+ * (census codes, pass +1 row paths, pass +1 sweeps and border chains, pass +1 border replays, pass -1 sweeps and
+ * chains, pass -1 row paths, pass -1 replays and finish). The executor takes census pairs whose path costs fit a
+ * byte (up to 256 planes). order (executor): 0 stage by stage in batch order, 1 the pairs with the longest chains
+ * first within each stage. ms: measured time of one batch under this schedule (0 when not measured). */
 typedef struct {
     int32_t bands;
     int32_t branches;
     float ms;
+    int32_t executor;
+    int32_t order;
 } lux_sgm_schedule;
 
 /* Enqueue the n pairs on `stream` (NULL = the context's stream) as one hipGraph launch. The graph is built on
@@ -390,6 +397,10 @@ typedef struct {
  * enqueued; results are those of lux_sgm_rectified_device on each pair alone, bit for bit. No counterpart in
  * the reference (its dense stage runs one layer at a time, DenseStereoSgm.cs:346-351). */
 int lux_sgm_batch(lux_sgm_ctx* ctx, int32_t n, const lux_sgm_pair* pairs, void* stream);
+
+/* Use schedule `sch` for batches of these pairs' shapes (as if lux_sgm_batch_tune had chosen it). An executor
+ * schedule for pairs the executor does not take fails with LUX_SGM_EUNSUPPORTED. */
+int lux_sgm_batch_set_schedule(lux_sgm_ctx* ctx, int32_t n, const lux_sgm_pair* pairs, const lux_sgm_schedule* sch);
 
 /* Autotuner: time each candidate schedule on this batch (each run once to warm, then `reps` timed runs) and
  * keep the fastest for batches of the same shapes. best receives it; tried (if not NULL) receives up to cap

@@ -15,6 +15,7 @@
 #include "../../include/lux_sgm.h"
 #include "ctx.h"
 #include "engine.h"
+#include "exec.h"
 
 namespace {
 
@@ -42,12 +43,17 @@ int rect_device(lux_sgm_ctx* c, const lux_sgm_rect_params* p, const void* dl, co
     return rc;
 }
 
+void destroy_graph(GraphEntry& g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    lux::exec_destroy(g.xb);
+    g = GraphEntry();
+}
+
 void destroy_graphs(lux_sgm_ctx* c)
 {
-    for (auto& kv : c->graphs) {
-        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-    }
+    for (auto& kv : c->graphs) destroy_graph(kv.second);
     c->graphs.clear();
 }
 
@@ -64,6 +70,8 @@ lux_sgm_schedule default_schedule(int n)
     s.bands = 0;
     s.branches = std::min(n, 4);
     s.ms = 0.f;
+    s.executor = 0;
+    s.order = 0;
     return s;
 }
 
@@ -121,10 +129,49 @@ int prepare_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<
     return e == hipSuccess ? 0 : hip_fail(c, e, "prepare batch");
 }
 
+// Capture the executor's launches of the batch into a graph (the work lists are uploaded first, outside the capture).
+int capture_exec(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<lux::Prepared>& prep,
+                 const lux_sgm_schedule& sch, GraphEntry* out)
+{
+    std::vector<lux::ExecPair> xp(n);
+    for (int i = 0; i < n; i++) {
+        if (!lux::exec_supported(&pairs[i].params, prep[i]))
+            return fail(c, LUX_SGM_EUNSUPPORTED, "the executor takes census pairs whose path costs fit a byte");
+        xp[i].p = &pairs[i].params;
+        xp[i].w = &prep[i];
+        xp[i].left = pairs[i].left;
+        xp[i].right = pairs[i].right;
+        xp[i].disp = pairs[i].disp;
+        xp[i].sub = pairs[i].sub;
+        xp[i].valid = pairs[i].valid;
+    }
+    lux::ExecBatch* xb = nullptr;
+    int rc = lux::exec_create(xp.data(), n, sch.executor, sch.order, c->device, &xb, &c->err);
+    if (rc) return rc;
+    hipStream_t cs = c->stream;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t el = lux::exec_enqueue(xb, 0, cs);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(cs, &graph);
+        if (e == hipSuccess) e = el;
+        if (e == hipSuccess) e = hipGraphInstantiate(&out->exec, graph, nullptr, nullptr, 0);
+        out->graph = graph;
+    }
+    out->xb = xb;
+    out->gen = c->gen;
+    if (e != hipSuccess) {
+        destroy_graph(*out);
+        return hip_fail(c, e, "capture executor batch");
+    }
+    return 0;
+}
+
 // Capture the batch into a graph: fork from the capture stream onto the branches, one request per pair.
 int capture_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<lux::Prepared>& prep,
                   const lux_sgm_schedule& sch, GraphEntry* out)
 {
+    if (sch.executor) return capture_exec(c, n, pairs, prep, sch, out);
     const int nb = std::max(1, std::min({sch.branches, n, kMaxBranches}));
     std::vector<int> br = assign_branches(n, pairs, nb);
     lux::EngineOpts opt;
@@ -173,7 +220,8 @@ int capture_batch(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, std::vector<
 }
 
 // The instantiated graph of this batch under schedule sch (built on first use, cached by descriptor).
-int batch_graph(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, const lux_sgm_schedule& sch, hipGraphExec_t* exec)
+int batch_graph(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, const lux_sgm_schedule& sch, hipGraphExec_t* exec,
+                GraphEntry** entry = nullptr)
 {
     int rc = ensure_batch_streams(c);
     if (rc) return rc;
@@ -181,12 +229,13 @@ int batch_graph(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, const lux_sgm_
     if ((rc = prepare_batch(c, n, pairs, &prep))) return rc;
     std::string key((const char*)&sch.bands, sizeof sch.bands);
     key.append((const char*)&sch.branches, sizeof sch.branches);
+    key.append((const char*)&sch.executor, sizeof sch.executor);
+    key.append((const char*)&sch.order, sizeof sch.order);
     key.append((const char*)&n, sizeof n);
     key.append((const char*)pairs, sizeof(lux_sgm_pair) * (size_t)n);
     auto it = c->graphs.find(key);
     if (it != c->graphs.end() && it->second.gen != c->gen) {  // a workspace moved since: rebuild
-        (void)hipGraphExecDestroy(it->second.exec);
-        (void)hipGraphDestroy(it->second.graph);
+        destroy_graph(it->second);
         c->graphs.erase(it);
         it = c->graphs.end();
     }
@@ -197,7 +246,14 @@ int batch_graph(lux_sgm_ctx* c, int n, const lux_sgm_pair* pairs, const lux_sgm_
         it = c->graphs.emplace(key, ge).first;
     }
     *exec = it->second.exec;
+    if (entry) *entry = &it->second;
     return 0;
+}
+
+// After the stream has drained: the executor's error word of this batch's graph (a stage wait that gave up).
+int batch_exec_check(lux_sgm_ctx* c, GraphEntry* ge, hipStream_t s)
+{
+    return ge && ge->xb ? lux::exec_check(ge->xb, s, &c->err) : 0;
 }
 
 }  // namespace
@@ -727,6 +783,24 @@ int lux_sgm_batch(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, void* st
     return e == hipSuccess ? 0 : hip_fail(c, e, "hipGraphLaunch");
 }
 
+int lux_sgm_batch_set_schedule(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, const lux_sgm_schedule* sch)
+{
+    if (!c) return LUX_SGM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n <= 0 || !pairs || !sch) return fail(c, LUX_SGM_EINVAL, "empty batch or null schedule");
+    if (sch->executor != 0 && sch->executor != 1 && sch->executor != 7)
+        return fail(c, LUX_SGM_EINVAL, "executor must be 0, 1 or 7");
+    if (sch->executor)
+        for (int i = 0; i < n; i++) {
+            int rc = lux::check_rect(&pairs[i].params, &c->err);
+            if (rc) return rc;
+            if (!lux::exec_params_supported(&pairs[i].params))
+                return fail(c, LUX_SGM_EUNSUPPORTED, "the executor takes census pairs whose path costs fit a byte");
+        }
+    c->tuned[shape_key(n, pairs)] = *sch;
+    return 0;
+}
+
 int lux_sgm_batch_tune(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, void* stream, int32_t reps,
                        lux_sgm_schedule* best, lux_sgm_schedule* tried, int32_t cap)
 {
@@ -745,12 +819,22 @@ int lux_sgm_batch_tune(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, voi
     for (int bands = 0; bands <= (bands_ok ? 1 : 0); bands++)
         for (int b : {1, 2, 3, 4, 6, 8}) {
             if (b > n) continue;
-            lux_sgm_schedule sc;
+            lux_sgm_schedule sc = default_schedule(n);
             sc.bands = bands;
             sc.branches = b;
-            sc.ms = 0.f;
             cand.push_back(sc);
         }
+    // the persistent executor, when it takes every pair: one launch or one per pass group, two item orders
+    bool exec_ok = true;
+    for (int i = 0; i < n && exec_ok; i++) exec_ok = lux::exec_params_supported(&pairs[i].params);
+    if (exec_ok)
+        for (int groups : {1, 7})
+            for (int order : {0, 1}) {
+                lux_sgm_schedule sc = default_schedule(n);
+                sc.executor = groups;
+                sc.order = order;
+                cand.push_back(sc);
+            }
     hipEvent_t e0, e1;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -760,7 +844,8 @@ int lux_sgm_batch_tune(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, voi
     win.ms = -1.f;
     for (auto& sc : cand) {
         hipGraphExec_t exec;
-        if ((rc = batch_graph(c, n, pairs, sc, &exec))) break;
+        GraphEntry* ge = nullptr;
+        if ((rc = batch_graph(c, n, pairs, sc, &exec, &ge))) break;
         e = hipGraphLaunch(exec, s);  // warm
         if (e == hipSuccess) e = hipEventRecord(e0, s);
         for (int r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(exec, s);
@@ -773,6 +858,7 @@ int lux_sgm_batch_tune(lux_sgm_ctx* c, int32_t n, const lux_sgm_pair* pairs, voi
             break;
         }
         for (int i = 0; i < n && !rc; i++) rc = lux::check_band_error(*c->bws[i], s, &c->err);
+        if (!rc) rc = batch_exec_check(c, ge, s);
         if (rc) break;
         sc.ms = ms / reps;
         if (tried && ntried < cap) tried[ntried] = sc;

@@ -16,10 +16,12 @@ namespace lux {
 
 constexpr int kMaxBranches = 8;
 
+struct ExecBatch;
 struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t gen = 0;
+    ExecBatch* xb = nullptr;  // the executor's work lists when the graph runs the executor
 };
 
 }  // namespace lux

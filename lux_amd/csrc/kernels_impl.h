@@ -1203,9 +1203,8 @@ __global__ __launch_bounds__(kBlock) void sgm_row_bwd_wta_pv(RowArgs a)
 // Column 0 (after the pass -1 replay) and the validity of every pixel of a row, one wavefront per row:
 // valid(x) = x - dL >= 0 && |dL - dR(x - dL)| <= tol (RealStereo.cs:418-421).
 template <int NP, bool VAR>
-__global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* valid)
+__device__ __forceinline__ void wta_finish_wave(const RowArgs& a, int tol, uint8_t* valid, int y, int lane)
 {
-    const int y = blockIdx.x, lane = threadIdx.x;
     const size_t rowcell = (size_t)y * a.W;
     const int W = a.W, cs = a.cs, D = VAR ? (int)a.vcount[rowcell] : a.D, lo = lane * 2 * NP;
     const int s0 = VAR ? (int)a.vstart[rowcell] : 0;
@@ -1253,6 +1252,11 @@ __global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* va
             const int dd = a.disp[rowcell + x], xr = x - dd;
             valid[rowcell + x] = (uint8_t)(xr >= 0 && abs(dd - (int)a.dr[rowcell + xr]) <= tol);
         }
+}
+template <int NP, bool VAR>
+__global__ __launch_bounds__(64) void wta_finish(RowArgs a, int tol, uint8_t* valid)
+{
+    wta_finish_wave<NP, VAR>(a, tol, valid, blockIdx.x, threadIdx.x);
 }
 
 // Ranged layers after the fused passes: the final aggregate, agg + the three path volumes (saturating, so the
@@ -2016,10 +2020,8 @@ __device__ __forceinline__ uint32_t step_from(Cell<NP> prev, Range from, uint32_
 // fixed by the plan (saved neighbour-column states, pass +1 seeds, or the margin slot for path 3), so
 // they do not chain. Each visit's sum of the three path costs goes to `partial`.
 template <int NP, bool MASKED, bool VAR>
-__global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
+__device__ __forceinline__ void replay_paths_wave(const ReplayArgs& a, int i, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (i >= a.nvis) return;
     const int y = a.visits[4 * i], l0 = a.visits[4 * i + 2], l1 = a.visits[4 * i + 3];
     const size_t cell = (size_t)y * a.W + a.col;
@@ -2043,6 +2045,11 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
 #pragma unroll
     for (int k = 0; k < NP; k++) sum.v[k] = padd(padd(lc0.v[k], lc1.v[k]), lc3.v[k]);
     store_u16<NP>(a.partial + (size_t)i * a.cs + lo, active, sum);
+}
+template <int NP, bool MASKED, bool VAR>
+__global__ __launch_bounds__(kBlock) void sgm_replay_paths(ReplayArgs a)
+{
+    replay_paths_wave<NP, MASKED, VAR>(a, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
 }
 
 // Path 2 of the border column: one chain through the column's visits in schedule order. Each visit reads
@@ -2250,9 +2257,8 @@ __global__ __launch_bounds__(kBlock) void sgm_pass_seg(PassArgs p)
 // Pass +1's seed-table entries of paths 1 and 3 for the border cell: recomputed for the last visit that wrote
 // half 0. Path 1 reads a saved neighbour-column state, so this runs after the sweeps (one wavefront).
 template <int NP, bool MASKED, bool VAR, bool ADP>
-__global__ __launch_bounds__(64) void sgm_replay_seeds(ReplayArgs a, const int* __restrict__ vis)
+__device__ __forceinline__ void replay_seeds_wave(const ReplayArgs& a, const int* __restrict__ vis, int lane)
 {
-    const int lane = threadIdx.x;
     const int cs = a.cs, lo = lane * 2 * NP;
     Cell<NP> mask;
     bool active;
@@ -2304,14 +2310,17 @@ __global__ __launch_bounds__(64) void sgm_replay_seeds(ReplayArgs a, const int* 
         }
     }
 }
+template <int NP, bool MASKED, bool VAR, bool ADP>
+__global__ __launch_bounds__(64) void sgm_replay_seeds(ReplayArgs a, const int* __restrict__ vis)
+{
+    replay_seeds_wave<NP, MASKED, VAR, ADP>(a, vis, threadIdx.x);
+}
 
 // Adds every border visit's four path costs to its cell (one wavefront per row of the border column;
 // `cellvis` lists each row's visits, at most two per pass).
 template <int NP, bool VAR>
-__global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int* cellvis)
+__device__ __forceinline__ void replay_add_wave(const ReplayArgs& a, const int* cellvis, int y, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int y = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
     if (y >= a.H) return;
     const int lo = lane * 2 * NP;
     const bool active = lo < (VAR ? (int)a.vcap[(size_t)y * a.W + a.col] : a.cap);
@@ -2338,6 +2347,11 @@ __global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int
     if (a.colagg) store_u16<NP>(a.colagg + (size_t)y * a.cs + lo, lo < a.cs, agg);
     else store_u16<NP>(a.agg + cell, active, agg);
 }
+template <int NP, bool VAR>
+__global__ __launch_bounds__(kBlock) void sgm_replay_add(ReplayArgs a, const int* cellvis)
+{
+    replay_add_wave<NP, VAR>(a, cellvis, (int)((blockIdx.x * kBlock + threadIdx.x) >> 6), threadIdx.x & 63);
+}
 
 // Seeds of interior cells when pass +1 never writes half 0 (H = 1): the initial fill.
 __global__ void fill_u16(uint16_t* p, size_t n, uint16_t v)
@@ -2357,11 +2371,11 @@ __global__ void agg_sum4(uint32_t* out, const uint32_t* a0, const uint32_t* a1, 
 }
 
 // ------------------------------------------------------------------ census and cost volume
+// The census code of pixel (x, y): bit k set where the k-th neighbour of the (2R+1)^2 window (row-major, the centre
+// left out, coordinates clamped to the image) is darker than the centre.
 template <typename T, int R>
-__global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* out)
+__device__ __forceinline__ uint64_t census_at(const T* img, int W, int H, int stride, int x, int y)
 {
-    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-    if (x >= W || y >= H) return;
     const int c = img[(size_t)y * stride + x];
     uint64_t code = 0;
     int bit = 0;
@@ -2377,7 +2391,14 @@ __global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* 
             bit++;
         }
     }
-    out[(size_t)y * W + x] = code;
+    return code;
+}
+template <typename T, int R>
+__global__ void census_kernel(const T* img, int W, int H, int stride, uint64_t* out)
+{
+    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    out[(size_t)y * W + x] = census_at<T, R>(img, W, H, stride, x, y);
 }
 
 // Truncated 3x3 absolute-difference cost, four planes per thread (consecutive threads write consecutive
@@ -2503,15 +2524,11 @@ __global__ void restride_kernel(const T* src, int scs, T* dst, int dcs, size_t c
 inline int grid_for(size_t total, int block) { return (int)std::min<size_t>((total + block - 1) / block, 1u << 20); }
 
 // ------------------------------------------------------------------ launchers
-template <int NP, bool MASKED, bool VAR, bool ADP, int SEG = 0>
-hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
-                   hipStream_t s, hipEvent_t* marks)
+// The argument blocks of one request's kernels: the sweeps' and row paths' (pass +1's border-column state slots) and
+// pass +1's border replay. pv: the request runs with byte path volumes.
+inline void sgm_base_args(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, bool pv,
+                          SweepArgs& a, RowArgs& ra, ReplayArgs& r)
 {
-    static_assert(SEG == 0 || (NP == 1 && MASKED && VAR && !ADP), "segmented chains: narrow per-pixel ranges only");
-    auto sblocks = [](int chains) { return (unsigned)(((size_t)chains * (SEG ? SEG : 64) + kBlock - 1) / kBlock); };
-    int mk = 0;
-    auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
-    SweepArgs a;
     a.raw = b.raw;
     a.agg = b.agg;
     a.W = g.W; a.H = g.H; a.cs = g.cs; a.cap = g.cap;
@@ -2526,22 +2543,45 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     a.seedcap = b.seedcap;
     a.ap = b.ap;
     a.seedguid = b.seedguid;
-    RowArgs ra;
     ra.raw = b.raw; ra.raw_out = b.raw_out; ra.cl = b.cl; ra.cr = b.cr; ra.agg = b.agg;
     ra.W = g.W; ra.H = g.H; ra.cs = g.cs; ra.cap = g.cap; ra.D = g.D;
     ra.p1x2 = a.p1x2; ra.p2 = a.p2;
     ra.disp = o.disp; ra.sub = o.sub; ra.dr = b.dr; ra.dr0 = b.dr0; ra.write_agg = o.write_agg;
     ra.vstart = b.vstart; ra.vcap = b.vcap; ra.vcount = b.vcount;
     ra.ap = b.ap;
-    // byte path volumes: uniform constant-P2 census requests of at most 256 planes whose path costs fit a byte
-    // (path_bytes in engine.hip decides and gives the volumes)
-    constexpr bool kPv = !VAR && !ADP && SEG == 0 && NP <= 2;
-    const bool pv = kPv && b.pv[0] != nullptr && b.cl != nullptr;
     if (pv) {
         for (int k = 0; k < 7; k++) ra.pv[k] = b.pv[k];
         ra.pvs = b.pvs;
         ra.bagg_plus = b.bagg[0];
     }
+    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
+    r.p1x2 = a.p1x2; r.p2 = a.p2;
+    r.seeds = b.seeds; r.seedmins = b.seedmins;
+    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
+    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial; r.chainp = b.chainp;
+    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
+    r.ap = b.ap; r.seedguid = b.seedguid;
+    r.colagg = pv ? b.bagg[0] : nullptr;
+    a.cols = b.cols; a.colmins = b.colmins;
+    ra.cols = b.cols; ra.colmins = b.colmins;
+}
+
+template <int NP, bool MASKED, bool VAR, bool ADP, int SEG = 0>
+hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, int p1, int p2, uint16_t* agg_pass1,
+                   hipStream_t s, hipEvent_t* marks)
+{
+    static_assert(SEG == 0 || (NP == 1 && MASKED && VAR && !ADP), "segmented chains: narrow per-pixel ranges only");
+    auto sblocks = [](int chains) { return (unsigned)(((size_t)chains * (SEG ? SEG : 64) + kBlock - 1) / kBlock); };
+    int mk = 0;
+    auto mark = [&]() { if (marks) (void)hipEventRecord(marks[mk++], s); };
+    // byte path volumes: uniform constant-P2 census requests of at most 256 planes whose path costs fit a byte
+    // (path_bytes in engine.hip decides and gives the volumes)
+    constexpr bool kPv = !VAR && !ADP && SEG == 0 && NP <= 2;
+    const bool pv = kPv && b.pv[0] != nullptr && b.cl != nullptr;
+    SweepArgs a;
+    RowArgs ra;
+    ReplayArgs r;
+    sgm_base_args(g, b, o, p1, p2, pv, a, ra, r);
     const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
     // one chain a wavefront: one-wavefront workgroups when the request has the device to itself (SgmBuffers::solo)
@@ -2671,18 +2711,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
             }
         }
     };
-    ReplayArgs r;
-    r.raw = b.raw; r.agg = b.agg; r.W = g.W; r.H = g.H; r.cs = g.cs; r.cap = g.cap;
-    r.p1x2 = a.p1x2; r.p2 = a.p2;
-    r.seeds = b.seeds; r.seedmins = b.seedmins;
-    r.dir = 1; r.col = g.W - 1; r.visits = b.visits_plus; r.nvis = b.n_plus;
-    r.cols = b.cols; r.colmins = b.colmins; r.partial = b.partial; r.chainp = b.chainp;
-    r.vstart = b.vstart; r.vcap = b.vcap; r.seedstart = b.seedstart; r.seedcap = b.seedcap;
-    r.ap = b.ap; r.seedguid = b.seedguid;
-    r.colagg = pv ? b.bagg[0] : nullptr;
     // pass +1: -> (with the census cost fused when codes are given), then down-right, down, down-left
-    a.cols = b.cols; a.colmins = b.colmins;
-    ra.cols = b.cols; ra.colmins = b.colmins;
     if (fused) {
         launch_pass(std::integral_constant<int, 1>(), r);
         mark();  // the row path and paths 1-3 in one launch: the next three stages are empty

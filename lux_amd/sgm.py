@@ -26,7 +26,7 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_sweep_cost", "lux_sgm_range_from_previous", "lux_sgm_dense_layer", "lux_sgm_dense_pyramid",
            "lux_sgm_dense_pyramid_defaults", "lux_sgm_dense_pyramid_device",
            "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
-           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_server_create", "lux_sgm_server_destroy",
+           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_batch_set_schedule", "lux_sgm_server_create", "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
            "lux_sgm_server_wait",
            "lux_sgm_server_cancel", "lux_sgm_server_info_get", "lux_sgm_planes", "lux_sgm_flow_matrix",
@@ -60,7 +60,9 @@ class Pair(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("bands", ctypes.c_int32), ("branches", ctypes.c_int32), ("ms", ctypes.c_float)]
+    """lux_sgm_schedule: graph branches (executor 0) or the persistent executor (1: one launch, 7: per pass group)."""
+    _fields_ = [("bands", ctypes.c_int32), ("branches", ctypes.c_int32), ("ms", ctypes.c_float),
+                ("executor", ctypes.c_int32), ("order", ctypes.c_int32)]
 
 
 class RequestStats(ctypes.Structure):
@@ -154,6 +156,7 @@ def lib():
         L.lux_sgm_pyramid_device.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.lux_sgm_batch.argtypes = [vp, i32, vp, vp]
         L.lux_sgm_batch_tune.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32]
+        L.lux_sgm_batch_set_schedule.argtypes = [vp, i32, vp, vp]
         L.lux_sgm_server_create.argtypes = [ctypes.c_int, u64, i32, ctypes.POINTER(vp)]
         L.lux_sgm_server_destroy.argtypes = [vp]
         L.lux_sgm_server_last_error.argtypes = [vp]
@@ -388,6 +391,13 @@ def batch(ctx, pairs, stream=0):
     _check(lib().lux_sgm_batch(ctx.handle, len(pairs), arr, ctypes.c_void_p(stream or None)), ctx.handle)
 
 
+def batch_set_schedule(ctx, pairs, *, executor=0, order=0, branches=4, bands=0):
+    """Use this schedule for batches of these pairs' shapes (lux_sgm_batch_set_schedule)."""
+    arr = (Pair * len(pairs))(*pairs)
+    sch = Schedule(bands, branches, 0.0, executor, order)
+    _check(lib().lux_sgm_batch_set_schedule(ctx.handle, len(pairs), arr, ctypes.byref(sch)), ctx.handle)
+
+
 def batch_tune(ctx, pairs, stream=0, reps=2):
     """Time every candidate schedule on this batch; returns (best, [tried...]) as dicts."""
     arr = (Pair * len(pairs))(*pairs)
@@ -397,7 +407,7 @@ def batch_tune(ctx, pairs, stream=0, reps=2):
                                  ctypes.byref(best), tried, 32)
     if n < 0:
         _check(n, ctx.handle)
-    as_dict = lambda s: dict(bands=s.bands, branches=s.branches, ms=s.ms)  # noqa: E731
+    as_dict = lambda s: dict(bands=s.bands, branches=s.branches, ms=s.ms, executor=s.executor, order=s.order)  # noqa: E731
     return as_dict(best), [as_dict(tried[i]) for i in range(min(n, 32))]
 
 

@@ -152,6 +152,19 @@ def test_batched_capture_matches_alone_and_oracle():
             for k, ((d, s, v), a) in enumerate(zip(outs, alone)):
                 got = dict(disp=d.cpu().numpy().view(np.uint16), sub=s.cpu().numpy(), valid=v.cpu().numpy())
                 _same(got, a, f"batch {rep} pair {k} vs alone")
+        for executor, order in ((1, 1), (7, 1)):  # the persistent executor at this size, whatever the tuner chose
+            sgm.batch_set_schedule(ctx, pairs, executor=executor, order=order)
+            for d, s, v in outs:
+                d.fill_(-1)
+                s.fill_(-7.0)
+                v.fill_(9)
+            torch.cuda.synchronize()
+            _say(f"executor {executor} order {order}")
+            sgm.batch(ctx, pairs, stream=stream.cuda_stream)
+            stream.synchronize()
+            for k, ((d, s, v), a) in enumerate(zip(outs, alone)):
+                got = dict(disp=d.cpu().numpy().view(np.uint16), sub=s.cpu().numpy(), valid=v.cpu().numpy())
+                _same(got, a, f"executor {executor}/{order} pair {k} vs alone")
         del keep
     for k in (0, 6):  # one pair of each size against the oracle
         W, H, D, left, right = scenes[k]

@@ -39,7 +39,7 @@ def cases():
             p = _params(W, H, D, bits, cost, radius)
             r = sgm.rectified(c, left, right, D, bits=bits, cost=cost, radius=max(radius, 1), ad_cap=20, p1=8, p2=96,
                               tile=16)
-            out.append(dict(W=W, H=H, D=D, bits=bits, left=left, right=right, p=p, alone=r))
+            out.append(dict(W=W, H=H, D=D, bits=bits, cost=cost, left=left, right=right, p=p, alone=r))
     return out
 
 
@@ -114,6 +114,39 @@ def test_batch_tuned_schedules(cases):
         sgm.batch(ctx, pairs, stream=stream.cuda_stream)  # the tuned schedule
         stream.synchronize()
         _check_outs(outs, cases, "tuned batch")
+        del keep
+
+
+@pytest.mark.parametrize("executor,order", [(1, 0), (1, 1), (7, 0), (7, 1)])
+def test_executor_batch_matches_alone(cases, executor, order):
+    """The persistent executor (one launch, or one per pass group; both item orders): every census pair of the mix,
+    replayed three times, equals the pair alone bit for bit."""
+    torch = pytest.importorskip("torch")
+    sel = [c for c in cases if c["cost"] == "census"]
+    with sgm.Context(0) as ctx:
+        pairs, outs, keep = _device_pairs(torch, sel)
+        sgm.batch_set_schedule(ctx, pairs, executor=executor, order=order)
+        stream = torch.cuda.Stream()
+        for rep in range(3):
+            print(f"executor {executor} order {order} run {rep}", flush=True)
+            for d, s, v in outs:
+                d.fill_(-1)
+                s.fill_(-7.0)
+                v.fill_(9)
+            torch.cuda.synchronize()
+            sgm.batch(ctx, pairs, stream=stream.cuda_stream)
+            stream.synchronize()
+            _check_outs(outs, sel, f"executor {executor}/{order} run {rep}")
+        del keep
+
+
+def test_executor_refuses_other_pairs(cases):
+    """AD pairs (and any pair whose path costs do not fit a byte) stay on the graph branches."""
+    torch = pytest.importorskip("torch")
+    with sgm.Context(0) as ctx:
+        pairs, outs, keep = _device_pairs(torch, cases)
+        with pytest.raises(sgm.NotSupported):
+            sgm.batch_set_schedule(ctx, pairs, executor=1)
         del keep
 
 
