@@ -186,8 +186,10 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
         {"seedmins", (size_t)3 * g.W * 2},
         {"cols", (size_t)4 * g.H * g.cs * 2},
         {"colmins", (size_t)4 * g.H * 2},
-        {"replay_partial", maxvis * g.cs * 2},
-        {"replay_chain", maxvis * g.cs * 2},
+        // one half per pass: the persistent executor runs the two passes' replays in one launch, where a buffer
+        // rewritten after another XCD read it could leave that XCD's cache stale
+        {"replay_partial", 2 * maxvis * g.cs * 2},
+        {"replay_chain", 2 * maxvis * g.cs * 2},
         {"wta_dr", cells * 2},
         {"wta_dr0", (size_t)g.H * 4},
         {"band_boundary", (size_t)6 * g.W * g.cs * 2},

@@ -6,9 +6,13 @@
 // visit, one census row block ...), so every result is the plain request's bit for bit. An item whose stage reads
 // another stage's results waits on that stage's completion counter (same pair) with a bounded spin: tickets are
 // handed out in an order where every prerequisite's items come first, so the smallest unfinished ticket can always
-// run and every wait ends. Results cross wavefronts and XCDs within a launch through agent-scope release / acquire
-// fences around the counters (an L2 write-back by the producer, an L2 invalidate by the consumer); across launches
-// the kernel boundary orders them.
+// run and every wait ends. Across launches the kernel boundary orders results. Within a launch they cross XCDs,
+// whose L2 caches write back lazily and are not coherent with each other, through flush epochs (XSync): the item
+// that completes a stage raises an epoch, each XCD writes its L2 back once per epoch (the next of its wavefronts
+// to reach a scheduling point does it) and acknowledges, and a consumer starts once every XCD has acknowledged the
+// epoch of the stage it reads (or has no wavefront left). That is one L2 write-back per stage and XCD instead of
+// one per item. No consumer needs an invalidate: within a run no line is read before the item that writes it has
+// finished (each pass has its own border-replay buffers), so no cache holds a stale copy.
 #include "exec.h"
 #include "kernels_impl.h"
 
@@ -105,11 +109,53 @@ struct ExecSeg {
     int job, stage, first;
 };
 
+// Flush epochs (see the file comment): req counts the epochs raised, ack[x] is the last epoch XCD x has written its
+// L2 back for, live[x] its running wavefronts (an XCD with none holds nothing unwritten: each leaves after a
+// write-back).
+constexpr int kXcds = 8;
+struct XSync {
+    uint32_t req;
+    uint32_t ack[kXcds];
+    int32_t live[kXcds];
+};
+
+__device__ __forceinline__ int xcc_id()
+{
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(v));
+    return (int)(v & (kXcds - 1));
+}
+
+__device__ __forceinline__ uint32_t ald(const uint32_t* p)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)__hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// This XCD's share of the protocol: write the L2 back if an epoch was raised that it has not acknowledged.
+__device__ __forceinline__ void xsync_poll(XSync* y, int x, int lane)
+{
+    const uint32_t r = ald(&y->req);
+    if (ald(&y->ack[x]) >= r) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // buffer_wbl2: every dirty line of this XCD's L2 to memory
+    if (lane == 0) __hip_atomic_fetch_max(&y->ack[x], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Whether epoch e has reached memory from every XCD.
+__device__ __forceinline__ bool xsync_flushed(const XSync* y, uint32_t e)
+{
+    for (int x = 0; x < kXcds; x++)
+        if (ald(&y->ack[x]) < e && (int)ald((const uint32_t*)&y->live[x]) > 0) return false;
+    return true;
+}
+
 struct ExecLaunch {
     const ExecJob* jobs;
     const ExecSeg* segs;
     uint32_t* ticket;  // this launch's ticket counter (zero before the launch)
     uint32_t* done;    // [job][X_NSTAGES] completed items (zero before the first launch of a run)
+    uint32_t* epoch;   // [job][X_NSTAGES] the flush epoch raised when the stage completed (0: not yet)
+    XSync* sync;
     int* err;          // a wait gave up
     uint32_t total;    // tickets of this launch
     int nseg;
@@ -134,19 +180,21 @@ __device__ __forceinline__ T uload(const T* p)
 
 constexpr uint32_t kSpinLimit = 1u << 23;  // polls of one wait, about 2 s with the sleeps below
 
-// Waits until every stage in `pre` of job `job` is complete; false if a wait gave up (the error word is set then).
-__device__ __forceinline__ bool exec_wait(const ExecLaunch& L, const ExecJob* J, int job, uint32_t pre, int lane)
+// Waits until every stage in `pre` of job `job` is complete and flushed from every XCD; false if a wait gave up
+// (the error word is set then). Waiting wavefronts keep their own XCD's share of the protocol going.
+__device__ __forceinline__ bool exec_wait(const ExecLaunch& L, const ExecJob* J, int job, uint32_t pre, int x, int lane)
 {
     for (int s = 0; s < X_NSTAGES; s++) {
         if (!(pre & xbit(s))) continue;
         const uint32_t need = (uint32_t)__builtin_amdgcn_readfirstlane((int)J->n[s]);
-        uint32_t* c = L.done + (size_t)job * X_NSTAGES + s;
+        const size_t k0 = (size_t)job * X_NSTAGES + s;
         for (uint32_t k = 0;; k++) {
-            const uint32_t v =
-                (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (v >= need) break;
-            const int e = __builtin_amdgcn_readfirstlane(__hip_atomic_load(L.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (e) return false;
+            xsync_poll(L.sync, x, lane);
+            if (ald(L.done + k0) >= need) {
+                const uint32_t e = ald(L.epoch + k0);
+                if (e && xsync_flushed(L.sync, e)) break;
+            }
+            if (ald((const uint32_t*)L.err)) return false;
             if (k >= kSpinLimit) {
                 if (lane == 0) __hip_atomic_store(L.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return false;
@@ -154,8 +202,22 @@ __device__ __forceinline__ bool exec_wait(const ExecLaunch& L, const ExecJob* J,
             __builtin_amdgcn_s_sleep(4);
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
+}
+
+// An item is done: its stores are in this XCD's L2 (waitcnt), then the count; the item that completes a stage
+// another stage of this launch reads raises a flush epoch and records it with the stage.
+__device__ __forceinline__ void item_done(const ExecLaunch& L, const ExecJob* J, int job, int stage, int lane)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    if (lane == 0) {
+        const size_t k0 = (size_t)job * X_NSTAGES + stage;
+        const uint32_t old = __hip_atomic_fetch_add(L.done + k0, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((L.release & xbit(stage)) && old + 1 == J->n[stage]) {
+            const uint32_t e = __hip_atomic_fetch_add(&L.sync->req, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            __hip_atomic_store(L.epoch + k0, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // Rows [y0, y0 + kXCensusRows) of one image's census codes, 64 pixels of a row at a time.
@@ -266,11 +328,14 @@ template <uint32_t SM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SM == kAll ? 4 : 1, 8))) void sgm_exec(ExecLaunch L)
 {
     const int lane = threadIdx.x;
+    const int x = xcc_id();
+    if (lane == 0) __hip_atomic_fetch_add(&L.sync->live[x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
+        if (L.release) xsync_poll(L.sync, x, lane);
         uint32_t t = 0;
         if (lane == 0) t = __hip_atomic_fetch_add(L.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-        if (t >= L.total) return;
+        if (t >= L.total) break;
         int lo = 0, hi = L.nseg - 1;  // the last segment starting at or before t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -281,7 +346,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SM == kAll ?
         const ExecJob* J = L.jobs + sg.job;
         const int item = sg.first + (int)(t - sg.t0);
         const uint32_t pre = exec_prereq(sg.stage) & L.wait;
-        if (pre && !exec_wait(L, J, sg.job, pre, lane)) continue;  // gave up: the error word tells the host
+        if (pre && !exec_wait(L, J, sg.job, pre, x, lane)) continue;  // gave up: the error word tells the host
         if ((SM & xbit(X_CENSUS)) && sg.stage == X_CENSUS) {
             census_item(J, item, lane);
         } else {
@@ -292,11 +357,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SM == kAll ?
                 default: chain_item<SM, 2, true>(J, sg.stage, item, lane); break;
             }
         }
-        if (L.release & xbit(sg.stage)) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0)
-            __hip_atomic_fetch_add(L.done + (size_t)sg.job * X_NSTAGES + sg.stage, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        item_done(L, J, sg.job, sg.stage, lane);
     }
+    // leave with nothing of this wavefront's unwritten in the L2 (an XCD with no wavefront left counts as flushed)
+    if (L.release) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_fetch_add(&L.sync->live[x], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 const void* exec_kernel(uint32_t sm)
@@ -357,7 +422,7 @@ struct ExecBatch {
     std::vector<uint32_t> totals;
     std::vector<int> resident;               // each launch's kernel: one-wavefront workgroups the device holds
     void* dev = nullptr;                     // jobs | segments | done counters | tickets | error word
-    size_t seg_off = 0, done_off = 0, ticket_off = 0, err_off = 0, bytes = 0;
+    size_t seg_off = 0, done_off = 0, epoch_off = 0, sync_off = 0, ticket_off = 0, err_off = 0, bytes = 0;
 };
 
 bool exec_supported(const lux_sgm_rect_params* p, const Prepared& w)
@@ -433,6 +498,10 @@ ExecJob make_job(const ExecPair& q)
     r.colagg = b.bagg[1];
     r.cols = b.cols + colsz;
     r.colmins = b.colmins + colminsz;
+    // pass -1's border replay buffers: the second half (prepare() gives two, one per pass)
+    const size_t maxvis = (size_t)std::max(b.n_plus, b.n_minus) + 1;
+    r.partial = b.partial + maxvis * g.cs;
+    r.chainp = b.chainp + maxvis * g.cs;
     j.cm = r;
     j.cellvis_p = b.cellvis_plus;
     j.cellvis_m = b.cellvis_minus;
@@ -518,6 +587,8 @@ int exec_create(const ExecPair* pairs, int n, int groups, int order, int device,
     for (auto& v : bt->segs) nsegs += v.size();
     bt->seg_off = take(sizeof(ExecSeg) * std::max<size_t>(nsegs, 1));
     bt->done_off = take(sizeof(uint32_t) * (size_t)n * X_NSTAGES);
+    bt->epoch_off = take(sizeof(uint32_t) * (size_t)n * X_NSTAGES);
+    bt->sync_off = take(sizeof(XSync) * bt->stages.size());
     bt->ticket_off = take(sizeof(uint32_t) * bt->stages.size());
     bt->err_off = take(sizeof(int));
     bt->bytes = off;
@@ -560,6 +631,8 @@ hipError_t exec_enqueue(ExecBatch* b, int waves, hipStream_t s)
         L.segs = (const ExecSeg*)(base + so);
         L.ticket = (uint32_t*)(base + b->ticket_off) + k;
         L.done = (uint32_t*)(base + b->done_off);
+        L.epoch = (uint32_t*)(base + b->epoch_off);
+        L.sync = (XSync*)(base + b->sync_off) + k;
         L.err = (int*)(base + b->err_off);
         L.total = b->totals[k];
         L.nseg = (int)b->segs[k].size();
