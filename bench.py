@@ -13,11 +13,14 @@ after one warm-up; `cpu_baseline_c0` is the same for the 640x480 uint8 D=64 work
 
 More legs go into the same line:
   `batched`: one capture of 10 pairs (6 x 2080x1560 D=128, 4 x 3120x2340 D=192, independent seeds) as one graph
-             launch with the autotuned schedule, against the same pairs run one after another (plain drop-in);
+             launch with the autotuned schedule (graph branches, or the persistent executor in one launch or one
+             launch per pass group), against the same pairs run one after another (plain drop-in);
   `served`:  a mixed stream of depth requests (sizes uniform over 640x480 D64, 1040x780 D96, 2080x1560 D128,
              4160x3120 D256; host images in, host results out) through the request loop under a 96 GB budget:
              latency p50/p99 at Poisson arrivals of 40 req/s, served and plain (the plain drop-in serving the same
              arrivals in order, one at a time), and throughput at saturation against the plain drop-in in order;
+             the same stream again through the device-resident loop (`served.device_loop`: the scheduler a
+             persistent kernel, admission into a device pool of the budget);
   `coarse_to_fine`: 2000 3-level requests (1040x780 D=64 -> 2080x1560 +-16 -> 4160x3120 +-8) in bursts of 16
              through the request loop under a 192 GB budget, latency p50/p99 against plain sequential execution;
   `l16_pyramid*`: DenseStereoPyramid.Run at 2080x1560 (device form), its oracle on the host, and a plane span whose
@@ -341,49 +344,71 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
         plain_poi.append((time.perf_counter() - arrive) * 1e3)
     ctx.close()
 
-    srv = sgm.Server(0, int(budget_gb * 1e9), slots=slots)
+    def serve(srv):
+        """Warm-up, then the stream at saturation and at the Poisson arrivals through server srv."""
 
-    def run(arrivals):
-        """Submit in order (at arrival times if given, else keeping `window` in flight); collect on a thread."""
-        q = queue.Queue()
-        stats, free = [], {k: [] for k in scenes}
-        sem = threading.Semaphore(window if arrivals is None else n_req)
+        def run(arrivals):
+            """Submit in order (at arrival times if given, else keeping `window` in flight); collect on a thread."""
+            q = queue.Queue()
+            stats, free = [], {k: [] for k in scenes}
+            sem = threading.Semaphore(window if arrivals is None else n_req)
 
-        def collector():
-            while True:
-                item = q.get()
-                if item is None:
-                    return
-                t, k, outs = item
-                stats.append(srv.wait(t))
-                free[k].append(outs)
-                sem.release()
+            def collector():
+                while True:
+                    item = q.get()
+                    if item is None:
+                        return
+                    t, k, outs = item
+                    stats.append(srv.wait(t))
+                    free[k].append(outs)
+                    sem.release()
 
-        th = threading.Thread(target=collector)
-        th.start()
-        t_start = time.perf_counter()
-        for i, k in enumerate(sizes):
-            if arrivals is not None:
-                delay = t_start + arrivals[i] - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-            sem.acquire()
-            outs = free[k].pop() if free[k] else outputs(k)
-            t = srv.submit(params[k], *scenes[k][i % 2], disp=outs[0], sub=outs[1], valid=outs[2])
-            q.put((t, k, outs))
-        q.put(None)
-        th.join()
-        return time.perf_counter() - t_start, stats
+            th = threading.Thread(target=collector)
+            th.start()
+            t_start = time.perf_counter()
+            for i, k in enumerate(sizes):
+                if arrivals is not None:
+                    delay = t_start + arrivals[i] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                sem.acquire()
+                outs = free[k].pop() if free[k] else outputs(k)
+                t = srv.submit(params[k], *scenes[k][i % 2], disp=outs[0], sub=outs[1], valid=outs[2])
+                q.put((t, k, outs))
+            q.put(None)
+            th.join()
+            return time.perf_counter() - t_start, stats
 
-    for k in scenes:  # warm every slot's workspace and the pinned staging of a full window, outputs included
-        keep = [outputs(k) for _ in range(window)]
-        run_w = [srv.submit(params[k], *scenes[k][0], disp=o[0], sub=o[1], valid=o[2]) for o in keep]
-        for t in run_w:
-            srv.wait(t)
-    sat_s, sat_stats = run(None)
-    poi_s, poi_stats = run(arrivals)
-    info = srv.info()
-    srv.close()
+        for k in scenes:  # warm every slot's workspace and the pinned staging of a full window, outputs included
+            keep = [outputs(k) for _ in range(window)]
+            run_w = [srv.submit(params[k], *scenes[k][0], disp=o[0], sub=o[1], valid=o[2]) for o in keep]
+            for t in run_w:
+                srv.wait(t)
+        sat_s, sat_stats = run(None)
+        print(f"bench: served at saturation {sat_s:.2f} s", file=sys.stderr, flush=True)
+        poi_s, poi_stats = run(arrivals)
+        print(f"bench: served at Poisson arrivals {poi_s:.2f} s", file=sys.stderr, flush=True)
+        info = srv.info()
+        srv.close()
+        return sat_s, sat_stats, poi_s, poi_stats, info
+
+    sat_s, sat_stats, poi_s, poi_stats, info = serve(sgm.Server(0, int(budget_gb * 1e9), slots=slots))
+    # the same stream through the device-resident loop (its scheduler a persistent kernel, the pool the budget)
+    d_sat_s, d_sat_stats, d_poi_s, d_poi_stats, d_info = serve(
+        sgm.Server(0, int(budget_gb * 1e9), slots=slots, device_loop=True))
+
+    def pct(v, q):
+        return round(float(np.percentile(np.array([st["total_ms"] for st in v]), q)), 3)
+
+    device_loop = {"poisson": {"rate_per_s": rate, "p50_ms": pct(d_poi_stats, 50), "p99_ms": pct(d_poi_stats, 99),
+                               "wall_s": round(d_poi_s, 3)},
+                   "saturation": {"window": window, "wall_s": round(d_sat_s, 3),
+                                  "gcells_s": round(cells * 8 / d_sat_s / 1e9, 3),
+                                  "req_per_s": round(n_req / d_sat_s, 3), "p50_ms": pct(d_sat_stats, 50),
+                                  "p99_ms": pct(d_sat_stats, 99)},
+                   "vs_plain_throughput": round(plain_s / d_sat_s, 3),
+                   "bytes_peak_gb": round(d_info["bytes_peak"] / 1e9, 3), "failed": d_info["failed"],
+                   "latency": "submit -> results in host memory"}
     lat = np.array([st["total_ms"] for st in poi_stats])
     sat_lat = np.array([st["total_ms"] for st in sat_stats])
     return {"workload": f"{n_req} requests, sizes uniform over 640x480 D64 / 1040x780 D96 / 2080x1560 D128 / "
@@ -405,7 +430,8 @@ def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
                                   "p99_ms": round(float(np.percentile(plain_poi, 99)), 3),
                                   "latency": "arrival -> results in host memory, requests served in arrival order"}},
             "served_vs_plain_throughput": round(plain_s / sat_s, 3),
-            "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"]}
+            "bytes_peak_gb": round(info["bytes_peak"] / 1e9, 3), "failed": info["failed"],
+            "device_loop": device_loop}
 
 
 PYR_W, PYR_H, PYR_D, PYR_RADII = 4160, 3120, 256, (16, 8)

@@ -439,6 +439,19 @@ typedef struct {
 
 /* hbm_budget bounds the device bytes of all slot workspaces together (0 = no bound); slots in 1..16. */
 int lux_sgm_server_create(int device, uint64_t hbm_budget, int32_t slots, lux_sgm_server** out);
+
+#define LUX_SGM_SERVER_DEVICE_LOOP 1u /* the request loop runs on the device (below) */
+
+/* lux_sgm_server_create with flags. LUX_SGM_SERVER_DEVICE_LOOP: the loop that admits and schedules requests is a
+ * persistent kernel rather than a host thread. Requests go through a ring in pinned host memory; the kernel admits
+ * them in arrival order while one of `slots` is free and the request's buffers fit in a device pool of hbm_budget
+ * bytes (required), runs the stages of every admitted request as work items, smallest request first, so their
+ * work overlaps on the device, and marks each done in the ring. Host images and outputs are staged by the kernel
+ * itself. It takes rectified census requests whose path costs fit a byte (up to 256 planes); others fail at
+ * submit with LUX_SGM_EUNSUPPORTED, as do coarse-to-fine requests and cancel. Results equal lux_sgm_rectified's.
+ * The kernel leaves after 20 ms with nothing to do and is launched again by the next submit. No counterpart in the
+ * reference (its job queue runs on the host, Lux.Library/Core/JobQueue.cs:23-31). */
+int lux_sgm_server_create_ex(int device, uint64_t hbm_budget, int32_t slots, uint32_t flags, lux_sgm_server** out);
 /* Waits for running requests, cancels pending ones, frees everything. */
 int lux_sgm_server_destroy(lux_sgm_server* srv);
 const char* lux_sgm_server_last_error(const lux_sgm_server* srv);

@@ -126,6 +126,15 @@ uint64_t rect_workspace_bytes(const lux_sgm_rect_params* p);
 // Plans are pure functions of (W, H, T, xsw): one process-wide cache.
 int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::string* err);
 
+// The buffers of a request in ws (Workspace::get: allocated, or placed in an arena) and w's buffer table over them,
+// with no device work. visits (may be null): the plan's visit lists [plus | minus | cellvis plus | cellvis minus]
+// are already on the device there, else they get a workspace buffer of their own (not filled here). named (may be
+// null) receives every buffer by name (the staging buffers img_l, img_r, out_* when host_img_bytes > 0).
+int prepare_layout(Workspace& ws, const Geometry& g, const Plan& plan, bool census, bool pv8, size_t host_img_bytes,
+                   const int* visits, Prepared* w, std::map<std::string, void*>* named, std::string* err);
+// The plan's visit lists into vis (plan_ints ints) on s.
+int upload_plan(const Plan& plan, int* vis, hipStream_t s, std::string* err);
+size_t plan_ints(const Plan& plan);
 // Allocate the workspace for geometry g and upload the plan's visit lists (copies from host memory on s).
 // Nothing here may be captured into a graph; enqueue_* below may.
 int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err,

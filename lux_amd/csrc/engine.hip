@@ -184,12 +184,12 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
         {"agg", pv8 ? 0 : cells * g.cs * 2},  // with byte path volumes only when the caller asks for the aggregate
         {"seeds", (size_t)3 * g.W * g.cs * 2},
         {"seedmins", (size_t)3 * g.W * 2},
-        {"cols", (size_t)4 * g.H * g.cs * 2},
-        {"colmins", (size_t)4 * g.H * 2},
+        {"cols", 2 * cols_half(g) * 2},
+        {"colmins", 2 * colmins_half(g) * 2},
         // one half per pass: the persistent executor runs the two passes' replays in one launch, where a buffer
         // rewritten after another XCD read it could leave that XCD's cache stale
-        {"replay_partial", 2 * maxvis * g.cs * 2},
-        {"replay_chain", 2 * maxvis * g.cs * 2},
+        {"replay_partial", 2 * replay_half(g, maxvis) * 2},
+        {"replay_chain", 2 * replay_half(g, maxvis) * 2},
         {"wta_dr", cells * 2},
         {"wta_dr0", (size_t)g.H * 4},
         {"band_boundary", (size_t)6 * g.W * g.cs * 2},
@@ -204,7 +204,7 @@ std::vector<BufferNeed> request_buffers(const Geometry& g, const Plan& plan, boo
     }
     if (pv8) {
         v.push_back({"path_bytes", 7 * cells * g.cs});
-        v.push_back({"border_agg", (size_t)2 * g.H * g.cs * 2});
+        v.push_back({"border_agg", 2 * border_half(g) * 2});
     }
     if (host_img_bytes) {
         v.push_back({"img_l", host_img_bytes});
@@ -257,42 +257,21 @@ int plan_for(int W, int H, int T, int xsw, std::shared_ptr<Plan>* out, std::stri
     return 0;
 }
 
-int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err,
-            bool pv8)
+int prepare_layout(Workspace& ws, const Geometry& g, const Plan& plan, bool census, bool pv8, size_t host_img_bytes,
+                   const int* visits, Prepared* w, std::map<std::string, void*>* named, std::string* err)
 {
-    std::shared_ptr<Plan> plan;
-    int rc = plan_for(g.W, g.H, T, xsw, &plan, err);
-    if (rc) return rc;
     std::map<std::string, void*> ptr;
-    for (const BufferNeed& nb : request_buffers(g, *plan, census, 0, pv8)) {
+    for (const BufferNeed& nb : request_buffers(g, plan, census, host_img_bytes, pv8)) {
+        if (visits && nb.name == visits_name(plan)) continue;  // the caller keeps the visit lists elsewhere
         void* q;
-        if ((rc = ws.get(nb.name.c_str(), nb.bytes, &q, err))) return rc;
+        int rc = ws.get(nb.name.c_str(), nb.bytes, &q, err);
+        if (rc) return rc;
         ptr[nb.name] = q;
     }
     void *raw = ptr["raw"], *agg = ptr["agg"], *seeds = ptr["seeds"], *seedmins = ptr["seedmins"], *cols = ptr["cols"],
-         *colmins = ptr["colmins"], *vis = ptr[visits_name(*plan)], *partial = ptr["replay_partial"], *drb = ptr["wta_dr"],
-         *dr0 = ptr["wta_dr0"], *bnd = ptr["band_boundary"], *bndmin = ptr["band_boundary_min"], *sync = ptr["band_sync"];
-    const int tiles = band_tiles(g.W, g.H);
-    if (sync != ws.sync_zeroed) {  // a new buffer: no flag may hold a live epoch
-        hipError_t ez = launch_fill_u32((uint32_t*)sync, 4 + 2 * (size_t)tiles, 0u, s);
-        if (ez != hipSuccess) return hip_status(ez, "zero band flags", err);
-        ws.sync_zeroed = sync;
-    }
-    if (++ws.epoch <= 0) ws.epoch = 1;
-    if (!ws.filled.count(visits_name(*plan))) {  // upload the visit lists once per plan
-        const size_t np = plan->plus.size(), nm = plan->minus.size();
-        hipError_t e = hipMemcpyAsync(vis, plan->plus.data(), np * sizeof(int), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync((int*)vis + np, plan->minus.data(), nm * sizeof(int), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync((int*)vis + np + nm, plan->cellvis_plus.data(), plan->cellvis_plus.size() * sizeof(int),
-                               hipMemcpyHostToDevice, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync((int*)vis + np + nm + 2 * (size_t)g.H, plan->cellvis_minus.data(),
-                               plan->cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return hip_status(e, "upload plan", err);
-        ws.filled.insert(visits_name(*plan));
-    }
+         *colmins = ptr["colmins"], *partial = ptr["replay_partial"], *drb = ptr["wta_dr"], *dr0 = ptr["wta_dr0"],
+         *bnd = ptr["band_boundary"], *bndmin = ptr["band_boundary_min"], *sync = ptr["band_sync"];
+    const int* vis = visits ? visits : (const int*)ptr[visits_name(plan)];
     if (census) {
         w->b.cl = (const uint64_t*)ptr["census_l"];
         w->b.cr = (const uint64_t*)ptr["census_r"];
@@ -311,7 +290,7 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     w->b.err = (int*)sync;
     w->b.ticket = (int*)sync + 2;
     w->b.flags = (int*)sync + 4;
-    w->b.max_tiles = tiles;
+    w->b.max_tiles = band_tiles(g.W, g.H);
     w->b.epoch = ws.epoch;
     w->b.seeds = (uint16_t*)seeds;
     w->b.seedmins = (uint16_t*)seedmins;
@@ -319,13 +298,13 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
     w->b.colmins = (uint16_t*)colmins;
     w->b.partial = (uint16_t*)partial;
     w->b.chainp = (uint16_t*)ptr["replay_chain"];
-    w->b.visits_plus = (const int*)vis;
-    w->b.n_plus = (int)plan->plus.size() / 4;
-    w->b.visits_minus = (const int*)vis + plan->plus.size();
-    w->b.n_minus = (int)plan->minus.size() / 4;
-    w->b.cellvis_plus = (const int*)vis + plan->plus.size() + plan->minus.size();
+    w->b.visits_plus = vis;
+    w->b.n_plus = (int)plan.plus.size() / 4;
+    w->b.visits_minus = vis + plan.plus.size();
+    w->b.n_minus = (int)plan.minus.size() / 4;
+    w->b.cellvis_plus = vis + plan.plus.size() + plan.minus.size();
     w->b.cellvis_minus = w->b.cellvis_plus + 2 * (size_t)g.H;
-    w->b.seed_row = plan->seed_row;
+    w->b.seed_row = plan.seed_row;
     w->b.disp_tmp = (uint16_t*)dtmp;
     // constant P2: the kernels' guidance loads read word 0 of an allocation that always exists
     w->b.ap = AdaptP2();
@@ -337,7 +316,50 @@ int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipSt
         for (int k = 0; k < 7; k++) w->b.pv[k] = (uint8_t*)ptr["path_bytes"] + k * vol;
         w->b.pvs = g.cs;
         w->b.bagg[0] = (uint16_t*)ptr["border_agg"];
-        w->b.bagg[1] = w->b.bagg[0] + (size_t)g.H * g.cs;
+        w->b.bagg[1] = w->b.bagg[0] + border_half(g);
+    }
+    if (named) *named = ptr;
+    return 0;
+}
+
+int upload_plan(const Plan& plan, int* vis, hipStream_t s, std::string* err)
+{
+    const size_t np = plan.plus.size(), nm = plan.minus.size();
+    hipError_t e = hipMemcpyAsync(vis, plan.plus.data(), np * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vis + np, plan.minus.data(), nm * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(vis + np + nm, plan.cellvis_plus.data(), plan.cellvis_plus.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(vis + np + nm + plan.cellvis_plus.size(), plan.cellvis_minus.data(),
+                           plan.cellvis_minus.size() * sizeof(int), hipMemcpyHostToDevice, s);
+    return e == hipSuccess ? 0 : hip_status(e, "upload plan", err);
+}
+
+size_t plan_ints(const Plan& plan)
+{
+    return plan.plus.size() + plan.minus.size() + plan.cellvis_plus.size() + plan.cellvis_minus.size();
+}
+
+int prepare(Workspace& ws, const Geometry& g, int T, int xsw, bool census, hipStream_t s, Prepared* w, std::string* err,
+            bool pv8)
+{
+    std::shared_ptr<Plan> plan;
+    int rc = plan_for(g.W, g.H, T, xsw, &plan, err);
+    if (rc) return rc;
+    std::map<std::string, void*> ptr;
+    if ((rc = prepare_layout(ws, g, *plan, census, pv8, 0, nullptr, w, &ptr, err))) return rc;
+    void* sync = ptr["band_sync"];
+    if (sync != ws.sync_zeroed) {  // a new buffer: no flag may hold a live epoch
+        hipError_t ez = launch_fill_u32((uint32_t*)sync, 4 + 2 * (size_t)w->b.max_tiles, 0u, s);
+        if (ez != hipSuccess) return hip_status(ez, "zero band flags", err);
+        ws.sync_zeroed = sync;
+    }
+    if (++ws.epoch <= 0) ws.epoch = 1;
+    w->b.epoch = ws.epoch;
+    if (!ws.filled.count(visits_name(*plan))) {  // upload the visit lists once per plan
+        if ((rc = upload_plan(*plan, (int*)ptr[visits_name(*plan)], s, err))) return rc;
+        ws.filled.insert(visits_name(*plan));
     }
     return 0;
 }

@@ -13,6 +13,16 @@ struct Geometry {
 };
 Geometry geometry(int W, int H, int D, int align);
 
+// Per-pass buffers (neighbour-column states and minima, border replay partial sums and chain costs, border-column
+// sums) keep pass -1's half this many u16 after pass +1's: whole 256-byte blocks, so no cache line holds both halves.
+// (The persistent executor reads one pass's half while the other's is still to be written, and a line cached by
+// such a read would be stale in the other half.)
+inline size_t pass_half_u16(size_t n) { return (n + 127) & ~(size_t)127; }
+inline size_t cols_half(const Geometry& g) { return pass_half_u16((size_t)2 * g.H * g.cs); }
+inline size_t colmins_half(const Geometry& g) { return pass_half_u16((size_t)2 * g.H); }
+inline size_t border_half(const Geometry& g) { return pass_half_u16((size_t)g.H * g.cs); }
+inline size_t replay_half(const Geometry& g, size_t maxvis) { return pass_half_u16(maxvis * g.cs); }
+
 // Image-adaptive P2 (DenseStereoSgm.cs:267-274): per path step, P2 = (int)(base * fastexp2(-S)) with
 // S = (|d0| + |d2|) + |d1|, d_c = (g_c - q_c) * w_c, g the cell's guidance and q the guidance of the cell whose
 // state the step reads (0 for the initial fill). guid: [H][W] RGBA8 as u32 (never null: a one-word dummy

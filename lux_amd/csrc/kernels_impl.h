@@ -2582,7 +2582,7 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
     RowArgs ra;
     ReplayArgs r;
     sgm_base_args(g, b, o, p1, p2, pv, a, ra, r);
-    const size_t colsz = (size_t)2 * g.H * g.cs, colminsz = (size_t)2 * g.H;
+    const size_t colsz = cols_half(g), colminsz = colmins_half(g);
     auto blocks = [&](int chains) { return (chains + 3) / 4; };
     // one chain a wavefront: one-wavefront workgroups when the request has the device to itself (SgmBuffers::solo)
     const unsigned wg = b.solo ? 64u : (unsigned)kBlock;

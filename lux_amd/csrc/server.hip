@@ -22,59 +22,16 @@
 #include <vector>
 
 #include "../../include/lux_sgm.h"
+#include "dloop.h"
 #include "engine.h"
+#include "pinned.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kLightBytes = 1536ull << 20;  // 1.5 GB: a light request may pass a heavy one that waits
-constexpr size_t kPinnedPoolCap = 4ull << 30;    // pinned staging kept for reuse
-
-// Pinned host staging, pooled by size (hipHostMalloc is slow; requests of one size class reuse blocks).
-struct PinnedPool {
-    std::mutex mu;
-    std::multimap<size_t, void*> free_blocks;
-    size_t held = 0;
-
-    void* get(size_t n, size_t* got)
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            auto it = free_blocks.lower_bound(n);
-            if (it != free_blocks.end() && it->first <= 2 * n + (1 << 20)) {
-                void* p = it->second;
-                *got = it->first;
-                held -= it->first;
-                free_blocks.erase(it);
-                return p;
-            }
-        }
-        void* p = nullptr;
-        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) return nullptr;
-        *got = n;
-        return p;
-    }
-    void put(void* p, size_t n)
-    {
-        if (!p) return;
-        std::unique_lock<std::mutex> lk(mu);
-        if (held + n > kPinnedPoolCap) {
-            lk.unlock();
-            (void)hipHostFree(p);
-            return;
-        }
-        free_blocks.emplace(n, p);
-        held += n;
-    }
-    void clear()
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        for (auto& kv : free_blocks) (void)hipHostFree(kv.second);
-        free_blocks.clear();
-        held = 0;
-    }
-};
+using lux::PinnedPool;
 
 struct Request {
     uint64_t ticket = 0;
@@ -115,6 +72,7 @@ struct CallbackArg;
 
 struct lux_sgm_server {
     int device = 0;
+    lux::DLoop* dloop = nullptr;  // LUX_SGM_SERVER_DEVICE_LOOP: the device-resident loop serves every call
     lux::Budget budget;
     std::vector<std::unique_ptr<Slot>> slots;
     std::deque<std::shared_ptr<Request>> pending;
@@ -356,6 +314,35 @@ int submit(lux_sgm_server* s, std::shared_ptr<Request> r, const void* left, cons
 
 extern "C" {
 
+int lux_sgm_server_create_ex(int device, uint64_t hbm_budget, int32_t nslots, uint32_t flags, lux_sgm_server** out)
+{
+    if (!(flags & LUX_SGM_SERVER_DEVICE_LOOP)) return lux_sgm_server_create(device, hbm_budget, nslots, out);
+    if (!out) {
+        g_server_create_error = "null out pointer";
+        return LUX_SGM_EINVAL;
+    }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        g_server_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return LUX_SGM_EDEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_server_create_error = "device index out of range";
+        return LUX_SGM_EINVAL;
+    }
+    lux::DLoop* d = nullptr;
+    int rc = lux::dloop_create(device, hbm_budget, nslots, &d, &g_server_create_error);
+    if (rc) return rc;
+    auto* s = new lux_sgm_server();
+    s->device = device;
+    s->budget.limit = hbm_budget;
+    s->dloop = d;
+    *out = s;
+    return LUX_SGM_OK;
+}
+
 int lux_sgm_server_create(int device, uint64_t hbm_budget, int32_t nslots, lux_sgm_server** out)
 {
     if (!out) {
@@ -404,6 +391,11 @@ int lux_sgm_server_create(int device, uint64_t hbm_budget, int32_t nslots, lux_s
 int lux_sgm_server_destroy(lux_sgm_server* s)
 {
     if (!s) return LUX_SGM_EINVAL;
+    if (s->dloop) {
+        lux::dloop_destroy(s->dloop);
+        delete s;
+        return LUX_SGM_OK;
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         for (auto& r : s->pending) {
@@ -442,6 +434,10 @@ int lux_sgm_server_submit(lux_sgm_server* s, const lux_sgm_rect_params* p, int32
 {
     if (!s || !ticket) return LUX_SGM_EINVAL;
     std::string err;
+    if (s->dloop) {
+        int rc = lux::dloop_submit(s->dloop, p, mem, left, right, disp, sub, valid, ticket, &err);
+        return rc ? fail(s, rc, err) : LUX_SGM_OK;
+    }
     int rc = lux::check_rect(p, &err);
     if (rc) return fail(s, rc, err);
     if (mem != LUX_SGM_MEM_HOST && mem != LUX_SGM_MEM_DEVICE) return fail(s, LUX_SGM_EINVAL, "unknown memory kind");
@@ -465,6 +461,7 @@ int lux_sgm_server_submit_pyramid(lux_sgm_server* s, const lux_sgm_pyramid_param
 {
     if (!s || !ticket) return LUX_SGM_EINVAL;
     std::string err;
+    if (s->dloop) return fail(s, LUX_SGM_EUNSUPPORTED, "coarse-to-fine requests are the host dispatcher's");
     int rc = lux::check_pyramid(p, &err);
     if (rc) return fail(s, rc, err);
     if (mem != LUX_SGM_MEM_HOST && mem != LUX_SGM_MEM_DEVICE) return fail(s, LUX_SGM_EINVAL, "unknown memory kind");
@@ -535,6 +532,11 @@ extern "C" {
 int lux_sgm_server_poll(lux_sgm_server* s, uint64_t ticket, int32_t* state)
 {
     if (!s || !state) return LUX_SGM_EINVAL;
+    if (s->dloop) {
+        std::string err;
+        int rc = lux::dloop_poll(s->dloop, ticket, state, &err);
+        return rc ? fail(s, rc, err) : LUX_SGM_OK;
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return fail(s, LUX_SGM_EINVAL, "unknown ticket");
@@ -545,6 +547,11 @@ int lux_sgm_server_poll(lux_sgm_server* s, uint64_t ticket, int32_t* state)
 int lux_sgm_server_wait(lux_sgm_server* s, uint64_t ticket, int32_t timeout_ms, lux_sgm_request_stats* stats)
 {
     if (!s) return LUX_SGM_EINVAL;
+    if (s->dloop) {
+        std::string err;
+        int rc = lux::dloop_wait(s->dloop, ticket, timeout_ms, stats, &err);
+        return rc ? fail(s, rc, err) : LUX_SGM_OK;
+    }
     std::shared_ptr<Request> r;
     {
         std::unique_lock<std::mutex> lk(s->mu);
@@ -586,6 +593,7 @@ int lux_sgm_server_wait(lux_sgm_server* s, uint64_t ticket, int32_t timeout_ms, 
 int lux_sgm_server_cancel(lux_sgm_server* s, uint64_t ticket)
 {
     if (!s) return LUX_SGM_EINVAL;
+    if (s->dloop) return fail(s, LUX_SGM_EUNSUPPORTED, "the device-resident loop admits requests as they come");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return fail(s, LUX_SGM_EINVAL, "unknown ticket");
@@ -603,6 +611,7 @@ int lux_sgm_server_cancel(lux_sgm_server* s, uint64_t ticket)
 int lux_sgm_server_info_get(lux_sgm_server* s, lux_sgm_server_info* info)
 {
     if (!s || !info) return LUX_SGM_EINVAL;
+    if (s->dloop) return lux::dloop_info(s->dloop, info);
     std::lock_guard<std::mutex> lk(s->mu);
     std::memset(info, 0, sizeof *info);
     info->submitted = s->submitted;

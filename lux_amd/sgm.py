@@ -26,13 +26,15 @@ EXPORTS = ("lux_sgm_version", "lux_sgm_create", "lux_sgm_destroy", "lux_sgm_last
            "lux_sgm_sweep_cost", "lux_sgm_range_from_previous", "lux_sgm_dense_layer", "lux_sgm_dense_pyramid",
            "lux_sgm_dense_pyramid_defaults", "lux_sgm_dense_pyramid_device",
            "lux_sgm_pyramid_bytes", "lux_sgm_pyramid", "lux_sgm_pyramid_device",
-           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_batch_set_schedule", "lux_sgm_server_create", "lux_sgm_server_destroy",
+           "lux_sgm_batch", "lux_sgm_batch_tune", "lux_sgm_batch_set_schedule", "lux_sgm_server_create", "lux_sgm_server_create_ex",
+           "lux_sgm_server_destroy",
            "lux_sgm_server_last_error", "lux_sgm_server_submit", "lux_sgm_server_submit_pyramid", "lux_sgm_server_poll",
            "lux_sgm_server_wait",
            "lux_sgm_server_cancel", "lux_sgm_server_info_get", "lux_sgm_planes", "lux_sgm_flow_matrix",
            "lux_sgm_homography_sqrt", "lux_sgm_rcpss_bits", "lux_sgm_rsqrtss_bits", "lux_sgm_calib_last_error")
 MEM_HOST, MEM_DEVICE = 0, 1
 REQ_PENDING, REQ_RUNNING, REQ_DONE = 0, 1, 2
+SERVER_DEVICE_LOOP = 1
 OPT_FUSED_BANDS = 1
 OPT_LITERAL = 2
 STAGES = ("census", "rows+", "vd+1", "vd+2", "vd+3", "replay+", "vd-1", "vd-2", "vd-3", "rows-", "replay-", "finish")
@@ -158,6 +160,7 @@ def lib():
         L.lux_sgm_batch_tune.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32]
         L.lux_sgm_batch_set_schedule.argtypes = [vp, i32, vp, vp]
         L.lux_sgm_server_create.argtypes = [ctypes.c_int, u64, i32, ctypes.POINTER(vp)]
+        L.lux_sgm_server_create_ex.argtypes = [ctypes.c_int, u64, i32, ctypes.c_uint32, ctypes.POINTER(vp)]
         L.lux_sgm_server_destroy.argtypes = [vp]
         L.lux_sgm_server_last_error.argtypes = [vp]
         L.lux_sgm_server_last_error.restype = ctypes.c_char_p
@@ -420,9 +423,11 @@ def _check_srv(rc, srv):
 class Server:
     """The served form (lux_sgm_server_*): requests admitted over `slots` device slots under an HBM budget."""
 
-    def __init__(self, device=0, hbm_budget=0, slots=4):
+    def __init__(self, device=0, hbm_budget=0, slots=4, device_loop=False):
+        """device_loop: the request loop runs on the device (LUX_SGM_SERVER_DEVICE_LOOP; needs hbm_budget)."""
         h = ctypes.c_void_p()
-        _check_srv(lib().lux_sgm_server_create(device, hbm_budget, slots, ctypes.byref(h)), None)
+        flags = SERVER_DEVICE_LOOP if device_loop else 0
+        _check_srv(lib().lux_sgm_server_create_ex(device, hbm_budget, slots, flags, ctypes.byref(h)), None)
         self.handle = h
         self._keep = {}
 

@@ -6,8 +6,8 @@
 * batched capture: 6 x 2080x1560 D=128 + 4 x 3120x2340 D=192 in one graph launch under the autotuned schedule:
   every pair equals itself run alone, and one pair of each size equals the oracle;
 * served mix: one request of each size 640x480 D=64, 1040x780 D=96, 2080x1560 D=128, 4160x3120 D=256 through the
-  request loop under a 96 GB budget: each equals itself run alone, and the 4160x3120 and 640x480 requests equal the
-  oracle.
+  request loop under a 96 GB budget, the host dispatcher's and the device-resident one: each equals itself run
+  alone, and the 4160x3120 and 640x480 requests equal the oracle.
 
 The oracle runs on the host (its aggregation serial, as the reference's, DenseStereoSgm.cs:171); the full-module
 case takes it about half a minute.
@@ -83,8 +83,17 @@ def test_served_sizes_match_alone_and_oracle(full_pair, full_oracle):
         for W, H, D, left, right in reqs:
             _say(f"alone {W}x{H} D={D}")
             alone.append(sgm.rectified(ctx, left, right, D, **KW))
-    _say("served: the four sizes, twice, over 4 slots under a 96 GB budget")
-    with sgm.Server(0, 96 << 30, slots=4) as srv:
+    for loop in (False, True):  # the host dispatcher, then the device-resident loop
+        _serve_four(reqs, alone, loop)
+    _same(alone[3], full_oracle, "4160x3120 D=256 served/alone vs oracle")
+    W, H, D, left, right = reqs[0]
+    _say(f"oracle {W}x{H} D={D}")
+    _same(alone[0], _oracle(left, right, D), f"{W}x{H} D={D} vs oracle")
+
+
+def _serve_four(reqs, alone, device_loop):
+    _say(f"served{' (device loop)' if device_loop else ''}: the four sizes, twice, over 4 slots under a 96 GB budget")
+    with sgm.Server(0, 96 << 30, slots=4, device_loop=device_loop) as srv:
         order = [3, 0, 2, 1, 0, 3, 1, 2]
         tickets = []
         for i in order:
@@ -100,10 +109,6 @@ def test_served_sizes_match_alone_and_oracle(full_pair, full_oracle):
         info = srv.info()
         assert info["completed"] == len(order) and info["failed"] == 0
         assert info["bytes_peak"] <= 96 << 30
-    _same(alone[3], full_oracle, "4160x3120 D=256 served/alone vs oracle")
-    W, H, D, left, right = reqs[0]
-    _say(f"oracle {W}x{H} D={D}")
-    _same(alone[0], _oracle(left, right, D), f"{W}x{H} D={D} vs oracle")
 
 
 BATCH = [(2080, 1560, 128)] * 6 + [(3120, 2340, 192)] * 4

@@ -236,6 +236,68 @@ def test_server_cancel_pending(cases):
             srv.wait(last, timeout_ms=60000)
 
 
+def _census_cases(cases):
+    return [i for i, c in enumerate(cases) if c["cost"] == "census"]
+
+
+def test_device_loop_mix_equals_alone(cases):
+    """The device-resident loop: the census requests of the mix, three times each in a shuffled order, every result
+    equal to the request alone, with several slots in use at once."""
+    order = _census_cases(cases) * 3
+    np.random.default_rng(1).shuffle(order)
+    with sgm.Server(0, 4 << 30, slots=4, device_loop=True) as srv:
+        print(f"device loop: mix of {len(order)} requests over 4 slots", flush=True)
+        tickets = _submit_all(srv, cases, order)
+        stats = _collect(srv, tickets, cases)
+        info = srv.info()
+        assert info["completed"] == len(order) and info["failed"] == 0
+        assert 0 < info["bytes_peak"] <= 4 << 30
+        assert len({st["slot"] for st in stats}) > 1
+        assert all(st["total_ms"] >= st["run_ms"] >= 0 for st in stats)
+
+
+def test_device_loop_device_memory_and_relaunch(cases):
+    """Device buffers in and out, and a second round after the loop has left for lack of work."""
+    import time
+
+    torch = pytest.importorskip("torch")
+    sel = [cases[i] for i in _census_cases(cases)]
+    pairs, outs, keep = _device_pairs(torch, sel)
+    torch.cuda.synchronize()
+    with sgm.Server(0, 4 << 30, slots=2, device_loop=True) as srv:
+        for rnd in range(2):
+            print(f"device loop, device memory, round {rnd}", flush=True)
+            for d, s, v in outs:
+                d.fill_(-1)
+                s.fill_(-7.0)
+                v.fill_(9)
+            torch.cuda.synchronize()
+            ts = [srv.submit_device(pr.params, pr.left, pr.right, pr.disp, pr.sub, pr.valid) for pr in pairs]
+            for t in ts:
+                srv.wait(t, timeout_ms=60000)
+            _check_outs(outs, sel, f"device loop round {rnd}")
+            time.sleep(0.1)  # longer than the loop's idle time: the next round launches it again
+    del keep
+
+
+def test_device_loop_refusals(cases):
+    """What the device loop does not take fails at submit; a request larger than the pool too."""
+    c = cases[5]
+    with sgm.Server(0, 1 << 30, slots=2, device_loop=True) as srv:
+        ad = cases[2]
+        assert ad["cost"] == "ad"
+        with pytest.raises(sgm.NotSupported):
+            srv.submit(ad["p"], ad["left"], ad["right"])
+        with pytest.raises(sgm.SgmError) as e:
+            huge = sgm.rect_params(2000, 1500, 128, bits=10, cost="census", radius=3, p1=8, p2=96, tile=16)
+            srv.submit(huge, np.zeros((1500, 2000), np.uint16), np.zeros((1500, 2000), np.uint16))
+        assert e.value.code == sgm.ENOMEM
+        t = srv.submit(c["p"], c["left"], c["right"])  # still serving after the refusals
+        srv.wait(t, timeout_ms=60000)
+    with pytest.raises(sgm.SgmError):
+        sgm.Server(0, 0, slots=2, device_loop=True)  # the pool needs a budget
+
+
 def test_server_pyramids_mixed_with_pairs(cases):
     """Coarse-to-fine requests in the same queue as plain pairs: each equals the same request run alone."""
     pyr = []
