@@ -1,6 +1,7 @@
 // The device-resident request loop (declared in dloop.h): the served form with its scheduler on the device.
 //
-// One persistent kernel holds the device. Its first wavefront is the scheduler, the others are workers.
+// One persistent kernel holds the device. Its first wavefront is the scheduler, the first of every other XCD's
+// wavefronts keeps that XCD's cache write-backs current (the flush epochs below), the others are workers.
 //  - The host writes each request into a ring in coherent pinned memory: the job descriptor of the executor
 //    (exec_impl.h) with every pointer into the request's workspace given as an offset, a bit per 64-bit word that
 //    says which words those are, and the workspace size. Host images are staged in pinned memory beside it.
@@ -21,6 +22,8 @@
 #include "dloop.h"
 
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -51,6 +54,16 @@ static_assert(sizeof(ExecJob) % 8 == 0, "the descriptor is relocated 64-bit word
 constexpr uint32_t kDLoopStages = kAll;
 constexpr uint64_t kPoolAlign = 4096;
 
+// Items one claim takes: the short items (staging chunks, border visits and sums, finish rows) in eights, the chains
+// (row paths, sweeps, border chains) and census row blocks one at a time.
+__host__ __device__ constexpr uint32_t stage_batch(int s)
+{
+    return (s == X_IN || s == X_RPATH_P || s == X_RADD_P || s == X_RPATH_M || s == X_RADD_M || s == X_FINISH ||
+            s == X_OUT)
+               ? 8u
+               : 1u;
+}
+
 enum : uint32_t { R_FREE = 0, R_SUBMITTED = 1, R_ADMITTED = 2, R_DONE = 3 };
 
 // One ring entry (coherent pinned host memory). The host writes everything but the last four fields, then state.
@@ -63,6 +76,7 @@ struct DReq {
     uint32_t slot;                 // the device writes these
     uint32_t pad_;
     uint64_t t_admit, t_done;      // device wall clock
+    uint64_t t_stage[X_NSTAGES];   // when each stage completed (device wall clock; 0: no items)
 };
 
 struct DSlot {                     // device memory; written by the scheduler, published by `live`
@@ -80,10 +94,12 @@ struct DSlot {                     // device memory; written by the scheduler, p
 struct DState {                    // device memory, kept across launches of the loop
     XSync sync;
     uint32_t quit;                 // the scheduler tells the workers to leave
+    uint32_t flusher[kXcds];       // an XCD's flusher is taken (cleared with quit before each launch)
     uint32_t inv;                  // admissions so far (the invalidate epochs)
     uint32_t admitted;             // ring entries admitted so far
     uint32_t nlive;
     uint32_t progress;             // items completed (the scheduler's watchdog)
+    uint32_t work;                 // bumped when work may have become ready (admission, stage completed or flushed)
     uint32_t fault;                // the watchdog saw no item complete for kStallTicks with requests running
     uint32_t order[kDSlots];       // live slots, smallest request first
     uint64_t used, peak;           // pool bytes
@@ -259,6 +275,7 @@ __device__ __forceinline__ void dl_scheduler(const DArgs& A, int x, int lane)
             seq++;
             if (lane == 0) ast(&S->admitted, seq);
             dl_order(A, lane);
+            if (lane == 0) __hip_atomic_fetch_add(&S->work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             busy = true;
         }
         const uint64_t now = wall_clock64();
@@ -312,11 +329,35 @@ __device__ __forceinline__ bool dl_ready(const DState* S, const DSlot* D, int s,
     return true;
 }
 
-// Finds and claims the next ready item of the live slots, smallest request first; false if none is ready. A stage
-// is taken when every stage it reads is complete and flushed; its items are claimed one atomic add at a time on the
-// stage's counter, whose high word is the slot's generation (a claim that lands on a newer admission of the slot
-// belongs to that request: it runs once that request's stage is ready, or not at all if the stage has no such item).
-__device__ __forceinline__ bool dl_claim(const DArgs& A, int lane, int* slot, int* stage, uint32_t* item, uint32_t* count)
+// A claim [t, t + batch) on stage s of slot D that landed on an admission other than the one the claimer looked at:
+// the items are that request's, so they run once its stage s is ready (or not at all past the stage's end).
+// st and n receive the request's stage starts and sizes (lane l: stage l).
+__device__ __forceinline__ bool dl_adopt(DState* S, DSlot* D, int s, uint32_t t, uint32_t& st, uint32_t& n, int lane)
+{
+    st = lane <= X_NSTAGES ? __hip_atomic_load(&D->start[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xffffffffu;
+    n = (uint32_t)__shfl_down((int)st, 1) - st;
+    if (t >= (uint32_t)__builtin_amdgcn_readlane((int)n, s)) return false;
+    while (!dl_ready(S, D, s, st)) {
+        if (ald(&S->quit)) return false;
+        __builtin_amdgcn_s_sleep(127);
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint64_t add64(uint64_t* p, uint64_t v, int lane)
+{
+    uint64_t r = 0;
+    if (lane == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(r >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)r);
+}
+
+// Finds and claims the next ready items of the live slots, smallest request first; false if none is ready. A stage
+// is taken when every stage it reads is complete and flushed; its items are claimed by atomic adds on the stage's
+// counter, whose high word is the slot's generation (a claim that lands on a newer admission of the slot belongs to
+// that request: dl_adopt).
+__device__ __forceinline__ bool dl_claim(const DArgs& A, int lane, int* slot, int* stage, uint32_t* item, uint32_t* count,
+                                         uint32_t* size, uint32_t* gen_out)
 {
     DState* S = A.st;
     const uint32_t nl = min(ald(&S->nlive), (uint32_t)kDSlots);
@@ -349,69 +390,117 @@ __device__ __forceinline__ bool dl_claim(const DArgs& A, int lane, int* slot, in
                     }
                 }
             }
-            uint64_t v = 0;
-            if (lane == 0) v = __hip_atomic_fetch_add(&D->cnt[s], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
-                (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+            const uint32_t batch = stage_batch(s);
+            const uint64_t v = add64(&D->cnt[s], batch, lane);
             const uint32_t g = (uint32_t)(v >> 32), t = (uint32_t)v;
-            if (g != gen) {  // the slot took a new request meanwhile: the claim is that request's
-                st = lane <= X_NSTAGES ? __hip_atomic_load(&D->start[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                       : 0xffffffffu;
-                n = (uint32_t)__shfl_down((int)st, 1) - st;
-                const uint32_t ns = (uint32_t)__builtin_amdgcn_readlane((int)n, s);
-                if (t >= ns) return false;
-                for (uint32_t w = 0; !dl_ready(S, D, s, st); w++) {
-                    if (ald(&S->quit)) return false;
-                    __builtin_amdgcn_s_sleep(127);
-                }
+            if (g != gen) {
+                if (!dl_adopt(S, D, s, t, st, n, lane)) return false;
             } else if (t >= (uint32_t)__builtin_amdgcn_readlane((int)n, s)) {
                 continue;  // the stage ran out meanwhile
             }
+            const uint32_t ns = (uint32_t)__builtin_amdgcn_readlane((int)n, s);
             *slot = k;
             *stage = s;
             *item = t;
-            *count = (uint32_t)__builtin_amdgcn_readlane((int)n, s);
+            *count = min(batch, ns - t);
+            *size = ns;
+            *gen_out = g;
             return true;
         }
     }
     return false;
 }
 
+// This XCD's share of the flush protocol; an acknowledgement may make a stage ready, so it bumps the work counter.
+__device__ __forceinline__ bool dl_flush(DState* S, int x, int lane)
+{
+    if (!xsync_poll(&S->sync, x, lane)) return false;
+    if (lane == 0) __hip_atomic_fetch_add(&S->work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// One wavefront per XCD does nothing but its XCD's write-backs, so a completed stage is flushed within microseconds
+// even while every other wavefront of the XCD is inside a long chain.
+__device__ __forceinline__ void dl_flusher(const DArgs& A, int x, int lane)
+{
+    DState* S = A.st;
+    while (!ald(&S->quit)) {
+        dl_flush(S, x, lane);
+        __builtin_amdgcn_s_sleep(32);
+    }
+}
+
 __device__ __forceinline__ void dl_worker(const DArgs& A, int x, int lane)
 {
     DState* S = A.st;
     uint32_t my_inv = 0, backoff = 1;
+    int fk = -1, fs = 0;     // the stage of the last claim: tried first (one atomic add while it lasts)
+    uint32_t fgen = 0, fn = 0;
     for (;;) {
-        xsync_poll(&S->sync, x, lane);
+        dl_flush(S, x, lane);
         if (ald(&S->quit)) break;
         int k = 0, s = 0;
-        uint32_t item = 0, n = 0;
-        if (!dl_claim(A, lane, &k, &s, &item, &n)) {
-            // nothing ready: look again after a while that doubles up to about 170 us (a wavefront that has just
-            // finished an item looks at once, so new work is taken up without waiting for the idle ones)
-            for (uint32_t w = 0; w < backoff; w++) __builtin_amdgcn_s_sleep(127);
+        uint32_t item = 0, cnt = 0, n = 0, gen = 0;
+        bool got = false;
+        if (fk >= 0) {
+            DSlot* D = &S->slot[fk];
+            const uint32_t batch = stage_batch(fs);
+            const uint64_t v = add64(&D->cnt[fs], batch, lane);
+            const uint32_t g = (uint32_t)(v >> 32), t = (uint32_t)v;
+            uint32_t st = 0, nn = 0;
+            if (g == fgen && t < fn) {
+                got = true;
+                n = fn;
+            } else if (g != fgen && dl_adopt(S, D, fs, t, st, nn, lane)) {
+                got = true;
+                n = (uint32_t)__builtin_amdgcn_readlane((int)nn, fs);
+            }
+            if (got) {
+                k = fk;
+                s = fs;
+                item = t;
+                cnt = min(batch, n - t);
+                gen = g;
+            } else {
+                fk = -1;
+            }
+        }
+        const uint32_t seen = ald(&S->work);
+        if (!got) got = dl_claim(A, lane, &k, &s, &item, &cnt, &n, &gen);
+        if (!got) {
+            // nothing ready: sleep until the work counter moves (checked every 5 us, and a look anyway after a while
+            // that doubles up to about 170 us), so that idle wavefronts read one word instead of the slot tables
+            for (uint32_t w = 0; w < backoff; w++) {
+                __builtin_amdgcn_s_sleep(127);
+                if (ald(&S->work) != seen || ald(&S->quit)) break;
+            }
             backoff = min(backoff * 2, 32u);
             continue;
         }
         backoff = 1;
         k = __builtin_amdgcn_readfirstlane(k);
         s = __builtin_amdgcn_readfirstlane(s);
-        item = (uint32_t)__builtin_amdgcn_readfirstlane((int)item);
-        n = (uint32_t)__builtin_amdgcn_readfirstlane((int)n);
+        fk = k;
+        fs = s;
+        fgen = gen;
+        fn = n;
         DSlot* D = &S->slot[k];
         const uint32_t ae = ald(&D->admit_epoch);
         if (ae > my_inv) {  // no stale line of an earlier request in this block
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             my_inv = ae;
         }
-        run_item<kDLoopStages>(&D->job, s, (int)item, lane);
+        for (uint32_t j = 0; j < cnt; j++) run_item<kDLoopStages>(&D->job, s, (int)(item + j), lane);
         __builtin_amdgcn_s_waitcnt(0);
         if (lane == 0) {
             __hip_atomic_fetch_add(&S->progress, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t o = __hip_atomic_fetch_add(&D->done[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (o + 1 == n) {
+            const uint32_t o = __hip_atomic_fetch_add(&D->done[s], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o + cnt == n) {
                 const uint32_t e = __hip_atomic_fetch_add(&S->sync.req, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
                 ast(&D->epoch[s], e);
+                __hip_atomic_store(&A.ring[ald(&D->ring)].t_stage[s], wall_clock64(), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_fetch_add(&S->work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -421,7 +510,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 {
     const int lane = threadIdx.x, x = xcc_id();
     if (lane == 0) __hip_atomic_fetch_add(&A.st->sync.live[x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t won = 0;
+    if (blockIdx.x != 0 && lane == 0) {
+        uint32_t zero = 0;
+        won = __hip_atomic_compare_exchange_strong(&A.st->flusher[x], &zero, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (blockIdx.x == 0) dl_scheduler(A, x, lane);
+    else if (__builtin_amdgcn_readfirstlane((int)won)) dl_flusher(A, x, lane);
     else dl_worker(A, x, lane);
     // leave with nothing unwritten in the L2 (an XCD with no wavefront left counts as flushed)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -470,7 +566,7 @@ namespace {
 hipError_t dl_launch(DLoop* d)
 {
     // quit is cleared after the previous loop kernel left (same stream); the kernel itself raises it when it leaves
-    hipError_t e = hipMemsetAsync(&d->st->quit, 0, sizeof(uint32_t), d->stream);
+    hipError_t e = hipMemsetAsync(&d->st->quit, 0, sizeof(uint32_t) * (1 + kXcds), d->stream);  // quit, flushers
     __atomic_store_n(&d->host->alive, 1u, __ATOMIC_SEQ_CST);
     DArgs A;
     A.st = d->st;
@@ -698,6 +794,7 @@ int dloop_submit(DLoop* d, const lux_sgm_rect_params* p, int mem, const void* le
     R.prio = (uint32_t)std::min<uint64_t>(cells, 0xffffffffu);
     R.slot = 0;
     R.t_admit = R.t_done = 0;
+    std::memset((void*)R.t_stage, 0, sizeof R.t_stage);
     tk.t_submit = Clock::now();
     __atomic_store_n(&R.state, (uint32_t)R_SUBMITTED, __ATOMIC_SEQ_CST);
     // then look at `alive` (the loop clears it, then looks at the ring): launch a loop if none owns the ring
@@ -758,6 +855,13 @@ int dloop_wait(DLoop* d, uint64_t ticket, int32_t timeout_ms, lux_sgm_request_st
         if (tk.disp) std::memcpy(tk.disp, so, tk.px * 2);
         if (tk.sub) std::memcpy(tk.sub, so + tk.px * 2, tk.px * 4);
         if (tk.valid) std::memcpy(tk.valid, so + tk.px * 6, tk.px);
+    }
+    if (std::getenv("LUX_SGM_DLOOP_TRACE")) {  // when each stage completed, ms after admission
+        std::string line = "dloop request " + std::to_string(ticket) + ":";
+        for (int k = 0; k < X_NSTAGES; k++)
+            if (R.t_stage[k])
+                line += " " + std::to_string(k) + "@" + std::to_string((double)(R.t_stage[k] - R.t_admit) / d->clock_khz);
+        std::fprintf(stderr, "%s\n", line.c_str());
     }
     if (stats) {
         const double total = std::chrono::duration<double, std::milli>(Clock::now() - tk.t_submit).count();

@@ -34,7 +34,7 @@ enum XStage : int {
     X_OUT,         // the outputs into host memory (the device-resident loop, host memory)
     X_NSTAGES
 };
-constexpr size_t kXChunk = 256 << 10;  // bytes one staging item copies
+constexpr size_t kXChunk = 64 << 10;  // bytes one staging item copies
 constexpr int kXCensusRows = 8;
 constexpr uint32_t xbit(int s) { return 1u << s; }
 
@@ -125,16 +125,17 @@ __device__ __forceinline__ uint32_t ald(const uint32_t* p)
 }
 
 // This XCD's share of the protocol: write the L2 back if an epoch was raised that it has not acknowledged. One
-// wavefront of the XCD takes each epoch on (the others would only repeat the write-back).
-__device__ __forceinline__ void xsync_poll(XSync* y, int x, int lane)
+// wavefront of the XCD takes each epoch on (the others would only repeat the write-back). True if this one did.
+__device__ __forceinline__ bool xsync_poll(XSync* y, int x, int lane)
 {
     const uint32_t r = ald(&y->req);
-    if (ald(&y->ack[x]) >= r || ald(&y->claim[x]) >= r) return;
+    if (ald(&y->ack[x]) >= r || ald(&y->claim[x]) >= r) return false;
     uint32_t prev = r;
     if (lane == 0) prev = __hip_atomic_fetch_max(&y->claim[x], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)__builtin_amdgcn_readfirstlane((int)prev) >= r) return;  // another one took it
+    if ((uint32_t)__builtin_amdgcn_readfirstlane((int)prev) >= r) return false;  // another one took it
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // buffer_wbl2: every dirty line of this XCD's L2 to memory
     if (lane == 0) __hip_atomic_fetch_max(&y->ack[x], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // Whether epoch e has reached memory from every XCD.
@@ -242,13 +243,23 @@ LUX_XITEM(x_finish)
 }
 #undef LUX_XITEM
 
-// A chunk of n bytes from src to dst, 16 bytes a lane when both are 16-byte aligned, the rest byte by byte.
+// A chunk of n bytes from src to dst, 16 bytes a lane (four loads in flight: one side may be host memory, where
+// each load waits a bus round trip) when both are 16-byte aligned, the rest byte by byte.
 __device__ __forceinline__ void copy_chunk(uint8_t* dst, const uint8_t* src, size_t n, int lane)
 {
     const size_t n16 = (((uintptr_t)dst | (uintptr_t)src) & 15) ? 0 : n / 16;
-    for (size_t k = (size_t)lane; k < n16; k += 64)
-        reinterpret_cast<uint4*>(dst)[k] = reinterpret_cast<const uint4*>(src)[k];
-    for (size_t k = n16 * 16 + (size_t)lane; k < n; k += 64) dst[k] = src[k];
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    size_t k = (size_t)lane;
+    for (; k + 192 < n16; k += 256) {
+        const uint4 a = s4[k], b = s4[k + 64], c = s4[k + 128], d = s4[k + 192];
+        d4[k] = a;
+        d4[k + 64] = b;
+        d4[k + 128] = c;
+        d4[k + 192] = d;
+    }
+    for (; k < n16; k += 64) d4[k] = s4[k];
+    for (size_t j = n16 * 16 + (size_t)lane; j < n; j += 64) dst[j] = src[j];
 }
 
 // Staging item i: chunks of each image in turn (X_IN), of each output in turn (X_OUT).
