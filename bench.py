@@ -14,7 +14,8 @@ after one warm-up; `cpu_baseline_c0` is the same for the 640x480 uint8 D=64 work
 More legs go into the same line:
   `batched`: one capture of 10 pairs (6 x 2080x1560 D=128, 4 x 3120x2340 D=192, independent seeds) as one graph
              launch with the autotuned schedule (graph branches, or the persistent executor in one launch or one
-             launch per pass group), against the same pairs run one after another (plain drop-in);
+             launch per pass group), against the same pairs run one after another (plain drop-in); the executor's
+             two forms are also timed on their own (`batched.executor`);
   `served`:  a mixed stream of depth requests (sizes uniform over 640x480 D64, 1040x780 D96, 2080x1560 D128,
              4160x3120 D256; host images in, host results out) through the request loop under a 96 GB budget:
              latency p50/p99 at Poisson arrivals of 40 req/s, served and plain (the plain drop-in serving the same
@@ -291,13 +292,23 @@ def bench_batched(np, torch, sgm, scene, reps):
     best, tried = sgm.batch_tune(ctx, pairs, stream=stream.cuda_stream, reps=2)
     ms_batch = timed(lambda: sgm.batch(ctx, pairs, stream=stream.cuda_stream))
     same = all(bool(torch.equal(a, b)) for a, b in zip(ref, keep[2::5]))
+    # the persistent executor on its own, whatever the tuner picked: one launch per pass group, and one launch
+    executor = {}
+    for groups, key in ((7, "per_pass_group"), (1, "one_launch")):
+        for t in keep[2::5]:
+            t.fill_(-1)
+        sgm.batch_set_schedule(ctx, pairs, executor=groups, order=1)
+        ms = timed(lambda: sgm.batch(ctx, pairs, stream=stream.cuda_stream))
+        executor[key] = {"launches": groups, "ms_per_batch": round(ms, 3), "gcells_s": round(cells * 8 / ms / 1e6, 3),
+                         "speedup_vs_plain": round(ms_plain / ms, 3),
+                         "identical_to_plain": all(bool(torch.equal(a, b)) for a, b in zip(ref, keep[2::5]))}
     ctx.close()
     del keep
     return {"workload": "6 x 2080x1560 D=128 + 4 x 3120x2340 D=192, uint16 10-bit, census 7x7, seeds 100..109",
             "cells": cells, "ms_per_batch": round(ms_batch, 3), "gcells_s": round(cells * 8 / ms_batch / 1e6, 3),
             "plain_ms": round(ms_plain, 3), "plain_gcells_s": round(cells * 8 / ms_plain / 1e6, 3),
             "speedup_vs_plain": round(ms_plain / ms_batch, 3), "schedule": best,
-            "candidates": tried, "identical_to_plain": same}
+            "candidates": tried, "identical_to_plain": same, "executor": executor}
 
 
 def bench_served(np, sgm, scene, n_req, rate, budget_gb, slots, window):
