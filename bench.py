@@ -48,10 +48,11 @@ PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1
 #   byte path volumes (census requests whose path costs fit a byte, lux_amd/csrc/engine.hip path_bytes):
 #     rows+ (pass +1 path ->, census cost fused): writes raw (1) and its path volume (1)
 #     vd+k / vd-k (one vertical or diagonal path): reads raw (1), writes its path volume (1)
-#     rows- (pass -1 path <-, the sum of the eight paths and the winner-take-all fused): raw (1) + seven volumes (7)
+#     rows- (pass -1 path <-, census cost recomputed from the codes, the sum of the eight paths and the winner-take-all
+#     fused): seven volumes (7)
 #   u16 aggregate (every other request): rows+ raw 1 + agg 2; a sweep raw 1 + agg read 2 + agg write 2; rows- raw 1 +
 #     agg 2
-STAGE_BYTES_PV = {"rows+": 2, "rows-": 8, **{k: 2 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
+STAGE_BYTES_PV = {"rows+": 2, "rows-": 7, **{k: 2 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
 STAGE_BYTES_U16 = {"rows+": 3, "rows-": 3, **{k: 5 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
 KERNEL_NAMES = {"rows+": "sgm_row_fwd (pass +1 path ->, census cost fused)",
                 "rows-": "sgm_row_bwd_wta (pass -1 path <-, eight-path sum and winner-take-all fused)",

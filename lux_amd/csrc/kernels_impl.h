@@ -796,6 +796,17 @@ __device__ __forceinline__ void census_advance(uint64_t (&win)[2 * NP], uint64_t
     for (int e = 2 * NP - 1; e > 0; e--) win[e] = win[e - 1];
     win[0] = ((uint64_t)hi << 32) | lo;
 }
+// The window of the cell to the left (x -> x-1): win[e] <- win[e+1], and win[2*NP-1] <- the right neighbour lane's
+// win[0] = cr(max(x - lo - 2*NP, 0)) (DPP wave_shl), lane 63 taking `fresh` = cr(max(x - 128*NP, 0)).
+template <int NP>
+__device__ __forceinline__ void census_retreat(uint64_t (&win)[2 * NP], uint64_t fresh)
+{
+    const uint64_t first = win[0];
+    const uint32_t lo = shl1((uint32_t)first, (uint32_t)fresh), hi = shl1((uint32_t)(first >> 32), (uint32_t)(fresh >> 32));
+#pragma unroll
+    for (int e = 0; e < 2 * NP - 1; e++) win[e] = win[e + 1];
+    win[2 * NP - 1] = ((uint64_t)hi << 32) | lo;
+}
 // Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
 // repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
 template <int NP>
@@ -991,6 +1002,9 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 {
     static_assert(!PV || (WTA && !VAR && !ADP), "byte path volumes: uniform constant-P2 requests");
     constexpr int PF = 8, E = 2 * NP, NV = PV ? 7 : 0;
+    // byte path volumes: each cell's census cost is computed again from the codes, as the forward row path does,
+    // instead of read from the raw volume (a window of right codes that moves one code a step, towards x = 0)
+    constexpr bool COST = PV;
     if (y >= a.H) return;
     const int W = a.W, cs = a.cs, lo = lane * E;
     int D = a.D;
@@ -1007,14 +1021,33 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     const int lol = (VAR ? lo < cs : active) ? lo : 0;
     // the row's base addresses are wave-uniform (scalar registers) and each load adds a 32-bit lane offset below
     // W * cs, so the loads take their address as base + offset with no 64-bit arithmetic per lane
-    const uint8_t* rawrow = a.raw + rowcell * cs;
+    const uint8_t* rawrow = COST ? nullptr : a.raw + rowcell * cs;
     const uint16_t* aggrow = a.agg + rowcell * cs;
     const uint8_t* pvrow[NV > 0 ? NV : 1];
 #pragma unroll
     for (int v = 0; v < NV; v++) pvrow[v] = a.pv[v] + rowcell * a.pvs;
-    RawW<NP> rb[PF];
+    RawW<NP> rb[COST ? 1 : PF];
     Cell<NP> ab[PF];
     RawW<NP> vb[PF][NV > 0 ? NV : 1];  // PV: the seven path volumes' bytes of the cells in flight
+    // COST: the lane's window win[e] = cr(max(x - lo - e, 0)), and per step i (x = W-1-i) the wave-uniform codes
+    // cl(x) and cr(max(x - 64*E, 0)) (the code lane 63 takes on the move to x-1), 64 steps a register (lane k: step
+    // i0 + k) in two register sets that alternate between blocks of 64 steps, read out with v_readlane
+    const uint64_t* crow = COST ? a.cr + rowcell : nullptr;
+    const uint64_t* clrow = COST ? a.cl + rowcell : nullptr;
+    uint64_t win[E];
+    uint64_t bl0 = 0, bf0 = 0, bl1 = 0, bf1 = 0;
+    if constexpr (COST) {
+#pragma unroll
+        for (int e = 0; e < E; e++) win[e] = crow[max(W - 1 - lo - e, 0)];
+        bl0 = clrow[max(W - 1 - lane, 0)];
+        bf0 = crow[max(W - 1 - lane - 64 * E, 0)];
+        bl1 = clrow[max(W - 1 - (64 + lane), 0)];
+        bf1 = crow[max(W - 1 - (64 + lane) - 64 * E, 0)];
+    }
+    const auto rl64 = [&](uint64_t v, int q) {
+        return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), q) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, q);
+    };
     Range from;
     // The step loop's stores do not depend on a condition (which would cost the prefetches their early issue, as
     // bstore_* explains): the aggregate is stored by every step or by none (WAGG, or a.write_agg: one branch for the
@@ -1043,7 +1076,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
 #pragma unroll
     for (int j = 0; j < PF; j++) {
         const int xj = max(W - 1 - j, 0);
-        ld_raw_w<NP>(rawrow + (uint32_t)(xj * cs + lol), rb[j]);
+        if constexpr (!COST) ld_raw_w<NP>(rawrow + (uint32_t)(xj * cs + lol), rb[j]);
         if constexpr (PV) {
 #pragma unroll
             for (int v = 0; v < NV; v++) ld_raw_w<NP>(pvrow[v] + (uint32_t)(xj * a.pvs + lol), vb[j][v]);
@@ -1056,9 +1089,16 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     uint32_t myd = 0, myr = 0;
     int myl = 0, myc = 0, myrr = 0;
     bool mypar = false;
-    auto body = [&](int x, int j, int xb) {
+    auto body = [&](int x, int j, int xb, uint64_t bl, uint64_t bf) {
         Cell<NP> raw;
-        widen<NP>(rb[j], raw);
+        if constexpr (COST) {
+            const int q = (W - 1 - x) & 63;
+            uint32_t bytes[NP / 2 + (NP == 1)];
+            census_cost<NP>(rl64(bl, q), win, D, lo, raw, bytes);
+            census_retreat<NP>(win, rl64(bf, q));
+        } else {
+            widen<NP>(rb[j], raw);
+        }
         Cell<NP> agg;
         if constexpr (PV) {  // the other seven paths' costs (and pass +1's border-column replay in column W-1)
 #pragma unroll
@@ -1089,7 +1129,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
             p2 = gq.at(W - 1 - x);
             gq.advance(W - 1 - x, gq_at, lane, a.ap, a.p2);
         }
-        ld_raw_w<NP>(rawrow + (uint32_t)(xp * cs + lol), rb[j]);
+        if constexpr (!COST) ld_raw_w<NP>(rawrow + (uint32_t)(xp * cs + lol), rb[j]);
         if constexpr (PV) {
 #pragma unroll
             for (int v = 0; v < NV; v++) ld_raw_w<NP>(pvrow[v] + (uint32_t)(xp * a.pvs + lol), vb[j][v]);
@@ -1153,28 +1193,52 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
     // x = W-1 .. 1: (W-1) steps, in blocks of 64 steps counted from x = W-1 (PF divides 64, so every block starts on
     // the same prefetch slot); a block's loop holds only the prefetch loads and, when asked, the aggregate stores
     const int n = W - 1;
-    int i0 = 0;
-    if constexpr (WTA) {
-        for (; i0 + 64 <= n; i0 += 64) {
-            const int xb = W - 64 - i0;
-            for (int i8 = i0; i8 < i0 + 64; i8 += PF) {
+    if constexpr (COST) {
+        // block ib: steps ib .. min(ib + 64, n) - 1 with the codes of register set (bl, bf), which is then reloaded in
+        // place with block ib + 128's codes
+        auto block = [&](int ib, uint64_t& bl, uint64_t& bf) {
+            const int xb = W - 64 - ib, ie = min(ib + 64, n);
+            int i = ib;
+            for (; i + PF <= ie; i += PF) {
 #pragma unroll
-                for (int j = 0; j < PF; j++) body(W - 1 - (i8 + j), j, xb);
+                for (int j = 0; j < PF; j++) body(W - 1 - (i + j), j, xb, bl, bf);
             }
-            flush(xb, 0, 63);
+#pragma unroll
+            for (int j = 0; j < PF; j++)
+                if (i + j < ie) body(W - 1 - (i + j), j, xb, bl, bf);
+            if (ie == ib + 64) flush(xb, 0, 63);
+            else flush(xb, 1 - xb, W - 1 - ib - xb);  // pixels 1 .. W-1-ib
+            bl = clrow[max(W - 1 - (ib + 128 + lane), 0)];
+            bf = crow[max(W - 1 - (ib + 128 + lane) - 64 * E, 0)];
+        };
+        for (int ib = 0; ib < n; ib += 128) {
+            block(ib, bl0, bf0);
+            if (ib + 64 < n) block(ib + 64, bl1, bf1);
         }
-    }
-    // the rest (the last block's fewer than 64 steps, or the whole row without the winner-take-all)
-    const int xb = W - 64 - i0, ilast = i0;
-    for (; i0 + PF <= n; i0 += PF) {
+    } else {
+        int i0 = 0;
+        if constexpr (WTA) {
+            for (; i0 + 64 <= n; i0 += 64) {
+                const int xb = W - 64 - i0;
+                for (int i8 = i0; i8 < i0 + 64; i8 += PF) {
 #pragma unroll
-        for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j, xb);
-    }
+                    for (int j = 0; j < PF; j++) body(W - 1 - (i8 + j), j, xb, 0, 0);
+                }
+                flush(xb, 0, 63);
+            }
+        }
+        // the rest (the last block's fewer than 64 steps, or the whole row without the winner-take-all)
+        const int xb = W - 64 - i0, ilast = i0;
+        for (; i0 + PF <= n; i0 += PF) {
 #pragma unroll
-    for (int j = 0; j < PF; j++)
-        if (i0 + j < n) body(W - 1 - (i0 + j), j, xb);
-    if constexpr (WTA) {
-        if (n > ilast) flush(xb, 1 - xb, W - 1 - ilast - xb);  // pixels 1 .. W-1-ilast
+            for (int j = 0; j < PF; j++) body(W - 1 - (i0 + j), j, xb, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < PF; j++)
+            if (i0 + j < n) body(W - 1 - (i0 + j), j, xb, 0, 0);
+        if constexpr (WTA) {
+            if (n > ilast) flush(xb, 1 - xb, W - 1 - ilast - xb);  // pixels 1 .. W-1-ilast
+        }
     }
     if (n > 0) {  // the column-1 states for the border replay
         store_u16<NP>(a.cols + (size_t)y * cs + lo, lo < cs, state);
