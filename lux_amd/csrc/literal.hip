@@ -29,8 +29,8 @@
 // host checks occupancy) every wait ends. A bounded spin sets the error word instead of hanging, and later
 // waits are skipped, so the grid always drains.
 //
-// Memory. All line-buffer, minimum, range, guidance and progress words go through agent-scope accesses (global
-// load/store sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
+// Memory. All line-buffer, minimum, range, guidance and progress words go through agent-scope accesses (loads and
+// stores with sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
 // (s_waitcnt) before its progress store. The aggregate of a cell is read and written by its own row's wavefront
 // only, with plain accesses (the second visit of the border column takes the first visit's sums from registers).
 // One wavefront per row, planes across lanes (NPL per lane; NPL 0: 64-plane chunks in turn, caps up to 1024).
@@ -70,10 +70,6 @@ struct LitArgs {
     uint32_t* err;
 };
 
-__device__ __forceinline__ uint16_t ald(const uint16_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ void ast(uint16_t* p, uint16_t v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -106,8 +102,36 @@ __device__ __forceinline__ int lit_before(const LitArgs& a, int dir, int j, int 
     return (int)std::min(std::max(lim, 0L), (long)a.W);
 }
 
-// Rd (:318-324): past the end of the buffer the reference reads zero pages.
-__device__ __forceinline__ int lit_rd(const LitArgs& a, long i) { return i < a.Llen ? (int)ald(a.L + i) : 0; }
+// A visit's reads of the shared words: loads with the sc1 bit (what the relaxed agent-scope atomic loads compile to)
+// through buffer resources. To the compiler they are plain loads, so the reads of a visit issue together after its
+// acquire instead of one round trip each. A read past a resource's end returns 0: for the line buffer that is Rd's
+// rule past the end of the buffer (:318-324); every index read is >= 0 (the lowest is a lead word).
+constexpr int kSc1 = 16;  // cache policy: sc1
+struct LitRes {
+    __amdgpu_buffer_rsrc_t L, Min, RS, RC, Pix;
+};
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lit_rsrc(const void* base, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ LitRes lit_res(const LitArgs& a)
+{
+    LitRes r;
+    r.L = lit_rsrc(a.L, a.Llen * 2);
+    r.Min = lit_rsrc(a.Min, 2L * a.halfM * 2);
+    r.RS = lit_rsrc(a.RS, 2L * a.halfR * 2);
+    r.RC = lit_rsrc(a.RC, 2L * a.halfR * 2);
+    r.Pix = lit_rsrc(a.Pix, 2L * a.halfR * 4);
+    return r;
+}
+__device__ __forceinline__ int bld16(__amdgpu_buffer_rsrc_t r, long i)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(i * 2), 0, kSc1);
+}
+__device__ __forceinline__ uint32_t bld32(__amdgpu_buffer_rsrc_t r, long i)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4), 0, kSc1);
+}
 
 __device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
 {
@@ -148,39 +172,53 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     const int start = cc.start, cap = cc.cap;
     const int q = start - a.minlo;
     const uint32_t g = cc.g;
-    const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
+    const LitRes R = lit_res(a);
+    const int sp = bld16(R.RS, cur * a.halfR + slot), cp = bld16(R.RC, cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
 
     // every read first: the visit never reads a word it writes (path 0 reads another slot of its half, paths
     // 1-3 the other half, and path 3's spill into the next slot comes after path 0's read in pass -1)
+    // All loads are issued before any of them is used (one round trip for the visit), then the P2 terms.
     int minPrev[4], p2p[4];
+    uint32_t gq[4];
     int lm[4][NPL], l0[4][NPL], lp[4][NPL];
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
         const int half = p == 0 ? cur : oth;
-        minPrev[p] = ald(a.Min + half * a.halfM + ns * 4 + p);
-        int P2 = a.p2;
-        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, ald32(a.Pix + half * a.halfR + ns));
-        p2p[p] = std::min(P2 + minPrev[p], 65535);
+        minPrev[p] = bld16(R.Min, half * a.halfM + ns * 4 + p);
+        gq[p] = ADP ? bld32(R.Pix, half * a.halfR + ns) : 0u;
         const long nb = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
+        // no branch around the loads: lanes past the cap read their wave's first word and drop it
 #pragma unroll
         for (int h = 0; h < NPL; h++) {
-            const int k = lane + 64 * h;
-            lm[p][h] = l0[p][h] = lp[p][h] = 0;
-            if (k < cap) {
-                lm[p][h] = lit_rd(a, nb + k - 1);
-                l0[p][h] = lit_rd(a, nb + k);
-                lp[p][h] = lit_rd(a, nb + k + 1);
-            }
+            const long i0 = nb + (lane + 64 * h < cap ? lane + 64 * h : 0);
+            lm[p][h] = bld16(R.L, i0 - 1);
+            l0[p][h] = bld16(R.L, i0);
+            lp[p][h] = bld16(R.L, i0 + 1);
         }
     }
+    // the cell's own raw costs and aggregate (no other row touches them), in the same round trip
     int rb[NPL], ag[NPL];
 #pragma unroll
     for (int h = 0; h < NPL; h++) {
-        const int k = lane + 64 * h;
-        rb[h] = k < cap ? (int)a.raw[cbase + k] : 0;
-        ag[h] = k < cap ? (re ? last[h] : (int)a.agg[cbase + k]) : 0;
+        const int k = lane + 64 * h, kk = k < cap ? k : 0;
+        const int r0 = (int)a.raw[cbase + kk], a0 = (int)a.agg[cbase + kk];
+        rb[h] = k < cap ? r0 : 0;
+        ag[h] = k < cap ? (re ? last[h] : a0) : 0;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        int P2 = a.p2;
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, gq[p]);
+        p2p[p] = std::min(P2 + minPrev[p], 65535);
+#pragma unroll
+        for (int h = 0; h < NPL; h++) {
+            const bool in = lane + 64 * h < cap;
+            lm[p][h] = in ? lm[p][h] : 0;
+            l0[p][h] = in ? l0[p][h] : 0;
+            lp[p][h] = in ? lp[p][h] : 0;
+        }
     }
 
     // the recurrence (:288-298 / :302-312)
@@ -264,7 +302,8 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     const int start = cc.start, cap = cc.cap;
     const int q = start - a.minlo;
     const uint32_t g = cc.g;
-    const int sp = ald(a.RS + cur * a.halfR + slot), cp = ald(a.RC + cur * a.halfR + slot);
+    const LitRes R = lit_res(a);
+    const int sp = bld16(R.RS, cur * a.halfR + slot), cp = bld16(R.RC, cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
     int minPrev[4], p2p[4];
     long nb[4];
@@ -272,9 +311,9 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     for (int p = 0; p < 4; p++) {
         const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
         const int half = p == 0 ? cur : oth;
-        minPrev[p] = ald(a.Min + half * a.halfM + ns * 4 + p);
+        minPrev[p] = bld16(R.Min, half * a.halfM + ns * 4 + p);
         int P2 = a.p2;
-        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, ald32(a.Pix + half * a.halfR + ns));
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, bld32(R.Pix, half * a.halfR + ns));
         p2p[p] = std::min(P2 + minPrev[p], 65535);
         nb[p] = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
     }
@@ -283,10 +322,12 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
         const bool act = k < cap;
         int lm[4], l0[4], lp[4];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            lm[p] = act ? lit_rd(a, nb[p] + k - 1) : 0;
-            l0[p] = act ? lit_rd(a, nb[p] + k) : 0;
-            lp[p] = act ? lit_rd(a, nb[p] + k + 1) : 0;
+        for (int p = 0; p < 4; p++) {  // loads without a branch, as in lit_visit
+            const long i0 = nb[p] + (act ? k : 0);
+            const int vm = bld16(R.L, i0 - 1), v0 = bld16(R.L, i0), vp = bld16(R.L, i0 + 1);
+            lm[p] = act ? vm : 0;
+            l0[p] = act ? v0 : 0;
+            lp[p] = act ? vp : 0;
         }
         const int rb = act ? (int)a.raw[cbase + k] : 0;
         int sum = act ? (re ? (int)aggl[k] : (int)a.agg[cbase + k]) : 0;
@@ -352,7 +393,38 @@ __device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, i
     for (;;) {
         seen = (int)ald32(a.progress + row2);
         if (seen >= need) return;
-        if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > (1 << 22)) {
+        if ((++spins & 63) == 0 &&
+            (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > (1 << 22))) {
+            atomicOr(a.err, 1u);
+            dead = true;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Lane 0 waits until each of four rows has completed need[d] visits (need 0: nothing to wait for). The progress
+// words of all four are read together, so one round trip serves them all, and the error word only every 64th
+// round, so a progress store is seen one round trip after it lands rather than two.
+__device__ __forceinline__ void lit_wait4(const LitArgs& a, const int (&row2)[4], const int (&need)[4], int (&seen)[4],
+                                          bool& dead)
+{
+    if (dead) return;
+    if (seen[0] >= need[0] && seen[1] >= need[1] && seen[2] >= need[2] && seen[3] >= need[3]) return;
+    int spins = 0;
+    for (;;) {
+        uint32_t v[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) v[d] = ald32(a.progress + row2[d]);
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            seen[d] = (int)v[d];
+            ok = ok && seen[d] >= need[d];
+        }
+        if (ok) return;
+        if ((++spins & 63) == 0 &&
+            (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > (1 << 22))) {
             atomicOr(a.err, 1u);
             dead = true;
             return;
@@ -397,11 +469,15 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
                 next = lit_cell<ADP>(a, (size_t)y * W + (DIR > 0 ? c1 : W - 1 - c1));
             }
             if (lane == 0 && pos < W) {
+                int rows2[4], need[4];
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
                     const int row2 = row + (d < 2 ? d - 2 : d - 1);
-                    if (row2 >= 0 && row2 < a.H) lit_wait(a, row2, lit_need(a, DIR, j, i, r, c, row2), seen[d], dead);
+                    const bool in = row2 >= 0 && row2 < a.H;
+                    rows2[d] = in ? row2 : row;  // a row outside the image: read this row's word, wait for nothing
+                    need[d] = in ? lit_need(a, DIR, j, i, r, c, row2) : 0;
                 }
+                lit_wait4(a, rows2, need, seen, dead);
             } else if (lane == 0) {
                 // the second visit of the border column reads the buffers as the band's earlier visits left them,
                 // whichever rows made those: it waits on every row of its band (and rows row-2, row-1 before it)
@@ -531,6 +607,7 @@ hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pa
     a.stride = 4 * l.maxhi + 5;
     a.halfL = (long)(l.W + 2) * a.stride;
     a.Llen = 2 * a.halfL;
+    if (a.Llen * 2 > 0x7fffffffL) return hipErrorInvalidValue;  // the visits' buffer loads take 32-bit byte offsets
     a.halfM = (l.W + 2) * 4;
     a.halfR = l.W + 2;
     a.L = (uint16_t*)(base + lay.L);
