@@ -133,15 +133,32 @@ __device__ __forceinline__ uint32_t bld32(__amdgpu_buffer_rsrc_t r, long i)
     return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4), 0, kSc1);
 }
 
+// Stores of the shared words: sc1 buffer stores (what the relaxed agent-scope atomic stores compile to), issued with
+// no branch; a store that should not happen gets an offset past every resource, which the range check drops.
+constexpr uint32_t kLitDrop = 0x80000000u;
+__device__ __forceinline__ void bst16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)byte_off, 0, kSc1);
+}
+
+// Minimum over the 64 lanes of two u16 halves at once (DPP row shifts and broadcasts), wave-uniform.
+typedef unsigned short lit_u16x2 __attribute__((ext_vector_type(2)));
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t lit_dpp_pmin(uint32_t v)
+{
+    const uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROWMASK, 0xf, false);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(lit_u16x2, v),
+                                                                  __builtin_bit_cast(lit_u16x2, o)));
+}
 __device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
-        const uint32_t lo = std::min(v & 0xffffu, w & 0xffffu), hi = std::min(v >> 16, w >> 16);
-        v = lo | (hi << 16);
-    }
-    return v;
+    v = lit_dpp_pmin<0x111, 0xf>(v);  // row_shr:1
+    v = lit_dpp_pmin<0x112, 0xf>(v);  // row_shr:2
+    v = lit_dpp_pmin<0x114, 0xf>(v);  // row_shr:4
+    v = lit_dpp_pmin<0x118, 0xf>(v);  // row_shr:8: lane 15 of each row holds the row's minimum
+    v = lit_dpp_pmin<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = lit_dpp_pmin<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3: lane 63 holds the minimum
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // The cell's own words, which no other row writes: loaded one visit ahead (LitCell), so a visit's line-buffer
@@ -247,14 +264,16 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     // words; one that runs past its segment continues into the next (the guard word, then the next segment,
     // or for path 3 the next slot's lead word and path-0 segment). A word is stored once, by the last
     // interval that covers it, so no two stores of a visit race.
-    const long B = (long)cur * a.halfL + (long)slot * a.stride;
-    const long q0 = sp - a.minlo;
-    long lo[8], hi[8];
+    // Word indexes fit 31 bits (the host checks the buffer's size); a word past the end of the buffer is not
+    // stored (the range check drops it).
+    const int B = cur * (int)a.halfL + slot * a.stride;
+    const int q0 = sp - a.minlo, Llen = (int)a.Llen;
+    int lo[8], hi[8];
 #pragma unroll
     for (int p = 0; p < 4; p++) {
-        const long bc = B + 1 + (long)p * (a.M + 1);
-        lo[2 * p] = bc + std::max(q0, 0L);
-        hi[2 * p] = bc + std::min(q0 + cp, a.Llen - bc);  // k1 = min(cp, L.Length - baseCur - q0) (:262)
+        const int bc = B + 1 + p * (a.M + 1);
+        lo[2 * p] = bc + std::max(q0, 0);
+        hi[2 * p] = bc + std::min(q0 + cp, Llen - bc);  // k1 = min(cp, L.Length - baseCur - q0) (:262)
         lo[2 * p + 1] = bc + q;
         hi[2 * p + 1] = bc + q + cap;
     }
@@ -262,12 +281,11 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     for (int o = 0; o < 8; o++) {
 #pragma unroll
         for (int h = 0; h < NPL; h++) {
-            const long w = lo[o] + lane + 64 * h;
-            if (w >= hi[o] || w >= a.Llen) continue;
-            bool last = true;
+            const int w = lo[o] + lane + 64 * h;
+            bool st = w < hi[o];
 #pragma unroll
-            for (int o2 = o + 1; o2 < 8; o2++) last = last && !(w >= lo[o2] && w < hi[o2]);
-            if (last) ast(a.L + w, (uint16_t)((o & 1) ? lc[o >> 1][h] : 2000));
+            for (int o2 = o + 1; o2 < 8; o2++) st = st & !((w >= lo[o2]) & (w < hi[o2]));  // no branch
+            bst16(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lc[o >> 1][h] : 2000u);
         }
     }
 #pragma unroll
@@ -350,26 +368,25 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     }
     const uint32_t mins01 = wave_pmin(run[0] | (run[1] << 16)), mins23 = wave_pmin(run[2] | (run[3] << 16));
     __syncthreads();
-    const long B = (long)cur * a.halfL + (long)slot * a.stride;
-    const long q0 = sp - a.minlo;
-    long lo[8], hi[8];
+    const int B = cur * (int)a.halfL + slot * a.stride;  // word indexes as in lit_visit
+    const int q0 = sp - a.minlo, Llen = (int)a.Llen;
+    int lo[8], hi[8];
 #pragma unroll
     for (int p = 0; p < 4; p++) {
-        const long bc = B + 1 + (long)p * (a.M + 1);
-        lo[2 * p] = bc + std::max(q0, 0L);
-        hi[2 * p] = bc + std::min(q0 + cp, a.Llen - bc);
+        const int bc = B + 1 + p * (a.M + 1);
+        lo[2 * p] = bc + std::max(q0, 0);
+        hi[2 * p] = bc + std::min(q0 + cp, Llen - bc);
         lo[2 * p + 1] = bc + q;
         hi[2 * p + 1] = bc + q + cap;
     }
 #pragma unroll
     for (int o = 0; o < 8; o++) {
-        for (long k = lane; k < hi[o] - lo[o]; k += 64) {
-            const long w = lo[o] + k;
-            if (w >= a.Llen) break;
-            bool last = true;
+        for (int k = lane; k < hi[o] - lo[o]; k += 64) {
+            const int w = lo[o] + k;
+            bool st = true;
 #pragma unroll
-            for (int o2 = o + 1; o2 < 8; o2++) last = last && !(w >= lo[o2] && w < hi[o2]);
-            if (last) ast(a.L + w, (o & 1) ? lcs[o >> 1][k] : (uint16_t)2000);
+            for (int o2 = o + 1; o2 < 8; o2++) st = st & !((w >= lo[o2]) & (w < hi[o2]));  // no branch
+            bst16(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lcs[o >> 1][k] : 2000u);
         }
     }
     if (lane < 4) {
