@@ -809,13 +809,14 @@ __device__ __forceinline__ void census_retreat(uint64_t (&win)[2 * NP], uint64_t
 }
 // Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
 // repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
-template <int NP>
+// FULL: every lane's planes are below D (D = 128 * NP), so no plane takes 255.
+template <int NP, bool FULL = false>
 __device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2 * NP], int D, int lo, Cell<NP>& raw,
                                             uint32_t (&bytes)[NP / 2 + (NP == 1)])
 {
     uint32_t v[2 * NP];
 #pragma unroll
-    for (int e = 0; e < 2 * NP; e++) v[e] = lo + e < D ? (uint32_t)__popcll(cl ^ win[e]) : 255u;
+    for (int e = 0; e < 2 * NP; e++) v[e] = (FULL || lo + e < D) ? (uint32_t)__popcll(cl ^ win[e]) : 255u;
 #pragma unroll
     for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
     if constexpr (NP == 1) {
@@ -902,7 +903,8 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
             };
             const uint64_t cl = rl64(bl), cr1 = rl64(br);
             uint32_t bytes[NP / 2 + (NP == 1)];
-            census_cost<NP>(cl, win, a.D, lo, raw, bytes);
+            // byte path volumes have no padding planes (cap = D), so an unmasked wave's planes are all below D
+            census_cost<NP, PV && !MASKED>(cl, win, a.D, lo, raw, bytes);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
             if (st_raw) {
                 if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
@@ -1094,7 +1096,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         if constexpr (COST) {
             const int q = (W - 1 - x) & 63;
             uint32_t bytes[NP / 2 + (NP == 1)];
-            census_cost<NP>(rl64(bl, q), win, D, lo, raw, bytes);
+            census_cost<NP, !MASKED>(rl64(bl, q), win, D, lo, raw, bytes);  // COST is PV: D = 128 * NP unmasked
             census_retreat<NP>(win, rl64(bf, q));
         } else {
             widen<NP>(rb[j], raw);
@@ -1153,7 +1155,7 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         for (int e = 0; e < E; e++) {
             const int d = lo + e;
             const uint32_t v = (e & 1) ? (fin.v[e >> 1] >> 16) : (fin.v[e >> 1] & 0xffffu);
-            key[e] = d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;
+            key[e] = (PV && !MASKED) || d < D ? ((v << 16) | (uint32_t)d) : 0xffffffffu;  // PV unmasked: D = 128 * NP
             best = key[e] < best ? key[e] : best;
         }
         uint32_t rdone = 0;
