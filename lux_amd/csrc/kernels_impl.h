@@ -744,8 +744,12 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
 template <int NP, int DIR, int P, bool MASKED>
 __global__ __launch_bounds__(kBlock) void sgm_sweep_pv(SweepArgs a)
 {
-    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(
-        a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
+    int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    // rightward diagonals longest first (their chains by index run long, short, then long again, so the last long
+    // ones would start late); the leftward ones keep the index order, which measured faster for them
+    constexpr int DX = DIR * (P == 1 ? 1 : P == 3 ? -1 : 0);
+    if constexpr (DX > 0) w = diag_order<1>(w, a.W, a.H);
+    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(a, w, threadIdx.x & 63);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
