@@ -47,18 +47,18 @@ PATH_BYTES_PER_CELL = 7  # write raw 1 + pass +1 (raw 1, agg 2) + pass -1 (raw 1
 # HBM bytes per cell each timed stage of this design moves (what it must move at minimum):
 #   byte path volumes (census requests whose path costs fit a byte, lux_amd/csrc/engine.hip path_bytes):
 #     rows+ (pass +1 path ->, census cost fused): writes raw (1) and its path volume (1)
-#     vd+k / vd-k (one vertical or diagonal path): reads raw (1), writes its path volume (1)
+#     vd+ / vd- (the three vertical and diagonal paths of a pass, one launch): each reads raw (1) and writes its path
+#     volume (1)
 #     rows- (pass -1 path <-, census cost recomputed from the codes, the sum of the eight paths and the winner-take-all
 #     fused): seven volumes (7)
-#   u16 aggregate (every other request): rows+ raw 1 + agg 2; a sweep raw 1 + agg read 2 + agg write 2; rows- raw 1 +
-#     agg 2
-STAGE_BYTES_PV = {"rows+": 2, "rows-": 7, **{k: 2 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
-STAGE_BYTES_U16 = {"rows+": 3, "rows-": 3, **{k: 5 for k in ("vd+1", "vd+2", "vd+3", "vd-1", "vd-2", "vd-3")}}
+#   u16 aggregate (every other request): rows+ raw 1 + agg 2; a sweep raw 1 + agg read 2 + agg write 2 (three a pass);
+#     rows- raw 1 + agg 2
+STAGE_BYTES_PV = {"rows+": 2, "vd+": 6, "vd-": 6, "rows-": 7}
+STAGE_BYTES_U16 = {"rows+": 3, "vd+": 15, "vd-": 15, "rows-": 3}
 KERNEL_NAMES = {"rows+": "sgm_row_fwd (pass +1 path ->, census cost fused)",
                 "rows-": "sgm_row_bwd_wta (pass -1 path <-, eight-path sum and winner-take-all fused)",
-                "vd+1": "sgm_sweep<pass +1, path 1 (down-right)>", "vd+2": "sgm_sweep<pass +1, path 2 (down)>",
-                "vd+3": "sgm_sweep<pass +1, path 3 (down-left)>", "vd-1": "sgm_sweep<pass -1, path 1 (up-left)>",
-                "vd-2": "sgm_sweep<pass -1, path 2 (up)>", "vd-3": "sgm_sweep<pass -1, path 3 (up-right)>"}
+                "vd+": "the sweeps of pass +1 (down-right, down, down-left; sgm_sweeps_pv: one launch)",
+                "vd-": "the sweeps of pass -1 (up-left, up, up-right; sgm_sweeps_pv: one launch)"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -181,9 +181,9 @@ def bench_c0(np, torch, sgm, scene, threads, model, ncpu, reps=20):
 
 def _stage_of(kernel):
     """The bench stage a kernel's dispatches belong to (rocprofv3's demangled names), or None."""
-    m = re.search(r"sgm_sweep(?:_pv)?<(\d+), (-?1), ([123])", kernel)
+    m = re.search(r"sgm_sweeps?(?:_pv)?<(\d+), (-?1)", kernel)
     if m:
-        return f"vd{'+' if m.group(2) == '1' else '-'}{m.group(3)}"
+        return f"vd{'+' if m.group(2) == '1' else '-'}"
     if re.search(r"sgm_row_fwd(?:_pv)?<", kernel):
         return "rows+"
     if re.search(r"sgm_row_bwd_wta(?:_pv)?<", kernel):
@@ -743,6 +743,10 @@ def main():
             stages.setdefault(k, []).append(v)
     ctx.set_profiling(False)
     avg = {k: float(np.mean(v)) for k, v in stages.items()}
+    for sign in "+-":  # the library marks each path of a pass; the byte-volume form runs the three in one launch
+        parts = [avg.pop(f"vd{sign}{k}") for k in (1, 2, 3) if f"vd{sign}{k}" in avg]
+        if parts:
+            avg[f"vd{sign}"] = sum(parts)
     pv = path_bytes(D, RADIUS, P2)
     stage_bytes = STAGE_BYTES_PV if pv else STAGE_BYTES_U16
     dom = max((k for k in avg if k in stage_bytes), key=lambda k: avg[k])

@@ -740,16 +740,33 @@ __global__ __launch_bounds__(kBlock) void sgm_sweep(SweepArgs a)
     sweep_wave<NP, DIR, P, FIRST, MASKED, VAR, ADP>(
         a, __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)), threadIdx.x & 63);
 }
-// The sweep of a request with byte path volumes: its costs into its own volume a.pv.
+// The three sweeps of one pass of a byte-volume request in one launch: wavefront w runs chain w / 3 of path 1 + w % 3,
+// so every SIMD holds chains of all three directions. Alone, the vertical sweep has only W chains (about four
+// wavefronts a SIMD at 4160 columns), too few to hide its step's latency; together the pass streams at the rate
+// of the diagonals. Results are those of the three launches (the chains are independent). The rightward diagonals run
+// longest first (their chains by index run long, short, then long again, so the last long ones would start late); the
+// leftward ones keep the index order, which measured faster for them.
+struct SweepArgs3 {
+    SweepArgs s[3];
+};
+// One of the three alone (requests that share the device with others: their launches interleave with the other
+// requests' kernels instead).
 template <int NP, int DIR, int P, bool MASKED>
 __global__ __launch_bounds__(kBlock) void sgm_sweep_pv(SweepArgs a)
 {
-    int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-    // rightward diagonals longest first (their chains by index run long, short, then long again, so the last long
-    // ones would start late); the leftward ones keep the index order, which measured faster for them
+    const int c = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     constexpr int DX = DIR * (P == 1 ? 1 : P == 3 ? -1 : 0);
-    if constexpr (DX > 0) w = diag_order<1>(w, a.W, a.H);
-    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(a, w, threadIdx.x & 63);
+    sweep_wave<NP, DIR, P, false, MASKED, false, false, true>(a, DX > 0 ? diag_order<1>(c, a.W, a.H) : c,
+                                                                threadIdx.x & 63);
+}
+template <int NP, int DIR, bool MASKED>
+__global__ __launch_bounds__(kBlock) void sgm_sweeps_pv(SweepArgs3 a)
+{
+    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int p = w % 3, c = w / 3, lane = threadIdx.x & 63;
+    if (p == 0) sweep_wave<NP, DIR, 1, false, MASKED, false, false, true>(a.s[0], DIR > 0 ? diag_order<1>(c, a.s[0].W, a.s[0].H) : c, lane);
+    else if (p == 1) sweep_wave<NP, DIR, 2, false, MASKED, false, false, true>(a.s[1], c, lane);
+    else sweep_wave<NP, DIR, 3, false, MASKED, false, false, true>(a.s[2], DIR < 0 ? diag_order<1>(c, a.s[2].W, a.s[2].H) : c, lane);
 }
 
 // ------------------------------------------------------------------ horizontal paths (rows)
@@ -2671,16 +2688,24 @@ hipError_t run_sgm(const Geometry& g, const SgmBuffers& b, const SgmOutputs& o, 
         constexpr int DIR = decltype(dirc)::value;
         if constexpr (kPv) {
             if (pv) {  // each path into its own byte volume: 1..3 (pass +1), 4..6 (pass -1)
-                SweepArgs q = a;
-                q.pvs = b.pvs;
-                q.pv = b.pv[DIR > 0 ? 1 : 4];
-                sgm_sweep_pv<NP, DIR, 1, MASKED><<<grid(nd), wg, 0, s>>>(q);
+                SweepArgs3 q;
+                for (int k = 0; k < 3; k++) {
+                    q.s[k] = a;
+                    q.s[k].pvs = b.pvs;
+                    q.s[k].pv = b.pv[(DIR > 0 ? 1 : 4) + k];
+                }
+                if (b.solo) {  // the device to itself: the three in one launch
+                    sgm_sweeps_pv<NP, DIR, MASKED><<<grid(3 * std::max(nd, nv)), wg, 0, s>>>(q);
+                    mark();  // the next two stages are empty
+                    mark();
+                    mark();
+                    return;
+                }
+                sgm_sweep_pv<NP, DIR, 1, MASKED><<<grid(nd), wg, 0, s>>>(q.s[0]);
                 mark();
-                q.pv = b.pv[DIR > 0 ? 2 : 5];
-                sgm_sweep_pv<NP, DIR, 2, MASKED><<<grid(nv), wg, 0, s>>>(q);
+                sgm_sweep_pv<NP, DIR, 2, MASKED><<<grid(nv), wg, 0, s>>>(q.s[1]);
                 mark();
-                q.pv = b.pv[DIR > 0 ? 3 : 6];
-                sgm_sweep_pv<NP, DIR, 3, MASKED><<<grid(nd), wg, 0, s>>>(q);
+                sgm_sweep_pv<NP, DIR, 3, MASKED><<<grid(nd), wg, 0, s>>>(q.s[2]);
                 mark();
                 return;
             }
