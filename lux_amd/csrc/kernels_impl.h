@@ -798,46 +798,46 @@ struct RowArgs {
 };
 
 // Census codes along a row. Lane l owns planes d = lo .. lo+2*NP-1 (lo = 2*NP*l) and keeps the window
-// win[e] = cr(max(x - lo - e, 0)) of the right-image codes its planes compare with at the current x. Moving
-// to x+1 the window shifts by one code: win[e] <- win[e-1], and win[0] <- the left neighbour lane's
-// win[2*NP-1] = cr(max(x + 1 - lo, 0)) (DPP wave_shr), lane 0 taking cr(x+1) itself. So a step reads two
-// wave-uniform codes, cl(x) and cr(x+1), instead of 2*NP per lane.
-__device__ __forceinline__ uint64_t shr1_64(uint64_t v)
-{
-    const uint32_t lo = shr1((uint32_t)v, 0u), hi = shr1((uint32_t)(v >> 32), 0u);
-    return ((uint64_t)hi << 32) | lo;
-}
+// win[e] = cr(max(x - lo - e, 0)) of the right-image codes its planes compare with at the current x.
+// - Moving to x+1 the window shifts by one code: win[e] <- win[e-1], and win[0] <- the left neighbour lane's
+//   win[2*NP-1] = cr(max(x + 1 - lo, 0)) (DPP wave_shr), lane 0 taking cr(x+1) itself. So a step reads two
+//   wave-uniform codes, cl(x) and cr(x+1), instead of 2*NP per lane.
+// - Moving to x-1: win[e] <- win[e+1], and win[2*NP-1] <- the right neighbour lane's win[0] = cr(max(x - lo - 2*NP,
+//   0)) (DPP wave_shl), lane 63 taking cr(max(x - 128*NP, 0)).
+// The window is kept rotated instead of shifted: logical element e sits in win[(e + rot) % E], E = 2 * NP, so a
+// step writes the one element that changes and moves none. A row path passes rot from its step index modulo E,
+// which the unrolled step loop makes a constant (every step group starts at a multiple of E).
+// Advance (x -> x+1): rot = (E - t % E) % E at step t; the new element takes the slot of the one dropped.
 template <int NP>
-__device__ __forceinline__ void census_advance(uint64_t (&win)[2 * NP], uint64_t fresh, int lane)
+__device__ __forceinline__ void census_advance_rot(uint64_t (&win)[2 * NP], uint64_t fresh, int rot)
 {
-    // wave_shr:1 with lane 0 (which has no source lane) taking `fresh` as the DPP old value: no select
-    const uint64_t last = win[2 * NP - 1];
+    constexpr int E = 2 * NP;
+    const int k = (rot + E - 1) % E;
+    const uint64_t last = win[k];
     const uint32_t lo = shr1((uint32_t)last, (uint32_t)fresh), hi = shr1((uint32_t)(last >> 32), (uint32_t)(fresh >> 32));
-#pragma unroll
-    for (int e = 2 * NP - 1; e > 0; e--) win[e] = win[e - 1];
-    win[0] = ((uint64_t)hi << 32) | lo;
+    win[k] = ((uint64_t)hi << 32) | lo;
 }
-// The window of the cell to the left (x -> x-1): win[e] <- win[e+1], and win[2*NP-1] <- the right neighbour lane's
-// win[0] = cr(max(x - lo - 2*NP, 0)) (DPP wave_shl), lane 63 taking `fresh` = cr(max(x - 128*NP, 0)).
+// Retreat (x -> x-1): rot = t % E at step t.
 template <int NP>
-__device__ __forceinline__ void census_retreat(uint64_t (&win)[2 * NP], uint64_t fresh)
+__device__ __forceinline__ void census_retreat_rot(uint64_t (&win)[2 * NP], uint64_t fresh, int rot)
 {
-    const uint64_t first = win[0];
+    const int k = rot;
+    const uint64_t first = win[k];
     const uint32_t lo = shl1((uint32_t)first, (uint32_t)fresh), hi = shl1((uint32_t)(first >> 32), (uint32_t)(fresh >> 32));
-#pragma unroll
-    for (int e = 0; e < 2 * NP - 1; e++) win[e] = win[e + 1];
-    win[2 * NP - 1] = ((uint64_t)hi << 32) | lo;
+    win[k] = ((uint64_t)hi << 32) | lo;
 }
 // Census Hamming costs popcount(cl(x) ^ cr(max(x-d, 0))) for d < D, 255 above (a plane left of the image
 // repeats the last in-image plane, DenseStereo.cs:273-274), widened for the path step and packed as bytes.
 // FULL: every lane's planes are below D (D = 128 * NP), so no plane takes 255.
+// rot: the window is rotated (census_advance_rot / census_retreat_rot), element e in win[(e + rot) % (2 * NP)].
 template <int NP, bool FULL = false>
 __device__ __forceinline__ void census_cost(uint64_t cl, const uint64_t (&win)[2 * NP], int D, int lo, Cell<NP>& raw,
-                                            uint32_t (&bytes)[NP / 2 + (NP == 1)])
+                                            uint32_t (&bytes)[NP / 2 + (NP == 1)], int rot = 0)
 {
     uint32_t v[2 * NP];
 #pragma unroll
-    for (int e = 0; e < 2 * NP; e++) v[e] = (FULL || lo + e < D) ? (uint32_t)__popcll(cl ^ win[e]) : 255u;
+    for (int e = 0; e < 2 * NP; e++)
+        v[e] = (FULL || lo + e < D) ? (uint32_t)__popcll(cl ^ win[(e + rot) % (2 * NP)]) : 255u;
 #pragma unroll
     for (int k = 0; k < NP; k++) raw.v[k] = v[2 * k] | (v[2 * k + 1] << 16);
     if constexpr (NP == 1) {
@@ -925,7 +925,9 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
             const uint64_t cl = rl64(bl), cr1 = rl64(br);
             uint32_t bytes[NP / 2 + (NP == 1)];
             // byte path volumes have no padding planes (cap = D), so an unmasked wave's planes are all below D
-            census_cost<NP, PV && !MASKED>(cl, win, a.D, lo, raw, bytes);
+            // the window rotates with the step (x % (2 NP) == j % (2 NP): step groups start at multiples of PF)
+            const int rot = (2 * NP - j % (2 * NP)) % (2 * NP);
+            census_cost<NP, PV && !MASKED>(cl, win, a.D, lo, raw, bytes, rot);
             uint8_t* o = a.raw_out + (rowcell + x) * cs + lo;
             if (st_raw) {
                 if constexpr (NP == 1) *(uint16_t*)o = (uint16_t)bytes[0];
@@ -934,7 +936,7 @@ __device__ __forceinline__ void row_fwd_wave(const RowArgs& a, int y, int lane)
                     for (int k = 0; k < NP / 2; k++) ((uint32_t*)o)[k] = bytes[k];
                 }
             }
-            census_advance<NP>(win, cr1, lane);
+            census_advance_rot<NP>(win, cr1, rot);
         } else {
             widen<NP>(rb[j], raw);
             ld_raw_w<NP>(a.raw + (rowcell + xp) * cs + lol, rb[j]);
@@ -1117,8 +1119,9 @@ __device__ __forceinline__ void row_bwd_wave(const RowArgs& a, int y, int lane)
         if constexpr (COST) {
             const int q = (W - 1 - x) & 63;
             uint32_t bytes[NP / 2 + (NP == 1)];
-            census_cost<NP, !MASKED>(rl64(bl, q), win, D, lo, raw, bytes);  // COST is PV: D = 128 * NP unmasked
-            census_retreat<NP>(win, rl64(bf, q));
+            const int rot = j % (2 * NP);  // the step (W - 1 - x) is j modulo PF: blocks start at multiples of 64
+            census_cost<NP, !MASKED>(rl64(bl, q), win, D, lo, raw, bytes, rot);  // COST is PV: D = 128 * NP unmasked
+            census_retreat_rot<NP>(win, rl64(bf, q), rot);
         } else {
             widen<NP>(rb[j], raw);
         }
