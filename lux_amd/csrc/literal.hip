@@ -166,14 +166,20 @@ __device__ __forceinline__ uint32_t wave_pmin(uint32_t v)
 struct LitCell {
     int start, cap;
     uint32_t g;
+    int prev_spill;  // the cell two rows before in pass +1 (the slot's previous occupant) writes past its path-3 segment
 };
+// Whether a range's path-3 write (and so the stale cleanup of it) runs past its segment into the next slot.
+__device__ __forceinline__ bool lit_spills(const LitArgs& a, int start, int cap) { return start - a.minlo + cap > a.M + 1; }
+// cell2: the cell two rows before in pass order at the same column, or -1.
 template <bool ADP>
-__device__ __forceinline__ LitCell lit_cell(const LitArgs& a, size_t cell)
+__device__ __forceinline__ LitCell lit_cell(const LitArgs& a, size_t cell, long cell2)
 {
     LitCell c;
     c.start = a.start ? (int)a.start[cell] : 0;
     c.cap = a.start ? (int)a.cap[cell] : a.ucap;
     c.g = ADP ? a.ap.guid[cell] : 0u;
+    c.prev_spill = 0;
+    if (cell2 >= 0) c.prev_spill = a.start ? lit_spills(a, (int)a.start[cell2], (int)a.cap[cell2]) : lit_spills(a, 0, a.ucap);
     return c;
 }
 
@@ -450,10 +456,12 @@ __device__ __forceinline__ void lit_wait4(const LitArgs& a, const int (&row2)[4]
     }
 }
 
-__device__ __forceinline__ int lit_need(const LitArgs& a, int dir, int j, int i, int r, int c, int row2)
+// reach: columns past c (in pass order) whose visits of row2 the visit waits for (kLitReach, or less when the
+// visit's ranges show that fewer can share a word with it: lit_reach).
+__device__ __forceinline__ int lit_need(const LitArgs& a, int dir, int j, int i, int r, int c, int row2, int reach = kLitReach)
 {
-    int mp = std::min(c + kLitReach, a.W - 1);
-    if (lit_rev(a, dir, row2 % a.T) && c + kLitReach >= a.W - 1) mp = a.W;
+    int mp = std::min(c + reach, a.W - 1);
+    if (lit_rev(a, dir, row2 % a.T) && c + reach >= a.W - 1) mp = a.W;
     return std::min(lit_before(a, dir, j, i, r, row2), mp + 1);
 }
 
@@ -476,23 +484,43 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
         const int cur = (r & 1) == 0 ? 1 : 0, oth = 1 - cur;  // :219
         const int nvis = W + lit_rev(a, DIR, r);
         int seen[4] = {0, 0, 0, 0};  // lane 0: progress last read of rows row-2, row-1, row+1, row+2
-        LitCell next = lit_cell<ADP>(a, (size_t)y * W + (DIR > 0 ? 0 : W - 1));
+        // pass +1 with even T: the slot's previous occupant is the cell two rows up (halves alternate row by row)
+        const bool prev2 = DIR > 0 && (T & 1) == 0 && y >= 2;
+        const auto cell_at = [&](int cc0) { return (size_t)y * W + (DIR > 0 ? cc0 : W - 1 - cc0); };
+        LitCell next = lit_cell<ADP>(a, cell_at(0), prev2 ? (long)cell_at(0) - 2L * W : -1L);
         for (int pos = 0; pos < nvis; pos++) {
             const int c = pos < W ? pos : W - 1;
             const int i = pos < W ? (pos + r) / T : a.gwN - 1;
             const LitCell cc = next;
             if (pos + 1 < nvis) {  // in flight during this visit's waits
                 const int c1 = pos + 1 < W ? pos + 1 : W - 1;
-                next = lit_cell<ADP>(a, (size_t)y * W + (DIR > 0 ? c1 : W - 1 - c1));
+                next = lit_cell<ADP>(a, cell_at(c1), prev2 ? (long)cell_at(c1) - 2L * W : -1L);
             }
             if (lane == 0 && pos < W) {
+                // The reach this visit needs (with even T, whose halves alternate row by row; odd T keeps
+                // kLitReach). Pass -1: one column, since the words a visit shares beyond its neighbours' slots (the
+                // next slot in memory) lie behind it in pass order. Pass +1: towards rows y+-1, one column, two when
+                // its path 3 reads past its segment into the next slot (the slot two columns on, written by that
+                // column's visit); towards rows y+-2 (the same half), two when its path-3 write or its cleanup of
+                // the previous occupant's range spills into the next slot, which the visit two columns on reads
+                // through path 0.
+                int reach1 = kLitReach, reach2 = kLitReach;
+                if ((T & 1) == 0) {
+                    if (DIR < 0) {
+                        reach1 = reach2 = 1;
+                    } else {
+                        const int q = cc.start - a.minlo;
+                        reach1 = 1 + (q + cc.cap >= a.M + 1 ? 1 : 0);
+                        reach2 = 1 + (lit_spills(a, cc.start, cc.cap) || cc.prev_spill ? 1 : 0);
+                    }
+                }
                 int rows2[4], need[4];
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
                     const int row2 = row + (d < 2 ? d - 2 : d - 1);
                     const bool in = row2 >= 0 && row2 < a.H;
                     rows2[d] = in ? row2 : row;  // a row outside the image: read this row's word, wait for nothing
-                    need[d] = in ? lit_need(a, DIR, j, i, r, c, row2) : 0;
+                    need[d] = in ? lit_need(a, DIR, j, i, r, c, row2, (d == 1 || d == 2) ? reach1 : reach2) : 0;
                 }
                 lit_wait4(a, rows2, need, seen, dead);
             } else if (lane == 0) {

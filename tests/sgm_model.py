@@ -234,9 +234,31 @@ def wta_sub_lr_model(agg, D, tol=1):
 
 
 # ------------------------------------------------------------------ the line-buffer-literal schedule
-# A restatement of literal.hip's row sequences and waits (lit_rev, lit_before, the reach of 3 columns), so the
-# tests can check on the CPU that every order the waits allow gives the reference's results.
+# A restatement of literal.hip's row sequences and waits (lit_rev, lit_before, lit_need and the reach of each visit),
+# so the tests can check on the CPU that every order the waits allow gives the reference's results.
 LIT_REACH = 2
+
+
+def literal_flags(start, count, align):
+    """Per cell [H][W] (image coordinates): pe, whether its path 3 reads past its segment into the next slot
+    (start - MinLo + cap >= MaxHi + 1), and sp, whether its path-3 write spills into the next slot (> MaxHi + 1)."""
+    caps = ((count.astype(np.int64) + align - 1) // align) * align
+    minlo = int(start.min())
+    maxhi = int(-(-int((start.astype(np.int64) + count).max()) // align) * align)
+    q = start.astype(np.int64) - minlo
+    return q + caps >= maxhi + 1, q + caps > maxhi + 1
+
+
+def literal_reach(T, d, row, row2, x, y, flags):
+    """Columns past a visit (in pass order) whose visits of row2 it waits for: literal.hip's reach1 / reach2."""
+    if flags is None or T % 2:
+        return LIT_REACH
+    if d < 0:
+        return 1
+    pe, sp = flags
+    if abs(row2 - row) == 1:
+        return 1 + int(pe[y, x])
+    return 1 + int(sp[y, x] or (y >= 2 and sp[y - 2, x]))
 
 
 def literal_rows(W, H, T, xsw, d):
@@ -257,8 +279,9 @@ def literal_rows(W, H, T, xsw, d):
     return rows
 
 
-def literal_need(W, H, T, xsw, d, row, pos, row2):
-    """How many of row2's visits the kernel waits for before row's visit pos."""
+def literal_need(W, H, T, xsw, d, row, pos, row2, reach=None):
+    """How many of row2's visits the kernel waits for before row's visit pos (reach: LIT_REACH when None)."""
+    reach = LIT_REACH if reach is None else reach
     gwN = (W + 2 * T - 1) // T
     j, r = divmod(row, T)
     jj, rr = divmod(row2, T)
@@ -273,8 +296,8 @@ def literal_need(W, H, T, xsw, d, row, pos, row2):
         before = W + rev2
     else:
         before = min(max(((i + 1) if rr < r else i) * T - rr, 0), W)
-    mp = min(c + LIT_REACH, W - 1)
-    if rev2 and c + LIT_REACH >= W - 1:
+    mp = min(c + reach, W - 1)
+    if rev2 and c + reach >= W - 1:
         mp = W
     return min(before, mp + 1)
 
@@ -294,9 +317,10 @@ def literal_wait_rows(W, H, T, row, pos):
     return sorted(rows)
 
 
-def literal_interleave(W, H, T, xsw, rng, resident):
+def literal_interleave(W, H, T, xsw, rng, resident, flags=None):
     """One order of both passes that the kernel's waits allow: `resident` wavefronts take rows by ticket and a
-    random runnable one advances. Returns the visits as (x, y, cur) rows, and n_plus. Raises on a deadlock."""
+    random runnable one advances. flags: literal_flags of the layer (the kernel's per-visit reach), or None for
+    LIT_REACH everywhere. Returns the visits as (x, y, cur) rows, and n_plus. Raises on a deadlock."""
     out = []
     n_plus = 0
     for d in (1, -1):
@@ -311,7 +335,10 @@ def literal_interleave(W, H, T, xsw, rng, resident):
             ready = []
             for row in active:
                 pos = prog[row]
-                if all(prog[r2] >= literal_need(W, H, T, xsw, d, row, pos, r2)
+                x, y = rows[row][min(pos, len(rows[row]) - 1)][:2]
+                # the second visit of the border column keeps LIT_REACH (it waits on its whole band)
+                if all(prog[r2] >= literal_need(W, H, T, xsw, d, row, pos, r2,
+                                                None if pos >= W else literal_reach(T, d, row, r2, x, y, flags))
                        for r2 in literal_wait_rows(W, H, T, row, pos)):
                     ready.append(row)
             if not ready:

@@ -56,7 +56,8 @@ def test_visit_order_entry_in_the_reference_order():
 
 
 SPILL_LAYERS = [(9, 7, 4, False, 40, 13, 8), (13, 11, 4, True, 24, 11, 8), (16, 12, 3, False, 30, 9, 2),
-                (12, 17, 8, False, 64, 21, 8), (21, 9, 2, False, 16, 7, 8), (6, 14, 5, True, 48, 30, 8)]
+                (12, 17, 8, False, 64, 21, 8), (21, 9, 2, False, 16, 7, 8), (6, 14, 5, True, 48, 30, 8),
+                (11, 16, 4, False, 63, 21, 8), (14, 19, 2, False, 55, 39, 8), (17, 13, 6, True, 45, 17, 8)]
 
 
 @pytest.mark.parametrize("W,H,T,xsw,planes,cmax,align", SPILL_LAYERS)
@@ -66,8 +67,9 @@ def test_allowed_orders_give_the_reference_results(W, H, T, xsw, planes, cmax, a
     assert spill > 0  # the layer exercises the words past the segments
     kw = dict(T=T, align=align, p1=8, p2=96, xsw=xsw, keep_pass1=True)
     agg, agg1, wta = o.layer(raw, W, H, start, count, **kw)
-    for seed, resident in ((1, M.literal_depth(W, T)), (2, M.literal_depth(W, T)), (3, 10 ** 6)):
-        visits, n_plus = M.literal_interleave(W, H, T, xsw, np.random.default_rng(seed), resident)
+    flags = M.literal_flags(start, count, align)
+    for seed, resident in ((1, M.literal_depth(W, T)), (2, M.literal_depth(W, T)), (3, 10 ** 6), (4, 10 ** 6)):
+        visits, n_plus = M.literal_interleave(W, H, T, xsw, np.random.default_rng(seed), resident, flags)
         got, got1, gw = o.layer(raw, W, H, start, count, visits=visits, n_plus=n_plus, **kw)
         np.testing.assert_array_equal(got1, agg1)
         np.testing.assert_array_equal(got, agg)
@@ -83,7 +85,8 @@ def test_allowed_orders_adaptive_p2():
     kw = dict(T=T, align=align, p1=1, guidance=guid, gw=(0.3, 0.02, 0.05, 0.0), p2_scale=40.0, adaptive=True,
               keep_pass1=True)
     agg, agg1, wta = o.layer(raw, W, H, start, count, **kw)
-    visits, n_plus = M.literal_interleave(W, H, T, False, np.random.default_rng(9), M.literal_depth(W, T))
+    visits, n_plus = M.literal_interleave(W, H, T, False, np.random.default_rng(9), M.literal_depth(W, T),
+                                          M.literal_flags(start, count, align))
     got, got1, gw = o.layer(raw, W, H, start, count, visits=visits, n_plus=n_plus, **kw)
     np.testing.assert_array_equal(got1, agg1)
     np.testing.assert_array_equal(got, agg)
@@ -104,4 +107,25 @@ def test_a_shorter_reach_breaks_the_order(monkeypatch):
         visits, n_plus = M.literal_interleave(W, H, T, False, np.random.default_rng(seed), 10 ** 6)
         got, _, _ = o.layer(raw, W, H, start, count, visits=visits, n_plus=n_plus, **kw)
         differs += int(not np.array_equal(got, agg))
+    assert differs > 0
+
+
+@pytest.mark.parametrize("drop", ["pe", "sp"])
+def test_the_per_visit_reach_has_teeth(drop):
+    # each of the two conditions that lengthen a visit's reach to two columns is needed: without it, orders the
+    # waits would allow give other results on spilling layers
+    cases = {"pe": (11, 16, 4, 63, 21, 8), "sp": (10, 20, 4, 48, 37, 8)}
+    W, H, T, planes, cmax, align = cases[drop]
+    differs = 0
+    for trial in range(4):
+        rng = np.random.default_rng(trial)
+        start, count, raw, _ = _layer_inputs(rng, W, H, planes, cmax, align)
+        kw = dict(T=T, align=align, p1=8, p2=96, keep_pass1=True)
+        agg, _, _ = o.layer(raw, W, H, start, count, **kw)
+        pe, sp = M.literal_flags(start, count, align)
+        flags = (np.zeros_like(pe), sp) if drop == "pe" else (pe, np.zeros_like(sp))
+        for seed in range(6):
+            visits, n_plus = M.literal_interleave(W, H, T, False, np.random.default_rng(seed), 10 ** 6, flags)
+            got, _, _ = o.layer(raw, W, H, start, count, visits=visits, n_plus=n_plus, **kw)
+            differs += int(not np.array_equal(got, agg))
     assert differs > 0
