@@ -53,6 +53,9 @@ constexpr int kRelocMasks = (kJobWords + 31) / 32;
 static_assert(sizeof(ExecJob) % 8 == 0, "the descriptor is relocated 64-bit word by word");
 constexpr uint32_t kDLoopStages = kAll;
 constexpr uint64_t kPoolAlign = 4096;
+// Workers that always look for work; the others look only while a live request has stages at least half as wide as
+// their index (claims beyond a stage's items only contend on its counter, and every idle look reads the slot tables).
+constexpr uint32_t kAwake = 1024;
 
 // Items one claim takes: the short items (staging chunks, border visits and sums, finish rows) in eights, the chains
 // (row paths, sweeps, border chains) and census row blocks one at a time.
@@ -84,6 +87,7 @@ struct DSlot {                     // device memory; written by the scheduler, p
     uint64_t ready;                // generation << 32 | stages whose inputs are complete and flushed
     uint32_t gen;                  // admissions of this slot
     uint32_t live, total, admit_epoch, ring, prio;
+    uint32_t width;                // claims the widest stage takes (items over the stage's batch)
     uint32_t start[X_NSTAGES + 1]; // first item of each stage
     uint32_t done[X_NSTAGES];
     uint32_t epoch[X_NSTAGES];     // flush epoch raised when the stage completed
@@ -102,6 +106,7 @@ struct DState {                    // device memory, kept across launches of the
     uint32_t work;                 // bumped when work may have become ready (admission, stage completed or flushed)
     uint32_t fault;                // the watchdog saw no item complete for kStallTicks with requests running
     uint32_t order[kDSlots];       // live slots, smallest request first
+    uint32_t span;                 // the widest live request's width: workers at or past kAwake and 2 * span park
     uint64_t used, peak;           // pool bytes
     DSlot slot[kDSlots];
 };
@@ -148,6 +153,7 @@ __device__ __forceinline__ void dl_order(const DArgs& A, int lane)
     DState* S = A.st;
     uint64_t last = 0;  // (prio << 8 | slot) + 1 of the previous entry
     int n = 0;
+    uint32_t span = 0;
     for (;; n++) {
         uint64_t best = ~0ull;
         for (int k = 0; k < A.nslots; k++) {
@@ -157,10 +163,14 @@ __device__ __forceinline__ void dl_order(const DArgs& A, int lane)
         }
         if (best == ~0ull) break;
         if (lane == 0) ast(&S->order[n], (uint32_t)((best - 1) & 0xff));
+        span = max(span, ald(&S->slot[(best - 1) & 0xff].width));
         last = best;
     }
     __builtin_amdgcn_s_waitcnt(0);
-    if (lane == 0) ast(&S->nlive, (uint32_t)n);
+    if (lane == 0) {
+        ast(&S->span, span);
+        ast(&S->nlive, (uint32_t)n);
+    }
 }
 
 // First fit of `bytes` in the pool among the live slots' blocks; ~0 if none.
@@ -239,12 +249,13 @@ __device__ __forceinline__ void dl_scheduler(const DArgs& A, int x, int lane)
                 if ((R->reloc[w >> 5] >> (w & 31)) & 1u) v += add;
                 dst[w] = v;
             }
-            uint32_t total = 0;
+            uint32_t total = 0, width = 0;
             const uint32_t gen = ald(&D->gen) + 1;
             if (lane == 0) {
                 for (int s = 0; s < X_NSTAGES; s++) {
                     D->start[s] = total;
                     total += R->job.n[s];
+                    width = max(width, (R->job.n[s] + stage_batch(s) - 1) / stage_batch(s));
                     D->done[s] = 0;
                     D->epoch[s] = 0;
                     D->cnt[s] = (uint64_t)gen << 32;
@@ -253,6 +264,7 @@ __device__ __forceinline__ void dl_scheduler(const DArgs& A, int x, int lane)
                 D->ready = (uint64_t)gen << 32;
                 D->start[X_NSTAGES] = total;
                 D->prio = R->prio;
+                D->width = width;
                 D->ring = seq % kRing;
                 D->base = base;
                 D->bytes = bytes;
@@ -430,7 +442,7 @@ __device__ __forceinline__ void dl_flusher(const DArgs& A, int x, int lane)
     }
 }
 
-__device__ __forceinline__ void dl_worker(const DArgs& A, int x, int lane)
+__device__ __forceinline__ void dl_worker(const DArgs& A, int x, int lane, uint32_t w)
 {
     DState* S = A.st;
     uint32_t my_inv = 0, backoff = 1;
@@ -464,6 +476,11 @@ __device__ __forceinline__ void dl_worker(const DArgs& A, int x, int lane)
             } else {
                 fk = -1;
             }
+        }
+        if (!got && w >= kAwake && w >= 2 * ald(&S->span)) {
+            // parked: a look about every 80 us, with no read of the work counter or the slot tables
+            for (int i = 0; i < 16; i++) __builtin_amdgcn_s_sleep(127);
+            continue;
         }
         const uint32_t seen = ald(&S->work);
         if (!got) got = dl_claim(A, lane, &k, &s, &item, &cnt, &n, &gen);
@@ -518,7 +535,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     }
     if (blockIdx.x == 0) dl_scheduler(A, x, lane);
     else if (__builtin_amdgcn_readfirstlane((int)won)) dl_flusher(A, x, lane);
-    else dl_worker(A, x, lane);
+    else dl_worker(A, x, lane, blockIdx.x);
     // leave with nothing unwritten in the L2 (an XCD with no wavefront left counts as flushed)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) __hip_atomic_fetch_add(&A.st->sync.live[x], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
