@@ -116,6 +116,35 @@ def cpu_baseline(left, right, rows, threads, d=D, radius=RADIUS, bits=BITS, p1=P
     return cells * 8 / ts[k] / 1e9, ts[k], parts[k], sorted(ts)
 
 
+def hbm_ceilings(torch, gib=4, reps=5):
+    """What plain streaming kernels reach on this device (torch's own copy, fill and max kernels over `gib` GiB), for
+    reading the roofline fractions: the published 8 TB/s is not reached by any access mix, and a mix of reads and
+    writes (the sweeps) streams slower than reads alone (the backward row path)."""
+    n = gib << 30
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def rate(fn, moved):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return round(moved * reps / (ev[0].elapsed_time(ev[1]) / 1e3) / 1e9, 1)
+
+    out = {"copy_gbs": rate(lambda: b.copy_(a), 2 * n),
+           "read_gbs": rate(lambda: a.view(torch.int64).max(), n),
+           "write_gbs": rate(lambda: b.fill_(3), n),
+           "how": f"torch copy_ (reads + writes), int64 max (reads) and fill_ (writes) over {gib} GiB, "
+                  f"mean of {reps} after one"}
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic():
     """The counted HBM bytes of one request (profiles/traffic.json), used only when the file was made from these kernel
     sources: (data or None, a note saying which)."""
@@ -775,6 +804,8 @@ def main():
 
     ctx.close()
     del dl, dr
+    print("bench: HBM streaming ceilings", file=sys.stderr, flush=True)
+    ceilings = hbm_ceilings(torch)
     forms = [f for f in args.forms.split(",") if f]
     batched = served = None
     if "batch" in forms:
@@ -840,6 +871,7 @@ def main():
                          "stages": stage_bytes,
                          "achieved": round(design_bpc * cells / (ms / 1e3) / 1e9, 1),
                          "frac": round(design_bpc * cells / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "hbm_measured": ceilings,
         "stage_ms": {k: round(v, 4) for k, v in avg.items()},
         "accuracy": {"valid_frac": round(float(v.mean()), 4), "within1_of_truth_on_valid": round(acc, 4)},
         "wall_s": round(t_wall, 3),
