@@ -25,18 +25,26 @@
 // border cells fall in the previous tile), so it waits on every row of its band.
 //
 // Progress. Waits only point to smaller keys, and a row never waits on a row of a later band, so a wait for a
-// later row reaches at most T rows ahead. Rows are taken by ticket, so with T + 4 wavefronts resident (the
-// host checks occupancy) every wait ends. A bounded spin sets the error word instead of hanging, and later
-// waits are skipped, so the grid always drains.
+// later row reaches at most T rows ahead. Rows are taken by ticket, so with T + 4 wavefronts resident (on the home
+// XCD when the pass runs on one; the host checks one XCD's occupancy) every wait ends. A bounded spin sets the error
+// word instead of hanging, and later waits are skipped, so the grid always drains.
 //
-// Memory. All line-buffer, minimum, range, guidance and progress words go through agent-scope accesses (loads and
-// stores with sc1), as the fused band tiles exchange their boundary states; a visit's stores are drained
-// (s_waitcnt) before its progress store. The aggregate of a cell is read and written by its own row's wavefront
-// only, with plain accesses (the second visit of the border column takes the first visit's sums from registers).
+// Memory. All line-buffer, minimum, range, guidance and progress words go through buffer loads and stores with a
+// cache policy chosen per launch (LOC), and a visit's stores are drained (s_waitcnt) before its progress store.
+//   - One XCD (LOC), when it holds every row of the layer at once: the first wavefront to start names its XCD the
+//     pass's home, and wavefronts that start on another leave at once. The words then go through that XCD's L2, which
+//     all its CUs share: loads and stores with sc0 and nt (the loads miss in the CU's L1; with sc0 alone a load may
+//     hit a line the L1 kept from an earlier read). A hand-over between two wavefronts (a store, then a poll that
+//     sees it) measured 0.21 us this way.
+//   - All XCDs otherwise (more rows in flight than one XCD holds): sc1, coherent across XCDs, 0.53 us a hand-over.
+//   LUX_SGM_LIT_LOCAL=0 or 1 forces either. Other kernels see the words through the kernel boundaries' write-back and
+//   invalidation, as any kernel's output. The aggregate of a cell is read and written by its own row's wavefront
+//   only, with plain accesses (the second visit of the border column takes the first visit's sums from registers).
 // One wavefront per row, planes across lanes (NPL per lane; NPL 0: 64-plane chunks in turn, caps up to 1024).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "kernels.h"
 #include "p2_dev.h"
@@ -45,7 +53,9 @@ namespace lux {
 namespace {
 
 constexpr int kLitReach = 2;   // columns either side whose visits may share line-buffer words with a visit
-constexpr int kLitGrid = 2048; // persistent single-wavefront workgroups per pass (rows go by ticket)
+constexpr int kLitGrid = 2048; // persistent single-wavefront workgroups of the home XCD per pass (rows go by ticket)
+constexpr int kLitXcds = 8;    // workgroups go to the XCDs in turn: the grid holds kLitGrid for each
+constexpr int kLitXcdCus = 32; // CUs of one XCD
 
 struct LitArgs {
     int W, H, T, gwN, rev_plus, align;
@@ -68,20 +78,17 @@ struct LitArgs {
     uint32_t* progress;              // [H] visits completed, per row in pass order
     uint32_t* ticket;
     uint32_t* err;
+    uint32_t* home;                  // the XCD the pass runs on (~0 until the first wavefront names its own)
 };
 
-__device__ __forceinline__ void ast(uint16_t* p, uint16_t v)
+__device__ __forceinline__ int lit_xcc()
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(v));
+    return (int)(v & 7u);
 }
-__device__ __forceinline__ uint32_t ald32(const uint32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void ast32(uint32_t* p, uint32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// A wave-uniform value into a scalar register (uniform words then take scalar arithmetic and branches).
+__device__ __forceinline__ int lit_u(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Whether tile row r ends with the second visit of the last column (first column in pass -1): the last
 // tile's row starts past the image (:213-216); LUX_SGM_XSW's clamp to W removes it in pass +1.
@@ -102,13 +109,16 @@ __device__ __forceinline__ int lit_before(const LitArgs& a, int dir, int j, int 
     return (int)std::min(std::max(lim, 0L), (long)a.W);
 }
 
-// A visit's reads of the shared words: loads with the sc1 bit (what the relaxed agent-scope atomic loads compile to)
-// through buffer resources. To the compiler they are plain loads, so the reads of a visit issue together after its
-// acquire instead of one round trip each. A read past a resource's end returns 0: for the line buffer that is Rd's
-// rule past the end of the buffer (:318-324); every index read is >= 0 (the lowest is a lead word).
-constexpr int kSc1 = 16;  // cache policy: sc1
+// A visit's reads of the shared words: loads with the launch's cache policy through buffer resources. To the
+// compiler they are plain loads, so the reads of a visit issue together after its acquire instead of one round trip
+// each (and a poll loop keeps its load inside the loop only behind a compiler barrier: lit_poll_barrier). A read past
+// a resource's end returns 0: for the line buffer that is Rd's rule past the end of the buffer (:318-324); every index
+// read is >= 0 (the lowest is a lead word).
+constexpr int kLitLoc = 3;  // cache policy on one XCD (LOC): sc0 | nt
+constexpr int kLitDev = 16; // across XCDs: sc1
+__device__ __forceinline__ void lit_poll_barrier() { asm volatile("" ::: "memory"); }
 struct LitRes {
-    __amdgpu_buffer_rsrc_t L, Min, RS, RC, Pix;
+    __amdgpu_buffer_rsrc_t L, Min, RS, RC, Pix, P;
 };
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t lit_rsrc(const void* base, long bytes)
 {
@@ -122,23 +132,32 @@ __device__ __forceinline__ LitRes lit_res(const LitArgs& a)
     r.RS = lit_rsrc(a.RS, 2L * a.halfR * 2);
     r.RC = lit_rsrc(a.RC, 2L * a.halfR * 2);
     r.Pix = lit_rsrc(a.Pix, 2L * a.halfR * 4);
+    r.P = lit_rsrc(a.progress, (long)a.H * 4);
     return r;
 }
+template <bool LOC>
 __device__ __forceinline__ int bld16(__amdgpu_buffer_rsrc_t r, long i)
 {
-    return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(i * 2), 0, kSc1);
+    return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(i * 2), 0, LOC ? kLitLoc : kLitDev);
 }
+template <bool LOC>
 __device__ __forceinline__ uint32_t bld32(__amdgpu_buffer_rsrc_t r, long i)
 {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4), 0, kSc1);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4), 0, LOC ? kLitLoc : kLitDev);
 }
 
-// Stores of the shared words: sc1 buffer stores (what the relaxed agent-scope atomic stores compile to), issued with
-// no branch; a store that should not happen gets an offset past every resource, which the range check drops.
+// Stores of the shared words: buffer stores with the launch's cache policy, issued with no branch; a store that
+// should not happen gets an offset past every resource, which the range check drops.
 constexpr uint32_t kLitDrop = 0x80000000u;
+template <bool LOC>
 __device__ __forceinline__ void bst16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, uint32_t v)
 {
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)byte_off, 0, kSc1);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)byte_off, 0, LOC ? kLitLoc : kLitDev);
+}
+template <bool LOC>
+__device__ __forceinline__ void bst32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)byte_off, 0, LOC ? kLitLoc : kLitDev);
 }
 
 // Minimum over the 64 lanes of two u16 halves at once (DPP row shifts and broadcasts), wave-uniform.
@@ -185,7 +204,7 @@ __device__ __forceinline__ LitCell lit_cell(const LitArgs& a, size_t cell, long 
 
 // One Pixel call (:234-317) at (x, y).
 // re: the second visit of the border column, whose aggregate the first visit (the one before) left in `last`.
-template <int DIR, int NPL, bool ADP>
+template <int DIR, int NPL, bool ADP, bool LOC>
 __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cur, int oth, bool re, const LitCell& cc,
                                           int (&last)[NPL])
 {
@@ -196,7 +215,7 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     const int q = start - a.minlo;
     const uint32_t g = cc.g;
     const LitRes R = lit_res(a);
-    const int sp = bld16(R.RS, cur * a.halfR + slot), cp = bld16(R.RC, cur * a.halfR + slot);
+    const int sp = bld16<LOC>(R.RS, cur * a.halfR + slot), cp = bld16<LOC>(R.RC, cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
 
     // every read first: the visit never reads a word it writes (path 0 reads another slot of its half, paths
@@ -209,16 +228,16 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     for (int p = 0; p < 4; p++) {
         const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
         const int half = p == 0 ? cur : oth;
-        minPrev[p] = bld16(R.Min, half * a.halfM + ns * 4 + p);
-        gq[p] = ADP ? bld32(R.Pix, half * a.halfR + ns) : 0u;
+        minPrev[p] = bld16<LOC>(R.Min, half * a.halfM + ns * 4 + p);
+        gq[p] = ADP ? bld32<LOC>(R.Pix, half * a.halfR + ns) : 0u;
         const long nb = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
         // no branch around the loads: lanes past the cap read their wave's first word and drop it
 #pragma unroll
         for (int h = 0; h < NPL; h++) {
             const long i0 = nb + (lane + 64 * h < cap ? lane + 64 * h : 0);
-            lm[p][h] = bld16(R.L, i0 - 1);
-            l0[p][h] = bld16(R.L, i0);
-            lp[p][h] = bld16(R.L, i0 + 1);
+            lm[p][h] = bld16<LOC>(R.L, i0 - 1);
+            l0[p][h] = bld16<LOC>(R.L, i0);
+            lp[p][h] = bld16<LOC>(R.L, i0 + 1);
         }
     }
     // the cell's own raw costs and aggregate (no other row touches them), in the same round trip
@@ -232,8 +251,9 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     }
 #pragma unroll
     for (int p = 0; p < 4; p++) {
+        minPrev[p] = lit_u(minPrev[p]);  // wave-uniform words: scalar arithmetic from here on
         int P2 = a.p2;
-        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, gq[p]);
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, (uint32_t)lit_u((int)gq[p]));
         p2p[p] = std::min(P2 + minPrev[p], 65535);
 #pragma unroll
         for (int h = 0; h < NPL; h++) {
@@ -273,13 +293,13 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     // Word indexes fit 31 bits (the host checks the buffer's size); a word past the end of the buffer is not
     // stored (the range check drops it).
     const int B = cur * (int)a.halfL + slot * a.stride;
-    const int q0 = sp - a.minlo, Llen = (int)a.Llen;
+    const int q0 = lit_u(sp) - a.minlo, Llen = (int)a.Llen, cpu = lit_u(cp);
     int lo[8], hi[8];
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const int bc = B + 1 + p * (a.M + 1);
         lo[2 * p] = bc + std::max(q0, 0);
-        hi[2 * p] = bc + std::min(q0 + cp, Llen - bc);  // k1 = min(cp, L.Length - baseCur - q0) (:262)
+        hi[2 * p] = bc + std::min(q0 + cpu, Llen - bc);  // k1 = min(cp, L.Length - baseCur - q0) (:262)
         lo[2 * p + 1] = bc + q;
         hi[2 * p + 1] = bc + q + cap;
     }
@@ -291,7 +311,7 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
             bool st = w < hi[o];
 #pragma unroll
             for (int o2 = o + 1; o2 < 8; o2++) st = st & !((w >= lo[o2]) & (w < hi[o2]));  // no branch
-            bst16(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lc[o >> 1][h] : 2000u);
+            bst16<LOC>(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lc[o >> 1][h] : 2000u);
         }
     }
 #pragma unroll
@@ -303,12 +323,12 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
     if (lane < 4) {
         const uint32_t pair = lane < 2 ? mins01 : mins23;
         const int rm = (lane & 1) ? (int)(pair >> 16) : (int)(pair & 0xffffu);
-        ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));  // :275, :314
+        bst16<LOC>(R.Min, (uint32_t)(cur * a.halfM + slot * 4 + lane) * 2u, (uint32_t)(cap == 0 ? 0xffff : rm));  // :275, :314
     }
     if (lane == 0) {
-        ast(a.RS + cur * a.halfR + slot, (uint16_t)start);  // :316
-        ast(a.RC + cur * a.halfR + slot, (uint16_t)cap);
-        if constexpr (ADP) ast32(a.Pix + cur * a.halfR + slot, g);  // :239
+        bst16<LOC>(R.RS, (uint32_t)(cur * a.halfR + slot) * 2u, (uint32_t)start);  // :316
+        bst16<LOC>(R.RC, (uint32_t)(cur * a.halfR + slot) * 2u, (uint32_t)cap);
+        if constexpr (ADP) bst32<LOC>(R.Pix, (uint32_t)(cur * a.halfR + slot) * 4u, g);  // :239
     }
 }
 
@@ -316,7 +336,7 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
 // the store phase.
 constexpr int kLitMaxCap = 1024;
 
-template <int DIR, bool ADP>
+template <int DIR, bool ADP, bool LOC>
 __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, int cur, int oth, bool re, const LitCell& cc,
                                               uint16_t (*lcs)[kLitMaxCap], uint16_t* aggl)
 {
@@ -327,7 +347,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     const int q = start - a.minlo;
     const uint32_t g = cc.g;
     const LitRes R = lit_res(a);
-    const int sp = bld16(R.RS, cur * a.halfR + slot), cp = bld16(R.RC, cur * a.halfR + slot);
+    const int sp = bld16<LOC>(R.RS, cur * a.halfR + slot), cp = bld16<LOC>(R.RC, cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
     int minPrev[4], p2p[4];
     long nb[4];
@@ -335,9 +355,9 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     for (int p = 0; p < 4; p++) {
         const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
         const int half = p == 0 ? cur : oth;
-        minPrev[p] = bld16(R.Min, half * a.halfM + ns * 4 + p);
+        minPrev[p] = bld16<LOC>(R.Min, half * a.halfM + ns * 4 + p);
         int P2 = a.p2;
-        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, bld32(R.Pix, half * a.halfR + ns));
+        if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, bld32<LOC>(R.Pix, half * a.halfR + ns));
         p2p[p] = std::min(P2 + minPrev[p], 65535);
         nb[p] = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
     }
@@ -348,7 +368,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
 #pragma unroll
         for (int p = 0; p < 4; p++) {  // loads without a branch, as in lit_visit
             const long i0 = nb[p] + (act ? k : 0);
-            const int vm = bld16(R.L, i0 - 1), v0 = bld16(R.L, i0), vp = bld16(R.L, i0 + 1);
+            const int vm = bld16<LOC>(R.L, i0 - 1), v0 = bld16<LOC>(R.L, i0), vp = bld16<LOC>(R.L, i0 + 1);
             lm[p] = act ? vm : 0;
             l0[p] = act ? v0 : 0;
             lp[p] = act ? vp : 0;
@@ -392,33 +412,45 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
             bool st = true;
 #pragma unroll
             for (int o2 = o + 1; o2 < 8; o2++) st = st & !((w >= lo[o2]) & (w < hi[o2]));  // no branch
-            bst16(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lcs[o >> 1][k] : 2000u);
+            bst16<LOC>(R.L, st ? (uint32_t)w * 2u : kLitDrop, (o & 1) ? (uint32_t)lcs[o >> 1][k] : 2000u);
         }
     }
     if (lane < 4) {
         const uint32_t pair = lane < 2 ? mins01 : mins23;
         const int rm = (lane & 1) ? (int)(pair >> 16) : (int)(pair & 0xffffu);
-        ast(a.Min + cur * a.halfM + slot * 4 + lane, (uint16_t)(cap == 0 ? 0xffff : rm));
+        bst16<LOC>(R.Min, (uint32_t)(cur * a.halfM + slot * 4 + lane) * 2u, (uint32_t)(cap == 0 ? 0xffff : rm));
     }
     if (lane == 0) {
-        ast(a.RS + cur * a.halfR + slot, (uint16_t)start);
-        ast(a.RC + cur * a.halfR + slot, (uint16_t)cap);
-        if constexpr (ADP) ast32(a.Pix + cur * a.halfR + slot, g);
+        bst16<LOC>(R.RS, (uint32_t)(cur * a.halfR + slot) * 2u, (uint32_t)start);
+        bst16<LOC>(R.RC, (uint32_t)(cur * a.halfR + slot) * 2u, (uint32_t)cap);
+        if constexpr (ADP) bst32<LOC>(R.Pix, (uint32_t)(cur * a.halfR + slot) * 4u, g);
     }
 }
 
-// Lane 0 waits until row2 has completed `need` visits; a spin past the bound (or another wavefront's) sets the
-// error word, after which no wavefront waits again.
+// The waits run on the whole wavefront with wave-uniform values (scalar registers and branches): every lane reads
+// the same progress word, and the bookkeeping stays off the vector unit.
+// A spin past the bound (or another wavefront's) sets the error word, after which no wait is made again.
+__device__ __forceinline__ bool lit_give_up(const LitArgs& a, int spins)
+{
+    if ((spins & 63) != 0) return false;
+    const bool bad = lit_u((int)__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0 ||
+                     spins > (1 << 22);
+    if (bad && threadIdx.x == 0) atomicOr(a.err, 1u);
+    return bad;
+}
+
+// Waits until row2 has completed `need` visits.
+template <bool LOC>
 __device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, int& seen, bool& dead)
 {
     if (dead || seen >= need) return;
+    const __amdgpu_buffer_rsrc_t P = lit_res(a).P;
     int spins = 0;
     for (;;) {
-        seen = (int)ald32(a.progress + row2);
+        lit_poll_barrier();
+        seen = lit_u((int)bld32<LOC>(P, row2));
         if (seen >= need) return;
-        if ((++spins & 63) == 0 &&
-            (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > (1 << 22))) {
-            atomicOr(a.err, 1u);
+        if (lit_give_up(a, ++spins)) {
             dead = true;
             return;
         }
@@ -426,29 +458,30 @@ __device__ __forceinline__ void lit_wait(const LitArgs& a, int row2, int need, i
     }
 }
 
-// Lane 0 waits until each of four rows has completed need[d] visits (need 0: nothing to wait for). The progress
-// words of all four are read together, so one round trip serves them all, and the error word only every 64th
-// round, so a progress store is seen one round trip after it lands rather than two.
+// Waits until each of four rows has completed need[d] visits (need 0: nothing to wait for). The progress words of
+// all four are read together, so one round trip serves them all, and the error word only every 64th round, so a
+// progress store is seen one round trip after it lands rather than two.
+template <bool LOC>
 __device__ __forceinline__ void lit_wait4(const LitArgs& a, const int (&row2)[4], const int (&need)[4], int (&seen)[4],
                                           bool& dead)
 {
     if (dead) return;
     if (seen[0] >= need[0] && seen[1] >= need[1] && seen[2] >= need[2] && seen[3] >= need[3]) return;
+    const __amdgpu_buffer_rsrc_t P = lit_res(a).P;
     int spins = 0;
     for (;;) {
+        lit_poll_barrier();
         uint32_t v[4];
 #pragma unroll
-        for (int d = 0; d < 4; d++) v[d] = ald32(a.progress + row2[d]);
+        for (int d = 0; d < 4; d++) v[d] = bld32<LOC>(P, row2[d]);
         bool ok = true;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            seen[d] = (int)v[d];
+            seen[d] = lit_u((int)v[d]);
             ok = ok && seen[d] >= need[d];
         }
         if (ok) return;
-        if ((++spins & 63) == 0 &&
-            (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > (1 << 22))) {
-            atomicOr(a.err, 1u);
+        if (lit_give_up(a, ++spins)) {
             dead = true;
             return;
         }
@@ -464,8 +497,21 @@ __device__ __forceinline__ int lit_need(const LitArgs& a, int dir, int j, int i,
     if (lit_rev(a, dir, row2 % a.T) && c + reach >= a.W - 1) mp = a.W;
     return std::min(lit_before(a, dir, j, i, r, row2), mp + 1);
 }
+// The same with row2's band jj, tile row rr and second-visit flag rev given (a row's neighbours keep them for all of
+// its visits, so a visit makes no division).
+__device__ __forceinline__ int lit_need_pre(const LitArgs& a, int j, int i, int r, int c, int jj, int rr, int rev, int reach)
+{
+    int before;
+    if (jj < j) before = a.W + rev;
+    else if (jj > j) before = 0;
+    else if (rr < r && i == a.gwN - 1) before = a.W + rev;
+    else before = std::min(std::max((rr < r ? i + 1 : i) * a.T - rr, 0), a.W);
+    int mp = std::min(c + reach, a.W - 1);
+    if (rev && c + reach >= a.W - 1) mp = a.W;
+    return std::min(before, mp + 1);
+}
 
-template <int DIR, int NPL, bool ADP>
+template <int DIR, int NPL, bool ADP, bool LOC>
 __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
 {
     const int lane = threadIdx.x;
@@ -474,29 +520,61 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
     __shared__ uint16_t aggl[NPL == 0 ? kLitMaxCap : 1];
     int last[NPL == 0 ? 1 : NPL];
     bool dead = false;  // lane 0: the error word is set, waits are skipped so the grid drains
+    // the pass's home XCD: named by the first wavefront to get here; the others of other XCDs leave
+    int stay = 1;
+    if (LOC && lane == 0) {
+        const uint32_t me = (uint32_t)lit_xcc();
+        uint32_t seen_home = 0xffffffffu;
+        __hip_atomic_compare_exchange_strong(a.home, &seen_home, me, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        stay = seen_home == 0xffffffffu || seen_home == me;
+    }
+    if (!__shfl(stay, 0)) return;
     for (;;) {
         int row = 0;
         if (lane == 0) row = (int)atomicAdd(a.ticket, 1u);
-        row = __shfl(row, 0);
+        row = lit_u(__shfl(row, 0));
         if (row >= a.H) break;
         const int j = row / T, r = row - j * T;
         const int y = DIR > 0 ? row : a.H - 1 - row;
         const int cur = (r & 1) == 0 ? 1 : 0, oth = 1 - cur;  // :219
         const int nvis = W + lit_rev(a, DIR, r);
-        int seen[4] = {0, 0, 0, 0};  // lane 0: progress last read of rows row-2, row-1, row+1, row+2
+        int seen[4] = {0, 0, 0, 0};  // progress last read of rows row-2, row-1, row+1, row+2
+        // those rows (this row for one outside the image, which is waited for by nobody), their bands, tile rows and
+        // second-visit flags
+        int rows2[4], jj2[4], rr2[4], rev2[4];
+        bool in2[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int row2 = row + (d < 2 ? d - 2 : d - 1);
+            in2[d] = row2 >= 0 && row2 < a.H;
+            rows2[d] = in2[d] ? row2 : row;
+            jj2[d] = rows2[d] / T;
+            rr2[d] = rows2[d] - jj2[d] * T;
+            rev2[d] = lit_rev(a, DIR, rr2[d]);
+        }
         // pass +1 with even T: the slot's previous occupant is the cell two rows up (halves alternate row by row)
         const bool prev2 = DIR > 0 && (T & 1) == 0 && y >= 2;
         const auto cell_at = [&](int cc0) { return (size_t)y * W + (DIR > 0 ? cc0 : W - 1 - cc0); };
         LitCell next = lit_cell<ADP>(a, cell_at(0), prev2 ? (long)cell_at(0) - 2L * W : -1L);
+        int ti = 0, tm = r;  // (pos + r) / T and (pos + r) % T, kept step by step
         for (int pos = 0; pos < nvis; pos++) {
             const int c = pos < W ? pos : W - 1;
-            const int i = pos < W ? (pos + r) / T : a.gwN - 1;
-            const LitCell cc = next;
+            const int i = pos < W ? ti : a.gwN - 1;
+            if (++tm == T) {
+                tm = 0;
+                ti++;
+            }
+            LitCell cc = next;
+            cc.start = lit_u(cc.start);
+            cc.cap = lit_u(cc.cap);
+            cc.g = (uint32_t)lit_u((int)cc.g);
+            cc.prev_spill = lit_u(cc.prev_spill);
             if (pos + 1 < nvis) {  // in flight during this visit's waits
                 const int c1 = pos + 1 < W ? pos + 1 : W - 1;
                 next = lit_cell<ADP>(a, cell_at(c1), prev2 ? (long)cell_at(c1) - 2L * W : -1L);
             }
-            if (lane == 0 && pos < W) {
+            if (pos < W) {
                 // The reach this visit needs (with even T, whose halves alternate row by row; odd T keeps
                 // kLitReach). Pass -1: one column, since the words a visit shares beyond its neighbours' slots (the
                 // next slot in memory) lie behind it in pass order. Pass +1: towards rows y+-1, one column, two when
@@ -514,30 +592,27 @@ __global__ __launch_bounds__(64) void lit_pass(LitArgs a)
                         reach2 = 1 + (lit_spills(a, cc.start, cc.cap) || cc.prev_spill ? 1 : 0);
                     }
                 }
-                int rows2[4], need[4];
+                int need[4];
 #pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    const int row2 = row + (d < 2 ? d - 2 : d - 1);
-                    const bool in = row2 >= 0 && row2 < a.H;
-                    rows2[d] = in ? row2 : row;  // a row outside the image: read this row's word, wait for nothing
-                    need[d] = in ? lit_need(a, DIR, j, i, r, c, row2, (d == 1 || d == 2) ? reach1 : reach2) : 0;
-                }
-                lit_wait4(a, rows2, need, seen, dead);
-            } else if (lane == 0) {
+                for (int d = 0; d < 4; d++)
+                    need[d] = in2[d] ? lit_need_pre(a, j, i, r, c, jj2[d], rr2[d], rev2[d], (d == 1 || d == 2) ? reach1 : reach2)
+                                     : 0;
+                lit_wait4<LOC>(a, rows2, need, seen, dead);
+            } else {
                 // the second visit of the border column reads the buffers as the band's earlier visits left them,
                 // whichever rows made those: it waits on every row of its band (and rows row-2, row-1 before it)
                 const int b1 = std::min(a.H, j * T + T);
                 for (int row2 = std::max(0, std::min(row - 2, j * T)); row2 < b1; row2++) {
                     int s = 0;
-                    if (row2 != row) lit_wait(a, row2, lit_need(a, DIR, j, i, r, c, row2), s, dead);
+                    if (row2 != row) lit_wait<LOC>(a, row2, lit_need(a, DIR, j, i, r, c, row2), s, dead);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, lcs, aggl);
-            else lit_visit<DIR, NPL, ADP>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, last);
+            if constexpr (NPL == 0) lit_visit_lds<DIR, ADP, LOC>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, lcs, aggl);
+            else lit_visit<DIR, NPL, ADP, LOC>(a, DIR > 0 ? c : W - 1 - c, y, cur, oth, pos == W, cc, last);
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (lane == 0) ast32(a.progress + row, (uint32_t)(pos + 1));
+            if (lane == 0) bst32<LOC>(lit_res(a).P, (uint32_t)row * 4u, (uint32_t)(pos + 1));
         }
     }
 }
@@ -572,7 +647,7 @@ __global__ void lit_fill32(uint32_t* p, size_t n, uint32_t v)
 }
 
 struct LitLayout {
-    size_t L, Min, RS, RC, Pix, progress, ctl, total;  // byte offsets; ctl = ticket, err
+    size_t L, Min, RS, RC, Pix, progress, ctl, total;  // byte offsets; ctl = ticket, err, home
 };
 LitLayout lit_layout(int W, int H, int M)
 {
@@ -599,23 +674,24 @@ LitLayout lit_layout(int W, int H, int M)
 }
 
 template <int DIR, int NPL, bool ADP>
-hipError_t lit_launch(const LitArgs& a, int grid, hipStream_t s)
+hipError_t lit_launch(const LitArgs& a, bool loc, int grid, hipStream_t s)
 {
-    lit_pass<DIR, NPL, ADP><<<grid, 64, 0, s>>>(a);
+    if (loc) lit_pass<DIR, NPL, ADP, true><<<grid, 64, 0, s>>>(a);
+    else lit_pass<DIR, NPL, ADP, false><<<grid, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template <int NPL, bool ADP>
-hipError_t lit_pass_dispatch(int dir, const LitArgs& a, int grid, hipStream_t s)
+hipError_t lit_pass_dispatch(int dir, const LitArgs& a, bool loc, int grid, hipStream_t s)
 {
-    return dir > 0 ? lit_launch<1, NPL, ADP>(a, grid, s) : lit_launch<-1, NPL, ADP>(a, grid, s);
+    return dir > 0 ? lit_launch<1, NPL, ADP>(a, loc, grid, s) : lit_launch<-1, NPL, ADP>(a, loc, grid, s);
 }
 
-hipError_t lit_run_pass(int dir, int npl, bool adp, const LitArgs& a, int grid, hipStream_t s)
+hipError_t lit_run_pass(int dir, int npl, bool adp, const LitArgs& a, bool loc, int grid, hipStream_t s)
 {
-    if (npl == 2) return adp ? lit_pass_dispatch<2, true>(dir, a, grid, s) : lit_pass_dispatch<2, false>(dir, a, grid, s);
-    if (npl == 4) return adp ? lit_pass_dispatch<4, true>(dir, a, grid, s) : lit_pass_dispatch<4, false>(dir, a, grid, s);
-    return adp ? lit_pass_dispatch<0, true>(dir, a, grid, s) : lit_pass_dispatch<0, false>(dir, a, grid, s);
+    if (npl == 2) return adp ? lit_pass_dispatch<2, true>(dir, a, loc, grid, s) : lit_pass_dispatch<2, false>(dir, a, loc, grid, s);
+    if (npl == 4) return adp ? lit_pass_dispatch<4, true>(dir, a, loc, grid, s) : lit_pass_dispatch<4, false>(dir, a, loc, grid, s);
+    return adp ? lit_pass_dispatch<0, true>(dir, a, loc, grid, s) : lit_pass_dispatch<0, false>(dir, a, loc, grid, s);
 }
 
 }  // namespace
@@ -629,11 +705,11 @@ int literal_resident(int npl, bool adp)
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    const void* k = npl == 2   ? (adp ? (const void*)lit_pass<1, 2, true> : (const void*)lit_pass<1, 2, false>)
-                    : npl == 4 ? (adp ? (const void*)lit_pass<1, 4, true> : (const void*)lit_pass<1, 4, false>)
-                               : (adp ? (const void*)lit_pass<1, 0, true> : (const void*)lit_pass<1, 0, false>);
+    const void* k = npl == 2   ? (adp ? (const void*)lit_pass<1, 2, true, true> : (const void*)lit_pass<1, 2, false, true>)
+                    : npl == 4 ? (adp ? (const void*)lit_pass<1, 4, true, true> : (const void*)lit_pass<1, 4, false, true>)
+                               : (adp ? (const void*)lit_pass<1, 0, true, true> : (const void*)lit_pass<1, 0, false, true>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64, 0) != hipSuccess) return 0;
-    return per * cus;
+    return per * std::min(cus, kLitXcdCus);  // a pass runs on one XCD
 }
 
 hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pass1, uint16_t* wta, hipStream_t s)
@@ -663,6 +739,7 @@ hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pa
     a.progress = (uint32_t*)(base + lay.progress);
     a.ticket = (uint32_t*)(base + lay.ctl);
     a.err = a.ticket + 1;
+    a.home = a.ticket + 2;
     a.start = l.start;
     a.cap = l.cap;
     a.ucap = l.ucap;
@@ -683,11 +760,15 @@ hipError_t launch_literal(const LiteralLayer& l, void* scratch, uint16_t* agg_pa
     lit_fill16<<<fb, 256, 0, s>>>(l.agg, cells * l.cs, 0);
     lit_fill32<<<1, 64, 0, s>>>(a.ticket, 2, 0u);
     hipError_t e = hipGetLastError();
-    const int grid = std::min(l.H, kLitGrid);
+    // one XCD when it holds every row at once (rows never wait for a wavefront to come free), else all of them
+    const char* ev = getenv("LUX_SGM_LIT_LOCAL");
+    const bool loc = ev && *ev ? atoi(ev) != 0 : l.H <= literal_resident(l.npl, l.ap.on != 0);
+    const int grid = std::min(l.H, kLitGrid) * (loc ? kLitXcds : 1);
     for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
         lit_fill32<<<fb, 256, 0, s>>>(a.progress, (size_t)l.H, 0u);
         lit_fill32<<<1, 64, 0, s>>>(a.ticket, 1, 0u);  // the error word (ticket + 1) stays set across passes
-        e = lit_run_pass(pass == 0 ? 1 : -1, l.npl, l.ap.on != 0, a, grid, s);
+        lit_fill32<<<1, 64, 0, s>>>(a.home, 1, 0xffffffffu);
+        e = lit_run_pass(pass == 0 ? 1 : -1, l.npl, l.ap.on != 0, a, loc, grid, s);
         if (e == hipSuccess && pass == 0 && agg_pass1)
             e = hipMemcpyAsync(agg_pass1, l.agg, cells * l.cs * 2, hipMemcpyDeviceToDevice, s);
     }

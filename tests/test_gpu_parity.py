@@ -187,6 +187,20 @@ SPILL_LAYERS = [(9, 7, 4, 40, 13, 8, False), (13, 11, 3, 24, 11, 8, False), (16,
 
 @pytest.mark.parametrize("W,H,T,planes,cmax,align,adaptive", SPILL_LAYERS)
 def test_layer_ranges_spilling_padding(W, H, T, planes, cmax, align, adaptive):
+    _spill_case(W, H, T, planes, cmax, align, adaptive)
+
+
+@pytest.mark.parametrize("local", ["0", "1"])
+def test_layer_spilling_padding_on_one_xcd_and_on_all(monkeypatch, local):
+    """The line-buffer-literal pass runs on one XCD (its words through that XCD's L2) when every row fits there at
+    once, else on all of them (words coherent across XCDs); LUX_SGM_LIT_LOCAL forces either. Both equal the oracle
+    (planes up to 128, 256 and 1024 a cell: two, four and the LDS form)."""
+    monkeypatch.setenv("LUX_SGM_LIT_LOCAL", local)
+    for case in (SPILL_LAYERS[3], SPILL_LAYERS[5], SPILL_LAYERS[8]):
+        _spill_case(*case)
+
+
+def _spill_case(W, H, T, planes, cmax, align, adaptive):
     rng = np.random.default_rng(W * 1000 + H + T)
     start, count = M.spill_ranges(rng, W, H, planes, cmax, align)
     assert _spills(start, count, align) > 0
