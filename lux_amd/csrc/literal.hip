@@ -136,12 +136,12 @@ __device__ __forceinline__ LitRes lit_res(const LitArgs& a)
     return r;
 }
 template <bool LOC>
-__device__ __forceinline__ int bld16(__amdgpu_buffer_rsrc_t r, long i)
+__device__ __forceinline__ int bld16(__amdgpu_buffer_rsrc_t r, int i)
 {
     return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(i * 2), 0, LOC ? kLitLoc : kLitDev);
 }
 template <bool LOC>
-__device__ __forceinline__ uint32_t bld32(__amdgpu_buffer_rsrc_t r, long i)
+__device__ __forceinline__ uint32_t bld32(__amdgpu_buffer_rsrc_t r, int i)
 {
     return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4), 0, LOC ? kLitLoc : kLitDev);
 }
@@ -230,11 +230,11 @@ __device__ __forceinline__ void lit_visit(const LitArgs& a, int x, int y, int cu
         const int half = p == 0 ? cur : oth;
         minPrev[p] = bld16<LOC>(R.Min, half * a.halfM + ns * 4 + p);
         gq[p] = ADP ? bld32<LOC>(R.Pix, half * a.halfR + ns) : 0u;
-        const long nb = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
+        const int nb = half * (int)a.halfL + ns * a.stride + 1 + p * (a.M + 1) + q;  // below 2^30 (host check)
         // no branch around the loads: lanes past the cap read their wave's first word and drop it
 #pragma unroll
         for (int h = 0; h < NPL; h++) {
-            const long i0 = nb + (lane + 64 * h < cap ? lane + 64 * h : 0);
+            const int i0 = nb + (lane + 64 * h < cap ? lane + 64 * h : 0);
             lm[p][h] = bld16<LOC>(R.L, i0 - 1);
             l0[p][h] = bld16<LOC>(R.L, i0);
             lp[p][h] = bld16<LOC>(R.L, i0 + 1);
@@ -367,7 +367,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
     const int sp = bld16<LOC>(R.RS, cur * a.halfR + slot), cp = bld16<LOC>(R.RC, cur * a.halfR + slot);
     const size_t cbase = cell * (size_t)a.cs;
     int minPrev[4], p2p[4];
-    long nb[4];
+    int nb[4];
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const int ns = slot + (p == 2 ? 0 : p == 3 ? DIR : -DIR);
@@ -376,7 +376,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
         int P2 = a.p2;
         if constexpr (ADP) P2 = (int)p2_step(a.ap, (uint32_t)a.p2, g, bld32<LOC>(R.Pix, half * a.halfR + ns));
         p2p[p] = std::min(P2 + minPrev[p], 65535);
-        nb[p] = (long)half * a.halfL + (long)ns * a.stride + 1 + (long)p * (a.M + 1) + q;
+        nb[p] = half * (int)a.halfL + ns * a.stride + 1 + p * (a.M + 1) + q;
     }
     uint32_t run[4] = {0xffffu, 0xffffu, 0xffffu, 0xffffu};
     for (int k = lane; k < ((cap + 63) & ~63); k += 64) {
@@ -384,7 +384,7 @@ __device__ __forceinline__ void lit_visit_lds(const LitArgs& a, int x, int y, in
         int lm[4], l0[4], lp[4];
 #pragma unroll
         for (int p = 0; p < 4; p++) {  // loads without a branch, as in lit_visit
-            const long i0 = nb[p] + (act ? k : 0);
+            const int i0 = nb[p] + (act ? k : 0);
             const int vm = bld16<LOC>(R.L, i0 - 1), v0 = bld16<LOC>(R.L, i0), vp = bld16<LOC>(R.L, i0 + 1);
             lm[p] = act ? vm : 0;
             l0[p] = act ? v0 : 0;
